@@ -1,0 +1,200 @@
+/*
+ * firepro.h — C ABI of the MI355X-native sparse-brick fire/smoke solver.
+ *
+ * The entry points replace the compute passes of Fire Pro's engine layer
+ * (src/engine/FluidSimulation.ts) behind the members that the library layer
+ * (src/library/FireSimulation.ts) calls. Every structure is plain data: numbers,
+ * fixed arrays and borrowed pointers, read during the call only. Host-side geometry
+ * (positions, sizes) is double, as JavaScript numbers are; the solver converts it to
+ * 32-bit floats where the reference packs Float32Arrays for the GPU.
+ *
+ * Status: every entry point returns FP_OK (0) or a negative FP_E* code; the message
+ * of the last failure on the calling thread is fp_last_error().
+ */
+#ifndef FIREPRO_H
+#define FIREPRO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FP_OK 0
+#define FP_EINVAL (-1)   /* invalid argument: the reference throws Error here */
+#define FP_EDEVICE (-2)  /* HIP runtime failure */
+#define FP_ESTATE (-3)   /* call out of order (released handle, missing step) */
+#define FP_ENOMEM (-4)   /* device or host allocation failed */
+
+#define FP_SHAPE_SPHERE 0
+#define FP_SHAPE_BOX 1
+
+#define FP_FORCE_WIND 0
+#define FP_FORCE_TURBULENCE 1
+#define FP_FORCE_VORTEX 2
+#define FP_FORCE_RADIAL 3
+
+#define FP_MAX_FLAME_KNOTS 8
+
+/* SimulationGrid (src/engine/types.ts:26-36). Construction-only. */
+typedef struct fp_grid {
+  double voxel_size;        /* meters, 0.0001..16 (FluidSimulation.ts:362-363) */
+  int32_t velocity_divisor; /* 1, 2 or 4 */
+  int32_t brick_size;       /* 8, 16 or 32 */
+  int32_t smoke_divisor;    /* 1 only: coarse smoke is not built */
+  int32_t scalar_maccormack;/* 1: bounded MacCormack scalars; 0: first order */
+} fp_grid;
+
+/* SimulationSettings (src/engine/types.ts:235-243). */
+typedef struct fp_settings {
+  double buoyancy, smoke_weight, velocity_damping, cooling, dissipation, vorticity;
+  double seed;
+} fp_settings;
+
+/* FlameSettings (src/engine/flame.ts:5-11); ramps: smoke, heat, expansion. */
+typedef struct fp_flame {
+  double lifespan, smoke_rate, heat_rate, expansion_rate;
+  int32_t knot_count[3];
+  double knots[3][FP_MAX_FLAME_KNOTS][2]; /* [lifetime, response] */
+} fp_flame;
+
+/* FuelSettings (src/engine/fuel.ts:3-8). */
+typedef struct fp_fuel {
+  int32_t enabled;
+  double ignition_heat, burn_rate;
+} fp_fuel;
+
+/* SimulationSource (src/engine/types.ts:246-268); sphere and box shapes. `id` stands in
+ * for the reference's string id: a new id asks for pool slots at once. */
+typedef struct fp_source {
+  uint64_t id;
+  int32_t shape;
+  double position[3], size[3];
+  double flame, heat_rate, smoke_rate, fuel_rate;
+  double velocity[3];
+  double velocity_response, outward_speed;
+  int32_t has_variation;
+  double variation_period, variation_strength;
+} fp_source;
+
+/* SimulationForce (src/engine/types.ts:284-293). `targeted` = sources given (even none). */
+typedef struct fp_force {
+  int32_t type;
+  double vector[3], center[3];
+  double strength, scale, lift, inward;
+  int32_t targeted;
+  int32_t source_count;
+  const fp_source *sources;
+} fp_force;
+
+/* SimulationCollider (src/engine/types.ts:271-281), sphere or box. */
+typedef struct fp_collider {
+  int32_t shape;
+  double position[3], size[3];
+  double axes[3][3]; /* the box's axes in world space (rows); world axes by default */
+} fp_collider;
+
+/* SimulationFrame (src/engine/types.ts:295-302). */
+typedef struct fp_frame {
+  fp_settings simulation;
+  fp_flame combustion;
+  int32_t has_fuel;
+  fp_fuel fuel;
+  int32_t source_count;
+  const fp_source *sources;
+  int32_t force_count;
+  const fp_force *forces;
+  int32_t collider_count;
+  const fp_collider *colliders;
+} fp_frame;
+
+/* The members FireSimulation.stats reads (FireSimulation.ts:295-334). */
+typedef struct fp_stats {
+  double time;
+  int64_t steps;
+  int64_t active_bricks;    /* activeBrickCount at the latest measurement */
+  int64_t active_voxels;    /* active_bricks * brick_size^3 */
+  int64_t pool_slots;       /* poolSlots */
+  int64_t used_slots;       /* usedSlots */
+  int32_t pool_limited;     /* poolLimited: gridLimited */
+  int32_t tile_capacity;    /* tile slots the per-brick arrays hold */
+  int32_t tile_count;       /* tiles in the directory */
+  int32_t pool_side;        /* slots per row and rows per layer */
+  int64_t memory_bytes;     /* device bytes this simulation holds */
+} fp_stats;
+
+/* Integer tables of the latest step, the bit-exact parity surface: every brick id's
+ * page, entry and record, the free stack, pool state and the sorted active and zeroed
+ * sets. Buffers are caller-owned; each *_count is the entries written. */
+typedef struct fp_tables {
+  int32_t *pages;       /* [brick_count] */
+  int32_t *entries;     /* [brick_count] */
+  int32_t *records;     /* [brick_count * 28] */
+  uint32_t *free_slots; /* [pool_slots] */
+  int32_t *active;      /* [brick_count] sorted brick ids */
+  int32_t *zeroed;      /* [2 * brick_count*2] (id, payload) pairs, sorted by id then payload */
+  int32_t brick_count, free_count, failures, active_count, zeroed_count;
+  int32_t pool_slots;
+} fp_tables;
+
+/* Render parameters for the headless ray-march (src/library/SceneVolumeRenderer.ts
+ * packing, volume-march.wgsl). Matrices are column-major like three.js Matrix4. */
+typedef struct fp_camera {
+  double inverse_view_projection[16];
+  double position[3];
+  int32_t width, height;
+} fp_camera;
+
+typedef struct fp_render_settings {
+  double fire_intensity, flame_opacity, smoke_density, scattering, shadow_density;
+  double temperature, soot_glow;
+  double smoke_color[3], fire_color[3]; /* linear RGB */
+  double sun_direction[3], sun_intensity;
+  int32_t filter;       /* 0 trilinear, 1 quadratic, 2 cubic */
+  int32_t ray_steps;
+  int32_t lighting_divisor;
+} fp_render_settings;
+
+typedef struct fp_sim fp_sim;
+
+const char *fp_last_error(void);
+const char *fp_version(void);
+
+/* FluidSimulation.create (FluidSimulation.ts:359-475): pools, tiles, curl noise. */
+int fp_sim_create(const fp_grid *grid, fp_sim **out);
+/* FluidSimulation.step (FluidSimulation.ts:1367-1373): one substep of `dt` in (0, 1/30]. */
+int fp_sim_step(fp_sim *sim, const fp_frame *frame, double dt);
+/* FluidSimulation.reset (FluidSimulation.ts:768-779). */
+int fp_sim_reset(fp_sim *sim);
+/* FluidSimulation.dispose (FluidSimulation.ts:1780-1802); NULL is accepted. */
+void fp_sim_destroy(fp_sim *sim);
+/* The cutoff, voxelBudget and ground setters (FireSimulation.ts:457-459). */
+int fp_sim_set_cutoff(fp_sim *sim, double cutoff);
+int fp_sim_set_budget(fp_sim *sim, double max_voxels);
+int fp_sim_set_ground(fp_sim *sim, int32_t ground);
+/* The getters FireSimulation reads (FireSimulation.ts:296-333). */
+int fp_sim_stats(const fp_sim *sim, fp_stats *out);
+/* readBricks (FluidSimulation.ts:1749-1779): min corner of each slotted brick, meters,
+ * in brick id order; `*count` in: capacity of `mins` (triples), out: bricks. */
+int fp_sim_read_bricks(fp_sim *sim, double *mins, int32_t *count);
+/* Integer tables of the latest step (see fp_tables). */
+int fp_sim_read_tables(fp_sim *sim, fp_tables *out);
+/* Cell values of the active bricks in active-list order: the field grid's four channels
+ * (smoke, heat, lifetime, fuel) per cell, the velocity grid's three MAC components, the
+ * pressure, and the expansion rate. NULL skips a field. Each buffer holds
+ * active_count * cells-per-brick * channels floats; `ids` receives the brick ids. */
+int fp_sim_read_state(fp_sim *sim, int32_t *ids, float *fields, float *velocity,
+                      float *pressure, float *expansion, int32_t capacity_bricks);
+/* Overwrite the cell values of the active bricks, same layout as fp_sim_read_state. */
+int fp_sim_write_state(fp_sim *sim, const float *fields, const float *velocity);
+/* The device stream the solver enqueues on (hipStream_t), for callers that time it. */
+void *fp_sim_stream(fp_sim *sim);
+/* Per-stage event timing of the next steps: enable, then read averages (ms). */
+int fp_sim_profile(fp_sim *sim, int32_t enable);
+int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

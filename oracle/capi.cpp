@@ -1,0 +1,196 @@
+// ORACLE — test infrastructure only (see solver.hpp). A C ABI over the CPU restatement,
+// loaded with ctypes by the tests, smoke() and bench.py's CPU baseline. Entry points mirror
+// include/firepro.h's, prefixed orc_, with the arithmetic width chosen at creation.
+#include <chrono>
+#include <string>
+
+#include "solver.hpp"
+
+namespace {
+thread_local std::string g_error;
+
+struct Base {
+  virtual ~Base() = default;
+  virtual void step(const fp_frame &f, double dt) = 0;
+  virtual void reset() = 0;
+  virtual void stats(fp_stats *s) = 0;
+  virtual void tables(fp_tables *t) = 0;
+  virtual int state(int32_t *ids, float *fields, float *vel, float *pressure, float *expansion, int cap) = 0;
+  virtual int state64(int32_t *ids, double *fields, double *vel, double *pressure, double *expansion, int cap) = 0;
+  virtual void write(const float *fields, const float *vel) = 0;
+  virtual void set(double cutoff, double budget, int ground, int which) = 0;
+  virtual const std::vector<float> &noise() = 0;
+  virtual void contentTiles(int32_t *out, int *count) = 0;
+  virtual void writeActivity(const uint32_t *brickFlags, int n, const int32_t *tiles, int tileCount) = 0;
+  virtual void readActivity(uint32_t *brickFlags, int n) = 0;
+};
+
+template <class R> struct Impl : Base {
+  oracle::Sim<R> sim;
+  explicit Impl(const fp_grid &g) : sim(g) {}
+  void step(const fp_frame &f, double dt) override { sim.step(f, dt); }
+  void reset() override { sim.reset(); }
+  void stats(fp_stats *s) override {
+    s->time = sim.time;
+    s->steps = sim.steps;
+    s->active_bricks = sim.activeBrickCount;
+    s->active_voxels = sim.activeBrickCount * (int64_t)sim.nF;
+    s->pool_slots = sim.poolSlots;
+    s->used_slots = sim.usedSlots;
+    s->pool_limited = sim.poolLimited;
+    s->tile_capacity = sim.capacity;
+    s->tile_count = sim.dir.count();
+    s->pool_side = sim.side;
+    s->memory_bytes = 0;
+  }
+  void tables(fp_tables *t) override {
+    int n = sim.brickCount();
+    t->brick_count = n;
+    t->pool_slots = sim.poolSlots;
+    t->free_count = sim.freeCount;
+    t->failures = (int32_t)sim.lastFailures;
+    if (t->pages)
+      for (int i = 0; i < n; i++) t->pages[i] = sim.pages[i];
+    if (t->entries)
+      for (int i = 0; i < n; i++) t->entries[i] = sim.entries[i];
+    if (t->records)
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < 28; k++)
+          t->records[(size_t)i * 28 + k] = sim.entries[i] >= 0 ? sim.records[(size_t)i * 28 + k] : 0;
+    if (t->free_slots)
+      for (int i = 0; i < sim.poolSlots; i++) t->free_slots[i] = sim.freeSlots[i];
+    t->active_count = (int32_t)sim.activeList.size();
+    if (t->active)
+      for (size_t i = 0; i < sim.activeList.size(); i++) t->active[i] = sim.activeList[i];
+    std::vector<std::pair<int32_t, uint32_t>> z = sim.zeroList;
+    std::sort(z.begin(), z.end());
+    t->zeroed_count = (int32_t)z.size();
+    if (t->zeroed)
+      for (size_t i = 0; i < z.size(); i++) {
+        t->zeroed[2 * i] = z[i].first;
+        t->zeroed[2 * i + 1] = (int32_t)z[i].second;
+      }
+  }
+  template <class T> int stateT(int32_t *ids, T *fields, T *vel, T *pressure, T *expansion, int cap) {
+    int n = (int)sim.activeList.size();
+    if (n > cap) return -1;
+    for (int i = 0; i < n; i++) {
+      int id = sim.activeList[i];
+      if (ids) ids[i] = id;
+      int s = sim.pageSlot(sim.entries[id] & 0xffffff);
+      if (fields)
+        for (int c = 0; c < sim.nF * 4; c++) fields[(size_t)i * sim.nF * 4 + c] = (T)sim.F[(size_t)s * sim.nF * 4 + c];
+      if (expansion)
+        for (int c = 0; c < sim.nF; c++) expansion[(size_t)i * sim.nF + c] = (T)sim.E[(size_t)s * sim.nF + c];
+      if (vel)
+        for (int c = 0; c < sim.nV * 3; c++) vel[(size_t)i * sim.nV * 3 + c] = (T)sim.V[(size_t)s * sim.nV * 3 + c];
+      if (pressure)
+        for (int c = 0; c < sim.nV; c++) pressure[(size_t)i * sim.nV + c] = (T)sim.Pfine[(size_t)s * sim.nV + c];
+    }
+    return n;
+  }
+  int state(int32_t *ids, float *f, float *v, float *p, float *e, int cap) override { return stateT(ids, f, v, p, e, cap); }
+  int state64(int32_t *ids, double *f, double *v, double *p, double *e, int cap) override {
+    return stateT(ids, f, v, p, e, cap);
+  }
+  void write(const float *fields, const float *vel) override {
+    int n = (int)sim.activeList.size();
+    for (int i = 0; i < n; i++) {
+      int id = sim.activeList[i];
+      int s = sim.pageSlot(sim.entries[id] & 0xffffff);
+      if (fields)
+        for (int c = 0; c < sim.nF * 4; c++) sim.F[(size_t)s * sim.nF * 4 + c] = (R)fields[(size_t)i * sim.nF * 4 + c];
+      if (vel)
+        for (int c = 0; c < sim.nV * 3; c++) sim.V[(size_t)s * sim.nV * 3 + c] = (R)vel[(size_t)i * sim.nV * 3 + c];
+    }
+  }
+  void set(double cutoff, double budget, int ground, int which) override {
+    if (which == 0) sim.cutoff = cutoff;
+    if (which == 1) sim.voxelBudget = budget;
+    if (which == 2) sim.ground = ground != 0;
+  }
+  const std::vector<float> &noise() override { return sim.noise; }
+  void contentTiles(int32_t *out, int *count) override {
+    int n = (int)sim.contentTiles.size();
+    if (out)
+      for (int i = 0; i < std::min(n, *count); i++) {
+        out[4 * i] = sim.contentTiles[i].first[0];
+        out[4 * i + 1] = sim.contentTiles[i].first[1];
+        out[4 * i + 2] = sim.contentTiles[i].first[2];
+        out[4 * i + 3] = (int32_t)sim.contentTiles[i].second;
+      }
+    *count = n;
+  }
+  void writeActivity(const uint32_t *flags, int n, const int32_t *tiles, int tileCount) override {
+    for (int i = 0; i < n && i < (int)sim.activity.size(); i++) sim.activity[i] = flags[i];
+    sim.contentTiles.clear();
+    for (int i = 0; i < tileCount; i++)
+      sim.contentTiles.push_back({{tiles[4 * i], tiles[4 * i + 1], tiles[4 * i + 2]}, (uint32_t)tiles[4 * i + 3]});
+  }
+  void readActivity(uint32_t *flags, int n) override {
+    for (int i = 0; i < n && i < (int)sim.activity.size(); i++) flags[i] = sim.activity[i];
+  }
+};
+
+template <class F> int guard(F fn) {
+  try {
+    fn();
+    return FP_OK;
+  } catch (const oracle::Error &e) {
+    g_error = e.what();
+    return FP_EINVAL;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return FP_ESTATE;
+  }
+}
+}  // namespace
+
+extern "C" {
+const char *orc_last_error(void) { return g_error.c_str(); }
+
+int orc_create(const fp_grid *grid, int bits, void **out) {
+  return guard([&] {
+    if (bits == 64) *out = new Impl<double>(*grid);
+    else *out = new Impl<float>(*grid);
+  });
+}
+void orc_destroy(void *h) { delete (Base *)h; }
+int orc_step(void *h, const fp_frame *f, double dt) { return guard([&] { ((Base *)h)->step(*f, dt); }); }
+int orc_reset(void *h) { return guard([&] { ((Base *)h)->reset(); }); }
+int orc_stats(void *h, fp_stats *s) { return guard([&] { ((Base *)h)->stats(s); }); }
+int orc_tables(void *h, fp_tables *t) { return guard([&] { ((Base *)h)->tables(t); }); }
+int orc_state(void *h, int32_t *ids, float *f, float *v, float *p, float *e, int cap) {
+  return ((Base *)h)->state(ids, f, v, p, e, cap);
+}
+int orc_state64(void *h, int32_t *ids, double *f, double *v, double *p, double *e, int cap) {
+  return ((Base *)h)->state64(ids, f, v, p, e, cap);
+}
+int orc_write_state(void *h, const float *f, const float *v) { return guard([&] { ((Base *)h)->write(f, v); }); }
+int orc_set_cutoff(void *h, double c) { return guard([&] { ((Base *)h)->set(c, 0, 0, 0); }); }
+int orc_set_budget(void *h, double v) { return guard([&] { ((Base *)h)->set(0, v, 0, 1); }); }
+int orc_set_ground(void *h, int g) { return guard([&] { ((Base *)h)->set(0, 0, g, 2); }); }
+int orc_noise(void *h, float *out) {
+  const auto &n = ((Base *)h)->noise();
+  std::memcpy(out, n.data(), n.size() * sizeof(float));
+  return FP_OK;
+}
+int orc_content_tiles(void *h, int32_t *out, int *count) {
+  ((Base *)h)->contentTiles(out, count);
+  return FP_OK;
+}
+int orc_write_activity(void *h, const uint32_t *flags, int n, const int32_t *tiles, int tileCount) {
+  return guard([&] { ((Base *)h)->writeActivity(flags, n, tiles, tileCount); });
+}
+int orc_read_activity(void *h, uint32_t *flags, int n) {
+  ((Base *)h)->readActivity(flags, n);
+  return FP_OK;
+}
+float orc_fexp(float x) { return oracle::fexp(x); }
+// Time `steps` of an oracle run, in seconds, for the CPU baseline.
+double orc_time_steps(void *h, const fp_frame *f, double dt, int steps) {
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < steps; i++) ((Base *)h)->step(*f, dt);
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+}

@@ -1,0 +1,166 @@
+// Device-side building blocks of the solver: parameters, directory lookups, trilinear
+// sampling and scalar math. Every float expression is written in the evaluation order the
+// kernels' contract fixes (no contraction: the library is built with -ffp-contract=off),
+// so a result does not depend on the compiler's scheduling.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fp {
+
+constexpr int FRESH_ENTRY = 0x2000000;        // tiles.wgsl:24
+constexpr uint32_t FRESH_SLOT = 0x80000000u;  // fluid-allocation.wgsl:30
+constexpr uint32_t CONTENT = 1u, EXPANDS = 2u, MAY_EXPAND = 4u;
+constexpr uint32_t BRICK_FREES = 8u, BRICK_REQUESTS = 16u;
+constexpr int RECORD_WORDS = 28, PRESSURE_FLAG = 27;
+constexpr int TILE_RECORD_WORDS = 32;
+constexpr uint8_t STORED_SOLID = 64, UNKNOWN = 128;
+constexpr int MAX_EMITTERS = 64, MAX_FORCES = 16, MAX_COLLIDERS = 16, MAX_REGIONS = 64;
+
+// Scene records as the kernels read them.
+struct Emitter {
+  float pos[3], size[3];
+  int box;
+  float flame, heat, smoke, response;
+  float vel[3];
+  float outward, period, strength, fuel;
+};
+struct Collider {
+  float center[3], extent[3];
+  int box;
+  float axes[3][3];  // columns
+};
+struct Region {
+  float center[3], kind, size[3];
+};
+struct ForceRec {
+  float axis[3], strength, center[3], kind, scale, lift, inward;
+  int targeted, firstRegion, regionCount;
+};
+
+// Per-step parameters (fluid-common.wgsl Params + FlameParams), by value.
+struct Params {
+  float hf, hv, dt;
+  float buoyancy, cooling, smokeWeight, damping, dissipation, vorticity, cutoff;
+  int lifetimeZero;
+  float time, seed;
+  uint32_t seedBits;
+  int colliders, ground;
+  float ignition, burnRate;
+  int fuelEnabled;
+  float fdt, lifespan, fsmoke, fheat, fexpansion;
+  int knotCount[3];
+  float knots[3][8][2];
+  int emitterCount, forceCount;
+  int maccormack;
+};
+
+// Geometry of the grids and pools (constant per simulation).
+struct Layout {
+  int B, b, sF, sV, nF, nV;  // field and velocity brick edges, shifts, cells
+  int side;                   // pool slots per row and rows per layer
+  int brickLevels;
+};
+
+// The tile directory and brick entries as device arrays.
+struct Dir {
+  const int *tileRec;  // [cap * 32]
+  const int *hash;     // [size * 4]
+  uint32_t hashMask;
+  const int *entries;  // [cap * 64]
+  const int *records;  // [cap * 64 * 28]
+  int capacity;
+};
+
+__device__ __forceinline__ uint32_t tileHash(int x, int y, int z) {
+  return ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+}
+
+// findTile (tiles.wgsl:57-73): open addressing, at most half full.
+__device__ __forceinline__ int findTile(const Dir &d, int x, int y, int z) {
+  uint32_t h = tileHash(x, y, z) & d.hashMask;
+  for (uint32_t probe = 0; probe <= d.hashMask; probe++) {
+    const int4 e = *reinterpret_cast<const int4 *>(d.hash + (size_t)h * 4);
+    if (e.w < 0) return -1;
+    if (e.x == x && e.y == y && e.z == z) return e.w;
+    h = (h + 1u) & d.hashMask;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int brickIdOf(const Dir &d, int bx, int by, int bz) {
+  int t = findTile(d, bx >> 2, by >> 2, bz >> 2);
+  if (t < 0) return -1;
+  return t * 64 + (((bz & 3) * 4 + (by & 3)) * 4 + (bx & 3));
+}
+
+__device__ __forceinline__ int entryLookup(const Dir &d, int bx, int by, int bz) {
+  int id = brickIdOf(d, bx, by, bz);
+  return id < 0 ? -1 : d.entries[id];
+}
+
+__device__ __forceinline__ int pageToSlot(int page, int side) {
+  uint32_t p = (uint32_t)page;
+  return (int)(((p >> 16) * (uint32_t)side + ((p >> 8) & 255u)) * (uint32_t)side + (p & 255u));
+}
+__device__ __forceinline__ int slotToPage(int slot, int side) {
+  uint32_t s = (uint32_t)slot, r = (uint32_t)side;
+  return (int)((s % r) | (((s / r) % r) << 8) | ((s / (r * r)) << 16));
+}
+__device__ __forceinline__ int entrySlot(int e, int side) {
+  return e < 0 ? -1 : pageToSlot(e & 0xffffff, side);
+}
+
+// A work item's home brick: its coordinates and the 27 neighbor entries of its record,
+// staged in LDS, so lookups within one brick need no hash probe.
+struct Home {
+  int bx, by, bz;
+  const int *rec;  // 27 entries (LDS)
+};
+
+__device__ __forceinline__ int entryAt(const Home &h, const Dir &d, int bx, int by, int bz) {
+  int dx = bx - h.bx, dy = by - h.by, dz = bz - h.bz;
+  if ((unsigned)(dx + 1) <= 2u && (unsigned)(dy + 1) <= 2u && (unsigned)(dz + 1) <= 2u)
+    return h.rec[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+  return entryLookup(d, bx, by, bz);
+}
+
+// ---- scalar math, identical trees on host and device -----------------------------------
+__device__ __forceinline__ float fminv(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float fmaxv(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ float fabsv(float a) { return a < 0.0f ? -a : a; }
+__device__ __forceinline__ float clampv(float x, float lo, float hi) { return fminv(fmaxv(x, lo), hi); }
+__device__ __forceinline__ float mixv(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float smoothstepv(float e0, float e1, float x) {
+  float t = clampv((x - e0) / (e1 - e0), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+__device__ __forceinline__ float length3(float x, float y, float z) { return __fsqrt_rn(x * x + y * y + z * z); }
+
+// exp in single precision: Cody & Waite reduction x = k ln2 + r with a two-part ln2,
+// the degree-7 Taylor polynomial of e^r (|r| <= ln2/2) in Horner form, and an exact
+// scaling by 2^k. Relative error below 2 ulp; evaluated with IEEE basic operations only.
+__device__ __forceinline__ float fexp(float x) {
+  if (!(x < 88.7228f)) return x != x ? x : __int_as_float(0x7f800000);
+  if (x < -103.972f) return 0.0f;
+  float k = floorf(x * 1.44269502f + 0.5f);
+  float r = x - k * 0.693145752f;
+  r = r - k * 1.42860677e-06f;
+  float p = 1.98412701e-04f;
+  p = p * r + 1.38888892e-03f;
+  p = p * r + 8.33333377e-03f;
+  p = p * r + 4.16666679e-02f;
+  p = p * r + 1.66666672e-01f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  int ki = (int)k;
+  if (ki > 127) {
+    p = p * 2.0f;
+    ki -= 1;
+  }
+  if (ki > -126) return p * __int_as_float((ki + 127) << 23);
+  return (p * __int_as_float((ki + 64 + 127) << 23)) * __int_as_float((127 - 64) << 23);
+}
+
+}  // namespace fp

@@ -1,0 +1,805 @@
+// Transport, combustion, forces, divergence and projection over the active bricks
+// (fluid-velocity-transport.wgsl, fluid-scalar-transport.wgsl, fluid-field-evolution.wgsl,
+// fluid-content.wgsl, fluid-forces.wgsl, force-fields.wgsl). One workgroup covers one
+// active brick (or a 256-cell chunk of one); the brick's neighbor record is staged in LDS.
+#include "kernels.hpp"
+
+namespace fp {
+
+#define BRICK_PREAMBLE(CHUNKS)                              \
+  const int bi = blockIdx.x / (CHUNKS);                     \
+  const int chunk = blockIdx.x % (CHUNKS);                  \
+  (void)chunk;                                              \
+  if (bi >= s.counts[0]) return;                            \
+  const int id = s.active[bi];                              \
+  __shared__ int rec[27];                                   \
+  Ctx c;                                                    \
+  c.dir = s.dir;                                            \
+  c.L = s.L;                                                \
+  c.side = s.L.side;                                        \
+  loadHome(s.dir, id, rec, c.home);                         \
+  const int slot = entrySlot(rec[13], s.L.side);
+
+__device__ __forceinline__ void velCellCoords(const Ctx &c, int l, int out[3]) {
+  const int b = c.L.b, sV = c.L.sV;
+  out[0] = c.home.bx * b + (l & (b - 1));
+  out[1] = c.home.by * b + ((l >> sV) & (b - 1));
+  out[2] = c.home.bz * b + (l >> (2 * sV));
+}
+__device__ __forceinline__ void fieldCellCoords(const Ctx &c, int l, int out[3]) {
+  const int B = c.L.B, sF = c.L.sF;
+  out[0] = c.home.bx * B + (l & (B - 1));
+  out[1] = c.home.by * B + ((l >> sF) & (B - 1));
+  out[2] = c.home.bz * B + (l >> (2 * sF));
+}
+
+// ---- velocity transport -------------------------------------------------------------
+
+// advectVelocityComponent (fluid-velocity-transport.wgsl:128-141).
+__device__ float advectComponent(const Ctx &c, const Params &P, const float *V, const int id[3],
+                                 const float pos[3], int comp, const float orig[3]) {
+  float off[3] = {0.f, 0.f, 0.f};
+  off[comp] = 0.5f * P.hv;
+  float fp[3], fv[3], mid[3], v[3];
+  for (int a = 0; a < 3; a++) fp[a] = pos[a] + off[a];
+  velocityOnFace(c, V, id, comp, orig, fv);
+  for (int a = 0; a < 3; a++) mid[a] = fp[a] - 0.5f * P.dt * fv[a];
+  velocityAt(c, P, V, mid[0], mid[1], mid[2], v);
+  float sp[3];
+  for (int a = 0; a < 3; a++) sp[a] = (fp[a] - P.dt * v[a] - off[a]) / P.hv;
+  return sampleVel(c, V, comp, sp[0], sp[1], sp[2]);
+}
+
+__global__ void __launch_bounds__(256) k_advect_velocity(StepArgs s) {
+  BRICK_PREAMBLE(1)
+  const int nV = s.L.nV;
+  float *out = s.p.Vadv + (size_t)slot * nV * 3;
+  const float *vs = s.p.V + (size_t)slot * nV * 3;
+  for (int l = threadIdx.x; l < nV; l += blockDim.x) {
+    int cc[3];
+    velCellCoords(c, l, cc);
+    float pos[3] = {((float)cc[0] + 0.5f) * s.P.hv, ((float)cc[1] + 0.5f) * s.P.hv, ((float)cc[2] + 0.5f) * s.P.hv};
+    if (solidAt(c, s.P, s.p.solids, pos[0], pos[1], pos[2])) {
+      out[l] = out[nV + l] = out[2 * nV + l] = 0.f;
+      continue;
+    }
+    float orig[3] = {vs[l], vs[nV + l], vs[2 * nV + l]};
+    for (int k = 0; k < 3; k++) out[k * nV + l] = advectComponent(c, s.P, s.p.V, cc, pos, k, orig);
+  }
+}
+
+// Donor bounds (fluid-velocity-transport.wgsl:165-212): the component's range over the
+// 2^3 cells from `base`, or [0, 0] when base's brick has no slot.
+__device__ void boundsAt(const Ctx &c, const float *V, const int base[3], int comp, float &lo, float &hi) {
+  const int sV = c.L.sV;
+  if (slotAtBrick(c, base[0] >> sV, base[1] >> sV, base[2] >> sV) < 0) {
+    lo = hi = 0.f;
+    return;
+  }
+  lo = 1e5f;
+  hi = -1e5f;
+  for (int z = 0; z < 2; z++)
+    for (int y = 0; y < 2; y++)
+      for (int x = 0; x < 2; x++) {
+        float v = velCell(c, V, comp, base[0] + x, base[1] + y, base[2] + z);
+        lo = fminv(lo, v);
+        hi = fmaxv(hi, v);
+      }
+}
+
+// correctVelocityComponent (fluid-velocity-transport.wgsl:214-255).
+__device__ float correctComponent(const Ctx &c, const Params &P, const Pools &p, const int id[3],
+                                  const float pos[3], int comp, const float orig[3], float predicted) {
+  float off[3] = {0.f, 0.f, 0.f};
+  off[comp] = 0.5f * P.hv;
+  float fp[3], fv[3], mid[3], v[3];
+  for (int a = 0; a < 3; a++) fp[a] = pos[a] + off[a];
+  velocityOnFace(c, p.V, id, comp, orig, fv);
+  for (int a = 0; a < 3; a++) mid[a] = fp[a] - 0.5f * P.dt * fv[a];
+  velocityAt(c, P, p.V, mid[0], mid[1], mid[2], v);
+  int base[3];
+  for (int a = 0; a < 3; a++) base[a] = (int)floorf((fp[a] - P.dt * v[a] - off[a]) / P.hv - 0.5f);
+  float lo, hi;
+  boundsAt(c, p.V, base, comp, lo, hi);
+  if (lo == hi && predicted == lo) return predicted;
+  float fmid[3], fw[3], sp[3];
+  for (int a = 0; a < 3; a++) fmid[a] = fp[a] + 0.5f * P.dt * fv[a];
+  velocityAt(c, P, p.V, fmid[0], fmid[1], fmid[2], fw);
+  for (int a = 0; a < 3; a++) sp[a] = (fp[a] + P.dt * fw[a] - off[a]) / P.hv;
+  float reversed = sampleVel(c, p.Vadv, comp, sp[0], sp[1], sp[2]);
+  float corrected = predicted + 0.5f * (orig[comp] - reversed);
+  return (corrected < lo || corrected > hi) ? predicted : corrected;
+}
+
+__global__ void __launch_bounds__(256) k_correct_velocity(StepArgs s) {
+  BRICK_PREAMBLE(1)
+  const int nV = s.L.nV;
+  float *out = s.p.Vcor + (size_t)slot * nV * 3;
+  const float *vs = s.p.V + (size_t)slot * nV * 3;
+  const float *as = s.p.Vadv + (size_t)slot * nV * 3;
+  for (int l = threadIdx.x; l < nV; l += blockDim.x) {
+    int cc[3];
+    velCellCoords(c, l, cc);
+    float pos[3] = {((float)cc[0] + 0.5f) * s.P.hv, ((float)cc[1] + 0.5f) * s.P.hv, ((float)cc[2] + 0.5f) * s.P.hv};
+    if (solidAt(c, s.P, s.p.solids, pos[0], pos[1], pos[2])) {
+      out[l] = out[nV + l] = out[2 * nV + l] = 0.f;
+      continue;
+    }
+    float orig[3] = {vs[l], vs[nV + l], vs[2 * nV + l]};
+    for (int k = 0; k < 3; k++) out[k * nV + l] = correctComponent(c, s.P, s.p, cc, pos, k, orig, as[k * nV + l]);
+  }
+}
+
+// ---- scalar transport -----------------------------------------------------------------
+
+struct Donors {
+  float mn[3], mx[3], lmn, lmx;  // smoke, heat, fuel; lifetime
+};
+
+// predictScalarsWithBounds (fluid-scalar-transport.wgsl:38-62).
+__device__ float4 predict(const Ctx &c, const float4 *F, const int base[3], const float frac[3], Donors &dn) {
+  for (int k = 0; k < 3; k++) {
+    dn.mn[k] = 65504.0f;
+    dn.mx[k] = -65504.0f;
+  }
+  dn.lmn = 1.0f;
+  dn.lmx = 0.0f;
+  float4 value = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int z = 0; z < 2; z++)
+    for (int y = 0; y < 2; y++)
+      for (int x = 0; x < 2; x++) {
+        float4 smp = fieldCell(c, F, base[0] + x, base[1] + y, base[2] + z);
+        dn.mn[0] = fminv(dn.mn[0], smp.x);
+        dn.mx[0] = fmaxv(dn.mx[0], smp.x);
+        dn.mn[1] = fminv(dn.mn[1], smp.y);
+        dn.mx[1] = fmaxv(dn.mx[1], smp.y);
+        dn.mn[2] = fminv(dn.mn[2], smp.w);
+        dn.mx[2] = fmaxv(dn.mx[2], smp.w);
+        dn.lmn = fminv(dn.lmn, smp.z);
+        dn.lmx = fmaxv(dn.lmx, smp.z);
+        float wx = x ? frac[0] : 1.0f - frac[0], wy = y ? frac[1] : 1.0f - frac[1], wz = z ? frac[2] : 1.0f - frac[2];
+        value.x += smp.x * wx * wy * wz;
+        value.y += smp.y * wx * wy * wz;
+        value.z += smp.z * wx * wy * wz;
+        value.w += smp.w * wx * wy * wz;
+      }
+  return value;
+}
+
+__device__ __forceinline__ void traceBack(const Ctx &c, const Params &P, const float *V, const float pos[3],
+                                          const float cv[3], float out[3]) {
+  float m[3], v[3];
+  for (int a = 0; a < 3; a++) m[a] = pos[a] - 0.5f * P.dt * cv[a];
+  velocityAt(c, P, V, m[0], m[1], m[2], v);
+  for (int a = 0; a < 3; a++) out[a] = pos[a] - P.dt * v[a];
+}
+__device__ __forceinline__ void traceForward(const Ctx &c, const Params &P, const float *V, const float pos[3],
+                                             const float cv[3], float out[3]) {
+  float m[3], v[3];
+  for (int a = 0; a < 3; a++) m[a] = pos[a] + 0.5f * P.dt * cv[a];
+  velocityAt(c, P, V, m[0], m[1], m[2], v);
+  for (int a = 0; a < 3; a++) out[a] = pos[a] + P.dt * v[a];
+}
+__device__ __forceinline__ void departureCell(const Params &P, const int id[3], const float pos[3],
+                                              const float dep[3], int base[3], float frac[3]) {
+  for (int a = 0; a < 3; a++) {
+    float cell = (float)id[a] + (dep[a] - pos[a]) / P.hf;
+    float fl = floorf(cell);
+    base[a] = (int)fl;
+    frac[a] = cell - fl;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_advect_scalars(StepArgs s, int chunks) {
+  BRICK_PREAMBLE(chunks)
+  const int nF = s.L.nF;
+  for (int l = chunk * 256 + threadIdx.x; l < nF; l += chunks * 256) {
+    int cc[3];
+    fieldCellCoords(c, l, cc);
+    float pos[3] = {((float)cc[0] + 0.5f) * s.P.hf, ((float)cc[1] + 0.5f) * s.P.hf, ((float)cc[2] + 0.5f) * s.P.hf};
+    float cv[3], dep[3];
+    velocityAt(c, s.P, s.p.V, pos[0], pos[1], pos[2], cv);
+    traceBack(c, s.P, s.p.V, pos, cv, dep);
+    float4 *out = s.p.Fp + (size_t)slot * nF + l;
+    if (!s.P.maccormack) {
+      float4 v = sampleField(c, s.p.F, dep[0] / s.P.hf, dep[1] / s.P.hf, dep[2] / s.P.hf);
+      *out = make_float4(v.x, v.y, clampv(v.z, 0.f, 1.f), v.w);
+      continue;
+    }
+    int base[3];
+    float frac[3];
+    departureCell(s.P, cc, pos, dep, base, frac);
+    Donors dn;
+    float4 v = predict(c, s.p.F, base, frac, dn);
+    *out = make_float4(v.x, v.y, s.P.lifetimeZero ? 0.f : v.z, v.w);
+  }
+}
+
+// correctedFineScalars (fluid-scalar-transport.wgsl:144-188), with the donor bounds
+// recomputed from the same backward trace instead of cached.
+__device__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p, const int id[3], int slot, int l,
+                                   const float pos[3], const float cv[3]) {
+  const float4 adv = p.Fp[(size_t)slot * c.L.nF + l];
+  if (!P.maccormack) return adv;
+  float dep[3];
+  traceBack(c, P, p.V, pos, cv, dep);
+  int base[3];
+  float frac[3];
+  departureCell(P, id, pos, dep, base, frac);
+  Donors dn;
+  predict(c, p.F, base, frac, dn);
+  bool same = dn.mn[0] == dn.mx[0] && dn.mn[1] == dn.mx[1] && dn.mn[2] == dn.mx[2] && dn.lmn == dn.lmx;
+  if (same) return make_float4(dn.mn[0], dn.mn[1], clampv(dn.lmn, 0.f, 1.f), dn.mn[2]);
+  float fw[3];
+  traceForward(c, P, p.V, pos, cv, fw);
+  const float4 orig = p.F[(size_t)slot * c.L.nF + l];
+  float lifetime = adv.z;
+  float rev[3];
+  if (dn.lmn != dn.lmx || adv.z != dn.lmn) {
+    // sampleScalarsExplicit (fluid-scalar-transport.wgsl:6-21) at the forward cell.
+    int bs[3];
+    float fr[3];
+    for (int a = 0; a < 3; a++) {
+      float cell = (float)id[a] + (fw[a] - pos[a]) / P.hf;
+      float fl = floorf(cell);
+      bs[a] = (int)fl;
+      fr[a] = cell - fl;
+    }
+    float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int z = 0; z < 2; z++)
+      for (int y = 0; y < 2; y++)
+        for (int x = 0; x < 2; x++) {
+          float4 smp = fieldCell(c, p.Fp, bs[0] + x, bs[1] + y, bs[2] + z);
+          float wx = x ? fr[0] : 1.0f - fr[0], wy = y ? fr[1] : 1.0f - fr[1], wz = z ? fr[2] : 1.0f - fr[2];
+          val.x += smp.x * wx * wy * wz;
+          val.y += smp.y * wx * wy * wz;
+          val.z += smp.z * wx * wy * wz;
+          val.w += smp.w * wx * wy * wz;
+        }
+    rev[0] = val.x;
+    rev[1] = val.y;
+    rev[2] = val.w;
+    float cl = adv.z + 0.5f * (orig.z - val.z);
+    lifetime = (cl < dn.lmn || cl > dn.lmx) ? adv.z : cl;
+  } else {
+    float4 val = sampleField(c, p.Fp, fw[0] / P.hf, fw[1] / P.hf, fw[2] / P.hf);
+    rev[0] = val.x;
+    rev[1] = val.y;
+    rev[2] = val.w;
+  }
+  const float a0 = adv.x, a1 = adv.y, a3 = adv.w;
+  float c0 = a0 + 0.5f * (orig.x - rev[0]);
+  float c1 = a1 + 0.5f * (orig.y - rev[1]);
+  float c3 = a3 + 0.5f * (orig.w - rev[2]);
+  float b0 = (c0 < dn.mn[0] || c0 > dn.mx[0]) ? a0 : c0;
+  float b1 = (c1 < dn.mn[1] || c1 > dn.mx[1]) ? a1 : c1;
+  float b3 = (c3 < dn.mn[2] || c3 > dn.mx[2]) ? a3 : c3;
+  return make_float4(b0, b1, clampv(lifetime, 0.f, 1.f), b3);
+}
+
+// ---- emission and combustion (fluid-sources.wgsl, flame-common.wgsl) -------------------
+
+__device__ float emitterWeight(const Emitter &e, const float w[3]) {
+  float d[3];
+  for (int a = 0; a < 3; a++) d[a] = (w[a] - e.pos[a]) / fmaxv(e.size[a], 0.03f);
+  float r = e.box ? fmaxv(fabsv(d[0]), fmaxv(fabsv(d[1]), fabsv(d[2]))) : length3(d[0], d[1], d[2]);
+  if (r >= 1.0f) return 0.f;
+  return 1.0f - smoothstepv(0.5f, 1.0f, r);
+}
+
+__device__ float burstGradient(uint32_t seedBits, int cx, int cy, int cz, float ox, float oy, float oz) {
+  uint32_t h = (uint32_t)cx * 374761393u + (uint32_t)cy * 668265263u + (uint32_t)cz * 2246822519u +
+               seedBits * 3266489917u;
+  h = (h ^ (h >> 13)) * 1274126177u;
+  h = (h ^ (h >> 16)) & 15u;
+  float a = h < 8u ? ox : oy;
+  float b = h < 4u ? oy : ((h == 12u || h == 14u) ? ox : oz);
+  return ((h & 1u) == 0u ? a : -a) + ((h & 2u) == 0u ? b : -b);
+}
+__device__ __forceinline__ float fade(float f) { return f * f * f * (f * (f * 6.0f - 15.0f) + 10.0f); }
+__device__ float burstPerlin(uint32_t seedBits, float px, float py, float pz) {
+  float flx = floorf(px), fly = floorf(py), flz = floorf(pz);
+  int cx = (int)flx, cy = (int)fly, cz = (int)flz;
+  float fx = px - flx, fy = py - fly, fz = pz - flz;
+  float sx = fade(fx), sy = fade(fy), sz = fade(fz);
+  float a = burstGradient(seedBits, cx, cy, cz, fx, fy, fz);
+  float b = burstGradient(seedBits, cx + 1, cy, cz, fx - 1.0f, fy, fz);
+  float cc = burstGradient(seedBits, cx, cy + 1, cz, fx, fy - 1.0f, fz);
+  float d = burstGradient(seedBits, cx + 1, cy + 1, cz, fx - 1.0f, fy - 1.0f, fz);
+  float e = burstGradient(seedBits, cx, cy, cz + 1, fx, fy, fz - 1.0f);
+  float g = burstGradient(seedBits, cx + 1, cy, cz + 1, fx - 1.0f, fy, fz - 1.0f);
+  float h = burstGradient(seedBits, cx, cy + 1, cz + 1, fx, fy - 1.0f, fz - 1.0f);
+  float j = burstGradient(seedBits, cx + 1, cy + 1, cz + 1, fx - 1.0f, fy - 1.0f, fz - 1.0f);
+  return mixv(mixv(mixv(a, b, sx), mixv(cc, d, sx), sy), mixv(mixv(e, g, sx), mixv(h, j, sx), sy), sz);
+}
+__device__ float burstPattern(uint32_t seedBits, float px, float py, float pz) {
+  float n = (burstPerlin(seedBits, px, py, pz) +
+             0.35f * burstPerlin(seedBits, px * 2.0f + 5.2f, py * 2.0f + 9.7f, pz * 2.0f + 3.1f)) /
+            1.35f;
+  return clampv(0.5f + 0.5f * n, 0.f, 1.f);
+}
+__device__ void burstVariation(uint32_t seedBits, const Emitter &e, const float w[3], float out[2]) {
+  if (e.strength == 0.f) {
+    out[0] = out[1] = 1.f;
+    return;
+  }
+  const float off[3] = {3.17f, 7.53f, 11.29f};
+  float q[3];
+  for (int a = 0; a < 3; a++) q[a] = (w[a] - e.pos[a]) / e.period + off[a];
+  float s0 = burstPattern(seedBits, q[0], q[1], q[2]);
+  float s1 = burstPattern(seedBits, q[0] + 17.3f, q[1] + 2.8f, q[2] + 8.6f);
+  out[0] = 1.0f - e.strength * s0;
+  out[1] = 1.0f - e.strength * s1;
+}
+
+__device__ float rampIntegral(const Params &P, int ch, float lo, float hi) {
+  float area = 0.f;
+  for (int i = 1; i < P.knotCount[ch]; i++) {
+    float ax = P.knots[ch][i - 1][0], ay = P.knots[ch][i - 1][1];
+    float bx = P.knots[ch][i][0], by = P.knots[ch][i][1];
+    float left = fmaxv(lo, ax), right = fminv(hi, bx);
+    if (right > left) {
+      float y0 = mixv(ay, by, (left - ax) / (bx - ax));
+      float y1 = mixv(ay, by, (right - ax) / (bx - ax));
+      area += (right - left) * (y0 + y1) * 0.5f;
+    }
+  }
+  return area;
+}
+__device__ void evaluateFlame(const Params &P, float before, float &after, float prod[3]) {
+  if (before <= 0.f) {
+    after = 0.f;
+    prod[0] = prod[1] = prod[2] = 0.f;
+    return;
+  }
+  after = fmaxv(0.f, before - P.fdt / P.lifespan);
+  prod[0] = P.fsmoke * P.lifespan * rampIntegral(P, 0, after, before);
+  prod[1] = P.fheat * P.lifespan * rampIntegral(P, 1, after, before);
+  prod[2] = P.fexpansion * (P.lifespan * rampIntegral(P, 2, after, before) / P.fdt);
+}
+
+// contentSupport (fluid-content.wgsl:29-39).
+__device__ uint32_t contentSupport(const Layout &L, const Params &P, const int id[3], const float vel[3]) {
+  float sidef = (float)L.B;
+  float ratio = P.hv / P.hf;
+  bool low[3], high[3];
+  for (int a = 0; a < 3; a++) {
+    float local = (float)(id[a] & (L.B - 1)) + 0.5f;
+    float pad = fminv(sidef, fmaxv(2.0f, 2.0f * ratio) + fabsv(vel[a]) * P.dt / P.hf);
+    low[a] = local < pad;
+    high[a] = local + pad >= sidef;
+  }
+  uint32_t row = 2u | (low[0] ? 1u : 0u) | (high[0] ? 4u : 0u);
+  uint32_t plane = (row << 3) | (low[1] ? row : 0u) | (high[1] ? row << 6 : 0u);
+  return (plane << 9) | (low[2] ? plane : 0u) | (high[2] ? plane << 18 : 0u);
+}
+
+// evolveFields (fluid-field-evolution.wgsl:208-283) with content recording.
+__global__ void __launch_bounds__(256) k_evolve_fields(StepArgs s, int chunks) {
+  BRICK_PREAMBLE(chunks)
+  __shared__ uint32_t busy, expands;
+  if (threadIdx.x == 0) {
+    busy = 0u;
+    expands = 0u;
+  }
+  __syncthreads();
+  const Params &P = s.P;
+  const int nF = s.L.nF;
+  uint32_t myBusy = 0u, myExpands = 0u;
+  for (int l = chunk * 256 + threadIdx.x; l < nF; l += chunks * 256) {
+    int cc[3];
+    fieldCellCoords(c, l, cc);
+    float pos[3] = {((float)cc[0] + 0.5f) * P.hf, ((float)cc[1] + 0.5f) * P.hf, ((float)cc[2] + 0.5f) * P.hf};
+    float value[4] = {0.f, 0.f, 0.f, 0.f};
+    float expansion = 0.f;
+    bool injected = false;
+    float cv[3];
+    velocityAt(c, P, s.p.V, pos[0], pos[1], pos[2], cv);
+    if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+      float4 tr = correctedScalars(c, P, s.p, cc, slot, l, pos, cv);
+      float lifetime = tr.z;
+      float sp0 = 0.f, sp1 = 0.f, injectedFuel = 0.f;
+      for (int i = 0; i < P.emitterCount; i++) {
+        const Emitter &e = s.emitters[i];
+        float wgt = emitterWeight(e, pos);
+        if (wgt == 0.f) continue;
+        float var[2];
+        burstVariation(P.seedBits, e, pos, var);
+        lifetime = fmaxv(lifetime, e.flame * wgt * var[1]);
+        sp0 += e.smoke * wgt * P.dt;
+        sp1 += e.heat * var[0] * wgt * P.dt;
+        injectedFuel += e.fuel * wgt * P.dt;
+      }
+      float remaining = 0.f, burned = 0.f;
+      if (P.fuelEnabled) {
+        remaining = tr.w + injectedFuel;
+        if (tr.y + sp1 >= P.ignition) {
+          lifetime = fmaxv(lifetime, fminv(remaining, 1.0f));
+          float unburned = remaining * fexp(-P.burnRate * P.dt);
+          burned = remaining - unburned;
+          remaining = unburned;
+        }
+      }
+      float after, prod[3];
+      evaluateFlame(P, clampv(lifetime, 0.f, 1.f), after, prod);
+      expansion = prod[2] + P.fexpansion * burned / P.dt;
+      float lossS = fexp(-P.dissipation * P.dt), lossH = fexp(-P.cooling * P.dt);
+      float halfS = fexp(-P.dissipation * P.dt * 0.5f), halfH = fexp(-P.cooling * P.dt * 0.5f);
+      float dens = tr.x * lossS + (sp0 + prod[0]) * halfS;
+      float heat = tr.y * lossH + (sp1 + prod[1]) * halfH;
+      value[0] = fmaxv(dens, 0.f);
+      value[1] = fmaxv(heat, 0.f);
+      value[2] = after;
+      value[3] = remaining;
+      injected = sp0 > 0.f || sp1 > 0.f || lifetime > tr.z || injectedFuel > 0.f;
+    }
+    if (!injected && value[2] == 0.f && value[0] < 0.001f && value[1] < 0.001f && value[3] < 0.001f)
+      value[0] = value[1] = value[2] = value[3] = 0.f;
+    s.p.Fn[(size_t)slot * nF + l] = make_float4(value[0], value[1], value[2], value[3]);
+    s.p.E[(size_t)slot * nF + l] = expansion;
+    if (expansion != 0.f) myExpands = 1u;
+    if (fmaxv(fmaxv(value[0], value[2]), value[3]) > P.cutoff) myBusy |= contentSupport(s.L, P, cc, cv);
+  }
+  if (myBusy) atomicOr(&busy, myBusy);
+  if (myExpands) atomicOr(&expands, 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t flags = 0u;
+    if (busy) {
+      flags = CONTENT | (busy << 3);
+      uint32_t local = (uint32_t)id & 63u;
+      uint32_t bits = (1u << (local & 3u)) | (16u << ((local >> 2) & 3u)) | (256u << (local >> 4));
+      atomicOr(&s.activity[s.brickCount + (id >> 6)], bits);
+    }
+    if (expands) flags |= EXPANDS;
+    if (flags) atomicOr(&s.activity[id], flags);
+  }
+}
+
+// ---- forces ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float curlCentered(const Ctx &c, const float *V, int x, int y, int z, int comp) {
+  int bx = x - (comp == 0), by = y - (comp == 1), bz = z - (comp == 2);
+  float a = velCell(c, V, comp, x, y, z), b = velCell(c, V, comp, bx, by, bz);
+  return 0.5f * (a + b);
+}
+
+__global__ void __launch_bounds__(256) k_curl(StepArgs s) {
+  BRICK_PREAMBLE(1)
+  const int nV = s.L.nV;
+  const float *V = s.p.Vcor;
+  for (int l = threadIdx.x; l < nV; l += blockDim.x) {
+    int p[3];
+    velCellCoords(c, l, p);
+    float hx = 0.5f / s.P.hv;
+    int x = p[0], y = p[1], z = p[2];
+    float dx0 = (curlCentered(c, V, x + 1, y, z, 1) - curlCentered(c, V, x - 1, y, z, 1)) * hx;
+    float dx1 = (curlCentered(c, V, x + 1, y, z, 2) - curlCentered(c, V, x - 1, y, z, 2)) * hx;
+    float dy0 = (curlCentered(c, V, x, y + 1, z, 0) - curlCentered(c, V, x, y - 1, z, 0)) * hx;
+    float dy1 = (curlCentered(c, V, x, y + 1, z, 2) - curlCentered(c, V, x, y - 1, z, 2)) * hx;
+    float dz0 = (curlCentered(c, V, x, y, z + 1, 0) - curlCentered(c, V, x, y, z - 1, 0)) * hx;
+    float dz1 = (curlCentered(c, V, x, y, z + 1, 1) - curlCentered(c, V, x, y, z - 1, 1)) * hx;
+    float c0 = dy1 - dz1, c1 = dz0 - dx1, c2 = dx0 - dy0;
+    s.p.Curl[(size_t)slot * nV + l] = make_float4(c0, c1, c2, length3(c0, c1, c2));
+  }
+}
+
+// curlNoise (fluid-forces.wgsl:3-9): the periodic table, trilinear with wrapping.
+__device__ void curlNoise(const float4 *noise, const float p[3], float out[3]) {
+  const float c0[3] = {0.36f, -0.8f, 0.48f}, c1[3] = {0.48f, 0.6f, 0.64f}, c2[3] = {-0.8f, 0.0f, 0.6f};
+  float u[3];
+  for (int a = 0; a < 3; a++) u[a] = (c0[a] * p[0] + c1[a] * p[1] + c2[a] * p[2]) / 16.0f;
+  float f[3];
+  int i0[3];
+  for (int a = 0; a < 3; a++) {
+    float t = u[a] * 64.0f - 0.5f;
+    float fl = floorf(t);
+    i0[a] = (int)fl;
+    f[a] = t - fl;
+  }
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  for (int z = 0; z < 2; z++)
+    for (int y = 0; y < 2; y++)
+      for (int x = 0; x < 2; x++) {
+        float w = (x ? f[0] : 1.0f - f[0]) * (y ? f[1] : 1.0f - f[1]) * (z ? f[2] : 1.0f - f[2]);
+        int ix = (i0[0] + x) & 63, iy = (i0[1] + y) & 63, iz = (i0[2] + z) & 63;
+        float4 t = noise[((size_t)iz * 64 + iy) * 64 + ix];
+        s0 += t.x * w;
+        s1 += t.y * w;
+        s2 += t.z * w;
+      }
+  out[0] = c0[0] * s0 + c0[1] * s1 + c0[2] * s2;
+  out[1] = c1[0] * s0 + c1[1] * s1 + c1[2] * s2;
+  out[2] = c2[0] * s0 + c2[1] * s1 + c2[2] * s2;
+}
+
+// coverage + externalForces (force-fields.wgsl:312-365).
+__device__ void externalForces(const StepArgs &s, const float p[3], float out[3]) {
+  const Params &P = s.P;
+  out[0] = out[1] = out[2] = 0.f;
+  for (int fi = 0; fi < P.forceCount; fi++) {
+    const ForceRec &f = s.forces[fi];
+    float weight = 1.f;
+    if (f.targeted) {
+      float covered = 0.f;
+      for (int r = 0; r < f.regionCount; r++) {
+        const Region &g = s.regions[f.firstRegion + r];
+        float dd[3];
+        for (int a = 0; a < 3; a++) dd[a] = fabsv((p[a] - g.center[a]) / fmaxv(g.size[a], 0.03f));
+        float dist = g.kind == 3.0f ? fmaxv(dd[0], fmaxv(dd[1], dd[2])) : length3(dd[0], dd[1], dd[2]);
+        covered = fmaxv(covered, 1.0f - smoothstepv(0.5f, 1.0f, dist));
+      }
+      weight = covered;
+    }
+    if (weight == 0.f) continue;
+    float acc[3] = {f.axis[0] * f.strength, f.axis[1] * f.strength, f.axis[2] * f.strength};
+    if (f.kind == 1.0f) {
+      float q[3] = {p[0] * f.scale * 0.65f + P.seed, p[1] * f.scale * 0.65f + -P.time * 0.7f,
+                    p[2] * f.scale * 0.65f + P.seed * 0.25f};
+      float n1[3], n2[3];
+      curlNoise(s.p.noise, q, n1);
+      float q2[3] = {q[0] * 2.03f + 13.7f, q[1] * 2.03f + 5.3f, q[2] * 2.03f + 19.1f};
+      curlNoise(s.p.noise, q2, n2);
+      for (int a = 0; a < 3; a++) acc[a] = (n1[a] + 0.45f * n2[a]) * f.strength;
+    } else if (f.kind >= 2.0f) {
+      float delta[3] = {p[0] - f.center[0], p[1] - f.center[1], p[2] - f.center[2]};
+      float radial[3] = {delta[0], delta[1], delta[2]};
+      if (f.kind == 2.0f) {
+        float dp = delta[0] * f.axis[0] + delta[1] * f.axis[1] + delta[2] * f.axis[2];
+        for (int a = 0; a < 3; a++) radial[a] = radial[a] - f.axis[a] * dp;
+      }
+      float distance = length3(radial[0], radial[1], radial[2]);
+      float radius = fmaxv(f.scale, 0.01f);
+      float dir[3];
+      for (int a = 0; a < 3; a++) dir[a] = radial[a] / fmaxv(distance, radius * 0.25f);
+      float rr = radial[0] * radial[0] + radial[1] * radial[1] + radial[2] * radial[2];
+      float falloff = 1.0f / (1.0f + rr / (radius * radius));
+      if (f.kind == 2.0f) {
+        float cr[3] = {f.axis[1] * dir[2] - f.axis[2] * dir[1], f.axis[2] * dir[0] - f.axis[0] * dir[2],
+                       f.axis[0] * dir[1] - f.axis[1] * dir[0]};
+        for (int a = 0; a < 3; a++) acc[a] = (cr[a] * f.strength - dir[a] * f.inward + f.axis[a] * f.lift) * falloff;
+      } else {
+        for (int a = 0; a < 3; a++) acc[a] = dir[a] * f.strength * falloff;
+      }
+    }
+    for (int a = 0; a < 3; a++) out[a] += acc[a] * weight;
+  }
+}
+
+// velocityCellAverage (fluid-sampling.wgsl:420-436) of the fields under velocity cell id.
+__device__ float4 velocityCellAverage(const Ctx &c, const float4 *F, int d, const int id[3]) {
+  if (d == 1) return fieldCell(c, F, id[0], id[1], id[2]);
+  int blocks = d / 2;
+  float ratio = (float)d;
+  float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int z = 0; z < blocks; z++)
+    for (int y = 0; y < blocks; y++)
+      for (int x = 0; x < blocks; x++) {
+        float4 v = sampleField(c, F, (float)id[0] * ratio + 1.0f + 2.0f * (float)x,
+                               (float)id[1] * ratio + 1.0f + 2.0f * (float)y,
+                               (float)id[2] * ratio + 1.0f + 2.0f * (float)z);
+        sum.x += v.x;
+        sum.y += v.y;
+        sum.z += v.z;
+        sum.w += v.w;
+      }
+  float n = (float)(blocks * blocks * blocks);
+  return make_float4(sum.x / n, sum.y / n, sum.z / n, sum.w / n);
+}
+
+// centerForce (fluid-forces.wgsl:79-104), zero where the brick does not compute.
+__device__ void cellForce(const Ctx &c, const StepArgs &s, const int p[3], float out[3]) {
+  const Params &P = s.P;
+  const int sV = c.L.sV;
+  if (entryAt(c.home, c.dir, p[0] >> sV, p[1] >> sV, p[2] >> sV) < 0) {
+    out[0] = out[1] = out[2] = 0.f;
+    return;
+  }
+  float w[3] = {((float)p[0] + 0.5f) * P.hv, ((float)p[1] + 0.5f) * P.hv, ((float)p[2] + 0.5f) * P.hv};
+  if (solidAt(c, P, s.p.solids, w[0], w[1], w[2])) {
+    out[0] = out[1] = out[2] = 0.f;
+    return;
+  }
+  int d = c.L.B / c.L.b;
+  float4 f = velocityCellAverage(c, s.p.F, d, p);
+  float conf[3] = {0.f, 0.f, 0.f};
+  if (P.vorticity != 0.f) {
+    float4 cc = curlCell(c, s.p.Curl, p[0], p[1], p[2]);
+    float eta[3] = {curlCell(c, s.p.Curl, p[0] + 1, p[1], p[2]).w - curlCell(c, s.p.Curl, p[0] - 1, p[1], p[2]).w,
+                    curlCell(c, s.p.Curl, p[0], p[1] + 1, p[2]).w - curlCell(c, s.p.Curl, p[0], p[1] - 1, p[2]).w,
+                    curlCell(c, s.p.Curl, p[0], p[1], p[2] + 1).w - curlCell(c, s.p.Curl, p[0], p[1], p[2] - 1).w};
+    float len = length3(eta[0], eta[1], eta[2]) + 0.0001f;
+    float n[3] = {eta[0] / len, eta[1] / len, eta[2] / len};
+    float cr[3] = {n[1] * cc.z - n[2] * cc.y, n[2] * cc.x - n[0] * cc.z, n[0] * cc.y - n[1] * cc.x};
+    for (int a = 0; a < 3; a++) conf[a] = cr[a] * P.vorticity * P.hv;
+  }
+  float ext[3];
+  externalForces(s, w, ext);
+  out[0] = conf[0] + ext[0] + 0.f;
+  out[1] = conf[1] + ext[1] + (P.buoyancy * f.y - P.smokeWeight * f.x);
+  out[2] = conf[2] + ext[2] + 0.f;
+}
+
+// sourceVelocity (fluid-forces.wgsl:122-149).
+__device__ float sourceVelocity(const StepArgs &s, float value, const float face[3], int comp) {
+  float result = value;
+  for (int i = 0; i < s.P.emitterCount; i++) {
+    const Emitter &e = s.emitters[i];
+    if (e.response == 0.f) continue;
+    float wgt = emitterWeight(e, face);
+    if (wgt == 0.f) continue;
+    float radial = 0.f;
+    if (e.outward != 0.f) {
+      float dd[3] = {face[0] - e.pos[0], face[1] - e.pos[1], face[2] - e.pos[2]};
+      radial = dd[comp] / fmaxv(length3(dd[0], dd[1], dd[2]), 1e-6f) * e.outward;
+    }
+    result = mixv(result, e.vel[comp] + radial, 1.0f - fexp(-e.response * wgt * s.P.dt));
+  }
+  return result;
+}
+
+// applyForces (fluid-forces.wgsl:151-182): forces of the brick and its +1 layer are
+// evaluated once into LDS and averaged onto each cell's three faces.
+constexpr int FORCE_TILE_EDGE = 9;  // velocity bricks up to 8 cells, plus one
+__global__ void __launch_bounds__(256) k_forces(StepArgs s) {
+  BRICK_PREAMBLE(1)
+  const Params &P = s.P;
+  const int b = s.L.b, nV = s.L.nV;
+  __shared__ float tile[FORCE_TILE_EDGE * FORCE_TILE_EDGE * FORCE_TILE_EDGE * 3];
+  const bool staged = b + 1 <= FORCE_TILE_EDGE;
+  const int e1 = b + 1;
+  if (staged) {
+    for (int t = threadIdx.x; t < e1 * e1 * e1; t += blockDim.x) {
+      int q[3] = {t % e1, (t / e1) % e1, t / (e1 * e1)};
+      int over = (q[0] == b) + (q[1] == b) + (q[2] == b);
+      if (over >= 2) continue;
+      int p[3] = {c.home.bx * b + q[0], c.home.by * b + q[1], c.home.bz * b + q[2]};
+      float f[3];
+      cellForce(c, s, p, f);
+      tile[t * 3 + 0] = f[0];
+      tile[t * 3 + 1] = f[1];
+      tile[t * 3 + 2] = f[2];
+    }
+  }
+  __syncthreads();
+  const float damp = fexp(-P.damping * P.dt);
+  const float *vc = s.p.Vcor + (size_t)slot * nV * 3;
+  float *o = s.p.Vf + (size_t)slot * nV * 3;
+  for (int l = threadIdx.x; l < nV; l += blockDim.x) {
+    int p[3];
+    velCellCoords(c, l, p);
+    float fc[3], fx[3], fy[3], fz[3];
+    if (staged) {
+      int q[3] = {p[0] - c.home.bx * b, p[1] - c.home.by * b, p[2] - c.home.bz * b};
+      int t = (q[2] * e1 + q[1]) * e1 + q[0];
+      for (int a = 0; a < 3; a++) {
+        fc[a] = tile[t * 3 + a];
+        fx[a] = tile[(t + 1) * 3 + a];
+        fy[a] = tile[(t + e1) * 3 + a];
+        fz[a] = tile[(t + e1 * e1) * 3 + a];
+      }
+    } else {
+      int px[3] = {p[0] + 1, p[1], p[2]}, py[3] = {p[0], p[1] + 1, p[2]}, pz[3] = {p[0], p[1], p[2] + 1};
+      cellForce(c, s, p, fc);
+      cellForce(c, s, px, fx);
+      cellForce(c, s, py, fy);
+      cellForce(c, s, pz, fz);
+    }
+    float force[3] = {0.5f * (fc[0] + fx[0]), 0.5f * (fc[1] + fy[1]), 0.5f * (fc[2] + fz[2])};
+    float pos[3] = {((float)p[0] + 0.5f) * P.hv, ((float)p[1] + 0.5f) * P.hv, ((float)p[2] + 0.5f) * P.hv};
+    float v[3];
+    for (int a = 0; a < 3; a++) v[a] = vc[a * nV + l] * damp + P.dt * force[a];
+    float half = 0.5f * P.hv;
+    float out[3];
+    for (int a = 0; a < 3; a++) {
+      float face[3] = {pos[0], pos[1], pos[2]};
+      face[a] = pos[a] + half;
+      out[a] = sourceVelocity(s, v[a], face, a);
+    }
+    if (solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+      o[l] = o[nV + l] = o[2 * nV + l] = 0.f;
+      continue;
+    }
+    for (int a = 0; a < 3; a++) {
+      float q[3] = {pos[0], pos[1], pos[2]};
+      q[a] = pos[a] + P.hv;
+      if (solidAt(c, P, s.p.solids, q[0], q[1], q[2])) out[a] = 0.f;
+      o[a * nV + l] = out[a];
+    }
+  }
+}
+
+// computeDivergence (fluid-forces.wgsl:184-203): rhs = h^2 (div u - expansion).
+__global__ void __launch_bounds__(256) k_divergence(StepArgs s, float *rhs0) {
+  BRICK_PREAMBLE(1)
+  const Params &P = s.P;
+  const int nV = s.L.nV;
+  const float *vf = s.p.Vf + (size_t)slot * nV * 3;
+  const int d = s.L.B / s.L.b;
+  for (int l = threadIdx.x; l < nV; l += blockDim.x) {
+    int p[3];
+    velCellCoords(c, l, p);
+    float h = P.hv;
+    float div = (vf[l] - velCell(c, s.p.Vf, 0, p[0] - 1, p[1], p[2])) / h +
+                (vf[nV + l] - velCell(c, s.p.Vf, 1, p[0], p[1] - 1, p[2])) / h +
+                (vf[2 * nV + l] - velCell(c, s.p.Vf, 2, p[0], p[1], p[2] - 1)) / h;
+    float avg;
+    if (d == 1) {
+      avg = fieldScalar(c, s.p.E, p[0], p[1], p[2]);
+    } else {
+      int blocks = d / 2;
+      float ratio = (float)d;
+      float sum = 0.f;
+      for (int z = 0; z < blocks; z++)
+        for (int y = 0; y < blocks; y++)
+          for (int x = 0; x < blocks; x++) {
+            float cx = (float)p[0] * ratio + 1.0f + 2.0f * (float)x, cy = (float)p[1] * ratio + 1.0f + 2.0f * (float)y,
+                  cz = (float)p[2] * ratio + 1.0f + 2.0f * (float)z;
+            float qx = cx - 0.5f, qy = cy - 0.5f, qz = cz - 0.5f;
+            float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+            int bx = (int)flx, by = (int)fly, bz = (int)flz;
+            float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+            float v = 0.f;
+            for (int zz = 0; zz < 2; zz++)
+              for (int yy = 0; yy < 2; yy++)
+                for (int xx = 0; xx < 2; xx++) {
+                  float w = (xx ? fx : 1.0f - fx) * (yy ? fy : 1.0f - fy) * (zz ? fz : 1.0f - fz);
+                  v += fieldScalar(c, s.p.E, bx + xx, by + yy, bz + zz) * w;
+                }
+            sum += v;
+          }
+      avg = sum / (float)(blocks * blocks * blocks);
+    }
+    rhs0[(size_t)slot * nV + l] = h * h * (div - avg);
+  }
+}
+
+// project (fluid-forces.wgsl:205-262).
+__device__ bool pressureUnknown(const Ctx &c, const Params &P, const int p[3]) {
+  const int sV = c.L.sV;
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = (p[a] - 1) >> sV;
+    hi[a] = p[a] >> sV;
+  }
+  if (P.ground) lo[1] = lo[1] > 0 ? lo[1] : 0;
+  for (int z = lo[2]; z <= hi[2]; z++)
+    for (int y = lo[1]; y <= hi[1]; y++)
+      for (int x = lo[0]; x <= hi[0]; x++)
+        if (entryAt(c.home, c.dir, x, y, z) < 0) return false;
+  return true;
+}
+
+__global__ void __launch_bounds__(256) k_project(StepArgs s, const float *pressure) {
+  BRICK_PREAMBLE(1)
+  const Params &P = s.P;
+  const int nV = s.L.nV;
+  const float *vf = s.p.Vf + (size_t)slot * nV * 3;
+  float *o = s.p.V + (size_t)slot * nV * 3;
+  for (int l = threadIdx.x; l < nV; l += blockDim.x) {
+    int p[3];
+    velCellCoords(c, l, p);
+    float pos[3] = {((float)p[0] + 0.5f) * P.hv, ((float)p[1] + 0.5f) * P.hv, ((float)p[2] + 0.5f) * P.hv};
+    if (solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+      o[l] = o[nV + l] = o[2 * nV + l] = 0.f;
+      continue;
+    }
+    float cp = pressure[(size_t)slot * nV + l];
+    bool here = pressureUnknown(c, P, p);
+    for (int a = 0; a < 3; a++) {
+      int nb[3] = {p[0], p[1], p[2]};
+      nb[a] += 1;
+      float npos[3] = {((float)nb[0] + 0.5f) * P.hv, ((float)nb[1] + 0.5f) * P.hv, ((float)nb[2] + 0.5f) * P.hv};
+      float value = vf[a * nV + l];
+      float res;
+      if (solidAt(c, P, s.p.solids, npos[0], npos[1], npos[2])) res = 0.f;
+      else if (!here && !pressureUnknown(c, P, nb)) res = 0.f;
+      else res = value - (velScalar(c, pressure, nb[0], nb[1], nb[2]) - cp) / P.hv;
+      o[a * nV + l] = res;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_correct_dummy() {}
+
+}  // namespace fp

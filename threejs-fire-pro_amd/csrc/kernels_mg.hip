@@ -1,0 +1,462 @@
+// Sparse geometric multigrid for the pressure (PressureSolver.ts:276-402, pressure.wgsl).
+// Brick levels halve the velocity brick down to 2^3 cells per pool slot; three tile levels
+// then hold 4^3, 2^3 and one cell per tile slot. Each step computes one byte per cell per
+// level: the cached open-side/solid topology plus whether the cell is a pressure unknown,
+// so every sweep reads a single flag byte per cell.
+#include "kernels.hpp"
+
+namespace fp {
+
+static __device__ __constant__ int OFF[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+struct MGCtx {
+  Dir dir;
+  Layout L;
+  int side;
+  bool hasBrick;
+  Home home;
+  bool hasTile;
+  int tc[3];
+  const int *tnb;  // the home tile's 27 neighbor slots (LDS)
+};
+
+__device__ __forceinline__ int levelShift(const Layout &L, int lv) {
+  return lv < L.brickLevels ? L.sV - lv : L.sV + 2 - lv;
+}
+
+__device__ __forceinline__ int tileSlotAt(const MGCtx &m, int x, int y, int z) {
+  if (m.hasTile) {
+    int dx = x - m.tc[0], dy = y - m.tc[1], dz = z - m.tc[2];
+    if ((unsigned)(dx + 1) <= 2u && (unsigned)(dy + 1) <= 2u && (unsigned)(dz + 1) <= 2u)
+      return m.tnb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+  }
+  return findTile(m.dir, x, y, z);
+}
+__device__ __forceinline__ int brickEntry(const MGCtx &m, int bx, int by, int bz) {
+  if (m.hasBrick) return entryAt(m.home, m.dir, bx, by, bz);
+  int t = tileSlotAt(m, bx >> 2, by >> 2, bz >> 2);
+  return t < 0 ? -1 : m.dir.entries[t * 64 + (((bz & 3) * 4 + (by & 3)) * 4 + (bx & 3))];
+}
+
+// cellAddress (pressure.wgsl:71-92).
+__device__ long addrAt(const MGCtx &m, const int p[3], int lv) {
+  int sh = levelShift(m.L, lv), sd = 1 << sh;
+  int ox = p[0] >> sh, oy = p[1] >> sh, oz = p[2] >> sh;
+  int owner = lv < m.L.brickLevels ? entrySlot(brickEntry(m, ox, oy, oz), m.side) : tileSlotAt(m, ox, oy, oz);
+  if (owner < 0) return -1;
+  int lx = p[0] & (sd - 1), ly = p[1] & (sd - 1), lz = p[2] & (sd - 1);
+  return (long)owner * sd * sd * sd + (lz * sd + ly) * sd + lx;
+}
+
+// activeCell (pressure.wgsl:101-132).
+__device__ bool activeCell(const MGCtx &m, const Params &P, const int p[3], int lv) {
+  const Layout &L = m.L;
+  if (lv >= L.brickLevels) {
+    int n = 1 << (lv - L.brickLevels);
+    for (int z = 0; z < n; z++)
+      for (int y = 0; y < n; y++)
+        for (int x = 0; x < n; x++) {
+          int bx = p[0] * n + x, by = p[1] * n + y, bz = p[2] * n + z;
+          int t = tileSlotAt(m, bx >> 2, by >> 2, bz >> 2);
+          if (t < 0) return false;
+          int id = t * 64 + (((bz & 3) * 4 + (by & 3)) * 4 + (bx & 3));
+          if (m.dir.entries[id] < 0 || m.dir.records[(size_t)id * RECORD_WORDS + PRESSURE_FLAG] == 0) return false;
+        }
+    return true;
+  }
+  int span = 1 << lv;
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = (p[a] * span - 1) >> L.sV;
+    hi[a] = (p[a] * span + span - 1) >> L.sV;
+  }
+  if (P.ground) lo[1] = lo[1] > 0 ? lo[1] : 0;
+  for (int z = lo[2]; z <= hi[2]; z++)
+    for (int y = lo[1]; y <= hi[1]; y++)
+      for (int x = lo[0]; x <= hi[0]; x++)
+        if (brickEntry(m, x, y, z) < 0) return false;
+  return true;
+}
+
+// solidAt for the pressure solver: through the directory, not a fluid home brick.
+__device__ bool solidAtMG(const MGCtx &m, const Params &P, const uint32_t *solids, float wx, float wy, float wz) {
+  int x = (int)floorf(wx / P.hf + 0.25f), y = (int)floorf(wy / P.hf + 0.25f), z = (int)floorf(wz / P.hf + 0.25f);
+  if (P.ground && y < 0) return true;
+  if (P.colliders == 0) return false;
+  const int sF = m.L.sF;
+  int s = entrySlot(brickEntry(m, x >> sF, y >> sF, z >> sF), m.side);
+  if (s < 0) return false;
+  int mk = m.L.B - 1;
+  uint32_t bit = (uint32_t)(((((z & mk) << sF) | (y & mk)) << sF) | (x & mk));
+  return ((solids[(size_t)s * (m.L.nF >> 5) + (bit >> 5)] >> (bit & 31u)) & 1u) != 0u;
+}
+__device__ bool blockedAt(const MGCtx &m, const Params &P, const uint32_t *solids, const int p[3], int lv) {
+  float scale = (float)(1 << lv);
+  return solidAtMG(m, P, solids, ((float)p[0] + 0.5f) * P.hv * scale, ((float)p[1] + 0.5f) * P.hv * scale,
+                   ((float)p[2] + 0.5f) * P.hv * scale);
+}
+// cellTopology (pressure.wgsl:225-235).
+__device__ uint8_t cellTopology(const MGCtx &m, const Params &P, const uint32_t *solids, const int p[3], int lv) {
+  if (blockedAt(m, P, solids, p, lv)) return STORED_SOLID;
+  uint8_t flags = 0;
+  for (int sd = 0; sd < 6; sd++) {
+    int q[3] = {p[0] + OFF[sd][0], p[1] + OFF[sd][1], p[2] + OFF[sd][2]};
+    if (!blockedAt(m, P, solids, q, lv)) flags |= (uint8_t)(1u << sd);
+  }
+  return flags;
+}
+
+__device__ __forceinline__ void levelCell(int home0, int home1, int home2, int sd, int l, int p[3]) {
+  p[0] = home0 * sd + l % sd;
+  p[1] = home1 * sd + (l / sd) % sd;
+  p[2] = home2 * sd + l / (sd * sd);
+}
+
+// Home set-up for a block over one active brick.
+#define MG_BRICK_HOME(ID)                          \
+  __shared__ int rec[27];                          \
+  MGCtx c;                                         \
+  c.dir = m.dir;                                   \
+  c.L = m.L;                                       \
+  c.side = m.L.side;                               \
+  c.hasBrick = true;                               \
+  c.hasTile = false;                               \
+  loadHome(m.dir, ID, rec, c.home);
+
+// Home set-up for a block over one tile slot; returns if the slot is free.
+#define MG_TILE_HOME(T)                                                         \
+  __shared__ int tnb[27];                                                       \
+  const int *trec = m.dir.tileRec + (size_t)(T) * TILE_RECORD_WORDS;            \
+  if (trec[3] == 0) return;                                                     \
+  if (threadIdx.x < 27) tnb[threadIdx.x] = trec[4 + threadIdx.x];               \
+  __syncthreads();                                                              \
+  MGCtx c;                                                                      \
+  c.dir = m.dir;                                                                \
+  c.L = m.L;                                                                    \
+  c.side = m.L.side;                                                            \
+  c.hasBrick = false;                                                           \
+  c.hasTile = true;                                                             \
+  c.tc[0] = trec[0];                                                            \
+  c.tc[1] = trec[1];                                                            \
+  c.tc[2] = trec[2];                                                            \
+  c.tnb = tnb;
+
+// buildBrickTopology (pressure.wgsl:259-278) for every active brick, or for the bricks
+// of the zeroed list that hold a slot.
+__global__ void __launch_bounds__(256) k_topology_bricks(MGArgs m, int fromZeroList) {
+  int id;
+  if (fromZeroList) {
+    if ((int)blockIdx.x >= m.counts[1] + m.counts[5]) return;
+    id = m.zeroList[2 * blockIdx.x];
+  } else {
+    if ((int)blockIdx.x >= m.counts[0]) return;
+    id = m.active[blockIdx.x];
+  }
+  if (m.dir.entries[id] < 0) return;
+  MG_BRICK_HOME(id)
+  int slot = entrySlot(rec[13], c.side);
+  for (int lv = 0; lv < m.L.brickLevels; lv++) {
+    int sd = m.lv[lv].side, n = sd * sd * sd;
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+      int p[3];
+      levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
+      m.lv[lv].topo[(size_t)slot * n + l] = cellTopology(c, m.P, m.solids, p, lv);
+    }
+  }
+}
+
+// buildTileTopology (pressure.wgsl:281-305) of one tile level, every tile slot.
+__global__ void __launch_bounds__(64) k_topology_tiles(MGArgs m, int lv) {
+  const int t = blockIdx.x;
+  const int sd = m.lv[lv].side, n = sd * sd * sd;
+  const int *trec0 = m.dir.tileRec + (size_t)t * TILE_RECORD_WORDS;
+  if (trec0[3] == 0) {
+    for (int l = threadIdx.x; l < n; l += blockDim.x) m.lv[lv].topo[(size_t)t * n + l] = 0;
+    return;
+  }
+  MG_TILE_HOME(t)
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    int p[3];
+    levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
+    m.lv[lv].topo[(size_t)t * n + l] = cellTopology(c, m.P, m.solids, p, lv);
+  }
+}
+
+// The step's flags of the brick levels: topology plus the unknown bit.
+__global__ void __launch_bounds__(256) k_eflags_bricks(MGArgs m) {
+  if ((int)blockIdx.x >= m.counts[0]) return;
+  const int id = m.active[blockIdx.x];
+  MG_BRICK_HOME(id)
+  int slot = entrySlot(rec[13], c.side);
+  for (int lv = 0; lv < m.L.brickLevels; lv++) {
+    int sd = m.lv[lv].side, n = sd * sd * sd;
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+      int p[3];
+      levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
+      size_t a = (size_t)slot * n + l;
+      m.lv[lv].eflag[a] = (uint8_t)(m.lv[lv].topo[a] | (activeCell(c, m.P, p, lv) ? UNKNOWN : 0));
+    }
+  }
+}
+__global__ void __launch_bounds__(64) k_eflags_tiles(MGArgs m, int lv) {
+  const int t = blockIdx.x;
+  MG_TILE_HOME(t)
+  const int sd = m.lv[lv].side, n = sd * sd * sd;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    int p[3];
+    levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
+    size_t a = (size_t)t * n + l;
+    m.lv[lv].eflag[a] = (uint8_t)(m.lv[lv].topo[a] | (activeCell(c, m.P, p, lv) ? UNKNOWN : 0));
+  }
+}
+
+__device__ __forceinline__ float readAt(const MGCtx &c, const float *v, const int p[3], int lv) {
+  long a = addrAt(c, p, lv);
+  return a < 0 ? 0.f : v[a];
+}
+__device__ __forceinline__ float openCount(int flags, int sd) {
+  return (float)((flags >> sd) & 1) + (float)((flags >> (sd + 1)) & 1);
+}
+__device__ __forceinline__ float diagonal(float w, int flags) {
+  return w * openCount(flags, 0) + w * openCount(flags, 2) + w * openCount(flags, 4);
+}
+// stencil (pressure.wgsl:376-402): the neighbor sum over open sides.
+__device__ float stencilSum(const MGCtx &c, const float *pv, const int p[3], int flags, int lv, float w) {
+  float n[6];
+  for (int sd = 0; sd < 6; sd++) {
+    if (!((flags >> sd) & 1)) {
+      n[sd] = 0.f;
+      continue;
+    }
+    int q[3] = {p[0] + OFF[sd][0], p[1] + OFF[sd][1], p[2] + OFF[sd][2]};
+    n[sd] = readAt(c, pv, q, lv);
+  }
+  return w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
+}
+
+// relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8.
+__device__ void relaxCell(const MGCtx &c, const Level &lv, int level, const int p[3], size_t a, float w) {
+  int e = lv.eflag[a];
+  if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
+    lv.q[a] = 0.f;
+    return;
+  }
+  int flags = e & 63;
+  float sum = stencilSum(c, lv.p, p, flags, level, w);
+  float cand = (sum - lv.rhs[a]) / fmaxv(diagonal(w, flags), 1e-12f);
+  lv.q[a] = mixv(lv.p[a], cand, 0.8f);
+}
+
+__global__ void __launch_bounds__(256) k_mg_init(MGArgs m) {
+  if ((int)blockIdx.x >= m.counts[0]) return;
+  const int id = m.active[blockIdx.x];
+  MG_BRICK_HOME(id)
+  int slot = entrySlot(rec[13], c.side);
+  const Level &lv = m.lv[0];
+  int sd = lv.side, n = sd * sd * sd;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    size_t a = (size_t)slot * n + l;
+    int e = lv.eflag[a];
+    float first = 0.f;
+    if (!(e & STORED_SOLID) && (e & UNKNOWN))
+      first = mixv(0.f, (0.f - lv.rhs[a]) / fmaxv(diagonal(m.weight, e & 63), 1e-12f), 0.8f);
+    lv.p[a] = first;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mg_relax_bricks(MGArgs m, int level) {
+  if ((int)blockIdx.x >= m.counts[0]) return;
+  const int id = m.active[blockIdx.x];
+  MG_BRICK_HOME(id)
+  int slot = entrySlot(rec[13], c.side);
+  const Level &lv = m.lv[level];
+  int sd = lv.side, n = sd * sd * sd;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    int p[3];
+    levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
+    relaxCell(c, lv, level, p, (size_t)slot * n + l, m.weight);
+  }
+}
+__global__ void __launch_bounds__(64) k_mg_relax_tiles(MGArgs m, int level) {
+  const int t = blockIdx.x;
+  MG_TILE_HOME(t)
+  const Level &lv = m.lv[level];
+  int sd = lv.side, n = sd * sd * sd;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    int p[3];
+    levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
+    relaxCell(c, lv, level, p, (size_t)t * n + l, m.weight);
+  }
+}
+
+// restrictCell (pressure.wgsl:601-647): the coarse right-hand side is half the sum of the
+// eight children's residuals; the coarse first iterate is one Jacobi step from zero.
+__device__ void restrictCell(const MGCtx &c, const MGArgs &m, int level, const int id[3]) {
+  const Level &fine = m.lv[level - 1], &coarse = m.lv[level];
+  float residual = 0.f;
+  bool every = true;
+  for (int z = 0; z < 2; z++)
+    for (int y = 0; y < 2; y++)
+      for (int x = 0; x < 2; x++) {
+        int p[3] = {id[0] * 2 + x, id[1] * 2 + y, id[2] * 2 + z};
+        long a = addrAt(c, p, level - 1);
+        int e = a < 0 ? 0 : fine.eflag[a];
+        bool unknown = (e & UNKNOWN) != 0;
+        every = every && unknown;
+        if (!unknown) continue;
+        if (e & STORED_SOLID) continue;
+        int flags = e & 63;
+        float sum = stencilSum(c, fine.p, p, flags, level - 1, m.weight);
+        float dg = diagonal(m.weight, flags);
+        residual += fine.rhs[a] - (sum - dg * fine.p[a]);
+      }
+  float rhs = residual * 0.5f;
+  long out = addrAt(c, id, level);
+  coarse.rhs[out] = rhs;
+  float first = 0.f;
+  if (every) {
+    int e = coarse.eflag[out];
+    if (!(e & STORED_SOLID)) first = mixv(0.f, (0.f - rhs) / fmaxv(diagonal(m.weight, e & 63), 1e-12f), 0.8f);
+  }
+  coarse.p[out] = first;
+}
+
+__global__ void __launch_bounds__(64) k_mg_restrict_bricks(MGArgs m, int level) {
+  if ((int)blockIdx.x >= m.counts[0]) return;
+  const int id = m.active[blockIdx.x];
+  MG_BRICK_HOME(id)
+  int sd = m.lv[level].side, n = sd * sd * sd;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    int p[3];
+    levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
+    restrictCell(c, m, level, p);
+  }
+}
+__global__ void __launch_bounds__(64) k_mg_restrict_tiles(MGArgs m, int level) {
+  const int t = blockIdx.x;
+  MG_TILE_HOME(t)
+  int sd = m.lv[level].side, n = sd * sd * sd;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    int p[3];
+    levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
+    restrictCell(c, m, level, p);
+  }
+}
+
+// coarseValue (pressure.wgsl:670-691): Neumann ghosts beside solids, a mirror below the floor.
+__device__ bool coarseSolid(const MGCtx &c, const Level &lv, const int q[3], int level) {
+  long a = addrAt(c, q, level);
+  if (a < 0) return false;
+  return (lv.topo[a] & STORED_SOLID) != 0;
+}
+__device__ float coarseValue(const MGCtx &c, const MGArgs &m, const int p[3], int level) {
+  const Level &cv = m.lv[level];
+  const int groundY = m.P.ground ? 0 : (int)0x80000000;
+  int cc[3] = {p[0], p[1] > groundY ? p[1] : groundY, p[2]};
+  if (!coarseSolid(c, cv, cc, level)) return readAt(c, cv.p, cc, level);
+  float sum = 0.f, count = 0.f;
+  for (int axis = 0; axis < 3; axis++)
+    for (int sd = -1; sd <= 1; sd += 2) {
+      int q[3] = {cc[0], cc[1], cc[2]};
+      q[axis] += sd;
+      q[1] = q[1] > groundY ? q[1] : groundY;
+      if (!coarseSolid(c, cv, q, level)) {
+        sum += readAt(c, cv.p, q, level);
+        count += 1.0f;
+      }
+    }
+  return sum / fmaxv(count, 1.0f);
+}
+
+// prolongateCell (pressure.wgsl:726-755): trilinear correction from the coarse level.
+__device__ void prolongCell(const MGCtx &c, const MGArgs &m, int level, const int p[3], size_t a) {
+  const Level &lv = m.lv[level];
+  int e = lv.eflag[a];
+  if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
+    lv.q[a] = 0.f;
+    return;
+  }
+  float f[3];
+  int i[3];
+  for (int k = 0; k < 3; k++) {
+    float q = ((float)p[k] + 0.5f) * 0.5f - 0.5f;
+    float fl = floorf(q);
+    i[k] = (int)fl;
+    f[k] = q - fl;
+  }
+  const bool plain = m.P.colliders == 0 && (!m.P.ground || i[1] >= 0);
+  float v[8];
+  for (int k = 0; k < 8; k++) {
+    int cc[3] = {i[0] + (k & 1), i[1] + ((k >> 1) & 1), i[2] + (k >> 2)};
+    v[k] = plain ? readAt(c, m.lv[level + 1].p, cc, level + 1) : coarseValue(c, m, cc, level + 1);
+  }
+  float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
+                    mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
+  lv.q[a] = lv.p[a] + corr;
+}
+
+__global__ void __launch_bounds__(256) k_mg_prolong_bricks(MGArgs m, int level) {
+  if ((int)blockIdx.x >= m.counts[0]) return;
+  const int id = m.active[blockIdx.x];
+  MG_BRICK_HOME(id)
+  int slot = entrySlot(rec[13], c.side);
+  int sd = m.lv[level].side, n = sd * sd * sd;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    int p[3];
+    levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
+    prolongCell(c, m, level, p, (size_t)slot * n + l);
+  }
+}
+__global__ void __launch_bounds__(64) k_mg_prolong_tiles(MGArgs m, int level) {
+  const int t = blockIdx.x;
+  MG_TILE_HOME(t)
+  int sd = m.lv[level].side, n = sd * sd * sd;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    int p[3];
+    levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
+    prolongCell(c, m, level, p, (size_t)t * n + l);
+  }
+}
+
+// relaxCoarse (pressure.wgsl:549-599): every bottom iteration in one workgroup, a tile slot
+// per lane, values in LDS. The result lands in lv.p.
+constexpr int BOTTOM_LANES = 1024;
+__global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom(MGArgs m, int iterations) {
+  __shared__ float value[BOTTOM_LANES];
+  const int lane = threadIdx.x;
+  const Level &lv = m.lv[m.bottom];
+  const int cap = m.dir.capacity;
+  const int *trec = m.dir.tileRec + (size_t)lane * TILE_RECORD_WORDS;
+  bool valid = lane < cap && trec[3] != 0;
+  int e = valid ? lv.eflag[lane] : 0;
+  bool live = valid && (e & UNKNOWN) && !(e & STORED_SOLID);
+  int flags = e & 63;
+  float rhs = valid ? lv.rhs[lane] : 0.f;
+  float mine = valid ? lv.p[lane] : 0.f;
+  if (valid && !(e & UNKNOWN)) mine = 0.f;
+  value[lane] = mine;
+  __syncthreads();
+  for (int it = 0; it < iterations; it++) {
+    float next = 0.f;
+    if (live) {
+      float n[6];
+      for (int sd = 0; sd < 6; sd++) {
+        n[sd] = 0.f;
+        if (!((flags >> sd) & 1)) continue;
+        int nb = trec[4 + (OFF[sd][2] + 1) * 9 + (OFF[sd][1] + 1) * 3 + (OFF[sd][0] + 1)];
+        if (nb >= 0) n[sd] = value[nb];
+      }
+      float w = m.weight;
+      float sum = w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
+      float cand = (sum - rhs) / fmaxv(diagonal(w, flags), 1e-12f);
+      next = mixv(mine, cand, 0.8f);
+    }
+    __syncthreads();
+    value[lane] = next;
+    mine = next;
+    __syncthreads();
+  }
+  if (valid) lv.p[lane] = mine;
+}
+
+}  // namespace fp

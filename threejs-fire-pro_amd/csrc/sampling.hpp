@@ -1,0 +1,145 @@
+// Cell access and trilinear sampling over the sparse brick pools, for kernels whose work
+// item lies in one active brick (the home brick, whose record is staged in LDS).
+#pragma once
+#include "device.hpp"
+
+namespace fp {
+
+// Device pools, slot-major. Field cells are float4 (smoke, heat, lifetime, fuel); velocity
+// is three planes per slot (the MAC faces +x, +y, +z of each cell).
+struct Pools {
+  float4 *F, *Fp, *Fn;
+  float *E;
+  float *V, *Vadv, *Vcor, *Vf;
+  float4 *Curl;
+  float *D;
+  uint32_t *solids;
+  const float4 *noise;
+};
+
+struct Ctx {
+  Home home;
+  Dir dir;
+  Layout L;
+  int side;
+};
+
+__device__ __forceinline__ int slotAtBrick(const Ctx &c, int bx, int by, int bz) {
+  return entrySlot(entryAt(c.home, c.dir, bx, by, bz), c.side);
+}
+
+__device__ __forceinline__ float4 fieldCell(const Ctx &c, const float4 *buf, int x, int y, int z) {
+  int s = slotAtBrick(c, x >> c.L.sF, y >> c.L.sF, z >> c.L.sF);
+  if (s < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+  int m = c.L.B - 1;
+  return buf[(size_t)s * c.L.nF + (((z & m) * c.L.B + (y & m)) * c.L.B + (x & m))];
+}
+__device__ __forceinline__ float fieldScalar(const Ctx &c, const float *buf, int x, int y, int z) {
+  int s = slotAtBrick(c, x >> c.L.sF, y >> c.L.sF, z >> c.L.sF);
+  if (s < 0) return 0.f;
+  int m = c.L.B - 1;
+  return buf[(size_t)s * c.L.nF + (((z & m) * c.L.B + (y & m)) * c.L.B + (x & m))];
+}
+__device__ __forceinline__ float velCell(const Ctx &c, const float *buf, int comp, int x, int y, int z) {
+  int s = slotAtBrick(c, x >> c.L.sV, y >> c.L.sV, z >> c.L.sV);
+  if (s < 0) return 0.f;
+  int m = c.L.b - 1;
+  return buf[(size_t)s * c.L.nV * 3 + (size_t)comp * c.L.nV + (((z & m) * c.L.b + (y & m)) * c.L.b + (x & m))];
+}
+__device__ __forceinline__ float velScalar(const Ctx &c, const float *buf, int x, int y, int z) {
+  int s = slotAtBrick(c, x >> c.L.sV, y >> c.L.sV, z >> c.L.sV);
+  if (s < 0) return 0.f;
+  int m = c.L.b - 1;
+  return buf[(size_t)s * c.L.nV + (((z & m) * c.L.b + (y & m)) * c.L.b + (x & m))];
+}
+__device__ __forceinline__ float4 curlCell(const Ctx &c, const float4 *buf, int x, int y, int z) {
+  int s = slotAtBrick(c, x >> c.L.sV, y >> c.L.sV, z >> c.L.sV);
+  if (s < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+  int m = c.L.b - 1;
+  return buf[(size_t)s * c.L.nV + (((z & m) * c.L.b + (y & m)) * c.L.b + (x & m))];
+}
+
+// solidCell / solidAt (fluid-common.wgsl:123-150).
+__device__ __forceinline__ bool solidCell(const Ctx &c, const Params &P, const uint32_t *solids, int x, int y,
+                                          int z) {
+  if (P.ground && y < 0) return true;
+  if (P.colliders == 0) return false;
+  int s = slotAtBrick(c, x >> c.L.sF, y >> c.L.sF, z >> c.L.sF);
+  if (s < 0) return false;
+  int m = c.L.B - 1, sh = c.L.sF;
+  uint32_t bit = (uint32_t)(((((z & m) << sh) | (y & m)) << sh) | (x & m));
+  return ((solids[(size_t)s * (c.L.nF >> 5) + (bit >> 5)] >> (bit & 31u)) & 1u) != 0u;
+}
+__device__ __forceinline__ bool solidAt(const Ctx &c, const Params &P, const uint32_t *solids, float wx,
+                                        float wy, float wz) {
+  int x = (int)floorf(wx / P.hf + 0.25f), y = (int)floorf(wy / P.hf + 0.25f), z = (int)floorf(wz / P.hf + 0.25f);
+  return solidCell(c, P, solids, x, y, z);
+}
+
+// Trilinear sample of the field grid, position in cell units with centers at .5
+// (fluid-sampling.wgsl:235-264): cell values weighted by (wx * wy) * wz.
+__device__ __forceinline__ float4 sampleField(const Ctx &c, const float4 *buf, float px, float py, float pz) {
+  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
+  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+  int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int z = 0; z < 2; z++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
+        float w = wx * wy * wz;
+        float4 v = fieldCell(c, buf, bx + x, by + y, bz + z);
+        out.x += v.x * w;
+        out.y += v.y * w;
+        out.z += v.z * w;
+        out.w += v.w * w;
+      }
+  return out;
+}
+__device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int comp, float px, float py,
+                                           float pz) {
+  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
+  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+  int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+  float out = 0.f;
+#pragma unroll
+  for (int z = 0; z < 2; z++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
+        out += velCell(c, buf, comp, bx + x, by + y, bz + z) * (wx * wy * wz);
+      }
+  return out;
+}
+// velocityAt (fluid-advection.wgsl:10-21).
+__device__ __forceinline__ void velocityAt(const Ctx &c, const Params &P, const float *V, float wx, float wy,
+                                           float wz, float out[3]) {
+  out[0] = sampleVel(c, V, 0, (wx - 0.5f * P.hv) / P.hv, wy / P.hv, wz / P.hv);
+  out[1] = sampleVel(c, V, 1, wx / P.hv, (wy - 0.5f * P.hv) / P.hv, wz / P.hv);
+  out[2] = sampleVel(c, V, 2, wx / P.hv, wy / P.hv, (wz - 0.5f * P.hv) / P.hv);
+}
+// velocityOnFace (fluid-advection.wgsl:26-44).
+__device__ __forceinline__ void velocityOnFace(const Ctx &c, const float *V, const int id[3], int comp,
+                                               const float orig[3], float out[3]) {
+  float fp[3];
+  for (int a = 0; a < 3; a++) fp[a] = (float)id[a] + 0.5f;
+  fp[comp] += 0.5f;
+  for (int k = 0; k < 3; k++) {
+    if (k == comp) {
+      out[k] = orig[k];
+      continue;
+    }
+    float pos[3] = {fp[0], fp[1], fp[2]};
+    pos[k] = fp[k] - 0.5f;
+    out[k] = sampleVel(c, V, k, pos[0], pos[1], pos[2]);
+  }
+}
+
+}  // namespace fp

@@ -1,0 +1,326 @@
+"""ctypes mirror of include/firepro.h and the loader of the HIP library.
+
+The structures follow the reference's plain-data frame (src/engine/types.ts:235-311):
+`SimulationFrame`, `SimulationSource`, `SimulationForce`, `SimulationCollider`. The loader
+fails loudly when libfirepro.so is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libfirepro.so")
+
+FP_OK, FP_EINVAL, FP_EDEVICE, FP_ESTATE, FP_ENOMEM = 0, -1, -2, -3, -4
+SHAPES = {"sphere": 0, "box": 1}
+FORCE_TYPES = {"wind": 0, "turbulence": 1, "vortex": 2, "radial": 3}
+MAX_FLAME_KNOTS = 8
+BRICK_RECORD_WORDS = 28
+FRESH_ENTRY = 0x2000000
+FRESH_SLOT = 0x80000000
+
+Vec3 = C.c_double * 3
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("voxel_size", C.c_double),
+        ("velocity_divisor", C.c_int32),
+        ("brick_size", C.c_int32),
+        ("smoke_divisor", C.c_int32),
+        ("scalar_maccormack", C.c_int32),
+    ]
+
+
+class Settings(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "buoyancy", "smoke_weight", "velocity_damping", "cooling", "dissipation", "vorticity",
+        "seed")]
+
+
+class Flame(C.Structure):
+    _fields_ = [
+        ("lifespan", C.c_double), ("smoke_rate", C.c_double), ("heat_rate", C.c_double),
+        ("expansion_rate", C.c_double),
+        ("knot_count", C.c_int32 * 3),
+        ("knots", ((C.c_double * 2) * MAX_FLAME_KNOTS) * 3),
+    ]
+
+
+class Fuel(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("ignition_heat", C.c_double), ("burn_rate", C.c_double)]
+
+
+class Source(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint64), ("shape", C.c_int32),
+        ("position", Vec3), ("size", Vec3),
+        ("flame", C.c_double), ("heat_rate", C.c_double), ("smoke_rate", C.c_double),
+        ("fuel_rate", C.c_double),
+        ("velocity", Vec3),
+        ("velocity_response", C.c_double), ("outward_speed", C.c_double),
+        ("has_variation", C.c_int32),
+        ("variation_period", C.c_double), ("variation_strength", C.c_double),
+    ]
+
+
+class Force(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("vector", Vec3), ("center", Vec3),
+        ("strength", C.c_double), ("scale", C.c_double), ("lift", C.c_double),
+        ("inward", C.c_double),
+        ("targeted", C.c_int32), ("source_count", C.c_int32),
+        ("sources", C.POINTER(Source)),
+    ]
+
+
+class Collider(C.Structure):
+    _fields_ = [("shape", C.c_int32), ("position", Vec3), ("size", Vec3),
+                ("axes", (C.c_double * 3) * 3)]
+
+
+class Frame(C.Structure):
+    _fields_ = [
+        ("simulation", Settings), ("combustion", Flame),
+        ("has_fuel", C.c_int32), ("fuel", Fuel),
+        ("source_count", C.c_int32), ("sources", C.POINTER(Source)),
+        ("force_count", C.c_int32), ("forces", C.POINTER(Force)),
+        ("collider_count", C.c_int32), ("colliders", C.POINTER(Collider)),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("time", C.c_double), ("steps", C.c_int64), ("active_bricks", C.c_int64),
+        ("active_voxels", C.c_int64), ("pool_slots", C.c_int64), ("used_slots", C.c_int64),
+        ("pool_limited", C.c_int32), ("tile_capacity", C.c_int32), ("tile_count", C.c_int32),
+        ("pool_side", C.c_int32), ("memory_bytes", C.c_int64),
+    ]
+
+
+class Tables(C.Structure):
+    _fields_ = [
+        ("pages", C.POINTER(C.c_int32)), ("entries", C.POINTER(C.c_int32)),
+        ("records", C.POINTER(C.c_int32)), ("free_slots", C.POINTER(C.c_uint32)),
+        ("active", C.POINTER(C.c_int32)), ("zeroed", C.POINTER(C.c_int32)),
+        ("brick_count", C.c_int32), ("free_count", C.c_int32), ("failures", C.c_int32),
+        ("active_count", C.c_int32), ("zeroed_count", C.c_int32), ("pool_slots", C.c_int32),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [("inverse_view_projection", C.c_double * 16), ("position", Vec3),
+                ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class RenderSettings(C.Structure):
+    _fields_ = [
+        ("fire_intensity", C.c_double), ("flame_opacity", C.c_double),
+        ("smoke_density", C.c_double), ("scattering", C.c_double),
+        ("shadow_density", C.c_double), ("temperature", C.c_double), ("soot_glow", C.c_double),
+        ("smoke_color", Vec3), ("fire_color", Vec3),
+        ("sun_direction", Vec3), ("sun_intensity", C.c_double),
+        ("filter", C.c_int32), ("ray_steps", C.c_int32), ("lighting_divisor", C.c_int32),
+    ]
+
+
+# ---- plain-data frame (SimulationFrame, src/engine/types.ts:235-311) --------------------
+
+DEFAULT_RAMPS = {  # DEFAULT_FLAME_SETTINGS.ramps (src/engine/flame.ts:14-38)
+    "smoke": [(0, 0), (0.2, 1), (0.5, 0), (1, 0)],
+    "heat": [(0, 0), (0.2, 1), (1, 1)],
+    "expansion": [(0, 0), (0.5, 0), (0.75, 1), (1, 1)],
+}
+
+
+@dataclass
+class SourceData:
+    id: int
+    shape: str = "sphere"
+    position: Sequence[float] = (0.0, 0.0, 0.0)
+    size: Sequence[float] = (0.3, 0.3, 0.3)
+    flame: float = 0.0
+    heat_rate: float = 0.0
+    smoke_rate: float = 0.0
+    fuel_rate: float = 0.0
+    velocity: Sequence[float] = (0.0, 0.0, 0.0)
+    velocity_response: float = 0.0
+    outward_speed: float = 0.0
+    variation: Optional[Sequence[float]] = None  # (period, strength)
+
+
+@dataclass
+class ForceData:
+    type: str
+    vector: Sequence[float] = (0.0, 1.0, 0.0)
+    center: Sequence[float] = (0.0, 0.0, 0.0)
+    strength: float = 0.0
+    scale: float = 1.0
+    lift: float = 0.0
+    inward: float = 0.0
+    sources: Optional[List[SourceData]] = None
+
+
+@dataclass
+class ColliderData:
+    shape: str
+    position: Sequence[float]
+    size: Sequence[float]
+    axes: Sequence[Sequence[float]] = ((1, 0, 0), (0, 1, 0), (0, 0, 1))
+
+
+@dataclass
+class FrameData:
+    buoyancy: float = 1.6
+    smoke_weight: float = 0.12
+    velocity_damping: float = 0.1
+    cooling: float = 0.62
+    dissipation: float = 0.35
+    vorticity: float = 3.2
+    seed: float = 1
+    lifespan: float = 1.2
+    flame_smoke_rate: float = 2.0
+    flame_heat_rate: float = 4.0
+    expansion_rate: float = 1.0
+    ramps: dict = field(default_factory=lambda: dict(DEFAULT_RAMPS))
+    fuel: Optional[Sequence] = None  # (enabled, ignition_heat, burn_rate)
+    sources: List[SourceData] = field(default_factory=list)
+    forces: List[ForceData] = field(default_factory=list)
+    colliders: List[ColliderData] = field(default_factory=list)
+
+
+def _source(s: SourceData) -> Source:
+    out = Source()
+    out.id = s.id
+    out.shape = SHAPES[s.shape]
+    out.position = Vec3(*s.position)
+    out.size = Vec3(*s.size)
+    out.flame, out.heat_rate, out.smoke_rate, out.fuel_rate = (
+        s.flame, s.heat_rate, s.smoke_rate, s.fuel_rate)
+    out.velocity = Vec3(*s.velocity)
+    out.velocity_response, out.outward_speed = s.velocity_response, s.outward_speed
+    if s.variation is not None:
+        out.has_variation = 1
+        out.variation_period, out.variation_strength = s.variation
+    return out
+
+
+class PackedFrame:
+    """An fp_frame and the arrays it points into, kept alive together."""
+
+    def __init__(self, f: FrameData):
+        fr = Frame()
+        fr.simulation = Settings(f.buoyancy, f.smoke_weight, f.velocity_damping, f.cooling,
+                                 f.dissipation, f.vorticity, f.seed)
+        fl = Flame()
+        fl.lifespan, fl.smoke_rate, fl.heat_rate, fl.expansion_rate = (
+            f.lifespan, f.flame_smoke_rate, f.flame_heat_rate, f.expansion_rate)
+        for ch, name in enumerate(("smoke", "heat", "expansion")):
+            knots = f.ramps[name]
+            fl.knot_count[ch] = len(knots)
+            for i, (x, y) in enumerate(knots[:MAX_FLAME_KNOTS]):
+                fl.knots[ch][i][0] = x
+                fl.knots[ch][i][1] = y
+        fr.combustion = fl
+        if f.fuel is not None:
+            fr.has_fuel = 1
+            fr.fuel = Fuel(int(bool(f.fuel[0])), f.fuel[1], f.fuel[2])
+        self.sources = (Source * max(1, len(f.sources)))(*[_source(s) for s in f.sources])
+        fr.source_count = len(f.sources)
+        fr.sources = self.sources
+        self.force_sources = []
+        forces = []
+        for fd in f.forces:
+            fo = Force()
+            fo.type = FORCE_TYPES[fd.type]
+            fo.vector = Vec3(*fd.vector)
+            fo.center = Vec3(*fd.center)
+            fo.strength, fo.scale, fo.lift, fo.inward = fd.strength, fd.scale, fd.lift, fd.inward
+            if fd.sources is not None:
+                fo.targeted = 1
+                arr = (Source * max(1, len(fd.sources)))(*[_source(s) for s in fd.sources])
+                self.force_sources.append(arr)
+                fo.source_count = len(fd.sources)
+                fo.sources = arr
+            forces.append(fo)
+        self.forces = (Force * max(1, len(forces)))(*forces)
+        fr.force_count = len(forces)
+        fr.forces = self.forces
+        cols = []
+        for cd in f.colliders:
+            co = Collider()
+            co.shape = SHAPES[cd.shape]
+            co.position = Vec3(*cd.position)
+            co.size = Vec3(*cd.size)
+            for k in range(3):
+                for a in range(3):
+                    co.axes[k][a] = cd.axes[k][a]
+            cols.append(co)
+        self.colliders = (Collider * max(1, len(cols)))(*cols)
+        fr.collider_count = len(cols)
+        fr.colliders = self.colliders
+        self.frame = fr
+
+    def pointer(self):
+        return C.byref(self.frame)
+
+
+# ---- the HIP library ------------------------------------------------------------------
+
+_lib = None
+
+
+class FireProError(RuntimeError):
+    pass
+
+
+def load(path: str = LIBRARY):
+    """Load libfirepro.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FireProError(
+            f"{path} is missing: build the HIP library first (python -c "
+            "'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    sigs = {
+        "fp_last_error": (C.c_char_p, []),
+        "fp_version": (C.c_char_p, []),
+        "fp_sim_create": (C.c_int, [P(Grid), P(C.c_void_p)]),
+        "fp_sim_step": (C.c_int, [C.c_void_p, P(Frame), C.c_double]),
+        "fp_sim_reset": (C.c_int, [C.c_void_p]),
+        "fp_sim_destroy": (None, [C.c_void_p]),
+        "fp_sim_set_cutoff": (C.c_int, [C.c_void_p, C.c_double]),
+        "fp_sim_set_budget": (C.c_int, [C.c_void_p, C.c_double]),
+        "fp_sim_set_ground": (C.c_int, [C.c_void_p, C.c_int32]),
+        "fp_sim_stats": (C.c_int, [C.c_void_p, P(Stats)]),
+        "fp_sim_read_bricks": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_int32)]),
+        "fp_sim_read_tables": (C.c_int, [C.c_void_p, P(Tables)]),
+        "fp_sim_read_state": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_float),
+                                        P(C.c_float), P(C.c_float), C.c_int32]),
+        "fp_sim_write_state": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float)]),
+        "fp_sim_stream": (C.c_void_p, [C.c_void_p]),
+        "fp_sim_profile": (C.c_int, [C.c_void_p, C.c_int32]),
+        "fp_sim_profile_read": (C.c_int, [C.c_void_p, P(C.c_char_p), P(C.c_double),
+                                          P(C.c_int32)]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int, lib=None):
+    if status != FP_OK:
+        lib = lib or _lib
+        msg = lib.fp_last_error().decode() if lib is not None else ""
+        if status == FP_EINVAL:
+            raise ValueError(msg)
+        raise FireProError(f"firepro status {status}: {msg}")

@@ -1,0 +1,179 @@
+"""FluidSimulation: the engine-layer object backed by the HIP library.
+
+Mirrors the members of src/engine/FluidSimulation.ts that src/library/FireSimulation.ts
+calls (create, step, reset, dispose, cutoff/voxelBudget/ground, stats getters, readBricks),
+plus the parity surface (integer tables and cell state of the active bricks).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+
+
+class FluidSimulation:
+    """One sparse-brick fire/smoke simulation resident on the GPU."""
+
+    def __init__(self, voxel_size: float, velocity_divisor: int = 1, brick_size: int = 16,
+                 smoke_divisor: int = 1, scalar_maccormack: bool = True):
+        self._lib = abi.load()
+        grid = abi.Grid(voxel_size, velocity_divisor, brick_size, smoke_divisor,
+                        int(bool(scalar_maccormack)))
+        handle = C.c_void_p()
+        abi.check(self._lib.fp_sim_create(C.byref(grid), C.byref(handle)), self._lib)
+        self._handle = handle
+        self.voxel_size = voxel_size
+        self.velocity_divisor = velocity_divisor
+        self.brick_size = brick_size
+        self.brick_cells = brick_size ** 3
+        self.velocity_cells = (brick_size // velocity_divisor) ** 3
+
+    # FluidSimulation.create (FluidSimulation.ts:419-475) is the constructor above.
+    @classmethod
+    def create(cls, grid: dict) -> "FluidSimulation":
+        return cls(grid["voxelSize"], grid.get("velocityDivisor", 1), grid.get("brickSize", 16),
+                   grid.get("smokeDivisor", 1), grid.get("scalarMacCormack", True))
+
+    def _h(self):
+        if self._handle is None:
+            raise abi.FireProError("This FluidSimulation has been disposed.")
+        return self._handle
+
+    def step(self, frame: "abi.FrameData | abi.PackedFrame", dt: float = 1 / 60) -> None:
+        packed = frame if isinstance(frame, abi.PackedFrame) else abi.PackedFrame(frame)
+        abi.check(self._lib.fp_sim_step(self._h(), packed.pointer(), dt), self._lib)
+
+    def reset(self) -> None:
+        abi.check(self._lib.fp_sim_reset(self._h()), self._lib)
+
+    def dispose(self) -> None:
+        if self._handle is not None:
+            self._lib.fp_sim_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.dispose()
+        except Exception:
+            pass
+
+    # Setters FireSimulation.step writes before each step (FireSimulation.ts:457-459).
+    @property
+    def cutoff(self):
+        return self._cutoff
+
+    @cutoff.setter
+    def cutoff(self, value: float):
+        abi.check(self._lib.fp_sim_set_cutoff(self._h(), value), self._lib)
+        self._cutoff = value
+
+    @property
+    def voxel_budget(self):
+        return self._budget
+
+    @voxel_budget.setter
+    def voxel_budget(self, value: float):
+        abi.check(self._lib.fp_sim_set_budget(self._h(), value), self._lib)
+        self._budget = value
+
+    @property
+    def ground(self):
+        return self._ground
+
+    @ground.setter
+    def ground(self, value: bool):
+        abi.check(self._lib.fp_sim_set_ground(self._h(), int(bool(value))), self._lib)
+        self._ground = bool(value)
+
+    def stats(self) -> abi.Stats:
+        s = abi.Stats()
+        abi.check(self._lib.fp_sim_stats(self._h(), C.byref(s)), self._lib)
+        return s
+
+    @property
+    def active_brick_count(self) -> int:
+        return self.stats().active_bricks
+
+    @property
+    def pool_slots(self) -> int:
+        return self.stats().pool_slots
+
+    @property
+    def pool_limited(self) -> bool:
+        return bool(self.stats().pool_limited)
+
+    @property
+    def time(self) -> float:
+        return self.stats().time
+
+    def read_bricks(self):
+        """readBricks (FluidSimulation.ts:1749-1779): each slotted brick's (min, max) corners."""
+        s = self.stats()
+        cap = C.c_int32(s.tile_capacity * 64)
+        mins = np.zeros((cap.value, 3), dtype=np.float64)
+        abi.check(self._lib.fp_sim_read_bricks(
+            self._h(), mins.ctypes.data_as(C.POINTER(C.c_double)), C.byref(cap)), self._lib)
+        size = self.brick_size * self.voxel_size
+        return [(tuple(m), tuple(m + size)) for m in mins[:cap.value]]
+
+    def read_tables(self) -> dict:
+        s = self.stats()
+        n = s.tile_capacity * 64
+        out = {
+            "pages": np.zeros(n, np.int32), "entries": np.zeros(n, np.int32),
+            "records": np.zeros(n * 28, np.int32),
+            "free_slots": np.zeros(s.pool_slots, np.uint32),
+            "active": np.zeros(n, np.int32), "zeroed": np.zeros(4 * n, np.int32),
+        }
+        t = abi.Tables()
+        for k, v in out.items():
+            setattr(t, k, v.ctypes.data_as(C.POINTER(C.c_uint32 if k == "free_slots" else C.c_int32)))
+        abi.check(self._lib.fp_sim_read_tables(self._h(), C.byref(t)), self._lib)
+        out["active"] = out["active"][:t.active_count]
+        out["zeroed"] = out["zeroed"][:2 * t.zeroed_count].reshape(-1, 2)
+        out["records"] = out["records"].reshape(-1, 28)
+        out.update(free_count=t.free_count, failures=t.failures, pool_slots=t.pool_slots,
+                   brick_count=t.brick_count)
+        return out
+
+    def read_state(self, fields=True, velocity=True, pressure=False, expansion=False) -> dict:
+        s = self.stats()
+        n = max(1, s.active_bricks)
+        ids = np.zeros(n, np.int32)
+        f = np.zeros((n, self.brick_cells, 4), np.float32) if fields else None
+        v = np.zeros((n, 3, self.velocity_cells), np.float32) if velocity else None
+        p = np.zeros((n, self.velocity_cells), np.float32) if pressure else None
+        e = np.zeros((n, self.brick_cells), np.float32) if expansion else None
+        ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+        count = self._lib.fp_sim_read_state(self._h(), ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            ptr(f), ptr(v), ptr(p), ptr(e), n)
+        if count < 0:
+            abi.check(count, self._lib)
+        out = {"ids": ids[:count]}
+        for k, a in (("fields", f), ("velocity", v), ("pressure", p), ("expansion", e)):
+            if a is not None:
+                out[k] = a[:count]
+        return out
+
+    def write_state(self, fields: Optional[np.ndarray], velocity: Optional[np.ndarray]) -> None:
+        ptr = lambda a: (np.ascontiguousarray(a, np.float32).ctypes.data_as(C.POINTER(C.c_float))
+                         if a is not None else None)
+        fa = np.ascontiguousarray(fields, np.float32) if fields is not None else None
+        va = np.ascontiguousarray(velocity, np.float32) if velocity is not None else None
+        abi.check(self._lib.fp_sim_write_state(self._h(), ptr(fa), ptr(va)), self._lib)
+
+    def stream(self) -> int:
+        return self._lib.fp_sim_stream(self._h())
+
+    def profile(self, enable: bool) -> None:
+        abi.check(self._lib.fp_sim_profile(self._h(), int(enable)), self._lib)
+
+    def profile_read(self) -> dict:
+        names = (C.c_char_p * 64)()
+        ms = (C.c_double * 64)()
+        count = C.c_int32(64)
+        abi.check(self._lib.fp_sim_profile_read(self._h(), names, ms, C.byref(count)), self._lib)
+        return {names[i].decode(): ms[i] for i in range(count.value)}
