@@ -23,6 +23,9 @@ struct Base {
   virtual void contentTiles(int32_t *out, int *count) = 0;
   virtual void writeActivity(const uint32_t *brickFlags, int n, const int32_t *tiles, int tileCount) = 0;
   virtual void readActivity(uint32_t *brickFlags, int n) = 0;
+  virtual int tileTables(int32_t *records, int32_t *hash, int cap, int hashCap) = 0;
+  virtual int brickCoords(const int32_t *ids, int n, int32_t *out) = 0;
+  virtual int rhs(double *out, int cap) = 0;
 };
 
 template <class R> struct Impl : Base {
@@ -127,6 +130,27 @@ template <class R> struct Impl : Base {
     for (int i = 0; i < tileCount; i++)
       sim.contentTiles.push_back({{tiles[4 * i], tiles[4 * i + 1], tiles[4 * i + 2]}, (uint32_t)tiles[4 * i + 3]});
   }
+  int tileTables(int32_t *records, int32_t *hash, int cap, int hashCap) override {
+    const auto &t = sim.tables;
+    if ((int)t.records.size() > cap * 32 || (int)t.hash.size() > hashCap * 4) return -1;
+    std::copy(t.records.begin(), t.records.end(), records);
+    std::copy(t.hash.begin(), t.hash.end(), hash);
+    return (int)(t.hash.size() / 4);
+  }
+  int brickCoords(const int32_t *ids, int n, int32_t *out) override {
+    for (int i = 0; i < n; i++) sim.brickOf(ids[i], out + 3 * i);
+    return n;
+  }
+  // Level 0's right-hand side h^2 (div u - expansion) of the last step, per active brick.
+  int rhs(double *out, int cap) override {
+    int n = (int)sim.activeList.size();
+    if (n > cap) return -1;
+    for (int i = 0; i < n; i++) {
+      int s = sim.pageSlot(sim.entries[sim.activeList[i]] & 0xffffff);
+      for (int c = 0; c < sim.nV; c++) out[(size_t)i * sim.nV + c] = (double)sim.D[(size_t)s * sim.nV + c];
+    }
+    return n;
+  }
   void readActivity(uint32_t *flags, int n) override {
     for (int i = 0; i < n && i < (int)sim.activity.size(); i++) flags[i] = sim.activity[i];
   }
@@ -187,6 +211,56 @@ int orc_read_activity(void *h, uint32_t *flags, int n) {
   return FP_OK;
 }
 float orc_fexp(float x) { return oracle::fexp(x); }
+int orc_tile_tables(void *h, int32_t *records, int32_t *hash, int cap, int hashCap) {
+  return ((Base *)h)->tileTables(records, hash, cap, hashCap);
+}
+int orc_brick_coords(void *h, const int32_t *ids, int n, int32_t *out) { return ((Base *)h)->brickCoords(ids, n, out); }
+int orc_rhs(void *h, double *out, int cap) { return ((Base *)h)->rhs(out, cap); }
+
+// Closed-form hooks: single functions of the restatement, evaluated in double.
+static oracle::Params<double> hookParams(const double *knots, const int *counts, double dt, double lifespan,
+                                         double smoke, double heat, double expansion) {
+  oracle::Params<double> p{};
+  for (int ch = 0; ch < 3; ch++) {
+    p.knotCount[ch] = counts[ch];
+    for (int i = 0; i < counts[ch]; i++) {
+      p.knots[ch][i][0] = knots[(ch * 8 + i) * 2];
+      p.knots[ch][i][1] = knots[(ch * 8 + i) * 2 + 1];
+    }
+  }
+  p.fdt = dt;
+  p.lifespan = lifespan;
+  p.fsmoke = smoke;
+  p.fheat = heat;
+  p.fexpansion = expansion;
+  return p;
+}
+double orc_ramp_integral(const double *knots, const int *counts, int ch, double lo, double hi) {
+  auto p = hookParams(knots, counts, 1, 1, 0, 0, 0);
+  return oracle::rampIntegral(p, ch, lo, hi);
+}
+void orc_evaluate_flame(const double *knots, const int *counts, double dt, double lifespan, double smoke, double heat,
+                        double expansion, double before, double *out4) {
+  auto p = hookParams(knots, counts, dt, lifespan, smoke, heat, expansion);
+  double after, prod[3];
+  oracle::evaluateFlame(p, before, after, prod);
+  out4[0] = after;
+  out4[1] = prod[0];
+  out4[2] = prod[1];
+  out4[3] = prod[2];
+}
+double orc_emitter_weight(const double *pos, const double *size, int box, const double *w) {
+  oracle::Emitter<double> e{};
+  for (int a = 0; a < 3; a++) {
+    e.pos[a] = pos[a];
+    e.size[a] = size[a];
+  }
+  e.box = box != 0;
+  return oracle::emitterWeight(e, w);
+}
+double orc_burst_perlin(uint32_t seedBits, double x, double y, double z) {
+  return oracle::burstPerlin<double>(seedBits, x, y, z);
+}
 // Time `steps` of an oracle run, in seconds, for the CPU baseline.
 double orc_time_steps(void *h, const fp_frame *f, double dt, int steps) {
   auto t0 = std::chrono::steady_clock::now();

@@ -55,6 +55,13 @@ def load():
         "orc_write_activity": (C.c_int, [v, P(C.c_uint32), C.c_int, P(C.c_int32), C.c_int]),
         "orc_read_activity": (C.c_int, [v, P(C.c_uint32), C.c_int]),
         "orc_fexp": (C.c_float, [C.c_float]),
+        "orc_tile_tables": (C.c_int, [v, P(C.c_int32), P(C.c_int32), C.c_int, C.c_int]),
+        "orc_brick_coords": (C.c_int, [v, P(C.c_int32), C.c_int, P(C.c_int32)]),
+        "orc_rhs": (C.c_int, [v, P(C.c_double), C.c_int]),
+        "orc_ramp_integral": (C.c_double, [P(C.c_double), P(C.c_int), C.c_int, C.c_double, C.c_double]),
+        "orc_evaluate_flame": (None, [P(C.c_double), P(C.c_int)] + [C.c_double] * 6 + [P(C.c_double)]),
+        "orc_emitter_weight": (C.c_double, [P(C.c_double), P(C.c_double), C.c_int, P(C.c_double)]),
+        "orc_burst_perlin": (C.c_double, [C.c_uint32, C.c_double, C.c_double, C.c_double]),
         "orc_time_steps": (C.c_double, [v, P(abi.Frame), C.c_double, C.c_int]),
     }
     for name, (res, args) in sigs.items():
@@ -176,6 +183,63 @@ class OracleSimulation:
         m = C.c_int(n.value)
         self._lib.orc_content_tiles(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(m))
         return out[:n.value]
+
+
+    def tile_tables(self):
+        s = self.stats()
+        cap = s.tile_capacity
+        rec = np.zeros(cap * 32, np.int32)
+        hsh = np.zeros(4 * 65536, np.int32)
+        n = self._lib.orc_tile_tables(self._h, rec.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      hsh.ctypes.data_as(C.POINTER(C.c_int32)), cap, 65536)
+        assert n > 0
+        return rec.reshape(cap, 32), hsh[:4 * n].reshape(n, 4)
+
+    def rhs(self) -> np.ndarray:
+        n = max(1, self.stats().active_bricks)
+        out = np.zeros((n, self.velocity_cells))
+        c = self._lib.orc_rhs(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), n)
+        return out[:c]
+
+    def brick_coords(self, ids) -> np.ndarray:
+        ids = np.ascontiguousarray(ids, np.int32)
+        out = np.zeros((len(ids), 3), np.int32)
+        self._lib.orc_brick_coords(self._h, ids.ctypes.data_as(C.POINTER(C.c_int32)), len(ids),
+                                   out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+
+def _ramps(ramps):
+    knots = np.zeros(3 * 8 * 2)
+    counts = (C.c_int * 3)()
+    for ch, name in enumerate(("smoke", "heat", "expansion")):
+        counts[ch] = len(ramps[name])
+        for i, (x, y) in enumerate(ramps[name]):
+            knots[(ch * 8 + i) * 2] = x
+            knots[(ch * 8 + i) * 2 + 1] = y
+    return knots, counts
+
+
+def ramp_integral(ramps, ch, lo, hi):
+    knots, counts = _ramps(ramps)
+    return load().orc_ramp_integral(knots.ctypes.data_as(C.POINTER(C.c_double)), counts, ch, lo, hi)
+
+
+def evaluate_flame(ramps, dt, lifespan, smoke, heat, expansion, before):
+    knots, counts = _ramps(ramps)
+    out = np.zeros(4)
+    load().orc_evaluate_flame(knots.ctypes.data_as(C.POINTER(C.c_double)), counts, dt, lifespan, smoke,
+                              heat, expansion, before, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def emitter_weight(pos, size, box, w):
+    arr = lambda v: (C.c_double * 3)(*v)
+    return load().orc_emitter_weight(arr(pos), arr(size), int(box), arr(w))
+
+
+def burst_perlin(seed_bits, x, y, z):
+    return load().orc_burst_perlin(seed_bits, x, y, z)
 
 
 def fexp(x: float) -> float:

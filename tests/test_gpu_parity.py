@@ -1,0 +1,112 @@
+"""HIP path against the FP32 restatement: integer tables and every field bit-exact.
+
+The HIP kernels and the restatement evaluate the same float expression trees with IEEE
+basic operations (no contraction, correctly rounded division and square root, exp from a
+shared published method), so the bar is equality, not a tolerance.
+"""
+import numpy as np
+import pytest
+
+from lockstep import apply, make_engines, run_lockstep
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes():
+    from firepro import scenes
+    return scenes
+
+
+@pytest.mark.parametrize("brick,divisor", [(8, 2), (16, 2), (8, 1), (16, 4)])
+def test_small_scene_bit_exact(brick, divisor):
+    sc = _scenes().small(steps=6, brick=brick, divisor=divisor)
+    print(f"case small brick={brick} divisor={divisor}", flush=True)
+    assert run_lockstep(sc) == []
+
+
+def test_first_order_scalars_bit_exact():
+    sc = _scenes().small(steps=4)
+    sc.sim.options["scalarMacCormack"] = False
+    assert run_lockstep(sc) == []
+
+
+def test_campfire_bit_exact():
+    sc = _scenes().campfire(steps=40)
+    assert run_lockstep(sc, check_every=5) == []
+
+
+def test_explosions_churn_tables():
+    """Brick churn: bursts allocate and free thousands of bricks; tables stay bit-exact."""
+    sc = _scenes().explosions(steps=40)
+    assert run_lockstep(sc, check_every=4) == []
+
+
+def test_four_fires_collider_turbulence():
+    sc = _scenes().four_fires(steps=4)
+    assert run_lockstep(sc, check_every=2) == []
+
+
+def test_pool_growth_and_budget():
+    """A tiny voxel budget forces failed requests, growth to the limit and poolLimited."""
+    from firepro import scenes
+    sc = scenes.small(steps=8)
+    sc.sim.options["grid"]["maxVoxels"] = 4096 * 3
+    gpu, cpu = make_engines(sc)
+    try:
+        for frame, g in sc.frames():
+            apply(gpu, cpu, g)
+            gpu.step(frame)
+            cpu.step(frame)
+            ta, tb = gpu.read_tables(), cpu.read_tables()
+            for k in ("pages", "active", "free_slots"):
+                assert np.array_equal(ta[k], tb[k]), k
+            assert ta["failures"] == tb["failures"]
+        sa, sb = gpu.stats(), cpu.stats()
+        assert sa.pool_limited == sb.pool_limited == 1
+        assert sa.pool_slots == sb.pool_slots
+    finally:
+        gpu.dispose()
+
+
+def test_reset_restarts_identically():
+    from firepro import scenes
+    sc = scenes.small(steps=3)
+    frames = list(sc.frames())
+    gpu, _ = make_engines(sc)
+    try:
+        def run():
+            for frame, g in frames:
+                gpu.cutoff, gpu.voxel_budget, gpu.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+                gpu.step(frame)
+            return gpu.read_state(pressure=True)
+        a = run()
+        gpu.reset()
+        assert gpu.stats().steps == 0
+        b = run()
+        for k in ("ids", "fields", "velocity", "pressure"):
+            assert np.array_equal(a[k], b[k]), k
+    finally:
+        gpu.dispose()
+
+
+def test_invalid_inputs_raise():
+    from firepro import abi, scenes
+    from firepro.engine import FluidSimulation
+    with pytest.raises(ValueError, match="voxelSize"):
+        FluidSimulation(0.0, 2, 16)
+    with pytest.raises(ValueError, match="brickSize"):
+        FluidSimulation(0.05, 2, 12)
+    sc = scenes.small(steps=1)
+    gpu = FluidSimulation.create(sc.grid())
+    try:
+        frame, g = next(sc.frames())
+        with pytest.raises(ValueError, match="Simulation step"):
+            gpu.step(frame, 0.1)
+        frame.ramps = dict(frame.ramps)
+        frame.ramps["heat"] = [(0, 0), (0.5, 1)]
+        with pytest.raises(ValueError, match="span lifetime"):
+            gpu.step(frame)
+    finally:
+        gpu.dispose()
+    with pytest.raises(abi.FireProError):
+        gpu.stats()

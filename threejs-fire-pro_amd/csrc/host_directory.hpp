@@ -1,0 +1,282 @@
+// Host side of the sparse grid: the tile directory (src/engine/TileDirectory.ts:61-246) and
+// the pool-layout arithmetic (src/engine/types.ts:65-129). The directory keeps its tiles
+// in creation order, because that order decides where tiles land in the open-addressed
+// hash and how clusters are numbered, both of which the GPU tables reproduce exactly.
+#pragma once
+#include <stdint.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <stdexcept>
+#include <unordered_map>
+#include <vector>
+
+namespace fp {
+namespace host {
+
+constexpr double MAX_TEXTURE_3D = 2048, MAX_BINDING = 134217728, MAX_BUFFER = 268435456;  // types.ts:56-60
+
+inline int poolSlotsPerAxis(int brick) { return (int)std::floor(MAX_TEXTURE_3D / (brick + 1)); }
+inline double maxPoolVoxels(int divisor, int brick) {
+  double cells = (double)brick * brick * brick;
+  double vb = (double)brick / divisor;
+  double bytes = std::max(vb * vb * vb * 2, cells / 8);
+  double perAxis = poolSlotsPerAxis(brick);
+  double side = divisor == 1 ? std::min(perAxis, std::floor(MAX_TEXTURE_3D / (brick * 2))) : perAxis;
+  return std::min(std::floor(std::min(MAX_BINDING, MAX_BUFFER) / bytes), side * side * perAxis) * cells;
+}
+inline int poolSide(int divisor, int brick) {
+  int perAxis = poolSlotsPerAxis(brick);
+  double slots = std::floor(maxPoolVoxels(divisor, brick) / ((double)brick * brick * brick));
+  int donorSide = divisor == 1 ? (int)std::floor(MAX_TEXTURE_3D / (brick * 2)) : perAxis;
+  return std::min(std::min(perAxis, donorSide), std::max(4, (int)std::ceil(std::sqrt(slots / perAxis))));
+}
+inline int poolRows(int slots, int side) {
+  int rows = std::max(1, (int)std::ceil((double)slots / side));
+  return rows <= side ? rows : (int)std::ceil((double)rows / side) * side;
+}
+inline int initialPoolSlots(int side, int limit) { return std::min(limit, side * (int)std::ceil(128.0 / side)); }
+
+struct TileBox {
+  int lo[3], hi[3];
+};
+
+class TileDirectory {
+ public:
+  struct Tile {
+    int c[3];
+    int slot;
+    int64_t needed;
+    bool closing;
+  };
+  static constexpr int LINGER_STEPS = 60, RECORD_WORDS = 32, HASH_ROW = 256;
+
+  bool ground = true;
+  bool dirty = true;
+  int version = 0;
+  int capacity = 0;
+
+  ~TileDirectory() { clear(); }
+
+  int count() const { return (int)index_.size(); }
+  int slotsUsed() const { return (int)slots_.size(); }
+
+  static int64_t key(int x, int y, int z) {
+    return ((int64_t)(x & 0x1fffff) << 42) | ((int64_t)(y & 0x1fffff) << 21) | (int64_t)(z & 0x1fffff);
+  }
+  const Tile *find(int x, int y, int z) const {
+    auto it = index_.find(key(x, y, z));
+    return it == index_.end() ? nullptr : it->second;
+  }
+  int slotAt(int x, int y, int z) const {
+    const Tile *t = find(x, y, z);
+    return t ? t->slot : -1;
+  }
+
+  // need (TileDirectory.ts:86-108): refresh or create every tile of the box.
+  void need(const int lo[3], const int hi[3], int64_t step) {
+    for (int z = lo[2]; z <= hi[2]; z++)
+      for (int y = ground ? std::max(0, lo[1]) : lo[1]; y <= hi[1]; y++)
+        for (int x = lo[0]; x <= hi[0]; x++) {
+          auto it = index_.find(key(x, y, z));
+          if (it != index_.end()) {
+            it->second->needed = step;
+            if (it->second->closing) {
+              it->second->closing = false;
+              dirty = true;
+            }
+            continue;
+          }
+          int slot = (int)slots_.size();
+          for (size_t i = 0; i < slots_.size(); i++)
+            if (!slots_[i]) {
+              slot = (int)i;
+              break;
+            }
+          Tile *t = new Tile{{x, y, z}, slot, step, false};
+          if (slot == (int)slots_.size()) slots_.push_back(t);
+          else slots_[slot] = t;
+          index_[key(x, y, z)] = t;
+          order_.push_back(t);
+          created_.push_back(slot);
+          dirty = true;
+        }
+  }
+
+  // retire (TileDirectory.ts:111-123).
+  void retire(int64_t step) {
+    size_t w = 0;
+    for (size_t r = 0; r < order_.size(); r++) {
+      Tile *t = order_[r];
+      if (t->closing) {
+        index_.erase(key(t->c[0], t->c[1], t->c[2]));
+        slots_[t->slot] = nullptr;
+        dirty = true;
+        delete t;
+        continue;
+      }
+      if (step - t->needed > LINGER_STEPS) {
+        t->closing = true;
+        dirty = true;
+      }
+      order_[w++] = t;
+    }
+    order_.resize(w);
+    while (!slots_.empty() && !slots_.back()) slots_.pop_back();
+  }
+
+  void clear() {
+    for (Tile *t : order_) delete t;
+    order_.clear();
+    index_.clear();
+    slots_.clear();
+    created_.clear();
+    dirty = true;
+  }
+
+  std::vector<int> takeCreated() {
+    std::vector<int> out;
+    for (int s : created_)
+      if (s < (int)slots_.size() && slots_[s]) out.push_back(s);
+    created_.clear();
+    return out;
+  }
+
+  // tables (TileDirectory.ts:151-186): tile records with 27 neighbor slots, the hash and
+  // the per-slot (coordinates, cluster) words; `coords` snapshots the slots' tiles.
+  void tables(int cap, std::vector<int32_t> &records, std::vector<int32_t> &hash, std::vector<TileBox> &clusters,
+              std::vector<int32_t> &tiles) {
+    if (cap < (int)slots_.size()) throw std::runtime_error("Tile capacity is below the tiles in use.");
+    capacity = cap;
+    records.assign((size_t)cap * RECORD_WORDS, -1);
+    int size = HASH_ROW;
+    while (size < (int)index_.size() * 2) size *= 2;
+    hash.assign((size_t)size * 4, -1);
+    for (int s = 0; s < cap; s++) {
+      int32_t *r = &records[(size_t)s * RECORD_WORDS];
+      r[0] = r[1] = r[2] = 0;
+      r[3] = 0;
+    }
+    const uint32_t mask = (uint32_t)size - 1;
+    for (const Tile *t : order_) {
+      int32_t *r = &records[(size_t)t->slot * RECORD_WORDS];
+      r[0] = t->c[0];
+      r[1] = t->c[1];
+      r[2] = t->c[2];
+      r[3] = t->closing ? 2 : 1;
+      uint32_t h = (((uint32_t)t->c[0] * 73856093u) ^ ((uint32_t)t->c[1] * 19349663u) ^
+                    ((uint32_t)t->c[2] * 83492791u)) & mask;
+      while (hash[(size_t)h * 4 + 3] >= 0) h = (h + 1) & mask;
+      for (int k = 0; k < 3; k++) hash[(size_t)h * 4 + k] = t->c[k];
+      hash[(size_t)h * 4 + 3] = t->slot;
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            const Tile *n = find(t->c[0] + dx, t->c[1] + dy, t->c[2] + dz);
+            if (n) r[4 + (dz + 1) * 9 + (dy + 1) * 3 + dx + 1] = n->slot;
+          }
+    }
+    std::unordered_map<const Tile *, int> clusterOf;
+    buildClusters(clusters, clusterOf);
+    tiles.assign((size_t)cap * 4, 0);
+    coords.assign(slots_.size(), {0, 0, 0, 0});
+    for (int s = 0; s < cap; s++) {
+      const Tile *t = s < (int)slots_.size() ? slots_[s] : nullptr;
+      int32_t *w = &tiles[(size_t)s * 4];
+      if (t) {
+        w[0] = t->c[0];
+        w[1] = t->c[1];
+        w[2] = t->c[2];
+        w[3] = clusterOf[t];
+        coords[s] = {t->c[0], t->c[1], t->c[2], 1};
+      } else {
+        w[3] = -1;
+      }
+    }
+    dirty = false;
+    version++;
+  }
+
+  std::vector<std::array<int, 4>> coords;
+  const std::vector<Tile *> &order() const { return order_; }
+
+ private:
+  // clusters (TileDirectory.ts:189-245): 26-connected groups, boxed, merged until disjoint.
+  void buildClusters(std::vector<TileBox> &out, std::unordered_map<const Tile *, int> &clusterOf) {
+    std::unordered_map<const Tile *, int> group;
+    std::vector<TileBox> boxes;
+    for (const Tile *first : order_) {
+      if (group.count(first)) continue;
+      int idx = (int)boxes.size();
+      TileBox box{{first->c[0], first->c[1], first->c[2]}, {first->c[0], first->c[1], first->c[2]}};
+      group[first] = idx;
+      std::vector<const Tile *> stack{first};
+      while (!stack.empty()) {
+        const Tile *t = stack.back();
+        stack.pop_back();
+        for (int a = 0; a < 3; a++) {
+          box.lo[a] = std::min(box.lo[a], t->c[a]);
+          box.hi[a] = std::max(box.hi[a], t->c[a]);
+        }
+        for (int dz = -1; dz <= 1; dz++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+              const Tile *n = find(t->c[0] + dx, t->c[1] + dy, t->c[2] + dz);
+              if (n && !group.count(n)) {
+                group[n] = idx;
+                stack.push_back(n);
+              }
+            }
+      }
+      boxes.push_back(box);
+    }
+    std::vector<int> into(boxes.size());
+    for (size_t i = 0; i < into.size(); i++) into[i] = (int)i;
+    auto root = [&](int i) {
+      int r = i;
+      while (into[r] != r) r = into[r];
+      while (into[i] != r) {
+        int n = into[i];
+        into[i] = r;
+        i = n;
+      }
+      return r;
+    };
+    auto overlaps = [](const TileBox &a, const TileBox &b) {
+      for (int k = 0; k < 3; k++)
+        if (!(a.lo[k] <= b.hi[k] && b.lo[k] <= a.hi[k])) return false;
+      return true;
+    };
+    for (bool merged = true; merged;) {
+      merged = false;
+      for (size_t i = 0; i < boxes.size(); i++)
+        for (size_t j = i + 1; j < boxes.size(); j++) {
+          int a = root((int)i), b = root((int)j);
+          if (a == b || !overlaps(boxes[a], boxes[b])) continue;
+          for (int k = 0; k < 3; k++) {
+            boxes[a].lo[k] = std::min(boxes[a].lo[k], boxes[b].lo[k]);
+            boxes[a].hi[k] = std::max(boxes[a].hi[k], boxes[b].hi[k]);
+          }
+          into[b] = a;
+          merged = true;
+        }
+    }
+    out.clear();
+    std::unordered_map<int, int> renumbered;
+    for (size_t i = 0; i < boxes.size(); i++)
+      if (root((int)i) == (int)i) {
+        renumbered[(int)i] = (int)out.size();
+        out.push_back(boxes[i]);
+      }
+    for (auto &kv : group) clusterOf[kv.first] = renumbered[root(kv.second)];
+  }
+
+  std::vector<Tile *> order_;
+  std::unordered_map<int64_t, Tile *> index_;
+  std::vector<Tile *> slots_;
+  std::vector<int> created_;
+};
+
+}  // namespace host
+}  // namespace fp

@@ -13,7 +13,8 @@ struct AllocArgs {
   uint32_t *brickState, *activity;
   uint32_t *stateTmp, *rankTmp, *blockSums, *blockPrefix;
   uint32_t *freeSlots;
-  int *poolState;  // free, failures, held
+  int *poolState;  // free, failures
+  int held;        // free slots the voxel budget holds back
   int *counts;     // active, fresh, free before, freed, requests, freed-zeroed
   int *active, *zeroList;
   const Emitter *emitters;

@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(AllocArgs a) {
   }
   if (threadIdx.x == 0) {
     int freed = (int)carry[0], requests = (int)carry[1], keeps = (int)carry[2];
-    int free0 = a.poolState[0], held = a.poolState[2];
+    int free0 = a.poolState[0], held = a.held;
     int freeAfter = free0 + freed;
     int room = freeAfter - held > 0 ? freeAfter - held : 0;
     int granted = requests < room ? requests : room;

@@ -1,0 +1,1280 @@
+// The solver runtime and the C ABI (include/firepro.h): host orchestration of one step
+// (src/engine/FluidSimulation.ts substep, :1374-1531) over device-resident pools.
+//
+// One HIP stream per simulation. The step enqueues allocation, transport, forces, the
+// multigrid V-cycle and the projection, then copies the activity readback (active count,
+// free slots, failures, per-tile content) into pinned memory and waits for it: the next
+// step's tile directory sees this step's content, the earliest a WebGPU mapAsync can
+// deliver it (FluidSimulation.ts:1701-1746).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/firepro.h"
+#include "host_directory.hpp"
+#include "kernels.hpp"
+
+namespace fp {
+
+thread_local std::string g_error;
+
+struct InvalidArgument : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct DeviceFailure : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(expr)                                                                              \
+  do {                                                                                              \
+    hipError_t err_ = (expr);                                                                       \
+    if (err_ != hipSuccess)                                                                         \
+      throw DeviceFailure(std::string(#expr) + ": " + hipGetErrorString(err_));                     \
+  } while (0)
+
+template <class T> struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+  }
+  // Reallocate to `count`, keeping the first min(n, count) elements.
+  void grow(size_t count, hipStream_t s) {
+    if (count <= n) return;
+    T *q = nullptr;
+    HIPCHECK(hipMalloc(&q, count * sizeof(T)));
+    if (p && n) HIPCHECK(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    release();
+    p = q;
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---- small housekeeping kernels ---------------------------------------------------------
+
+// growFreeList (fluid-pool-growth.wgsl:12-22): new slots go under the stack.
+__global__ void k_grow_free_list(uint32_t *freeSlots, int *poolState, int first, int added) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < added) freeSlots[i] = (uint32_t)(first + added - 1 - i);
+  if (i == 0) poolState[0] += added;
+}
+// A created tile's bricks start without slots (FluidSimulation.ts:728-734).
+__global__ void k_clear_tiles(int *pages, int *entries, uint32_t *state, uint32_t *activity, const int *slots,
+                              int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count * 64) return;
+  int id = slots[i >> 6] * 64 + (i & 63);
+  pages[id] = -1;
+  entries[id] = -1;
+  state[id] = 0u;
+  activity[id] = 0u;
+}
+__global__ void k_fill_int(int *p, int value, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+__global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+// Failed allocations count from one measurement to the next (FluidSimulation.ts:1525).
+__global__ void k_readback(const int *counts, int *poolState, const uint32_t *tileFlags, int cap, int *out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    out[0] = counts[0];
+    out[1] = poolState[0];
+    out[2] = poolState[1];
+    out[3] = counts[1] + counts[5];
+  }
+  if (i < cap) out[4 + i] = (int)tileFlags[i];
+}
+__global__ void k_reset_failures(int *poolState) { poolState[1] = 0; }
+
+// ---- the curl-noise table (curl-noise.wgsl:1-45), built once on the host -----------------
+
+static float noiseHash(float px, float py, float pz) {
+  uint32_t v[3] = {(uint32_t)(int)px & 15u, (uint32_t)(int)py & 15u, (uint32_t)(int)pz & 15u};
+  for (uint32_t &c : v) c = c * 1664525u + 1013904223u;
+  v[0] += v[1] * v[2];
+  v[1] += v[2] * v[0];
+  v[2] += v[0] * v[1];
+  for (uint32_t &c : v) c ^= c >> 16;
+  v[0] += v[1] * v[2];
+  v[1] += v[2] * v[0];
+  v[2] += v[0] * v[1];
+  return (float)(v[0] >> 8) / 16777216.0f;
+}
+static float hmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+static void noiseGradient(float px, float py, float pz, float out[3]) {
+  float ix = std::floor(px), iy = std::floor(py), iz = std::floor(pz);
+  float f[3] = {px - ix, py - iy, pz - iz};
+  float s[3], ds[3];
+  for (int a = 0; a < 3; a++) {
+    s[a] = f[a] * f[a] * f[a] * (f[a] * (f[a] * 6.0f - 15.0f) + 10.0f);
+    ds[a] = 30.0f * f[a] * f[a] * (f[a] - 1.0f) * (f[a] - 1.0f);
+  }
+  float a = noiseHash(ix, iy, iz), b = noiseHash(ix + 1, iy, iz), c = noiseHash(ix, iy + 1, iz);
+  float d = noiseHash(ix + 1, iy + 1, iz), e = noiseHash(ix, iy, iz + 1), f1 = noiseHash(ix + 1, iy, iz + 1);
+  float g = noiseHash(ix, iy + 1, iz + 1), h = noiseHash(ix + 1, iy + 1, iz + 1);
+  out[0] = hmix(hmix(b - a, d - c, s[1]), hmix(f1 - e, h - g, s[1]), s[2]) * ds[0];
+  out[1] = hmix(hmix(c - a, d - b, s[0]), hmix(g - e, h - f1, s[0]), s[2]) * ds[1];
+  out[2] = hmix(hmix(e - a, f1 - b, s[0]), hmix(g - c, h - d, s[0]), s[1]) * ds[2];
+}
+static std::vector<float4> curlNoiseTable() {
+  std::vector<float4> t(64 * 64 * 64);
+  for (int z = 0; z < 64; z++)
+    for (int y = 0; y < 64; y++)
+      for (int x = 0; x < 64; x++) {
+        float p[3] = {((float)x + 0.5f) / 64.0f * 16.0f, ((float)y + 0.5f) / 64.0f * 16.0f,
+                      ((float)z + 0.5f) / 64.0f * 16.0f};
+        float dx[3], dy[3], dz[3];
+        noiseGradient(p[0], p[1], p[2], dx);
+        noiseGradient(p[0] + 7.1f, p[1] + 3.7f, p[2] + 11.3f, dy);
+        noiseGradient(p[0] + 5.2f, p[1] + 7.9f, p[2] + 9.5f, dz);
+        t[((size_t)z * 64 + y) * 64 + x] = make_float4(dz[1] - dy[2], dx[2] - dz[0], dy[0] - dx[1], 0.0f);
+      }
+  return t;
+}
+
+constexpr int MIN_TILES = 16;
+constexpr double MAX_DT = 1.0 / 30.0;
+
+}  // namespace fp
+
+using namespace fp;
+
+struct fp_sim {
+  // Grid (construction-only) and derived layout.
+  double voxel = 0;
+  int d = 1, B = 16, b = 16, sF = 4, sV = 4, nF = 0, nV = 0, side = 0, maxSlots = 0, brickLevels = 0, bottom = 0;
+  bool maccormack = true;
+  // Settings written before each step.
+  double voxelBudget = INFINITY, cutoff = 0;
+  bool ground = true;
+  // Host state (FluidSimulation.ts:189-355).
+  int poolSlots = 0, usedSlots = 0, poolTarget = 0;
+  int64_t poolGrownAt = 0;
+  bool poolLimited = false;
+  int64_t activeBricks = 0;
+  double time = 0;
+  int64_t steps = 0;
+  bool mayHaveFlame = false, solidsStale = true, geometryDirty = true;
+  int colliderCount = 0;
+  std::vector<uint32_t> prevColliders;
+  std::vector<uint64_t> sourceIds;
+  std::vector<std::pair<std::array<int, 3>, uint32_t>> contentTiles;
+  int freeCount = 0, lastFailures = 0, lastZeroed = 0;
+  host::TileDirectory dir;
+  int capacity = 0;
+  std::vector<int32_t> tileRecHost, hashHost, tilesHost;
+  std::vector<host::TileBox> clusters;
+  uint32_t hashMask = 0;
+  hipStream_t stream = nullptr;
+  bool stepped = false;
+
+  // Per brick id.
+  DBuf<int> tileRec, hash, pages, entries, records, active, zeroList, poolState, counts, createdSlots, readback;
+  DBuf<uint32_t> brickState, activity, stateTmp, rankTmp, blockSums, blockPrefix, freeSlots;
+  // Pools, per slot.
+  DBuf<float4> F, Fp, Fn, Curl, noise;
+  DBuf<float> E, V, Vadv, Vcor, Vf, D, P0, P1;
+  DBuf<uint32_t> solids;
+  struct LevelBufs {
+    DBuf<float> p, q, rhs;
+    DBuf<uint8_t> topo, eflag;
+    int side = 0, tile = 0;
+  };
+  LevelBufs lv[MAX_LEVELS];
+  float *latestPressure = nullptr;
+  // Scene.
+  DBuf<Emitter> emitters;
+  DBuf<Collider> colliders;
+  DBuf<ForceRec> forces;
+  DBuf<Region> regions;
+  int emitterCount = 0, forceCount = 0;
+  Params P{};
+  // Readback.
+  int *pinned = nullptr;
+  size_t pinnedWords = 0;
+  // Profiling.
+  bool profiling = false;
+  struct Mark {
+    const char *name;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> eventPool;
+  std::map<std::string, std::pair<double, int>> profile;
+  std::vector<std::string> profileNames;
+
+  int brickCount() const { return capacity * 64; }
+  int slotLimit() const {
+    double voxels = std::min(voxelBudget, host::maxPoolVoxels(d, B));
+    return std::max(1, (int)std::min((double)maxSlots, std::floor(voxels / ((double)B * B * B))));
+  }
+  double tileSize() const { return 4.0 * B * voxel; }
+  double velocitySpacing() const { return voxel * d; }
+
+  Layout layout() const {
+    Layout L;
+    L.B = B;
+    L.b = b;
+    L.sF = sF;
+    L.sV = sV;
+    L.nF = nF;
+    L.nV = nV;
+    L.side = side;
+    L.brickLevels = brickLevels;
+    return L;
+  }
+  Dir dirArgs() const {
+    Dir r;
+    r.tileRec = tileRec.p;
+    r.hash = hash.p;
+    r.hashMask = hashMask;
+    r.entries = entries.p;
+    r.records = records.p;
+    r.capacity = capacity;
+    return r;
+  }
+
+  // ---- profiling marks ----
+  hipEvent_t event() {
+    if (!eventPool.empty()) {
+      hipEvent_t e = eventPool.back();
+      eventPool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  int begin() {
+    if (!profiling) return -1;
+    Mark m{nullptr, event(), event()};
+    HIPCHECK(hipEventRecord(m.a, stream));
+    marks.push_back(m);
+    return (int)marks.size() - 1;
+  }
+  void end(int i, const char *name) {
+    if (i < 0) return;
+    marks[i].name = name;
+    HIPCHECK(hipEventRecord(marks[i].b, stream));
+  }
+  void collect() {
+    for (Mark &m : marks) {
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, m.a, m.b));
+      auto &e = profile[m.name];
+      if (e.second == 0 && std::find(profileNames.begin(), profileNames.end(), m.name) == profileNames.end())
+        profileNames.push_back(m.name);
+      e.first += ms;
+      e.second += 1;
+      eventPool.push_back(m.a);
+      eventPool.push_back(m.b);
+    }
+    marks.clear();
+  }
+
+  // ---- construction and allocation (FluidSimulation.ts:359-532) ----
+  void create(const fp_grid &g) {
+    if (!std::isfinite(g.voxel_size) || g.voxel_size < 0.0001 || g.voxel_size > 16)
+      throw InvalidArgument("voxelSize must be from 0.0001 to 16 meters.");
+    voxel = g.voxel_size;
+    d = g.velocity_divisor;
+    if (d != 1 && d != 2 && d != 4) throw InvalidArgument("velocityDivisor must be 1, 2 or 4.");
+    B = g.brick_size;
+    if (B != 8 && B != 16 && B != 32) throw InvalidArgument("brickSize must be 8, 16 or 32 fine cells.");
+    if (g.smoke_divisor != 1) throw InvalidArgument("smokeDivisor must be 1: coarse smoke is not built.");
+    maccormack = g.scalar_maccormack != 0;
+    b = B / d;
+    sF = 0;
+    while ((1 << sF) < B) sF++;
+    sV = 0;
+    while ((1 << sV) < b) sV++;
+    nF = B * B * B;
+    nV = b * b * b;
+    if (b < 2) throw InvalidArgument("velocity bricks need at least 2 cells a side.");
+    side = host::poolSide(d, B);
+    maxSlots = side * side * host::poolSlotsPerAxis(B);
+    brickLevels = sV;
+    bottom = brickLevels + 2;
+    if (bottom + 1 > MAX_LEVELS) throw InvalidArgument("too many multigrid levels.");
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    poolState.alloc(4);
+    counts.alloc(8);
+    std::vector<float4> table = curlNoiseTable();
+    noise.alloc(table.size());
+    HIPCHECK(hipMemcpyAsync(noise.p, table.data(), table.size() * sizeof(float4), hipMemcpyHostToDevice, stream));
+    allocate();
+  }
+
+  void allocate() {
+    int slots = host::initialPoolSlots(side, slotLimit());
+    poolSlots = 0;
+    usedSlots = 0;
+    poolTarget = 0;
+    sourceIds.clear();
+    poolLimited = false;
+    allocatePools(slots, false);
+    std::vector<uint32_t> stack(slots);
+    for (int i = 0; i < slots; i++) stack[i] = (uint32_t)(slots - 1 - i);
+    freeSlots.alloc(slots);
+    HIPCHECK(hipMemcpyAsync(freeSlots.p, stack.data(), slots * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    int state[4] = {slots, 0, 0, 0};
+    HIPCHECK(hipMemcpyAsync(poolState.p, state, sizeof(state), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    poolSlots = slots;
+    freeCount = slots;
+    solidsStale = true;
+    geometryDirty = true;
+    dir.clear();
+    contentTiles.clear();
+    capacity = 0;
+    growTiles(MIN_TILES);
+    applyTiles();
+  }
+
+  // Pools for `slots` slots; with `keep`, live state (fields, expansion, velocity, solids)
+  // carries over and every slot keeps its place.
+  void allocatePools(int slots, bool keep) {
+    size_t s = (size_t)slots;
+    if (keep) {
+      F.grow(s * nF, stream);
+      E.grow(s * nF, stream);
+      V.grow(s * nV * 3, stream);
+      solids.grow(s * (nF / 32), stream);
+    } else {
+      F.alloc(s * nF);
+      E.alloc(s * nF);
+      V.alloc(s * nV * 3);
+      solids.alloc(s * (nF / 32));
+      HIPCHECK(hipMemsetAsync(solids.p, 0, solids.bytes(), stream));
+    }
+    Fp.alloc(s * nF);
+    Fn.alloc(s * nF);
+    Curl.alloc(s * nV);
+    Vadv.alloc(s * nV * 3);
+    Vcor.alloc(s * nV * 3);
+    Vf.alloc(s * nV * 3);
+    D.alloc(s * nV);
+    P0.alloc(s * nV);
+    P1.alloc(s * nV);
+    latestPressure = P0.p;
+    for (int L = 0; L < brickLevels; L++) {
+      int sd = b >> L;
+      size_t n = s * sd * sd * sd;
+      lv[L].side = sd;
+      lv[L].tile = 0;
+      if (L > 0) {
+        lv[L].p.alloc(n);
+        lv[L].q.alloc(n);
+        lv[L].rhs.alloc(n);
+      }
+      lv[L].topo.alloc(n);
+      lv[L].eflag.alloc(n);
+    }
+  }
+
+  // Tile resources for `cap` tile slots, keeping each brick's page, state, flags, entry and
+  // record (FluidSimulation.ts:663-705).
+  void growTiles(int cap) {
+    size_t bricks = (size_t)cap * 64;
+    size_t old = (size_t)capacity * 64;
+    auto growFill = [&](DBuf<int> &buf, size_t count, int fill) {
+      buf.grow(count, stream);
+      if (count > old) k_fill_int<<<(unsigned)((count - old + 255) / 256), 256, 0, stream>>>(buf.p + old, fill, count - old);
+    };
+    growFill(pages, bricks, -1);
+    growFill(entries, bricks, -1);
+    records.grow(bricks * RECORD_WORDS, stream);
+    brickState.grow(bricks, stream);
+    if (bricks > old) k_fill_u32<<<(unsigned)((bricks - old + 255) / 256), 256, 0, stream>>>(brickState.p + old, 0u, bricks - old);
+    // Brick flags are kept; the per-tile words after them are rewritten every step.
+    {
+      DBuf<uint32_t> next;
+      next.alloc(bricks + cap);
+      k_fill_u32<<<(unsigned)((bricks + cap + 255) / 256), 256, 0, stream>>>(next.p, 0u, bricks + cap);
+      if (activity.p && old)
+        HIPCHECK(hipMemcpyAsync(next.p, activity.p, old * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      activity.release();
+      activity = next;
+      next.p = nullptr;
+      next.n = 0;
+    }
+    stateTmp.alloc(bricks);
+    rankTmp.alloc(bricks);
+    size_t blocks = (bricks + 255) / 256;
+    blockSums.alloc(blocks * 3);
+    blockPrefix.alloc(blocks * 3);
+    active.alloc(bricks);
+    zeroList.alloc(bricks * 4);
+    tileRec.alloc((size_t)cap * 32);
+    capacity = cap;
+    for (int L = brickLevels; L <= bottom; L++) {
+      int sd = 4 >> (L - brickLevels);
+      size_t n = (size_t)cap * sd * sd * sd;
+      lv[L].side = sd;
+      lv[L].tile = 1;
+      lv[L].p.alloc(n);
+      lv[L].q.alloc(n);
+      lv[L].rhs.alloc(n);
+      lv[L].topo.alloc(n);
+      lv[L].eflag.alloc(n);
+    }
+    size_t words = 4 + (size_t)cap;
+    readback.alloc(words);
+    if (words > pinnedWords) {
+      if (pinned) (void)hipHostFree(pinned);
+      HIPCHECK(hipHostMalloc((void **)&pinned, words * sizeof(int), hipHostMallocDefault));
+      pinnedWords = words;
+    }
+    geometryDirty = true;
+  }
+
+  // applyTiles (FluidSimulation.ts:663-737).
+  void applyTiles() {
+    int needed = std::max(MIN_TILES, dir.slotsUsed());
+    if (needed > capacity) {
+      int cap = capacity;
+      while (cap < needed) cap *= 2;
+      growTiles(cap);
+    }
+    dir.tables(capacity, tileRecHost, hashHost, clusters, tilesHost);
+    HIPCHECK(hipMemcpyAsync(tileRec.p, tileRecHost.data(), tileRecHost.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice, stream));
+    if (hash.n != hashHost.size()) hash.alloc(hashHost.size());
+    HIPCHECK(hipMemcpyAsync(hash.p, hashHost.data(), hashHost.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    hashMask = (uint32_t)(hashHost.size() / 4 - 1);
+    std::vector<int> created = dir.takeCreated();
+    if (!created.empty()) {
+      if (createdSlots.n < created.size()) createdSlots.alloc(std::max<size_t>(created.size(), 64));
+      HIPCHECK(hipMemcpyAsync(createdSlots.p, created.data(), created.size() * sizeof(int), hipMemcpyHostToDevice,
+                              stream));
+      int n = (int)created.size() * 64;
+      k_clear_tiles<<<(n + 255) / 256, 256, 0, stream>>>(pages.p, entries.p, brickState.p, activity.p,
+                                                          createdSlots.p, (int)created.size());
+    }
+    HIPCHECK(hipStreamSynchronize(stream));  // host vectors are reused
+    geometryDirty = true;
+  }
+
+  // growPool (FluidSimulation.ts:1013-1093).
+  void growPool(int target) {
+    int limit = slotLimit();
+    int rows = host::poolRows(std::min(target, limit), side);
+    int slots = std::min(limit, rows * side);
+    if (slots <= poolSlots) return;
+    int added = slots - poolSlots;
+    allocatePools(slots, true);
+    DBuf<uint32_t> next;
+    next.alloc(slots);
+    HIPCHECK(hipMemcpyAsync(next.p + added, freeSlots.p, (size_t)poolSlots * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                            stream));
+    k_grow_free_list<<<(added + 255) / 256, 256, 0, stream>>>(next.p, poolState.p, poolSlots, added);
+    HIPCHECK(hipStreamSynchronize(stream));
+    freeSlots.release();
+    freeSlots = next;
+    next.p = nullptr;
+    next.n = 0;
+    freeCount += added;
+    poolSlots = slots;
+    poolGrownAt = steps;
+    geometryDirty = true;
+  }
+
+  // ---- one step ----
+  int sourceBricks(const fp_source &s, double dt) const {
+    double size = B * voxel;
+    double count = 1;
+    for (int a = 0; a < 3; a++) {
+      double motion = std::fabs(s.velocity[a]) + std::fabs(s.outward_speed);
+      double pad = std::min(size, 2 * velocitySpacing() + motion * dt);
+      count *= std::ceil((2 * (std::max(s.size[a], 0.03) + pad)) / size) + 1;
+    }
+    return (int)count;
+  }
+  void needBox(const double lo[3], const double hi[3], double margin) {
+    double size = tileSize();
+    int l[3], h[3];
+    for (int a = 0; a < 3; a++) {
+      l[a] = (int)std::floor((lo[a] - margin) / size);
+      h[a] = (int)std::floor((hi[a] + margin) / size);
+    }
+    dir.need(l, h, steps);
+  }
+  // updateTiles (FluidSimulation.ts:873-912).
+  void updateTiles(const fp_frame &f) {
+    dir.retire(steps);
+    double margin = 2 * B * voxel + tileSize();
+    for (int i = 0; i < f.source_count; i++) {
+      const fp_source &s = f.sources[i];
+      double extent = std::max(std::max(std::max(s.size[0], s.size[1]), s.size[2]), 0.03) + s.outward_speed * 0.25;
+      double lo[3], hi[3];
+      for (int a = 0; a < 3; a++) {
+        lo[a] = s.position[a] - extent;
+        hi[a] = s.position[a] + extent;
+      }
+      needBox(lo, hi, margin);
+    }
+    for (auto &ct : contentTiles) {
+      int lo[3], hi[3];
+      for (int a = 0; a < 3; a++) {
+        lo[a] = ct.first[a] - 1;
+        hi[a] = ct.first[a] + 1;
+      }
+      dir.need(lo, hi, steps);
+    }
+    if (dir.dirty) applyTiles();
+  }
+
+  static double finite32(const char *name, double v) {
+    float f = (float)v;
+    if (!std::isfinite(v) || !std::isfinite(f)) throw InvalidArgument(std::string(name) + " must be finite in f32.");
+    return (double)f;
+  }
+  // packFlameStep (flame.ts:43-87) with its validation.
+  void packFlame(const fp_flame &fl, double dt) {
+    double stepf = finite32("Flame timestep", dt), lifespan = finite32("Flame lifespan", fl.lifespan);
+    if (stepf <= 0 || lifespan <= 0) throw InvalidArgument("Flame timestep and lifespan must be positive in f32.");
+    finite32("Lifetime decrement", stepf / lifespan);
+    double smoke = finite32("Smoke rate", fl.smoke_rate), heat = finite32("Heat rate", fl.heat_rate);
+    double expansion = finite32("Expansion rate", fl.expansion_rate);
+    if (smoke < 0 || heat < 0) throw InvalidArgument("Smoke and heat rates must be nonnegative.");
+    for (double rate : {smoke, heat, expansion}) finite32("Integrated flame output", rate * lifespan);
+    const char *names[3] = {"smoke", "heat", "expansion"};
+    for (int ch = 0; ch < 3; ch++) {
+      int n = fl.knot_count[ch];
+      if (n < 2 || n > 8) throw InvalidArgument(std::string(names[ch]) + " ramp needs 2–8 knots.");
+      double previous = -1;
+      for (int i = 0; i < n; i++) {
+        double x = finite32("lifetime", fl.knots[ch][i][0]), y = finite32("response", fl.knots[ch][i][1]);
+        if (x < 0 || x > 1 || y < 0 || y > 1 || x <= previous)
+          throw InvalidArgument(std::string(names[ch]) + " knots must increase in f32 with coordinates in [0,1].");
+        if ((i == 0 && x != 0) || (i == n - 1 && x != 1))
+          throw InvalidArgument(std::string(names[ch]) + " ramp must span lifetime 0–1.");
+        P.knots[ch][i][0] = (float)x;
+        P.knots[ch][i][1] = (float)y;
+        previous = x;
+      }
+      P.knotCount[ch] = n;
+    }
+    P.fdt = (float)stepf;
+    P.lifespan = (float)lifespan;
+    P.fsmoke = (float)smoke;
+    P.fheat = (float)heat;
+    P.fexpansion = (float)expansion;
+  }
+
+  template <class T> void upload(DBuf<T> &buf, const std::vector<T> &host) {
+    size_t n = std::max<size_t>(1, host.size());
+    if (buf.n < n) buf.alloc(n + n / 2);
+    if (!host.empty())
+      HIPCHECK(hipMemcpyAsync(buf.p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+  }
+
+  // packScene (FluidSimulation.ts:1150-1366): sphere/box emitters and colliders, analytic
+  // forces. Returns whether the colliders' packed words changed.
+  std::vector<Emitter> hostEmitters;
+  std::vector<Collider> hostColliders;
+  std::vector<ForceRec> hostForces;
+  std::vector<Region> hostRegions;
+  bool packScene(const fp_frame &f) {
+    hostEmitters.clear();
+    for (int i = 0; i < f.source_count; i++) {
+      const fp_source &s = f.sources[i];
+      Emitter e;
+      for (int a = 0; a < 3; a++) {
+        e.pos[a] = (float)s.position[a];
+        e.size[a] = (float)s.size[a];
+        e.vel[a] = (float)s.velocity[a];
+      }
+      e.box = s.shape == FP_SHAPE_BOX;
+      e.flame = (float)s.flame;
+      e.heat = (float)s.heat_rate;
+      e.smoke = (float)s.smoke_rate;
+      e.response = (float)s.velocity_response;
+      e.outward = (float)s.outward_speed;
+      e.period = (float)(s.has_variation ? s.variation_period : 0.5);
+      e.strength = (float)(s.has_variation ? s.variation_strength : 0.0);
+      e.fuel = (float)s.fuel_rate;
+      hostEmitters.push_back(e);
+    }
+    int n = f.collider_count;
+    std::vector<uint32_t> packed((size_t)std::max(1, n) * 28, 0u);
+    hostColliders.clear();
+    for (int i = 0; i < n; i++) {
+      const fp_collider &c = f.colliders[i];
+      if (c.shape != FP_SHAPE_SPHERE && c.shape != FP_SHAPE_BOX)
+        throw InvalidArgument("collider.shape.type must be sphere or box.");
+      float w[20] = {0};
+      for (int a = 0; a < 3; a++) {
+        w[a] = (float)c.position[a];
+        w[4 + a] = (float)c.size[a];
+        for (int k = 0; k < 3; k++) w[8 + 4 * k + a] = (float)c.axes[k][a];
+      }
+      w[3] = c.shape == FP_SHAPE_BOX ? 1.0f : 0.0f;
+      std::memcpy(&packed[(size_t)i * 28], w, sizeof(w));
+      Collider cr;
+      for (int a = 0; a < 3; a++) {
+        cr.center[a] = w[a];
+        cr.extent[a] = w[4 + a];
+        for (int k = 0; k < 3; k++) cr.axes[k][a] = w[8 + 4 * k + a];
+      }
+      cr.box = c.shape == FP_SHAPE_BOX;
+      hostColliders.push_back(cr);
+    }
+    bool changed = packed != prevColliders || n != colliderCount;
+    prevColliders = packed;
+    colliderCount = n;
+    hostForces.clear();
+    hostRegions.clear();
+    for (int i = 0; i < f.force_count; i++) {
+      const fp_force &fc = f.forces[i];
+      if (fc.type < 0 || fc.type > 3) throw InvalidArgument("force.type must be wind, turbulence, vortex or radial.");
+      ForceRec r;
+      for (int a = 0; a < 3; a++) {
+        r.axis[a] = (float)fc.vector[a];
+        r.center[a] = (float)fc.center[a];
+      }
+      r.strength = (float)fc.strength;
+      r.kind = (float)fc.type;
+      r.scale = (float)fc.scale;
+      r.lift = (float)fc.lift;
+      r.inward = (float)fc.inward;
+      r.targeted = fc.targeted != 0;
+      r.firstRegion = (int)hostRegions.size();
+      r.regionCount = fc.targeted ? fc.source_count : 0;
+      for (int k = 0; k < r.regionCount; k++) {
+        const fp_source &s = fc.sources[k];
+        Region g;
+        for (int a = 0; a < 3; a++) {
+          g.center[a] = (float)s.position[a];
+          g.size[a] = (float)s.size[a];
+        }
+        g.kind = s.shape == FP_SHAPE_BOX ? 3.0f : 1.0f;
+        hostRegions.push_back(g);
+      }
+      hostForces.push_back(r);
+    }
+    upload(emitters, hostEmitters);
+    upload(colliders, hostColliders);
+    upload(forces, hostForces);
+    upload(regions, hostRegions);
+    emitterCount = (int)hostEmitters.size();
+    forceCount = (int)hostForces.size();
+    return changed;
+  }
+
+  void setParams(const fp_frame &f, double dt) {
+    const fp_settings &s = f.simulation;
+    P.hf = (float)voxel;
+    P.hv = (float)velocitySpacing();
+    P.dt = (float)dt;
+    P.buoyancy = (float)s.buoyancy;
+    P.cooling = (float)s.cooling;
+    P.smokeWeight = (float)s.smoke_weight;
+    P.damping = (float)s.velocity_damping;
+    P.dissipation = (float)s.dissipation;
+    P.vorticity = (float)s.vorticity;
+    P.cutoff = (float)cutoff;
+    P.lifetimeZero = !mayHaveFlame;
+    P.time = (float)time;
+    P.seed = (float)s.seed;
+    std::memcpy(&P.seedBits, &P.seed, 4);
+    P.colliders = colliderCount;
+    P.ground = ground;
+    fp_fuel fuel = f.has_fuel ? f.fuel : fp_fuel{0, 0.5, 4.0};
+    P.ignition = (float)fuel.ignition_heat;
+    P.burnRate = (float)fuel.burn_rate;
+    P.fuelEnabled = fuel.enabled != 0;
+    P.emitterCount = emitterCount;
+    P.forceCount = forceCount;
+    P.maccormack = maccormack;
+  }
+
+  AllocArgs allocArgs() {
+    AllocArgs a;
+    a.dir = dirArgs();
+    a.L = layout();
+    a.P = P;
+    a.brickCount = brickCount();
+    a.blocks = (brickCount() + 255) / 256;
+    a.pages = pages.p;
+    a.entries = entries.p;
+    a.records = records.p;
+    a.brickState = brickState.p;
+    a.activity = activity.p;
+    a.stateTmp = stateTmp.p;
+    a.rankTmp = rankTmp.p;
+    a.blockSums = blockSums.p;
+    a.blockPrefix = blockPrefix.p;
+    a.freeSlots = freeSlots.p;
+    a.poolState = poolState.p;
+    a.held = std::max(0, poolSlots - slotLimit());
+    a.counts = counts.p;
+    a.active = active.p;
+    a.zeroList = zeroList.p;
+    a.emitters = emitters.p;
+    a.colliders = colliders.p;
+    a.solids = solids.p;
+    a.colliderCount = colliderCount;
+    return a;
+  }
+  Pools pools() {
+    Pools p;
+    p.F = F.p;
+    p.Fp = Fp.p;
+    p.Fn = Fn.p;
+    p.E = E.p;
+    p.V = V.p;
+    p.Vadv = Vadv.p;
+    p.Vcor = Vcor.p;
+    p.Vf = Vf.p;
+    p.Curl = Curl.p;
+    p.D = D.p;
+    p.solids = solids.p;
+    p.noise = noise.p;
+    return p;
+  }
+  StepArgs stepArgs() {
+    StepArgs s;
+    s.dir = dirArgs();
+    s.L = layout();
+    s.P = P;
+    s.p = pools();
+    s.counts = counts.p;
+    s.active = active.p;
+    s.activity = activity.p;
+    s.brickCount = brickCount();
+    s.emitters = emitters.p;
+    s.forces = forces.p;
+    s.regions = regions.p;
+    return s;
+  }
+  MGArgs mgArgs() {
+    MGArgs m;
+    m.dir = dirArgs();
+    m.L = layout();
+    m.P = P;
+    m.counts = counts.p;
+    m.active = active.p;
+    m.zeroList = zeroList.p;
+    m.solids = solids.p;
+    for (int L = 0; L <= bottom; L++) {
+      m.lv[L].p = lv[L].p.p;
+      m.lv[L].q = lv[L].q.p;
+      m.lv[L].rhs = lv[L].rhs.p;
+      m.lv[L].topo = lv[L].topo.p;
+      m.lv[L].eflag = lv[L].eflag.p;
+      m.lv[L].side = lv[L].side;
+      m.lv[L].tile = lv[L].tile;
+    }
+    m.lv[0].p = P0.p;
+    m.lv[0].q = P1.p;
+    m.lv[0].rhs = D.p;
+    m.bottom = bottom;
+    float inv = 1.0f / P.hv;
+    float ref = 1.0f / inv;
+    float nrm = ref * inv;
+    m.weight = nrm * nrm;
+    return m;
+  }
+
+  void step(const fp_frame &f, double dt) {
+    if (!std::isfinite(dt) || dt <= 0 || dt > MAX_DT)
+      throw InvalidArgument("Simulation step must be in (0, 1/30]. Use fixed substeps.");
+    if (f.source_count < 0 || f.force_count < 0 || f.collider_count < 0)
+      throw InvalidArgument("frame counts must be nonnegative.");
+    substep(f, dt);
+    time += dt;
+    steps++;
+    stepped = true;
+  }
+
+  void substep(const fp_frame &f, double dt) {
+    if (dir.ground != ground) {
+      dir.ground = ground;
+      geometryDirty = true;
+    }
+    updateTiles(f);
+    int wanted = 0;
+    for (int i = 0; i < f.source_count; i++)
+      if (std::find(sourceIds.begin(), sourceIds.end(), f.sources[i].id) == sourceIds.end())
+        wanted += sourceBricks(f.sources[i], dt);
+    sourceIds.clear();
+    for (int i = 0; i < f.source_count; i++) sourceIds.push_back(f.sources[i].id);
+    if (wanted > 0) poolTarget = std::max(poolTarget, (int)std::ceil((usedSlots + wanted) * 1.5));
+    if (poolTarget > poolSlots) growPool(poolTarget);
+    if (packScene(f)) {
+      solidsStale = true;
+      geometryDirty = true;
+    }
+    packFlame(f.combustion, dt);
+    fp_fuel fuel = f.has_fuel ? f.fuel : fp_fuel{0, 0.5, 4.0};
+    bool anyFlame = false;
+    for (int i = 0; i < f.source_count; i++) anyFlame = anyFlame || f.sources[i].flame > 0;
+    mayHaveFlame = mayHaveFlame || fuel.enabled || anyFlame;
+    setParams(f, dt);
+    enqueueStep();
+    readActivity();
+  }
+
+  // Every kernel of one step, in stream order.
+  void enqueueStep() {
+    const int n = brickCount();
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    const unsigned slotsGrid = (unsigned)std::max(1, poolSlots);
+    AllocArgs a = allocArgs();
+    int m0 = begin();
+    k_build_bricks<<<blocks, 256, 0, stream>>>(a);
+    k_scan_blocks<<<1, 1024, 0, stream>>>(a);
+    k_free_bricks<<<blocks, 256, 0, stream>>>(a);
+    k_allocate_bricks<<<blocks, 256, 0, stream>>>(a);
+    k_link_bricks<<<(slotsGrid + 63) / 64, 64, 0, stream>>>(a);
+    if (colliderCount > 0 || solidsStale) k_build_solids<<<slotsGrid, 64, 0, stream>>>(a, solidsStale ? 1 : 0);
+    solidsStale = false;
+    const int zchunks = std::max(1, nF / 1024);
+    k_zero_fresh<<<slotsGrid * zchunks, 256, 0, stream>>>(a, pools(), zchunks);
+    end(m0, "allocation");
+
+    StepArgs s = stepArgs();
+    int m1 = begin();
+    k_advect_velocity<<<slotsGrid, 256, 0, stream>>>(s);
+    end(m1, "advect_velocity");
+    int m2 = begin();
+    k_correct_velocity<<<slotsGrid, 256, 0, stream>>>(s);
+    end(m2, "correct_velocity");
+    const int fchunks = std::max(1, nF / 256);
+    int m3 = begin();
+    k_advect_scalars<<<slotsGrid * fchunks, 256, 0, stream>>>(s, fchunks);
+    end(m3, "advect_scalars");
+    int m4 = begin();
+    k_evolve_fields<<<slotsGrid * fchunks, 256, 0, stream>>>(s, fchunks);
+    end(m4, "evolve_fields");
+    std::swap(F, Fn);
+    s.p.F = F.p;
+    s.p.Fn = Fn.p;
+    int m5 = begin();
+    if (P.vorticity != 0.f) k_curl<<<slotsGrid, 256, 0, stream>>>(s);
+    k_forces<<<slotsGrid, 256, 0, stream>>>(s);
+    k_divergence<<<slotsGrid, 256, 0, stream>>>(s, D.p);
+    end(m5, "forces_divergence");
+    int m6 = begin();
+    solvePressure(slotsGrid);
+    end(m6, "pressure");
+    int m7 = begin();
+    k_project<<<slotsGrid, 256, 0, stream>>>(s, latestPressure);
+    end(m7, "project");
+    k_readback<<<(unsigned)((capacity + 255) / 256 + 1), 256, 0, stream>>>(counts.p, poolState.p, activity.p + n,
+                                                                           capacity, readback.p);
+    HIPCHECK(hipMemcpyAsync(pinned, readback.p, (4 + (size_t)capacity) * sizeof(int), hipMemcpyDeviceToHost, stream));
+    k_reset_failures<<<1, 1, 0, stream>>>(poolState.p);
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipGetLastError());
+    if (profiling) collect();
+  }
+
+  // PressureSolver.solve (PressureSolver.ts:371-402): one cold-start V-cycle.
+  void solvePressure(unsigned slotsGrid) {
+    MGArgs m = mgArgs();
+    if (geometryDirty) {
+      k_topology_bricks<<<slotsGrid, 256, 0, stream>>>(m, 0);
+      for (int L = brickLevels; L <= bottom; L++) k_topology_tiles<<<capacity, 64, 0, stream>>>(m, L);
+      geometryDirty = false;
+    }
+    k_topology_bricks<<<slotsGrid * 2, 256, 0, stream>>>(m, 1);
+    k_eflags_bricks<<<slotsGrid, 256, 0, stream>>>(m);
+    for (int L = brickLevels; L <= bottom; L++) k_eflags_tiles<<<capacity, 64, 0, stream>>>(m, L);
+    cycle(m, 0, slotsGrid);
+    latestPressure = m.lv[0].p;
+    // Keep the buffers' roles: level 0's iterate lives in P0 or P1.
+  }
+  void relax(MGArgs &m, int L, unsigned slotsGrid) {
+    if (L < brickLevels) k_mg_relax_bricks<<<slotsGrid, 256, 0, stream>>>(m, L);
+    else k_mg_relax_tiles<<<capacity, 64, 0, stream>>>(m, L);
+    std::swap(m.lv[L].p, m.lv[L].q);
+  }
+  void cycle(MGArgs &m, int L, unsigned slotsGrid) {
+    if (L == bottom) {
+      if (capacity <= 1024) k_mg_bottom<<<1, 1024, 0, stream>>>(m, 19);
+      else
+        for (int i = 0; i < 19; i++) relax(m, L, slotsGrid);
+      return;
+    }
+    if (L == 0) k_mg_init<<<slotsGrid, 256, 0, stream>>>(m);
+    else {
+      relax(m, L, slotsGrid);
+      relax(m, L, slotsGrid);
+    }
+    if (L + 1 < brickLevels) k_mg_restrict_bricks<<<slotsGrid, 64, 0, stream>>>(m, L + 1);
+    else k_mg_restrict_tiles<<<capacity, 64, 0, stream>>>(m, L + 1);
+    cycle(m, L + 1, slotsGrid);
+    if (L < brickLevels) k_mg_prolong_bricks<<<slotsGrid, 256, 0, stream>>>(m, L);
+    else k_mg_prolong_tiles<<<capacity, 64, 0, stream>>>(m, L);
+    std::swap(m.lv[L].p, m.lv[L].q);
+    relax(m, L, slotsGrid);
+    relax(m, L, slotsGrid);
+    relax(m, L, slotsGrid);
+  }
+
+  // readActivity (FluidSimulation.ts:1701-1746), applied at once.
+  void readActivity() {
+    int bricks = pinned[0], free = pinned[1], failures = pinned[2];
+    lastZeroed = pinned[3];
+    contentTiles.clear();
+    for (int t = 0; t < capacity; t++) {
+      uint32_t flags = (uint32_t)pinned[4 + t];
+      if (flags && t < (int)dir.coords.size() && dir.coords[t][3])
+        contentTiles.push_back({{dir.coords[t][0], dir.coords[t][1], dir.coords[t][2]}, flags});
+    }
+    lastFailures = failures;
+    freeCount = free;
+    activeBricks = bricks;
+    if (steps < poolGrownAt) return;
+    usedSlots = poolSlots - free;
+    int limit = slotLimit();
+    poolLimited = failures > 0 && poolSlots >= limit;
+    if (failures > 0 && poolSlots < limit)
+      poolTarget = std::max(std::max(poolTarget, (int)std::ceil((usedSlots + (double)failures) * 1.5)), poolSlots + side);
+  }
+
+  void reset() {
+    allocate();
+    time = 0;
+    steps = 0;
+    mayHaveFlame = false;
+    poolGrownAt = 0;
+    activeBricks = 0;
+    stepped = false;
+  }
+
+  size_t memoryBytes() const {
+    size_t total = tileRec.bytes() + hash.bytes() + pages.bytes() + entries.bytes() + records.bytes() +
+                   active.bytes() + zeroList.bytes() + brickState.bytes() + activity.bytes() + stateTmp.bytes() +
+                   rankTmp.bytes() + blockSums.bytes() + blockPrefix.bytes() + freeSlots.bytes() + F.bytes() +
+                   Fp.bytes() + Fn.bytes() + Curl.bytes() + noise.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
+                   Vcor.bytes() + Vf.bytes() + D.bytes() + P0.bytes() + P1.bytes() + solids.bytes() +
+                   emitters.bytes() + colliders.bytes() + forces.bytes() + regions.bytes();
+    for (int L = 0; L <= bottom; L++)
+      total += lv[L].p.bytes() + lv[L].q.bytes() + lv[L].rhs.bytes() + lv[L].topo.bytes() + lv[L].eflag.bytes();
+    return total;
+  }
+
+  void destroy() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    DBuf<int> *ints[] = {&tileRec, &hash, &pages, &entries, &records, &active, &zeroList, &poolState, &counts, &createdSlots, &readback};
+    for (auto *x : ints) x->release();
+    DBuf<uint32_t> *u32s[] = {&brickState, &activity, &stateTmp, &rankTmp, &blockSums, &blockPrefix, &freeSlots, &solids};
+    for (auto *x : u32s) x->release();
+    DBuf<float4> *f4s[] = {&F, &Fp, &Fn, &Curl, &noise};
+    for (auto *x : f4s) x->release();
+    DBuf<float> *fs[] = {&E, &V, &Vadv, &Vcor, &Vf, &D, &P0, &P1};
+    for (auto *x : fs) x->release();
+    for (int L = 0; L < MAX_LEVELS; L++) {
+      lv[L].p.release();
+      lv[L].q.release();
+      lv[L].rhs.release();
+      lv[L].topo.release();
+      lv[L].eflag.release();
+    }
+    emitters.release();
+    colliders.release();
+    forces.release();
+    regions.release();
+    for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
+    eventPool.clear();
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+namespace {
+template <class Fn> int guard(Fn fn) {
+  try {
+    fn();
+    return FP_OK;
+  } catch (const InvalidArgument &e) {
+    g_error = e.what();
+    return FP_EINVAL;
+  } catch (const DeviceFailure &e) {
+    g_error = e.what();
+    return FP_EDEVICE;
+  } catch (const std::bad_alloc &e) {
+    g_error = "host allocation failed";
+    return FP_ENOMEM;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return FP_ESTATE;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char *fp_last_error(void) { return g_error.c_str(); }
+const char *fp_version(void) { return "firepro-mi355x 0.1 (gfx950)"; }
+
+int fp_sim_create(const fp_grid *grid, fp_sim **out) {
+  if (!grid || !out) {
+    g_error = "fp_sim_create: null argument";
+    return FP_EINVAL;
+  }
+  *out = nullptr;
+  fp_sim *sim = new fp_sim();
+  int rc = guard([&] { sim->create(*grid); });
+  if (rc != FP_OK) {
+    sim->destroy();
+    delete sim;
+    return rc;
+  }
+  *out = sim;
+  return FP_OK;
+}
+
+int fp_sim_step(fp_sim *sim, const fp_frame *frame, double dt) {
+  if (!sim || !frame) {
+    g_error = "fp_sim_step: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] { sim->step(*frame, dt); });
+}
+
+int fp_sim_reset(fp_sim *sim) {
+  if (!sim) {
+    g_error = "fp_sim_reset: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] { sim->reset(); });
+}
+
+void fp_sim_destroy(fp_sim *sim) {
+  if (!sim) return;
+  sim->destroy();
+  delete sim;
+}
+
+int fp_sim_set_cutoff(fp_sim *sim, double cutoff) {
+  if (!sim || !std::isfinite(cutoff)) {
+    g_error = "cutoff must be a finite number.";
+    return FP_EINVAL;
+  }
+  sim->cutoff = cutoff;
+  return FP_OK;
+}
+int fp_sim_set_budget(fp_sim *sim, double maxVoxels) {
+  if (!sim || !(maxVoxels > 0)) {
+    g_error = "voxel budget must be positive.";
+    return FP_EINVAL;
+  }
+  sim->voxelBudget = maxVoxels;
+  return FP_OK;
+}
+int fp_sim_set_ground(fp_sim *sim, int32_t ground) {
+  if (!sim) {
+    g_error = "fp_sim_set_ground: null argument";
+    return FP_EINVAL;
+  }
+  sim->ground = ground != 0;
+  return FP_OK;
+}
+
+int fp_sim_stats(const fp_sim *sim, fp_stats *out) {
+  if (!sim || !out) {
+    g_error = "fp_sim_stats: null argument";
+    return FP_EINVAL;
+  }
+  out->time = sim->time;
+  out->steps = sim->steps;
+  out->active_bricks = sim->activeBricks;
+  out->active_voxels = sim->activeBricks * (int64_t)sim->nF;
+  out->pool_slots = sim->poolSlots;
+  out->used_slots = sim->usedSlots;
+  out->pool_limited = sim->poolLimited;
+  out->tile_capacity = sim->capacity;
+  out->tile_count = sim->dir.count();
+  out->pool_side = sim->side;
+  out->memory_bytes = (int64_t)sim->memoryBytes();
+  return FP_OK;
+}
+
+int fp_sim_read_bricks(fp_sim *sim, double *mins, int32_t *count) {
+  if (!sim || !count) {
+    g_error = "fp_sim_read_bricks: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] {
+    int n = sim->brickCount();
+    std::vector<int32_t> pages(n);
+    HIPCHECK(hipMemcpy(pages.data(), sim->pages.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    double size = sim->B * sim->voxel;
+    int written = 0;
+    for (int id = 0; id < n; id++) {
+      int t = id / 64;
+      if (pages[id] < 0 || t >= (int)sim->dir.coords.size() || !sim->dir.coords[t][3]) continue;
+      int l = id % 64;
+      int loc[3] = {l & 3, (l >> 2) & 3, l >> 4};
+      if (mins && written < *count)
+        for (int a = 0; a < 3; a++) mins[written * 3 + a] = (sim->dir.coords[t][a] * 4 + loc[a]) * size;
+      written++;
+    }
+    *count = written;
+  });
+}
+
+int fp_sim_read_tables(fp_sim *sim, fp_tables *t) {
+  if (!sim || !t) {
+    g_error = "fp_sim_read_tables: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] {
+    int n = sim->brickCount();
+    t->brick_count = n;
+    t->pool_slots = sim->poolSlots;
+    t->free_count = sim->freeCount;
+    t->failures = sim->lastFailures;
+    int counts[8];
+    HIPCHECK(hipMemcpy(counts, sim->counts.p, sizeof(counts), hipMemcpyDeviceToHost));
+    int activeCount = sim->stepped ? counts[0] : 0;
+    int zeroCount = sim->stepped ? counts[1] + counts[5] : 0;
+    std::vector<int32_t> entries(n);
+    HIPCHECK(hipMemcpy(entries.data(), sim->entries.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (t->pages) HIPCHECK(hipMemcpy(t->pages, sim->pages.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (t->entries) std::memcpy(t->entries, entries.data(), n * sizeof(int32_t));
+    if (t->records) {
+      HIPCHECK(hipMemcpy(t->records, sim->records.p, (size_t)n * 28 * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; i++)
+        if (entries[i] < 0) std::memset(t->records + (size_t)i * 28, 0, 28 * sizeof(int32_t));
+    }
+    if (t->free_slots)
+      HIPCHECK(hipMemcpy(t->free_slots, sim->freeSlots.p, sim->poolSlots * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    t->active_count = activeCount;
+    if (t->active && activeCount)
+      HIPCHECK(hipMemcpy(t->active, sim->active.p, activeCount * sizeof(int32_t), hipMemcpyDeviceToHost));
+    t->zeroed_count = zeroCount;
+    if (t->zeroed && zeroCount) {
+      std::vector<std::pair<int32_t, uint32_t>> z(zeroCount);
+      std::vector<int32_t> raw(2 * zeroCount);
+      HIPCHECK(hipMemcpy(raw.data(), sim->zeroList.p, raw.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (int i = 0; i < zeroCount; i++) z[i] = {raw[2 * i], (uint32_t)raw[2 * i + 1]};
+      std::sort(z.begin(), z.end());
+      for (int i = 0; i < zeroCount; i++) {
+        t->zeroed[2 * i] = z[i].first;
+        t->zeroed[2 * i + 1] = (int32_t)z[i].second;
+      }
+    }
+  });
+}
+
+int fp_sim_read_state(fp_sim *sim, int32_t *ids, float *fields, float *velocity, float *pressure, float *expansion,
+                      int32_t capacity) {
+  if (!sim) {
+    g_error = "fp_sim_read_state: null argument";
+    return FP_EINVAL;
+  }
+  int result = 0;
+  int rc = guard([&] {
+    int count = 0;
+    if (sim->stepped) HIPCHECK(hipMemcpy(&count, sim->counts.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (count > capacity) throw InvalidArgument("state buffers hold fewer bricks than are active.");
+    std::vector<int32_t> act(count), entries(sim->brickCount());
+    if (count) HIPCHECK(hipMemcpy(act.data(), sim->active.p, count * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(entries.data(), sim->entries.p, entries.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int nF = sim->nF, nV = sim->nV;
+    for (int i = 0; i < count; i++) {
+      int id = act[i];
+      if (ids) ids[i] = id;
+      int slot = (int)((((uint32_t)(entries[id] & 0xffffff) >> 16) * sim->side + (((uint32_t)entries[id] >> 8) & 255u)) *
+                           sim->side +
+                       ((uint32_t)entries[id] & 255u));
+      if (fields)
+        HIPCHECK(hipMemcpy(fields + (size_t)i * nF * 4, sim->F.p + (size_t)slot * nF, nF * sizeof(float4),
+                           hipMemcpyDeviceToHost));
+      if (expansion)
+        HIPCHECK(hipMemcpy(expansion + (size_t)i * nF, sim->E.p + (size_t)slot * nF, nF * sizeof(float),
+                           hipMemcpyDeviceToHost));
+      if (velocity)
+        HIPCHECK(hipMemcpy(velocity + (size_t)i * nV * 3, sim->V.p + (size_t)slot * nV * 3, nV * 3 * sizeof(float),
+                           hipMemcpyDeviceToHost));
+      if (pressure)
+        HIPCHECK(hipMemcpy(pressure + (size_t)i * nV, sim->latestPressure + (size_t)slot * nV, nV * sizeof(float),
+                           hipMemcpyDeviceToHost));
+    }
+    result = count;
+  });
+  return rc != FP_OK ? rc : result;
+}
+
+int fp_sim_write_state(fp_sim *sim, const float *fields, const float *velocity) {
+  if (!sim) {
+    g_error = "fp_sim_write_state: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] {
+    int count = 0;
+    if (sim->stepped) HIPCHECK(hipMemcpy(&count, sim->counts.p, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int32_t> act(count), entries(sim->brickCount());
+    if (count) HIPCHECK(hipMemcpy(act.data(), sim->active.p, count * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(entries.data(), sim->entries.p, entries.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int nF = sim->nF, nV = sim->nV;
+    for (int i = 0; i < count; i++) {
+      uint32_t e = (uint32_t)entries[act[i]] & 0xffffffu;
+      int slot = (int)(((e >> 16) * sim->side + ((e >> 8) & 255u)) * sim->side + (e & 255u));
+      if (fields)
+        HIPCHECK(hipMemcpy(sim->F.p + (size_t)slot * nF, fields + (size_t)i * nF * 4, nF * sizeof(float4),
+                           hipMemcpyHostToDevice));
+      if (velocity)
+        HIPCHECK(hipMemcpy(sim->V.p + (size_t)slot * nV * 3, velocity + (size_t)i * nV * 3, nV * 3 * sizeof(float),
+                           hipMemcpyHostToDevice));
+    }
+  });
+}
+
+void *fp_sim_stream(fp_sim *sim) { return sim ? (void *)sim->stream : nullptr; }
+
+int fp_sim_profile(fp_sim *sim, int32_t enable) {
+  if (!sim) {
+    g_error = "fp_sim_profile: null argument";
+    return FP_EINVAL;
+  }
+  sim->profiling = enable != 0;
+  sim->profile.clear();
+  sim->profileNames.clear();
+  return FP_OK;
+}
+
+int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *count) {
+  if (!sim || !count) {
+    g_error = "fp_sim_profile_read: null argument";
+    return FP_EINVAL;
+  }
+  int n = std::min<int>(*count, (int)sim->profileNames.size());
+  for (int i = 0; i < n; i++) {
+    const auto &e = sim->profile[sim->profileNames[i]];
+    if (names) names[i] = sim->profileNames[i].c_str();
+    if (ms) ms[i] = e.second ? e.first / e.second : 0.0;
+  }
+  *count = n;
+  return FP_OK;
+}
+
+}  // extern "C"
