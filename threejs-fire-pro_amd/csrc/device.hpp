@@ -135,7 +135,9 @@ __device__ __forceinline__ float smoothstepv(float e0, float e1, float x) {
   float t = clampv((x - e0) / (e1 - e0), 0.0f, 1.0f);
   return t * t * (3.0f - 2.0f * t);
 }
-__device__ __forceinline__ float length3(float x, float y, float z) { return __fsqrt_rn(x * x + y * y + z * z); }
+// sqrtf is the correctly rounded square root (v_sqrt_f32 plus its fix-up); __fsqrt_rn
+// compiles to the bare approximation on gfx950.
+__device__ __forceinline__ float length3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 
 // exp in single precision: Cody & Waite reduction x = k ln2 + r with a two-part ln2,
 // the degree-7 Taylor polynomial of e^r (|r| <= ln2/2) in Horner form, and an exact
