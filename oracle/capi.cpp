@@ -256,6 +256,7 @@ double orc_emitter_weight(const double *pos, const double *size, int box, const 
     e.size[a] = size[a];
   }
   e.box = box != 0;
+  for (int a = 0; a < 3; a++) e.invSize[a] = 1.0 / std::max(e.size[a], 0.03);
   return oracle::emitterWeight(e, w);
 }
 double orc_burst_perlin(uint32_t seedBits, double x, double y, double z) {

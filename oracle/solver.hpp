@@ -65,6 +65,8 @@ template <class R> struct Emitter {
   R flame, heat, smoke, response;
   R vel[3];
   R outward, period, strength, fuel;
+  R invSize[3];  // 1 / max(size, 0.03)
+  R invPeriod;   // 1 / period
 };
 template <class R> struct Collider {
   R center[3], extent[3];
@@ -75,6 +77,7 @@ template <class R> struct Region {
   R center[3];
   R kind;  // 1 sphere, 3 box
   R size[3];
+  R invSize[3];  // 1 / max(size, 0.03)
 };
 template <class R> struct Force {
   R axis[3], strength, center[3], kind, scale, lift, inward;
@@ -82,8 +85,11 @@ template <class R> struct Force {
   std::vector<Region<R>> regions;
 };
 
+// World-to-cell conversions multiply by the reciprocal of the spacing, rounded once: WGSL
+// allows its division 2.5 ulp, and the product stays within that.
 template <class R> struct Params {
   R hf, hv, dt;
+  R invHf, invHv;
   R buoyancy, cooling, smokeWeight, damping, dissipation, vorticity, cutoff;
   bool lifetimeZero;
   R time, seed;
@@ -130,8 +136,11 @@ template <class R> void evaluateFlame(const Params<R> &p, R before, R &after, R 
 
 // emitterWeight (fluid-sources.wgsl:98-111).
 template <class R> R emitterWeight(const Emitter<R> &e, const R w[3]) {
+  // Past the padded size on any axis |d| >= 1 there, so r >= 1: no weight.
+  for (int a = 0; a < 3; a++)
+    if (fabs_(w[a] - e.pos[a]) >= fmax(e.size[a], R(0.03))) return 0;
   R d[3];
-  for (int a = 0; a < 3; a++) d[a] = (w[a] - e.pos[a]) / fmax(e.size[a], R(0.03));
+  for (int a = 0; a < 3; a++) d[a] = (w[a] - e.pos[a]) * e.invSize[a];
   R r = e.box ? fmax(fabs_(d[0]), fmax(fabs_(d[1]), fabs_(d[2]))) : length3(d[0], d[1], d[2]);
   if (r >= R(1)) return 0;
   return R(1) - smoothstepv(R(0.5), R(1), r);
@@ -176,7 +185,7 @@ template <class R> void burstVariation(uint32_t seedBits, const Emitter<R> &e, c
   }
   R q[3];
   const R off[3] = {R(3.17), R(7.53), R(11.29)};
-  for (int a = 0; a < 3; a++) q[a] = (w[a] - e.pos[a]) / e.period + off[a];
+  for (int a = 0; a < 3; a++) q[a] = (w[a] - e.pos[a]) * e.invPeriod + off[a];
   R s0 = burstPattern<R>(seedBits, q[0], q[1], q[2]);
   R s1 = burstPattern<R>(seedBits, q[0] + R(17.3), q[1] + R(2.8), q[2] + R(8.6));
   out[0] = R(1) - e.strength * s0;
@@ -540,7 +549,7 @@ template <class R> class Sim {
   // fieldCell + solidAt (fluid-common.wgsl:123-150): the quarter-cell bias.
   bool solidAt(const R w[3]) const {
     int c[3];
-    for (int a = 0; a < 3; a++) c[a] = (int)ffloor(w[a] / P.hf + R(0.25));
+    for (int a = 0; a < 3; a++) c[a] = (int)ffloor(w[a] * P.invHf + R(0.25));
     return solidCell(c[0], c[1], c[2]);
   }
 
@@ -589,7 +598,7 @@ template <class R> class Sim {
   void velocityAt(const R w[3], R out[3]) const {
     for (int c = 0; c < 3; c++) {
       R pos[3];
-      for (int a = 0; a < 3; a++) pos[a] = (a == c ? w[a] - R(0.5) * P.hv : w[a]) / P.hv;
+      for (int a = 0; a < 3; a++) pos[a] = (a == c ? w[a] - R(0.5) * P.hv : w[a]) * P.invHv;
       out[c] = sampleVel(V, c, pos);
     }
   }
@@ -747,6 +756,8 @@ template <class R> class Sim {
       e.period = (R)(float)(s.has_variation ? s.variation_period : 0.5);
       e.strength = (R)(float)(s.has_variation ? s.variation_strength : 0.0);
       e.fuel = (R)(float)s.fuel_rate;
+      for (int a = 0; a < 3; a++) e.invSize[a] = R(1) / fmax(e.size[a], R(0.03));
+      e.invPeriod = R(1) / e.period;
       emitters.push_back(e);
     }
     // Colliders: 28 words each, compared with the last upload (BufferUploads.ts:6-16).
@@ -798,6 +809,7 @@ template <class R> class Sim {
           r.size[a] = (R)(float)s.size[a];
         }
         r.kind = s.shape == FP_SHAPE_BOX ? R(3) : R(1);
+        for (int a = 0; a < 3; a++) r.invSize[a] = R(1) / fmax(r.size[a], R(0.03));
         F_.regions.push_back(r);
       }
       forces.push_back(F_);
@@ -809,6 +821,8 @@ template <class R> class Sim {
     const fp_settings &s = f.simulation;
     P.hf = (R)(float)voxel;
     P.hv = (R)(float)velocitySpacing();
+    P.invHf = R(1) / P.hf;
+    P.invHv = R(1) / P.hv;
     P.dt = (R)(float)dt;
     P.buoyancy = (R)(float)s.buoyancy;
     P.cooling = (R)(float)s.cooling;
@@ -1085,8 +1099,8 @@ template <class R> class Sim {
     R sp[3];
     for (int a = 0; a < 3; a++) {
       R rel = dep[a] - off[a];
-      donor[a] = (int)ffloor(rel / P.hv - R(0.5));
-      sp[a] = rel / P.hv;
+      donor[a] = (int)ffloor(rel * P.invHv - R(0.5));
+      sp[a] = rel * P.invHv;
     }
     return sampleVel(V, c, sp);
   }
@@ -1133,14 +1147,14 @@ template <class R> class Sim {
     for (int a = 0; a < 3; a++) mid[a] = fp[a] - R(0.5) * P.dt * fv[a];
     velocityAt(mid, v);
     int base[3];
-    for (int a = 0; a < 3; a++) base[a] = (int)ffloor((fp[a] - P.dt * v[a] - off[a]) / P.hv - R(0.5));
+    for (int a = 0; a < 3; a++) base[a] = (int)ffloor((fp[a] - P.dt * v[a] - off[a]) * P.invHv - R(0.5));
     R lo, hi;
     boundsAt(base, c, lo, hi);
     if (lo == hi && predicted == lo) return predicted;
     R fmid[3], fw[3], sp[3];
     for (int a = 0; a < 3; a++) fmid[a] = fp[a] + R(0.5) * P.dt * fv[a];
     velocityAt(fmid, fw);
-    for (int a = 0; a < 3; a++) sp[a] = (fp[a] + P.dt * fw[a] - off[a]) / P.hv;
+    for (int a = 0; a < 3; a++) sp[a] = (fp[a] + P.dt * fw[a] - off[a]) * P.invHv;
     R reversed = sampleVel(Vadv, c, sp);
     R corrected = predicted + R(0.5) * (orig[c] - reversed);
     return (corrected < lo || corrected > hi) ? predicted : corrected;
@@ -1210,7 +1224,7 @@ template <class R> class Sim {
   // The departure cell of a backward trace in cell coordinates (:134-136).
   void departureCell(const int id[3], const R pos[3], const R dep[3], int base[3], R frac[3]) const {
     for (int a = 0; a < 3; a++) {
-      R cell = (R)id[a] + (dep[a] - pos[a]) / P.hf;
+      R cell = (R)id[a] + (dep[a] - pos[a]) * P.invHf;
       R fl = ffloor(cell);
       base[a] = (int)fl;
       frac[a] = cell - fl;
@@ -1225,7 +1239,7 @@ template <class R> class Sim {
       R *out = &Fp[((size_t)s * nF + l) * 4];
       if (!maccormack) {
         R sp[3], v[4];
-        for (int a = 0; a < 3; a++) sp[a] = dep[a] / P.hf;
+        for (int a = 0; a < 3; a++) sp[a] = dep[a] * P.invHf;
         sampleField(F, sp, v);
         out[0] = v[0];
         out[1] = v[1];
@@ -1277,7 +1291,7 @@ template <class R> class Sim {
       R cell[3], fr[3];
       int bs[3];
       for (int a = 0; a < 3; a++) {
-        cell[a] = (R)id[a] + (fw[a] - pos[a]) / P.hf;
+        cell[a] = (R)id[a] + (fw[a] - pos[a]) * P.invHf;
         R fl = ffloor(cell[a]);
         bs[a] = (int)fl;
         fr[a] = cell[a] - fl;
@@ -1298,7 +1312,7 @@ template <class R> class Sim {
       lifetime = (cl < dn.lmn || cl > dn.lmx) ? adv[2] : cl;
     } else {
       R sp[3], val[4];
-      for (int a = 0; a < 3; a++) sp[a] = fw[a] / P.hf;
+      for (int a = 0; a < 3; a++) sp[a] = fw[a] * P.invHf;
       sampleField(Fp, sp, val);
       rev[0] = val[0];
       rev[1] = val[1];
@@ -1323,7 +1337,7 @@ template <class R> class Sim {
     bool low[3], high[3];
     for (int a = 0; a < 3; a++) {
       R local = (R)(id[a] & (B - 1)) + R(0.5);
-      R pad = fmin(sidef, fmax(R(2), R(2) * ratio) + fabs_(vel[a]) * P.dt / P.hf);
+      R pad = fmin(sidef, fmax(R(2), R(2) * ratio) + fabs_(vel[a]) * P.dt * P.invHf);
       low[a] = local < pad;
       high[a] = local + pad >= sidef;
     }
@@ -1469,7 +1483,7 @@ template <class R> class Sim {
         R covered = 0;
         for (const Region<R> &r : f.regions) {
           R dd[3];
-          for (int a = 0; a < 3; a++) dd[a] = fabs_((p[a] - r.center[a]) / fmax(r.size[a], R(0.03)));
+          for (int a = 0; a < 3; a++) dd[a] = fabs_((p[a] - r.center[a]) * r.invSize[a]);
           R dist = r.kind == R(3) ? fmax(dd[0], fmax(dd[1], dd[2])) : length3(dd[0], dd[1], dd[2]);
           covered = fmax(covered, R(1) - smoothstepv(R(0.5), R(1), dist));
         }
@@ -1528,6 +1542,10 @@ template <class R> class Sim {
           sampleField(F, corner, v);
           for (int k = 0; k < 4; k++) sum[k] += v[k];
         }
+    if (blocks == 1) {
+      for (int k = 0; k < 4; k++) out[k] = sum[k];
+      return;
+    }
     R n = (R)(blocks * blocks * blocks);
     for (int k = 0; k < 4; k++) out[k] = sum[k] / n;
   }
@@ -1559,7 +1577,7 @@ template <class R> class Sim {
               }
           sum += v;
         }
-    return sum / (R)(blocks * blocks * blocks);
+    return blocks == 1 ? sum : sum / (R)(blocks * blocks * blocks);
   }
   // centerForce (fluid-forces.wgsl:79-104).
   void centerForce(const int p[3], R out[3]) const {
@@ -1651,9 +1669,9 @@ template <class R> class Sim {
     forActiveVelocityCells([&](int s, int l, const int p[3]) {
       R h = P.hv;
       const R *vf = &Vf[(size_t)s * nV * 3];
-      R div = (vf[l] - velCell(Vf, 0, p[0] - 1, p[1], p[2])) / h +
-              (vf[nV + l] - velCell(Vf, 1, p[0], p[1] - 1, p[2])) / h +
-              (vf[2 * nV + l] - velCell(Vf, 2, p[0], p[1], p[2] - 1)) / h;
+      R div = (vf[l] - velCell(Vf, 0, p[0] - 1, p[1], p[2])) * P.invHv +
+              (vf[nV + l] - velCell(Vf, 1, p[0], p[1] - 1, p[2])) * P.invHv +
+              (vf[2 * nV + l] - velCell(Vf, 2, p[0], p[1], p[2] - 1)) * P.invHv;
       D[(size_t)s * nV + l] = h * h * (div - expansionAverage(p));
     });
   }
@@ -1995,7 +2013,7 @@ template <class R> class Sim {
         R res;
         if (solidAt(npos)) res = 0;
         else if (!here && !pressureUnknown(nb)) res = 0;
-        else res = value - (velScalar(Pfine, nb[0], nb[1], nb[2]) - cp) / P.hv;
+        else res = value - (velScalar(Pfine, nb[0], nb[1], nb[2]) - cp) * P.invHv;
         o[a * nV + l] = res;
       }
     });

@@ -24,6 +24,8 @@ struct Emitter {
   float flame, heat, smoke, response;
   float vel[3];
   float outward, period, strength, fuel;
+  float invSize[3];  // 1 / max(size, 0.03)
+  float invPeriod;   // 1 / period
 };
 struct Collider {
   float center[3], extent[3];
@@ -32,6 +34,7 @@ struct Collider {
 };
 struct Region {
   float center[3], kind, size[3];
+  float invSize[3];  // 1 / max(size, 0.03)
 };
 struct ForceRec {
   float axis[3], strength, center[3], kind, scale, lift, inward;
@@ -41,6 +44,7 @@ struct ForceRec {
 // Per-step parameters (fluid-common.wgsl Params + FlameParams), by value.
 struct Params {
   float hf, hv, dt;
+  float invHf, invHv;  // 1/hf and 1/hv, rounded once on the host
   float buoyancy, cooling, smokeWeight, damping, dissipation, vorticity, cutoff;
   int lifetimeZero;
   float time, seed;
@@ -115,7 +119,8 @@ __device__ __forceinline__ int entrySlot(int e, int side) {
 // staged in LDS, so lookups within one brick need no hash probe.
 struct Home {
   int bx, by, bz;
-  const int *rec;  // 27 entries (LDS)
+  const int *rec;   // 27 entries (LDS)
+  const int *slot;  // the same 27 bricks' pool slots, -1 without one (LDS)
 };
 
 __device__ __forceinline__ int entryAt(const Home &h, const Dir &d, int bx, int by, int bz) {

@@ -71,7 +71,8 @@ __global__ void k_correct_velocity(StepArgs s);
 __global__ void k_advect_scalars(StepArgs s, int chunks);
 __global__ void k_evolve_fields(StepArgs s, int chunks);
 __global__ void k_curl(StepArgs s);
-__global__ void k_forces(StepArgs s);
+__global__ void k_cell_forces(StepArgs s, float *out);
+__global__ void k_forces(StepArgs s, const float *cellF);
 __global__ void k_divergence(StepArgs s, float *rhs0);
 __global__ void k_project(StepArgs s, const float *pressure);
 
@@ -89,14 +90,19 @@ __global__ void k_mg_prolong_tiles(MGArgs m, int level);
 __global__ void k_mg_bottom(MGArgs m, int iterations);
 
 // Loads the home brick's record into LDS; every lane of the block calls it.
-__device__ __forceinline__ void loadHome(const Dir &dir, int id, int *rec, Home &h) {
-  if (threadIdx.x < 27) rec[threadIdx.x] = dir.records[(size_t)id * RECORD_WORDS + threadIdx.x];
+__device__ __forceinline__ void loadHome(const Dir &dir, int id, int *rec, Home &h, int side) {
+  if (threadIdx.x < 27) {
+    int e = dir.records[(size_t)id * RECORD_WORDS + threadIdx.x];
+    rec[threadIdx.x] = e;
+    rec[27 + threadIdx.x] = entrySlot(e, side);
+  }
   const int *tr = dir.tileRec + (size_t)(id >> 6) * TILE_RECORD_WORDS;
   int l = id & 63;
   h.bx = tr[0] * 4 + (l & 3);
   h.by = tr[1] * 4 + ((l >> 2) & 3);
   h.bz = tr[2] * 4 + (l >> 4);
   h.rec = rec;
+  h.slot = rec + 27;
   __syncthreads();
 }
 

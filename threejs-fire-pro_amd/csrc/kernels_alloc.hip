@@ -11,7 +11,7 @@ constexpr int RANK_BLOCK = 256;
 
 // Whether an emitter's footprint, padded by the stencil and a step of motion, reaches
 // brick b (sourceNear, fluid-allocation.wgsl:305-323).
-__device__ bool sourceNear(const AllocArgs &a, const int b[3]) {
+__device__ __forceinline__ bool sourceNear(const AllocArgs &a, const int b[3]) {
   float size = (float)(1 << a.L.sF) * a.P.hf;
   float lo[3], hi[3];
   for (int k = 0; k < 3; k++) {
@@ -33,7 +33,7 @@ __device__ bool sourceNear(const AllocArgs &a, const int b[3]) {
 }
 
 // Per-lane ranks of three 0/1 flags within a 256-lane block, and the block totals.
-__device__ void blockRanks3(const bool f[3], uint32_t rank[3], uint32_t total[3], uint32_t *lds) {
+__device__ __forceinline__ void blockRanks3(const bool f[3], uint32_t rank[3], uint32_t total[3], uint32_t *lds) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t inwave[3];
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(64) k_link_bricks(AllocArgs a) {
 
 // Colliders into the solid bits of one brick (buildSolids, :485-556): a workgroup per
 // brick, a word of 32 cells per lane and pass.
-__device__ bool insideCollider(const Collider &c, float hf, int x, int y, int z) {
+__device__ __forceinline__ bool insideCollider(const Collider &c, float hf, int x, int y, int z) {
   float w[3] = {((float)x + 0.5f) * hf, ((float)y + 0.5f) * hf, ((float)z + 0.5f) * hf};
   float rel[3] = {w[0] - c.center[0], w[1] - c.center[1], w[2] - c.center[2]};
   float dd[3];

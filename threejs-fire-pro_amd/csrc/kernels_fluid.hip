@@ -12,12 +12,12 @@ namespace fp {
   (void)chunk;                                              \
   if (bi >= s.counts[0]) return;                            \
   const int id = s.active[bi];                              \
-  __shared__ int rec[27];                                   \
+  __shared__ int rec[54];                                   \
   Ctx c;                                                    \
   c.dir = s.dir;                                            \
   c.L = s.L;                                                \
   c.side = s.L.side;                                        \
-  loadHome(s.dir, id, rec, c.home);                         \
+  loadHome(s.dir, id, rec, c.home, s.L.side);               \
   const int slot = entrySlot(rec[13], s.L.side);
 
 __device__ __forceinline__ void velCellCoords(const Ctx &c, int l, int out[3]) {
@@ -36,7 +36,7 @@ __device__ __forceinline__ void fieldCellCoords(const Ctx &c, int l, int out[3])
 // ---- velocity transport -------------------------------------------------------------
 
 // advectVelocityComponent (fluid-velocity-transport.wgsl:128-141).
-__device__ float advectComponent(const Ctx &c, const Params &P, const float *V, const int id[3],
+__device__ __forceinline__ float advectComponent(const Ctx &c, const Params &P, const float *V, const int id[3],
                                  const float pos[3], int comp, const float orig[3]) {
   float off[3] = {0.f, 0.f, 0.f};
   off[comp] = 0.5f * P.hv;
@@ -46,7 +46,7 @@ __device__ float advectComponent(const Ctx &c, const Params &P, const float *V, 
   for (int a = 0; a < 3; a++) mid[a] = fp[a] - 0.5f * P.dt * fv[a];
   velocityAt(c, P, V, mid[0], mid[1], mid[2], v);
   float sp[3];
-  for (int a = 0; a < 3; a++) sp[a] = (fp[a] - P.dt * v[a] - off[a]) / P.hv;
+  for (int a = 0; a < 3; a++) sp[a] = (fp[a] - P.dt * v[a] - off[a]) * P.invHv;
   return sampleVel(c, V, comp, sp[0], sp[1], sp[2]);
 }
 
@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(256) k_advect_velocity(StepArgs s) {
 
 // Donor bounds (fluid-velocity-transport.wgsl:165-212): the component's range over the
 // 2^3 cells from `base`, or [0, 0] when base's brick has no slot.
-__device__ void boundsAt(const Ctx &c, const float *V, const int base[3], int comp, float &lo, float &hi) {
+__device__ __forceinline__ void boundsAt(const Ctx &c, const float *V, const int base[3], int comp, float &lo, float &hi) {
   const int sV = c.L.sV;
   if (slotAtBrick(c, base[0] >> sV, base[1] >> sV, base[2] >> sV) < 0) {
     lo = hi = 0.f;
@@ -88,7 +88,7 @@ __device__ void boundsAt(const Ctx &c, const float *V, const int base[3], int co
 }
 
 // correctVelocityComponent (fluid-velocity-transport.wgsl:214-255).
-__device__ float correctComponent(const Ctx &c, const Params &P, const Pools &p, const int id[3],
+__device__ __forceinline__ float correctComponent(const Ctx &c, const Params &P, const Pools &p, const int id[3],
                                   const float pos[3], int comp, const float orig[3], float predicted) {
   float off[3] = {0.f, 0.f, 0.f};
   off[comp] = 0.5f * P.hv;
@@ -98,14 +98,14 @@ __device__ float correctComponent(const Ctx &c, const Params &P, const Pools &p,
   for (int a = 0; a < 3; a++) mid[a] = fp[a] - 0.5f * P.dt * fv[a];
   velocityAt(c, P, p.V, mid[0], mid[1], mid[2], v);
   int base[3];
-  for (int a = 0; a < 3; a++) base[a] = (int)floorf((fp[a] - P.dt * v[a] - off[a]) / P.hv - 0.5f);
+  for (int a = 0; a < 3; a++) base[a] = (int)floorf((fp[a] - P.dt * v[a] - off[a]) * P.invHv - 0.5f);
   float lo, hi;
   boundsAt(c, p.V, base, comp, lo, hi);
   if (lo == hi && predicted == lo) return predicted;
   float fmid[3], fw[3], sp[3];
   for (int a = 0; a < 3; a++) fmid[a] = fp[a] + 0.5f * P.dt * fv[a];
   velocityAt(c, P, p.V, fmid[0], fmid[1], fmid[2], fw);
-  for (int a = 0; a < 3; a++) sp[a] = (fp[a] + P.dt * fw[a] - off[a]) / P.hv;
+  for (int a = 0; a < 3; a++) sp[a] = (fp[a] + P.dt * fw[a] - off[a]) * P.invHv;
   float reversed = sampleVel(c, p.Vadv, comp, sp[0], sp[1], sp[2]);
   float corrected = predicted + 0.5f * (orig[comp] - reversed);
   return (corrected < lo || corrected > hi) ? predicted : corrected;
@@ -137,7 +137,7 @@ struct Donors {
 };
 
 // predictScalarsWithBounds (fluid-scalar-transport.wgsl:38-62).
-__device__ float4 predict(const Ctx &c, const float4 *F, const int base[3], const float frac[3], Donors &dn) {
+__device__ __forceinline__ float4 predict(const Ctx &c, const float4 *F, const int base[3], const float frac[3], Donors &dn) {
   for (int k = 0; k < 3; k++) {
     dn.mn[k] = 65504.0f;
     dn.mx[k] = -65504.0f;
@@ -183,7 +183,7 @@ __device__ __forceinline__ void traceForward(const Ctx &c, const Params &P, cons
 __device__ __forceinline__ void departureCell(const Params &P, const int id[3], const float pos[3],
                                               const float dep[3], int base[3], float frac[3]) {
   for (int a = 0; a < 3; a++) {
-    float cell = (float)id[a] + (dep[a] - pos[a]) / P.hf;
+    float cell = (float)id[a] + (dep[a] - pos[a]) * P.invHf;
     float fl = floorf(cell);
     base[a] = (int)fl;
     frac[a] = cell - fl;
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) k_advect_scalars(StepArgs s, int chunks) 
     traceBack(c, s.P, s.p.V, pos, cv, dep);
     float4 *out = s.p.Fp + (size_t)slot * nF + l;
     if (!s.P.maccormack) {
-      float4 v = sampleField(c, s.p.F, dep[0] / s.P.hf, dep[1] / s.P.hf, dep[2] / s.P.hf);
+      float4 v = sampleField(c, s.p.F, dep[0] * s.P.invHf, dep[1] * s.P.invHf, dep[2] * s.P.invHf);
       *out = make_float4(v.x, v.y, clampv(v.z, 0.f, 1.f), v.w);
       continue;
     }
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(256) k_advect_scalars(StepArgs s, int chunks) 
 
 // correctedFineScalars (fluid-scalar-transport.wgsl:144-188), with the donor bounds
 // recomputed from the same backward trace instead of cached.
-__device__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p, const int id[3], int slot, int l,
+__device__ __forceinline__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p, const int id[3], int slot, int l,
                                    const float pos[3], const float cv[3]) {
   const float4 adv = p.Fp[(size_t)slot * c.L.nF + l];
   if (!P.maccormack) return adv;
@@ -240,7 +240,7 @@ __device__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p
     int bs[3];
     float fr[3];
     for (int a = 0; a < 3; a++) {
-      float cell = (float)id[a] + (fw[a] - pos[a]) / P.hf;
+      float cell = (float)id[a] + (fw[a] - pos[a]) * P.invHf;
       float fl = floorf(cell);
       bs[a] = (int)fl;
       fr[a] = cell - fl;
@@ -262,7 +262,7 @@ __device__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p
     float cl = adv.z + 0.5f * (orig.z - val.z);
     lifetime = (cl < dn.lmn || cl > dn.lmx) ? adv.z : cl;
   } else {
-    float4 val = sampleField(c, p.Fp, fw[0] / P.hf, fw[1] / P.hf, fw[2] / P.hf);
+    float4 val = sampleField(c, p.Fp, fw[0] * P.invHf, fw[1] * P.invHf, fw[2] * P.invHf);
     rev[0] = val.x;
     rev[1] = val.y;
     rev[2] = val.w;
@@ -279,15 +279,18 @@ __device__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p
 
 // ---- emission and combustion (fluid-sources.wgsl, flame-common.wgsl) -------------------
 
-__device__ float emitterWeight(const Emitter &e, const float w[3]) {
+__device__ __forceinline__ float emitterWeight(const Emitter &e, const float w[3]) {
+  // Past the padded size on any axis |d| >= 1 there, so r >= 1: no weight.
+  for (int a = 0; a < 3; a++)
+    if (fabsv(w[a] - e.pos[a]) >= fmaxv(e.size[a], 0.03f)) return 0.f;
   float d[3];
-  for (int a = 0; a < 3; a++) d[a] = (w[a] - e.pos[a]) / fmaxv(e.size[a], 0.03f);
+  for (int a = 0; a < 3; a++) d[a] = (w[a] - e.pos[a]) * e.invSize[a];
   float r = e.box ? fmaxv(fabsv(d[0]), fmaxv(fabsv(d[1]), fabsv(d[2]))) : length3(d[0], d[1], d[2]);
   if (r >= 1.0f) return 0.f;
   return 1.0f - smoothstepv(0.5f, 1.0f, r);
 }
 
-__device__ float burstGradient(uint32_t seedBits, int cx, int cy, int cz, float ox, float oy, float oz) {
+__device__ __forceinline__ float burstGradient(uint32_t seedBits, int cx, int cy, int cz, float ox, float oy, float oz) {
   uint32_t h = (uint32_t)cx * 374761393u + (uint32_t)cy * 668265263u + (uint32_t)cz * 2246822519u +
                seedBits * 3266489917u;
   h = (h ^ (h >> 13)) * 1274126177u;
@@ -297,7 +300,7 @@ __device__ float burstGradient(uint32_t seedBits, int cx, int cy, int cz, float 
   return ((h & 1u) == 0u ? a : -a) + ((h & 2u) == 0u ? b : -b);
 }
 __device__ __forceinline__ float fade(float f) { return f * f * f * (f * (f * 6.0f - 15.0f) + 10.0f); }
-__device__ float burstPerlin(uint32_t seedBits, float px, float py, float pz) {
+__device__ __forceinline__ float burstPerlin(uint32_t seedBits, float px, float py, float pz) {
   float flx = floorf(px), fly = floorf(py), flz = floorf(pz);
   int cx = (int)flx, cy = (int)fly, cz = (int)flz;
   float fx = px - flx, fy = py - fly, fz = pz - flz;
@@ -312,31 +315,37 @@ __device__ float burstPerlin(uint32_t seedBits, float px, float py, float pz) {
   float j = burstGradient(seedBits, cx + 1, cy + 1, cz + 1, fx - 1.0f, fy - 1.0f, fz - 1.0f);
   return mixv(mixv(mixv(a, b, sx), mixv(cc, d, sx), sy), mixv(mixv(e, g, sx), mixv(h, j, sx), sy), sz);
 }
-__device__ float burstPattern(uint32_t seedBits, float px, float py, float pz) {
+__device__ __forceinline__ float burstPattern(uint32_t seedBits, float px, float py, float pz) {
   float n = (burstPerlin(seedBits, px, py, pz) +
              0.35f * burstPerlin(seedBits, px * 2.0f + 5.2f, py * 2.0f + 9.7f, pz * 2.0f + 3.1f)) /
             1.35f;
   return clampv(0.5f + 0.5f * n, 0.f, 1.f);
 }
-__device__ void burstVariation(uint32_t seedBits, const Emitter &e, const float w[3], float out[2]) {
+__device__ __forceinline__ void burstVariation(uint32_t seedBits, const Emitter &e, const float w[3], float out[2]) {
   if (e.strength == 0.f) {
     out[0] = out[1] = 1.f;
     return;
   }
   const float off[3] = {3.17f, 7.53f, 11.29f};
   float q[3];
-  for (int a = 0; a < 3; a++) q[a] = (w[a] - e.pos[a]) / e.period + off[a];
+  for (int a = 0; a < 3; a++) q[a] = (w[a] - e.pos[a]) * e.invPeriod + off[a];
   float s0 = burstPattern(seedBits, q[0], q[1], q[2]);
   float s1 = burstPattern(seedBits, q[0] + 17.3f, q[1] + 2.8f, q[2] + 8.6f);
   out[0] = 1.0f - e.strength * s0;
   out[1] = 1.0f - e.strength * s1;
 }
 
-__device__ float rampIntegral(const Params &P, int ch, float lo, float hi) {
+// Flame ramps staged in LDS: knots [channel][knot][x, y], then the knot counts.
+struct Ramps {
+  const float *knots;
+  const int *counts;
+};
+__device__ __forceinline__ float rampIntegral(const Ramps &R, int ch, float lo, float hi) {
   float area = 0.f;
-  for (int i = 1; i < P.knotCount[ch]; i++) {
-    float ax = P.knots[ch][i - 1][0], ay = P.knots[ch][i - 1][1];
-    float bx = P.knots[ch][i][0], by = P.knots[ch][i][1];
+  const float *k = R.knots + ch * 16;
+  for (int i = 1; i < R.counts[ch]; i++) {
+    float ax = k[2 * (i - 1)], ay = k[2 * (i - 1) + 1];
+    float bx = k[2 * i], by = k[2 * i + 1];
     float left = fmaxv(lo, ax), right = fminv(hi, bx);
     if (right > left) {
       float y0 = mixv(ay, by, (left - ax) / (bx - ax));
@@ -346,26 +355,27 @@ __device__ float rampIntegral(const Params &P, int ch, float lo, float hi) {
   }
   return area;
 }
-__device__ void evaluateFlame(const Params &P, float before, float &after, float prod[3]) {
+__device__ __forceinline__ void evaluateFlame(const Params &P, const Ramps &R, float before, float &after,
+                                              float prod[3]) {
   if (before <= 0.f) {
     after = 0.f;
     prod[0] = prod[1] = prod[2] = 0.f;
     return;
   }
   after = fmaxv(0.f, before - P.fdt / P.lifespan);
-  prod[0] = P.fsmoke * P.lifespan * rampIntegral(P, 0, after, before);
-  prod[1] = P.fheat * P.lifespan * rampIntegral(P, 1, after, before);
-  prod[2] = P.fexpansion * (P.lifespan * rampIntegral(P, 2, after, before) / P.fdt);
+  prod[0] = P.fsmoke * P.lifespan * rampIntegral(R, 0, after, before);
+  prod[1] = P.fheat * P.lifespan * rampIntegral(R, 1, after, before);
+  prod[2] = P.fexpansion * (P.lifespan * rampIntegral(R, 2, after, before) / P.fdt);
 }
 
 // contentSupport (fluid-content.wgsl:29-39).
-__device__ uint32_t contentSupport(const Layout &L, const Params &P, const int id[3], const float vel[3]) {
+__device__ __forceinline__ uint32_t contentSupport(const Layout &L, const Params &P, const int id[3], const float vel[3]) {
   float sidef = (float)L.B;
   float ratio = P.hv / P.hf;
   bool low[3], high[3];
   for (int a = 0; a < 3; a++) {
     float local = (float)(id[a] & (L.B - 1)) + 0.5f;
-    float pad = fminv(sidef, fmaxv(2.0f, 2.0f * ratio) + fabsv(vel[a]) * P.dt / P.hf);
+    float pad = fminv(sidef, fmaxv(2.0f, 2.0f * ratio) + fabsv(vel[a]) * P.dt * P.invHf);
     low[a] = local < pad;
     high[a] = local + pad >= sidef;
   }
@@ -385,7 +395,26 @@ __global__ void __launch_bounds__(256) k_evolve_fields(StepArgs s, int chunks) {
   __syncthreads();
   const Params &P = s.P;
   const int nF = s.L.nF;
+  __shared__ float knots[48];
+  __shared__ int knotCounts[3];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+      knotCounts[ch] = P.knotCount[ch];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        knots[ch * 16 + 2 * i] = P.knots[ch][i][0];
+        knots[ch * 16 + 2 * i + 1] = P.knots[ch][i][1];
+      }
+    }
+  }
+  __syncthreads();
+  const Ramps ramps{knots, knotCounts};
   uint32_t myBusy = 0u, myExpands = 0u;
+  // Step-uniform factors, evaluated once per lane.
+  const float burnKeep = fexp(-P.burnRate * P.dt);
+  const float lossS = fexp(-P.dissipation * P.dt), lossH = fexp(-P.cooling * P.dt);
+  const float halfS = fexp(-P.dissipation * P.dt * 0.5f), halfH = fexp(-P.cooling * P.dt * 0.5f);
   for (int l = chunk * 256 + threadIdx.x; l < nF; l += chunks * 256) {
     int cc[3];
     fieldCellCoords(c, l, cc);
@@ -415,16 +444,14 @@ __global__ void __launch_bounds__(256) k_evolve_fields(StepArgs s, int chunks) {
         remaining = tr.w + injectedFuel;
         if (tr.y + sp1 >= P.ignition) {
           lifetime = fmaxv(lifetime, fminv(remaining, 1.0f));
-          float unburned = remaining * fexp(-P.burnRate * P.dt);
+          float unburned = remaining * burnKeep;
           burned = remaining - unburned;
           remaining = unburned;
         }
       }
       float after, prod[3];
-      evaluateFlame(P, clampv(lifetime, 0.f, 1.f), after, prod);
+      evaluateFlame(P, ramps, clampv(lifetime, 0.f, 1.f), after, prod);
       expansion = prod[2] + P.fexpansion * burned / P.dt;
-      float lossS = fexp(-P.dissipation * P.dt), lossH = fexp(-P.cooling * P.dt);
-      float halfS = fexp(-P.dissipation * P.dt * 0.5f), halfH = fexp(-P.cooling * P.dt * 0.5f);
       float dens = tr.x * lossS + (sp0 + prod[0]) * halfS;
       float heat = tr.y * lossH + (sp1 + prod[1]) * halfH;
       value[0] = fmaxv(dens, 0.f);
@@ -485,7 +512,7 @@ __global__ void __launch_bounds__(256) k_curl(StepArgs s) {
 }
 
 // curlNoise (fluid-forces.wgsl:3-9): the periodic table, trilinear with wrapping.
-__device__ void curlNoise(const float4 *noise, const float p[3], float out[3]) {
+__device__ __forceinline__ void curlNoise(const float4 *noise, const float p[3], float out[3]) {
   const float c0[3] = {0.36f, -0.8f, 0.48f}, c1[3] = {0.48f, 0.6f, 0.64f}, c2[3] = {-0.8f, 0.0f, 0.6f};
   float u[3];
   for (int a = 0; a < 3; a++) u[a] = (c0[a] * p[0] + c1[a] * p[1] + c2[a] * p[2]) / 16.0f;
@@ -514,7 +541,7 @@ __device__ void curlNoise(const float4 *noise, const float p[3], float out[3]) {
 }
 
 // coverage + externalForces (force-fields.wgsl:312-365).
-__device__ void externalForces(const StepArgs &s, const float p[3], float out[3]) {
+__device__ __forceinline__ void externalForces(const StepArgs &s, const float p[3], float out[3]) {
   const Params &P = s.P;
   out[0] = out[1] = out[2] = 0.f;
   for (int fi = 0; fi < P.forceCount; fi++) {
@@ -525,7 +552,7 @@ __device__ void externalForces(const StepArgs &s, const float p[3], float out[3]
       for (int r = 0; r < f.regionCount; r++) {
         const Region &g = s.regions[f.firstRegion + r];
         float dd[3];
-        for (int a = 0; a < 3; a++) dd[a] = fabsv((p[a] - g.center[a]) / fmaxv(g.size[a], 0.03f));
+        for (int a = 0; a < 3; a++) dd[a] = fabsv((p[a] - g.center[a]) * g.invSize[a]);
         float dist = g.kind == 3.0f ? fmaxv(dd[0], fmaxv(dd[1], dd[2])) : length3(dd[0], dd[1], dd[2]);
         covered = fmaxv(covered, 1.0f - smoothstepv(0.5f, 1.0f, dist));
       }
@@ -567,7 +594,7 @@ __device__ void externalForces(const StepArgs &s, const float p[3], float out[3]
 }
 
 // velocityCellAverage (fluid-sampling.wgsl:420-436) of the fields under velocity cell id.
-__device__ float4 velocityCellAverage(const Ctx &c, const float4 *F, int d, const int id[3]) {
+__device__ __forceinline__ float4 velocityCellAverage(const Ctx &c, const float4 *F, int d, const int id[3]) {
   if (d == 1) return fieldCell(c, F, id[0], id[1], id[2]);
   int blocks = d / 2;
   float ratio = (float)d;
@@ -583,12 +610,13 @@ __device__ float4 velocityCellAverage(const Ctx &c, const float4 *F, int d, cons
         sum.z += v.z;
         sum.w += v.w;
       }
+  if (blocks == 1) return sum;
   float n = (float)(blocks * blocks * blocks);
   return make_float4(sum.x / n, sum.y / n, sum.z / n, sum.w / n);
 }
 
 // centerForce (fluid-forces.wgsl:79-104), zero where the brick does not compute.
-__device__ void cellForce(const Ctx &c, const StepArgs &s, const int p[3], float out[3]) {
+__device__ __forceinline__ void cellForce(const Ctx &c, const StepArgs &s, const int p[3], float out[3]) {
   const Params &P = s.P;
   const int sV = c.L.sV;
   if (entryAt(c.home, c.dir, p[0] >> sV, p[1] >> sV, p[2] >> sV) < 0) {
@@ -621,7 +649,7 @@ __device__ void cellForce(const Ctx &c, const StepArgs &s, const int p[3], float
 }
 
 // sourceVelocity (fluid-forces.wgsl:122-149).
-__device__ float sourceVelocity(const StepArgs &s, float value, const float face[3], int comp) {
+__device__ __forceinline__ float sourceVelocity(const StepArgs &s, float value, const float face[3], int comp) {
   float result = value;
   for (int i = 0; i < s.P.emitterCount; i++) {
     const Emitter &e = s.emitters[i];
@@ -630,82 +658,68 @@ __device__ float sourceVelocity(const StepArgs &s, float value, const float face
     if (wgt == 0.f) continue;
     float radial = 0.f;
     if (e.outward != 0.f) {
-      float dd[3] = {face[0] - e.pos[0], face[1] - e.pos[1], face[2] - e.pos[2]};
-      radial = dd[comp] / fmaxv(length3(dd[0], dd[1], dd[2]), 1e-6f) * e.outward;
+      float d0 = face[0] - e.pos[0], d1 = face[1] - e.pos[1], d2 = face[2] - e.pos[2];
+      float dc = comp == 0 ? d0 : comp == 1 ? d1 : d2;
+      radial = dc / fmaxv(length3(d0, d1, d2), 1e-6f) * e.outward;
     }
-    result = mixv(result, e.vel[comp] + radial, 1.0f - fexp(-e.response * wgt * s.P.dt));
+    float vc = comp == 0 ? e.vel[0] : comp == 1 ? e.vel[1] : e.vel[2];
+    result = mixv(result, vc + radial, 1.0f - fexp(-e.response * wgt * s.P.dt));
   }
   return result;
 }
 
-// applyForces (fluid-forces.wgsl:151-182): forces of the brick and its +1 layer are
-// evaluated once into LDS and averaged onto each cell's three faces.
-constexpr int FORCE_TILE_EDGE = 9;  // velocity bricks up to 8 cells, plus one
-__global__ void __launch_bounds__(256) k_forces(StepArgs s) {
+// The force at each velocity cell of the active bricks (centerForce, fluid-forces.wgsl:
+// 79-118), evaluated once per cell into `out` ([slot][axis][cell]); applyForces reads a
+// cell's and its +x/+y/+z neighbors' (zero where a brick does not compute).
+__global__ void __launch_bounds__(256) k_cell_forces(StepArgs s, float *out) {
+  BRICK_PREAMBLE(1)
+  const int nV = s.L.nV;
+  float *o = out + (size_t)slot * nV * 3;
+  for (int l = threadIdx.x; l < nV; l += blockDim.x) {
+    int p[3];
+    velCellCoords(c, l, p);
+    float f[3];
+    cellForce(c, s, p, f);
+    o[l] = f[0];
+    o[nV + l] = f[1];
+    o[2 * nV + l] = f[2];
+  }
+}
+
+// applyForces (fluid-forces.wgsl:151-182): cell forces averaged onto the faces, damping,
+// emitter velocity coupling and closed solid faces.
+__global__ void __launch_bounds__(256) k_forces(StepArgs s, const float *cellF) {
   BRICK_PREAMBLE(1)
   const Params &P = s.P;
-  const int b = s.L.b, nV = s.L.nV;
-  __shared__ float tile[FORCE_TILE_EDGE * FORCE_TILE_EDGE * FORCE_TILE_EDGE * 3];
-  const bool staged = b + 1 <= FORCE_TILE_EDGE;
-  const int e1 = b + 1;
-  if (staged) {
-    for (int t = threadIdx.x; t < e1 * e1 * e1; t += blockDim.x) {
-      int q[3] = {t % e1, (t / e1) % e1, t / (e1 * e1)};
-      int over = (q[0] == b) + (q[1] == b) + (q[2] == b);
-      if (over >= 2) continue;
-      int p[3] = {c.home.bx * b + q[0], c.home.by * b + q[1], c.home.bz * b + q[2]};
-      float f[3];
-      cellForce(c, s, p, f);
-      tile[t * 3 + 0] = f[0];
-      tile[t * 3 + 1] = f[1];
-      tile[t * 3 + 2] = f[2];
-    }
-  }
-  __syncthreads();
+  const int nV = s.L.nV;
   const float damp = fexp(-P.damping * P.dt);
   const float *vc = s.p.Vcor + (size_t)slot * nV * 3;
+  const float *fh = cellF + (size_t)slot * nV * 3;
   float *o = s.p.Vf + (size_t)slot * nV * 3;
   for (int l = threadIdx.x; l < nV; l += blockDim.x) {
     int p[3];
     velCellCoords(c, l, p);
-    float fc[3], fx[3], fy[3], fz[3];
-    if (staged) {
-      int q[3] = {p[0] - c.home.bx * b, p[1] - c.home.by * b, p[2] - c.home.bz * b};
-      int t = (q[2] * e1 + q[1]) * e1 + q[0];
-      for (int a = 0; a < 3; a++) {
-        fc[a] = tile[t * 3 + a];
-        fx[a] = tile[(t + 1) * 3 + a];
-        fy[a] = tile[(t + e1) * 3 + a];
-        fz[a] = tile[(t + e1 * e1) * 3 + a];
-      }
-    } else {
-      int px[3] = {p[0] + 1, p[1], p[2]}, py[3] = {p[0], p[1] + 1, p[2]}, pz[3] = {p[0], p[1], p[2] + 1};
-      cellForce(c, s, p, fc);
-      cellForce(c, s, px, fx);
-      cellForce(c, s, py, fy);
-      cellForce(c, s, pz, fz);
-    }
-    float force[3] = {0.5f * (fc[0] + fx[0]), 0.5f * (fc[1] + fy[1]), 0.5f * (fc[2] + fz[2])};
+    float force[3] = {0.5f * (fh[l] + velCell(c, cellF, 0, p[0] + 1, p[1], p[2])),
+                      0.5f * (fh[nV + l] + velCell(c, cellF, 1, p[0], p[1] + 1, p[2])),
+                      0.5f * (fh[2 * nV + l] + velCell(c, cellF, 2, p[0], p[1], p[2] + 1))};
     float pos[3] = {((float)p[0] + 0.5f) * P.hv, ((float)p[1] + 0.5f) * P.hv, ((float)p[2] + 0.5f) * P.hv};
     float v[3];
     for (int a = 0; a < 3; a++) v[a] = vc[a * nV + l] * damp + P.dt * force[a];
     float half = 0.5f * P.hv;
-    float out[3];
-    for (int a = 0; a < 3; a++) {
-      float face[3] = {pos[0], pos[1], pos[2]};
-      face[a] = pos[a] + half;
-      out[a] = sourceVelocity(s, v[a], face, a);
-    }
+    float fx[3] = {pos[0] + half, pos[1], pos[2]}, fy[3] = {pos[0], pos[1] + half, pos[2]},
+          fz[3] = {pos[0], pos[1], pos[2] + half};
+    float out0 = sourceVelocity(s, v[0], fx, 0), out1 = sourceVelocity(s, v[1], fy, 1),
+          out2 = sourceVelocity(s, v[2], fz, 2);
     if (solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
       o[l] = o[nV + l] = o[2 * nV + l] = 0.f;
       continue;
     }
-    for (int a = 0; a < 3; a++) {
-      float q[3] = {pos[0], pos[1], pos[2]};
-      q[a] = pos[a] + P.hv;
-      if (solidAt(c, P, s.p.solids, q[0], q[1], q[2])) out[a] = 0.f;
-      o[a * nV + l] = out[a];
-    }
+    if (solidAt(c, P, s.p.solids, pos[0] + P.hv, pos[1], pos[2])) out0 = 0.f;
+    if (solidAt(c, P, s.p.solids, pos[0], pos[1] + P.hv, pos[2])) out1 = 0.f;
+    if (solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2] + P.hv)) out2 = 0.f;
+    o[l] = out0;
+    o[nV + l] = out1;
+    o[2 * nV + l] = out2;
   }
 }
 
@@ -720,9 +734,9 @@ __global__ void __launch_bounds__(256) k_divergence(StepArgs s, float *rhs0) {
     int p[3];
     velCellCoords(c, l, p);
     float h = P.hv;
-    float div = (vf[l] - velCell(c, s.p.Vf, 0, p[0] - 1, p[1], p[2])) / h +
-                (vf[nV + l] - velCell(c, s.p.Vf, 1, p[0], p[1] - 1, p[2])) / h +
-                (vf[2 * nV + l] - velCell(c, s.p.Vf, 2, p[0], p[1], p[2] - 1)) / h;
+    float div = (vf[l] - velCell(c, s.p.Vf, 0, p[0] - 1, p[1], p[2])) * P.invHv +
+                (vf[nV + l] - velCell(c, s.p.Vf, 1, p[0], p[1] - 1, p[2])) * P.invHv +
+                (vf[2 * nV + l] - velCell(c, s.p.Vf, 2, p[0], p[1], p[2] - 1)) * P.invHv;
     float avg;
     if (d == 1) {
       avg = fieldScalar(c, s.p.E, p[0], p[1], p[2]);
@@ -748,14 +762,14 @@ __global__ void __launch_bounds__(256) k_divergence(StepArgs s, float *rhs0) {
                 }
             sum += v;
           }
-      avg = sum / (float)(blocks * blocks * blocks);
+      avg = blocks == 1 ? sum : sum / (float)(blocks * blocks * blocks);
     }
     rhs0[(size_t)slot * nV + l] = h * h * (div - avg);
   }
 }
 
 // project (fluid-forces.wgsl:205-262).
-__device__ bool pressureUnknown(const Ctx &c, const Params &P, const int p[3]) {
+__device__ __forceinline__ bool pressureUnknown(const Ctx &c, const Params &P, const int p[3]) {
   const int sV = c.L.sV;
   int lo[3], hi[3];
   for (int a = 0; a < 3; a++) {
@@ -794,12 +808,12 @@ __global__ void __launch_bounds__(256) k_project(StepArgs s, const float *pressu
       float res;
       if (solidAt(c, P, s.p.solids, npos[0], npos[1], npos[2])) res = 0.f;
       else if (!here && !pressureUnknown(c, P, nb)) res = 0.f;
-      else res = value - (velScalar(c, pressure, nb[0], nb[1], nb[2]) - cp) / P.hv;
+      else res = value - (velScalar(c, pressure, nb[0], nb[1], nb[2]) - cp) * P.invHv;
       o[a * nV + l] = res;
     }
   }
 }
 
-__global__ void __launch_bounds__(256) k_correct_dummy() {}
+
 
 }  // namespace fp

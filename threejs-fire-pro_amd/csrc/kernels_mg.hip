@@ -39,7 +39,7 @@ __device__ __forceinline__ int brickEntry(const MGCtx &m, int bx, int by, int bz
 }
 
 // cellAddress (pressure.wgsl:71-92).
-__device__ long addrAt(const MGCtx &m, const int p[3], int lv) {
+__device__ __forceinline__ long addrAt(const MGCtx &m, const int p[3], int lv) {
   int sh = levelShift(m.L, lv), sd = 1 << sh;
   int ox = p[0] >> sh, oy = p[1] >> sh, oz = p[2] >> sh;
   int owner = lv < m.L.brickLevels ? entrySlot(brickEntry(m, ox, oy, oz), m.side) : tileSlotAt(m, ox, oy, oz);
@@ -49,7 +49,7 @@ __device__ long addrAt(const MGCtx &m, const int p[3], int lv) {
 }
 
 // activeCell (pressure.wgsl:101-132).
-__device__ bool activeCell(const MGCtx &m, const Params &P, const int p[3], int lv) {
+__device__ __forceinline__ bool activeCell(const MGCtx &m, const Params &P, const int p[3], int lv) {
   const Layout &L = m.L;
   if (lv >= L.brickLevels) {
     int n = 1 << (lv - L.brickLevels);
@@ -79,8 +79,8 @@ __device__ bool activeCell(const MGCtx &m, const Params &P, const int p[3], int 
 }
 
 // solidAt for the pressure solver: through the directory, not a fluid home brick.
-__device__ bool solidAtMG(const MGCtx &m, const Params &P, const uint32_t *solids, float wx, float wy, float wz) {
-  int x = (int)floorf(wx / P.hf + 0.25f), y = (int)floorf(wy / P.hf + 0.25f), z = (int)floorf(wz / P.hf + 0.25f);
+__device__ __forceinline__ bool solidAtMG(const MGCtx &m, const Params &P, const uint32_t *solids, float wx, float wy, float wz) {
+  int x = (int)floorf(wx * P.invHf + 0.25f), y = (int)floorf(wy * P.invHf + 0.25f), z = (int)floorf(wz * P.invHf + 0.25f);
   if (P.ground && y < 0) return true;
   if (P.colliders == 0) return false;
   const int sF = m.L.sF;
@@ -90,13 +90,13 @@ __device__ bool solidAtMG(const MGCtx &m, const Params &P, const uint32_t *solid
   uint32_t bit = (uint32_t)(((((z & mk) << sF) | (y & mk)) << sF) | (x & mk));
   return ((solids[(size_t)s * (m.L.nF >> 5) + (bit >> 5)] >> (bit & 31u)) & 1u) != 0u;
 }
-__device__ bool blockedAt(const MGCtx &m, const Params &P, const uint32_t *solids, const int p[3], int lv) {
+__device__ __forceinline__ bool blockedAt(const MGCtx &m, const Params &P, const uint32_t *solids, const int p[3], int lv) {
   float scale = (float)(1 << lv);
   return solidAtMG(m, P, solids, ((float)p[0] + 0.5f) * P.hv * scale, ((float)p[1] + 0.5f) * P.hv * scale,
                    ((float)p[2] + 0.5f) * P.hv * scale);
 }
 // cellTopology (pressure.wgsl:225-235).
-__device__ uint8_t cellTopology(const MGCtx &m, const Params &P, const uint32_t *solids, const int p[3], int lv) {
+__device__ __forceinline__ uint8_t cellTopology(const MGCtx &m, const Params &P, const uint32_t *solids, const int p[3], int lv) {
   if (blockedAt(m, P, solids, p, lv)) return STORED_SOLID;
   uint8_t flags = 0;
   for (int sd = 0; sd < 6; sd++) {
@@ -114,14 +114,14 @@ __device__ __forceinline__ void levelCell(int home0, int home1, int home2, int s
 
 // Home set-up for a block over one active brick.
 #define MG_BRICK_HOME(ID)                          \
-  __shared__ int rec[27];                          \
+  __shared__ int rec[54];                          \
   MGCtx c;                                         \
   c.dir = m.dir;                                   \
   c.L = m.L;                                       \
   c.side = m.L.side;                               \
   c.hasBrick = true;                               \
   c.hasTile = false;                               \
-  loadHome(m.dir, ID, rec, c.home);
+  loadHome(m.dir, ID, rec, c.home, m.L.side);
 
 // Home set-up for a block over one tile slot; returns if the slot is free.
 #define MG_TILE_HOME(T)                                                         \
@@ -221,7 +221,7 @@ __device__ __forceinline__ float diagonal(float w, int flags) {
   return w * openCount(flags, 0) + w * openCount(flags, 2) + w * openCount(flags, 4);
 }
 // stencil (pressure.wgsl:376-402): the neighbor sum over open sides.
-__device__ float stencilSum(const MGCtx &c, const float *pv, const int p[3], int flags, int lv, float w) {
+__device__ __forceinline__ float stencilSum(const MGCtx &c, const float *pv, const int p[3], int flags, int lv, float w) {
   float n[6];
   for (int sd = 0; sd < 6; sd++) {
     if (!((flags >> sd) & 1)) {
@@ -235,7 +235,7 @@ __device__ float stencilSum(const MGCtx &c, const float *pv, const int p[3], int
 }
 
 // relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8.
-__device__ void relaxCell(const MGCtx &c, const Level &lv, int level, const int p[3], size_t a, float w) {
+__device__ __forceinline__ void relaxCell(const MGCtx &c, const Level &lv, int level, const int p[3], size_t a, float w) {
   int e = lv.eflag[a];
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
     lv.q[a] = 0.f;
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(64) k_mg_relax_tiles(MGArgs m, int level) {
 
 // restrictCell (pressure.wgsl:601-647): the coarse right-hand side is half the sum of the
 // eight children's residuals; the coarse first iterate is one Jacobi step from zero.
-__device__ void restrictCell(const MGCtx &c, const MGArgs &m, int level, const int id[3]) {
+__device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgs &m, int level, const int id[3]) {
   const Level &fine = m.lv[level - 1], &coarse = m.lv[level];
   float residual = 0.f;
   bool every = true;
@@ -344,12 +344,12 @@ __global__ void __launch_bounds__(64) k_mg_restrict_tiles(MGArgs m, int level) {
 }
 
 // coarseValue (pressure.wgsl:670-691): Neumann ghosts beside solids, a mirror below the floor.
-__device__ bool coarseSolid(const MGCtx &c, const Level &lv, const int q[3], int level) {
+__device__ __forceinline__ bool coarseSolid(const MGCtx &c, const Level &lv, const int q[3], int level) {
   long a = addrAt(c, q, level);
   if (a < 0) return false;
   return (lv.topo[a] & STORED_SOLID) != 0;
 }
-__device__ float coarseValue(const MGCtx &c, const MGArgs &m, const int p[3], int level) {
+__device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgs &m, const int p[3], int level) {
   const Level &cv = m.lv[level];
   const int groundY = m.P.ground ? 0 : (int)0x80000000;
   int cc[3] = {p[0], p[1] > groundY ? p[1] : groundY, p[2]};
@@ -369,7 +369,7 @@ __device__ float coarseValue(const MGCtx &c, const MGArgs &m, const int p[3], in
 }
 
 // prolongateCell (pressure.wgsl:726-755): trilinear correction from the coarse level.
-__device__ void prolongCell(const MGCtx &c, const MGArgs &m, int level, const int p[3], size_t a) {
+__device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgs &m, int level, const int p[3], size_t a) {
   const Level &lv = m.lv[level];
   int e = lv.eflag[a];
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {

@@ -615,6 +615,8 @@ struct fp_sim {
       e.period = (float)(s.has_variation ? s.variation_period : 0.5);
       e.strength = (float)(s.has_variation ? s.variation_strength : 0.0);
       e.fuel = (float)s.fuel_rate;
+      for (int a = 0; a < 3; a++) e.invSize[a] = 1.0f / std::max(e.size[a], 0.03f);
+      e.invPeriod = 1.0f / e.period;
       hostEmitters.push_back(e);
     }
     int n = f.collider_count;
@@ -670,6 +672,7 @@ struct fp_sim {
           g.size[a] = (float)s.size[a];
         }
         g.kind = s.shape == FP_SHAPE_BOX ? 3.0f : 1.0f;
+        for (int a = 0; a < 3; a++) g.invSize[a] = 1.0f / std::max(g.size[a], 0.03f);
         hostRegions.push_back(g);
       }
       hostForces.push_back(r);
@@ -687,6 +690,8 @@ struct fp_sim {
     const fp_settings &s = f.simulation;
     P.hf = (float)voxel;
     P.hv = (float)velocitySpacing();
+    P.invHf = 1.0f / P.hf;
+    P.invHv = 1.0f / P.hv;
     P.dt = (float)dt;
     P.buoyancy = (float)s.buoyancy;
     P.cooling = (float)s.cooling;
@@ -874,7 +879,9 @@ struct fp_sim {
     s.p.Fn = Fn.p;
     int m5 = begin();
     if (P.vorticity != 0.f) k_curl<<<slotsGrid, 256, 0, stream>>>(s);
-    k_forces<<<slotsGrid, 256, 0, stream>>>(s);
+    // The velocity predictor is dead after correction: it holds the cell forces.
+    k_cell_forces<<<slotsGrid, 256, 0, stream>>>(s, Vadv.p);
+    k_forces<<<slotsGrid, 256, 0, stream>>>(s, Vadv.p);
     k_divergence<<<slotsGrid, 256, 0, stream>>>(s, D.p);
     end(m5, "forces_divergence");
     int m6 = begin();

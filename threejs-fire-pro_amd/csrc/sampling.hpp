@@ -25,7 +25,10 @@ struct Ctx {
 };
 
 __device__ __forceinline__ int slotAtBrick(const Ctx &c, int bx, int by, int bz) {
-  return entrySlot(entryAt(c.home, c.dir, bx, by, bz), c.side);
+  int dx = bx - c.home.bx, dy = by - c.home.by, dz = bz - c.home.bz;
+  if ((unsigned)(dx + 1) <= 2u && (unsigned)(dy + 1) <= 2u && (unsigned)(dz + 1) <= 2u)
+    return c.home.slot[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+  return entrySlot(entryLookup(c.dir, bx, by, bz), c.side);
 }
 
 __device__ __forceinline__ float4 fieldCell(const Ctx &c, const float4 *buf, int x, int y, int z) {
@@ -72,7 +75,8 @@ __device__ __forceinline__ bool solidCell(const Ctx &c, const Params &P, const u
 }
 __device__ __forceinline__ bool solidAt(const Ctx &c, const Params &P, const uint32_t *solids, float wx,
                                         float wy, float wz) {
-  int x = (int)floorf(wx / P.hf + 0.25f), y = (int)floorf(wy / P.hf + 0.25f), z = (int)floorf(wz / P.hf + 0.25f);
+  int x = (int)floorf(wx * P.invHf + 0.25f), y = (int)floorf(wy * P.invHf + 0.25f),
+      z = (int)floorf(wz * P.invHf + 0.25f);
   return solidCell(c, P, solids, x, y, z);
 }
 
@@ -121,9 +125,10 @@ __device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int c
 // velocityAt (fluid-advection.wgsl:10-21).
 __device__ __forceinline__ void velocityAt(const Ctx &c, const Params &P, const float *V, float wx, float wy,
                                            float wz, float out[3]) {
-  out[0] = sampleVel(c, V, 0, (wx - 0.5f * P.hv) / P.hv, wy / P.hv, wz / P.hv);
-  out[1] = sampleVel(c, V, 1, wx / P.hv, (wy - 0.5f * P.hv) / P.hv, wz / P.hv);
-  out[2] = sampleVel(c, V, 2, wx / P.hv, wy / P.hv, (wz - 0.5f * P.hv) / P.hv);
+  const float cx = wx * P.invHv, cy = wy * P.invHv, cz = wz * P.invHv;
+  out[0] = sampleVel(c, V, 0, (wx - 0.5f * P.hv) * P.invHv, cy, cz);
+  out[1] = sampleVel(c, V, 1, cx, (wy - 0.5f * P.hv) * P.invHv, cz);
+  out[2] = sampleVel(c, V, 2, cx, cy, (wz - 0.5f * P.hv) * P.invHv);
 }
 // velocityOnFace (fluid-advection.wgsl:26-44).
 __device__ __forceinline__ void velocityOnFace(const Ctx &c, const float *V, const int id[3], int comp,
