@@ -138,23 +138,52 @@ typedef struct fp_tables {
   int32_t pool_slots;
 } fp_tables;
 
-/* Render parameters for the headless ray-march (src/library/SceneVolumeRenderer.ts
- * packing, volume-march.wgsl). Matrices are column-major like three.js Matrix4. */
+/* The camera of the headless ray-march (scene-volume.wgsl fireProSceneVolume inputs).
+ * inverse_view_projection maps clip coordinates to world space, column-major like
+ * three.js Matrix4 (camera.matrixWorld x projectionMatrixInverse,
+ * SceneVolumeRenderer.ts:452). Pixel (x, y) has uv ((x+.5)/width, (y+.5)/height), row 0
+ * at the top. `position` is kept for callers that light a scene from the viewer. */
 typedef struct fp_camera {
   double inverse_view_projection[16];
   double position[3];
   int32_t width, height;
 } fp_camera;
 
+/* RenderSettings (src/engine/types.ts) and RenderingOptions (src/library/options.ts),
+ * with the scene's lights as SceneVolumeRenderer.readSceneLighting gathers them
+ * (SceneVolumeRenderer.ts:298-325). Colors are linear RGB. */
 typedef struct fp_render_settings {
   double fire_intensity, flame_opacity, smoke_density, scattering, shadow_density;
   double temperature, soot_glow;
-  double smoke_color[3], fire_color[3]; /* linear RGB */
-  double sun_direction[3], sun_intensity;
-  int32_t filter;       /* 0 trilinear, 1 quadratic, 2 cubic */
-  int32_t ray_steps;
-  int32_t lighting_divisor;
+  double smoke_color[3], fire_color[3];
+  double sun_direction[3]; /* from the scene toward the directional light */
+  double sun_intensity;    /* 0: no directional light, no shadows */
+  double sun_color[3];     /* the directional light's color */
+  double ambient[3];       /* sum of ambient light color x intensity */
+  int32_t filter;          /* 0 trilinear, 1 quadratic, 2 cubic B-spline */
+  int32_t ray_steps;       /* samples a ray takes at most across the boxes */
+  int32_t lighting_divisor;/* 1, 2 or 4 field cells per lighting texel */
 } fp_render_settings;
+
+/* The render preparation of the latest render, for parity and inspection: the boxes
+ * (8 words each: first brick xyz, first page-table entry, bricks per axis xyz, 0), the
+ * page table, and for every listed brick (an active brick with a slot inside a box,
+ * ascending id) its render field ((B+1)^3 float4 with apron), incident light
+ * ((B/lighting_divisor+1)^3 float4 with apron) and block flags ((B/8)^3 bytes: bit 0
+ * flame, bit 1 visible); and the 512-texel blackbody table. NULL buffers are skipped;
+ * each *_count is what exists, written up to the matching capacity. */
+typedef struct fp_render_tables {
+  int32_t *boxes;
+  int32_t *page_table;
+  int32_t *bricks;
+  float *field;
+  float *light;
+  uint8_t *occupancy;
+  float *blackbody;
+  int32_t box_capacity, page_capacity, brick_capacity;
+  int32_t box_count, page_count, brick_count;
+  int32_t light_work; /* 1 every listed brick lit, 2 only slots given this step, 0 none, -1 idle */
+} fp_render_tables;
 
 typedef struct fp_sim fp_sim;
 
@@ -193,6 +222,18 @@ void *fp_sim_stream(fp_sim *sim);
 /* Per-stage event timing of the next steps: enable, then read averages (ms). */
 int fp_sim_profile(fp_sim *sim, int32_t enable);
 int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *count);
+
+/* The ray-march of the simulation's latest step (a12, a13): brings the render
+ * preparation up to date on VolumeLighting.encode's schedule (VolumeLighting.ts:249-375)
+ * and marches every pixel (scene-volume.wgsl:38-48, :146-167; volume-march.wgsl). Writes
+ * width x height premultiplied (radiance rgb, 1 - transmittance) floats, row-major, to host
+ * memory. Replaces SceneVolumeRenderer.prepare + the NodeMaterial's fragment pass. */
+int fp_sim_render(fp_sim *sim, const fp_camera *camera, const fp_render_settings *settings, float *rgba);
+/* The same into device memory (hipMalloc'ed, width*height*4 floats), enqueued on the
+ * simulation's stream without waiting. */
+int fp_sim_render_device(fp_sim *sim, const fp_camera *camera, const fp_render_settings *settings,
+                         float *rgba_device);
+int fp_sim_read_render(fp_sim *sim, fp_render_tables *out);
 
 #ifdef __cplusplus
 }

@@ -2,8 +2,10 @@
 // loaded with ctypes by the tests, smoke() and bench.py's CPU baseline. Entry points mirror
 // include/firepro.h's, prefixed orc_, with the arithmetic width chosen at creation.
 #include <chrono>
+#include <type_traits>
 #include <string>
 
+#include "render.hpp"
 #include "solver.hpp"
 
 namespace {
@@ -26,13 +28,19 @@ struct Base {
   virtual int tileTables(int32_t *records, int32_t *hash, int cap, int hashCap) = 0;
   virtual int brickCoords(const int32_t *ids, int n, int32_t *out) = 0;
   virtual int rhs(double *out, int cap) = 0;
+  virtual int render(const fp_camera &cam, const fp_render_settings &rs, float *rgba) = 0;
+  virtual int readRender(fp_render_tables *t) = 0;
+  virtual int probe(const fp_camera &cam, const fp_render_settings &rs, int what, const float *in, float *out) = 0;
 };
 
 template <class R> struct Impl : Base {
   oracle::Sim<R> sim;
   explicit Impl(const fp_grid &g) : sim(g) {}
   void step(const fp_frame &f, double dt) override { sim.step(f, dt); }
-  void reset() override { sim.reset(); }
+  void reset() override {
+    sim.reset();
+    cache.invalidate();
+  }
   void stats(fp_stats *s) override {
     s->time = sim.time;
     s->steps = sim.steps;
@@ -154,6 +162,94 @@ template <class R> struct Impl : Base {
   void readActivity(uint32_t *flags, int n) override {
     for (int i = 0; i < n && i < (int)sim.activity.size(); i++) flags[i] = sim.activity[i];
   }
+
+  // ---- render (single precision only) ----
+  oracle::RenderCache cache;
+  int lastWork = -1;
+  void checkRender(const fp_render_settings &rs) {
+    if constexpr (!std::is_same<R, float>::value) throw std::runtime_error("the render oracle is single precision");
+    int ld = rs.lighting_divisor;
+    if (ld != 1 && ld != 2 && ld != 4) throw oracle::Error("lightingDivisor must be 1, 2 or 4.");
+    if (sim.B / ld < 1) throw oracle::Error("lightingDivisor exceeds the brick size.");
+  }
+  int render(const fp_camera &cam, const fp_render_settings &rs, float *rgba) override {
+    checkRender(rs);
+    if constexpr (std::is_same<R, float>::value) {
+      auto p = oracle::renderParams(sim.voxel, sim.B, cam, rs, sim.time);
+      lastWork = cache.prepare(sim, p);
+      if (rgba) cache.view(sim, p).image(rgba);
+    }
+    return 0;
+  }
+  int readRender(fp_render_tables *t) override {
+    if constexpr (std::is_same<R, float>::value) {
+      t->light_work = lastWork;
+      t->box_count = (int32_t)cache.boxes.size();
+      if (t->boxes)
+        for (int i = 0; i < std::min(t->box_count, t->box_capacity); i++) {
+          const auto &b = cache.boxes[i];
+          int32_t w[8] = {b.origin[0], b.origin[1], b.origin[2], b.first, b.size[0], b.size[1], b.size[2], 0};
+          std::copy(w, w + 8, t->boxes + 8 * i);
+        }
+      int pages = 0;
+      for (const auto &b : cache.boxes) pages += b.size[0] * b.size[1] * b.size[2];
+      t->page_count = pages;
+      if (t->page_table)
+        for (int i = 0; i < std::min(pages, t->page_capacity); i++) t->page_table[i] = cache.table[i];
+      int E = sim.B + 1, LE = (1 << (sim.sF - (cache.lightDivisor == 4 ? 2 : cache.lightDivisor == 2 ? 1 : 0))) + 1;
+      int nO = (sim.B / 8) * (sim.B / 8) * (sim.B / 8);
+      size_t nR = (size_t)E * E * E * 4, nL = (size_t)LE * LE * LE * 4;
+      oracle::RenderParams p{};
+      p.lightShift = sim.sF;
+      p.lightDivisor = 1;
+      auto view = cache.view(sim, p);
+      int n = 0;
+      for (int id : sim.activeList) {
+        int page, bi, brick[3];
+        if (!view.slotted(id, page, bi, brick)) continue;
+        if (n < t->brick_capacity) {
+          int s = sim.pageSlot(page);
+          if (t->bricks) t->bricks[n] = id;
+          if (t->field) std::copy(&cache.R[s * nR], &cache.R[s * nR] + nR, t->field + n * nR);
+          if (t->light) std::copy(&cache.L[s * nL], &cache.L[s * nL] + nL, t->light + n * nL);
+          if (t->occupancy) std::copy(&cache.O[(size_t)s * nO], &cache.O[(size_t)s * nO] + nO, t->occupancy + (size_t)n * nO);
+        }
+        n++;
+      }
+      t->brick_count = n;
+      if (t->blackbody) std::copy(cache.lut.begin(), cache.lut.end(), t->blackbody);
+      return 0;
+    }
+    return -1;
+  }
+  // Single functions of the render restatement on the latest prepared state, for tests:
+  // 0 reconstruct(cell xyz, mode) in box 0; 1 incidentLight(world xyz) in box 0;
+  // 2 the jitter of pixel (x, y) at time t; 3 fexp2(x).
+  int probe(const fp_camera &cam, const fp_render_settings &rs, int what, const float *in, float *out) override {
+    if constexpr (std::is_same<R, float>::value) {
+      auto p = oracle::renderParams(sim.voxel, sim.B, cam, rs, sim.time);
+      auto view = cache.view(sim, p);
+      if (what == 2) {
+        out[0] = oracle::Renderer::jitter((uint32_t)in[0], (uint32_t)in[1], in[2]);
+        return 0;
+      }
+      if (what == 3) {
+        out[0] = oracle::fexp2(in[0]);
+        return 0;
+      }
+      if (cache.boxes.empty()) return -1;
+      oracle::Vec4 v{};
+      if (what == 0) v = view.reconstruct(cache.boxes[0], in, in[3]);
+      else if (what == 1) v = view.incidentLight(cache.boxes[0], in);
+      else return -1;
+      out[0] = v.x;
+      out[1] = v.y;
+      out[2] = v.z;
+      out[3] = v.w;
+      return 0;
+    }
+    return -1;
+  }
 };
 
 template <class F> int guard(F fn) {
@@ -216,6 +312,20 @@ int orc_tile_tables(void *h, int32_t *records, int32_t *hash, int cap, int hashC
 }
 int orc_brick_coords(void *h, const int32_t *ids, int n, int32_t *out) { return ((Base *)h)->brickCoords(ids, n, out); }
 int orc_rhs(void *h, double *out, int cap) { return ((Base *)h)->rhs(out, cap); }
+int orc_render(void *h, const fp_camera *cam, const fp_render_settings *rs, float *rgba) {
+  return guard([&] { ((Base *)h)->render(*cam, *rs, rgba); });
+}
+int orc_read_render(void *h, fp_render_tables *t) { return ((Base *)h)->readRender(t); }
+int orc_render_probe(void *h, const fp_camera *cam, const fp_render_settings *rs, int what, const float *in,
+                     float *out) {
+  int r = -1;
+  int s = guard([&] { r = ((Base *)h)->probe(*cam, *rs, what, in, out); });
+  return s != FP_OK ? s : r;
+}
+void orc_blackbody(float *out) {
+  auto t = oracle::blackbodyTable();
+  std::copy(t.begin(), t.end(), out);
+}
 
 // Closed-form hooks: single functions of the restatement, evaluated in double.
 static oracle::Params<double> hookParams(const double *knots, const int *counts, double dt, double lifespan,

@@ -48,6 +48,25 @@ inline float fexp(float x) {
 }
 inline double fexp(double x) { return std::exp(x); }
 
+// 2^x in single precision for -126 <= x <= 127: k = the nearest integer, r = (x - k) ln2
+// (x - k is exact, |r| <= ln2/2), the polynomial of fexp for e^r, and an exact 2^k.
+inline float fexp2(float x) {
+  float k = std::floor(x + 0.5f);
+  float r = (x - k) * 0.693147182f;
+  float p = 1.98412701e-04f;
+  p = p * r + 1.38888892e-03f;
+  p = p * r + 8.33333377e-03f;
+  p = p * r + 4.16666679e-02f;
+  p = p * r + 1.66666672e-01f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  uint32_t bits = (uint32_t)((int)k + 127) << 23;
+  float s;
+  std::memcpy(&s, &bits, 4);
+  return p * s;
+}
+
 inline float fsqrt(float x) { return std::sqrt(x); }
 inline double fsqrt(double x) { return std::sqrt(x); }
 template <class R> inline R ffloor(R x) { return std::floor(x); }

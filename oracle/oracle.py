@@ -58,6 +58,11 @@ def load():
         "orc_tile_tables": (C.c_int, [v, P(C.c_int32), P(C.c_int32), C.c_int, C.c_int]),
         "orc_brick_coords": (C.c_int, [v, P(C.c_int32), C.c_int, P(C.c_int32)]),
         "orc_rhs": (C.c_int, [v, P(C.c_double), C.c_int]),
+        "orc_render": (C.c_int, [v, P(abi.Camera), P(abi.RenderSettings), P(C.c_float)]),
+        "orc_read_render": (C.c_int, [v, P(abi.RenderTables)]),
+        "orc_render_probe": (C.c_int, [v, P(abi.Camera), P(abi.RenderSettings), C.c_int,
+                                       P(C.c_float), P(C.c_float)]),
+        "orc_blackbody": (None, [P(C.c_float)]),
         "orc_ramp_integral": (C.c_double, [P(C.c_double), P(C.c_int), C.c_int, C.c_double, C.c_double]),
         "orc_evaluate_flame": (None, [P(C.c_double), P(C.c_int)] + [C.c_double] * 6 + [P(C.c_double)]),
         "orc_emitter_weight": (C.c_double, [P(C.c_double), P(C.c_double), C.c_int, P(C.c_double)]),
@@ -201,6 +206,33 @@ class OracleSimulation:
         c = self._lib.orc_rhs(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), n)
         return out[:c]
 
+    def render(self, camera, settings=None, image=True):
+        rs = settings if settings is not None else abi.render_settings()
+        img = np.zeros((camera.height, camera.width, 4), np.float32)
+        _check(self._lib.orc_render(self._h, C.byref(camera), C.byref(rs),
+                                    img.ctypes.data_as(C.POINTER(C.c_float)) if image else None), self._lib)
+        self._render = (camera, rs)
+        return img
+
+    def read_render(self) -> dict:
+        def call(t):
+            if self._lib.orc_read_render(self._h, C.byref(t)) != 0:
+                raise RuntimeError("render tables need a single-precision oracle")
+        return abi.read_render_tables(call, self.brick_size, self._render[1].lighting_divisor)
+
+    def probe(self, what: int, values) -> np.ndarray:
+        """A single function of the render restatement on the latest render's state:
+        0 reconstruct(cell x, y, z, mode) in box 0; 1 incidentLight(world x, y, z) in box 0;
+        2 jitter(pixel x, y, time); 3 fexp2(x)."""
+        cam, rs = getattr(self, "_render", (abi.camera((0, 0, 1), (0, 0, 0), 45, 1, 1),
+                                            abi.render_settings()))
+        inp = (C.c_float * 4)(*[float(v) for v in values] + [0.0] * (4 - len(values)))
+        out = (C.c_float * 4)()
+        r = self._lib.orc_render_probe(self._h, C.byref(cam), C.byref(rs), what, inp, out)
+        if r != 0:
+            raise RuntimeError(f"probe {what} failed")
+        return np.array(out[:], np.float32)
+
     def brick_coords(self, ids) -> np.ndarray:
         ids = np.ascontiguousarray(ids, np.int32)
         out = np.zeros((len(ids), 3), np.int32)
@@ -244,3 +276,9 @@ def burst_perlin(seed_bits, x, y, z):
 
 def fexp(x: float) -> float:
     return load().orc_fexp(x)
+
+
+def blackbody() -> np.ndarray:
+    out = np.zeros((512, 4), np.float32)
+    load().orc_blackbody(out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out

@@ -286,6 +286,17 @@ template <class R> class Sim {
   std::vector<std::pair<std::array<int, 3>, uint32_t>> contentTiles;
   int64_t contentStep = -1;
   Directory dir;
+  // The boxes the renderer marches (FluidSimulation.ts:163-174, :296-305): lo and hi
+  // bricks per cluster with something to draw, the clusters they box, every tile slot's
+  // coordinates with its box index (or -1), and a version that changes with either.
+  struct BrickBox {
+    int lo[3], hi[3];
+  };
+  std::vector<BrickBox> renderBoxes;
+  std::vector<int> renderClusters;
+  std::vector<int32_t> renderTiles;
+  int renderVersion = 0;
+  bool renderStale = true;
   int capacity = 0;
   std::vector<float> noise;
 
@@ -355,6 +366,7 @@ template <class R> class Sim {
     capacity = 0;
     createTiles(MIN_TILES);
     applyTiles();
+    updateBoxes({});
   }
 
   void resizePools(int slots, bool keep) {
@@ -440,6 +452,7 @@ template <class R> class Sim {
       }
     }
     geometryDirty = true;  // setBricks resets the solver's plans (PressureSolver.ts:120-134)
+    renderStale = true;    // FluidSimulation.ts:708-709
   }
 
   // ---- per-brick lookups (tiles.wgsl:75-181) -----------------------------------------
@@ -652,11 +665,12 @@ template <class R> class Sim {
     dir.need(l, h, steps);
   }
 
-  // updateTiles (FluidSimulation.ts:873-912), without the render boxes.
+  // updateTiles (FluidSimulation.ts:873-912).
   void updateTiles(const fp_frame &f) {
     dir.retire(steps);
     double reach = 2 * B * voxel;
     double margin = reach + tileSize();
+    std::vector<BrickBox> sourceBricks;
     for (int i = 0; i < f.source_count; i++) {
       const fp_source &s = f.sources[i];
       double extent = std::max(std::max(std::max(s.size[0], s.size[1]), s.size[2]), 0.03) +
@@ -667,6 +681,14 @@ template <class R> class Sim {
         hi[a] = s.position[a] + extent;
       }
       needBox(lo, hi, margin);
+      // bricksOver (FluidSimulation.ts:862-869)
+      BrickBox bb;
+      double size = B * voxel;
+      for (int a = 0; a < 3; a++) {
+        bb.lo[a] = (int)std::floor(lo[a] / size);
+        bb.hi[a] = (int)std::floor(hi[a] / size);
+      }
+      sourceBricks.push_back(bb);
     }
     for (auto &ct : contentTiles) {
       int lo[3], hi[3];
@@ -677,6 +699,82 @@ template <class R> class Sim {
       dir.need(lo, hi, steps);
     }
     if (dir.dirty) applyTiles();
+    updateBoxes(sourceBricks);
+  }
+
+  static int floorDiv(int a, int b) { return (int)std::floor((double)a / b); }
+
+  // updateBoxes (FluidSimulation.ts:913-969): per cluster of tiles, the union of the
+  // bricks holding content at the latest measurement and of `extra`, grown by a margin
+  // of two bricks inside the cluster's tiles; clusters in ascending order.
+  void updateBoxes(const std::vector<BrickBox> &extra) {
+    std::vector<int> order;
+    std::vector<BrickBox> found;
+    auto include = [&](const BrickBox &box) {
+      int center[3];
+      for (int a = 0; a < 3; a++) center[a] = floorDiv(box.lo[a] + box.hi[a], 2);
+      if (ground) center[1] = std::max(0, center[1]);
+      int slot = dir.slotAt(floorDiv(center[0], TILE_BRICKS), floorDiv(center[1], TILE_BRICKS),
+                            floorDiv(center[2], TILE_BRICKS));
+      if (slot < 0) return;
+      int cluster = tables.tiles[(size_t)slot * 4 + 3];
+      for (size_t i = 0; i < order.size(); i++)
+        if (order[i] == cluster) {
+          for (int a = 0; a < 3; a++) {
+            found[i].lo[a] = std::min(found[i].lo[a], box.lo[a]);
+            found[i].hi[a] = std::max(found[i].hi[a], box.hi[a]);
+          }
+          return;
+        }
+      order.push_back(cluster);
+      found.push_back(box);
+    };
+    for (auto &ct : contentTiles) {
+      BrickBox box;
+      bool any = true;
+      for (int a = 0; a < 3; a++) {
+        uint32_t used = (ct.second >> (a * 4)) & 15u;
+        if (!used) any = false;
+        int lowest = 0, highest = 0;
+        for (int k = 0; k < 4; k++)
+          if (used >> k & 1u) {
+            if (!(used & ((1u << k) - 1u))) lowest = k;
+            highest = k;
+          }
+        box.lo[a] = ct.first[a] * TILE_BRICKS + lowest;
+        box.hi[a] = ct.first[a] * TILE_BRICKS + highest;
+      }
+      if (any) include(box);
+    }
+    for (auto &b : extra) include(b);
+    std::vector<int> clusters = order;
+    std::sort(clusters.begin(), clusters.end());
+    std::vector<BrickBox> boxes;
+    for (int c : clusters) {
+      size_t i = std::find(order.begin(), order.end(), c) - order.begin();
+      const TileBox &tiles = tables.clusters[c];
+      BrickBox out;
+      for (int a = 0; a < 3; a++) {
+        out.lo[a] = std::max(found[i].lo[a] - 2, tiles.lo[a] * TILE_BRICKS);
+        out.hi[a] = std::min(found[i].hi[a] + 2, (tiles.hi[a] + 1) * TILE_BRICKS - 1);
+      }
+      boxes.push_back(out);
+    }
+    bool same = clusters == renderClusters && boxes.size() == renderBoxes.size();
+    for (size_t i = 0; same && i < boxes.size(); i++)
+      for (int a = 0; a < 3; a++)
+        if (boxes[i].lo[a] != renderBoxes[i].lo[a] || boxes[i].hi[a] != renderBoxes[i].hi[a]) same = false;
+    if (same && !renderStale) return;
+    renderStale = false;
+    renderBoxes = boxes;
+    renderClusters = clusters;
+    renderTiles = tables.tiles;
+    for (size_t w = 3; w < renderTiles.size(); w += 4)
+      if (renderTiles[w] >= 0) {
+        auto it = std::find(clusters.begin(), clusters.end(), renderTiles[w]);
+        renderTiles[w] = it == clusters.end() ? -1 : (int)(it - clusters.begin());
+      }
+    renderVersion++;
   }
 
   // growPool (FluidSimulation.ts:1013-1093).

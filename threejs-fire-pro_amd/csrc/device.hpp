@@ -170,4 +170,20 @@ __device__ __forceinline__ float fexp(float x) {
   return (p * __int_as_float((ki + 64 + 127) << 23)) * __int_as_float((127 - 64) << 23);
 }
 
+// 2^x in single precision for -126 <= x <= 127: k = the nearest integer, r = (x - k) ln2
+// (x - k is exact, |r| <= ln2/2), the polynomial of fexp for e^r, and an exact 2^k.
+__device__ __forceinline__ float fexp2(float x) {
+  float k = floorf(x + 0.5f);
+  float r = (x - k) * 0.693147182f;
+  float p = 1.98412701e-04f;
+  p = p * r + 1.38888892e-03f;
+  p = p * r + 8.33333377e-03f;
+  p = p * r + 4.16666679e-02f;
+  p = p * r + 1.66666672e-01f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  return p * __int_as_float(((int)k + 127) << 23);
+}
+
 }  // namespace fp

@@ -89,6 +89,51 @@ __global__ void k_mg_prolong_bricks(MGArgs m, int level);
 __global__ void k_mg_prolong_tiles(MGArgs m, int level);
 __global__ void k_mg_bottom(MGArgs m, int iterations);
 
+// The render's parameters (volume-common.wgsl RenderParams, packed at
+// SceneVolumeRenderer.ts:329-345, and the camera uniforms of :440-456).
+struct RenderParams {
+  float voxel, brickSize, texel;
+  int shift, lightShift, lightDivisor;
+  float flameOpacity;
+  float optical[4];  // smoke density, fire intensity, shadow density, anisotropy
+  float light[4];    // direction toward the light, intensity
+  float smoke[3], sootGlow;
+  float fire[4];     // tint, temperature
+  float sceneLight[3], ambient[3];
+  float filter, raySteps, time;
+  float ivp[16];     // inverse view-projection, column-major
+  int width, height;
+};
+
+// The render preparation's tables and pools. Render field and light slots carry a
+// one-texel apron at their far faces: (B+1)^3 and (B/ld+1)^3 float4 per slot.
+struct RenderArgs {
+  int B, sF, side, nF;
+  int E, nR;                // render field slot edge and texels
+  int lside, LE, nL;        // light brick edge, slot edge, texels
+  int perSide, nO;          // occupancy blocks per axis, per slot
+  const int4 *boxes;        // two words each: first brick + first entry, size
+  int boxCount;
+  int *table;               // page tables
+  const int4 *tiles;        // each tile slot's coordinates and box, or -1
+  const int *pages;         // each brick id's page
+  const int *counts;        // [0] active, [1] fresh
+  const int *active;
+  const int *zeroList;
+  int brickCount;
+  const float4 *F;
+  float4 *R, *Lt;
+  uint8_t *O;
+  const float4 *lut;
+  RenderParams P;
+};
+
+__global__ void k_export_pages(RenderArgs r);
+__global__ void k_smooth_smoke(RenderArgs r, int chunks, int staged);
+__global__ void k_occupancy(RenderArgs r);
+__global__ void k_light(RenderArgs r, int fresh);
+__global__ void k_march(RenderArgs r, float4 *out);
+
 // Loads the home brick's record into LDS; every lane of the block calls it.
 __device__ __forceinline__ void loadHome(const Dir &dir, int id, int *rec, Home &h, int side) {
   if (threadIdx.x < 27) {

@@ -20,6 +20,7 @@
 #include "../../include/firepro.h"
 #include "host_directory.hpp"
 #include "kernels.hpp"
+#include "render_host.hpp"
 
 namespace fp {
 
@@ -202,6 +203,22 @@ struct fp_sim {
   };
   LevelBufs lv[MAX_LEVELS];
   float *latestPressure = nullptr;
+  // Render: the boxes (updated with the tiles each step) and the preparation's state
+  // carried between renders (VolumeLighting.ts:28-63).
+  render::Boxes rboxes;
+  struct RenderCache {
+    int version = -1;
+    bool stale = true;
+    int64_t lastStep = -1, lastLightStep = -1, occupancyStep = -1;
+    std::string lastKey;
+    int poolSlots = -1, lightDivisor = -1, lastWork = -1;
+    int boxCount = 0, pageCount = 0;
+    std::vector<int32_t> boxWords;
+    DBuf<int4> boxes, tiles;
+    DBuf<int> table;
+    DBuf<float4> R, L, lut, image;
+    DBuf<uint8_t> O;
+  } rc;
   // Scene.
   DBuf<Emitter> emitters;
   DBuf<Collider> colliders;
@@ -349,6 +366,7 @@ struct fp_sim {
     capacity = 0;
     growTiles(MIN_TILES);
     applyTiles();
+    rboxes.update(contentTiles, {}, ground, dir, tilesHost, clusters);
   }
 
   // Pools for `slots` slots; with `keep`, live state (fields, expansion, velocity, solids)
@@ -474,6 +492,7 @@ struct fp_sim {
     }
     HIPCHECK(hipStreamSynchronize(stream));  // host vectors are reused
     geometryDirty = true;
+    rboxes.stale = true;  // FluidSimulation.ts:708-709
   }
 
   // growPool (FluidSimulation.ts:1013-1093).
@@ -524,6 +543,7 @@ struct fp_sim {
   void updateTiles(const fp_frame &f) {
     dir.retire(steps);
     double margin = 2 * B * voxel + tileSize();
+    std::vector<render::BrickBox> sourceBricks;
     for (int i = 0; i < f.source_count; i++) {
       const fp_source &s = f.sources[i];
       double extent = std::max(std::max(std::max(s.size[0], s.size[1]), s.size[2]), 0.03) + s.outward_speed * 0.25;
@@ -533,6 +553,12 @@ struct fp_sim {
         hi[a] = s.position[a] + extent;
       }
       needBox(lo, hi, margin);
+      render::BrickBox bb;  // bricksOver (FluidSimulation.ts:862-869)
+      for (int a = 0; a < 3; a++) {
+        bb.lo[a] = (int)std::floor(lo[a] / (B * voxel));
+        bb.hi[a] = (int)std::floor(hi[a] / (B * voxel));
+      }
+      sourceBricks.push_back(bb);
     }
     for (auto &ct : contentTiles) {
       int lo[3], hi[3];
@@ -543,6 +569,7 @@ struct fp_sim {
       dir.need(lo, hi, steps);
     }
     if (dir.dirty) applyTiles();
+    rboxes.update(contentTiles, sourceBricks, ground, dir, tilesHost, clusters);
   }
 
   static double finite32(const char *name, double v) {
@@ -971,6 +998,257 @@ struct fp_sim {
     poolGrownAt = 0;
     activeBricks = 0;
     stepped = false;
+    rc.lastKey.clear();  // SceneVolumeRenderer.reset -> invalidate (SceneVolumeRenderer.ts:254-261)
+  }
+
+  // ---- render (VolumeLighting.encode, VolumeLighting.ts:249-375; the camera march) ----
+  RenderParams renderParams(const fp_camera &cam, const fp_render_settings &s) const {
+    auto finite = [](double v, const char *name) {
+      if (!std::isfinite(v)) throw InvalidArgument(std::string(name) + " must be finite.");
+      return (float)v;
+    };
+    if (cam.width < 1 || cam.height < 1 || cam.width > 16384 || cam.height > 16384)
+      throw InvalidArgument("The image must be 1 to 16384 pixels a side.");
+    int ld = s.lighting_divisor;
+    if (ld != 1 && ld != 2 && ld != 4) throw InvalidArgument("rendering.lightingDivisor must be 1, 2 or 4.");
+    if (B / ld < 1) throw InvalidArgument("rendering.lightingDivisor exceeds the brick size.");
+    if (s.filter < 0 || s.filter > 2) throw InvalidArgument("rendering.filter must be trilinear, quadratic, cubic.");
+    if (s.ray_steps < 1) throw InvalidArgument("rendering.raySteps must be a positive integer.");
+    RenderParams P{};
+    P.voxel = (float)voxel;
+    P.shift = sF;
+    P.lightDivisor = ld;
+    P.lightShift = sF - (ld == 4 ? 2 : ld == 2 ? 1 : 0);
+    P.brickSize = P.voxel * (float)B;
+    P.texel = P.voxel * (float)ld;
+    P.flameOpacity = finite(s.flame_opacity, "flame.opacity");
+    P.optical[0] = finite(s.smoke_density, "smoke.density");
+    P.optical[1] = finite(s.fire_intensity, "flame.brightness");
+    P.optical[2] = finite(s.shadow_density, "smoke.shadowDensity");
+    P.optical[3] = finite(s.scattering, "smoke.scattering");
+    for (int a = 0; a < 3; a++) {
+      P.light[a] = finite(s.sun_direction[a], "light direction");
+      P.smoke[a] = finite(s.smoke_color[a], "smoke.color");
+      P.fire[a] = finite(s.fire_color[a], "flame.color");
+      P.sceneLight[a] = finite(s.sun_color[a], "light color");
+      P.ambient[a] = finite(s.ambient[a], "ambient light");
+    }
+    P.light[3] = finite(s.sun_intensity, "light intensity");
+    P.sootGlow = finite(s.soot_glow, "flame.sootGlow");
+    P.fire[3] = finite(s.temperature, "flame.temperature");
+    P.filter = (float)s.filter;
+    P.raySteps = (float)s.ray_steps;
+    P.time = (float)time;
+    for (int i = 0; i < 16; i++) P.ivp[i] = finite(cam.inverse_view_projection[i], "inverse view-projection");
+    P.width = cam.width;
+    P.height = cam.height;
+    return P;
+  }
+
+  RenderArgs renderArgs(const RenderParams &P) const {
+    RenderArgs r{};
+    r.B = B;
+    r.sF = sF;
+    r.side = side;
+    r.nF = nF;
+    r.E = B + 1;
+    r.nR = r.E * r.E * r.E;
+    r.lside = 1 << P.lightShift;
+    r.LE = r.lside + 1;
+    r.nL = r.LE * r.LE * r.LE;
+    r.perSide = B / 8;
+    r.nO = r.perSide * r.perSide * r.perSide;
+    r.boxes = rc.boxes.p;
+    r.boxCount = rc.boxCount;
+    r.table = rc.table.p;
+    r.tiles = rc.tiles.p;
+    r.pages = pages.p;
+    r.counts = counts.p;
+    r.active = active.p;
+    r.zeroList = zeroList.p;
+    r.brickCount = brickCount();
+    r.F = F.p;
+    r.R = rc.R.p;
+    r.Lt = rc.L.p;
+    r.O = rc.O.p;
+    r.lut = rc.lut.p;
+    r.P = P;
+    return r;
+  }
+
+  static std::string settingsKey(const RenderParams &p) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "%a,%a,%a,%a,%a,%a,%a,%a,%a,%a,%a,%a,%a,%a,%a,%d", p.optical[0], p.optical[2],
+             p.optical[1], p.flameOpacity, p.fire[3], p.fire[0], p.fire[1], p.fire[2], p.smoke[0], p.smoke[1],
+             p.smoke[2], p.sootGlow, p.light[0], p.light[1], p.light[2], p.light[3] > 0.0f ? 1 : 0);
+    return buf;
+  }
+
+  // VolumeLighting.sync + sizePools + encode: bring the page tables, render field,
+  // occupancy and light up to the latest step.
+  void renderPrepare(const RenderParams &P) {
+    if (!rc.lut.p) {
+      std::vector<float> t = render::blackbody();
+      rc.lut.alloc(t.size() / 4);
+      HIPCHECK(hipMemcpyAsync(rc.lut.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    }
+    if (rboxes.version != rc.version) {
+      rc.version = rboxes.version;
+      rc.boxCount = (int)rboxes.boxes.size();
+      rc.boxWords.assign((size_t)std::max(1, rc.boxCount) * 8, 0);
+      int entries = 0;
+      for (int i = 0; i < rc.boxCount; i++) {
+        const auto &b = rboxes.boxes[i];
+        int32_t *w = &rc.boxWords[(size_t)i * 8];
+        int size[3];
+        for (int a = 0; a < 3; a++) size[a] = b.hi[a] - b.lo[a] + 1;
+        w[0] = b.lo[0], w[1] = b.lo[1], w[2] = b.lo[2], w[3] = entries;
+        w[4] = size[0], w[5] = size[1], w[6] = size[2], w[7] = 0;
+        entries += size[0] * size[1] * size[2];
+      }
+      rc.pageCount = entries;
+      if (rc.boxes.n < rc.boxWords.size() / 4) rc.boxes.alloc(rc.boxWords.size() / 4);
+      HIPCHECK(hipMemcpyAsync(rc.boxes.p, rc.boxWords.data(), rc.boxWords.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, stream));
+      if (rc.tiles.n < rboxes.tiles.size() / 4) rc.tiles.alloc(rboxes.tiles.size() / 4);
+      HIPCHECK(hipMemcpyAsync(rc.tiles.p, rboxes.tiles.data(), rboxes.tiles.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, stream));
+      size_t tableWords = (size_t)std::max(1, entries);
+      if (rc.table.n < tableWords) rc.table.alloc(tableWords);
+      k_fill_int<<<(unsigned)((tableWords + 255) / 256), 256, 0, stream>>>(rc.table.p, -1, tableWords);
+      HIPCHECK(hipStreamSynchronize(stream));  // rc.boxWords and rboxes.tiles are host memory
+      rc.stale = true;
+    }
+    bool resized = poolSlots != rc.poolSlots || P.lightDivisor != rc.lightDivisor;
+    if (resized) {
+      rc.poolSlots = poolSlots;
+      rc.lightDivisor = P.lightDivisor;
+      RenderArgs r = renderArgs(P);
+      rc.R.alloc((size_t)poolSlots * r.nR);
+      rc.L.alloc((size_t)poolSlots * r.nL);
+      rc.O.alloc((size_t)poolSlots * r.nO);
+      HIPCHECK(hipMemsetAsync(rc.R.p, 0, rc.R.bytes(), stream));
+      HIPCHECK(hipMemsetAsync(rc.L.p, 0, rc.L.bytes(), stream));
+      HIPCHECK(hipMemsetAsync(rc.O.p, 0, rc.O.bytes(), stream));
+    }
+    bool changed = rc.stale;
+    std::string key = settingsKey(P);
+    bool stepNew = steps != rc.lastStep;
+    bool rebuild = changed || resized;
+    bool smooth = stepNew || rebuild;
+    bool pagesDue = stepNew || changed;
+    bool occupancy = rc.occupancyStep != steps || rebuild;
+    bool lightDue = steps < rc.lastLightStep || steps - rc.lastLightStep >= 2;
+    bool buildLight = (stepNew && lightDue) || rebuild || key != rc.lastKey;
+    bool fresh = stepNew && !buildLight;
+    if (rc.boxCount == 0 || !(smooth || pagesDue || occupancy || buildLight || fresh)) {
+      rc.lastWork = -1;
+      return;
+    }
+    RenderArgs r = renderArgs(P);
+    unsigned slotsGrid = (unsigned)std::max(1, poolSlots);
+    int t = begin();
+    if (pagesDue) k_export_pages<<<(unsigned)((brickCount() + 255) / 256), 256, 0, stream>>>(r);
+    if (smooth) {
+      int chunks = (r.nR + 1023) / 1024;
+      bool staged = B <= 16;
+      size_t lds = staged ? (size_t)(B + 3) * (B + 3) * (B + 3) * sizeof(float) : 0;
+      k_smooth_smoke<<<slotsGrid * chunks, 256, lds, stream>>>(r, chunks, staged ? 1 : 0);
+    }
+    if (occupancy) k_occupancy<<<slotsGrid, 256, 0, stream>>>(r);
+    end(t, "render_prepare");
+    t = begin();
+    if (buildLight) k_light<<<slotsGrid, 64, 0, stream>>>(r, 0);
+    else if (fresh) k_light<<<slotsGrid, 64, 0, stream>>>(r, 1);
+    end(t, "render_light");
+    rc.lastWork = buildLight ? 1 : fresh ? 2 : 0;
+    if (occupancy) rc.occupancyStep = steps;
+    if (buildLight) {
+      rc.lastLightStep = steps;
+      rc.lastKey = key;
+    }
+    rc.lastStep = steps;
+    rc.stale = false;
+  }
+
+  void renderImage(const fp_camera &cam, const fp_render_settings &s, float *out, bool deviceOut) {
+    RenderParams P = renderParams(cam, s);
+    renderPrepare(P);
+    size_t pixels = (size_t)P.width * P.height;
+    float4 *target = reinterpret_cast<float4 *>(out);
+    if (!deviceOut) {
+      if (rc.image.n < pixels) rc.image.alloc(pixels);
+      target = rc.image.p;
+    }
+    RenderArgs r = renderArgs(P);
+    dim3 grid((unsigned)((P.width + 15) / 16), (unsigned)((P.height + 15) / 16));
+    int t = begin();
+    k_march<<<grid, 256, 0, stream>>>(r, target);
+    end(t, "render_march");
+    HIPCHECK(hipGetLastError());
+    if (!deviceOut) {
+      HIPCHECK(hipMemcpyAsync(out, rc.image.p, pixels * sizeof(float4), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+    }
+    if (profiling) {
+      HIPCHECK(hipStreamSynchronize(stream));
+      collect();
+    }
+  }
+
+  void readRender(fp_render_tables *t) {
+    HIPCHECK(hipStreamSynchronize(stream));
+    t->light_work = rc.lastWork;
+    t->box_count = rc.boxCount;
+    t->page_count = rc.pageCount;
+    if (t->boxes && rc.boxCount)
+      std::memcpy(t->boxes, rc.boxWords.data(), (size_t)std::min(rc.boxCount, t->box_capacity) * 8 * sizeof(int32_t));
+    if (t->page_table && rc.pageCount)
+      HIPCHECK(hipMemcpy(t->page_table, rc.table.p, (size_t)std::min(rc.pageCount, t->page_capacity) * sizeof(int32_t),
+                         hipMemcpyDeviceToHost));
+    if (t->blackbody && rc.lut.p) HIPCHECK(hipMemcpy(t->blackbody, rc.lut.p, rc.lut.bytes(), hipMemcpyDeviceToHost));
+    // Listed bricks: active, slotted, inside their tile's box.
+    int n = (int)std::min<int64_t>(activeBricks, (int64_t)brickCount());
+    std::vector<int32_t> ids(n), pg(brickCount());
+    if (n) HIPCHECK(hipMemcpy(ids.data(), active.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(pg.data(), pages.p, pg.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::sort(ids.begin(), ids.end());
+    int E = B + 1, LE = (rc.lightDivisor > 0 ? (B / rc.lightDivisor) : B) + 1, nO = (B / 8) * (B / 8) * (B / 8);
+    size_t nR = (size_t)E * E * E, nL = (size_t)LE * LE * LE;
+    int listed = 0;
+    for (int id : ids) {
+      int page = pg[id];
+      const int32_t *tile = &rboxes.tiles[(size_t)(id >> 6) * 4];
+      if (page < 0 || (size_t)(id >> 6) * 4 >= rboxes.tiles.size() || tile[3] < 0 || rc.version != rboxes.version)
+        continue;
+      const int32_t *w = &rc.boxWords[(size_t)tile[3] * 8];
+      int l = id & 63, loc[3] = {l & 3, (l >> 2) & 3, l >> 4};
+      bool inside = true;
+      for (int a = 0; a < 3; a++) {
+        int local = tile[a] * 4 + loc[a] - w[a];
+        if (local < 0 || local >= w[4 + a]) inside = false;
+      }
+      if (!inside) continue;
+      if (listed < t->brick_capacity && rc.R.p) {
+        int slot = pageToSlotHost(page);
+        if (t->bricks) t->bricks[listed] = id;
+        if (t->field)
+          HIPCHECK(hipMemcpy(t->field + (size_t)listed * nR * 4, rc.R.p + (size_t)slot * nR, nR * sizeof(float4),
+                             hipMemcpyDeviceToHost));
+        if (t->light)
+          HIPCHECK(hipMemcpy(t->light + (size_t)listed * nL * 4, rc.L.p + (size_t)slot * nL, nL * sizeof(float4),
+                             hipMemcpyDeviceToHost));
+        if (t->occupancy)
+          HIPCHECK(hipMemcpy(t->occupancy + (size_t)listed * nO, rc.O.p + (size_t)slot * nO, (size_t)nO,
+                             hipMemcpyDeviceToHost));
+      }
+      listed++;
+    }
+    t->brick_count = listed;
+  }
+  int pageToSlotHost(int page) const {
+    uint32_t p = (uint32_t)page;
+    return (int)(((p >> 16) * (uint32_t)side + ((p >> 8) & 255u)) * (uint32_t)side + (p & 255u));
   }
 
   size_t memoryBytes() const {
@@ -1006,6 +1284,14 @@ struct fp_sim {
     colliders.release();
     forces.release();
     regions.release();
+    rc.boxes.release();
+    rc.tiles.release();
+    rc.table.release();
+    rc.R.release();
+    rc.L.release();
+    rc.lut.release();
+    rc.image.release();
+    rc.O.release();
     for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
     eventPool.clear();
     if (pinned) (void)hipHostFree(pinned);
@@ -1282,6 +1568,31 @@ int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *co
   }
   *count = n;
   return FP_OK;
+}
+
+int fp_sim_render(fp_sim *sim, const fp_camera *camera, const fp_render_settings *settings, float *rgba) {
+  if (!sim || !camera || !settings || !rgba) {
+    g_error = "fp_sim_render: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] { sim->renderImage(*camera, *settings, rgba, false); });
+}
+
+int fp_sim_render_device(fp_sim *sim, const fp_camera *camera, const fp_render_settings *settings,
+                         float *rgba_device) {
+  if (!sim || !camera || !settings || !rgba_device) {
+    g_error = "fp_sim_render_device: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] { sim->renderImage(*camera, *settings, rgba_device, true); });
+}
+
+int fp_sim_read_render(fp_sim *sim, fp_render_tables *out) {
+  if (!sim || !out) {
+    g_error = "fp_sim_read_render: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] { sim->readRender(out); });
 }
 
 }  // extern "C"

@@ -123,8 +123,124 @@ class RenderSettings(C.Structure):
         ("shadow_density", C.c_double), ("temperature", C.c_double), ("soot_glow", C.c_double),
         ("smoke_color", Vec3), ("fire_color", Vec3),
         ("sun_direction", Vec3), ("sun_intensity", C.c_double),
+        ("sun_color", Vec3), ("ambient", Vec3),
         ("filter", C.c_int32), ("ray_steps", C.c_int32), ("lighting_divisor", C.c_int32),
     ]
+
+
+class RenderTables(C.Structure):
+    _fields_ = [
+        ("boxes", C.POINTER(C.c_int32)), ("page_table", C.POINTER(C.c_int32)),
+        ("bricks", C.POINTER(C.c_int32)), ("field", C.POINTER(C.c_float)),
+        ("light", C.POINTER(C.c_float)), ("occupancy", C.POINTER(C.c_uint8)),
+        ("blackbody", C.POINTER(C.c_float)),
+        ("box_capacity", C.c_int32), ("page_capacity", C.c_int32), ("brick_capacity", C.c_int32),
+        ("box_count", C.c_int32), ("page_count", C.c_int32), ("brick_count", C.c_int32),
+        ("light_work", C.c_int32),
+    ]
+
+
+FILTERS = ("trilinear", "quadratic", "cubic")
+
+
+def srgb_to_linear(c: float) -> float:
+    """IEC 61966-2-1 sRGB decoding (three.js Color's hex input is sRGB)."""
+    return c / 12.92 if c <= 0.04045 else ((c + 0.055) / 1.055) ** 2.4
+
+
+def hex_color(h: int):
+    return tuple(srgb_to_linear(((h >> s) & 255) / 255.0) for s in (16, 8, 0))
+
+
+# Rendering defaults (src/library/options.ts:340-365): flame brightness, opacity,
+# temperature, soot glow and color; smoke density, scattering, shadow density and color;
+# the cubic filter at 256 ray steps with lighting every fourth cell. The sun is the
+# editor's (editor/runtime.ts:99): white, intensity 3.
+RENDER_DEFAULTS = {
+    "fire_intensity": 1.4, "flame_opacity": 0.55, "temperature": 3200.0, "soot_glow": 1.0,
+    "fire_color": hex_color(0xFFFFFF), "smoke_density": 2.5, "scattering": 0.38,
+    "shadow_density": 3.5, "smoke_color": hex_color(0x25292E),
+    "sun_direction": (0.5, 1.0, 0.3), "sun_intensity": 3.0, "sun_color": (1.0, 1.0, 1.0),
+    "ambient": (0.0, 0.0, 0.0), "filter": "cubic", "ray_steps": 256, "lighting_divisor": 4,
+}
+
+
+def render_settings(**overrides) -> RenderSettings:
+    o = dict(RENDER_DEFAULTS)
+    for k, v in overrides.items():
+        if k not in o:
+            raise ValueError(f"unknown render setting {k}")
+        o[k] = v
+    rs = RenderSettings()
+    for k in ("fire_intensity", "flame_opacity", "smoke_density", "scattering", "shadow_density",
+              "temperature", "soot_glow", "sun_intensity"):
+        setattr(rs, k, float(o[k]))
+    for k in ("smoke_color", "fire_color", "sun_direction", "sun_color", "ambient"):
+        setattr(rs, k, Vec3(*[float(x) for x in o[k]]))
+    f = o["filter"]
+    rs.filter = FILTERS.index(f) if isinstance(f, str) else int(f)
+    rs.ray_steps = int(o["ray_steps"])
+    rs.lighting_divisor = int(o["lighting_divisor"])
+    return rs
+
+
+def read_render_tables(call, brick_size: int, lighting_divisor: int) -> dict:
+    """Run a read-render entry point (`call(RenderTables) -> status`) twice: once for the
+    counts, once into arrays of those sizes."""
+    import numpy as np
+    t = RenderTables()
+    call(t)
+    E, LE, nO = brick_size + 1, brick_size // lighting_divisor + 1, (brick_size // 8) ** 3
+    out = {
+        "boxes": np.zeros((max(1, t.box_count), 8), np.int32),
+        "page_table": np.zeros(max(1, t.page_count), np.int32),
+        "bricks": np.zeros(max(1, t.brick_count), np.int32),
+        "field": np.zeros((max(1, t.brick_count), E ** 3, 4), np.float32),
+        "light": np.zeros((max(1, t.brick_count), LE ** 3, 4), np.float32),
+        "occupancy": np.zeros((max(1, t.brick_count), nO), np.uint8),
+        "blackbody": np.zeros((512, 4), np.float32),
+    }
+    u = RenderTables()
+    for k, a in out.items():
+        ct = {np.int32: C.c_int32, np.float32: C.c_float, np.uint8: C.c_uint8}[a.dtype.type]
+        setattr(u, k, a.ctypes.data_as(C.POINTER(ct)))
+    u.box_capacity, u.page_capacity, u.brick_capacity = t.box_count, t.page_count, t.brick_count
+    call(u)
+    out["boxes"] = out["boxes"][:u.box_count]
+    out["page_table"] = out["page_table"][:u.page_count]
+    for k in ("bricks", "field", "light", "occupancy"):
+        out[k] = out[k][:u.brick_count]
+    out["light_work"] = u.light_work
+    return out
+
+
+def camera(position, target, fov_degrees: float, width: int, height: int, near: float = 0.05,
+           far: float = 250.0, up=(0.0, 1.0, 0.0)) -> Camera:
+    """A three.js PerspectiveCamera at `position` looking at `target`, with WebGPU's
+    [0, 1] clip depth: inverse_view_projection = matrixWorld x projectionMatrixInverse."""
+    import numpy as np
+    eye, at, upv = (np.asarray(v, dtype=np.float64) for v in (position, target, up))
+    z = eye - at
+    z /= np.linalg.norm(z)
+    x = np.cross(upv, z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    world = np.eye(4)
+    world[:3, 0], world[:3, 1], world[:3, 2], world[:3, 3] = x, y, z, eye
+    top = near * np.tan(np.radians(0.5 * fov_degrees))
+    right = top * (width / height)
+    proj = np.zeros((4, 4))
+    proj[0, 0] = near / right
+    proj[1, 1] = near / top
+    proj[2, 2] = -far / (far - near)
+    proj[2, 3] = -far * near / (far - near)
+    proj[3, 2] = -1.0
+    inv = world @ np.linalg.inv(proj)
+    cam = Camera()
+    cam.inverse_view_projection = (C.c_double * 16)(*inv.T.reshape(-1))  # column-major
+    cam.position = Vec3(*eye)
+    cam.width, cam.height = int(width), int(height)
+    return cam
 
 
 # ---- plain-data frame (SimulationFrame, src/engine/types.ts:235-311) --------------------
@@ -308,6 +424,9 @@ def load(path: str = LIBRARY):
         "fp_sim_profile": (C.c_int, [C.c_void_p, C.c_int32]),
         "fp_sim_profile_read": (C.c_int, [C.c_void_p, P(C.c_char_p), P(C.c_double),
                                           P(C.c_int32)]),
+        "fp_sim_render": (C.c_int, [C.c_void_p, P(Camera), P(RenderSettings), P(C.c_float)]),
+        "fp_sim_render_device": (C.c_int, [C.c_void_p, P(Camera), P(RenderSettings), C.c_void_p]),
+        "fp_sim_read_render": (C.c_int, [C.c_void_p, P(RenderTables)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

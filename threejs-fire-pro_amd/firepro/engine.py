@@ -165,6 +165,29 @@ class FluidSimulation:
         va = np.ascontiguousarray(velocity, np.float32) if velocity is not None else None
         abi.check(self._lib.fp_sim_write_state(self._h(), ptr(fa), ptr(va)), self._lib)
 
+    # ---- render (SceneVolumeRenderer.prepare + the fragment march) ----
+    def render(self, camera: abi.Camera, settings: Optional[abi.RenderSettings] = None) -> np.ndarray:
+        """The premultiplied (radiance rgb, 1 - transmittance) image, (height, width, 4)."""
+        rs = settings if settings is not None else abi.render_settings()
+        img = np.zeros((camera.height, camera.width, 4), np.float32)
+        abi.check(self._lib.fp_sim_render(self._h(), C.byref(camera), C.byref(rs),
+                                          img.ctypes.data_as(C.POINTER(C.c_float))), self._lib)
+        self._lighting_divisor = rs.lighting_divisor
+        return img
+
+    def render_device(self, camera: abi.Camera, settings: abi.RenderSettings, pointer: int) -> None:
+        """Render into device memory at `pointer` (width * height float4), enqueued on the
+        simulation's stream."""
+        abi.check(self._lib.fp_sim_render_device(self._h(), C.byref(camera), C.byref(settings),
+                                                 C.c_void_p(pointer)), self._lib)
+        self._lighting_divisor = settings.lighting_divisor
+
+    def read_render(self) -> dict:
+        ld = getattr(self, "_lighting_divisor", 4)
+        return abi.read_render_tables(
+            lambda t: abi.check(self._lib.fp_sim_read_render(self._h(), C.byref(t)), self._lib),
+            self.brick_size, ld)
+
     def stream(self) -> int:
         return self._lib.fp_sim_stream(self._h())
 
