@@ -44,7 +44,24 @@ def parse():
     p.add_argument("--seed", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--render-steps", type=int, default=6,
+                   help="steps after the timed ones, each followed by a 1920x1080 render")
+    p.add_argument("--copy-probe", type=int, default=0,
+                   help="only copy this many MiB device to device 5 times (counter calibration)")
     return p.parse_args()
+
+
+def copy_probe(mib: int) -> None:
+    """A plain device copy of known size: the calibration for HBM byte counters."""
+    from firepro import abi
+    n = mib << 20
+    a, b = abi.DeviceBuffer(n), abi.DeviceBuffer(n)
+    for _ in range(5):
+        b.copy_from(a, n)
+    b.synchronize()
+    a.close()
+    b.close()
+    print(json.dumps({"copy_probe_bytes": n, "copies": 5}))
 
 
 def cpu_baseline(scene_fn, seconds: float) -> dict:
@@ -69,11 +86,59 @@ def cpu_baseline(scene_fn, seconds: float) -> dict:
                       f"{elapsed:.1f} s"}
 
 
+def render_phase(sim, frames, step) -> dict:
+    """After the timed steps: a step, then the 1920x1080 ray-march with self-shadowing,
+    several times; the render stages' event-timed averages (preparation every step,
+    the whole light every second step, the march every frame)."""
+    from firepro import abi
+    w, h = 1920, 1080
+    cam = abi.camera((0.0, 3.0, 9.0), (0.0, 1.5, 0.0), 50, w, h)
+    rs = abi.render_settings()
+    buf = abi.DeviceBuffer(w * h * 16)
+    try:
+        step(*frames[0])
+        sim.render_device(cam, rs, buf.ptr.value)  # first build of the render pools
+        buf.synchronize()
+        sim.profile(True)
+        walls = []
+        for frame, g in frames[1:]:
+            step(frame, g)
+            t0 = time.perf_counter()
+            sim.render_device(cam, rs, buf.ptr.value)
+            buf.synchronize()
+            walls.append(time.perf_counter() - t0)
+        stages = sim.profile_read()
+    finally:
+        buf.close()
+    out = {"width": w, "height": h, "renders": len(walls),
+           "wall_ms_mean": 1e3 * float(np.mean(walls)) if walls else None}
+    for k in ("render_prepare", "render_light", "render_march"):
+        if k in stages:
+            out[k + "_ms"] = round(stages[k], 4)
+    solver = {k: round(v, 4) for k, v in stages.items() if not k.startswith("render_")}
+    return out, solver
+
+
+def measured_traffic(cells: float):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE scaled by the copy calibration, WRITE_SIZE), per active cell,
+    times this run's cells; None without the summary."""
+    path = os.path.join(ROOT, "profiles", "evolve_fields_four_fires_b8_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t["bytes_per_cell"] * cells
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     from firepro import abi, scenes
     from firepro.engine import FluidSimulation
     try:
@@ -82,7 +147,8 @@ def main():
     except Exception:
         sync = lambda: None  # noqa: E731
 
-    scene_fn = lambda: scenes.four_fires(seed=args.seed, steps=args.warmup + args.steps)  # noqa: E731
+    total = args.warmup + args.steps + args.render_steps
+    scene_fn = lambda: scenes.four_fires(seed=args.seed, steps=total)  # noqa: E731
     sc = scene_fn()
     frames = [(abi.PackedFrame(f), g) for f, g in sc.frames()]
     sim = FluidSimulation.create(sc.grid())
@@ -93,11 +159,11 @@ def main():
 
     for frame, g in frames[: args.warmup]:
         step(frame, g)
-    sim.profile(True)
+    sim.profile(2)  # events around the dominant kernel only, inside the timed region
     sync()
     latencies, voxels = [], 0
     t0 = time.perf_counter()
-    for frame, g in frames[args.warmup:]:
+    for frame, g in frames[args.warmup:args.warmup + args.steps]:
         s0 = time.perf_counter()
         step(frame, g)  # returns once the step's readback has landed
         latencies.append(time.perf_counter() - s0)
@@ -123,10 +189,14 @@ def main():
                    "active_bricks": bricks, "active_voxels": bricks * 512},
         "p50_step_ms": float(np.percentile(latencies, 50) * 1e3),
         "p99_step_ms": float(np.percentile(latencies, 99) * 1e3),
-        "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        "timed_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": {"kernel": best, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(cells),
+                     "algorithmic_bytes": STAGE_BYTES_PER_FIELD_CELL[best] * cells},
     }
+    if args.render_steps > 0:
+        line["render"], line["stage_ms"] = render_phase(sim, frames[args.warmup + args.steps:], step)
     sim.dispose()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lambda: scenes.four_fires(seed=args.seed), args.cpu_seconds)

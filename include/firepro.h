@@ -219,7 +219,10 @@ int fp_sim_read_state(fp_sim *sim, int32_t *ids, float *fields, float *velocity,
 int fp_sim_write_state(fp_sim *sim, const float *fields, const float *velocity);
 /* The device stream the solver enqueues on (hipStream_t), for callers that time it. */
 void *fp_sim_stream(fp_sim *sim);
-/* Per-stage event timing of the next steps: enable, then read averages (ms). */
+/* Timing of the next steps, read back as averages in ms: enable 1 records device events
+ * around every stage (and the render's) plus the host phases of a step (host_*); enable 2
+ * records events around the dominant stage (evolve_fields) only, so the others run
+ * without event gaps; 0 stops. Enabling clears earlier averages. */
 int fp_sim_profile(fp_sim *sim, int32_t enable);
 int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *count);
 

@@ -9,7 +9,8 @@
 #include <array>
 #include <cmath>
 #include <stdexcept>
-#include <unordered_map>
+#include <climits>
+#include <utility>
 #include <vector>
 
 namespace fp {
@@ -42,6 +43,58 @@ struct TileBox {
   int lo[3], hi[3];
 };
 
+// Tile coordinates -> tile, open-addressed with linear probing, at most half full.
+template <class V> class FlatIndex {
+ public:
+  static constexpr int64_t EMPTY = INT64_MIN;
+  size_t size() const { return count_; }
+  V find(int64_t k) const {
+    if (keys_.empty()) return V();
+    for (size_t h = mix(k) & mask_;; h = (h + 1) & mask_) {
+      if (keys_[h] == k) return vals_[h];
+      if (keys_[h] == EMPTY) return V();
+    }
+  }
+  void insert(int64_t k, V v) {
+    if ((count_ + 1) * 2 > keys_.size()) grow();
+    size_t h = mix(k) & mask_;
+    while (keys_[h] != EMPTY && keys_[h] != k) h = (h + 1) & mask_;
+    if (keys_[h] == EMPTY) count_++;
+    keys_[h] = k;
+    vals_[h] = v;
+  }
+  void clear() {
+    keys_.clear();
+    vals_.clear();
+    count_ = 0;
+    mask_ = 0;
+  }
+  // Rebuild from (key, value) pairs, after removals.
+  template <class It> void rebuild(It begin, It end) {
+    clear();
+    for (It i = begin; i != end; ++i) insert(i->first, i->second);
+  }
+
+ private:
+  static size_t mix(int64_t k) {
+    uint64_t x = (uint64_t)k * 0x9E3779B97F4A7C15ull;
+    return (size_t)(x ^ (x >> 29));
+  }
+  void grow() {
+    std::vector<int64_t> keys(std::max<size_t>(64, keys_.size() * 2), EMPTY);
+    std::vector<V> vals(keys.size());
+    std::swap(keys, keys_);
+    std::swap(vals, vals_);
+    mask_ = keys_.size() - 1;
+    count_ = 0;
+    for (size_t i = 0; i < keys.size(); i++)
+      if (keys[i] != EMPTY) insert(keys[i], vals[i]);
+  }
+  std::vector<int64_t> keys_;
+  std::vector<V> vals_;
+  size_t count_ = 0, mask_ = 0;
+};
+
 class TileDirectory {
  public:
   struct Tile {
@@ -49,6 +102,7 @@ class TileDirectory {
     int slot;
     int64_t needed;
     bool closing;
+    int group;  // scratch of the cluster pass
   };
   static constexpr int LINGER_STEPS = 60, RECORD_WORDS = 32, HASH_ROW = 256;
 
@@ -65,10 +119,7 @@ class TileDirectory {
   static int64_t key(int x, int y, int z) {
     return ((int64_t)(x & 0x1fffff) << 42) | ((int64_t)(y & 0x1fffff) << 21) | (int64_t)(z & 0x1fffff);
   }
-  const Tile *find(int x, int y, int z) const {
-    auto it = index_.find(key(x, y, z));
-    return it == index_.end() ? nullptr : it->second;
-  }
+  const Tile *find(int x, int y, int z) const { return index_.find(key(x, y, z)); }
   int slotAt(int x, int y, int z) const {
     const Tile *t = find(x, y, z);
     return t ? t->slot : -1;
@@ -79,25 +130,23 @@ class TileDirectory {
     for (int z = lo[2]; z <= hi[2]; z++)
       for (int y = ground ? std::max(0, lo[1]) : lo[1]; y <= hi[1]; y++)
         for (int x = lo[0]; x <= hi[0]; x++) {
-          auto it = index_.find(key(x, y, z));
-          if (it != index_.end()) {
-            it->second->needed = step;
-            if (it->second->closing) {
-              it->second->closing = false;
+          Tile *found = index_.find(key(x, y, z));
+          if (found) {
+            found->needed = step;
+            if (found->closing) {
+              found->closing = false;
               dirty = true;
             }
             continue;
           }
-          int slot = (int)slots_.size();
-          for (size_t i = 0; i < slots_.size(); i++)
-            if (!slots_[i]) {
-              slot = (int)i;
-              break;
-            }
-          Tile *t = new Tile{{x, y, z}, slot, step, false};
+          // The lowest free slot (TileDirectory.ts:100); none below firstFree_.
+          int slot = firstFree_;
+          while (slot < (int)slots_.size() && slots_[slot]) slot++;
+          Tile *t = new Tile{{x, y, z}, slot, step, false, -1};
           if (slot == (int)slots_.size()) slots_.push_back(t);
           else slots_[slot] = t;
-          index_[key(x, y, z)] = t;
+          firstFree_ = slot + 1;
+          index_.insert(key(x, y, z), t);
           order_.push_back(t);
           created_.push_back(slot);
           dirty = true;
@@ -107,12 +156,14 @@ class TileDirectory {
   // retire (TileDirectory.ts:111-123).
   void retire(int64_t step) {
     size_t w = 0;
+    bool removed = false;
     for (size_t r = 0; r < order_.size(); r++) {
       Tile *t = order_[r];
       if (t->closing) {
-        index_.erase(key(t->c[0], t->c[1], t->c[2]));
         slots_[t->slot] = nullptr;
+        firstFree_ = std::min(firstFree_, t->slot);
         dirty = true;
+        removed = true;
         delete t;
         continue;
       }
@@ -124,6 +175,13 @@ class TileDirectory {
     }
     order_.resize(w);
     while (!slots_.empty() && !slots_.back()) slots_.pop_back();
+    firstFree_ = std::min(firstFree_, (int)slots_.size());
+    if (removed) {
+      std::vector<std::pair<int64_t, Tile *>> live;
+      live.reserve(order_.size());
+      for (Tile *t : order_) live.push_back({key(t->c[0], t->c[1], t->c[2]), t});
+      index_.rebuild(live.begin(), live.end());
+    }
   }
 
   void clear() {
@@ -131,6 +189,7 @@ class TileDirectory {
     order_.clear();
     index_.clear();
     slots_.clear();
+    firstFree_ = 0;
     created_.clear();
     dirty = true;
   }
@@ -159,6 +218,7 @@ class TileDirectory {
       r[3] = 0;
     }
     const uint32_t mask = (uint32_t)size - 1;
+    buildDense();
     for (const Tile *t : order_) {
       int32_t *r = &records[(size_t)t->slot * RECORD_WORDS];
       r[0] = t->c[0];
@@ -173,11 +233,11 @@ class TileDirectory {
       for (int dz = -1; dz <= 1; dz++)
         for (int dy = -1; dy <= 1; dy++)
           for (int dx = -1; dx <= 1; dx++) {
-            const Tile *n = find(t->c[0] + dx, t->c[1] + dy, t->c[2] + dz);
+            const Tile *n = near(t->c[0] + dx, t->c[1] + dy, t->c[2] + dz);
             if (n) r[4 + (dz + 1) * 9 + (dy + 1) * 3 + dx + 1] = n->slot;
           }
     }
-    std::unordered_map<const Tile *, int> clusterOf;
+    std::vector<int> clusterOf;  // per tile of order_
     buildClusters(clusters, clusterOf);
     tiles.assign((size_t)cap * 4, 0);
     coords.assign(slots_.size(), {0, 0, 0, 0});
@@ -188,7 +248,7 @@ class TileDirectory {
         w[0] = t->c[0];
         w[1] = t->c[1];
         w[2] = t->c[2];
-        w[3] = clusterOf[t];
+        w[3] = clusterOf[t->group];
         coords[s] = {t->c[0], t->c[1], t->c[2], 1};
       } else {
         w[3] = -1;
@@ -203,17 +263,19 @@ class TileDirectory {
 
  private:
   // clusters (TileDirectory.ts:189-245): 26-connected groups, boxed, merged until disjoint.
-  void buildClusters(std::vector<TileBox> &out, std::unordered_map<const Tile *, int> &clusterOf) {
-    std::unordered_map<const Tile *, int> group;
+  // Each tile's `group` is its connected group; clusterOf maps groups to clusters.
+  void buildClusters(std::vector<TileBox> &out, std::vector<int> &clusterOf) {
+    for (Tile *t : order_) t->group = -1;
     std::vector<TileBox> boxes;
-    for (const Tile *first : order_) {
-      if (group.count(first)) continue;
+    std::vector<Tile *> stack;
+    for (Tile *first : order_) {
+      if (first->group >= 0) continue;
       int idx = (int)boxes.size();
       TileBox box{{first->c[0], first->c[1], first->c[2]}, {first->c[0], first->c[1], first->c[2]}};
-      group[first] = idx;
-      std::vector<const Tile *> stack{first};
+      first->group = idx;
+      stack.assign(1, first);
       while (!stack.empty()) {
-        const Tile *t = stack.back();
+        Tile *t = stack.back();
         stack.pop_back();
         for (int a = 0; a < 3; a++) {
           box.lo[a] = std::min(box.lo[a], t->c[a]);
@@ -222,9 +284,9 @@ class TileDirectory {
         for (int dz = -1; dz <= 1; dz++)
           for (int dy = -1; dy <= 1; dy++)
             for (int dx = -1; dx <= 1; dx++) {
-              const Tile *n = find(t->c[0] + dx, t->c[1] + dy, t->c[2] + dz);
-              if (n && !group.count(n)) {
-                group[n] = idx;
+              Tile *n = near(t->c[0] + dx, t->c[1] + dy, t->c[2] + dz);
+              if (n && n->group < 0) {
+                n->group = idx;
                 stack.push_back(n);
               }
             }
@@ -263,17 +325,56 @@ class TileDirectory {
         }
     }
     out.clear();
-    std::unordered_map<int, int> renumbered;
+    std::vector<int> renumbered(boxes.size(), -1);
     for (size_t i = 0; i < boxes.size(); i++)
       if (root((int)i) == (int)i) {
-        renumbered[(int)i] = (int)out.size();
+        renumbered[i] = (int)out.size();
         out.push_back(boxes[i]);
       }
-    for (auto &kv : group) clusterOf[kv.first] = renumbered[root(kv.second)];
+    clusterOf.assign(boxes.size(), -1);
+    for (size_t i = 0; i < boxes.size(); i++) clusterOf[i] = renumbered[root((int)i)];
+  }
+
+  // A dense grid of the tiles over their bounding box (one cell of margin), for the
+  // neighbor lookups of tables(); the flat index serves when the box is too large.
+  void buildDense() {
+    dense_.clear();
+    if (order_.empty()) return;
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) lo[a] = hi[a] = order_[0]->c[a];
+    for (const Tile *t : order_)
+      for (int a = 0; a < 3; a++) {
+        lo[a] = std::min(lo[a], t->c[a]);
+        hi[a] = std::max(hi[a], t->c[a]);
+      }
+    size_t cells = 1;
+    for (int a = 0; a < 3; a++) {
+      denseLo_[a] = lo[a] - 1;
+      denseDim_[a] = hi[a] - lo[a] + 3;
+      cells *= (size_t)denseDim_[a];
+    }
+    if (cells > ((size_t)1 << 24)) return;
+    dense_.assign(cells, nullptr);
+    for (Tile *t : order_)
+      dense_[((size_t)(t->c[2] - denseLo_[2]) * denseDim_[1] + (t->c[1] - denseLo_[1])) * denseDim_[0] +
+             (t->c[0] - denseLo_[0])] = t;
+  }
+  Tile *near(int x, int y, int z) const {
+    if (dense_.empty()) return index_.find(key(x, y, z));
+    x -= denseLo_[0];
+    y -= denseLo_[1];
+    z -= denseLo_[2];
+    if ((unsigned)x >= (unsigned)denseDim_[0] || (unsigned)y >= (unsigned)denseDim_[1] ||
+        (unsigned)z >= (unsigned)denseDim_[2])
+      return nullptr;
+    return dense_[((size_t)z * denseDim_[1] + y) * denseDim_[0] + x];
   }
 
   std::vector<Tile *> order_;
-  std::unordered_map<int64_t, Tile *> index_;
+  FlatIndex<Tile *> index_;
+  int firstFree_ = 0;
+  std::vector<Tile *> dense_;
+  int denseLo_[3] = {0, 0, 0}, denseDim_[3] = {0, 0, 0};
   std::vector<Tile *> slots_;
   std::vector<int> created_;
 };

@@ -7,10 +7,11 @@
 namespace fp {
 
 #define BRICK_PREAMBLE(CHUNKS)                              \
-  const int bi = blockIdx.x / (CHUNKS);                     \
-  const int chunk = blockIdx.x % (CHUNKS);                  \
+  const int item = xcdItem(s.counts[0] * (CHUNKS));         \
+  if (item < 0) return;                                     \
+  const int bi = item / (CHUNKS);                           \
+  const int chunk = item % (CHUNKS);                        \
   (void)chunk;                                              \
-  if (bi >= s.counts[0]) return;                            \
   const int id = s.active[bi];                              \
   __shared__ int rec[54];                                   \
   Ctx c;                                                    \

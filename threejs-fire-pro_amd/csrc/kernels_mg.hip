@@ -145,13 +145,9 @@ __device__ __forceinline__ void levelCell(int home0, int home1, int home2, int s
 // of the zeroed list that hold a slot.
 __global__ void __launch_bounds__(256) k_topology_bricks(MGArgs m, int fromZeroList) {
   int id;
-  if (fromZeroList) {
-    if ((int)blockIdx.x >= m.counts[1] + m.counts[5]) return;
-    id = m.zeroList[2 * blockIdx.x];
-  } else {
-    if ((int)blockIdx.x >= m.counts[0]) return;
-    id = m.active[blockIdx.x];
-  }
+  const int bi = xcdItem(fromZeroList ? m.counts[1] + m.counts[5] : m.counts[0]);
+  if (bi < 0) return;
+  id = fromZeroList ? m.zeroList[2 * bi] : m.active[bi];
   if (m.dir.entries[id] < 0) return;
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
@@ -184,8 +180,9 @@ __global__ void __launch_bounds__(64) k_topology_tiles(MGArgs m, int lv) {
 
 // The step's flags of the brick levels: topology plus the unknown bit.
 __global__ void __launch_bounds__(256) k_eflags_bricks(MGArgs m) {
-  if ((int)blockIdx.x >= m.counts[0]) return;
-  const int id = m.active[blockIdx.x];
+  const int bi = xcdItem(m.counts[0]);
+  if (bi < 0) return;
+  const int id = m.active[bi];
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
   for (int lv = 0; lv < m.L.brickLevels; lv++) {
@@ -248,8 +245,9 @@ __device__ __forceinline__ void relaxCell(const MGCtx &c, const Level &lv, int l
 }
 
 __global__ void __launch_bounds__(256) k_mg_init(MGArgs m) {
-  if ((int)blockIdx.x >= m.counts[0]) return;
-  const int id = m.active[blockIdx.x];
+  const int bi = xcdItem(m.counts[0]);
+  if (bi < 0) return;
+  const int id = m.active[bi];
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
   const Level &lv = m.lv[0];
@@ -265,8 +263,9 @@ __global__ void __launch_bounds__(256) k_mg_init(MGArgs m) {
 }
 
 __global__ void __launch_bounds__(256) k_mg_relax_bricks(MGArgs m, int level) {
-  if ((int)blockIdx.x >= m.counts[0]) return;
-  const int id = m.active[blockIdx.x];
+  const int bi = xcdItem(m.counts[0]);
+  if (bi < 0) return;
+  const int id = m.active[bi];
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
   const Level &lv = m.lv[level];
@@ -322,8 +321,9 @@ __device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgs &m, in
 }
 
 __global__ void __launch_bounds__(64) k_mg_restrict_bricks(MGArgs m, int level) {
-  if ((int)blockIdx.x >= m.counts[0]) return;
-  const int id = m.active[blockIdx.x];
+  const int bi = xcdItem(m.counts[0]);
+  if (bi < 0) return;
+  const int id = m.active[bi];
   MG_BRICK_HOME(id)
   int sd = m.lv[level].side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
@@ -396,8 +396,9 @@ __device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgs &m, int
 }
 
 __global__ void __launch_bounds__(256) k_mg_prolong_bricks(MGArgs m, int level) {
-  if ((int)blockIdx.x >= m.counts[0]) return;
-  const int id = m.active[blockIdx.x];
+  const int bi = xcdItem(m.counts[0]);
+  if (bi < 0) return;
+  const int id = m.active[bi];
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
   int sd = m.lv[level].side, n = sd * sd * sd;

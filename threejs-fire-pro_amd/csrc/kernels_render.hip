@@ -249,8 +249,9 @@ __global__ void __launch_bounds__(256) k_export_pages(RenderArgs r) {
 // taps, clamped to the box, has its two cells per axis there. The pages of the 27 bricks
 // around are staged either way.
 __global__ void __launch_bounds__(256) k_smooth_smoke(RenderArgs r, int chunks, int staged) {
-  const int i = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
-  if (i >= r.counts[0]) return;
+  const int item = xcdItem(r.counts[0] * chunks);
+  if (item < 0) return;
+  const int i = item / chunks, chunk = item % chunks;
   const int id = r.active[i];
   int page, brick[3];
   RBox bx;
@@ -328,8 +329,8 @@ __global__ void __launch_bounds__(256) k_smooth_smoke(RenderArgs r, int chunks, 
 // 12^3 cells of the block and its two-cell halo whose bricks have slots. A workgroup per
 // listed brick; its lanes OR their flags through LDS.
 __global__ void __launch_bounds__(256) k_occupancy(RenderArgs r) {
-  const int i = blockIdx.x;
-  if (i >= r.counts[0]) return;
+  const int i = xcdItem(r.counts[0]);
+  if (i < 0) return;
   const int id = r.active[i];
   int page, brick[3];
   RBox bx;
@@ -366,15 +367,9 @@ __global__ void __launch_bounds__(256) k_occupancy(RenderArgs r) {
 // slot, and texels on the near faces also fill the aprons of the bricks before) or per
 // slot given out this step (`fresh` = 1, lightFresh: every texel of the slot).
 __global__ void __launch_bounds__(64) k_light(RenderArgs r, int fresh) {
-  const int i = blockIdx.x;
-  int id;
-  if (fresh) {
-    if (i >= r.counts[1]) return;
-    id = r.zeroList[2 * i];
-  } else {
-    if (i >= r.counts[0]) return;
-    id = r.active[i];
-  }
+  const int i = xcdItem(fresh ? r.counts[1] : r.counts[0]);
+  if (i < 0) return;
+  const int id = fresh ? r.zeroList[2 * i] : r.active[i];
   int page, brick[3];
   RBox bx;
   if (!slotted(r, id, page, bx, brick)) return;

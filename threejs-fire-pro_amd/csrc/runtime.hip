@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -231,6 +232,18 @@ struct fp_sim {
   size_t pinnedWords = 0;
   // Profiling.
   bool profiling = false;
+  int profileLevel = 0;
+  using Clock = std::chrono::steady_clock;
+  // A host phase of the step, in ms.
+  void hostMark(const char *name, Clock::time_point t0) {
+    if (!profiling) return;
+    double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    auto &e = profile[name];
+    if (e.second == 0 && std::find(profileNames.begin(), profileNames.end(), name) == profileNames.end())
+      profileNames.push_back(name);
+    e.first += ms;
+    e.second += 1;
+  }
   struct Mark {
     const char *name;
     hipEvent_t a, b;
@@ -282,8 +295,9 @@ struct fp_sim {
     HIPCHECK(hipEventCreate(&e));
     return e;
   }
-  int begin() {
-    if (!profiling) return -1;
+  // A device interval; at level 2 only the dominant stage's.
+  int begin(bool dominant = false) {
+    if (!profiling || (profileLevel == 2 && !dominant)) return -1;
     Mark m{nullptr, event(), event()};
     HIPCHECK(hipEventRecord(m.a, stream));
     marks.push_back(m);
@@ -475,7 +489,9 @@ struct fp_sim {
       while (cap < needed) cap *= 2;
       growTiles(cap);
     }
+    Clock::time_point t0 = Clock::now();
     dir.tables(capacity, tileRecHost, hashHost, clusters, tilesHost);
+    hostMark("host_tiles_tables", t0);
     HIPCHECK(hipMemcpyAsync(tileRec.p, tileRecHost.data(), tileRecHost.size() * sizeof(int32_t),
                             hipMemcpyHostToDevice, stream));
     if (hash.n != hashHost.size()) hash.alloc(hashHost.size());
@@ -568,8 +584,14 @@ struct fp_sim {
       }
       dir.need(lo, hi, steps);
     }
-    if (dir.dirty) applyTiles();
+    if (dir.dirty) {
+      Clock::time_point a0 = Clock::now();
+      applyTiles();
+      hostMark("host_tiles_apply", a0);
+    }
+    Clock::time_point b0 = Clock::now();
     rboxes.update(contentTiles, sourceBricks, ground, dir, tilesHost, clusters);
+    hostMark("host_tiles_boxes", b0);
   }
 
   static double finite32(const char *name, double v) {
@@ -846,7 +868,10 @@ struct fp_sim {
       dir.ground = ground;
       geometryDirty = true;
     }
+    Clock::time_point h0 = Clock::now();
     updateTiles(f);
+    hostMark("host_tiles", h0);
+    h0 = Clock::now();
     int wanted = 0;
     for (int i = 0; i < f.source_count; i++)
       if (std::find(sourceIds.begin(), sourceIds.end(), f.sources[i].id) == sourceIds.end())
@@ -865,15 +890,19 @@ struct fp_sim {
     for (int i = 0; i < f.source_count; i++) anyFlame = anyFlame || f.sources[i].flame > 0;
     mayHaveFlame = mayHaveFlame || fuel.enabled || anyFlame;
     setParams(f, dt);
+    hostMark("host_scene", h0);
     enqueueStep();
+    h0 = Clock::now();
     readActivity();
+    hostMark("host_activity", h0);
   }
 
   // Every kernel of one step, in stream order.
   void enqueueStep() {
+    const Clock::time_point enqueued = Clock::now();
     const int n = brickCount();
     const unsigned blocks = (unsigned)((n + 255) / 256);
-    const unsigned slotsGrid = (unsigned)std::max(1, poolSlots);
+    const unsigned slotsGrid = (unsigned)(std::max(1, poolSlots) + 7) / 8 * 8;  // whole rounds over the XCDs
     AllocArgs a = allocArgs();
     int m0 = begin();
     k_build_bricks<<<blocks, 256, 0, stream>>>(a);
@@ -898,7 +927,7 @@ struct fp_sim {
     int m3 = begin();
     k_advect_scalars<<<slotsGrid * fchunks, 256, 0, stream>>>(s, fchunks);
     end(m3, "advect_scalars");
-    int m4 = begin();
+    int m4 = begin(true);
     k_evolve_fields<<<slotsGrid * fchunks, 256, 0, stream>>>(s, fchunks);
     end(m4, "evolve_fields");
     std::swap(F, Fn);
@@ -921,7 +950,10 @@ struct fp_sim {
                                                                            capacity, readback.p);
     HIPCHECK(hipMemcpyAsync(pinned, readback.p, (4 + (size_t)capacity) * sizeof(int), hipMemcpyDeviceToHost, stream));
     k_reset_failures<<<1, 1, 0, stream>>>(poolState.p);
+    hostMark("host_enqueue", enqueued);
+    Clock::time_point w0 = Clock::now();
     HIPCHECK(hipStreamSynchronize(stream));
+    hostMark("host_wait", w0);
     HIPCHECK(hipGetLastError());
     if (profiling) collect();
   }
@@ -1146,7 +1178,7 @@ struct fp_sim {
       return;
     }
     RenderArgs r = renderArgs(P);
-    unsigned slotsGrid = (unsigned)std::max(1, poolSlots);
+    unsigned slotsGrid = (unsigned)(std::max(1, poolSlots) + 7) / 8 * 8;  // whole rounds over the XCDs
     int t = begin();
     if (pagesDue) k_export_pages<<<(unsigned)((brickCount() + 255) / 256), 256, 0, stream>>>(r);
     if (smooth) {
@@ -1550,6 +1582,7 @@ int fp_sim_profile(fp_sim *sim, int32_t enable) {
     return FP_EINVAL;
   }
   sim->profiling = enable != 0;
+  sim->profileLevel = enable;
   sim->profile.clear();
   sim->profileNames.clear();
   return FP_OK;

@@ -184,6 +184,39 @@ def render_settings(**overrides) -> RenderSettings:
     return rs
 
 
+class DeviceBuffer:
+    """Device memory from the HIP runtime libfirepro.so uses (for device-output entry
+    points and probes), released on close()."""
+
+    def __init__(self, nbytes: int):
+        load()
+        with open("/proc/self/maps") as f:
+            paths = sorted({line.split()[-1] for line in f if "libamdhip64" in line})
+        if not paths:
+            raise FireProError("libfirepro.so has no HIP runtime loaded")
+        self._hip = hip = C.CDLL(paths[0])
+        hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        hip.hipFree.argtypes = [C.c_void_p]
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.ptr = C.c_void_p()
+        self.nbytes = int(nbytes)
+        if hip.hipMalloc(C.byref(self.ptr), self.nbytes) != 0:
+            raise FireProError(f"hipMalloc of {nbytes} bytes failed")
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int) -> None:
+        if self._hip.hipMemcpy(self.ptr, other.ptr, int(nbytes), 3) != 0:  # device to device
+            raise FireProError("hipMemcpy failed")
+
+    def synchronize(self) -> None:
+        if self._hip.hipDeviceSynchronize() != 0:
+            raise FireProError("hipDeviceSynchronize failed")
+
+    def close(self) -> None:
+        if self.ptr.value:
+            self._hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
 def read_render_tables(call, brick_size: int, lighting_divisor: int) -> dict:
     """Run a read-render entry point (`call(RenderTables) -> status`) twice: once for the
     counts, once into arrays of those sizes."""

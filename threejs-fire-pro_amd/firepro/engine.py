@@ -191,7 +191,8 @@ class FluidSimulation:
     def stream(self) -> int:
         return self._lib.fp_sim_stream(self._h())
 
-    def profile(self, enable: bool) -> None:
+    def profile(self, enable) -> None:
+        """0 off; 1 (True) every stage and host phase; 2 the dominant stage only."""
         abi.check(self._lib.fp_sim_profile(self._h(), int(enable)), self._lib)
 
     def profile_read(self) -> dict:
