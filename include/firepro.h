@@ -37,6 +37,12 @@ extern "C" {
 
 #define FP_MAX_FLAME_KNOTS 8
 
+/* Pressure policies (fp_sim_set_pressure). */
+#define FP_PRESSURE_VCYCLE 0 /* the reference: one cold V-cycle, FP32 (PressureSolver.ts:371-402) */
+#define FP_PRESSURE_FP32 1   /* defect correction, V-cycle levels stored in FP32 */
+#define FP_PRESSURE_BF16 2   /* defect correction, V-cycle levels stored in BF16 */
+#define FP_PRESSURE_FP16 3   /* defect correction, V-cycle levels stored in FP16 */
+
 /* SimulationGrid (src/engine/types.ts:26-36). Construction-only. */
 typedef struct fp_grid {
   double voxel_size;        /* meters, 0.0001..16 (FluidSimulation.ts:362-363) */
@@ -225,6 +231,17 @@ void *fp_sim_stream(fp_sim *sim);
  * without event gaps; 0 stops. Enabling clears earlier averages. */
 int fp_sim_profile(fp_sim *sim, int32_t enable);
 int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *count);
+
+/* The pressure policy of the next steps. Defect correction keeps the solution x in FP32,
+ * starts from x = 0 and repeats r = b - A x (FP32, each row's dot product and the norms in
+ * FP64), e = one V-cycle on
+ * A e = r (arithmetic FP32, levels stored at the mode's precision, coarsest level in
+ * FP64), x += e, until |r| / |b| <= tolerance or max_cycles V-cycles have run. With
+ * FP_PRESSURE_FP32 and max_cycles 1 it is FP_PRESSURE_VCYCLE but for the coarsest
+ * level's FP64 sweeps. No counterpart in the reference, which runs one V-cycle. */
+int fp_sim_set_pressure(fp_sim *sim, int32_t mode, double tolerance, int32_t max_cycles);
+/* V-cycles the latest step ran and its final relative residual (-1 for FP_PRESSURE_VCYCLE). */
+int fp_sim_pressure_report(const fp_sim *sim, int32_t *cycles, double *relative_residual);
 
 /* The ray-march of the simulation's latest step (a12, a13): brings the render
  * preparation up to date on VolumeLighting.encode's schedule (VolumeLighting.ts:249-375)

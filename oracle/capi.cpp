@@ -28,6 +28,8 @@ struct Base {
   virtual int tileTables(int32_t *records, int32_t *hash, int cap, int hashCap) = 0;
   virtual int brickCoords(const int32_t *ids, int n, int32_t *out) = 0;
   virtual int rhs(double *out, int cap) = 0;
+  virtual void setPressure(int mode, double tol, int maxCycles) = 0;
+  virtual void pressureReport(int *cycles, double *residual) = 0;
   virtual int render(const fp_camera &cam, const fp_render_settings &rs, float *rgba) = 0;
   virtual int readRender(fp_render_tables *t) = 0;
   virtual int probe(const fp_camera &cam, const fp_render_settings &rs, int what, const float *in, float *out) = 0;
@@ -161,6 +163,17 @@ template <class R> struct Impl : Base {
   }
   void readActivity(uint32_t *flags, int n) override {
     for (int i = 0; i < n && i < (int)sim.activity.size(); i++) flags[i] = sim.activity[i];
+  }
+
+  void setPressure(int mode, double tol, int maxCycles) override {
+    if (mode < 0 || mode > 3) throw oracle::Error("pressure mode must be 0 to 3.");
+    sim.pressureMode = mode;
+    sim.pressureTol = tol;
+    sim.pressureMaxCycles = maxCycles;
+  }
+  void pressureReport(int *cycles, double *residual) override {
+    if (cycles) *cycles = sim.lastCycles;
+    if (residual) *residual = sim.lastResidual;
   }
 
   // ---- render (single precision only) ----
@@ -307,11 +320,22 @@ int orc_read_activity(void *h, uint32_t *flags, int n) {
   return FP_OK;
 }
 float orc_fexp(float x) { return oracle::fexp(x); }
+// The multigrid storage rounding: 2 bfloat16, 3 binary16, else none.
+float orc_round16(int mode, float x) {
+  return mode == 2 ? oracle::roundBF16(x) : mode == 3 ? oracle::roundF16(x) : x;
+}
 int orc_tile_tables(void *h, int32_t *records, int32_t *hash, int cap, int hashCap) {
   return ((Base *)h)->tileTables(records, hash, cap, hashCap);
 }
 int orc_brick_coords(void *h, const int32_t *ids, int n, int32_t *out) { return ((Base *)h)->brickCoords(ids, n, out); }
 int orc_rhs(void *h, double *out, int cap) { return ((Base *)h)->rhs(out, cap); }
+int orc_set_pressure(void *h, int mode, double tol, int max_cycles) {
+  return guard([&] { ((Base *)h)->setPressure(mode, tol, max_cycles); });
+}
+int orc_pressure_report(void *h, int *cycles, double *residual) {
+  ((Base *)h)->pressureReport(cycles, residual);
+  return FP_OK;
+}
 int orc_render(void *h, const fp_camera *cam, const fp_render_settings *rs, float *rgba) {
   return guard([&] { ((Base *)h)->render(*cam, *rs, rgba); });
 }

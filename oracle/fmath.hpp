@@ -67,6 +67,38 @@ inline float fexp2(float x) {
   return p * s;
 }
 
+// Storage rounding of the mixed-precision multigrid: float to bfloat16 and to IEEE
+// binary16, round to nearest even (binary16 subnormals kept, overflow to infinity), and
+// back to float.
+inline float roundBF16(float v) {
+  uint32_t b;
+  std::memcpy(&b, &v, 4);
+  b = (b + 0x7fffu + ((b >> 16) & 1u)) & 0xffff0000u;
+  std::memcpy(&v, &b, 4);
+  return v;
+}
+inline float roundF16(float v) {
+  uint32_t b;
+  std::memcpy(&b, &v, 4);
+  uint32_t sign = b & 0x80000000u, mag = b & 0x7fffffffu;
+  float out;
+  if (mag >= 0x477ff000u) {  // 65520 and up round to infinity
+    uint32_t inf = sign | 0x7f800000u;
+    std::memcpy(&out, &inf, 4);
+    return out;
+  }
+  if (mag < 0x38800000u) {  // below 2^-14: multiples of 2^-24
+    float a = std::fabs(v) * 16777216.0f;
+    float n = std::nearbyint(a);  // round to nearest even
+    out = n * 5.9604644775390625e-08f;
+    return sign ? -out : out;
+  }
+  mag = (mag + 0xfffu + ((mag >> 13) & 1u)) & ~0x1fffu;
+  uint32_t r = sign | mag;
+  std::memcpy(&out, &r, 4);
+  return out;
+}
+
 inline float fsqrt(float x) { return std::sqrt(x); }
 inline double fsqrt(double x) { return std::sqrt(x); }
 template <class R> inline R ffloor(R x) { return std::floor(x); }

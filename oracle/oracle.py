@@ -55,6 +55,7 @@ def load():
         "orc_write_activity": (C.c_int, [v, P(C.c_uint32), C.c_int, P(C.c_int32), C.c_int]),
         "orc_read_activity": (C.c_int, [v, P(C.c_uint32), C.c_int]),
         "orc_fexp": (C.c_float, [C.c_float]),
+        "orc_round16": (C.c_float, [C.c_int, C.c_float]),
         "orc_tile_tables": (C.c_int, [v, P(C.c_int32), P(C.c_int32), C.c_int, C.c_int]),
         "orc_brick_coords": (C.c_int, [v, P(C.c_int32), C.c_int, P(C.c_int32)]),
         "orc_rhs": (C.c_int, [v, P(C.c_double), C.c_int]),
@@ -63,6 +64,8 @@ def load():
         "orc_render_probe": (C.c_int, [v, P(abi.Camera), P(abi.RenderSettings), C.c_int,
                                        P(C.c_float), P(C.c_float)]),
         "orc_blackbody": (None, [P(C.c_float)]),
+        "orc_set_pressure": (C.c_int, [v, C.c_int, C.c_double, C.c_int]),
+        "orc_pressure_report": (C.c_int, [v, P(C.c_int), P(C.c_double)]),
         "orc_ramp_integral": (C.c_double, [P(C.c_double), P(C.c_int), C.c_int, C.c_double, C.c_double]),
         "orc_evaluate_flame": (None, [P(C.c_double), P(C.c_int)] + [C.c_double] * 6 + [P(C.c_double)]),
         "orc_emitter_weight": (C.c_double, [P(C.c_double), P(C.c_double), C.c_int, P(C.c_double)]),
@@ -206,6 +209,14 @@ class OracleSimulation:
         c = self._lib.orc_rhs(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), n)
         return out[:c]
 
+    def set_pressure(self, mode: int, tolerance: float = 1e-6, max_cycles: int = 20) -> None:
+        _check(self._lib.orc_set_pressure(self._h, mode, tolerance, max_cycles), self._lib)
+
+    def pressure_report(self):
+        cycles, res = C.c_int(), C.c_double()
+        self._lib.orc_pressure_report(self._h, C.byref(cycles), C.byref(res))
+        return cycles.value, res.value
+
     def render(self, camera, settings=None, image=True):
         rs = settings if settings is not None else abi.render_settings()
         img = np.zeros((camera.height, camera.width, 4), np.float32)
@@ -282,3 +293,8 @@ def blackbody() -> np.ndarray:
     out = np.zeros((512, 4), np.float32)
     load().orc_blackbody(out.ctypes.data_as(C.POINTER(C.c_float)))
     return out
+
+
+def round16(mode: int, x: float) -> float:
+    """The multigrid storage rounding of the restatement: 2 bfloat16, 3 binary16."""
+    return load().orc_round16(mode, float(x))

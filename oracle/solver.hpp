@@ -29,6 +29,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../include/firepro.h"
@@ -321,6 +322,22 @@ template <class R> class Sim {
     std::vector<uint8_t> topo;
   };
   std::vector<Level> levels;
+  // Pressure policy (no counterpart in the reference): 0 one cold V-cycle; 1, 2, 3 defect
+  // correction with V-cycle levels stored in FP32, BF16, FP16 (storage rounding applies to
+  // the float restatement) and the coarsest level in FP64.
+  int pressureMode = 0;
+  double pressureTol = 1e-6;
+  int pressureMaxCycles = 20;
+  int lastCycles = 1;
+  double lastResidual = -1;
+  std::vector<R> dcX;
+  R stored(R v) const {
+    if constexpr (std::is_same<R, float>::value) {
+      if (pressureMode == 2) return roundBF16(v);
+      if (pressureMode == 3) return roundF16(v);
+    }
+    return v;
+  }
 
   // Dense directory view for cell lookups: each brick's entry in the tiles' bounding box.
   int glo[3] = {0, 0, 0}, gdim[3] = {0, 0, 0};
@@ -1934,7 +1951,7 @@ template <class R> class Sim {
       R sum, diag;
       stencil(lv.p, p, flags, L, sum, diag);
       R cand = (sum - lv.rhs[a]) / fmax(diag, R(1e-12));
-      lv.q[a] = mixv(lv.p[a], cand, R(0.8));
+      lv.q[a] = stored(mixv(lv.p[a], cand, R(0.8)));
     });
     std::swap(lv.p, lv.q);
   }
@@ -1950,7 +1967,7 @@ template <class R> class Sim {
         R diag = w * openCount(flags, 0) + w * openCount(flags, 2) + w * openCount(flags, 4);
         first = mixv(R(0), (R(0) - lv.rhs[a]) / fmax(diag, R(1e-12)), R(0.8));
       }
-      lv.p[a] = first;
+      lv.p[a] = stored(first);
     });
   }
   // restrictCell (pressure.wgsl:601-647): coarse rhs and its first iterate.
@@ -1975,7 +1992,7 @@ template <class R> class Sim {
           }
       R rhs = residual * R(0.5);
       long out = addr(id, L + 1);
-      coarse.rhs[out] = rhs;
+      coarse.rhs[out] = stored(rhs);
       R first = 0;
       if (every) {
         int flags = flagsOf(id, L + 1);
@@ -1985,7 +2002,7 @@ template <class R> class Sim {
           first = mixv(R(0), (R(0) - rhs) / fmax(diag, R(1e-12)), R(0.8));
         }
       }
-      coarse.p[out] = first;
+      coarse.p[out] = stored(first);
     });
   }
   bool coarseSolid(const int c[3], int L) const {
@@ -2038,13 +2055,65 @@ template <class R> class Sim {
       };
       R corr = mixv(mixv(mixv(at(0, 0, 0), at(1, 0, 0), f[0]), mixv(at(0, 1, 0), at(1, 1, 0), f[0]), f[1]),
                     mixv(mixv(at(0, 0, 1), at(1, 0, 1), f[0]), mixv(at(0, 1, 1), at(1, 1, 1), f[0]), f[1]), f[2]);
-      lv.q[a] = lv.p[a] + corr;
+      lv.q[a] = stored(lv.p[a] + corr);
     });
     std::swap(lv.p, lv.q);
   }
+  // The coarsest level's weighted Jacobi sweeps in FP64 (one cell per tile slot), the
+  // result rounded to FP32 for the float restatement, then stored.
+  void relaxBottom64(int iterations) {
+    Level &lv = levels[bottom];
+    size_t n = lv.p.size();
+    std::vector<double> val(n, 0.0), next(n, 0.0);
+    std::vector<char> live(n, 0);
+    std::vector<int> flg(n, 0);
+    forLevelCells(bottom, [&](const int p[3]) {
+      long a = addr(p, bottom);
+      int flags = flagsOf(p, bottom);
+      bool unknown = activeCell(p, bottom);
+      live[a] = unknown && !(flags & 1024);
+      flg[a] = flags & 63;
+      val[a] = unknown ? (double)lv.p[a] : 0.0;
+    });
+    const double w = (double)axisWeight();
+    static const int OFF[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (int it = 0; it < iterations; it++) {
+      forLevelCells(bottom, [&](const int p[3]) {
+        long a = addr(p, bottom);
+        if (!live[a]) {
+          next[a] = 0.0;
+          return;
+        }
+        double nb[6];
+        for (int sd = 0; sd < 6; sd++) {
+          nb[sd] = 0.0;
+          if (!((flg[a] >> sd) & 1)) continue;
+          int q[3] = {p[0] + OFF[sd][0], p[1] + OFF[sd][1], p[2] + OFF[sd][2]};
+          long b = addr(q, bottom);
+          if (b >= 0) nb[sd] = val[b];
+        }
+        double sum = w * (nb[0] + nb[1]) + w * (nb[2] + nb[3]) + w * (nb[4] + nb[5]);
+        double diag = w * (double)openCount(flg[a], 0) + w * (double)openCount(flg[a], 2) +
+                      w * (double)openCount(flg[a], 4);
+        double cand = (sum - (double)lv.rhs[a]) / (diag < 1e-12 ? 1e-12 : diag);
+        next[a] = val[a] * (1.0 - 0.8) + cand * 0.8;
+      });
+      forLevelCells(bottom, [&](const int p[3]) {
+        long a = addr(p, bottom);
+        val[a] = next[a];
+      });
+    }
+    forLevelCells(bottom, [&](const int p[3]) {
+      long a = addr(p, bottom);
+      if constexpr (std::is_same<R, float>::value) lv.p[a] = stored((float)val[a]);
+      else lv.p[a] = (R)val[a];
+    });
+  }
   void cycle(int L) {
     if (L == bottom) {
-      for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax(L);
+      if (pressureMode != 0) relaxBottom64(BOTTOM_ITERATIONS);
+      else
+        for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax(L);
       return;
     }
     if (L == 0) initialize0();
@@ -2070,8 +2139,79 @@ template <class R> class Sim {
     }
     // Fresh slots' topology, every step (PressureSolver.ts:291-302).
     for (auto &z : zeroList) brickTopology(z.first);
-    cycle(0);
-    Pfine = levels[0].p;
+    if (pressureMode == 0) {
+      cycle(0);
+      Pfine = levels[0].p;
+      lastCycles = 1;
+      lastResidual = -1;
+      return;
+    }
+    correct();
+    Pfine = dcX;
+  }
+  // Defect correction: x = 0; r = b - A x (sums of r^2 and b^2 in FP64); while
+  // |r| / |b| > tol, one V-cycle on A e = r / s, s the largest power of two not above
+  // max |r| (1 for zero or subnormal residuals), and x += s e.
+  void correct() {
+    const std::vector<R> b = D;
+    dcX.assign(b.size(), R(0));
+    std::vector<R> r(b.size(), R(0));
+    lastCycles = 0;
+    lastResidual = 1;
+    for (int it = 0; it <= pressureMaxCycles; it++) {
+      double rr = 0, bb = 0;
+      R big = 0;
+      forLevelCells(0, [&](const int p[3]) {
+        long a = addr(p, 0);
+        int flags = flagsOf(p, 0);
+        R v = 0;
+        if (!(flags & 1024) && activeCell(p, 0)) {
+          // The row's dot product in FP64 from the stored x, rounded once.
+          static const int OFF[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+          const double w = (double)axisWeight();
+          double sum = 0;
+          for (int k = 0; k < 6; k++) {
+            if (!((flags >> k) & 1)) continue;
+            int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
+            sum += w * (double)readAt(dcX, q, 0);
+          }
+          R w1 = axisWeight();
+          R diag = w1 * openCount(flags, 0) + w1 * openCount(flags, 2) + w1 * openCount(flags, 4);
+          v = (R)((double)b[a] - (sum - (double)diag * (double)dcX[a]));
+          rr += (double)v * (double)v;
+          bb += (double)b[a] * (double)b[a];
+          big = fmax(big, fabs_(v));
+        }
+        r[a] = v;
+      });
+      lastResidual = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+      if (lastResidual <= pressureTol || it == pressureMaxCycles) break;
+      R scale = 1;
+      if constexpr (std::is_same<R, float>::value) {
+        uint32_t bits;
+        float m = big;
+        std::memcpy(&bits, &m, 4);
+        bits &= 0x7f800000u;
+        if (bits) std::memcpy(&scale, &bits, 4);
+      } else {
+        int e;
+        std::frexp(big, &e);
+        scale = big > 0 ? std::ldexp(1.0, e - 1) : 1.0;
+      }
+      R inv = R(1) / scale;
+      Level &l0 = levels[0];
+      forLevelCells(0, [&](const int p[3]) {
+        long a = addr(p, 0);
+        l0.rhs[a] = stored(r[a] * inv);
+      });
+      cycle(0);
+      forLevelCells(0, [&](const int p[3]) {
+        long a = addr(p, 0);
+        int flags = flagsOf(p, 0);
+        dcX[a] = (!(flags & 1024) && activeCell(p, 0)) ? dcX[a] + scale * levels[0].p[a] : R(0);
+      });
+      lastCycles = it + 1;
+    }
   }
 
   // project (fluid-forces.wgsl:205-262).

@@ -38,14 +38,35 @@ struct StepArgs {
   const Region *regions;
 };
 
+// Storage of the multigrid levels' iterates and right-hand sides. Arithmetic is FP32 in
+// every case; values are rounded to nearest even when stored.
+struct StoreF32 {
+  using T = float;
+  static __device__ __forceinline__ float ld(float v) { return v; }
+  static __device__ __forceinline__ float st(float v) { return v; }
+};
+struct StoreBF16 {
+  using T = uint16_t;  // bfloat16 bits
+  static __device__ __forceinline__ float ld(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+  static __device__ __forceinline__ uint16_t st(float v) {
+    uint32_t b = __float_as_uint(v);
+    return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+  }
+};
+struct StoreF16 {
+  using T = _Float16;
+  static __device__ __forceinline__ float ld(_Float16 v) { return (float)v; }
+  static __device__ __forceinline__ _Float16 st(float v) { return (_Float16)v; }
+};
+
 // Multigrid level storage: brick levels per pool slot, tile levels per tile slot.
-struct Level {
-  float *p, *q, *rhs;
+template <class T> struct LevelT {
+  T *p, *q, *rhs;
   uint8_t *topo, *eflag;
   int side, tile;
 };
 constexpr int MAX_LEVELS = 8;
-struct MGArgs {
+template <class T> struct MGArgsT {
   Dir dir;
   Layout L;
   Params P;
@@ -53,10 +74,12 @@ struct MGArgs {
   const int *active;
   const int *zeroList;
   const uint32_t *solids;
-  Level lv[MAX_LEVELS];
+  LevelT<T> lv[MAX_LEVELS];
   int bottom;
   float weight;  // axisWeights, equal on every axis for cubic cells
 };
+using Level = LevelT<float>;
+using MGArgs = MGArgsT<float>;
 
 __global__ void k_build_bricks(AllocArgs a);
 __global__ void k_scan_blocks(AllocArgs a);
@@ -80,14 +103,25 @@ __global__ void k_topology_bricks(MGArgs m, int fromZeroList);
 __global__ void k_topology_tiles(MGArgs m, int level);
 __global__ void k_eflags_bricks(MGArgs m);
 __global__ void k_eflags_tiles(MGArgs m, int level);
-__global__ void k_mg_init(MGArgs m);
-__global__ void k_mg_relax_bricks(MGArgs m, int level);
-__global__ void k_mg_relax_tiles(MGArgs m, int level);
-__global__ void k_mg_restrict_bricks(MGArgs m, int level);
-__global__ void k_mg_restrict_tiles(MGArgs m, int level);
-__global__ void k_mg_prolong_bricks(MGArgs m, int level);
-__global__ void k_mg_prolong_tiles(MGArgs m, int level);
-__global__ void k_mg_bottom(MGArgs m, int iterations);
+template <class St> __global__ void k_mg_init(MGArgsT<typename St::T> m);
+template <class St> __global__ void k_mg_relax_bricks(MGArgsT<typename St::T> m, int level);
+template <class St> __global__ void k_mg_relax_tiles(MGArgsT<typename St::T> m, int level);
+template <class St> __global__ void k_mg_restrict_bricks(MGArgsT<typename St::T> m, int level);
+template <class St> __global__ void k_mg_restrict_tiles(MGArgsT<typename St::T> m, int level);
+template <class St> __global__ void k_mg_prolong_bricks(MGArgsT<typename St::T> m, int level);
+template <class St> __global__ void k_mg_prolong_tiles(MGArgsT<typename St::T> m, int level);
+template <class St> __global__ void k_mg_bottom(MGArgsT<typename St::T> m, int iterations);
+// The coarsest level in FP64 (any tile count up to BOTTOM64_TILES), stored back rounded.
+constexpr int BOTTOM64_TILES = 8192;
+template <class St> __global__ void k_mg_bottom64(MGArgsT<typename St::T> m, int iterations);
+// Defect correction: r = b - A x (FP32) with per-brick FP64 sums of r^2, b^2 and the
+// largest |r|; their totals and a power-of-two scale; level 0's right-hand side r / scale;
+// x += scale * (level 0's solution).
+template <class St> __global__ void k_dc_residual(MGArgsT<typename St::T> m, const float *b, const float *x, float *r,
+                                                  double *partials);
+__global__ void k_dc_norms(const double *partials, const int *counts, double *out);
+template <class St> __global__ void k_dc_scale(MGArgsT<typename St::T> m, const float *r, const double *norms);
+template <class St> __global__ void k_dc_update(MGArgsT<typename St::T> m, float *x, const double *norms);
 
 // The render's parameters (volume-common.wgsl RenderParams, packed at
 // SceneVolumeRenderer.ts:329-345, and the camera uniforms of :440-456).

@@ -207,9 +207,10 @@ __global__ void __launch_bounds__(64) k_eflags_tiles(MGArgs m, int lv) {
   }
 }
 
-__device__ __forceinline__ float readAt(const MGCtx &c, const float *v, const int p[3], int lv) {
+template <class St>
+__device__ __forceinline__ float readAt(const MGCtx &c, const typename St::T *v, const int p[3], int lv) {
   long a = addrAt(c, p, lv);
-  return a < 0 ? 0.f : v[a];
+  return a < 0 ? 0.f : St::ld(v[a]);
 }
 __device__ __forceinline__ float openCount(int flags, int sd) {
   return (float)((flags >> sd) & 1) + (float)((flags >> (sd + 1)) & 1);
@@ -218,7 +219,9 @@ __device__ __forceinline__ float diagonal(float w, int flags) {
   return w * openCount(flags, 0) + w * openCount(flags, 2) + w * openCount(flags, 4);
 }
 // stencil (pressure.wgsl:376-402): the neighbor sum over open sides.
-__device__ __forceinline__ float stencilSum(const MGCtx &c, const float *pv, const int p[3], int flags, int lv, float w) {
+template <class St>
+__device__ __forceinline__ float stencilSum(const MGCtx &c, const typename St::T *pv, const int p[3], int flags, int lv,
+                                            float w) {
   float n[6];
   for (int sd = 0; sd < 6; sd++) {
     if (!((flags >> sd) & 1)) {
@@ -226,72 +229,76 @@ __device__ __forceinline__ float stencilSum(const MGCtx &c, const float *pv, con
       continue;
     }
     int q[3] = {p[0] + OFF[sd][0], p[1] + OFF[sd][1], p[2] + OFF[sd][2]};
-    n[sd] = readAt(c, pv, q, lv);
+    n[sd] = readAt<St>(c, pv, q, lv);
   }
   return w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
 }
 
 // relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8.
-__device__ __forceinline__ void relaxCell(const MGCtx &c, const Level &lv, int level, const int p[3], size_t a, float w) {
+template <class St>
+__device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelT<typename St::T> &lv, int level, const int p[3],
+                                          size_t a, float w) {
   int e = lv.eflag[a];
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
-    lv.q[a] = 0.f;
+    lv.q[a] = St::st(0.f);
     return;
   }
   int flags = e & 63;
-  float sum = stencilSum(c, lv.p, p, flags, level, w);
-  float cand = (sum - lv.rhs[a]) / fmaxv(diagonal(w, flags), 1e-12f);
-  lv.q[a] = mixv(lv.p[a], cand, 0.8f);
+  float sum = stencilSum<St>(c, lv.p, p, flags, level, w);
+  float cand = (sum - St::ld(lv.rhs[a])) / fmaxv(diagonal(w, flags), 1e-12f);
+  lv.q[a] = St::st(mixv(St::ld(lv.p[a]), cand, 0.8f));
 }
 
-__global__ void __launch_bounds__(256) k_mg_init(MGArgs m) {
+template <class St> __global__ void __launch_bounds__(256) k_mg_init(MGArgsT<typename St::T> m) {
   const int bi = xcdItem(m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
-  const Level &lv = m.lv[0];
+  const auto &lv = m.lv[0];
   int sd = lv.side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     size_t a = (size_t)slot * n + l;
     int e = lv.eflag[a];
     float first = 0.f;
     if (!(e & STORED_SOLID) && (e & UNKNOWN))
-      first = mixv(0.f, (0.f - lv.rhs[a]) / fmaxv(diagonal(m.weight, e & 63), 1e-12f), 0.8f);
-    lv.p[a] = first;
+      first = mixv(0.f, (0.f - St::ld(lv.rhs[a])) / fmaxv(diagonal(m.weight, e & 63), 1e-12f), 0.8f);
+    lv.p[a] = St::st(first);
   }
 }
 
-__global__ void __launch_bounds__(256) k_mg_relax_bricks(MGArgs m, int level) {
+template <class St> __global__ void __launch_bounds__(256) k_mg_relax_bricks(MGArgsT<typename St::T> m, int level) {
   const int bi = xcdItem(m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
-  const Level &lv = m.lv[level];
+  const auto &lv = m.lv[level];
   int sd = lv.side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-    relaxCell(c, lv, level, p, (size_t)slot * n + l, m.weight);
+    relaxCell<St>(c, lv, level, p, (size_t)slot * n + l, m.weight);
   }
 }
-__global__ void __launch_bounds__(64) k_mg_relax_tiles(MGArgs m, int level) {
+template <class St> __global__ void __launch_bounds__(64) k_mg_relax_tiles(MGArgsT<typename St::T> m, int level) {
   const int t = blockIdx.x;
   MG_TILE_HOME(t)
-  const Level &lv = m.lv[level];
+  const auto &lv = m.lv[level];
   int sd = lv.side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    relaxCell(c, lv, level, p, (size_t)t * n + l, m.weight);
+    relaxCell<St>(c, lv, level, p, (size_t)t * n + l, m.weight);
   }
 }
 
 // restrictCell (pressure.wgsl:601-647): the coarse right-hand side is half the sum of the
 // eight children's residuals; the coarse first iterate is one Jacobi step from zero.
-__device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgs &m, int level, const int id[3]) {
-  const Level &fine = m.lv[level - 1], &coarse = m.lv[level];
+template <class St>
+__device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, const int id[3]) {
+  const auto &fine = m.lv[level - 1];
+  const auto &coarse = m.lv[level];
   float residual = 0.f;
   bool every = true;
   for (int z = 0; z < 2; z++)
@@ -305,22 +312,22 @@ __device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgs &m, in
         if (!unknown) continue;
         if (e & STORED_SOLID) continue;
         int flags = e & 63;
-        float sum = stencilSum(c, fine.p, p, flags, level - 1, m.weight);
+        float sum = stencilSum<St>(c, fine.p, p, flags, level - 1, m.weight);
         float dg = diagonal(m.weight, flags);
-        residual += fine.rhs[a] - (sum - dg * fine.p[a]);
+        residual += St::ld(fine.rhs[a]) - (sum - dg * St::ld(fine.p[a]));
       }
   float rhs = residual * 0.5f;
   long out = addrAt(c, id, level);
-  coarse.rhs[out] = rhs;
+  coarse.rhs[out] = St::st(rhs);
   float first = 0.f;
   if (every) {
     int e = coarse.eflag[out];
     if (!(e & STORED_SOLID)) first = mixv(0.f, (0.f - rhs) / fmaxv(diagonal(m.weight, e & 63), 1e-12f), 0.8f);
   }
-  coarse.p[out] = first;
+  coarse.p[out] = St::st(first);
 }
 
-__global__ void __launch_bounds__(64) k_mg_restrict_bricks(MGArgs m, int level) {
+template <class St> __global__ void __launch_bounds__(64) k_mg_restrict_bricks(MGArgsT<typename St::T> m, int level) {
   const int bi = xcdItem(m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
@@ -329,39 +336,40 @@ __global__ void __launch_bounds__(64) k_mg_restrict_bricks(MGArgs m, int level) 
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-    restrictCell(c, m, level, p);
+    restrictCell<St>(c, m, level, p);
   }
 }
-__global__ void __launch_bounds__(64) k_mg_restrict_tiles(MGArgs m, int level) {
+template <class St> __global__ void __launch_bounds__(64) k_mg_restrict_tiles(MGArgsT<typename St::T> m, int level) {
   const int t = blockIdx.x;
   MG_TILE_HOME(t)
   int sd = m.lv[level].side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    restrictCell(c, m, level, p);
+    restrictCell<St>(c, m, level, p);
   }
 }
 
 // coarseValue (pressure.wgsl:670-691): Neumann ghosts beside solids, a mirror below the floor.
-__device__ __forceinline__ bool coarseSolid(const MGCtx &c, const Level &lv, const int q[3], int level) {
+__device__ __forceinline__ bool coarseSolid(const MGCtx &c, const uint8_t *topo, const int q[3], int level) {
   long a = addrAt(c, q, level);
   if (a < 0) return false;
-  return (lv.topo[a] & STORED_SOLID) != 0;
+  return (topo[a] & STORED_SOLID) != 0;
 }
-__device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgs &m, const int p[3], int level) {
-  const Level &cv = m.lv[level];
+template <class St>
+__device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typename St::T> &m, const int p[3], int level) {
+  const auto &cv = m.lv[level];
   const int groundY = m.P.ground ? 0 : (int)0x80000000;
   int cc[3] = {p[0], p[1] > groundY ? p[1] : groundY, p[2]};
-  if (!coarseSolid(c, cv, cc, level)) return readAt(c, cv.p, cc, level);
+  if (!coarseSolid(c, cv.topo, cc, level)) return readAt<St>(c, cv.p, cc, level);
   float sum = 0.f, count = 0.f;
   for (int axis = 0; axis < 3; axis++)
     for (int sd = -1; sd <= 1; sd += 2) {
       int q[3] = {cc[0], cc[1], cc[2]};
       q[axis] += sd;
       q[1] = q[1] > groundY ? q[1] : groundY;
-      if (!coarseSolid(c, cv, q, level)) {
-        sum += readAt(c, cv.p, q, level);
+      if (!coarseSolid(c, cv.topo, q, level)) {
+        sum += readAt<St>(c, cv.p, q, level);
         count += 1.0f;
       }
     }
@@ -369,11 +377,13 @@ __device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgs &m, co
 }
 
 // prolongateCell (pressure.wgsl:726-755): trilinear correction from the coarse level.
-__device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgs &m, int level, const int p[3], size_t a) {
-  const Level &lv = m.lv[level];
+template <class St>
+__device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, const int p[3],
+                                            size_t a) {
+  const auto &lv = m.lv[level];
   int e = lv.eflag[a];
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
-    lv.q[a] = 0.f;
+    lv.q[a] = St::st(0.f);
     return;
   }
   float f[3];
@@ -388,14 +398,14 @@ __device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgs &m, int
   float v[8];
   for (int k = 0; k < 8; k++) {
     int cc[3] = {i[0] + (k & 1), i[1] + ((k >> 1) & 1), i[2] + (k >> 2)};
-    v[k] = plain ? readAt(c, m.lv[level + 1].p, cc, level + 1) : coarseValue(c, m, cc, level + 1);
+    v[k] = plain ? readAt<St>(c, m.lv[level + 1].p, cc, level + 1) : coarseValue<St>(c, m, cc, level + 1);
   }
   float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
                     mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
-  lv.q[a] = lv.p[a] + corr;
+  lv.q[a] = St::st(St::ld(lv.p[a]) + corr);
 }
 
-__global__ void __launch_bounds__(256) k_mg_prolong_bricks(MGArgs m, int level) {
+template <class St> __global__ void __launch_bounds__(256) k_mg_prolong_bricks(MGArgsT<typename St::T> m, int level) {
   const int bi = xcdItem(m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
@@ -405,35 +415,35 @@ __global__ void __launch_bounds__(256) k_mg_prolong_bricks(MGArgs m, int level) 
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-    prolongCell(c, m, level, p, (size_t)slot * n + l);
+    prolongCell<St>(c, m, level, p, (size_t)slot * n + l);
   }
 }
-__global__ void __launch_bounds__(64) k_mg_prolong_tiles(MGArgs m, int level) {
+template <class St> __global__ void __launch_bounds__(64) k_mg_prolong_tiles(MGArgsT<typename St::T> m, int level) {
   const int t = blockIdx.x;
   MG_TILE_HOME(t)
   int sd = m.lv[level].side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    prolongCell(c, m, level, p, (size_t)t * n + l);
+    prolongCell<St>(c, m, level, p, (size_t)t * n + l);
   }
 }
 
 // relaxCoarse (pressure.wgsl:549-599): every bottom iteration in one workgroup, a tile slot
 // per lane, values in LDS. The result lands in lv.p.
 constexpr int BOTTOM_LANES = 1024;
-__global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom(MGArgs m, int iterations) {
+template <class St> __global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom(MGArgsT<typename St::T> m, int iterations) {
   __shared__ float value[BOTTOM_LANES];
   const int lane = threadIdx.x;
-  const Level &lv = m.lv[m.bottom];
+  const auto &lv = m.lv[m.bottom];
   const int cap = m.dir.capacity;
   const int *trec = m.dir.tileRec + (size_t)lane * TILE_RECORD_WORDS;
   bool valid = lane < cap && trec[3] != 0;
   int e = valid ? lv.eflag[lane] : 0;
   bool live = valid && (e & UNKNOWN) && !(e & STORED_SOLID);
   int flags = e & 63;
-  float rhs = valid ? lv.rhs[lane] : 0.f;
-  float mine = valid ? lv.p[lane] : 0.f;
+  float rhs = valid ? St::ld(lv.rhs[lane]) : 0.f;
+  float mine = valid ? St::ld(lv.p[lane]) : 0.f;
   if (valid && !(e & UNKNOWN)) mine = 0.f;
   value[lane] = mine;
   __syncthreads();
@@ -457,7 +467,205 @@ __global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom(MGArgs m, int iterat
     mine = next;
     __syncthreads();
   }
-  if (valid) lv.p[lane] = mine;
+  if (valid) lv.p[lane] = St::st(mine);
 }
+
+// The coarsest level's sweeps in FP64: the same weighted Jacobi as relaxCoarse, one tile
+// slot per lane and up to BOTTOM64_TILES / 1024 slots per lane, values in LDS. The result
+// is rounded to FP32, then stored.
+__device__ __forceinline__ double mix64(double a, double b, double t) { return a * (1.0 - t) + b * t; }
+template <class St> __global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom64(MGArgsT<typename St::T> m, int iterations) {
+  constexpr int PER = BOTTOM64_TILES / BOTTOM_LANES;
+  __shared__ double value[BOTTOM64_TILES];
+  const auto &lv = m.lv[m.bottom];
+  const int cap = m.dir.capacity;
+  const double w = (double)m.weight;
+  double mine[PER], rhs[PER];
+  int flags[PER];
+  bool live[PER];
+  for (int k = 0; k < PER; k++) {
+    const int t = threadIdx.x + k * BOTTOM_LANES;
+    bool valid = t < cap && m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] != 0;
+    int e = valid ? lv.eflag[t] : 0;
+    live[k] = valid && (e & UNKNOWN) && !(e & STORED_SOLID);
+    flags[k] = e & 63;
+    rhs[k] = valid ? (double)St::ld(lv.rhs[t]) : 0.0;
+    mine[k] = valid && (e & UNKNOWN) ? (double)St::ld(lv.p[t]) : 0.0;
+    if (t < cap) value[t] = mine[k];
+  }
+  __syncthreads();
+  for (int it = 0; it < iterations; it++) {
+    double next[PER];
+    for (int k = 0; k < PER; k++) {
+      const int t = threadIdx.x + k * BOTTOM_LANES;
+      next[k] = 0.0;
+      if (!live[k]) continue;
+      const int *trec = m.dir.tileRec + (size_t)t * TILE_RECORD_WORDS;
+      double n[6];
+      for (int sd = 0; sd < 6; sd++) {
+        n[sd] = 0.0;
+        if (!((flags[k] >> sd) & 1)) continue;
+        int nb = trec[4 + (OFF[sd][2] + 1) * 9 + (OFF[sd][1] + 1) * 3 + (OFF[sd][0] + 1)];
+        if (nb >= 0) n[sd] = value[nb];
+      }
+      double sum = w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
+      double diag = w * (double)openCount(flags[k], 0) + w * (double)openCount(flags[k], 2) +
+                    w * (double)openCount(flags[k], 4);
+      double cand = (sum - rhs[k]) / (diag < 1e-12 ? 1e-12 : diag);
+      next[k] = mix64(mine[k], cand, 0.8);
+    }
+    __syncthreads();
+    for (int k = 0; k < PER; k++) {
+      const int t = threadIdx.x + k * BOTTOM_LANES;
+      mine[k] = next[k];
+      if (t < cap) value[t] = next[k];
+    }
+    __syncthreads();
+  }
+  for (int k = 0; k < PER; k++) {
+    const int t = threadIdx.x + k * BOTTOM_LANES;
+    if (t < cap && m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] != 0) lv.p[t] = St::st((float)mine[k]);
+  }
+}
+
+// ---- defect correction -----------------------------------------------------------------
+// The level-0 operator of relaxCell as a residual: r = b - (sum - diag x) on unknown cells,
+// zero elsewhere, each row's dot product accumulated in FP64 from the FP32 x and rounded
+// to FP32; per-brick FP64 sums of r^2 and b^2 and the largest |r|.
+template <class St>
+__global__ void __launch_bounds__(256) k_dc_residual(MGArgsT<typename St::T> m, const float *b, const float *x,
+                                                      float *r, double *partials) {
+  const int bi = xcdItem(m.counts[0]);
+  if (bi < 0) return;
+  const int id = m.active[bi];
+  MG_BRICK_HOME(id)
+  __shared__ double red[3][256];
+  int slot = entrySlot(rec[13], c.side);
+  const auto &lv = m.lv[0];
+  int sd = lv.side, n = sd * sd * sd;
+  double rr = 0.0, bb = 0.0;
+  float big = 0.f;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    size_t a = (size_t)slot * n + l;
+    int e = lv.eflag[a];
+    float v = 0.f;
+    if (!(e & STORED_SOLID) && (e & UNKNOWN)) {
+      int p[3];
+      levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
+      int flags = e & 63;
+      const double w = (double)m.weight;
+      double sum = 0.0;
+      for (int k = 0; k < 6; k++) {
+        if (!((flags >> k) & 1)) continue;
+        int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
+        sum += w * (double)readAt<StoreF32>(c, x, q, 0);
+      }
+      float bv = b[a];
+      v = (float)((double)bv - (sum - (double)diagonal(m.weight, flags) * (double)x[a]));
+      rr += (double)v * (double)v;
+      bb += (double)bv * (double)bv;
+      big = fmaxv(big, fabsv(v));
+    }
+    r[a] = v;
+  }
+  red[0][threadIdx.x] = rr;
+  red[1][threadIdx.x] = bb;
+  red[2][threadIdx.x] = (double)big;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+      red[2][threadIdx.x] = red[2][threadIdx.x] < red[2][threadIdx.x + s] ? red[2][threadIdx.x + s] : red[2][threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[3 * bi] = red[0][0];
+    partials[3 * bi + 1] = red[1][0];
+    partials[3 * bi + 2] = red[2][0];
+  }
+}
+// The totals of the per-brick sums in a fixed order: out = (sum r^2, sum b^2, max |r|,
+// scale), the scale being the largest power of two not above max |r| (1 for zero or
+// subnormal residuals). Scaled by it, the correction equation's values lie in (-2, 2),
+// inside every storage format's range.
+__global__ void __launch_bounds__(256) k_dc_norms(const double *partials, const int *counts, double *out) {
+  __shared__ double red[3][256];
+  const int blocks = counts[0];
+  double rr = 0.0, bb = 0.0, big = 0.0;
+  for (int i = threadIdx.x; i < blocks; i += 256) {
+    rr += partials[3 * i];
+    bb += partials[3 * i + 1];
+    big = big < partials[3 * i + 2] ? partials[3 * i + 2] : big;
+  }
+  red[0][threadIdx.x] = rr;
+  red[1][threadIdx.x] = bb;
+  red[2][threadIdx.x] = big;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+      red[2][threadIdx.x] = red[2][threadIdx.x] < red[2][threadIdx.x + s] ? red[2][threadIdx.x + s] : red[2][threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = red[0][0];
+    out[1] = red[1][0];
+    out[2] = red[2][0];
+    uint32_t bits = __float_as_uint((float)red[2][0]) & 0x7f800000u;
+    out[3] = bits ? (double)__uint_as_float(bits) : 1.0;
+  }
+}
+// Level 0's right-hand side: r / scale, stored.
+template <class St>
+__global__ void __launch_bounds__(256) k_dc_scale(MGArgsT<typename St::T> m, const float *r, const double *norms) {
+  const int bi = xcdItem(m.counts[0]);
+  if (bi < 0) return;
+  const int id = m.active[bi];
+  const int slot = entrySlot(m.dir.entries[id], m.L.side);
+  const auto &lv = m.lv[0];
+  const float inv = 1.0f / (float)norms[3];
+  int n = lv.side * lv.side * lv.side;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    size_t a = (size_t)slot * n + l;
+    lv.rhs[a] = St::st(r[a] * inv);
+  }
+}
+// x += scale * e, level 0's solution of the scaled correction equation, on unknown cells.
+template <class St>
+__global__ void __launch_bounds__(256) k_dc_update(MGArgsT<typename St::T> m, float *x, const double *norms) {
+  const int bi = xcdItem(m.counts[0]);
+  if (bi < 0) return;
+  const int id = m.active[bi];
+  const int slot = entrySlot(m.dir.entries[id], m.L.side);
+  const auto &lv = m.lv[0];
+  const float scale = (float)norms[3];
+  int n = lv.side * lv.side * lv.side;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    size_t a = (size_t)slot * n + l;
+    int e = lv.eflag[a];
+    x[a] = (!(e & STORED_SOLID) && (e & UNKNOWN)) ? x[a] + scale * St::ld(lv.p[a]) : 0.f;
+  }
+}
+
+#define FP_MG_INSTANTIATE(St)                                                                             \
+  template __global__ void k_mg_init<St>(MGArgsT<St::T>);                                                  \
+  template __global__ void k_mg_relax_bricks<St>(MGArgsT<St::T>, int);                                     \
+  template __global__ void k_mg_relax_tiles<St>(MGArgsT<St::T>, int);                                      \
+  template __global__ void k_mg_restrict_bricks<St>(MGArgsT<St::T>, int);                                  \
+  template __global__ void k_mg_restrict_tiles<St>(MGArgsT<St::T>, int);                                   \
+  template __global__ void k_mg_prolong_bricks<St>(MGArgsT<St::T>, int);                                   \
+  template __global__ void k_mg_prolong_tiles<St>(MGArgsT<St::T>, int);                                    \
+  template __global__ void k_mg_bottom<St>(MGArgsT<St::T>, int);                                           \
+  template __global__ void k_mg_bottom64<St>(MGArgsT<St::T>, int);                                         \
+  template __global__ void k_dc_residual<St>(MGArgsT<St::T>, const float *, const float *, float *, double *); \
+  template __global__ void k_dc_scale<St>(MGArgsT<St::T>, const float *, const double *);                   \
+  template __global__ void k_dc_update<St>(MGArgsT<St::T>, float *, const double *);
+FP_MG_INSTANTIATE(StoreF32)
+FP_MG_INSTANTIATE(StoreBF16)
+FP_MG_INSTANTIATE(StoreF16)
 
 }  // namespace fp

@@ -204,6 +204,21 @@ struct fp_sim {
   };
   LevelBufs lv[MAX_LEVELS];
   float *latestPressure = nullptr;
+  // Pressure policy: 0 the reference's one cold V-cycle; 1-3 defect correction (an FP32
+  // outer loop to a relative residual) around V-cycles whose levels are stored in FP32,
+  // BF16 or FP16, with the coarsest level solved in FP64.
+  int pressureMode = 0;
+  double pressureTol = 1e-6;
+  int pressureMaxCycles = 20;
+  int lastCycles = 1;
+  double lastResidual = -1.0;
+  struct HalfLevel {
+    DBuf<uint16_t> p, q, rhs;
+  };
+  HalfLevel lv16[MAX_LEVELS];
+  DBuf<float> dcRhs, dcX, dcR;
+  DBuf<double> dcPartials, dcNorms;
+  double *pinnedNorms = nullptr;
   // Render: the boxes (updated with the tiles each step) and the preparation's state
   // carried between renders (VolumeLighting.ts:28-63).
   render::Boxes rboxes;
@@ -958,7 +973,8 @@ struct fp_sim {
     if (profiling) collect();
   }
 
-  // PressureSolver.solve (PressureSolver.ts:371-402): one cold-start V-cycle.
+  // PressureSolver.solve (PressureSolver.ts:371-402): one cold-start V-cycle, or the
+  // defect-correction loop around V-cycles.
   void solvePressure(unsigned slotsGrid) {
     MGArgs m = mgArgs();
     if (geometryDirty) {
@@ -969,36 +985,121 @@ struct fp_sim {
     k_topology_bricks<<<slotsGrid * 2, 256, 0, stream>>>(m, 1);
     k_eflags_bricks<<<slotsGrid, 256, 0, stream>>>(m);
     for (int L = brickLevels; L <= bottom; L++) k_eflags_tiles<<<capacity, 64, 0, stream>>>(m, L);
-    cycle(m, 0, slotsGrid);
-    latestPressure = m.lv[0].p;
-    // Keep the buffers' roles: level 0's iterate lives in P0 or P1.
-  }
-  void relax(MGArgs &m, int L, unsigned slotsGrid) {
-    if (L < brickLevels) k_mg_relax_bricks<<<slotsGrid, 256, 0, stream>>>(m, L);
-    else k_mg_relax_tiles<<<capacity, 64, 0, stream>>>(m, L);
-    std::swap(m.lv[L].p, m.lv[L].q);
-  }
-  void cycle(MGArgs &m, int L, unsigned slotsGrid) {
-    if (L == bottom) {
-      if (capacity <= 1024) k_mg_bottom<<<1, 1024, 0, stream>>>(m, 19);
-      else
-        for (int i = 0; i < 19; i++) relax(m, L, slotsGrid);
+    if (pressureMode == 0) {
+      cycle<StoreF32>(m, 0, slotsGrid, false);
+      latestPressure = m.lv[0].p;  // level 0's iterate lives in P0 or P1
+      lastCycles = 1;
+      lastResidual = -1.0;
       return;
     }
-    if (L == 0) k_mg_init<<<slotsGrid, 256, 0, stream>>>(m);
-    else {
-      relax(m, L, slotsGrid);
-      relax(m, L, slotsGrid);
+    ensureCorrectionBuffers();
+    if (pressureMode == 1) {
+      MGArgs mf = m;
+      mf.lv[0].rhs = dcRhs.p;
+      correct<StoreF32>(mf, slotsGrid);
+    } else if (pressureMode == 2) {
+      correct<StoreBF16>(halfArgs<uint16_t>(m), slotsGrid);
+    } else {
+      correct<StoreF16>(halfArgs<_Float16>(m), slotsGrid);
     }
-    if (L + 1 < brickLevels) k_mg_restrict_bricks<<<slotsGrid, 64, 0, stream>>>(m, L + 1);
-    else k_mg_restrict_tiles<<<capacity, 64, 0, stream>>>(m, L + 1);
-    cycle(m, L + 1, slotsGrid);
-    if (L < brickLevels) k_mg_prolong_bricks<<<slotsGrid, 256, 0, stream>>>(m, L);
-    else k_mg_prolong_tiles<<<capacity, 64, 0, stream>>>(m, L);
+    latestPressure = dcX.p;
+  }
+
+  // The level arrays in 16-bit storage, with the topology and flags of `m`.
+  template <class T> MGArgsT<T> halfArgs(const MGArgs &m) {
+    MGArgsT<T> h;
+    h.dir = m.dir;
+    h.L = m.L;
+    h.P = m.P;
+    h.counts = m.counts;
+    h.active = m.active;
+    h.zeroList = m.zeroList;
+    h.solids = m.solids;
+    h.bottom = m.bottom;
+    h.weight = m.weight;
+    for (int L = 0; L <= bottom; L++) {
+      h.lv[L].p = reinterpret_cast<T *>(lv16[L].p.p);
+      h.lv[L].q = reinterpret_cast<T *>(lv16[L].q.p);
+      h.lv[L].rhs = reinterpret_cast<T *>(lv16[L].rhs.p);
+      h.lv[L].topo = m.lv[L].topo;
+      h.lv[L].eflag = m.lv[L].eflag;
+      h.lv[L].side = m.lv[L].side;
+      h.lv[L].tile = m.lv[L].tile;
+    }
+    return h;
+  }
+
+  void ensureCorrectionBuffers() {
+    if (capacity > BOTTOM64_TILES) throw InvalidArgument("defect correction supports up to 8192 tile slots.");
+    size_t cells0 = (size_t)poolSlots * nV;
+    if (dcX.n != cells0) {
+      dcX.alloc(cells0);
+      dcR.alloc(cells0);
+      dcRhs.alloc(cells0);
+      dcPartials.alloc(3 * (size_t)std::max(1, poolSlots));
+    }
+    if (!dcNorms.p) dcNorms.alloc(4);
+    if (!pinnedNorms) HIPCHECK(hipHostMalloc((void **)&pinnedNorms, 4 * sizeof(double), hipHostMallocDefault));
+    if (pressureMode >= 2)
+      for (int L = 0; L <= bottom; L++) {
+        size_t n = L == 0 ? cells0 : lv[L].p.n;
+        if (lv16[L].p.n != n) {
+          lv16[L].p.alloc(n);
+          lv16[L].q.alloc(n);
+          lv16[L].rhs.alloc(n);
+        }
+      }
+  }
+
+  // Defect correction: x = 0; r = b - A x; while |r| / |b| > tol, solve A e = r / s with
+  // one V-cycle in the levels' storage (s the power of two under max |r|) and add s e to x.
+  // Norms are FP64 sums on the device; the host reads them to decide.
+  template <class St> void correct(MGArgsT<typename St::T> m, unsigned slotsGrid) {
+    HIPCHECK(hipMemsetAsync(dcX.p, 0, dcX.bytes(), stream));
+    lastCycles = 0;
+    lastResidual = 1.0;
+    for (int it = 0; it <= pressureMaxCycles; it++) {
+      k_dc_residual<St><<<slotsGrid, 256, 0, stream>>>(m, D.p, dcX.p, dcR.p, dcPartials.p);
+      k_dc_norms<<<1, 256, 0, stream>>>(dcPartials.p, counts.p, dcNorms.p);
+      HIPCHECK(hipMemcpyAsync(pinnedNorms, dcNorms.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      lastResidual = pinnedNorms[1] > 0 ? std::sqrt(pinnedNorms[0] / pinnedNorms[1]) : 0.0;
+      if (lastResidual <= pressureTol || it == pressureMaxCycles) break;
+      k_dc_scale<St><<<slotsGrid, 256, 0, stream>>>(m, dcR.p, dcNorms.p);
+      MGArgsT<typename St::T> c = m;
+      cycle<St>(c, 0, slotsGrid, true);
+      k_dc_update<St><<<slotsGrid, 256, 0, stream>>>(c, dcX.p, dcNorms.p);
+      lastCycles = it + 1;
+    }
+  }
+
+  template <class St> void relax(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid) {
+    if (L < brickLevels) k_mg_relax_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, L);
+    else k_mg_relax_tiles<St><<<capacity, 64, 0, stream>>>(m, L);
     std::swap(m.lv[L].p, m.lv[L].q);
-    relax(m, L, slotsGrid);
-    relax(m, L, slotsGrid);
-    relax(m, L, slotsGrid);
+  }
+  template <class St> void cycle(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid, bool bottom64) {
+    if (L == bottom) {
+      if (bottom64) k_mg_bottom64<St><<<1, 1024, 0, stream>>>(m, 19);
+      else if (capacity <= 1024) k_mg_bottom<St><<<1, 1024, 0, stream>>>(m, 19);
+      else
+        for (int i = 0; i < 19; i++) relax<St>(m, L, slotsGrid);
+      return;
+    }
+    if (L == 0) k_mg_init<St><<<slotsGrid, 256, 0, stream>>>(m);
+    else {
+      relax<St>(m, L, slotsGrid);
+      relax<St>(m, L, slotsGrid);
+    }
+    if (L + 1 < brickLevels) k_mg_restrict_bricks<St><<<slotsGrid, 64, 0, stream>>>(m, L + 1);
+    else k_mg_restrict_tiles<St><<<capacity, 64, 0, stream>>>(m, L + 1);
+    cycle<St>(m, L + 1, slotsGrid, bottom64);
+    if (L < brickLevels) k_mg_prolong_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, L);
+    else k_mg_prolong_tiles<St><<<capacity, 64, 0, stream>>>(m, L);
+    std::swap(m.lv[L].p, m.lv[L].q);
+    relax<St>(m, L, slotsGrid);
+    relax<St>(m, L, slotsGrid);
+    relax<St>(m, L, slotsGrid);
   }
 
   // readActivity (FluidSimulation.ts:1701-1746), applied at once.
@@ -1316,6 +1417,18 @@ struct fp_sim {
     colliders.release();
     forces.release();
     regions.release();
+    for (int L = 0; L < MAX_LEVELS; L++) {
+      lv16[L].p.release();
+      lv16[L].q.release();
+      lv16[L].rhs.release();
+    }
+    dcRhs.release();
+    dcX.release();
+    dcR.release();
+    dcPartials.release();
+    dcNorms.release();
+    if (pinnedNorms) (void)hipHostFree(pinnedNorms);
+    pinnedNorms = nullptr;
     rc.boxes.release();
     rc.tiles.release();
     rc.table.release();
@@ -1600,6 +1713,35 @@ int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *co
     if (ms) ms[i] = e.second ? e.first / e.second : 0.0;
   }
   *count = n;
+  return FP_OK;
+}
+
+int fp_sim_set_pressure(fp_sim *sim, int32_t mode, double tolerance, int32_t max_cycles) {
+  if (!sim) {
+    g_error = "fp_sim_set_pressure: null argument";
+    return FP_EINVAL;
+  }
+  if (mode < FP_PRESSURE_VCYCLE || mode > FP_PRESSURE_FP16) {
+    g_error = "pressure mode must be one of FP_PRESSURE_VCYCLE, _FP32, _BF16, _FP16.";
+    return FP_EINVAL;
+  }
+  if (!(tolerance >= 0) || max_cycles < 1 || max_cycles > 1000) {
+    g_error = "pressure tolerance must be >= 0 and max_cycles from 1 to 1000.";
+    return FP_EINVAL;
+  }
+  sim->pressureMode = mode;
+  sim->pressureTol = tolerance;
+  sim->pressureMaxCycles = max_cycles;
+  return FP_OK;
+}
+
+int fp_sim_pressure_report(const fp_sim *sim, int32_t *cycles, double *relative_residual) {
+  if (!sim) {
+    g_error = "fp_sim_pressure_report: null argument";
+    return FP_EINVAL;
+  }
+  if (cycles) *cycles = sim->lastCycles;
+  if (relative_residual) *relative_residual = sim->lastResidual;
   return FP_OK;
 }
 

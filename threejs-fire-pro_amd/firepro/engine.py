@@ -165,6 +165,19 @@ class FluidSimulation:
         va = np.ascontiguousarray(velocity, np.float32) if velocity is not None else None
         abi.check(self._lib.fp_sim_write_state(self._h(), ptr(fa), ptr(va)), self._lib)
 
+    # ---- pressure policy ----
+    def set_pressure(self, mode, tolerance: float = 1e-6, max_cycles: int = 20) -> None:
+        """'vcycle' (the reference's one V-cycle) or defect correction with V-cycle levels
+        stored in 'fp32', 'bf16' or 'fp16' (FP32 outer loop, FP64 norms and coarsest level)."""
+        m = abi.PRESSURE_MODES[mode] if isinstance(mode, str) else int(mode)
+        abi.check(self._lib.fp_sim_set_pressure(self._h(), m, tolerance, max_cycles), self._lib)
+
+    def pressure_report(self):
+        """(V-cycles of the latest step, its final relative residual or -1)."""
+        cycles, res = C.c_int32(), C.c_double()
+        abi.check(self._lib.fp_sim_pressure_report(self._h(), C.byref(cycles), C.byref(res)), self._lib)
+        return cycles.value, res.value
+
     # ---- render (SceneVolumeRenderer.prepare + the fragment march) ----
     def render(self, camera: abi.Camera, settings: Optional[abi.RenderSettings] = None) -> np.ndarray:
         """The premultiplied (radiance rgb, 1 - transmittance) image, (height, width, 4)."""
