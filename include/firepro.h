@@ -192,6 +192,7 @@ typedef struct fp_render_tables {
 } fp_render_tables;
 
 typedef struct fp_sim fp_sim;
+typedef struct fp_batch fp_batch;
 
 const char *fp_last_error(void);
 const char *fp_version(void);
@@ -231,6 +232,20 @@ void *fp_sim_stream(fp_sim *sim);
  * without event gaps; 0 stops. Enabling clears earlier averages. */
 int fp_sim_profile(fp_sim *sim, int32_t enable);
 int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *count);
+
+/* Batched stepping (the serving form; no counterpart in the reference, whose simulations
+ * step one at a time). fp_batch_step advances `count` simulations, all of one brick size
+ * and velocity divisor and on the V-cycle pressure policy, by one step each with
+ * frames[i] for sims[i]: their host halves run in turn, then every stage is one launch
+ * over all of their active bricks and one readback returns all their activity. Each
+ * simulation ends exactly as fp_sim_step would have left it. Any set of simulations may
+ * be passed on each call (continuous batching); none may be stepped or rendered elsewhere
+ * during the call. */
+int fp_batch_create(fp_batch **out);
+void fp_batch_destroy(fp_batch *batch);
+int fp_batch_step(fp_batch *batch, fp_sim *const *sims, const fp_frame *frames, int32_t count, double dt);
+/* Wall time of the latest fp_batch_step, host halves included, in ms. */
+int fp_batch_last_ms(const fp_batch *batch, double *ms);
 
 /* The pressure policy of the next steps. Defect correction keeps the solution x in FP32,
  * starts from x = 0 and repeats r = b - A x (FP32, each row's dot product and the norms in

@@ -1,0 +1,109 @@
+"""Batched stepping (the serving form) on the GPU: each simulation's tables and fields are
+the same alone and in the mix, bit for bit, while simulations join and leave the batch
+(continuous batching) and grow their pools."""
+import numpy as np
+import pytest
+
+from lockstep import apply, state_mismatches, table_mismatches
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes():
+    from firepro import scenes
+    return [
+        scenes.small(seed=1, steps=8, brick=8, divisor=2),
+        scenes.small(seed=2, steps=8, brick=8, divisor=2, collider=False),
+        scenes.small(seed=3, steps=8, brick=8, divisor=2, burst=False),
+        scenes.campfire(steps=8),
+    ]
+
+
+def test_batch_equals_alone_with_joins_and_leaves():
+    from firepro import scenes
+    from firepro.engine import Batch, FluidSimulation
+    # The campfire uses 16-cell bricks; a batch holds one layout, so use 8-cell ones here.
+    scs = _scenes()[:3] + [scenes.small(seed=4, steps=8, brick=8, divisor=2)]
+    # Simulation i runs steps [start_i, start_i + len) of the batch's clock.
+    start = [0, 0, 2, 3]
+    frames = [list(sc.frames()) for sc in scs]
+    alone = [FluidSimulation.create(sc.grid()) for sc in scs]
+    mixed = [FluidSimulation.create(sc.grid()) for sc in scs]
+    batch = Batch()
+    try:
+        for i, sim in enumerate(alone):
+            for frame, g in frames[i]:
+                sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+                sim.step(frame)
+        for t in range(max(s + len(f) for s, f in zip(start, frames))):
+            members, fr = [], []
+            for i, sim in enumerate(mixed):
+                k = t - start[i]
+                if 0 <= k < len(frames[i]):
+                    frame, g = frames[i][k]
+                    sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+                    members.append(sim)
+                    fr.append(frame)
+            print(f"case batch clock {t}: {len(members)} members", flush=True)
+            batch.step(members, fr)
+        for i in range(len(scs)):
+            msgs = table_mismatches(mixed[i].read_tables(), alone[i].read_tables())
+            msgs += state_mismatches(mixed[i].read_state(pressure=True, expansion=True),
+                                     alone[i].read_state(pressure=True, expansion=True))
+            assert msgs == [], (i, msgs)
+            assert mixed[i].stats().steps == alone[i].stats().steps
+    finally:
+        batch.dispose()
+        for s in alone + mixed:
+            s.dispose()
+
+
+def test_batch_pool_growth_and_budget():
+    """A member that outgrows its pool and hits its voxel budget inside a batch."""
+    from firepro import scenes
+    from firepro.engine import Batch, FluidSimulation
+    a = scenes.small(seed=1, steps=8)
+    a.sim.options["grid"]["maxVoxels"] = 4096 * 3
+    b = scenes.small(seed=5, steps=8)
+    scs = [a, b]
+    frames = [list(sc.frames()) for sc in scs]
+    alone = [FluidSimulation.create(sc.grid()) for sc in scs]
+    mixed = [FluidSimulation.create(sc.grid()) for sc in scs]
+    batch = Batch()
+    try:
+        for k in range(8):
+            fr = []
+            for i in range(2):
+                frame, g = frames[i][k]
+                for sim in (alone[i], mixed[i]):
+                    sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+                alone[i].step(frame)
+                fr.append(frame)
+            batch.step(mixed, fr)
+        for i in range(2):
+            assert table_mismatches(mixed[i].read_tables(), alone[i].read_tables()) == []
+            assert state_mismatches(mixed[i].read_state(pressure=True), alone[i].read_state(pressure=True),
+                                    keys=("fields", "velocity", "pressure")) == []
+        assert mixed[0].pool_limited
+    finally:
+        batch.dispose()
+        for s in alone + mixed:
+            s.dispose()
+
+
+def test_batch_rejects_mixed_layouts():
+    from firepro import scenes
+    from firepro.engine import Batch, FluidSimulation
+    a = FluidSimulation.create(scenes.small(steps=1, brick=8).grid())
+    b = FluidSimulation.create(scenes.small(steps=1, brick=16).grid())
+    batch = Batch()
+    try:
+        f = [next(iter(scenes.small(steps=1).frames()))[0]] * 2
+        with pytest.raises(ValueError, match="brick size"):
+            batch.step([a, b], f)
+        with pytest.raises(ValueError, match="twice"):
+            batch.step([a, a], f)
+    finally:
+        batch.dispose()
+        a.dispose()
+        b.dispose()

@@ -130,14 +130,14 @@ __device__ __forceinline__ int entryAt(const Home &h, const Dir &d, int bx, int 
   return entryLookup(d, bx, by, bz);
 }
 
-// The work item of this workgroup among `count`, such that each XCD works through one
+// The work item of (virtual) workgroup `w` among `count`, such that each XCD works through one
 // contiguous run of items. MI355X hands workgroups to its 8 XCDs round-robin by id, and
 // each XCD has its own L2; items in brick-id order are spatial neighbors, so a contiguous
 // run per XCD keeps the neighbor reads of a brick in the L2 that already holds them.
 // Launches must have at least 8 * ceil(count / 8) workgroups. -1 when there is no item.
 constexpr int XCDS = 8;
-__device__ __forceinline__ int xcdItem(int count) {
-  const int w = (int)blockIdx.x, per = (count + XCDS - 1) / XCDS;
+__device__ __forceinline__ int xcdItem(int w, int count) {
+  const int per = (count + XCDS - 1) / XCDS;
   const int j = w / XCDS;
   if (j >= per) return -1;
   const int i = (w % XCDS) * per + j;

@@ -21,6 +21,9 @@ struct AllocArgs {
   const Collider *colliders;
   uint32_t *solids;
   int colliderCount;
+  int buildSolids;  // draw the colliders into solid bits this step
+  int solidsEvery;  // into every active slot's, else the fresh slots'
+  Pools p;
 };
 
 // Arguments of the per-cell kernels over the active bricks.
@@ -36,6 +39,7 @@ struct StepArgs {
   const Emitter *emitters;
   const ForceRec *forces;
   const Region *regions;
+  const float *pressure;  // the solved pressure the projection reads
 };
 
 // Storage of the multigrid levels' iterates and right-hand sides. Arithmetic is FP32 in
@@ -76,52 +80,105 @@ template <class T> struct MGArgsT {
   const uint32_t *solids;
   LevelT<T> lv[MAX_LEVELS];
   int bottom;
-  float weight;  // axisWeights, equal on every axis for cubic cells
+  float weight;     // axisWeights, equal on every axis for cubic cells
+  int topologyAll;  // the geometry changed: rebuild every cell's topology
+  // Defect correction: right-hand side b, FP32 solution x, residual r, per-brick partial
+  // sums and the norms (sum r^2, sum b^2, max |r|, scale).
+  const float *dcB;
+  float *dcX, *dcR;
+  double *dcPartials, *dcNorms;
 };
 using Level = LevelT<float>;
 using MGArgs = MGArgsT<float>;
 
-__global__ void k_build_bricks(AllocArgs a);
-__global__ void k_scan_blocks(AllocArgs a);
-__global__ void k_free_bricks(AllocArgs a);
-__global__ void k_allocate_bricks(AllocArgs a);
-__global__ void k_link_bricks(AllocArgs a);
-__global__ void k_build_solids(AllocArgs a, int everySlot);
-__global__ void k_zero_fresh(AllocArgs a, Pools p, int chunks);
+// Ragged batches. A batched launch (k_*_batch) advances many simulations: its workgroups
+// stride over virtual blocks, starts[i] being simulation i's first and starts[n] the total,
+// each a multiple of 8 so that the XCD-aware mapping holds inside every simulation; each
+// virtual block runs the single-simulation kernel's body with that simulation's arguments,
+// read from device memory. The grid is fixed (persistent workgroups), so one captured
+// launch serves any mix of simulations.
+__device__ __forceinline__ int batchOwner(const int *starts, int n, int v) {
+  int lo = 0, hi = n - 1;  // the last i with starts[i] <= v
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+#define FP_BATCH_LOOP(NAME, CALL)                                  \
+  const int total = starts[n];                                     \
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {            \
+    const int sim = __builtin_amdgcn_readfirstlane(batchOwner(starts, n, v)); \
+    CALL;                                                          \
+    __syncthreads();                                               \
+  }
+// A kernel body NAME_body(args, virtual block[, one more parameter]) as the single-simulation
+// kernel k_NAME and the batched k_NAME_batch.
+#define FP_KERNEL_PAIR(NAME, BOUNDS, ARGS)                                                                  \
+  __global__ void __launch_bounds__(BOUNDS) k_##NAME(ARGS a) { NAME##_body(a, (int)blockIdx.x); }           \
+  __global__ void __launch_bounds__(BOUNDS) k_##NAME##_batch(const ARGS *args, const int *starts, int n) {  \
+    FP_BATCH_LOOP(NAME, NAME##_body(args[sim], v - starts[sim]))                                             \
+  }
+#define FP_KERNEL_PAIR1(NAME, BOUNDS, ARGS, PTYPE, PNAME)                                                     \
+  __global__ void __launch_bounds__(BOUNDS) k_##NAME(ARGS a, PTYPE PNAME) { NAME##_body(a, (int)blockIdx.x, PNAME); } \
+  __global__ void __launch_bounds__(BOUNDS) k_##NAME##_batch(const ARGS *args, const int *starts, int n, PTYPE PNAME) { \
+    FP_BATCH_LOOP(NAME, NAME##_body(args[sim], v - starts[sim], PNAME))                                      \
+  }
+#define FP_DECLARE_PAIR(NAME, ARGS) \
+  __global__ void k_##NAME(ARGS a);   \
+  __global__ void k_##NAME##_batch(const ARGS *args, const int *starts, int n);
+#define FP_DECLARE_PAIR1(NAME, ARGS, PTYPE)  \
+  __global__ void k_##NAME(ARGS a, PTYPE); \
+  __global__ void k_##NAME##_batch(const ARGS *args, const int *starts, int n, PTYPE);
 
-__global__ void k_advect_velocity(StepArgs s);
-__global__ void k_correct_velocity(StepArgs s);
-__global__ void k_advect_scalars(StepArgs s, int chunks);
-__global__ void k_evolve_fields(StepArgs s, int chunks);
-__global__ void k_curl(StepArgs s);
-__global__ void k_cell_forces(StepArgs s, float *out);
-__global__ void k_forces(StepArgs s, const float *cellF);
-__global__ void k_divergence(StepArgs s, float *rhs0);
-__global__ void k_project(StepArgs s, const float *pressure);
+FP_DECLARE_PAIR(build_bricks, AllocArgs)
+FP_DECLARE_PAIR(scan_blocks, AllocArgs)
+FP_DECLARE_PAIR(free_bricks, AllocArgs)
+FP_DECLARE_PAIR(allocate_bricks, AllocArgs)
+FP_DECLARE_PAIR(link_bricks, AllocArgs)
+FP_DECLARE_PAIR(build_solids, AllocArgs)
+FP_DECLARE_PAIR1(zero_fresh, AllocArgs, int)
 
-__global__ void k_topology_bricks(MGArgs m, int fromZeroList);
-__global__ void k_topology_tiles(MGArgs m, int level);
-__global__ void k_eflags_bricks(MGArgs m);
-__global__ void k_eflags_tiles(MGArgs m, int level);
-template <class St> __global__ void k_mg_init(MGArgsT<typename St::T> m);
-template <class St> __global__ void k_mg_relax_bricks(MGArgsT<typename St::T> m, int level);
-template <class St> __global__ void k_mg_relax_tiles(MGArgsT<typename St::T> m, int level);
-template <class St> __global__ void k_mg_restrict_bricks(MGArgsT<typename St::T> m, int level);
-template <class St> __global__ void k_mg_restrict_tiles(MGArgsT<typename St::T> m, int level);
-template <class St> __global__ void k_mg_prolong_bricks(MGArgsT<typename St::T> m, int level);
-template <class St> __global__ void k_mg_prolong_tiles(MGArgsT<typename St::T> m, int level);
-template <class St> __global__ void k_mg_bottom(MGArgsT<typename St::T> m, int iterations);
-// The coarsest level in FP64 (any tile count up to BOTTOM64_TILES), stored back rounded.
+FP_DECLARE_PAIR(advect_velocity, StepArgs)
+FP_DECLARE_PAIR(correct_velocity, StepArgs)
+FP_DECLARE_PAIR1(advect_scalars, StepArgs, int)
+FP_DECLARE_PAIR1(evolve_fields, StepArgs, int)
+FP_DECLARE_PAIR(curl, StepArgs)
+FP_DECLARE_PAIR(cell_forces, StepArgs)
+FP_DECLARE_PAIR(forces, StepArgs)
+FP_DECLARE_PAIR(divergence, StepArgs)
+FP_DECLARE_PAIR(project, StepArgs)
+
+// Multigrid kernels. A level's iterate alternates between its p and q arrays; bit L of
+// `flips` says that level L's current iterate is in q. Level kernels take
+// `code` = level | flips << 8.
+FP_DECLARE_PAIR1(topology_bricks, MGArgs, int)
+FP_DECLARE_PAIR1(topology_tiles, MGArgs, int)
+FP_DECLARE_PAIR(eflags_bricks, MGArgs)
+FP_DECLARE_PAIR1(eflags_tiles, MGArgs, int)
+#define FP_DECLARE_MG(NAME)                                                                       \
+  template <class St> __global__ void k_##NAME(MGArgsT<typename St::T> a, int code);              \
+  template <class St>                                                                             \
+  __global__ void k_##NAME##_batch(const MGArgsT<typename St::T> *args, const int *starts, int n, int code);
+FP_DECLARE_MG(mg_init)
+FP_DECLARE_MG(mg_relax_bricks)
+FP_DECLARE_MG(mg_relax_tiles)
+FP_DECLARE_MG(mg_restrict_bricks)
+FP_DECLARE_MG(mg_restrict_tiles)
+FP_DECLARE_MG(mg_prolong_bricks)
+FP_DECLARE_MG(mg_prolong_tiles)
+FP_DECLARE_MG(mg_bottom)    // relaxCoarse: 19 sweeps in one workgroup, up to 1024 tile slots
+FP_DECLARE_MG(mg_bottom64)  // the same in FP64, up to BOTTOM64_TILES tile slots
+// Defect correction: r = b - A x (FP32 with FP64 row dot products) and per-brick FP64 sums;
+// their totals and the power-of-two scale (one workgroup per simulation); level 0's
+// right-hand side r / scale; x += scale * (level 0's solution).
+FP_DECLARE_MG(dc_residual)
+FP_DECLARE_MG(dc_norms)
+FP_DECLARE_MG(dc_scale)
+FP_DECLARE_MG(dc_update)
 constexpr int BOTTOM64_TILES = 8192;
-template <class St> __global__ void k_mg_bottom64(MGArgsT<typename St::T> m, int iterations);
-// Defect correction: r = b - A x (FP32) with per-brick FP64 sums of r^2, b^2 and the
-// largest |r|; their totals and a power-of-two scale; level 0's right-hand side r / scale;
-// x += scale * (level 0's solution).
-template <class St> __global__ void k_dc_residual(MGArgsT<typename St::T> m, const float *b, const float *x, float *r,
-                                                  double *partials);
-__global__ void k_dc_norms(const double *partials, const int *counts, double *out);
-template <class St> __global__ void k_dc_scale(MGArgsT<typename St::T> m, const float *r, const double *norms);
-template <class St> __global__ void k_dc_update(MGArgsT<typename St::T> m, float *x, const double *norms);
+constexpr int BOTTOM_ITERATIONS = 19;  // PressureSolver.ts:326
 
 // The render's parameters (volume-common.wgsl RenderParams, packed at
 // SceneVolumeRenderer.ts:329-345, and the camera uniforms of :440-456).

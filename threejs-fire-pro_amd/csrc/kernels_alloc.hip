@@ -56,9 +56,9 @@ __device__ __forceinline__ void blockRanks3(const bool f[3], uint32_t rank[3], u
 }
 
 // buildBricks (:343-390): each brick's state; per-block totals of frees, requests, keeps.
-__global__ void __launch_bounds__(256) k_build_bricks(AllocArgs a) {
+__device__ __forceinline__ void build_bricks_body(const AllocArgs &a, const int vb) {
   __shared__ uint32_t lds[12];
-  const int index = blockIdx.x * RANK_BLOCK + threadIdx.x;
+  const int index = vb * RANK_BLOCK + threadIdx.x;
   const int n = a.brickCount;
   if (index < a.dir.capacity) a.activity[n + index] = 0u;  // this step's tile content
   uint32_t state = 0;
@@ -95,19 +95,21 @@ __global__ void __launch_bounds__(256) k_build_bricks(AllocArgs a) {
   uint32_t rank[3], total[3];
   blockRanks3(f, rank, total, lds);
   if (threadIdx.x == 0) {
-    a.blockSums[blockIdx.x * 3 + 0] = total[0];
-    a.blockSums[blockIdx.x * 3 + 1] = total[1];
-    a.blockSums[blockIdx.x * 3 + 2] = total[2];
+    a.blockSums[vb * 3 + 0] = total[0];
+    a.blockSums[vb * 3 + 1] = total[1];
+    a.blockSums[vb * 3 + 2] = total[2];
   }
   if (index < n) {
     a.stateTmp[index] = state | ((uint32_t)keep << 5);
     a.rankTmp[index] = rank[0] | (rank[1] << 10) | (rank[2] << 20);
   }
 }
+FP_KERNEL_PAIR(build_bricks, 256, AllocArgs)
+
 
 // One workgroup: exclusive prefix of the block totals, then the step's commit
 // (finishBricks, :567-602): granted = min(requests, max(free + freed - held, 0)).
-__global__ void __launch_bounds__(1024) k_scan_blocks(AllocArgs a) {
+__device__ __forceinline__ void scan_blocks_body(const AllocArgs &a, const int vb) {
   __shared__ uint32_t part[3][1024];
   __shared__ uint32_t carry[3];
   const int blocks = a.blocks;
@@ -147,14 +149,16 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(AllocArgs a) {
     a.counts[5] = 0;                // freed slots appended to the zeroed list
   }
 }
+FP_KERNEL_PAIR(scan_blocks, 1024, AllocArgs)
+
 
 // freeBricks (:394-410): freed slots go onto the stack in brick order.
-__global__ void __launch_bounds__(256) k_free_bricks(AllocArgs a) {
-  const int index = blockIdx.x * RANK_BLOCK + threadIdx.x;
+__device__ __forceinline__ void free_bricks_body(const AllocArgs &a, const int vb) {
+  const int index = vb * RANK_BLOCK + threadIdx.x;
   if (index >= a.brickCount) return;
   uint32_t state = a.stateTmp[index];
   if (!(state & BRICK_FREES)) return;
-  int rank = (int)(a.blockPrefix[blockIdx.x * 3] + (a.rankTmp[index] & 1023u));
+  int rank = (int)(a.blockPrefix[vb * 3] + (a.rankTmp[index] & 1023u));
   int page = a.pages[index];
   a.freeSlots[a.counts[2] + rank] = (uint32_t)pageToSlot(page, a.L.side);
   if (state & 4u) {
@@ -164,15 +168,17 @@ __global__ void __launch_bounds__(256) k_free_bricks(AllocArgs a) {
   }
   a.pages[index] = -1;
 }
+FP_KERNEL_PAIR(free_bricks, 256, AllocArgs)
+
 
 // allocateBricks (:415-445): grants from the stack top in brick order, entries, the
 // compacted active list and the activity reset.
-__global__ void __launch_bounds__(256) k_allocate_bricks(AllocArgs a) {
-  const int index = blockIdx.x * RANK_BLOCK + threadIdx.x;
+__device__ __forceinline__ void allocate_bricks_body(const AllocArgs &a, const int vb) {
+  const int index = vb * RANK_BLOCK + threadIdx.x;
   if (index >= a.brickCount) return;
   uint32_t state = a.stateTmp[index];
   uint32_t ranks = a.rankTmp[index];
-  const uint32_t *pre = a.blockPrefix + blockIdx.x * 3;
+  const uint32_t *pre = a.blockPrefix + vb * 3;
   const int granted = a.counts[1];
   int reqBefore = (int)(pre[1] + ((ranks >> 10) & 1023u));
   int keepBefore = (int)(pre[2] + ((ranks >> 20) & 1023u));
@@ -192,10 +198,12 @@ __global__ void __launch_bounds__(256) k_allocate_bricks(AllocArgs a) {
   uint32_t flags = a.activity[index];
   a.activity[index] = (flags & EXPANDS) ? MAY_EXPAND : 0u;
 }
+FP_KERNEL_PAIR(allocate_bricks, 256, AllocArgs)
+
 
 // linkBricks (:449-483): each active brick's 27 neighbor entries and the pressure flag.
-__global__ void __launch_bounds__(64) k_link_bricks(AllocArgs a) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
+__device__ __forceinline__ void link_bricks_body(const AllocArgs &a, const int vb) {
+  const int i = vb * 64 + threadIdx.x;
   if (i >= a.counts[0]) return;
   const int index = a.active[i];
   const int *tr = a.dir.tileRec + (size_t)(index >> 6) * TILE_RECORD_WORDS;
@@ -218,6 +226,8 @@ __global__ void __launch_bounds__(64) k_link_bricks(AllocArgs a) {
   int *out = a.records + (size_t)index * RECORD_WORDS;
   for (int k = 0; k < 28; k++) out[k] = w[k];
 }
+FP_KERNEL_PAIR(link_bricks, 64, AllocArgs)
+
 
 // Colliders into the solid bits of one brick (buildSolids, :485-556): a workgroup per
 // brick, a word of 32 cells per lane and pass.
@@ -233,8 +243,10 @@ __device__ __forceinline__ bool insideCollider(const Collider &c, float hf, int 
   return dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] < 1.0f;
 }
 
-__global__ void __launch_bounds__(64) k_build_solids(AllocArgs a, int everySlot) {
-  const int i = blockIdx.x;
+__device__ __forceinline__ void build_solids_body(const AllocArgs &a, const int vb) {
+  if (!a.buildSolids) return;
+  const int everySlot = a.solidsEvery;
+  const int i = vb;
   int index, page;
   if (everySlot) {
     if (i >= a.counts[0]) return;
@@ -266,11 +278,14 @@ __global__ void __launch_bounds__(64) k_build_solids(AllocArgs a, int everySlot)
     a.solids[(size_t)slot * words + word] = bits;
   }
 }
+FP_KERNEL_PAIR(build_solids, 64, AllocArgs)
+
 
 // A fresh slot is still, empty air (zeroVelocityBricks / zeroFieldBricks,
 // fluid-zeroing.wgsl:139-169): its fields, expansion and velocity are cleared.
-__global__ void __launch_bounds__(256) k_zero_fresh(AllocArgs a, Pools p, int chunks) {
-  const int i = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+__device__ __forceinline__ void zero_fresh_body(const AllocArgs &a, const int vb, int chunks) {
+  const Pools &p = a.p;
+  const int i = vb / chunks, chunk = vb % chunks;
   if (i >= a.counts[1]) return;
   int slot = pageToSlot(a.zeroList[2 * i + 1] & 0xffffff, a.L.side);
   const int nF = a.L.nF, nV = a.L.nV;
@@ -280,5 +295,7 @@ __global__ void __launch_bounds__(256) k_zero_fresh(AllocArgs a, Pools p, int ch
   }
   for (int c = chunk * 256 + threadIdx.x; c < 3 * nV; c += chunks * 256) p.V[(size_t)slot * nV * 3 + c] = 0.f;
 }
+FP_KERNEL_PAIR1(zero_fresh, 256, AllocArgs, int, chunks)
+
 
 }  // namespace fp

@@ -7,7 +7,7 @@
 namespace fp {
 
 #define BRICK_PREAMBLE(CHUNKS)                              \
-  const int item = xcdItem(s.counts[0] * (CHUNKS));         \
+  const int item = xcdItem(vb, s.counts[0] * (CHUNKS));     \
   if (item < 0) return;                                     \
   const int bi = item / (CHUNKS);                           \
   const int chunk = item % (CHUNKS);                        \
@@ -51,7 +51,7 @@ __device__ __forceinline__ float advectComponent(const Ctx &c, const Params &P, 
   return sampleVel(c, V, comp, sp[0], sp[1], sp[2]);
 }
 
-__global__ void __launch_bounds__(256) k_advect_velocity(StepArgs s) {
+__device__ __forceinline__ void advect_velocity_body(const StepArgs &s, const int vb) {
   BRICK_PREAMBLE(1)
   const int nV = s.L.nV;
   float *out = s.p.Vadv + (size_t)slot * nV * 3;
@@ -68,6 +68,8 @@ __global__ void __launch_bounds__(256) k_advect_velocity(StepArgs s) {
     for (int k = 0; k < 3; k++) out[k * nV + l] = advectComponent(c, s.P, s.p.V, cc, pos, k, orig);
   }
 }
+FP_KERNEL_PAIR(advect_velocity, 256, StepArgs)
+
 
 // Donor bounds (fluid-velocity-transport.wgsl:165-212): the component's range over the
 // 2^3 cells from `base`, or [0, 0] when base's brick has no slot.
@@ -112,7 +114,7 @@ __device__ __forceinline__ float correctComponent(const Ctx &c, const Params &P,
   return (corrected < lo || corrected > hi) ? predicted : corrected;
 }
 
-__global__ void __launch_bounds__(256) k_correct_velocity(StepArgs s) {
+__device__ __forceinline__ void correct_velocity_body(const StepArgs &s, const int vb) {
   BRICK_PREAMBLE(1)
   const int nV = s.L.nV;
   float *out = s.p.Vcor + (size_t)slot * nV * 3;
@@ -130,6 +132,8 @@ __global__ void __launch_bounds__(256) k_correct_velocity(StepArgs s) {
     for (int k = 0; k < 3; k++) out[k * nV + l] = correctComponent(c, s.P, s.p, cc, pos, k, orig, as[k * nV + l]);
   }
 }
+FP_KERNEL_PAIR(correct_velocity, 256, StepArgs)
+
 
 // ---- scalar transport -----------------------------------------------------------------
 
@@ -191,7 +195,7 @@ __device__ __forceinline__ void departureCell(const Params &P, const int id[3], 
   }
 }
 
-__global__ void __launch_bounds__(256) k_advect_scalars(StepArgs s, int chunks) {
+__device__ __forceinline__ void advect_scalars_body(const StepArgs &s, const int vb, int chunks) {
   BRICK_PREAMBLE(chunks)
   const int nF = s.L.nF;
   for (int l = chunk * 256 + threadIdx.x; l < nF; l += chunks * 256) {
@@ -215,6 +219,8 @@ __global__ void __launch_bounds__(256) k_advect_scalars(StepArgs s, int chunks) 
     *out = make_float4(v.x, v.y, s.P.lifetimeZero ? 0.f : v.z, v.w);
   }
 }
+FP_KERNEL_PAIR1(advect_scalars, 256, StepArgs, int, chunks)
+
 
 // correctedFineScalars (fluid-scalar-transport.wgsl:144-188), with the donor bounds
 // recomputed from the same backward trace instead of cached.
@@ -386,7 +392,7 @@ __device__ __forceinline__ uint32_t contentSupport(const Layout &L, const Params
 }
 
 // evolveFields (fluid-field-evolution.wgsl:208-283) with content recording.
-__global__ void __launch_bounds__(256) k_evolve_fields(StepArgs s, int chunks) {
+__device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int vb, int chunks) {
   BRICK_PREAMBLE(chunks)
   __shared__ uint32_t busy, expands;
   if (threadIdx.x == 0) {
@@ -483,6 +489,8 @@ __global__ void __launch_bounds__(256) k_evolve_fields(StepArgs s, int chunks) {
     if (flags) atomicOr(&s.activity[id], flags);
   }
 }
+FP_KERNEL_PAIR1(evolve_fields, 256, StepArgs, int, chunks)
+
 
 // ---- forces ---------------------------------------------------------------------------
 
@@ -492,7 +500,8 @@ __device__ __forceinline__ float curlCentered(const Ctx &c, const float *V, int 
   return 0.5f * (a + b);
 }
 
-__global__ void __launch_bounds__(256) k_curl(StepArgs s) {
+__device__ __forceinline__ void curl_body(const StepArgs &s, const int vb) {
+  if (s.P.vorticity == 0.f) return;
   BRICK_PREAMBLE(1)
   const int nV = s.L.nV;
   const float *V = s.p.Vcor;
@@ -511,6 +520,8 @@ __global__ void __launch_bounds__(256) k_curl(StepArgs s) {
     s.p.Curl[(size_t)slot * nV + l] = make_float4(c0, c1, c2, length3(c0, c1, c2));
   }
 }
+FP_KERNEL_PAIR(curl, 256, StepArgs)
+
 
 // curlNoise (fluid-forces.wgsl:3-9): the periodic table, trilinear with wrapping.
 __device__ __forceinline__ void curlNoise(const float4 *noise, const float p[3], float out[3]) {
@@ -672,7 +683,8 @@ __device__ __forceinline__ float sourceVelocity(const StepArgs &s, float value, 
 // The force at each velocity cell of the active bricks (centerForce, fluid-forces.wgsl:
 // 79-118), evaluated once per cell into `out` ([slot][axis][cell]); applyForces reads a
 // cell's and its +x/+y/+z neighbors' (zero where a brick does not compute).
-__global__ void __launch_bounds__(256) k_cell_forces(StepArgs s, float *out) {
+__device__ __forceinline__ void cell_forces_body(const StepArgs &s, const int vb) {
+  float *out = s.p.Vadv;
   BRICK_PREAMBLE(1)
   const int nV = s.L.nV;
   float *o = out + (size_t)slot * nV * 3;
@@ -686,10 +698,13 @@ __global__ void __launch_bounds__(256) k_cell_forces(StepArgs s, float *out) {
     o[2 * nV + l] = f[2];
   }
 }
+FP_KERNEL_PAIR(cell_forces, 256, StepArgs)
+
 
 // applyForces (fluid-forces.wgsl:151-182): cell forces averaged onto the faces, damping,
 // emitter velocity coupling and closed solid faces.
-__global__ void __launch_bounds__(256) k_forces(StepArgs s, const float *cellF) {
+__device__ __forceinline__ void forces_body(const StepArgs &s, const int vb) {
+  const float *cellF = s.p.Vadv;
   BRICK_PREAMBLE(1)
   const Params &P = s.P;
   const int nV = s.L.nV;
@@ -723,9 +738,12 @@ __global__ void __launch_bounds__(256) k_forces(StepArgs s, const float *cellF) 
     o[2 * nV + l] = out2;
   }
 }
+FP_KERNEL_PAIR(forces, 256, StepArgs)
+
 
 // computeDivergence (fluid-forces.wgsl:184-203): rhs = h^2 (div u - expansion).
-__global__ void __launch_bounds__(256) k_divergence(StepArgs s, float *rhs0) {
+__device__ __forceinline__ void divergence_body(const StepArgs &s, const int vb) {
+  float *rhs0 = s.p.D;
   BRICK_PREAMBLE(1)
   const Params &P = s.P;
   const int nV = s.L.nV;
@@ -768,6 +786,8 @@ __global__ void __launch_bounds__(256) k_divergence(StepArgs s, float *rhs0) {
     rhs0[(size_t)slot * nV + l] = h * h * (div - avg);
   }
 }
+FP_KERNEL_PAIR(divergence, 256, StepArgs)
+
 
 // project (fluid-forces.wgsl:205-262).
 __device__ __forceinline__ bool pressureUnknown(const Ctx &c, const Params &P, const int p[3]) {
@@ -785,7 +805,8 @@ __device__ __forceinline__ bool pressureUnknown(const Ctx &c, const Params &P, c
   return true;
 }
 
-__global__ void __launch_bounds__(256) k_project(StepArgs s, const float *pressure) {
+__device__ __forceinline__ void project_body(const StepArgs &s, const int vb) {
+  const float *pressure = s.pressure;
   BRICK_PREAMBLE(1)
   const Params &P = s.P;
   const int nV = s.L.nV;
@@ -814,6 +835,8 @@ __global__ void __launch_bounds__(256) k_project(StepArgs s, const float *pressu
     }
   }
 }
+FP_KERNEL_PAIR(project, 256, StepArgs)
+
 
 
 

@@ -141,13 +141,13 @@ __device__ __forceinline__ void levelCell(int home0, int home1, int home2, int s
   c.tc[2] = trec[2];                                                            \
   c.tnb = tnb;
 
-// buildBrickTopology (pressure.wgsl:259-278) for every active brick, or for the bricks
-// of the zeroed list that hold a slot.
-__global__ void __launch_bounds__(256) k_topology_bricks(MGArgs m, int fromZeroList) {
-  int id;
-  const int bi = xcdItem(fromZeroList ? m.counts[1] + m.counts[5] : m.counts[0]);
+// buildBrickTopology (pressure.wgsl:259-278) for every active brick (when the geometry
+// changed), or for the bricks of the zeroed list that hold a slot.
+__device__ __forceinline__ void topology_bricks_body(const MGArgs &m, const int vb, int fromZeroList) {
+  if (!fromZeroList && !m.topologyAll) return;
+  const int bi = xcdItem(vb, fromZeroList ? m.counts[1] + m.counts[5] : m.counts[0]);
   if (bi < 0) return;
-  id = fromZeroList ? m.zeroList[2 * bi] : m.active[bi];
+  const int id = fromZeroList ? m.zeroList[2 * bi] : m.active[bi];
   if (m.dir.entries[id] < 0) return;
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
@@ -160,10 +160,13 @@ __global__ void __launch_bounds__(256) k_topology_bricks(MGArgs m, int fromZeroL
     }
   }
 }
+FP_KERNEL_PAIR1(topology_bricks, 256, MGArgs, int, fromZeroList)
 
 // buildTileTopology (pressure.wgsl:281-305) of one tile level, every tile slot.
-__global__ void __launch_bounds__(64) k_topology_tiles(MGArgs m, int lv) {
-  const int t = blockIdx.x;
+__device__ __forceinline__ void topology_tiles_body(const MGArgs &m, const int vb, int lv) {
+  if (!m.topologyAll) return;
+  const int t = vb;
+  if (t >= m.dir.capacity) return;
   const int sd = m.lv[lv].side, n = sd * sd * sd;
   const int *trec0 = m.dir.tileRec + (size_t)t * TILE_RECORD_WORDS;
   if (trec0[3] == 0) {
@@ -177,10 +180,11 @@ __global__ void __launch_bounds__(64) k_topology_tiles(MGArgs m, int lv) {
     m.lv[lv].topo[(size_t)t * n + l] = cellTopology(c, m.P, m.solids, p, lv);
   }
 }
+FP_KERNEL_PAIR1(topology_tiles, 64, MGArgs, int, lv)
 
 // The step's flags of the brick levels: topology plus the unknown bit.
-__global__ void __launch_bounds__(256) k_eflags_bricks(MGArgs m) {
-  const int bi = xcdItem(m.counts[0]);
+__device__ __forceinline__ void eflags_bricks_body(const MGArgs &m, const int vb) {
+  const int bi = xcdItem(vb, m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
   MG_BRICK_HOME(id)
@@ -195,8 +199,10 @@ __global__ void __launch_bounds__(256) k_eflags_bricks(MGArgs m) {
     }
   }
 }
-__global__ void __launch_bounds__(64) k_eflags_tiles(MGArgs m, int lv) {
-  const int t = blockIdx.x;
+FP_KERNEL_PAIR(eflags_bricks, 256, MGArgs)
+__device__ __forceinline__ void eflags_tiles_body(const MGArgs &m, const int vb, int lv) {
+  const int t = vb;
+  if (t >= m.dir.capacity) return;
   MG_TILE_HOME(t)
   const int sd = m.lv[lv].side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
@@ -205,6 +211,19 @@ __global__ void __launch_bounds__(64) k_eflags_tiles(MGArgs m, int lv) {
     size_t a = (size_t)t * n + l;
     m.lv[lv].eflag[a] = (uint8_t)(m.lv[lv].topo[a] | (activeCell(c, m.P, p, lv) ? UNKNOWN : 0));
   }
+}
+FP_KERNEL_PAIR1(eflags_tiles, 64, MGArgs, int, lv)
+
+// A level's arrays as one launch sees them: its current iterate p, the other array q.
+template <class T> struct LevelView {
+  T *p, *q, *rhs;
+  const uint8_t *topo, *eflag;
+  int side;
+};
+template <class T> __device__ __forceinline__ LevelView<T> levelView(const MGArgsT<T> &m, int L, int flips) {
+  const LevelT<T> &l = m.lv[L];
+  const bool f = ((flips >> L) & 1) != 0;
+  return LevelView<T>{f ? l.q : l.p, f ? l.p : l.q, l.rhs, l.topo, l.eflag, l.side};
 }
 
 template <class St>
@@ -234,9 +253,9 @@ __device__ __forceinline__ float stencilSum(const MGCtx &c, const typename St::T
   return w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
 }
 
-// relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8.
+// relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8, p to q.
 template <class St>
-__device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelT<typename St::T> &lv, int level, const int p[3],
+__device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelView<typename St::T> &lv, int level, const int p[3],
                                           size_t a, float w) {
   int e = lv.eflag[a];
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
@@ -249,13 +268,23 @@ __device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelT<typename 
   lv.q[a] = St::st(mixv(St::ld(lv.p[a]), cand, 0.8f));
 }
 
-template <class St> __global__ void __launch_bounds__(256) k_mg_init(MGArgsT<typename St::T> m) {
-  const int bi = xcdItem(m.counts[0]);
+#define FP_MG_PAIR(NAME, BOUNDS)                                                                          \
+  template <class St> __global__ void __launch_bounds__(BOUNDS) k_##NAME(MGArgsT<typename St::T> a, int code) { \
+    NAME##_body<St>(a, (int)blockIdx.x, code);                                                            \
+  }                                                                                                       \
+  template <class St>                                                                                     \
+  __global__ void __launch_bounds__(BOUNDS)                                                               \
+      k_##NAME##_batch(const MGArgsT<typename St::T> *args, const int *starts, int n, int code) {          \
+    FP_BATCH_LOOP(NAME, NAME##_body<St>(args[sim], v - starts[sim], code))                                \
+  }
+
+// initializePressure (pressure.wgsl:485-505): level 0's first iterate from zero.
+template <class St> __device__ __forceinline__ void mg_init_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int bi = xcdItem(vb, m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
-  MG_BRICK_HOME(id)
-  int slot = entrySlot(rec[13], c.side);
-  const auto &lv = m.lv[0];
+  const auto lv = levelView(m, 0, code >> 8);
+  const int slot = entrySlot(m.dir.entries[id], m.L.side);
   int sd = lv.side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     size_t a = (size_t)slot * n + l;
@@ -266,14 +295,17 @@ template <class St> __global__ void __launch_bounds__(256) k_mg_init(MGArgsT<typ
     lv.p[a] = St::st(first);
   }
 }
+FP_MG_PAIR(mg_init, 256)
 
-template <class St> __global__ void __launch_bounds__(256) k_mg_relax_bricks(MGArgsT<typename St::T> m, int level) {
-  const int bi = xcdItem(m.counts[0]);
+template <class St>
+__device__ __forceinline__ void mg_relax_bricks_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int level = code & 255;
+  const int bi = xcdItem(vb, m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
-  const auto &lv = m.lv[level];
+  const auto lv = levelView(m, level, code >> 8);
   int sd = lv.side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
@@ -281,10 +313,14 @@ template <class St> __global__ void __launch_bounds__(256) k_mg_relax_bricks(MGA
     relaxCell<St>(c, lv, level, p, (size_t)slot * n + l, m.weight);
   }
 }
-template <class St> __global__ void __launch_bounds__(64) k_mg_relax_tiles(MGArgsT<typename St::T> m, int level) {
-  const int t = blockIdx.x;
+FP_MG_PAIR(mg_relax_bricks, 256)
+template <class St>
+__device__ __forceinline__ void mg_relax_tiles_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int level = code & 255;
+  const int t = vb;
+  if (t >= m.dir.capacity) return;
   MG_TILE_HOME(t)
-  const auto &lv = m.lv[level];
+  const auto lv = levelView(m, level, code >> 8);
   int sd = lv.side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
@@ -292,13 +328,15 @@ template <class St> __global__ void __launch_bounds__(64) k_mg_relax_tiles(MGArg
     relaxCell<St>(c, lv, level, p, (size_t)t * n + l, m.weight);
   }
 }
+FP_MG_PAIR(mg_relax_tiles, 64)
 
 // restrictCell (pressure.wgsl:601-647): the coarse right-hand side is half the sum of the
 // eight children's residuals; the coarse first iterate is one Jacobi step from zero.
 template <class St>
-__device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, const int id[3]) {
-  const auto &fine = m.lv[level - 1];
-  const auto &coarse = m.lv[level];
+__device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, int flips,
+                                             const int id[3]) {
+  const auto fine = levelView(m, level - 1, flips);
+  const auto coarse = levelView(m, level, flips);
   float residual = 0.f;
   bool every = true;
   for (int z = 0; z < 2; z++)
@@ -326,9 +364,10 @@ __device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typen
   }
   coarse.p[out] = St::st(first);
 }
-
-template <class St> __global__ void __launch_bounds__(64) k_mg_restrict_bricks(MGArgsT<typename St::T> m, int level) {
-  const int bi = xcdItem(m.counts[0]);
+template <class St>
+__device__ __forceinline__ void mg_restrict_bricks_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int level = code & 255;
+  const int bi = xcdItem(vb, m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
   MG_BRICK_HOME(id)
@@ -336,19 +375,24 @@ template <class St> __global__ void __launch_bounds__(64) k_mg_restrict_bricks(M
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-    restrictCell<St>(c, m, level, p);
+    restrictCell<St>(c, m, level, code >> 8, p);
   }
 }
-template <class St> __global__ void __launch_bounds__(64) k_mg_restrict_tiles(MGArgsT<typename St::T> m, int level) {
-  const int t = blockIdx.x;
+FP_MG_PAIR(mg_restrict_bricks, 64)
+template <class St>
+__device__ __forceinline__ void mg_restrict_tiles_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int level = code & 255;
+  const int t = vb;
+  if (t >= m.dir.capacity) return;
   MG_TILE_HOME(t)
   int sd = m.lv[level].side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    restrictCell<St>(c, m, level, p);
+    restrictCell<St>(c, m, level, code >> 8, p);
   }
 }
+FP_MG_PAIR(mg_restrict_tiles, 64)
 
 // coarseValue (pressure.wgsl:670-691): Neumann ghosts beside solids, a mirror below the floor.
 __device__ __forceinline__ bool coarseSolid(const MGCtx &c, const uint8_t *topo, const int q[3], int level) {
@@ -357,8 +401,8 @@ __device__ __forceinline__ bool coarseSolid(const MGCtx &c, const uint8_t *topo,
   return (topo[a] & STORED_SOLID) != 0;
 }
 template <class St>
-__device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typename St::T> &m, const int p[3], int level) {
-  const auto &cv = m.lv[level];
+__device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typename St::T> &m, const LevelView<typename St::T> &cv,
+                                             const int p[3], int level) {
   const int groundY = m.P.ground ? 0 : (int)0x80000000;
   int cc[3] = {p[0], p[1] > groundY ? p[1] : groundY, p[2]};
   if (!coarseSolid(c, cv.topo, cc, level)) return readAt<St>(c, cv.p, cc, level);
@@ -376,11 +420,12 @@ __device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typen
   return sum / fmaxv(count, 1.0f);
 }
 
-// prolongateCell (pressure.wgsl:726-755): trilinear correction from the coarse level.
+// prolongateCell (pressure.wgsl:726-755): trilinear correction from the coarse level, p to q.
 template <class St>
-__device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, const int p[3],
-                                            size_t a) {
-  const auto &lv = m.lv[level];
+__device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, int flips,
+                                            const int p[3], size_t a) {
+  const auto lv = levelView(m, level, flips);
+  const auto cv = levelView(m, level + 1, flips);
   int e = lv.eflag[a];
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
     lv.q[a] = St::st(0.f);
@@ -398,15 +443,16 @@ __device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgsT<typena
   float v[8];
   for (int k = 0; k < 8; k++) {
     int cc[3] = {i[0] + (k & 1), i[1] + ((k >> 1) & 1), i[2] + (k >> 2)};
-    v[k] = plain ? readAt<St>(c, m.lv[level + 1].p, cc, level + 1) : coarseValue<St>(c, m, cc, level + 1);
+    v[k] = plain ? readAt<St>(c, cv.p, cc, level + 1) : coarseValue<St>(c, m, cv, cc, level + 1);
   }
   float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
                     mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
   lv.q[a] = St::st(St::ld(lv.p[a]) + corr);
 }
-
-template <class St> __global__ void __launch_bounds__(256) k_mg_prolong_bricks(MGArgsT<typename St::T> m, int level) {
-  const int bi = xcdItem(m.counts[0]);
+template <class St>
+__device__ __forceinline__ void mg_prolong_bricks_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int level = code & 255;
+  const int bi = xcdItem(vb, m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
   MG_BRICK_HOME(id)
@@ -415,29 +461,35 @@ template <class St> __global__ void __launch_bounds__(256) k_mg_prolong_bricks(M
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-    prolongCell<St>(c, m, level, p, (size_t)slot * n + l);
+    prolongCell<St>(c, m, level, code >> 8, p, (size_t)slot * n + l);
   }
 }
-template <class St> __global__ void __launch_bounds__(64) k_mg_prolong_tiles(MGArgsT<typename St::T> m, int level) {
-  const int t = blockIdx.x;
+FP_MG_PAIR(mg_prolong_bricks, 256)
+template <class St>
+__device__ __forceinline__ void mg_prolong_tiles_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int level = code & 255;
+  const int t = vb;
+  if (t >= m.dir.capacity) return;
   MG_TILE_HOME(t)
   int sd = m.lv[level].side, n = sd * sd * sd;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    prolongCell<St>(c, m, level, p, (size_t)t * n + l);
+    prolongCell<St>(c, m, level, code >> 8, p, (size_t)t * n + l);
   }
 }
+FP_MG_PAIR(mg_prolong_tiles, 64)
 
 // relaxCoarse (pressure.wgsl:549-599): every bottom iteration in one workgroup, a tile slot
-// per lane, values in LDS. The result lands in lv.p.
+// per lane, values in LDS. The result lands in the bottom level's current iterate.
 constexpr int BOTTOM_LANES = 1024;
-template <class St> __global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom(MGArgsT<typename St::T> m, int iterations) {
+template <class St> __device__ __forceinline__ void mg_bottom_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  if (vb != 0) return;
   __shared__ float value[BOTTOM_LANES];
   const int lane = threadIdx.x;
-  const auto &lv = m.lv[m.bottom];
+  const auto lv = levelView(m, m.bottom, code >> 8);
   const int cap = m.dir.capacity;
-  const int *trec = m.dir.tileRec + (size_t)lane * TILE_RECORD_WORDS;
+  const int *trec = m.dir.tileRec + (size_t)(lane < cap ? lane : 0) * TILE_RECORD_WORDS;
   bool valid = lane < cap && trec[3] != 0;
   int e = valid ? lv.eflag[lane] : 0;
   bool live = valid && (e & UNKNOWN) && !(e & STORED_SOLID);
@@ -447,7 +499,7 @@ template <class St> __global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom(
   if (valid && !(e & UNKNOWN)) mine = 0.f;
   value[lane] = mine;
   __syncthreads();
-  for (int it = 0; it < iterations; it++) {
+  for (int it = 0; it < BOTTOM_ITERATIONS; it++) {
     float next = 0.f;
     if (live) {
       float n[6];
@@ -469,15 +521,18 @@ template <class St> __global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom(
   }
   if (valid) lv.p[lane] = St::st(mine);
 }
+FP_MG_PAIR(mg_bottom, 1024)
 
-// The coarsest level's sweeps in FP64: the same weighted Jacobi as relaxCoarse, one tile
-// slot per lane and up to BOTTOM64_TILES / 1024 slots per lane, values in LDS. The result
-// is rounded to FP32, then stored.
+// The coarsest level's sweeps in FP64: the same weighted Jacobi as relaxCoarse, up to
+// BOTTOM64_TILES / 1024 tile slots per lane, values in LDS. The result is rounded to FP32,
+// then stored.
 __device__ __forceinline__ double mix64(double a, double b, double t) { return a * (1.0 - t) + b * t; }
-template <class St> __global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom64(MGArgsT<typename St::T> m, int iterations) {
+template <class St>
+__device__ __forceinline__ void mg_bottom64_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  if (vb != 0) return;
   constexpr int PER = BOTTOM64_TILES / BOTTOM_LANES;
   __shared__ double value[BOTTOM64_TILES];
-  const auto &lv = m.lv[m.bottom];
+  const auto lv = levelView(m, m.bottom, code >> 8);
   const int cap = m.dir.capacity;
   const double w = (double)m.weight;
   double mine[PER], rhs[PER];
@@ -494,7 +549,7 @@ template <class St> __global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom6
     if (t < cap) value[t] = mine[k];
   }
   __syncthreads();
-  for (int it = 0; it < iterations; it++) {
+  for (int it = 0; it < BOTTOM_ITERATIONS; it++) {
     double next[PER];
     for (int k = 0; k < PER; k++) {
       const int t = threadIdx.x + k * BOTTOM_LANES;
@@ -527,21 +582,23 @@ template <class St> __global__ void __launch_bounds__(BOTTOM_LANES) k_mg_bottom6
     if (t < cap && m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] != 0) lv.p[t] = St::st((float)mine[k]);
   }
 }
+FP_MG_PAIR(mg_bottom64, 1024)
 
 // ---- defect correction -----------------------------------------------------------------
 // The level-0 operator of relaxCell as a residual: r = b - (sum - diag x) on unknown cells,
 // zero elsewhere, each row's dot product accumulated in FP64 from the FP32 x and rounded
 // to FP32; per-brick FP64 sums of r^2 and b^2 and the largest |r|.
 template <class St>
-__global__ void __launch_bounds__(256) k_dc_residual(MGArgsT<typename St::T> m, const float *b, const float *x,
-                                                      float *r, double *partials) {
-  const int bi = xcdItem(m.counts[0]);
+__device__ __forceinline__ void dc_residual_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  (void)code;
+  const int bi = xcdItem(vb, m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
   MG_BRICK_HOME(id)
   __shared__ double red[3][256];
   int slot = entrySlot(rec[13], c.side);
   const auto &lv = m.lv[0];
+  const float *x = m.dcX;
   int sd = lv.side, n = sd * sd * sd;
   double rr = 0.0, bb = 0.0;
   float big = 0.f;
@@ -560,13 +617,13 @@ __global__ void __launch_bounds__(256) k_dc_residual(MGArgsT<typename St::T> m, 
         int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
         sum += w * (double)readAt<StoreF32>(c, x, q, 0);
       }
-      float bv = b[a];
+      float bv = m.dcB[a];
       v = (float)((double)bv - (sum - (double)diagonal(m.weight, flags) * (double)x[a]));
       rr += (double)v * (double)v;
       bb += (double)bv * (double)bv;
       big = fmaxv(big, fabsv(v));
     }
-    r[a] = v;
+    m.dcR[a] = v;
   }
   red[0][threadIdx.x] = rr;
   red[1][threadIdx.x] = bb;
@@ -581,23 +638,26 @@ __global__ void __launch_bounds__(256) k_dc_residual(MGArgsT<typename St::T> m, 
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partials[3 * bi] = red[0][0];
-    partials[3 * bi + 1] = red[1][0];
-    partials[3 * bi + 2] = red[2][0];
+    m.dcPartials[3 * bi] = red[0][0];
+    m.dcPartials[3 * bi + 1] = red[1][0];
+    m.dcPartials[3 * bi + 2] = red[2][0];
   }
 }
-// The totals of the per-brick sums in a fixed order: out = (sum r^2, sum b^2, max |r|,
-// scale), the scale being the largest power of two not above max |r| (1 for zero or
-// subnormal residuals). Scaled by it, the correction equation's values lie in (-2, 2),
-// inside every storage format's range.
-__global__ void __launch_bounds__(256) k_dc_norms(const double *partials, const int *counts, double *out) {
+FP_MG_PAIR(dc_residual, 256)
+// The totals of the per-brick sums in a fixed order: (sum r^2, sum b^2, max |r|, scale), the
+// scale being the largest power of two not above max |r| (1 for zero or subnormal
+// residuals). Scaled by it, the correction equation's values lie in (-2, 2), inside every
+// storage format's range.
+template <class St> __device__ __forceinline__ void dc_norms_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  (void)code;
+  if (vb != 0) return;
   __shared__ double red[3][256];
-  const int blocks = counts[0];
+  const int blocks = m.counts[0];
   double rr = 0.0, bb = 0.0, big = 0.0;
   for (int i = threadIdx.x; i < blocks; i += 256) {
-    rr += partials[3 * i];
-    bb += partials[3 * i + 1];
-    big = big < partials[3 * i + 2] ? partials[3 * i + 2] : big;
+    rr += m.dcPartials[3 * i];
+    bb += m.dcPartials[3 * i + 1];
+    big = big < m.dcPartials[3 * i + 2] ? m.dcPartials[3 * i + 2] : big;
   }
   red[0][threadIdx.x] = rr;
   red[1][threadIdx.x] = bb;
@@ -612,58 +672,64 @@ __global__ void __launch_bounds__(256) k_dc_norms(const double *partials, const 
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = red[0][0];
-    out[1] = red[1][0];
-    out[2] = red[2][0];
+    m.dcNorms[0] = red[0][0];
+    m.dcNorms[1] = red[1][0];
+    m.dcNorms[2] = red[2][0];
     uint32_t bits = __float_as_uint((float)red[2][0]) & 0x7f800000u;
-    out[3] = bits ? (double)__uint_as_float(bits) : 1.0;
+    m.dcNorms[3] = bits ? (double)__uint_as_float(bits) : 1.0;
   }
 }
+FP_MG_PAIR(dc_norms, 256)
 // Level 0's right-hand side: r / scale, stored.
-template <class St>
-__global__ void __launch_bounds__(256) k_dc_scale(MGArgsT<typename St::T> m, const float *r, const double *norms) {
-  const int bi = xcdItem(m.counts[0]);
+template <class St> __device__ __forceinline__ void dc_scale_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  (void)code;
+  const int bi = xcdItem(vb, m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
   const int slot = entrySlot(m.dir.entries[id], m.L.side);
   const auto &lv = m.lv[0];
-  const float inv = 1.0f / (float)norms[3];
+  const float inv = 1.0f / (float)m.dcNorms[3];
   int n = lv.side * lv.side * lv.side;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     size_t a = (size_t)slot * n + l;
-    lv.rhs[a] = St::st(r[a] * inv);
+    lv.rhs[a] = St::st(m.dcR[a] * inv);
   }
 }
+FP_MG_PAIR(dc_scale, 256)
 // x += scale * e, level 0's solution of the scaled correction equation, on unknown cells.
-template <class St>
-__global__ void __launch_bounds__(256) k_dc_update(MGArgsT<typename St::T> m, float *x, const double *norms) {
-  const int bi = xcdItem(m.counts[0]);
+template <class St> __device__ __forceinline__ void dc_update_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int bi = xcdItem(vb, m.counts[0]);
   if (bi < 0) return;
   const int id = m.active[bi];
   const int slot = entrySlot(m.dir.entries[id], m.L.side);
-  const auto &lv = m.lv[0];
-  const float scale = (float)norms[3];
+  const auto lv = levelView(m, 0, code >> 8);
+  const float scale = (float)m.dcNorms[3];
   int n = lv.side * lv.side * lv.side;
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     size_t a = (size_t)slot * n + l;
     int e = lv.eflag[a];
-    x[a] = (!(e & STORED_SOLID) && (e & UNKNOWN)) ? x[a] + scale * St::ld(lv.p[a]) : 0.f;
+    m.dcX[a] = (!(e & STORED_SOLID) && (e & UNKNOWN)) ? m.dcX[a] + scale * St::ld(lv.p[a]) : 0.f;
   }
 }
+FP_MG_PAIR(dc_update, 256)
 
-#define FP_MG_INSTANTIATE(St)                                                                             \
-  template __global__ void k_mg_init<St>(MGArgsT<St::T>);                                                  \
-  template __global__ void k_mg_relax_bricks<St>(MGArgsT<St::T>, int);                                     \
-  template __global__ void k_mg_relax_tiles<St>(MGArgsT<St::T>, int);                                      \
-  template __global__ void k_mg_restrict_bricks<St>(MGArgsT<St::T>, int);                                  \
-  template __global__ void k_mg_restrict_tiles<St>(MGArgsT<St::T>, int);                                   \
-  template __global__ void k_mg_prolong_bricks<St>(MGArgsT<St::T>, int);                                   \
-  template __global__ void k_mg_prolong_tiles<St>(MGArgsT<St::T>, int);                                    \
-  template __global__ void k_mg_bottom<St>(MGArgsT<St::T>, int);                                           \
-  template __global__ void k_mg_bottom64<St>(MGArgsT<St::T>, int);                                         \
-  template __global__ void k_dc_residual<St>(MGArgsT<St::T>, const float *, const float *, float *, double *); \
-  template __global__ void k_dc_scale<St>(MGArgsT<St::T>, const float *, const double *);                   \
-  template __global__ void k_dc_update<St>(MGArgsT<St::T>, float *, const double *);
+#define FP_MG_INSTANCE(NAME, St)                                         \
+  template __global__ void k_##NAME<St>(MGArgsT<St::T>, int);           \
+  template __global__ void k_##NAME##_batch<St>(const MGArgsT<St::T> *, const int *, int, int);
+#define FP_MG_INSTANTIATE(St)              \
+  FP_MG_INSTANCE(mg_init, St)              \
+  FP_MG_INSTANCE(mg_relax_bricks, St)      \
+  FP_MG_INSTANCE(mg_relax_tiles, St)       \
+  FP_MG_INSTANCE(mg_restrict_bricks, St)   \
+  FP_MG_INSTANCE(mg_restrict_tiles, St)    \
+  FP_MG_INSTANCE(mg_prolong_bricks, St)    \
+  FP_MG_INSTANCE(mg_prolong_tiles, St)     \
+  FP_MG_INSTANCE(mg_bottom, St)            \
+  FP_MG_INSTANCE(mg_bottom64, St)          \
+  FP_MG_INSTANCE(dc_residual, St)          \
+  FP_MG_INSTANCE(dc_norms, St)             \
+  FP_MG_INSTANCE(dc_scale, St)             \
+  FP_MG_INSTANCE(dc_update, St)
 FP_MG_INSTANTIATE(StoreF32)
 FP_MG_INSTANTIATE(StoreBF16)
 FP_MG_INSTANTIATE(StoreF16)

@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) k_export_pages(RenderArgs r) {
 // taps, clamped to the box, has its two cells per axis there. The pages of the 27 bricks
 // around are staged either way.
 __global__ void __launch_bounds__(256) k_smooth_smoke(RenderArgs r, int chunks, int staged) {
-  const int item = xcdItem(r.counts[0] * chunks);
+  const int item = xcdItem((int)blockIdx.x, r.counts[0] * chunks);
   if (item < 0) return;
   const int i = item / chunks, chunk = item % chunks;
   const int id = r.active[i];
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(256) k_smooth_smoke(RenderArgs r, int chunks, 
 // 12^3 cells of the block and its two-cell halo whose bricks have slots. A workgroup per
 // listed brick; its lanes OR their flags through LDS.
 __global__ void __launch_bounds__(256) k_occupancy(RenderArgs r) {
-  const int i = xcdItem(r.counts[0]);
+  const int i = xcdItem((int)blockIdx.x, r.counts[0]);
   if (i < 0) return;
   const int id = r.active[i];
   int page, brick[3];
@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(256) k_occupancy(RenderArgs r) {
 // slot, and texels on the near faces also fill the aprons of the bricks before) or per
 // slot given out this step (`fresh` = 1, lightFresh: every texel of the slot).
 __global__ void __launch_bounds__(64) k_light(RenderArgs r, int fresh) {
-  const int i = xcdItem(fresh ? r.counts[1] : r.counts[0]);
+  const int i = xcdItem((int)blockIdx.x, fresh ? r.counts[1] : r.counts[0]);
   if (i < 0) return;
   const int id = fresh ? r.zeroList[2 * i] : r.active[i];
   int page, brick[3];

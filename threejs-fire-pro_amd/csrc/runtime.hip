@@ -11,9 +11,11 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -107,6 +109,29 @@ __global__ void k_readback(const int *counts, int *poolState, const uint32_t *ti
   if (i < cap) out[4 + i] = (int)tileFlags[i];
 }
 __global__ void k_reset_failures(int *poolState) { poolState[1] = 0; }
+
+// The readbacks of a batch, a workgroup per simulation, each into its own words of one
+// buffer; the failure counters restart for the next step.
+struct ReadbackArgs {
+  const int *counts;
+  int *poolState;
+  const uint32_t *tileFlags;
+  int cap;
+  int *out;
+};
+__global__ void __launch_bounds__(256) k_readback_batch(const ReadbackArgs *args, int n) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const ReadbackArgs a = args[i];
+  if (threadIdx.x == 0) {
+    a.out[0] = a.counts[0];
+    a.out[1] = a.poolState[0];
+    a.out[2] = a.poolState[1];
+    a.out[3] = a.counts[1] + a.counts[5];
+    a.poolState[1] = 0;
+  }
+  for (int t = threadIdx.x; t < a.cap; t += blockDim.x) a.out[4 + t] = (int)a.tileFlags[t];
+}
 
 // ---- the curl-noise table (curl-noise.wgsl:1-45), built once on the host -----------------
 
@@ -805,6 +830,9 @@ struct fp_sim {
     a.colliders = colliders.p;
     a.solids = solids.p;
     a.colliderCount = colliderCount;
+    a.buildSolids = (colliderCount > 0 || solidsStale) ? 1 : 0;
+    a.solidsEvery = solidsStale ? 1 : 0;
+    a.p = pools();
     return a;
   }
   Pools pools() {
@@ -867,18 +895,23 @@ struct fp_sim {
     return m;
   }
 
-  void step(const fp_frame &f, double dt) {
+  static void checkStep(const fp_frame &f, double dt) {
     if (!std::isfinite(dt) || dt <= 0 || dt > MAX_DT)
       throw InvalidArgument("Simulation step must be in (0, 1/30]. Use fixed substeps.");
     if (f.source_count < 0 || f.force_count < 0 || f.collider_count < 0)
       throw InvalidArgument("frame counts must be nonnegative.");
+  }
+  void step(const fp_frame &f, double dt) {
+    checkStep(f, dt);
     substep(f, dt);
     time += dt;
     steps++;
     stepped = true;
   }
 
-  void substep(const fp_frame &f, double dt) {
+  // The host half of a step before its kernels (FluidSimulation.ts:1374-1420): tiles,
+  // pool growth, the scene and the parameters.
+  void prepare(const fp_frame &f, double dt) {
     if (dir.ground != ground) {
       dir.ground = ground;
       geometryDirty = true;
@@ -906,9 +939,13 @@ struct fp_sim {
     mayHaveFlame = mayHaveFlame || fuel.enabled || anyFlame;
     setParams(f, dt);
     hostMark("host_scene", h0);
+  }
+
+  void substep(const fp_frame &f, double dt) {
+    prepare(f, dt);
     enqueueStep();
-    h0 = Clock::now();
-    readActivity();
+    Clock::time_point h0 = Clock::now();
+    readActivity(pinned);
     hostMark("host_activity", h0);
   }
 
@@ -925,10 +962,10 @@ struct fp_sim {
     k_free_bricks<<<blocks, 256, 0, stream>>>(a);
     k_allocate_bricks<<<blocks, 256, 0, stream>>>(a);
     k_link_bricks<<<(slotsGrid + 63) / 64, 64, 0, stream>>>(a);
-    if (colliderCount > 0 || solidsStale) k_build_solids<<<slotsGrid, 64, 0, stream>>>(a, solidsStale ? 1 : 0);
+    if (a.buildSolids) k_build_solids<<<slotsGrid, 64, 0, stream>>>(a);
     solidsStale = false;
     const int zchunks = std::max(1, nF / 1024);
-    k_zero_fresh<<<slotsGrid * zchunks, 256, 0, stream>>>(a, pools(), zchunks);
+    k_zero_fresh<<<slotsGrid * zchunks, 256, 0, stream>>>(a, zchunks);
     end(m0, "allocation");
 
     StepArgs s = stepArgs();
@@ -951,15 +988,16 @@ struct fp_sim {
     int m5 = begin();
     if (P.vorticity != 0.f) k_curl<<<slotsGrid, 256, 0, stream>>>(s);
     // The velocity predictor is dead after correction: it holds the cell forces.
-    k_cell_forces<<<slotsGrid, 256, 0, stream>>>(s, Vadv.p);
-    k_forces<<<slotsGrid, 256, 0, stream>>>(s, Vadv.p);
-    k_divergence<<<slotsGrid, 256, 0, stream>>>(s, D.p);
+    k_cell_forces<<<slotsGrid, 256, 0, stream>>>(s);
+    k_forces<<<slotsGrid, 256, 0, stream>>>(s);
+    k_divergence<<<slotsGrid, 256, 0, stream>>>(s);
     end(m5, "forces_divergence");
     int m6 = begin();
     solvePressure(slotsGrid);
     end(m6, "pressure");
     int m7 = begin();
-    k_project<<<slotsGrid, 256, 0, stream>>>(s, latestPressure);
+    s.pressure = latestPressure;
+    k_project<<<slotsGrid, 256, 0, stream>>>(s);
     end(m7, "project");
     k_readback<<<(unsigned)((capacity + 255) / 256 + 1), 256, 0, stream>>>(counts.p, poolState.p, activity.p + n,
                                                                            capacity, readback.p);
@@ -977,6 +1015,7 @@ struct fp_sim {
   // defect-correction loop around V-cycles.
   void solvePressure(unsigned slotsGrid) {
     MGArgs m = mgArgs();
+    m.topologyAll = geometryDirty ? 1 : 0;
     if (geometryDirty) {
       k_topology_bricks<<<slotsGrid, 256, 0, stream>>>(m, 0);
       for (int L = brickLevels; L <= bottom; L++) k_topology_tiles<<<capacity, 64, 0, stream>>>(m, L);
@@ -986,8 +1025,9 @@ struct fp_sim {
     k_eflags_bricks<<<slotsGrid, 256, 0, stream>>>(m);
     for (int L = brickLevels; L <= bottom; L++) k_eflags_tiles<<<capacity, 64, 0, stream>>>(m, L);
     if (pressureMode == 0) {
-      cycle<StoreF32>(m, 0, slotsGrid, false);
-      latestPressure = m.lv[0].p;  // level 0's iterate lives in P0 or P1
+      int flips = 0;
+      cycle<StoreF32>(m, 0, slotsGrid, false, flips);
+      latestPressure = (flips & 1) ? m.lv[0].q : m.lv[0].p;  // P0 or P1
       lastCycles = 1;
       lastResidual = -1.0;
       return;
@@ -1017,6 +1057,7 @@ struct fp_sim {
     h.solids = m.solids;
     h.bottom = m.bottom;
     h.weight = m.weight;
+    h.topologyAll = m.topologyAll;
     for (int L = 0; L <= bottom; L++) {
       h.lv[L].p = reinterpret_cast<T *>(lv16[L].p.p);
       h.lv[L].q = reinterpret_cast<T *>(lv16[L].q.p);
@@ -1050,65 +1091,75 @@ struct fp_sim {
         }
       }
   }
+  template <class T> void setCorrection(MGArgsT<T> &m) {
+    m.dcB = D.p;
+    m.dcX = dcX.p;
+    m.dcR = dcR.p;
+    m.dcPartials = dcPartials.p;
+    m.dcNorms = dcNorms.p;
+  }
 
   // Defect correction: x = 0; r = b - A x; while |r| / |b| > tol, solve A e = r / s with
   // one V-cycle in the levels' storage (s the power of two under max |r|) and add s e to x.
   // Norms are FP64 sums on the device; the host reads them to decide.
   template <class St> void correct(MGArgsT<typename St::T> m, unsigned slotsGrid) {
+    setCorrection(m);
     HIPCHECK(hipMemsetAsync(dcX.p, 0, dcX.bytes(), stream));
     lastCycles = 0;
     lastResidual = 1.0;
     for (int it = 0; it <= pressureMaxCycles; it++) {
-      k_dc_residual<St><<<slotsGrid, 256, 0, stream>>>(m, D.p, dcX.p, dcR.p, dcPartials.p);
-      k_dc_norms<<<1, 256, 0, stream>>>(dcPartials.p, counts.p, dcNorms.p);
+      k_dc_residual<St><<<slotsGrid, 256, 0, stream>>>(m, 0);
+      k_dc_norms<St><<<1, 256, 0, stream>>>(m, 0);
       HIPCHECK(hipMemcpyAsync(pinnedNorms, dcNorms.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
       lastResidual = pinnedNorms[1] > 0 ? std::sqrt(pinnedNorms[0] / pinnedNorms[1]) : 0.0;
       if (lastResidual <= pressureTol || it == pressureMaxCycles) break;
-      k_dc_scale<St><<<slotsGrid, 256, 0, stream>>>(m, dcR.p, dcNorms.p);
-      MGArgsT<typename St::T> c = m;
-      cycle<St>(c, 0, slotsGrid, true);
-      k_dc_update<St><<<slotsGrid, 256, 0, stream>>>(c, dcX.p, dcNorms.p);
+      k_dc_scale<St><<<slotsGrid, 256, 0, stream>>>(m, 0);
+      int flips = 0;
+      cycle<St>(m, 0, slotsGrid, true, flips);
+      k_dc_update<St><<<slotsGrid, 256, 0, stream>>>(m, flips << 8);
       lastCycles = it + 1;
     }
   }
 
-  template <class St> void relax(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid) {
-    if (L < brickLevels) k_mg_relax_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, L);
-    else k_mg_relax_tiles<St><<<capacity, 64, 0, stream>>>(m, L);
-    std::swap(m.lv[L].p, m.lv[L].q);
+  // One V-cycle from level L (PressureSolver.ts:303-363). `flips` tracks which array holds
+  // each level's iterate: a sweep writes the other array and flips its level's bit.
+  template <class St> void relax(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid, int &flips) {
+    if (L < brickLevels) k_mg_relax_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, L | flips << 8);
+    else k_mg_relax_tiles<St><<<capacity, 64, 0, stream>>>(m, L | flips << 8);
+    flips ^= 1 << L;
   }
-  template <class St> void cycle(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid, bool bottom64) {
+  template <class St> void cycle(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid, bool bottom64, int &flips) {
     if (L == bottom) {
-      if (bottom64) k_mg_bottom64<St><<<1, 1024, 0, stream>>>(m, 19);
-      else if (capacity <= 1024) k_mg_bottom<St><<<1, 1024, 0, stream>>>(m, 19);
+      if (bottom64) k_mg_bottom64<St><<<1, 1024, 0, stream>>>(m, flips << 8);
+      else if (capacity <= 1024) k_mg_bottom<St><<<1, 1024, 0, stream>>>(m, flips << 8);
       else
-        for (int i = 0; i < 19; i++) relax<St>(m, L, slotsGrid);
+        for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax<St>(m, L, slotsGrid, flips);
       return;
     }
-    if (L == 0) k_mg_init<St><<<slotsGrid, 256, 0, stream>>>(m);
+    if (L == 0) k_mg_init<St><<<slotsGrid, 256, 0, stream>>>(m, flips << 8);
     else {
-      relax<St>(m, L, slotsGrid);
-      relax<St>(m, L, slotsGrid);
+      relax<St>(m, L, slotsGrid, flips);
+      relax<St>(m, L, slotsGrid, flips);
     }
-    if (L + 1 < brickLevels) k_mg_restrict_bricks<St><<<slotsGrid, 64, 0, stream>>>(m, L + 1);
-    else k_mg_restrict_tiles<St><<<capacity, 64, 0, stream>>>(m, L + 1);
-    cycle<St>(m, L + 1, slotsGrid, bottom64);
-    if (L < brickLevels) k_mg_prolong_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, L);
-    else k_mg_prolong_tiles<St><<<capacity, 64, 0, stream>>>(m, L);
-    std::swap(m.lv[L].p, m.lv[L].q);
-    relax<St>(m, L, slotsGrid);
-    relax<St>(m, L, slotsGrid);
-    relax<St>(m, L, slotsGrid);
+    if (L + 1 < brickLevels) k_mg_restrict_bricks<St><<<slotsGrid, 64, 0, stream>>>(m, (L + 1) | flips << 8);
+    else k_mg_restrict_tiles<St><<<capacity, 64, 0, stream>>>(m, (L + 1) | flips << 8);
+    cycle<St>(m, L + 1, slotsGrid, bottom64, flips);
+    if (L < brickLevels) k_mg_prolong_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, L | flips << 8);
+    else k_mg_prolong_tiles<St><<<capacity, 64, 0, stream>>>(m, L | flips << 8);
+    flips ^= 1 << L;
+    relax<St>(m, L, slotsGrid, flips);
+    relax<St>(m, L, slotsGrid, flips);
+    relax<St>(m, L, slotsGrid, flips);
   }
 
   // readActivity (FluidSimulation.ts:1701-1746), applied at once.
-  void readActivity() {
-    int bricks = pinned[0], free = pinned[1], failures = pinned[2];
-    lastZeroed = pinned[3];
+  void readActivity(const int *words) {
+    int bricks = words[0], free = words[1], failures = words[2];
+    lastZeroed = words[3];
     contentTiles.clear();
     for (int t = 0; t < capacity; t++) {
-      uint32_t flags = (uint32_t)pinned[4 + t];
+      uint32_t flags = (uint32_t)words[4 + t];
       if (flags && t < (int)dir.coords.size() && dir.coords[t][3])
         contentTiles.push_back({{dir.coords[t][0], dir.coords[t][1], dir.coords[t][2]}, flags});
     }
@@ -1467,6 +1518,244 @@ template <class Fn> int guard(Fn fn) {
 }
 }  // namespace
 
+
+// A batch advances many simulations of one brick layout together (the serving form): their
+// host halves run one after another, then every stage is one launch over all of them
+// (k_*_batch), with one upload of the arguments and one readback. Results equal those of
+// stepping each simulation alone, bit for bit: the same kernel bodies run on the same data.
+struct fp_batch {
+  hipStream_t stream = nullptr;
+  DBuf<char> devArgs;
+  char *hostArgs = nullptr;
+  size_t hostArgsBytes = 0;
+  DBuf<int> readback;
+  int *pinnedReadback = nullptr;
+  size_t pinnedWords = 0;
+  int grid = 2048;  // persistent workgroups per launch, a multiple of 8
+  double lastMs = 0;
+
+  void create() { HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
+  void destroy() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    devArgs.release();
+    readback.release();
+    if (hostArgs) (void)hipHostFree(hostArgs);
+    if (pinnedReadback) (void)hipHostFree(pinnedReadback);
+    if (stream) (void)hipStreamDestroy(stream);
+    hostArgs = nullptr;
+    pinnedReadback = nullptr;
+    stream = nullptr;
+  }
+
+  // Packs the per-simulation arrays and block prefixes into one pinned buffer.
+  struct Packer {
+    std::vector<char> bytes;
+    template <class T> size_t put(const std::vector<T> &v) {
+      size_t off = (bytes.size() + 255) / 256 * 256;
+      bytes.resize(off + v.size() * sizeof(T));
+      if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+      return off;
+    }
+  };
+  // Prefix of per-simulation block counts, each rounded up to `round`.
+  static std::vector<int> prefix(const std::vector<int> &counts, int round) {
+    std::vector<int> out(counts.size() + 1, 0);
+    for (size_t i = 0; i < counts.size(); i++) out[i + 1] = out[i] + (counts[i] + round - 1) / round * round;
+    return out;
+  }
+
+  template <class Args, class Kernel, class... Extra>
+  void launch(Kernel kernel, unsigned threads, size_t argOff, size_t startOff, int total, int n, Extra... extra) {
+    if (total <= 0) return;
+    unsigned g = (unsigned)std::min(total, grid);
+    const Args *args = reinterpret_cast<const Args *>(devArgs.p + argOff);
+    const int *starts = reinterpret_cast<const int *>(devArgs.p + startOff);
+    kernel<<<g, threads, 0, stream>>>(args, starts, n, extra...);
+  }
+
+  void step(fp_sim *const *sims, const fp_frame *frames, int n, double dt) {
+    if (n <= 0) return;
+    for (int i = 0; i < n; i++) {
+      if (!sims[i]) throw InvalidArgument("batch member is null.");
+      if (sims[i]->B != sims[0]->B || sims[i]->d != sims[0]->d)
+        throw InvalidArgument("a batch's simulations must share brick size and velocity divisor.");
+      if (sims[i]->pressureMode != 0) throw InvalidArgument("batched simulations use the V-cycle pressure policy.");
+      for (int j = 0; j < i; j++)
+        if (sims[j] == sims[i]) throw InvalidArgument("a simulation appears twice in a batch.");
+      fp_sim::checkStep(frames[i], dt);
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    // Host halves, enqueued on the batch's stream.
+    std::vector<hipStream_t> own(n);
+    for (int i = 0; i < n; i++) {
+      fp_sim *s = sims[i];
+      HIPCHECK(hipStreamSynchronize(s->stream));
+      own[i] = s->stream;
+      s->stream = stream;
+    }
+    try {
+      for (int i = 0; i < n; i++) sims[i]->prepare(frames[i], dt);
+      enqueue(sims, n);
+    } catch (...) {
+      for (int i = 0; i < n; i++) sims[i]->stream = own[i];
+      throw;
+    }
+    for (int i = 0; i < n; i++) sims[i]->stream = own[i];
+    size_t off = 0;
+    for (int i = 0; i < n; i++) {
+      fp_sim *s = sims[i];
+      s->readActivity(pinnedReadback + off);
+      off += 4 + (size_t)s->capacity;
+      s->time += dt;
+      s->steps++;
+      s->stepped = true;
+    }
+    lastMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  void enqueue(fp_sim *const *sims, int n) {
+    fp_sim &s0 = *sims[0];
+    const int nF = s0.nF, brickLevels = s0.brickLevels, bottom = s0.bottom;
+    const int zchunks = std::max(1, nF / 1024), fchunks = std::max(1, nF / 256);
+    std::vector<AllocArgs> alloc(n);
+    std::vector<StepArgs> pre(n), post(n);
+    std::vector<MGArgs> mg(n);
+    std::vector<ReadbackArgs> rb(n);
+    std::vector<int> cBlocks(n), cOne(n, 1), cLink(n), cSlots(n), cSlots2(n), cZero(n), cField(n), cTiles(n);
+    size_t words = 0;
+    bool smallBottom = true;
+    for (int i = 0; i < n; i++) {
+      fp_sim &s = *sims[i];
+      const int slotsGrid = (std::max(1, s.poolSlots) + 7) / 8 * 8;
+      cBlocks[i] = (s.brickCount() + 255) / 256;
+      cLink[i] = (slotsGrid + 63) / 64;
+      cSlots[i] = slotsGrid;
+      cSlots2[i] = slotsGrid * 2;
+      cZero[i] = slotsGrid * zchunks;
+      cField[i] = slotsGrid * fchunks;
+      cTiles[i] = s.capacity;
+      smallBottom = smallBottom && s.capacity <= 1024;
+      alloc[i] = s.allocArgs();
+      s.solidsStale = false;
+      pre[i] = s.stepArgs();
+      std::swap(s.F, s.Fn);
+      post[i] = s.stepArgs();
+      mg[i] = s.mgArgs();
+      mg[i].topologyAll = s.geometryDirty ? 1 : 0;
+      s.geometryDirty = false;
+      words += 4 + (size_t)s.capacity;
+    }
+    // The V-cycle's launches, run dry first to learn where level 0's final iterate lands
+    // (P0 or P1, the same for every simulation), then for real.
+    bool dry = true;
+    int f = 0;
+    size_t oMg = 0, sSlots = 0, sTiles = 0, sOne = 0;
+    int tSlots = 0, tTiles = 0, tOne = 0;
+    auto mgLaunch = [&](auto kernel, unsigned threads, size_t so, int total, int code) {
+      if (!dry) launch<MGArgs>(kernel, threads, oMg, so, total, n, code);
+    };
+    std::function<void(int)> relax = [&](int L) {
+      if (L < brickLevels) mgLaunch(k_mg_relax_bricks_batch<StoreF32>, 256, sSlots, tSlots, L | f << 8);
+      else mgLaunch(k_mg_relax_tiles_batch<StoreF32>, 64, sTiles, tTiles, L | f << 8);
+      f ^= 1 << L;
+    };
+    std::function<void(int)> cycle = [&](int L) {
+      if (L == bottom) {
+        if (smallBottom) mgLaunch(k_mg_bottom_batch<StoreF32>, 1024, sOne, tOne, f << 8);
+        else
+          for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax(L);
+        return;
+      }
+      if (L == 0) mgLaunch(k_mg_init_batch<StoreF32>, 256, sSlots, tSlots, f << 8);
+      else {
+        relax(L);
+        relax(L);
+      }
+      if (L + 1 < brickLevels) mgLaunch(k_mg_restrict_bricks_batch<StoreF32>, 64, sSlots, tSlots, (L + 1) | f << 8);
+      else mgLaunch(k_mg_restrict_tiles_batch<StoreF32>, 64, sTiles, tTiles, (L + 1) | f << 8);
+      cycle(L + 1);
+      if (L < brickLevels) mgLaunch(k_mg_prolong_bricks_batch<StoreF32>, 256, sSlots, tSlots, L | f << 8);
+      else mgLaunch(k_mg_prolong_tiles_batch<StoreF32>, 64, sTiles, tTiles, L | f << 8);
+      f ^= 1 << L;
+      relax(L);
+      relax(L);
+      relax(L);
+    };
+    cycle(0);
+    const int flips = f;
+    for (int i = 0; i < n; i++) post[i].pressure = (flips & 1) ? mg[i].lv[0].q : mg[i].lv[0].p;
+    if (readback.n < words) readback.alloc(words + 1024);
+    if (pinnedWords < words) {
+      if (pinnedReadback) (void)hipHostFree(pinnedReadback);
+      pinnedWords = words + 1024;
+      HIPCHECK(hipHostMalloc((void **)&pinnedReadback, pinnedWords * sizeof(int), hipHostMallocDefault));
+    }
+    size_t o = 0;
+    for (int i = 0; i < n; i++) {
+      fp_sim &s = *sims[i];
+      rb[i] = ReadbackArgs{s.counts.p, s.poolState.p, s.activity.p + s.brickCount(), s.capacity, readback.p + o};
+      o += 4 + (size_t)s.capacity;
+    }
+    Packer pk;
+    const size_t oAlloc = pk.put(alloc), oPre = pk.put(pre), oPost = pk.put(post), oRb = pk.put(rb);
+    oMg = pk.put(mg);
+    const std::vector<int> pBlocks = prefix(cBlocks, 1), pOne = prefix(cOne, 1), pLink = prefix(cLink, 1),
+                           pSlots = prefix(cSlots, 8), pSlots2 = prefix(cSlots2, 8), pZero = prefix(cZero, 8),
+                           pField = prefix(cField, 8), pTiles = prefix(cTiles, 8);
+    const size_t sBlocks = pk.put(pBlocks), sLink = pk.put(pLink), sSlots2 = pk.put(pSlots2), sZero = pk.put(pZero),
+                 sField = pk.put(pField);
+    sOne = pk.put(pOne);
+    sSlots = pk.put(pSlots);
+    sTiles = pk.put(pTiles);
+    if (hostArgsBytes < pk.bytes.size()) {
+      if (hostArgs) (void)hipHostFree(hostArgs);
+      hostArgsBytes = pk.bytes.size() * 2;
+      HIPCHECK(hipHostMalloc((void **)&hostArgs, hostArgsBytes, hipHostMallocDefault));
+      devArgs.alloc(hostArgsBytes);
+    }
+    HIPCHECK(hipStreamSynchronize(stream));  // the previous step's upload has been read
+    std::memcpy(hostArgs, pk.bytes.data(), pk.bytes.size());
+    HIPCHECK(hipMemcpyAsync(devArgs.p, hostArgs, pk.bytes.size(), hipMemcpyHostToDevice, stream));
+    const int tBlocks = pBlocks[n], tLink = pLink[n], tSlots2 = pSlots2[n], tZero = pZero[n], tField = pField[n];
+    tOne = pOne[n];
+    tSlots = pSlots[n];
+    tTiles = pTiles[n];
+
+    // Allocation (fluid-allocation.wgsl).
+    launch<AllocArgs>(k_build_bricks_batch, 256, oAlloc, sBlocks, tBlocks, n);
+    launch<AllocArgs>(k_scan_blocks_batch, 1024, oAlloc, sOne, tOne, n);
+    launch<AllocArgs>(k_free_bricks_batch, 256, oAlloc, sBlocks, tBlocks, n);
+    launch<AllocArgs>(k_allocate_bricks_batch, 256, oAlloc, sBlocks, tBlocks, n);
+    launch<AllocArgs>(k_link_bricks_batch, 64, oAlloc, sLink, tLink, n);
+    launch<AllocArgs>(k_build_solids_batch, 64, oAlloc, sSlots, tSlots, n);
+    launch<AllocArgs>(k_zero_fresh_batch, 256, oAlloc, sZero, tZero, n, zchunks);
+    // Transport, combustion, forces.
+    launch<StepArgs>(k_advect_velocity_batch, 256, oPre, sSlots, tSlots, n);
+    launch<StepArgs>(k_correct_velocity_batch, 256, oPre, sSlots, tSlots, n);
+    launch<StepArgs>(k_advect_scalars_batch, 256, oPre, sField, tField, n, fchunks);
+    launch<StepArgs>(k_evolve_fields_batch, 256, oPre, sField, tField, n, fchunks);
+    launch<StepArgs>(k_curl_batch, 256, oPost, sSlots, tSlots, n);
+    launch<StepArgs>(k_cell_forces_batch, 256, oPost, sSlots, tSlots, n);
+    launch<StepArgs>(k_forces_batch, 256, oPost, sSlots, tSlots, n);
+    launch<StepArgs>(k_divergence_batch, 256, oPost, sSlots, tSlots, n);
+    // Pressure: topology, flags, one V-cycle.
+    launch<MGArgs>(k_topology_bricks_batch, 256, oMg, sSlots, tSlots, n, 0);
+    for (int L = brickLevels; L <= bottom; L++) launch<MGArgs>(k_topology_tiles_batch, 64, oMg, sTiles, tTiles, n, L);
+    launch<MGArgs>(k_topology_bricks_batch, 256, oMg, sSlots2, tSlots2, n, 1);
+    launch<MGArgs>(k_eflags_bricks_batch, 256, oMg, sSlots, tSlots, n);
+    for (int L = brickLevels; L <= bottom; L++) launch<MGArgs>(k_eflags_tiles_batch, 64, oMg, sTiles, tTiles, n, L);
+    dry = false;
+    f = 0;
+    cycle(0);
+    launch<StepArgs>(k_project_batch, 256, oPost, sSlots, tSlots, n);
+    k_readback_batch<<<(unsigned)n, 256, 0, stream>>>(reinterpret_cast<const ReadbackArgs *>(devArgs.p + oRb), n);
+    HIPCHECK(hipMemcpyAsync(pinnedReadback, readback.p, words * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipGetLastError());
+    for (int i = 0; i < n; i++) sims[i]->latestPressure = const_cast<float *>(post[i].pressure);
+  }
+};
+
 extern "C" {
 
 const char *fp_last_error(void) { return g_error.c_str(); }
@@ -1713,6 +2002,41 @@ int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *co
     if (ms) ms[i] = e.second ? e.first / e.second : 0.0;
   }
   *count = n;
+  return FP_OK;
+}
+
+int fp_batch_create(fp_batch **out) {
+  if (!out) {
+    g_error = "fp_batch_create: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] {
+    std::unique_ptr<fp_batch> b(new fp_batch());
+    b->create();
+    *out = b.release();
+  });
+}
+
+void fp_batch_destroy(fp_batch *batch) {
+  if (!batch) return;
+  batch->destroy();
+  delete batch;
+}
+
+int fp_batch_step(fp_batch *batch, fp_sim *const *sims, const fp_frame *frames, int32_t count, double dt) {
+  if (!batch || (count > 0 && (!sims || !frames)) || count < 0) {
+    g_error = "fp_batch_step: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] { batch->step(sims, frames, count, dt); });
+}
+
+int fp_batch_last_ms(const fp_batch *batch, double *ms) {
+  if (!batch || !ms) {
+    g_error = "fp_batch_last_ms: null argument";
+    return FP_EINVAL;
+  }
+  *ms = batch->lastMs;
   return FP_OK;
 }
 

@@ -461,6 +461,10 @@ def load(path: str = LIBRARY):
         "fp_sim_profile": (C.c_int, [C.c_void_p, C.c_int32]),
         "fp_sim_profile_read": (C.c_int, [C.c_void_p, P(C.c_char_p), P(C.c_double),
                                           P(C.c_int32)]),
+        "fp_batch_create": (C.c_int, [P(C.c_void_p)]),
+        "fp_batch_destroy": (None, [C.c_void_p]),
+        "fp_batch_step": (C.c_int, [C.c_void_p, P(C.c_void_p), P(Frame), C.c_int32, C.c_double]),
+        "fp_batch_last_ms": (C.c_int, [C.c_void_p, P(C.c_double)]),
         "fp_sim_set_pressure": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32]),
         "fp_sim_pressure_report": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_double)]),
         "fp_sim_render": (C.c_int, [C.c_void_p, P(Camera), P(RenderSettings), P(C.c_float)]),

@@ -214,3 +214,44 @@ class FluidSimulation:
         count = C.c_int32(64)
         abi.check(self._lib.fp_sim_profile_read(self._h(), names, ms, C.byref(count)), self._lib)
         return {names[i].decode(): ms[i] for i in range(count.value)}
+
+
+class Batch:
+    """Steps many simulations of one brick layout together: one launch per stage over all
+    of their bricks (fp_batch_step). Any set of simulations may be stepped on each call."""
+
+    def __init__(self):
+        self._lib = abi.load()
+        h = C.c_void_p()
+        abi.check(self._lib.fp_batch_create(C.byref(h)), self._lib)
+        self._handle = h
+
+    def step(self, sims, frames, dt: float = 1 / 60) -> None:
+        n = len(sims)
+        if n != len(frames):
+            raise ValueError("one frame per simulation")
+        packed = [f if isinstance(f, abi.PackedFrame) else abi.PackedFrame(f) for f in frames]
+        handles = (C.c_void_p * max(1, n))(*[s._h().value for s in sims])
+        arr = (abi.Frame * max(1, n))()
+        for i, p in enumerate(packed):
+            arr[i] = p.frame
+        abi.check(self._lib.fp_batch_step(self._handle, handles, arr, n, dt), self._lib)
+        self._keep = packed  # the frames' arrays stay alive through the call
+
+    @property
+    def last_ms(self) -> float:
+        ms = C.c_double()
+        abi.check(self._lib.fp_batch_last_ms(self._handle, C.byref(ms)), self._lib)
+        return ms.value
+
+    def dispose(self) -> None:
+        if self._handle is not None:
+            self._lib.fp_batch_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.dispose()
+        except Exception:
+            pass
+
