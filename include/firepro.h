@@ -246,6 +246,9 @@ void fp_batch_destroy(fp_batch *batch);
 int fp_batch_step(fp_batch *batch, fp_sim *const *sims, const fp_frame *frames, int32_t count, double dt);
 /* Wall time of the latest fp_batch_step, host halves included, in ms. */
 int fp_batch_last_ms(const fp_batch *batch, double *ms);
+/* Its phases in ms: ms[0] host halves, [1] argument packing, [2] launches, [3] the wait for
+ * the device, [4] the activity readbacks. */
+int fp_batch_phases(const fp_batch *batch, double *ms);
 
 /* The pressure policy of the next steps. Defect correction keeps the solution x in FP32,
  * starts from x = 0 and repeats r = b - A x (FP32, each row's dot product and the norms in

@@ -107,3 +107,18 @@ def test_batch_rejects_mixed_layouts():
         batch.dispose()
         a.dispose()
         b.dispose()
+
+
+def test_serving_loop_completes():
+    """A short request stream through the continuous-batching server and the
+    one-simulation-per-launch loop: every request finishes its steps in both."""
+    from firepro import serving
+    for batched in (True, False):
+        reqs = serving.requests(count=12, rate=200.0, seed=3)
+        for r in reqs:
+            r.steps = min(r.steps, 20)
+        rep = serving.serve(reqs, batched=batched)
+        print(f"case serving batched={batched}: {rep.summary()}", flush=True)
+        assert rep.requests == 12
+        assert rep.sim_steps == sum(r.steps for r in reqs)
+        assert rep.voxel_steps > 0

@@ -1533,6 +1533,9 @@ struct fp_batch {
   size_t pinnedWords = 0;
   int grid = 2048;  // persistent workgroups per launch, a multiple of 8
   double lastMs = 0;
+  // Wall ms of the latest step's phases: host halves, argument packing, launches, the wait
+  // for the device, the activity readbacks.
+  double phaseMs[5] = {0, 0, 0, 0, 0};
 
   void create() { HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
   void destroy() {
@@ -1594,7 +1597,9 @@ struct fp_batch {
       s->stream = stream;
     }
     try {
+      auto p0 = std::chrono::steady_clock::now();
       for (int i = 0; i < n; i++) sims[i]->prepare(frames[i], dt);
+      phaseMs[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
       enqueue(sims, n);
     } catch (...) {
       for (int i = 0; i < n; i++) sims[i]->stream = own[i];
@@ -1602,6 +1607,7 @@ struct fp_batch {
     }
     for (int i = 0; i < n; i++) sims[i]->stream = own[i];
     size_t off = 0;
+    auto f0 = std::chrono::steady_clock::now();
     for (int i = 0; i < n; i++) {
       fp_sim *s = sims[i];
       s->readActivity(pinnedReadback + off);
@@ -1610,10 +1616,12 @@ struct fp_batch {
       s->steps++;
       s->stepped = true;
     }
+    phaseMs[4] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f0).count();
     lastMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 
   void enqueue(fp_sim *const *sims, int n) {
+    const auto e0 = std::chrono::steady_clock::now();
     fp_sim &s0 = *sims[0];
     const int nF = s0.nF, brickLevels = s0.brickLevels, bottom = s0.bottom;
     const int zchunks = std::max(1, nF / 1024), fchunks = std::max(1, nF / 256);
@@ -1714,6 +1722,8 @@ struct fp_batch {
       devArgs.alloc(hostArgsBytes);
     }
     HIPCHECK(hipStreamSynchronize(stream));  // the previous step's upload has been read
+    auto l0 = std::chrono::steady_clock::now();
+    phaseMs[1] = std::chrono::duration<double, std::milli>(l0 - e0).count();
     std::memcpy(hostArgs, pk.bytes.data(), pk.bytes.size());
     HIPCHECK(hipMemcpyAsync(devArgs.p, hostArgs, pk.bytes.size(), hipMemcpyHostToDevice, stream));
     const int tBlocks = pBlocks[n], tLink = pLink[n], tSlots2 = pSlots2[n], tZero = pZero[n], tField = pField[n];
@@ -1750,7 +1760,10 @@ struct fp_batch {
     launch<StepArgs>(k_project_batch, 256, oPost, sSlots, tSlots, n);
     k_readback_batch<<<(unsigned)n, 256, 0, stream>>>(reinterpret_cast<const ReadbackArgs *>(devArgs.p + oRb), n);
     HIPCHECK(hipMemcpyAsync(pinnedReadback, readback.p, words * sizeof(int), hipMemcpyDeviceToHost, stream));
+    auto w0 = std::chrono::steady_clock::now();
+    phaseMs[2] = std::chrono::duration<double, std::milli>(w0 - l0).count();
     HIPCHECK(hipStreamSynchronize(stream));
+    phaseMs[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     HIPCHECK(hipGetLastError());
     for (int i = 0; i < n; i++) sims[i]->latestPressure = const_cast<float *>(post[i].pressure);
   }
@@ -2036,7 +2049,16 @@ int fp_batch_last_ms(const fp_batch *batch, double *ms) {
     g_error = "fp_batch_last_ms: null argument";
     return FP_EINVAL;
   }
-  *ms = batch->lastMs;
+  ms[0] = batch->lastMs;
+  return FP_OK;
+}
+
+int fp_batch_phases(const fp_batch *batch, double *ms) {
+  if (!batch || !ms) {
+    g_error = "fp_batch_phases: null argument";
+    return FP_EINVAL;
+  }
+  for (int i = 0; i < 5; i++) ms[i] = batch->phaseMs[i];
   return FP_OK;
 }
 

@@ -244,6 +244,12 @@ class Batch:
         abi.check(self._lib.fp_batch_last_ms(self._handle, C.byref(ms)), self._lib)
         return ms.value
 
+    @property
+    def phases(self) -> dict:
+        ms = (C.c_double * 5)()
+        abi.check(self._lib.fp_batch_phases(self._handle, ms), self._lib)
+        return dict(zip(("host", "pack", "launch", "wait", "readback"), ms[:]))
+
     def dispose(self) -> None:
         if self._handle is not None:
             self._lib.fp_batch_destroy(self._handle)
