@@ -330,7 +330,8 @@ template <class R> class Sim {
   int pressureMaxCycles = 20;
   int lastCycles = 1;
   double lastResidual = -1;
-  std::vector<R> dcX;
+  std::vector<R> dcX, dcXlo;  // the solution as an unevaluated sum hi + lo
+  std::vector<R> PfineLo;     // the pressure's low part when it is such a pair, else empty
   R stored(R v) const {
     if constexpr (std::is_same<R, float>::value) {
       if (pressureMode == 2) return roundBF16(v);
@@ -402,6 +403,7 @@ template <class R> class Sim {
     Curl.assign((size_t)slots * nV * 4, R(0));
     D.assign((size_t)slots * nV, R(0));
     Pfine.assign((size_t)slots * nV, R(0));
+    PfineLo.clear();
     if (keep) solids.resize((size_t)slots * nF / 32, 0u);
     else solids.assign((size_t)slots * nF / 32, 0u);
     levels.assign(bottom + 1, Level());
@@ -2142,12 +2144,25 @@ template <class R> class Sim {
     if (pressureMode == 0) {
       cycle(0);
       Pfine = levels[0].p;
+      PfineLo.clear();
       lastCycles = 1;
       lastResidual = -1;
       return;
     }
     correct();
     Pfine = dcX;
+    PfineLo = dcXlo;
+  }
+  // hi + lo += v: Knuth's two-sum of hi and v, its error added to lo, then renormalized so
+  // that hi is the rounded value of hi + lo (no counterpart in the reference).
+  static void addToPair(R &hi, R &lo, R v) {
+    const R s = hi + v;
+    const R bp = s - hi;
+    const R err = (hi - (s - bp)) + (v - bp);
+    const R l = lo + err;
+    const R h = s + l;
+    lo = l - (h - s);
+    hi = h;
   }
   // Defect correction: x = 0; r = b - A x (sums of r^2 and b^2 in FP64); while
   // |r| / |b| > tol, one V-cycle on A e = r / s, s the largest power of two not above
@@ -2155,6 +2170,7 @@ template <class R> class Sim {
   void correct() {
     const std::vector<R> b = D;
     dcX.assign(b.size(), R(0));
+    dcXlo.assign(b.size(), R(0));
     std::vector<R> r(b.size(), R(0));
     lastCycles = 0;
     lastResidual = 1;
@@ -2166,18 +2182,18 @@ template <class R> class Sim {
         int flags = flagsOf(p, 0);
         R v = 0;
         if (!(flags & 1024) && activeCell(p, 0)) {
-          // The row's dot product in FP64 from the stored x, rounded once.
+          // The row's dot product in FP64 from x = hi + lo, rounded once.
           static const int OFF[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
           const double w = (double)axisWeight();
           double sum = 0;
           for (int k = 0; k < 6; k++) {
             if (!((flags >> k) & 1)) continue;
             int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
-            sum += w * (double)readAt(dcX, q, 0);
+            sum += w * ((double)readAt(dcX, q, 0) + (double)readAt(dcXlo, q, 0));
           }
           R w1 = axisWeight();
           R diag = w1 * openCount(flags, 0) + w1 * openCount(flags, 2) + w1 * openCount(flags, 4);
-          v = (R)((double)b[a] - (sum - (double)diag * (double)dcX[a]));
+          v = (R)((double)b[a] - (sum - (double)diag * ((double)dcX[a] + (double)dcXlo[a])));
           rr += (double)v * (double)v;
           bb += (double)b[a] * (double)b[a];
           big = fmax(big, fabs_(v));
@@ -2208,7 +2224,14 @@ template <class R> class Sim {
       forLevelCells(0, [&](const int p[3]) {
         long a = addr(p, 0);
         int flags = flagsOf(p, 0);
-        dcX[a] = (!(flags & 1024) && activeCell(p, 0)) ? dcX[a] + scale * levels[0].p[a] : R(0);
+        R hi = 0, lo = 0;
+        if (!(flags & 1024) && activeCell(p, 0)) {
+          hi = dcX[a];
+          lo = dcXlo[a];
+          addToPair(hi, lo, scale * levels[0].p[a]);
+        }
+        dcX[a] = hi;
+        dcXlo[a] = lo;
       });
       lastCycles = it + 1;
     }
@@ -2240,6 +2263,8 @@ template <class R> class Sim {
         return;
       }
       R cp = velScalar(Pfine, p[0], p[1], p[2]);
+      const bool pair = !PfineLo.empty();
+      const R cpLo = pair ? velScalar(PfineLo, p[0], p[1], p[2]) : R(0);
       const R *vf = &Vf[(size_t)s * nV * 3];
       bool here = pressureUnknown(p);
       for (int a = 0; a < 3; a++) {
@@ -2251,7 +2276,11 @@ template <class R> class Sim {
         R res;
         if (solidAt(npos)) res = 0;
         else if (!here && !pressureUnknown(nb)) res = 0;
-        else res = value - (velScalar(Pfine, nb[0], nb[1], nb[2]) - cp) * P.invHv;
+        else if (!pair) res = value - (velScalar(Pfine, nb[0], nb[1], nb[2]) - cp) * P.invHv;
+        else {
+          const R g = (velScalar(Pfine, nb[0], nb[1], nb[2]) - cp) + (velScalar(PfineLo, nb[0], nb[1], nb[2]) - cpLo);
+          res = value - g * P.invHv;
+        }
         o[a * nV + l] = res;
       }
     });

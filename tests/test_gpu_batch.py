@@ -19,7 +19,21 @@ def _scenes():
     ]
 
 
-def test_batch_equals_alone_with_joins_and_leaves():
+# Pressure policies: the reference's single V-cycle, and defect correction with levels in
+# FP32, BF16 and FP16 to per-simulation tolerances and cycle budgets (some steps stop on
+# the tolerance, some on the budget). With x in FP32 these scenes' relative residuals level
+# off between 1e-6 and 1e-4 (about the FP32 unit roundoff times the operator's condition
+# number), so the tolerances that must be met sit at 1e-4.
+POLICIES = [
+    ("vcycle", None),
+    ("fp32", [(1e-4, 12), (1e-5, 12), (1e-5, 2), (1e-6, 12)]),
+    ("bf16", [(1e-4, 16), (1e-4, 8), (1e-6, 2), (1e-5, 16)]),
+    ("fp16", [(1e-4, 16), (1e-5, 16), (1e-4, 16), (1e-6, 3)]),
+]
+
+
+@pytest.mark.parametrize("mode,tolerances", POLICIES, ids=[p[0] for p in POLICIES])
+def test_batch_equals_alone_with_joins_and_leaves(mode, tolerances):
     from firepro import scenes
     from firepro.engine import Batch, FluidSimulation
     # The campfire uses 16-cell bricks; a batch holds one layout, so use 8-cell ones here.
@@ -29,12 +43,18 @@ def test_batch_equals_alone_with_joins_and_leaves():
     frames = [list(sc.frames()) for sc in scs]
     alone = [FluidSimulation.create(sc.grid()) for sc in scs]
     mixed = [FluidSimulation.create(sc.grid()) for sc in scs]
+    if tolerances:
+        for i in range(len(scs)):
+            for sim in (alone[i], mixed[i]):
+                sim.set_pressure(mode, *tolerances[i])
+    reports = [[] for _ in scs]
     batch = Batch()
     try:
         for i, sim in enumerate(alone):
             for frame, g in frames[i]:
                 sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
                 sim.step(frame)
+                reports[i].append(sim.pressure_report())
         for t in range(max(s + len(f) for s, f in zip(start, frames))):
             members, fr = [], []
             for i, sim in enumerate(mixed):
@@ -44,8 +64,17 @@ def test_batch_equals_alone_with_joins_and_leaves():
                     sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
                     members.append(sim)
                     fr.append(frame)
-            print(f"case batch clock {t}: {len(members)} members", flush=True)
+            print(f"case batch {mode} clock {t}: {len(members)} members", flush=True)
             batch.step(members, fr)
+            for i, sim in enumerate(mixed):
+                k = t - start[i]
+                if 0 <= k < len(frames[i]):
+                    assert sim.pressure_report() == reports[i][k], (i, k)
+        if tolerances:
+            # Some steps stop on the tolerance, some on the cycle budget.
+            flat = [(c, r, tolerances[i]) for i in range(len(scs)) for c, r in reports[i]]
+            assert any(r <= tol for c, r, (tol, m) in flat), reports
+            assert any(c == m and r > tol for c, r, (tol, m) in flat), reports
         for i in range(len(scs)):
             msgs = table_mismatches(mixed[i].read_tables(), alone[i].read_tables())
             msgs += state_mismatches(mixed[i].read_state(pressure=True, expansion=True),
@@ -103,6 +132,13 @@ def test_batch_rejects_mixed_layouts():
             batch.step([a, b], f)
         with pytest.raises(ValueError, match="twice"):
             batch.step([a, a], f)
+        c = FluidSimulation.create(scenes.small(steps=1, brick=8).grid())
+        try:
+            c.set_pressure("bf16", 1e-6, 8)
+            with pytest.raises(ValueError, match="pressure policy"):
+                batch.step([a, c], f)
+        finally:
+            c.dispose()
     finally:
         batch.dispose()
         a.dispose()

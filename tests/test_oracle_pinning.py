@@ -336,3 +336,21 @@ def test_half_precision_levels_converge(mode):
         cycles, residual = sim.pressure_report()
         worst = max(worst, residual)
     assert worst <= 1e-6, worst
+
+
+def test_compensated_solution_passes_the_fp32_floor():
+    """Defect correction keeps x as a pair of FP32 values hi + lo. With a single FP32 x
+    these scenes' relative residuals level off between 1e-6 and 1e-4 (the unit roundoff
+    times the operator's condition number); with the pair, FP32 arithmetic drives them
+    below 1e-8. A dropped low part would stall at the single-FP32 level."""
+    for seed, kw in [(1, {}), (2, {"collider": False})]:
+        sc = scenes.small(seed=seed, steps=6, brick=8, divisor=2, **kw)
+        sim = oracle.OracleSimulation.create(sc.grid(), 32)
+        sim.set_pressure(1, 1e-8, 40)
+        worst = 0.0
+        for frame, g in sc.frames():
+            sim.apply_grid(g)
+            sim.step(frame)
+            cycles, residual = sim.pressure_report()
+            worst = max(worst, residual)
+        assert worst <= 1e-8, (seed, worst)

@@ -39,7 +39,8 @@ struct StepArgs {
   const Emitter *emitters;
   const ForceRec *forces;
   const Region *regions;
-  const float *pressure;  // the solved pressure the projection reads
+  const float *pressure = nullptr;    // the solved pressure the projection reads
+  const float *pressureLo = nullptr;  // its low part when it is a pair hi + lo (defect correction), else null
 };
 
 // Storage of the multigrid levels' iterates and right-hand sides. Arithmetic is FP32 in
@@ -82,12 +83,20 @@ template <class T> struct MGArgsT {
   int bottom;
   float weight;     // axisWeights, equal on every axis for cubic cells
   int topologyAll;  // the geometry changed: rebuild every cell's topology
-  // Defect correction: right-hand side b, FP32 solution x, residual r, per-brick partial
-  // sums and the norms (sum r^2, sum b^2, max |r|, scale).
-  const float *dcB;
-  float *dcX, *dcR;
-  double *dcPartials, *dcNorms;
+  // Defect correction: right-hand side b, the solution x as an unevaluated sum of two FP32
+  // values dcX + dcXlo, residual r, per-brick partial sums and the norms (sum r^2, sum b^2,
+  // max |r|, scale, relative residual, finished, cycles). Once `finished` is set every
+  // later kernel of the solve skips this simulation.
+  const float *dcB = nullptr;
+  float *dcX = nullptr, *dcXlo = nullptr, *dcR = nullptr;
+  double *dcPartials = nullptr, *dcNorms = nullptr;
+  double dcTol = 0.0;   // finish when the relative residual is at most this
+  int dcMaxCycles = 0;  // or after this many V-cycles
+  int dcSlots = 0;      // pool slots of x
 };
+template <class T> __device__ __forceinline__ bool dcFinished(const MGArgsT<T> &m) {
+  return m.dcNorms != nullptr && m.dcNorms[5] != 0.0;
+}
 using Level = LevelT<float>;
 using MGArgs = MGArgsT<float>;
 
@@ -171,8 +180,10 @@ FP_DECLARE_MG(mg_prolong_tiles)
 FP_DECLARE_MG(mg_bottom)    // relaxCoarse: 19 sweeps in one workgroup, up to 1024 tile slots
 FP_DECLARE_MG(mg_bottom64)  // the same in FP64, up to BOTTOM64_TILES tile slots
 // Defect correction: r = b - A x (FP32 with FP64 row dot products) and per-brick FP64 sums;
-// their totals and the power-of-two scale (one workgroup per simulation); level 0's
-// right-hand side r / scale; x += scale * (level 0's solution).
+// their totals, the power-of-two scale and the stopping test of cycle `code` (one
+// workgroup per simulation); level 0's right-hand side r / scale; x += scale * (level 0's
+// solution).
+FP_DECLARE_MG(dc_start)  // x = 0 and the finished flag cleared, before the loop
 FP_DECLARE_MG(dc_residual)
 FP_DECLARE_MG(dc_norms)
 FP_DECLARE_MG(dc_scale)

@@ -821,6 +821,7 @@ __device__ __forceinline__ void project_body(const StepArgs &s, const int vb) {
       continue;
     }
     float cp = pressure[(size_t)slot * nV + l];
+    const float cpLo = s.pressureLo ? s.pressureLo[(size_t)slot * nV + l] : 0.f;
     bool here = pressureUnknown(c, P, p);
     for (int a = 0; a < 3; a++) {
       int nb[3] = {p[0], p[1], p[2]};
@@ -830,7 +831,14 @@ __device__ __forceinline__ void project_body(const StepArgs &s, const int vb) {
       float res;
       if (solidAt(c, P, s.p.solids, npos[0], npos[1], npos[2])) res = 0.f;
       else if (!here && !pressureUnknown(c, P, nb)) res = 0.f;
-      else res = value - (velScalar(c, pressure, nb[0], nb[1], nb[2]) - cp) * P.invHv;
+      else if (!s.pressureLo) res = value - (velScalar(c, pressure, nb[0], nb[1], nb[2]) - cp) * P.invHv;
+      else {
+        // A pair hi + lo: the high parts' difference (exact for neighbors within a factor
+        // of two) plus the low parts'.
+        const float g = (velScalar(c, pressure, nb[0], nb[1], nb[2]) - cp) +
+                        (velScalar(c, s.pressureLo, nb[0], nb[1], nb[2]) - cpLo);
+        res = value - g * P.invHv;
+      }
       o[a * nV + l] = res;
     }
   }

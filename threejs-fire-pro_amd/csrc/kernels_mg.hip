@@ -268,14 +268,17 @@ __device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelView<typena
   lv.q[a] = St::st(mixv(St::ld(lv.p[a]), cand, 0.8f));
 }
 
+// The single and batched kernels of a solve stage; a simulation whose defect correction
+// has finished is skipped.
 #define FP_MG_PAIR(NAME, BOUNDS)                                                                          \
   template <class St> __global__ void __launch_bounds__(BOUNDS) k_##NAME(MGArgsT<typename St::T> a, int code) { \
+    if (dcFinished(a)) return;                                                                            \
     NAME##_body<St>(a, (int)blockIdx.x, code);                                                            \
   }                                                                                                       \
   template <class St>                                                                                     \
   __global__ void __launch_bounds__(BOUNDS)                                                               \
       k_##NAME##_batch(const MGArgsT<typename St::T> *args, const int *starts, int n, int code) {          \
-    FP_BATCH_LOOP(NAME, NAME##_body<St>(args[sim], v - starts[sim], code))                                \
+    FP_BATCH_LOOP(NAME, if (!dcFinished(args[sim])) NAME##_body<St>(args[sim], v - starts[sim], code))    \
   }
 
 // initializePressure (pressure.wgsl:485-505): level 0's first iterate from zero.
@@ -585,8 +588,33 @@ __device__ __forceinline__ void mg_bottom64_body(const MGArgsT<typename St::T> &
 FP_MG_PAIR(mg_bottom64, 1024)
 
 // ---- defect correction -----------------------------------------------------------------
+// The loop's start, one workgroup per pool slot: x = 0 on the slot's level-0 cells; the
+// first workgroup clears the finished flag and the cycle count.
+template <class St> __device__ __forceinline__ void dc_start_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  (void)code;
+  const int sd = m.lv[0].side, n = sd * sd * sd;
+  if (vb < m.dcSlots)
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+      m.dcX[(size_t)vb * n + l] = 0.f;
+      m.dcXlo[(size_t)vb * n + l] = 0.f;
+    }
+  if (vb == 0 && threadIdx.x == 0) {
+    m.dcNorms[4] = 1.0;
+    m.dcNorms[5] = 0.0;
+    m.dcNorms[6] = 0.0;
+  }
+}
+template <class St> __global__ void __launch_bounds__(256) k_dc_start(MGArgsT<typename St::T> a, int code) {
+  dc_start_body<St>(a, (int)blockIdx.x, code);
+}
+template <class St>
+__global__ void __launch_bounds__(256)
+    k_dc_start_batch(const MGArgsT<typename St::T> *args, const int *starts, int n, int code) {
+  FP_BATCH_LOOP(dc_start, dc_start_body<St>(args[sim], v - starts[sim], code))
+}
+
 // The level-0 operator of relaxCell as a residual: r = b - (sum - diag x) on unknown cells,
-// zero elsewhere, each row's dot product accumulated in FP64 from the FP32 x and rounded
+// zero elsewhere, each row's dot product accumulated in FP64 from x = hi + lo and rounded
 // to FP32; per-brick FP64 sums of r^2 and b^2 and the largest |r|.
 template <class St>
 __device__ __forceinline__ void dc_residual_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
@@ -615,10 +643,10 @@ __device__ __forceinline__ void dc_residual_body(const MGArgsT<typename St::T> &
       for (int k = 0; k < 6; k++) {
         if (!((flags >> k) & 1)) continue;
         int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
-        sum += w * (double)readAt<StoreF32>(c, x, q, 0);
+        sum += w * ((double)readAt<StoreF32>(c, x, q, 0) + (double)readAt<StoreF32>(c, m.dcXlo, q, 0));
       }
       float bv = m.dcB[a];
-      v = (float)((double)bv - (sum - (double)diagonal(m.weight, flags) * (double)x[a]));
+      v = (float)((double)bv - (sum - (double)diagonal(m.weight, flags) * ((double)x[a] + (double)m.dcXlo[a])));
       rr += (double)v * (double)v;
       bb += (double)bv * (double)bv;
       big = fmaxv(big, fabsv(v));
@@ -677,6 +705,14 @@ template <class St> __device__ __forceinline__ void dc_norms_body(const MGArgsT<
     m.dcNorms[2] = red[2][0];
     uint32_t bits = __float_as_uint((float)red[2][0]) & 0x7f800000u;
     m.dcNorms[3] = bits ? (double)__uint_as_float(bits) : 1.0;
+    // The stopping test after `code` cycles: |r| / |b| at most the tolerance, or the
+    // cycle budget spent.
+    const double rel = red[1][0] > 0.0 ? sqrt(red[0][0] / red[1][0]) : 0.0;
+    m.dcNorms[4] = rel;
+    if (rel <= m.dcTol || code >= m.dcMaxCycles) {
+      m.dcNorms[6] = (double)code;
+      m.dcNorms[5] = 1.0;
+    }
   }
 }
 FP_MG_PAIR(dc_norms, 256)
@@ -697,6 +733,20 @@ template <class St> __device__ __forceinline__ void dc_scale_body(const MGArgsT<
 }
 FP_MG_PAIR(dc_scale, 256)
 // x += scale * e, level 0's solution of the scaled correction equation, on unknown cells.
+// x is the pair hi + lo: the sum hi + v is split exactly into its rounded value and error
+// (Knuth's two-sum), the error joins lo, and the pair is renormalized so that hi is the
+// rounded value of hi + lo. A single FP32 x could not hold the solution closer than about
+// the unit roundoff times the operator's condition number, and the residual would level
+// off there.
+__device__ __forceinline__ void addToPair(float &hi, float &lo, float v) {
+  const float s = hi + v;
+  const float bp = s - hi;
+  const float err = (hi - (s - bp)) + (v - bp);
+  const float l = lo + err;
+  const float h = s + l;
+  lo = l - (h - s);
+  hi = h;
+}
 template <class St> __device__ __forceinline__ void dc_update_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   const int bi = xcdItem(vb, m.counts[0]);
   if (bi < 0) return;
@@ -708,7 +758,14 @@ template <class St> __device__ __forceinline__ void dc_update_body(const MGArgsT
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     size_t a = (size_t)slot * n + l;
     int e = lv.eflag[a];
-    m.dcX[a] = (!(e & STORED_SOLID) && (e & UNKNOWN)) ? m.dcX[a] + scale * St::ld(lv.p[a]) : 0.f;
+    float hi = 0.f, lo = 0.f;
+    if (!(e & STORED_SOLID) && (e & UNKNOWN)) {
+      hi = m.dcX[a];
+      lo = m.dcXlo[a];
+      addToPair(hi, lo, scale * St::ld(lv.p[a]));
+    }
+    m.dcX[a] = hi;
+    m.dcXlo[a] = lo;
   }
 }
 FP_MG_PAIR(dc_update, 256)
@@ -726,6 +783,7 @@ FP_MG_PAIR(dc_update, 256)
   FP_MG_INSTANCE(mg_prolong_tiles, St)     \
   FP_MG_INSTANCE(mg_bottom, St)            \
   FP_MG_INSTANCE(mg_bottom64, St)          \
+  FP_MG_INSTANCE(dc_start, St)             \
   FP_MG_INSTANCE(dc_residual, St)          \
   FP_MG_INSTANCE(dc_norms, St)             \
   FP_MG_INSTANCE(dc_scale, St)             \

@@ -16,6 +16,8 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -43,19 +45,173 @@ struct DeviceFailure : std::runtime_error {
       throw DeviceFailure(std::string(#expr) + ": " + hipGetErrorString(err_));                     \
   } while (0)
 
+// ---- recycled device and pinned host blocks --------------------------------------------
+// Simulations come and go while others run (a serving loop creates and destroys one per
+// request), and hipMalloc / hipFree / hipHostMalloc cost tens of microseconds to
+// milliseconds each. Freed blocks are kept per device in size classes (quarter steps
+// between powers of two, so at most 25% over the request) and handed out again. A block
+// freed while work enqueued before the free may still touch it waits in `pending`; only a
+// device-wide synchronization moves pending blocks to `ready`, so a block is never reused
+// before every earlier user of it has finished, whatever stream that user ran on. The cache
+// holds at most `limit` bytes; beyond that, and when an allocation fails, blocks go back to
+// the runtime.
+class BlockCache {
+ public:
+  BlockCache(hipError_t (*get)(void **, size_t), hipError_t (*put)(void *), size_t limit)
+      : get_(get), put_(put), limit_(limit) {}
+
+  static size_t sizeClass(size_t bytes) {
+    if (bytes <= 256) return 256;
+    int k = 63 - __builtin_clzll((unsigned long long)bytes);
+    size_t step = (size_t)1 << (k - 2);
+    return (bytes + step - 1) / step * step;
+  }
+
+  void *take(size_t bytes) {
+    const size_t c = sizeClass(bytes);
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      Pool &pool = pools_[device()];
+      auto it = pool.ready.find(c);
+      if (it == pool.ready.end() && pool.pendingClasses.count(c)) {
+        HIPCHECK(hipDeviceSynchronize());
+        settle(pool);
+        it = pool.ready.find(c);
+      }
+      if (it != pool.ready.end()) {
+        void *p = it->second;
+        pool.ready.erase(it);
+        pool.bytes -= c;
+        return p;
+      }
+    }
+    void *p = nullptr;
+    if (get_(&p, c) != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      HIPCHECK(get_(&p, c));
+    }
+    return p;
+  }
+
+  void give(void *p, size_t bytes) {
+    if (!p) return;
+    const size_t c = sizeClass(bytes);
+    std::lock_guard<std::mutex> lock(mu_);
+    Pool &pool = pools_[device()];
+    if (pool.bytes + c > limit_) {
+      (void)put_(p);  // the runtime's free waits for the device itself
+      return;
+    }
+    pool.pending.push_back({c, p});
+    pool.pendingClasses.insert(c);
+    pool.bytes += c;
+  }
+
+  // Returns every cached block of the current device to the runtime.
+  void trim() {
+    std::lock_guard<std::mutex> lock(mu_);
+    Pool &pool = pools_[device()];
+    (void)hipDeviceSynchronize();
+    settle(pool);
+    for (auto &kv : pool.ready) (void)put_(kv.second);
+    pool.ready.clear();
+    pool.bytes = 0;
+  }
+
+ private:
+  struct Pool {
+    std::multimap<size_t, void *> ready;
+    std::vector<std::pair<size_t, void *>> pending;
+    std::multiset<size_t> pendingClasses;
+    size_t bytes = 0;
+  };
+  static int device() {
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return d;
+  }
+  static void settle(Pool &pool) {
+    for (auto &b : pool.pending) pool.ready.insert(b);
+    pool.pending.clear();
+    pool.pendingClasses.clear();
+  }
+  hipError_t (*get_)(void **, size_t);
+  hipError_t (*put_)(void *);
+  size_t limit_;
+  std::mutex mu_;
+  std::map<int, Pool> pools_;
+};
+
+static hipError_t deviceGet(void **p, size_t bytes) { return hipMalloc(p, bytes); }
+static hipError_t deviceFree(void *p) { return hipFree(p); }
+static hipError_t pinnedGet(void **p, size_t bytes) { return hipHostMalloc(p, bytes, hipHostMallocDefault); }
+static hipError_t pinnedFree(void *p) { return hipHostFree(p); }
+
+// Device blocks, up to 64 GiB cached; pinned host blocks, up to 1 GiB.
+BlockCache &deviceBlocks() {
+  static BlockCache cache(deviceGet, deviceFree, (size_t)64 << 30);
+  return cache;
+}
+BlockCache &pinnedBlocks() {
+  static BlockCache cache(pinnedGet, pinnedFree, (size_t)1 << 30);
+  return cache;
+}
+template <class T> T *pinnedAlloc(size_t count) { return static_cast<T *>(pinnedBlocks().take(count * sizeof(T))); }
+template <class T> void pinnedRelease(T *&p, size_t count) {
+  pinnedBlocks().give(p, count * sizeof(T));
+  p = nullptr;
+}
+
+// Streams of finished simulations, idle, for the next ones.
+class StreamPool {
+ public:
+  hipStream_t take() {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      int d = 0;
+      (void)hipGetDevice(&d);
+      auto &v = free_[d];
+      if (!v.empty()) {
+        hipStream_t s = v.back();
+        v.pop_back();
+        return s;
+      }
+    }
+    hipStream_t s;
+    HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+  }
+  // `s` has no work left.
+  void give(hipStream_t s) {
+    if (!s) return;
+    std::lock_guard<std::mutex> lock(mu_);
+    int d = 0;
+    (void)hipGetDevice(&d);
+    free_[d].push_back(s);
+  }
+
+ private:
+  std::mutex mu_;
+  std::map<int, std::vector<hipStream_t>> free_;
+};
+StreamPool &streams() {
+  static StreamPool pool;
+  return pool;
+}
+
 template <class T> struct DBuf {
   T *p = nullptr;
   size_t n = 0;
   void alloc(size_t count) {
     release();
     n = count;
-    if (count) HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+    if (count) p = static_cast<T *>(deviceBlocks().take(count * sizeof(T)));
   }
   // Reallocate to `count`, keeping the first min(n, count) elements.
   void grow(size_t count, hipStream_t s) {
     if (count <= n) return;
-    T *q = nullptr;
-    HIPCHECK(hipMalloc(&q, count * sizeof(T)));
+    T *q = static_cast<T *>(deviceBlocks().take(count * sizeof(T)));
     if (p && n) HIPCHECK(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
     release();
@@ -63,7 +219,7 @@ template <class T> struct DBuf {
     n = count;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) deviceBlocks().give(p, n * sizeof(T));
     p = nullptr;
     n = 0;
   }
@@ -118,6 +274,8 @@ struct ReadbackArgs {
   const uint32_t *tileFlags;
   int cap;
   int *out;
+  const double *norms;  // defect correction's norms, or null
+  double *report;       // their relative residual and cycle count
 };
 __global__ void __launch_bounds__(256) k_readback_batch(const ReadbackArgs *args, int n) {
   const int i = blockIdx.x;
@@ -129,6 +287,10 @@ __global__ void __launch_bounds__(256) k_readback_batch(const ReadbackArgs *args
     a.out[2] = a.poolState[1];
     a.out[3] = a.counts[1] + a.counts[5];
     a.poolState[1] = 0;
+    if (a.norms) {
+      a.report[0] = a.norms[4];
+      a.report[1] = a.norms[6];
+    }
   }
   for (int t = threadIdx.x; t < a.cap; t += blockDim.x) a.out[4 + t] = (int)a.tileFlags[t];
 }
@@ -179,6 +341,25 @@ static std::vector<float4> curlNoiseTable() {
   return t;
 }
 
+// The table on the current device, built and uploaded at its first use and shared,
+// read-only, by every simulation on that device until the process ends.
+static const float4 *sharedNoise() {
+  static std::mutex mu;
+  static std::map<int, float4 *> tables;
+  std::lock_guard<std::mutex> lock(mu);
+  int d = 0;
+  HIPCHECK(hipGetDevice(&d));
+  float4 *&t = tables[d];
+  if (!t) {
+    static const std::vector<float4> host = curlNoiseTable();
+    float4 *p = nullptr;
+    HIPCHECK(hipMalloc(&p, host.size() * sizeof(float4)));
+    HIPCHECK(hipMemcpy(p, host.data(), host.size() * sizeof(float4), hipMemcpyHostToDevice));
+    t = p;
+  }
+  return t;
+}
+
 constexpr int MIN_TILES = 16;
 constexpr double MAX_DT = 1.0 / 30.0;
 
@@ -219,7 +400,8 @@ struct fp_sim {
   DBuf<int> tileRec, hash, pages, entries, records, active, zeroList, poolState, counts, createdSlots, readback;
   DBuf<uint32_t> brickState, activity, stateTmp, rankTmp, blockSums, blockPrefix, freeSlots;
   // Pools, per slot.
-  DBuf<float4> F, Fp, Fn, Curl, noise;
+  DBuf<float4> F, Fp, Fn, Curl;
+  const float4 *noise = nullptr;  // the shared curl-noise table
   DBuf<float> E, V, Vadv, Vcor, Vf, D, P0, P1;
   DBuf<uint32_t> solids;
   struct LevelBufs {
@@ -229,6 +411,7 @@ struct fp_sim {
   };
   LevelBufs lv[MAX_LEVELS];
   float *latestPressure = nullptr;
+  float *latestPressureLo = nullptr;  // the low part when the pressure is a pair
   // Pressure policy: 0 the reference's one cold V-cycle; 1-3 defect correction (an FP32
   // outer loop to a relative residual) around V-cycles whose levels are stored in FP32,
   // BF16 or FP16, with the coarsest level solved in FP64.
@@ -241,7 +424,7 @@ struct fp_sim {
     DBuf<uint16_t> p, q, rhs;
   };
   HalfLevel lv16[MAX_LEVELS];
-  DBuf<float> dcRhs, dcX, dcR;
+  DBuf<float> dcRhs, dcX, dcXlo, dcR;
   DBuf<double> dcPartials, dcNorms;
   double *pinnedNorms = nullptr;
   // Render: the boxes (updated with the tiles each step) and the preparation's state
@@ -387,12 +570,10 @@ struct fp_sim {
     brickLevels = sV;
     bottom = brickLevels + 2;
     if (bottom + 1 > MAX_LEVELS) throw InvalidArgument("too many multigrid levels.");
-    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    stream = streams().take();
     poolState.alloc(4);
     counts.alloc(8);
-    std::vector<float4> table = curlNoiseTable();
-    noise.alloc(table.size());
-    HIPCHECK(hipMemcpyAsync(noise.p, table.data(), table.size() * sizeof(float4), hipMemcpyHostToDevice, stream));
+    noise = sharedNoise();
     allocate();
   }
 
@@ -449,6 +630,7 @@ struct fp_sim {
     P0.alloc(s * nV);
     P1.alloc(s * nV);
     latestPressure = P0.p;
+    latestPressureLo = nullptr;
     for (int L = 0; L < brickLevels; L++) {
       int sd = b >> L;
       size_t n = s * sd * sd * sd;
@@ -514,8 +696,8 @@ struct fp_sim {
     size_t words = 4 + (size_t)cap;
     readback.alloc(words);
     if (words > pinnedWords) {
-      if (pinned) (void)hipHostFree(pinned);
-      HIPCHECK(hipHostMalloc((void **)&pinned, words * sizeof(int), hipHostMallocDefault));
+      if (pinned) pinnedRelease(pinned, pinnedWords);
+      pinned = pinnedAlloc<int>(words);
       pinnedWords = words;
     }
     geometryDirty = true;
@@ -848,7 +1030,7 @@ struct fp_sim {
     p.Curl = Curl.p;
     p.D = D.p;
     p.solids = solids.p;
-    p.noise = noise.p;
+    p.noise = noise;
     return p;
   }
   StepArgs stepArgs() {
@@ -997,6 +1179,7 @@ struct fp_sim {
     end(m6, "pressure");
     int m7 = begin();
     s.pressure = latestPressure;
+    s.pressureLo = latestPressureLo;
     k_project<<<slotsGrid, 256, 0, stream>>>(s);
     end(m7, "project");
     k_readback<<<(unsigned)((capacity + 255) / 256 + 1), 256, 0, stream>>>(counts.p, poolState.p, activity.p + n,
@@ -1028,21 +1211,17 @@ struct fp_sim {
       int flips = 0;
       cycle<StoreF32>(m, 0, slotsGrid, false, flips);
       latestPressure = (flips & 1) ? m.lv[0].q : m.lv[0].p;  // P0 or P1
+      latestPressureLo = nullptr;
       lastCycles = 1;
       lastResidual = -1.0;
       return;
     }
     ensureCorrectionBuffers();
-    if (pressureMode == 1) {
-      MGArgs mf = m;
-      mf.lv[0].rhs = dcRhs.p;
-      correct<StoreF32>(mf, slotsGrid);
-    } else if (pressureMode == 2) {
-      correct<StoreBF16>(halfArgs<uint16_t>(m), slotsGrid);
-    } else {
-      correct<StoreF16>(halfArgs<_Float16>(m), slotsGrid);
-    }
+    if (pressureMode == 1) correct<StoreF32>(correctionArgs32(m), slotsGrid);
+    else if (pressureMode == 2) correct<StoreBF16>(correctionArgs16<uint16_t>(m), slotsGrid);
+    else correct<StoreF16>(correctionArgs16<_Float16>(m), slotsGrid);
     latestPressure = dcX.p;
+    latestPressureLo = dcXlo.p;
   }
 
   // The level arrays in 16-bit storage, with the topology and flags of `m`.
@@ -1075,12 +1254,13 @@ struct fp_sim {
     size_t cells0 = (size_t)poolSlots * nV;
     if (dcX.n != cells0) {
       dcX.alloc(cells0);
+      dcXlo.alloc(cells0);
       dcR.alloc(cells0);
       dcRhs.alloc(cells0);
       dcPartials.alloc(3 * (size_t)std::max(1, poolSlots));
     }
-    if (!dcNorms.p) dcNorms.alloc(4);
-    if (!pinnedNorms) HIPCHECK(hipHostMalloc((void **)&pinnedNorms, 4 * sizeof(double), hipHostMallocDefault));
+    if (!dcNorms.p) dcNorms.alloc(8);
+    if (!pinnedNorms) pinnedNorms = pinnedAlloc<double>(8);
     if (pressureMode >= 2)
       for (int L = 0; L <= bottom; L++) {
         size_t n = L == 0 ? cells0 : lv[L].p.n;
@@ -1094,26 +1274,43 @@ struct fp_sim {
   template <class T> void setCorrection(MGArgsT<T> &m) {
     m.dcB = D.p;
     m.dcX = dcX.p;
+    m.dcXlo = dcXlo.p;
     m.dcR = dcR.p;
     m.dcPartials = dcPartials.p;
     m.dcNorms = dcNorms.p;
+    m.dcTol = pressureTol;
+    m.dcMaxCycles = pressureMaxCycles;
+    m.dcSlots = poolSlots;
+  }
+  // The solve's arguments in the storage of pressure mode 1 (FP32 levels, level 0's
+  // right-hand side apart from b) or 2 and 3 (16-bit levels).
+  MGArgs correctionArgs32(const MGArgs &m) {
+    MGArgs mf = m;
+    mf.lv[0].rhs = dcRhs.p;
+    setCorrection(mf);
+    return mf;
+  }
+  template <class T> MGArgsT<T> correctionArgs16(const MGArgs &m) {
+    MGArgsT<T> h = halfArgs<T>(m);
+    setCorrection(h);
+    return h;
   }
 
   // Defect correction: x = 0; r = b - A x; while |r| / |b| > tol, solve A e = r / s with
   // one V-cycle in the levels' storage (s the power of two under max |r|) and add s e to x.
-  // Norms are FP64 sums on the device; the host reads them to decide.
+  // Norms and the stopping test are FP64 on the device; the host reads the outcome to
+  // decide whether to enqueue the next cycle.
   template <class St> void correct(MGArgsT<typename St::T> m, unsigned slotsGrid) {
-    setCorrection(m);
-    HIPCHECK(hipMemsetAsync(dcX.p, 0, dcX.bytes(), stream));
+    k_dc_start<St><<<slotsGrid, 256, 0, stream>>>(m, 0);
     lastCycles = 0;
     lastResidual = 1.0;
     for (int it = 0; it <= pressureMaxCycles; it++) {
       k_dc_residual<St><<<slotsGrid, 256, 0, stream>>>(m, 0);
-      k_dc_norms<St><<<1, 256, 0, stream>>>(m, 0);
-      HIPCHECK(hipMemcpyAsync(pinnedNorms, dcNorms.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      k_dc_norms<St><<<1, 256, 0, stream>>>(m, it);
+      HIPCHECK(hipMemcpyAsync(pinnedNorms, dcNorms.p, 8 * sizeof(double), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
-      lastResidual = pinnedNorms[1] > 0 ? std::sqrt(pinnedNorms[0] / pinnedNorms[1]) : 0.0;
-      if (lastResidual <= pressureTol || it == pressureMaxCycles) break;
+      lastResidual = pinnedNorms[4];
+      if (pinnedNorms[5] != 0.0) break;
       k_dc_scale<St><<<slotsGrid, 256, 0, stream>>>(m, 0);
       int flips = 0;
       cycle<St>(m, 0, slotsGrid, true, flips);
@@ -1439,7 +1636,7 @@ struct fp_sim {
     size_t total = tileRec.bytes() + hash.bytes() + pages.bytes() + entries.bytes() + records.bytes() +
                    active.bytes() + zeroList.bytes() + brickState.bytes() + activity.bytes() + stateTmp.bytes() +
                    rankTmp.bytes() + blockSums.bytes() + blockPrefix.bytes() + freeSlots.bytes() + F.bytes() +
-                   Fp.bytes() + Fn.bytes() + Curl.bytes() + noise.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
+                   Fp.bytes() + Fn.bytes() + Curl.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
                    Vcor.bytes() + Vf.bytes() + D.bytes() + P0.bytes() + P1.bytes() + solids.bytes() +
                    emitters.bytes() + colliders.bytes() + forces.bytes() + regions.bytes();
     for (int L = 0; L <= bottom; L++)
@@ -1453,7 +1650,7 @@ struct fp_sim {
     for (auto *x : ints) x->release();
     DBuf<uint32_t> *u32s[] = {&brickState, &activity, &stateTmp, &rankTmp, &blockSums, &blockPrefix, &freeSlots, &solids};
     for (auto *x : u32s) x->release();
-    DBuf<float4> *f4s[] = {&F, &Fp, &Fn, &Curl, &noise};
+    DBuf<float4> *f4s[] = {&F, &Fp, &Fn, &Curl};
     for (auto *x : f4s) x->release();
     DBuf<float> *fs[] = {&E, &V, &Vadv, &Vcor, &Vf, &D, &P0, &P1};
     for (auto *x : fs) x->release();
@@ -1475,11 +1672,11 @@ struct fp_sim {
     }
     dcRhs.release();
     dcX.release();
+    dcXlo.release();
     dcR.release();
     dcPartials.release();
     dcNorms.release();
-    if (pinnedNorms) (void)hipHostFree(pinnedNorms);
-    pinnedNorms = nullptr;
+    if (pinnedNorms) pinnedRelease(pinnedNorms, 8);
     rc.boxes.release();
     rc.tiles.release();
     rc.table.release();
@@ -1490,9 +1687,9 @@ struct fp_sim {
     rc.O.release();
     for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
     eventPool.clear();
-    if (pinned) (void)hipHostFree(pinned);
-    pinned = nullptr;
-    if (stream) (void)hipStreamDestroy(stream);
+    if (pinned) pinnedRelease(pinned, pinnedWords);
+    pinnedWords = 0;
+    streams().give(stream);
     stream = nullptr;
   }
 };
@@ -1531,6 +1728,8 @@ struct fp_batch {
   DBuf<int> readback;
   int *pinnedReadback = nullptr;
   size_t pinnedWords = 0;
+  DBuf<double> reports;  // per simulation: defect correction's relative residual and cycles
+  double *pinnedReports = nullptr;
   int grid = 2048;  // persistent workgroups per launch, a multiple of 8
   double lastMs = 0;
   // Wall ms of the latest step's phases: host halves, argument packing, launches, the wait
@@ -1542,8 +1741,11 @@ struct fp_batch {
     if (stream) (void)hipStreamSynchronize(stream);
     devArgs.release();
     readback.release();
+    reports.release();
     if (hostArgs) (void)hipHostFree(hostArgs);
     if (pinnedReadback) (void)hipHostFree(pinnedReadback);
+    if (pinnedReports) (void)hipHostFree(pinnedReports);
+    pinnedReports = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     hostArgs = nullptr;
     pinnedReadback = nullptr;
@@ -1582,7 +1784,8 @@ struct fp_batch {
       if (!sims[i]) throw InvalidArgument("batch member is null.");
       if (sims[i]->B != sims[0]->B || sims[i]->d != sims[0]->d)
         throw InvalidArgument("a batch's simulations must share brick size and velocity divisor.");
-      if (sims[i]->pressureMode != 0) throw InvalidArgument("batched simulations use the V-cycle pressure policy.");
+      if (sims[i]->pressureMode != sims[0]->pressureMode)
+        throw InvalidArgument("a batch's simulations must share the pressure policy.");
       for (int j = 0; j < i; j++)
         if (sims[j] == sims[i]) throw InvalidArgument("a simulation appears twice in a batch.");
       fp_sim::checkStep(frames[i], dt);
@@ -1620,18 +1823,85 @@ struct fp_batch {
     lastMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 
+  // Offsets of a solve's arguments and block prefixes in the uploaded buffer.
+  struct SolveShape {
+    size_t args, slots, tiles, one;
+    int tSlots, tTiles, tOne, n, brickLevels, bottom;
+    bool smallBottom, bottom64;
+  };
+
+  // One V-cycle over the batch (PressureSolver.ts:303-363) in storage St; returns the level
+  // flips (bit L: level L's iterate is in q). With `dry`, only the flips.
+  template <class St> int vcycle(const SolveShape &c, bool dry) {
+    using Args = MGArgsT<typename St::T>;
+    int f = 0;
+    auto go = [&](auto kernel, unsigned threads, size_t starts, int total, int code) {
+      if (!dry) launch<Args>(kernel, threads, c.args, starts, total, c.n, code);
+    };
+    auto relax = [&](int L) {
+      if (L < c.brickLevels) go(k_mg_relax_bricks_batch<St>, 256, c.slots, c.tSlots, L | f << 8);
+      else go(k_mg_relax_tiles_batch<St>, 64, c.tiles, c.tTiles, L | f << 8);
+      f ^= 1 << L;
+    };
+    std::function<void(int)> cycle = [&](int L) {
+      if (L == c.bottom) {
+        if (c.bottom64) go(k_mg_bottom64_batch<St>, 1024, c.one, c.tOne, f << 8);
+        else if (c.smallBottom) go(k_mg_bottom_batch<St>, 1024, c.one, c.tOne, f << 8);
+        else
+          for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax(L);
+        return;
+      }
+      if (L == 0) go(k_mg_init_batch<St>, 256, c.slots, c.tSlots, f << 8);
+      else {
+        relax(L);
+        relax(L);
+      }
+      if (L + 1 < c.brickLevels) go(k_mg_restrict_bricks_batch<St>, 64, c.slots, c.tSlots, (L + 1) | f << 8);
+      else go(k_mg_restrict_tiles_batch<St>, 64, c.tiles, c.tTiles, (L + 1) | f << 8);
+      cycle(L + 1);
+      if (L < c.brickLevels) go(k_mg_prolong_bricks_batch<St>, 256, c.slots, c.tSlots, L | f << 8);
+      else go(k_mg_prolong_tiles_batch<St>, 64, c.tiles, c.tTiles, L | f << 8);
+      f ^= 1 << L;
+      relax(L);
+      relax(L);
+      relax(L);
+    };
+    cycle(0);
+    return f;
+  }
+
+  // Defect correction over the batch: every simulation runs the loop of
+  // fp_sim::correct, for up to `maxCycles` cycles; one that has finished (its own
+  // tolerance met or its own cycle budget spent) is skipped by the rest, on the device,
+  // so no launch waits for the host.
+  template <class St> void correctBatch(const SolveShape &c, int maxCycles) {
+    using Args = MGArgsT<typename St::T>;
+    launch<Args>(k_dc_start_batch<St>, 256, c.args, c.slots, c.tSlots, c.n, 0);
+    for (int it = 0; it <= maxCycles; it++) {
+      launch<Args>(k_dc_residual_batch<St>, 256, c.args, c.slots, c.tSlots, c.n, 0);
+      launch<Args>(k_dc_norms_batch<St>, 256, c.args, c.one, c.tOne, c.n, it);
+      if (it == maxCycles) break;
+      launch<Args>(k_dc_scale_batch<St>, 256, c.args, c.slots, c.tSlots, c.n, 0);
+      const int flips = vcycle<St>(c, false);
+      launch<Args>(k_dc_update_batch<St>, 256, c.args, c.slots, c.tSlots, c.n, flips << 8);
+    }
+  }
+
   void enqueue(fp_sim *const *sims, int n) {
     const auto e0 = std::chrono::steady_clock::now();
     fp_sim &s0 = *sims[0];
-    const int nF = s0.nF, brickLevels = s0.brickLevels, bottom = s0.bottom;
+    const int nF = s0.nF, mode = s0.pressureMode;
     const int zchunks = std::max(1, nF / 1024), fchunks = std::max(1, nF / 256);
     std::vector<AllocArgs> alloc(n);
     std::vector<StepArgs> pre(n), post(n);
-    std::vector<MGArgs> mg(n);
+    std::vector<MGArgs> mg(n), solve32;
+    std::vector<MGArgsT<uint16_t>> solveBF16;
+    std::vector<MGArgsT<_Float16>> solveF16;
     std::vector<ReadbackArgs> rb(n);
     std::vector<int> cBlocks(n), cOne(n, 1), cLink(n), cSlots(n), cSlots2(n), cZero(n), cField(n), cTiles(n);
     size_t words = 0;
     bool smallBottom = true;
+    int maxCycles = 0;
     for (int i = 0; i < n; i++) {
       fp_sim &s = *sims[i];
       const int slotsGrid = (std::max(1, s.poolSlots) + 7) / 8 * 8;
@@ -1651,70 +1921,59 @@ struct fp_batch {
       mg[i] = s.mgArgs();
       mg[i].topologyAll = s.geometryDirty ? 1 : 0;
       s.geometryDirty = false;
+      if (mode != 0) {
+        s.ensureCorrectionBuffers();
+        if (mode == 1) solve32.push_back(s.correctionArgs32(mg[i]));
+        else if (mode == 2) solveBF16.push_back(s.correctionArgs16<uint16_t>(mg[i]));
+        else solveF16.push_back(s.correctionArgs16<_Float16>(mg[i]));
+        maxCycles = std::max(maxCycles, s.pressureMaxCycles);
+      }
       words += 4 + (size_t)s.capacity;
     }
-    // The V-cycle's launches, run dry first to learn where level 0's final iterate lands
-    // (P0 or P1, the same for every simulation), then for real.
-    bool dry = true;
-    int f = 0;
-    size_t oMg = 0, sSlots = 0, sTiles = 0, sOne = 0;
-    int tSlots = 0, tTiles = 0, tOne = 0;
-    auto mgLaunch = [&](auto kernel, unsigned threads, size_t so, int total, int code) {
-      if (!dry) launch<MGArgs>(kernel, threads, oMg, so, total, n, code);
-    };
-    std::function<void(int)> relax = [&](int L) {
-      if (L < brickLevels) mgLaunch(k_mg_relax_bricks_batch<StoreF32>, 256, sSlots, tSlots, L | f << 8);
-      else mgLaunch(k_mg_relax_tiles_batch<StoreF32>, 64, sTiles, tTiles, L | f << 8);
-      f ^= 1 << L;
-    };
-    std::function<void(int)> cycle = [&](int L) {
-      if (L == bottom) {
-        if (smallBottom) mgLaunch(k_mg_bottom_batch<StoreF32>, 1024, sOne, tOne, f << 8);
-        else
-          for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax(L);
-        return;
-      }
-      if (L == 0) mgLaunch(k_mg_init_batch<StoreF32>, 256, sSlots, tSlots, f << 8);
-      else {
-        relax(L);
-        relax(L);
-      }
-      if (L + 1 < brickLevels) mgLaunch(k_mg_restrict_bricks_batch<StoreF32>, 64, sSlots, tSlots, (L + 1) | f << 8);
-      else mgLaunch(k_mg_restrict_tiles_batch<StoreF32>, 64, sTiles, tTiles, (L + 1) | f << 8);
-      cycle(L + 1);
-      if (L < brickLevels) mgLaunch(k_mg_prolong_bricks_batch<StoreF32>, 256, sSlots, tSlots, L | f << 8);
-      else mgLaunch(k_mg_prolong_tiles_batch<StoreF32>, 64, sTiles, tTiles, L | f << 8);
-      f ^= 1 << L;
-      relax(L);
-      relax(L);
-      relax(L);
-    };
-    cycle(0);
-    const int flips = f;
-    for (int i = 0; i < n; i++) post[i].pressure = (flips & 1) ? mg[i].lv[0].q : mg[i].lv[0].p;
+    SolveShape shape{};
+    shape.n = n;
+    shape.brickLevels = s0.brickLevels;
+    shape.bottom = s0.bottom;
+    shape.smallBottom = smallBottom;
+    shape.bottom64 = mode != 0;
+    // Where level 0's final iterate lands (P0 or P1, the same for every simulation), from a
+    // dry run of the V-cycle; defect correction leaves its solution in x.
+    const int flips = mode == 0 ? vcycle<StoreF32>(shape, true) : 0;
+    for (int i = 0; i < n; i++) {
+      post[i].pressure = mode != 0 ? sims[i]->dcX.p : (flips & 1) ? mg[i].lv[0].q : mg[i].lv[0].p;
+      post[i].pressureLo = mode != 0 ? sims[i]->dcXlo.p : nullptr;
+    }
     if (readback.n < words) readback.alloc(words + 1024);
     if (pinnedWords < words) {
       if (pinnedReadback) (void)hipHostFree(pinnedReadback);
       pinnedWords = words + 1024;
       HIPCHECK(hipHostMalloc((void **)&pinnedReadback, pinnedWords * sizeof(int), hipHostMallocDefault));
     }
+    if (reports.n < 2 * (size_t)n) {
+      reports.alloc(2 * (size_t)n + 64);
+      if (pinnedReports) (void)hipHostFree(pinnedReports);
+      HIPCHECK(hipHostMalloc((void **)&pinnedReports, reports.bytes(), hipHostMallocDefault));
+    }
     size_t o = 0;
     for (int i = 0; i < n; i++) {
       fp_sim &s = *sims[i];
-      rb[i] = ReadbackArgs{s.counts.p, s.poolState.p, s.activity.p + s.brickCount(), s.capacity, readback.p + o};
+      rb[i] = ReadbackArgs{s.counts.p, s.poolState.p, s.activity.p + s.brickCount(), s.capacity, readback.p + o,
+                           mode != 0 ? s.dcNorms.p : nullptr, reports.p + 2 * i};
       o += 4 + (size_t)s.capacity;
     }
     Packer pk;
     const size_t oAlloc = pk.put(alloc), oPre = pk.put(pre), oPost = pk.put(post), oRb = pk.put(rb);
-    oMg = pk.put(mg);
+    const size_t oMg = pk.put(mg);
+    shape.args = mode == 0 ? oMg : mode == 1 ? pk.put(solve32) : mode == 2 ? pk.put(solveBF16) : pk.put(solveF16);
     const std::vector<int> pBlocks = prefix(cBlocks, 1), pOne = prefix(cOne, 1), pLink = prefix(cLink, 1),
                            pSlots = prefix(cSlots, 8), pSlots2 = prefix(cSlots2, 8), pZero = prefix(cZero, 8),
                            pField = prefix(cField, 8), pTiles = prefix(cTiles, 8);
     const size_t sBlocks = pk.put(pBlocks), sLink = pk.put(pLink), sSlots2 = pk.put(pSlots2), sZero = pk.put(pZero),
                  sField = pk.put(pField);
-    sOne = pk.put(pOne);
-    sSlots = pk.put(pSlots);
-    sTiles = pk.put(pTiles);
+    shape.one = pk.put(pOne);
+    shape.slots = pk.put(pSlots);
+    shape.tiles = pk.put(pTiles);
+    const size_t sSlots = shape.slots, sTiles = shape.tiles, sOne = shape.one;
     if (hostArgsBytes < pk.bytes.size()) {
       if (hostArgs) (void)hipHostFree(hostArgs);
       hostArgsBytes = pk.bytes.size() * 2;
@@ -1727,13 +1986,15 @@ struct fp_batch {
     std::memcpy(hostArgs, pk.bytes.data(), pk.bytes.size());
     HIPCHECK(hipMemcpyAsync(devArgs.p, hostArgs, pk.bytes.size(), hipMemcpyHostToDevice, stream));
     const int tBlocks = pBlocks[n], tLink = pLink[n], tSlots2 = pSlots2[n], tZero = pZero[n], tField = pField[n];
-    tOne = pOne[n];
-    tSlots = pSlots[n];
-    tTiles = pTiles[n];
+    shape.tOne = pOne[n];
+    shape.tSlots = pSlots[n];
+    shape.tTiles = pTiles[n];
+    const int tSlots = shape.tSlots, tTiles = shape.tTiles;
+    (void)sOne;
 
     // Allocation (fluid-allocation.wgsl).
     launch<AllocArgs>(k_build_bricks_batch, 256, oAlloc, sBlocks, tBlocks, n);
-    launch<AllocArgs>(k_scan_blocks_batch, 1024, oAlloc, sOne, tOne, n);
+    launch<AllocArgs>(k_scan_blocks_batch, 1024, oAlloc, shape.one, shape.tOne, n);
     launch<AllocArgs>(k_free_bricks_batch, 256, oAlloc, sBlocks, tBlocks, n);
     launch<AllocArgs>(k_allocate_bricks_batch, 256, oAlloc, sBlocks, tBlocks, n);
     launch<AllocArgs>(k_link_bricks_batch, 64, oAlloc, sLink, tLink, n);
@@ -1748,24 +2009,33 @@ struct fp_batch {
     launch<StepArgs>(k_cell_forces_batch, 256, oPost, sSlots, tSlots, n);
     launch<StepArgs>(k_forces_batch, 256, oPost, sSlots, tSlots, n);
     launch<StepArgs>(k_divergence_batch, 256, oPost, sSlots, tSlots, n);
-    // Pressure: topology, flags, one V-cycle.
+    // Pressure: topology, flags, then one V-cycle or defect correction.
     launch<MGArgs>(k_topology_bricks_batch, 256, oMg, sSlots, tSlots, n, 0);
-    for (int L = brickLevels; L <= bottom; L++) launch<MGArgs>(k_topology_tiles_batch, 64, oMg, sTiles, tTiles, n, L);
+    for (int L = s0.brickLevels; L <= s0.bottom; L++) launch<MGArgs>(k_topology_tiles_batch, 64, oMg, sTiles, tTiles, n, L);
     launch<MGArgs>(k_topology_bricks_batch, 256, oMg, sSlots2, tSlots2, n, 1);
     launch<MGArgs>(k_eflags_bricks_batch, 256, oMg, sSlots, tSlots, n);
-    for (int L = brickLevels; L <= bottom; L++) launch<MGArgs>(k_eflags_tiles_batch, 64, oMg, sTiles, tTiles, n, L);
-    dry = false;
-    f = 0;
-    cycle(0);
+    for (int L = s0.brickLevels; L <= s0.bottom; L++) launch<MGArgs>(k_eflags_tiles_batch, 64, oMg, sTiles, tTiles, n, L);
+    if (mode == 0) vcycle<StoreF32>(shape, false);
+    else if (mode == 1) correctBatch<StoreF32>(shape, maxCycles);
+    else if (mode == 2) correctBatch<StoreBF16>(shape, maxCycles);
+    else correctBatch<StoreF16>(shape, maxCycles);
     launch<StepArgs>(k_project_batch, 256, oPost, sSlots, tSlots, n);
     k_readback_batch<<<(unsigned)n, 256, 0, stream>>>(reinterpret_cast<const ReadbackArgs *>(devArgs.p + oRb), n);
     HIPCHECK(hipMemcpyAsync(pinnedReadback, readback.p, words * sizeof(int), hipMemcpyDeviceToHost, stream));
+    if (mode != 0)
+      HIPCHECK(hipMemcpyAsync(pinnedReports, reports.p, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
     auto w0 = std::chrono::steady_clock::now();
     phaseMs[2] = std::chrono::duration<double, std::milli>(w0 - l0).count();
     HIPCHECK(hipStreamSynchronize(stream));
     phaseMs[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     HIPCHECK(hipGetLastError());
-    for (int i = 0; i < n; i++) sims[i]->latestPressure = const_cast<float *>(post[i].pressure);
+    for (int i = 0; i < n; i++) {
+      fp_sim &s = *sims[i];
+      s.latestPressure = const_cast<float *>(post[i].pressure);
+      s.latestPressureLo = const_cast<float *>(post[i].pressureLo);
+      s.lastCycles = mode != 0 ? (int)pinnedReports[2 * i + 1] : 1;
+      s.lastResidual = mode != 0 ? pinnedReports[2 * i] : -1.0;
+    }
   }
 };
 

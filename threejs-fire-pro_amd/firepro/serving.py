@@ -32,10 +32,12 @@ DIVISOR = 2
 
 def memory_estimate(target_bricks: int, brick: int = BRICK, divisor: int = DIVISOR) -> int:
     """Device bytes a simulation holds at its voxel budget: per pool slot its field,
-    expansion, velocity, curl, divergence, pressure and multigrid arrays, plus the tile
-    tables (a tile per 16 bricks, generously)."""
+    expansion, velocity, curl, divergence, pressure and multigrid arrays and defect
+    correction's x, r and right-hand side with 16-bit level copies, plus the tile tables (a
+    tile per 16 bricks, generously)."""
     nf, nv = brick ** 3, (brick // divisor) ** 3
-    per_slot = nf * (16 * 3 + 4) + nf // 8 + nv * (4 * 3 * 4 + 16 + 4 + 8) + int(nv * 14 * 1.15)
+    per_slot = (nf * (16 * 3 + 4) + nf // 8 + nv * (4 * 3 * 4 + 16 + 4 + 8) + int(nv * 14 * 1.15)
+                + nv * 3 * 4 + int(nv * 6 * 1.15))
     tiles = max(16, target_bricks // 16)
     per_tile = 64 * (4 * 3 + 28 * 4 + 4 * 3) + 32 * 4 + 64 * 14
     return int(target_bricks * per_slot + tiles * per_tile + (4 << 20))
@@ -101,6 +103,12 @@ class Report:
     peak_resident: int = 0
     peak_reserved_bytes: int = 0
     phases: dict = field(default_factory=dict)  # batched: summed phase ms of the steps
+    admit_s: float = 0.0  # creating admitted simulations
+    account_s: float = 0.0  # per-round statistics and disposal of finished simulations
+    pressure: Optional[tuple] = None
+    cycles: int = 0  # defect correction's V-cycles over all steps
+    max_residual: float = 0.0  # the largest final relative residual |b - Ax| / |b| of a step
+    within_tolerance: int = 0  # steps whose solve met the tolerance
 
     def summary(self) -> dict:
         pct = lambda a, q: float(np.percentile(a, q)) if len(a) else None  # noqa: E731
@@ -114,12 +122,23 @@ class Report:
             "p50_request_s": pct(self.request_s, 50), "p99_request_s": pct(self.request_s, 99),
             "peak_resident": self.peak_resident, "peak_reserved_gb": self.peak_reserved_bytes / 1e9,
             "phase_ms": {k: round(v, 1) for k, v in self.phases.items()},
+            "admit_s": self.admit_s, "account_s": self.account_s,
+            "pressure": list(self.pressure) if self.pressure else "vcycle",
+            "mean_cycles": self.cycles / self.sim_steps if self.pressure and self.sim_steps else None,
+            "max_relative_residual": self.max_residual if self.pressure else None,
+            "steps_within_tolerance": self.within_tolerance / self.sim_steps if self.pressure and self.sim_steps else None,
         }
 
 
-def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, dt: float = 1 / 60) -> Report:
-    """Runs the request stream in real time and returns its report."""
+def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, dt: float = 1 / 60,
+          pressure: Optional[tuple] = None, progress=None) -> Report:
+    """Runs the request stream in real time and returns its report. `pressure`, as the
+    arguments of FluidSimulation.set_pressure, is every simulation's pressure policy (the
+    reference's single V-cycle when None). `progress`, if given, is called with a line of
+    text every few seconds."""
     rep = Report("batched" if batched else "one simulation per launch")
+    rep.pressure = pressure
+    said = time.perf_counter()
     pending = sorted(reqs, key=lambda r: r.arrival)
     resident: List[Request] = []
     reserved = 0
@@ -129,14 +148,18 @@ def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, 
         while pending or resident:
             now = time.perf_counter() - t0
             # Admission: arrived requests in order, while their memory fits.
+            a0 = time.perf_counter()
             while pending and pending[0].arrival <= now and reserved + pending[0].memory <= hbm_budget:
                 r = pending.pop(0)
                 r.sim = FluidSimulation.create(r.grid)
                 g = r.settings
                 r.sim.cutoff, r.sim.voxel_budget, r.sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+                if pressure is not None:
+                    r.sim.set_pressure(*pressure)
                 r.admitted = time.perf_counter() - t0
                 reserved += r.memory
                 resident.append(r)
+            rep.admit_s += time.perf_counter() - a0
             rep.peak_resident = max(rep.peak_resident, len(resident))
             rep.peak_reserved_bytes = max(rep.peak_reserved_bytes, reserved)
             if not resident:
@@ -158,10 +181,16 @@ def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, 
                     rep.step_ms.append((time.perf_counter() - a) * 1e3)
             rep.busy_s += time.perf_counter() - s0
             rep.rounds += 1
+            c0 = time.perf_counter()
             still = []
             for r in resident:
                 st = r.sim.stats()
                 rep.voxel_steps += st.active_voxels
+                if pressure is not None:
+                    cycles, residual = r.sim.pressure_report()
+                    rep.cycles += cycles
+                    rep.max_residual = max(rep.max_residual, residual)
+                    rep.within_tolerance += residual <= pressure[1]
                 rep.sim_steps += 1
                 r.done += 1
                 if r.done >= r.steps:
@@ -174,6 +203,11 @@ def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, 
                 else:
                     still.append(r)
             resident = still
+            rep.account_s += time.perf_counter() - c0
+            if progress is not None and time.perf_counter() - said > 5.0:
+                said = time.perf_counter()
+                progress(f"{rep.mode}: {rep.requests} of {len(reqs)} requests done, {len(resident)} resident, "
+                         f"{rep.rounds} rounds, {said - t0:.0f} s")
         rep.wall_s = time.perf_counter() - t0
     finally:
         for r in resident:
