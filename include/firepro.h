@@ -234,13 +234,15 @@ int fp_sim_profile(fp_sim *sim, int32_t enable);
 int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *count);
 
 /* Batched stepping (the serving form; no counterpart in the reference, whose simulations
- * step one at a time). fp_batch_step advances `count` simulations, all of one brick size
- * and velocity divisor and on the V-cycle pressure policy, by one step each with
- * frames[i] for sims[i]: their host halves run in turn, then every stage is one launch
- * over all of their active bricks and one readback returns all their activity. Each
- * simulation ends exactly as fp_sim_step would have left it. Any set of simulations may
- * be passed on each call (continuous batching); none may be stepped or rendered elsewhere
- * during the call. */
+ * step one at a time). fp_batch_step advances `count` simulations, all of one brick size,
+ * velocity divisor and pressure policy (each with its own tolerance and cycle budget), by
+ * one step each with frames[i] for sims[i]: their host halves run in turn, then every
+ * stage is one launch of persistent workgroups over all of their active bricks, defect
+ * correction stops each simulation on the device when it has converged, and one readback
+ * returns all their activity. The launch sequence is captured once as a HIP graph per
+ * layout and replayed. Each simulation ends exactly as fp_sim_step would have left it. Any
+ * set of simulations may be passed on each call (continuous batching); none may be
+ * stepped or rendered elsewhere during the call. */
 int fp_batch_create(fp_batch **out);
 void fp_batch_destroy(fp_batch *batch);
 int fp_batch_step(fp_batch *batch, fp_sim *const *sims, const fp_frame *frames, int32_t count, double dt);
@@ -249,6 +251,8 @@ int fp_batch_last_ms(const fp_batch *batch, double *ms);
 /* Its phases in ms: ms[0] host halves, [1] argument packing, [2] launches, [3] the wait for
  * the device, [4] the activity readbacks. */
 int fp_batch_phases(const fp_batch *batch, double *ms);
+/* 1 (the default) replays captured HIP graphs; 0 launches every kernel on each step. */
+int fp_batch_use_graphs(fp_batch *batch, int32_t enable);
 
 /* The pressure policy of the next steps. Defect correction keeps the solution x in FP32,
  * starts from x = 0 and repeats r = b - A x (FP32, each row's dot product and the norms in

@@ -277,10 +277,10 @@ struct ReadbackArgs {
   const double *norms;  // defect correction's norms, or null
   double *report;       // their relative residual and cycle count
 };
-__global__ void __launch_bounds__(256) k_readback_batch(const ReadbackArgs *args, int n) {
-  const int i = blockIdx.x;
-  if (i >= n) return;
-  const ReadbackArgs a = args[i];
+// One simulation's activity words and defect-correction report; entries past the batch's
+// members are empty (no output).
+__device__ __forceinline__ void readbackOne(const ReadbackArgs &a) {
+  if (!a.out) return;
   if (threadIdx.x == 0) {
     a.out[0] = a.counts[0];
     a.out[1] = a.poolState[0];
@@ -293,6 +293,12 @@ __global__ void __launch_bounds__(256) k_readback_batch(const ReadbackArgs *args
     }
   }
   for (int t = threadIdx.x; t < a.cap; t += blockDim.x) a.out[4 + t] = (int)a.tileFlags[t];
+}
+__global__ void __launch_bounds__(256) k_readback_batch(const ReadbackArgs *args, int n) {
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    readbackOne(args[i]);
+    __syncthreads();
+  }
 }
 
 // ---- the curl-noise table (curl-noise.wgsl:1-45), built once on the host -----------------
@@ -1724,21 +1730,60 @@ struct fp_batch {
   hipStream_t stream = nullptr;
   DBuf<char> devArgs;
   char *hostArgs = nullptr;
-  size_t hostArgsBytes = 0;
   DBuf<int> readback;
   int *pinnedReadback = nullptr;
   size_t pinnedWords = 0;
   DBuf<double> reports;  // per simulation: defect correction's relative residual and cycles
   double *pinnedReports = nullptr;
-  int grid = 2048;  // persistent workgroups per launch, a multiple of 8
+  int grid = 2048;    // persistent workgroups per launch, a multiple of 8
+  bool graphs = true;  // replay captured launch sequences
   double lastMs = 0;
   // Wall ms of the latest step's phases: host halves, argument packing, launches, the wait
   // for the device, the activity readbacks.
   double phaseMs[5] = {0, 0, 0, 0, 0};
 
+  // The uploaded arguments: per-simulation records and block prefixes at offsets fixed by
+  // the member capacity `cap`. A step with n <= cap members pads every array to cap
+  // entries (records zeroed, prefixes repeating their total), so kernel parameters and
+  // grids depend on the capacity alone and a captured launch sequence serves any members.
+  struct Layout {
+    int cap = 0;
+    size_t alloc, pre, post, rb, mg, solve, blocks, one, link, slots, slots2, zero, field, tiles, bytes;
+  } lay;
+  static Layout layoutFor(int cap) {
+    Layout L;
+    L.cap = cap;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+      size_t off = (o + 255) / 256 * 256;
+      o = off + bytes;
+      return off;
+    };
+    const size_t solveBytes = std::max({sizeof(MGArgs), sizeof(MGArgsT<uint16_t>), sizeof(MGArgsT<_Float16>)});
+    L.alloc = take(cap * sizeof(AllocArgs));
+    L.pre = take(cap * sizeof(StepArgs));
+    L.post = take(cap * sizeof(StepArgs));
+    L.rb = take(cap * sizeof(ReadbackArgs));
+    L.mg = take(cap * sizeof(MGArgs));
+    L.solve = take(cap * solveBytes);
+    size_t *prefixes[] = {&L.blocks, &L.one, &L.link, &L.slots, &L.slots2, &L.zero, &L.field, &L.tiles};
+    for (size_t *q : prefixes) *q = take((cap + 1) * sizeof(int));
+    L.bytes = (o + 255) / 256 * 256;
+    return L;
+  }
+
+  // The launch sequence of a step depends on these; one captured graph per distinct key.
+  using GraphKey = std::array<int, 8>;
+  std::map<GraphKey, hipGraphExec_t> execs;
+
   void create() { HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
+  void dropGraphs() {
+    for (auto &kv : execs) (void)hipGraphExecDestroy(kv.second);
+    execs.clear();
+  }
   void destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
+    dropGraphs();
     devArgs.release();
     readback.release();
     reports.release();
@@ -1752,31 +1797,45 @@ struct fp_batch {
     stream = nullptr;
   }
 
-  // Packs the per-simulation arrays and block prefixes into one pinned buffer.
-  struct Packer {
-    std::vector<char> bytes;
-    template <class T> size_t put(const std::vector<T> &v) {
-      size_t off = (bytes.size() + 255) / 256 * 256;
-      bytes.resize(off + v.size() * sizeof(T));
-      if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
-      return off;
-    }
-  };
-  // Prefix of per-simulation block counts, each rounded up to `round`.
-  static std::vector<int> prefix(const std::vector<int> &counts, int round) {
-    std::vector<int> out(counts.size() + 1, 0);
+  // Room for `n` members: a new layout (and new graphs) when the capacity is exceeded.
+  void reserve(int n) {
+    if (n <= lay.cap) return;
+    HIPCHECK(hipStreamSynchronize(stream));
+    dropGraphs();
+    lay = layoutFor(std::max({n, 2 * lay.cap, 16}));
+    if (hostArgs) (void)hipHostFree(hostArgs);
+    hostArgs = nullptr;
+    HIPCHECK(hipHostMalloc((void **)&hostArgs, lay.bytes, hipHostMallocDefault));
+    devArgs.alloc(lay.bytes);
+    reports.alloc(2 * (size_t)lay.cap);
+    if (pinnedReports) (void)hipHostFree(pinnedReports);
+    pinnedReports = nullptr;
+    HIPCHECK(hipHostMalloc((void **)&pinnedReports, reports.bytes(), hipHostMallocDefault));
+  }
+  // Writes n records at `off`, zeroing the rest of the capacity.
+  template <class T> void put(size_t off, const std::vector<T> &v) {
+    if (!v.empty()) std::memcpy(hostArgs + off, v.data(), v.size() * sizeof(T));
+    std::memset(hostArgs + off + v.size() * sizeof(T), 0, (lay.cap - v.size()) * sizeof(T));
+  }
+  // Prefix of per-simulation block counts, each rounded up to `round`, padded to the
+  // capacity; returns the total.
+  int putPrefix(size_t off, const std::vector<int> &counts, int round) {
+    int *out = reinterpret_cast<int *>(hostArgs + off);
+    out[0] = 0;
     for (size_t i = 0; i < counts.size(); i++) out[i + 1] = out[i] + (counts[i] + round - 1) / round * round;
-    return out;
+    for (int i = (int)counts.size() + 1; i <= lay.cap; i++) out[i] = out[counts.size()];
+    return out[counts.size()];
   }
 
+  // Launches kernel(args, starts, cap, extra...) on `g` persistent workgroups.
   template <class Args, class Kernel, class... Extra>
-  void launch(Kernel kernel, unsigned threads, size_t argOff, size_t startOff, int total, int n, Extra... extra) {
-    if (total <= 0) return;
-    unsigned g = (unsigned)std::min(total, grid);
+  void launch(Kernel kernel, unsigned threads, size_t argOff, size_t startOff, unsigned g, Extra... extra) {
     const Args *args = reinterpret_cast<const Args *>(devArgs.p + argOff);
     const int *starts = reinterpret_cast<const int *>(devArgs.p + startOff);
-    kernel<<<g, threads, 0, stream>>>(args, starts, n, extra...);
+    kernel<<<g, threads, 0, stream>>>(args, starts, lay.cap, extra...);
   }
+  unsigned wide() const { return (unsigned)grid; }
+  unsigned perSim() const { return (unsigned)std::min(grid, (lay.cap + 7) / 8 * 8); }
 
   void step(fp_sim *const *sims, const fp_frame *frames, int n, double dt) {
     if (n <= 0) return;
@@ -1805,6 +1864,7 @@ struct fp_batch {
       phaseMs[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
       enqueue(sims, n);
     } catch (...) {
+      (void)hipStreamSynchronize(stream);
       for (int i = 0; i < n; i++) sims[i]->stream = own[i];
       throw;
     }
@@ -1823,10 +1883,10 @@ struct fp_batch {
     lastMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 
-  // Offsets of a solve's arguments and block prefixes in the uploaded buffer.
+  // The solve's structure, the same for every member.
   struct SolveShape {
-    size_t args, slots, tiles, one;
-    int tSlots, tTiles, tOne, n, brickLevels, bottom;
+    size_t args;
+    int brickLevels, bottom;
     bool smallBottom, bottom64;
   };
 
@@ -1835,32 +1895,32 @@ struct fp_batch {
   template <class St> int vcycle(const SolveShape &c, bool dry) {
     using Args = MGArgsT<typename St::T>;
     int f = 0;
-    auto go = [&](auto kernel, unsigned threads, size_t starts, int total, int code) {
-      if (!dry) launch<Args>(kernel, threads, c.args, starts, total, c.n, code);
+    auto go = [&](auto kernel, unsigned threads, size_t starts, unsigned g, int code) {
+      if (!dry) launch<Args>(kernel, threads, c.args, starts, g, code);
     };
     auto relax = [&](int L) {
-      if (L < c.brickLevels) go(k_mg_relax_bricks_batch<St>, 256, c.slots, c.tSlots, L | f << 8);
-      else go(k_mg_relax_tiles_batch<St>, 64, c.tiles, c.tTiles, L | f << 8);
+      if (L < c.brickLevels) go(k_mg_relax_bricks_batch<St>, 256, lay.slots, wide(), L | f << 8);
+      else go(k_mg_relax_tiles_batch<St>, 64, lay.tiles, wide(), L | f << 8);
       f ^= 1 << L;
     };
     std::function<void(int)> cycle = [&](int L) {
       if (L == c.bottom) {
-        if (c.bottom64) go(k_mg_bottom64_batch<St>, 1024, c.one, c.tOne, f << 8);
-        else if (c.smallBottom) go(k_mg_bottom_batch<St>, 1024, c.one, c.tOne, f << 8);
+        if (c.bottom64) go(k_mg_bottom64_batch<St>, 1024, lay.one, perSim(), f << 8);
+        else if (c.smallBottom) go(k_mg_bottom_batch<St>, 1024, lay.one, perSim(), f << 8);
         else
           for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax(L);
         return;
       }
-      if (L == 0) go(k_mg_init_batch<St>, 256, c.slots, c.tSlots, f << 8);
+      if (L == 0) go(k_mg_init_batch<St>, 256, lay.slots, wide(), f << 8);
       else {
         relax(L);
         relax(L);
       }
-      if (L + 1 < c.brickLevels) go(k_mg_restrict_bricks_batch<St>, 64, c.slots, c.tSlots, (L + 1) | f << 8);
-      else go(k_mg_restrict_tiles_batch<St>, 64, c.tiles, c.tTiles, (L + 1) | f << 8);
+      if (L + 1 < c.brickLevels) go(k_mg_restrict_bricks_batch<St>, 64, lay.slots, wide(), (L + 1) | f << 8);
+      else go(k_mg_restrict_tiles_batch<St>, 64, lay.tiles, wide(), (L + 1) | f << 8);
       cycle(L + 1);
-      if (L < c.brickLevels) go(k_mg_prolong_bricks_batch<St>, 256, c.slots, c.tSlots, L | f << 8);
-      else go(k_mg_prolong_tiles_batch<St>, 64, c.tiles, c.tTiles, L | f << 8);
+      if (L < c.brickLevels) go(k_mg_prolong_bricks_batch<St>, 256, lay.slots, wide(), L | f << 8);
+      else go(k_mg_prolong_tiles_batch<St>, 64, lay.tiles, wide(), L | f << 8);
       f ^= 1 << L;
       relax(L);
       relax(L);
@@ -1876,15 +1936,51 @@ struct fp_batch {
   // so no launch waits for the host.
   template <class St> void correctBatch(const SolveShape &c, int maxCycles) {
     using Args = MGArgsT<typename St::T>;
-    launch<Args>(k_dc_start_batch<St>, 256, c.args, c.slots, c.tSlots, c.n, 0);
+    launch<Args>(k_dc_start_batch<St>, 256, c.args, lay.slots, wide(), 0);
     for (int it = 0; it <= maxCycles; it++) {
-      launch<Args>(k_dc_residual_batch<St>, 256, c.args, c.slots, c.tSlots, c.n, 0);
-      launch<Args>(k_dc_norms_batch<St>, 256, c.args, c.one, c.tOne, c.n, it);
+      launch<Args>(k_dc_residual_batch<St>, 256, c.args, lay.slots, wide(), 0);
+      launch<Args>(k_dc_norms_batch<St>, 256, c.args, lay.one, perSim(), it);
       if (it == maxCycles) break;
-      launch<Args>(k_dc_scale_batch<St>, 256, c.args, c.slots, c.tSlots, c.n, 0);
+      launch<Args>(k_dc_scale_batch<St>, 256, c.args, lay.slots, wide(), 0);
       const int flips = vcycle<St>(c, false);
-      launch<Args>(k_dc_update_batch<St>, 256, c.args, c.slots, c.tSlots, c.n, flips << 8);
+      launch<Args>(k_dc_update_batch<St>, 256, c.args, lay.slots, wide(), flips << 8);
     }
+  }
+
+  // Every kernel of a step over the uploaded arguments, in stream order.
+  void launchStep(const SolveShape &shape, int mode, int maxCycles, int zchunks, int fchunks) {
+    // Allocation (fluid-allocation.wgsl).
+    launch<AllocArgs>(k_build_bricks_batch, 256, lay.alloc, lay.blocks, wide());
+    launch<AllocArgs>(k_scan_blocks_batch, 1024, lay.alloc, lay.one, perSim());
+    launch<AllocArgs>(k_free_bricks_batch, 256, lay.alloc, lay.blocks, wide());
+    launch<AllocArgs>(k_allocate_bricks_batch, 256, lay.alloc, lay.blocks, wide());
+    launch<AllocArgs>(k_link_bricks_batch, 64, lay.alloc, lay.link, wide());
+    launch<AllocArgs>(k_build_solids_batch, 64, lay.alloc, lay.slots, wide());
+    launch<AllocArgs>(k_zero_fresh_batch, 256, lay.alloc, lay.zero, wide(), zchunks);
+    // Transport, combustion, forces.
+    launch<StepArgs>(k_advect_velocity_batch, 256, lay.pre, lay.slots, wide());
+    launch<StepArgs>(k_correct_velocity_batch, 256, lay.pre, lay.slots, wide());
+    launch<StepArgs>(k_advect_scalars_batch, 256, lay.pre, lay.field, wide(), fchunks);
+    launch<StepArgs>(k_evolve_fields_batch, 256, lay.pre, lay.field, wide(), fchunks);
+    launch<StepArgs>(k_curl_batch, 256, lay.post, lay.slots, wide());
+    launch<StepArgs>(k_cell_forces_batch, 256, lay.post, lay.slots, wide());
+    launch<StepArgs>(k_forces_batch, 256, lay.post, lay.slots, wide());
+    launch<StepArgs>(k_divergence_batch, 256, lay.post, lay.slots, wide());
+    // Pressure: topology, flags, then one V-cycle or defect correction.
+    launch<MGArgs>(k_topology_bricks_batch, 256, lay.mg, lay.slots, wide(), 0);
+    for (int L = shape.brickLevels; L <= shape.bottom; L++)
+      launch<MGArgs>(k_topology_tiles_batch, 64, lay.mg, lay.tiles, wide(), L);
+    launch<MGArgs>(k_topology_bricks_batch, 256, lay.mg, lay.slots2, wide(), 1);
+    launch<MGArgs>(k_eflags_bricks_batch, 256, lay.mg, lay.slots, wide());
+    for (int L = shape.brickLevels; L <= shape.bottom; L++)
+      launch<MGArgs>(k_eflags_tiles_batch, 64, lay.mg, lay.tiles, wide(), L);
+    if (mode == 0) vcycle<StoreF32>(shape, false);
+    else if (mode == 1) correctBatch<StoreF32>(shape, maxCycles);
+    else if (mode == 2) correctBatch<StoreBF16>(shape, maxCycles);
+    else correctBatch<StoreF16>(shape, maxCycles);
+    launch<StepArgs>(k_project_batch, 256, lay.post, lay.slots, wide());
+    k_readback_batch<<<perSim(), 256, 0, stream>>>(reinterpret_cast<const ReadbackArgs *>(devArgs.p + lay.rb),
+                                                   lay.cap);
   }
 
   void enqueue(fp_sim *const *sims, int n) {
@@ -1930,14 +2026,15 @@ struct fp_batch {
       }
       words += 4 + (size_t)s.capacity;
     }
+    reserve(n);
     SolveShape shape{};
-    shape.n = n;
+    shape.args = mode == 0 ? lay.mg : lay.solve;
     shape.brickLevels = s0.brickLevels;
     shape.bottom = s0.bottom;
     shape.smallBottom = smallBottom;
     shape.bottom64 = mode != 0;
     // Where level 0's final iterate lands (P0 or P1, the same for every simulation), from a
-    // dry run of the V-cycle; defect correction leaves its solution in x.
+    // dry run of the V-cycle; defect correction leaves its solution in the pair x.
     const int flips = mode == 0 ? vcycle<StoreF32>(shape, true) : 0;
     for (int i = 0; i < n; i++) {
       post[i].pressure = mode != 0 ? sims[i]->dcX.p : (flips & 1) ? mg[i].lv[0].q : mg[i].lv[0].p;
@@ -1946,13 +2043,9 @@ struct fp_batch {
     if (readback.n < words) readback.alloc(words + 1024);
     if (pinnedWords < words) {
       if (pinnedReadback) (void)hipHostFree(pinnedReadback);
+      pinnedReadback = nullptr;
       pinnedWords = words + 1024;
       HIPCHECK(hipHostMalloc((void **)&pinnedReadback, pinnedWords * sizeof(int), hipHostMallocDefault));
-    }
-    if (reports.n < 2 * (size_t)n) {
-      reports.alloc(2 * (size_t)n + 64);
-      if (pinnedReports) (void)hipHostFree(pinnedReports);
-      HIPCHECK(hipHostMalloc((void **)&pinnedReports, reports.bytes(), hipHostMallocDefault));
     }
     size_t o = 0;
     for (int i = 0; i < n; i++) {
@@ -1961,66 +2054,45 @@ struct fp_batch {
                            mode != 0 ? s.dcNorms.p : nullptr, reports.p + 2 * i};
       o += 4 + (size_t)s.capacity;
     }
-    Packer pk;
-    const size_t oAlloc = pk.put(alloc), oPre = pk.put(pre), oPost = pk.put(post), oRb = pk.put(rb);
-    const size_t oMg = pk.put(mg);
-    shape.args = mode == 0 ? oMg : mode == 1 ? pk.put(solve32) : mode == 2 ? pk.put(solveBF16) : pk.put(solveF16);
-    const std::vector<int> pBlocks = prefix(cBlocks, 1), pOne = prefix(cOne, 1), pLink = prefix(cLink, 1),
-                           pSlots = prefix(cSlots, 8), pSlots2 = prefix(cSlots2, 8), pZero = prefix(cZero, 8),
-                           pField = prefix(cField, 8), pTiles = prefix(cTiles, 8);
-    const size_t sBlocks = pk.put(pBlocks), sLink = pk.put(pLink), sSlots2 = pk.put(pSlots2), sZero = pk.put(pZero),
-                 sField = pk.put(pField);
-    shape.one = pk.put(pOne);
-    shape.slots = pk.put(pSlots);
-    shape.tiles = pk.put(pTiles);
-    const size_t sSlots = shape.slots, sTiles = shape.tiles, sOne = shape.one;
-    if (hostArgsBytes < pk.bytes.size()) {
-      if (hostArgs) (void)hipHostFree(hostArgs);
-      hostArgsBytes = pk.bytes.size() * 2;
-      HIPCHECK(hipHostMalloc((void **)&hostArgs, hostArgsBytes, hipHostMallocDefault));
-      devArgs.alloc(hostArgsBytes);
-    }
     HIPCHECK(hipStreamSynchronize(stream));  // the previous step's upload has been read
+    put(lay.alloc, alloc);
+    put(lay.pre, pre);
+    put(lay.post, post);
+    put(lay.rb, rb);
+    put(lay.mg, mg);
+    if (mode == 1) put(lay.solve, solve32);
+    else if (mode == 2) put(lay.solve, solveBF16);
+    else if (mode == 3) put(lay.solve, solveF16);
+    putPrefix(lay.blocks, cBlocks, 1);
+    putPrefix(lay.one, cOne, 1);
+    putPrefix(lay.link, cLink, 1);
+    putPrefix(lay.slots, cSlots, 8);
+    putPrefix(lay.slots2, cSlots2, 8);
+    putPrefix(lay.zero, cZero, 8);
+    putPrefix(lay.field, cField, 8);
+    putPrefix(lay.tiles, cTiles, 8);
     auto l0 = std::chrono::steady_clock::now();
     phaseMs[1] = std::chrono::duration<double, std::milli>(l0 - e0).count();
-    std::memcpy(hostArgs, pk.bytes.data(), pk.bytes.size());
-    HIPCHECK(hipMemcpyAsync(devArgs.p, hostArgs, pk.bytes.size(), hipMemcpyHostToDevice, stream));
-    const int tBlocks = pBlocks[n], tLink = pLink[n], tSlots2 = pSlots2[n], tZero = pZero[n], tField = pField[n];
-    shape.tOne = pOne[n];
-    shape.tSlots = pSlots[n];
-    shape.tTiles = pTiles[n];
-    const int tSlots = shape.tSlots, tTiles = shape.tTiles;
-    (void)sOne;
-
-    // Allocation (fluid-allocation.wgsl).
-    launch<AllocArgs>(k_build_bricks_batch, 256, oAlloc, sBlocks, tBlocks, n);
-    launch<AllocArgs>(k_scan_blocks_batch, 1024, oAlloc, shape.one, shape.tOne, n);
-    launch<AllocArgs>(k_free_bricks_batch, 256, oAlloc, sBlocks, tBlocks, n);
-    launch<AllocArgs>(k_allocate_bricks_batch, 256, oAlloc, sBlocks, tBlocks, n);
-    launch<AllocArgs>(k_link_bricks_batch, 64, oAlloc, sLink, tLink, n);
-    launch<AllocArgs>(k_build_solids_batch, 64, oAlloc, sSlots, tSlots, n);
-    launch<AllocArgs>(k_zero_fresh_batch, 256, oAlloc, sZero, tZero, n, zchunks);
-    // Transport, combustion, forces.
-    launch<StepArgs>(k_advect_velocity_batch, 256, oPre, sSlots, tSlots, n);
-    launch<StepArgs>(k_correct_velocity_batch, 256, oPre, sSlots, tSlots, n);
-    launch<StepArgs>(k_advect_scalars_batch, 256, oPre, sField, tField, n, fchunks);
-    launch<StepArgs>(k_evolve_fields_batch, 256, oPre, sField, tField, n, fchunks);
-    launch<StepArgs>(k_curl_batch, 256, oPost, sSlots, tSlots, n);
-    launch<StepArgs>(k_cell_forces_batch, 256, oPost, sSlots, tSlots, n);
-    launch<StepArgs>(k_forces_batch, 256, oPost, sSlots, tSlots, n);
-    launch<StepArgs>(k_divergence_batch, 256, oPost, sSlots, tSlots, n);
-    // Pressure: topology, flags, then one V-cycle or defect correction.
-    launch<MGArgs>(k_topology_bricks_batch, 256, oMg, sSlots, tSlots, n, 0);
-    for (int L = s0.brickLevels; L <= s0.bottom; L++) launch<MGArgs>(k_topology_tiles_batch, 64, oMg, sTiles, tTiles, n, L);
-    launch<MGArgs>(k_topology_bricks_batch, 256, oMg, sSlots2, tSlots2, n, 1);
-    launch<MGArgs>(k_eflags_bricks_batch, 256, oMg, sSlots, tSlots, n);
-    for (int L = s0.brickLevels; L <= s0.bottom; L++) launch<MGArgs>(k_eflags_tiles_batch, 64, oMg, sTiles, tTiles, n, L);
-    if (mode == 0) vcycle<StoreF32>(shape, false);
-    else if (mode == 1) correctBatch<StoreF32>(shape, maxCycles);
-    else if (mode == 2) correctBatch<StoreBF16>(shape, maxCycles);
-    else correctBatch<StoreF16>(shape, maxCycles);
-    launch<StepArgs>(k_project_batch, 256, oPost, sSlots, tSlots, n);
-    k_readback_batch<<<(unsigned)n, 256, 0, stream>>>(reinterpret_cast<const ReadbackArgs *>(devArgs.p + oRb), n);
+    HIPCHECK(hipMemcpyAsync(devArgs.p, hostArgs, lay.bytes, hipMemcpyHostToDevice, stream));
+    if (graphs) {
+      const GraphKey key{mode, maxCycles, shape.brickLevels, shape.bottom, smallBottom ? 1 : 0, zchunks, fchunks,
+                         lay.cap};
+      auto it = execs.find(key);
+      if (it == execs.end()) {
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        launchStep(shape, mode, maxCycles, zchunks, fchunks);
+        HIPCHECK(hipStreamEndCapture(stream, &g));
+        hipGraphExec_t exec = nullptr;
+        hipError_t err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(err);
+        it = execs.emplace(key, exec).first;
+      }
+      HIPCHECK(hipGraphLaunch(it->second, stream));
+    } else {
+      launchStep(shape, mode, maxCycles, zchunks, fchunks);
+    }
     HIPCHECK(hipMemcpyAsync(pinnedReadback, readback.p, words * sizeof(int), hipMemcpyDeviceToHost, stream));
     if (mode != 0)
       HIPCHECK(hipMemcpyAsync(pinnedReports, reports.p, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -2321,6 +2393,17 @@ int fp_batch_last_ms(const fp_batch *batch, double *ms) {
   }
   ms[0] = batch->lastMs;
   return FP_OK;
+}
+
+int fp_batch_use_graphs(fp_batch *batch, int32_t enable) {
+  if (!batch) {
+    g_error = "fp_batch_use_graphs: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] {
+    HIPCHECK(hipStreamSynchronize(batch->stream));
+    batch->graphs = enable != 0;
+  });
 }
 
 int fp_batch_phases(const fp_batch *batch, double *ms) {

@@ -466,6 +466,7 @@ def load(path: str = LIBRARY):
         "fp_batch_step": (C.c_int, [C.c_void_p, P(C.c_void_p), P(Frame), C.c_int32, C.c_double]),
         "fp_batch_last_ms": (C.c_int, [C.c_void_p, P(C.c_double)]),
         "fp_batch_phases": (C.c_int, [C.c_void_p, P(C.c_double)]),
+        "fp_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
         "fp_sim_set_pressure": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32]),
         "fp_sim_pressure_report": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_double)]),
         "fp_sim_render": (C.c_int, [C.c_void_p, P(Camera), P(RenderSettings), P(C.c_float)]),

@@ -244,6 +244,10 @@ class Batch:
         abi.check(self._lib.fp_batch_last_ms(self._handle, C.byref(ms)), self._lib)
         return ms.value
 
+    def use_graphs(self, enable: bool) -> None:
+        """Replay captured HIP graphs (the default) or launch every kernel each step."""
+        abi.check(self._lib.fp_batch_use_graphs(self._handle, int(bool(enable))), self._lib)
+
     @property
     def phases(self) -> dict:
         ms = (C.c_double * 5)()
