@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--serving-count", type=int, default=256)
     p.add_argument("--serving-extra", action="store_true",
                    help="with --serving: also every request at once and the single V-cycle policy")
+    p.add_argument("--serving-no-graphs", action="store_true",
+                   help="with --serving: launch the batched kernels one by one (for kernel tracing)")
     p.add_argument("--serving-rate", type=float, default=20.0)
     p.add_argument("--copy-probe", type=int, default=0,
                    help="only copy this many MiB device to device 5 times (counter calibration)")
@@ -112,7 +114,8 @@ def serving_line(args) -> dict:
         print(line, file=sys.stderr, flush=True)
 
     def run(rate, batched, pressure):
-        rep = serving.serve(serving.requests(count, rate), batched=batched, pressure=pressure, progress=say)
+        rep = serving.serve(serving.requests(count, rate), batched=batched, pressure=pressure, progress=say,
+                            graphs=not args.serving_no_graphs)
         out = rep.summary()
         say(f"served: {out['mode']} {out['pressure']} rate {rate}: {out['voxel_steps_per_s']:.3e} voxel-steps/s")
         return out
@@ -120,7 +123,7 @@ def serving_line(args) -> dict:
     warm = serving.requests(count, 1e9)
     for r in warm:
         r.steps = 1
-    serving.serve(warm, batched=True, pressure=mixed)  # device blocks and streams, first made
+    serving.serve(warm, batched=True, pressure=mixed, graphs=not args.serving_no_graphs)  # blocks, streams
     stream_b, stream_s = run(args.serving_rate, True, mixed), run(args.serving_rate, False, plain)
     ratio = lambda a, b: a["voxel_steps_per_s"] / b["voxel_steps_per_s"] if b["voxel_steps_per_s"] else None  # noqa: E731
     line = {
@@ -133,6 +136,7 @@ def serving_line(args) -> dict:
         "speedup": ratio(stream_b, stream_s),
         "p50_step_ms": stream_b["p50_step_ms"], "p99_step_ms": stream_b["p99_step_ms"],
         "batched_bf16": stream_b, "one_sim_per_launch_fp32": stream_s,
+        "graphs": not args.serving_no_graphs,
     }
     if args.serving_extra:
         burst_b, burst_s = run(1e9, True, mixed), run(1e9, False, plain)

@@ -141,6 +141,110 @@ __device__ __forceinline__ void levelCell(int home0, int home1, int home2, int s
   c.tc[2] = trec[2];                                                            \
   c.tnb = tnb;
 
+// Brick-level and tile-level sweeps touch few cells per brick at the coarse levels (64,
+// 8 or 1 per velocity brick of 4^3), so a workgroup takes a group of consecutive items
+// (active bricks or tile slots) with `cells` lanes each: group g covers items g*k .. g*k+k-1
+// with k = lanes / cells (at most MG_GROUP); lane t works on cell t % cells of item
+// t / cells. With cells >= lanes, k = 1 and the lanes stride over the item's cells.
+constexpr int MG_GROUP = 32;
+__device__ __forceinline__ int groupSize(int cells, int lanes) {
+  return cells >= lanes ? 1 : (lanes / cells < MG_GROUP ? lanes / cells : MG_GROUP);
+}
+struct Group {
+  int k, g, j, first, step;  // items per group, group, this lane's item in it, first cell, cell stride
+};
+__device__ __forceinline__ Group groupOf(int g, int cells) {
+  Group r;
+  r.k = groupSize(cells, (int)blockDim.x);
+  r.g = g;
+  if (r.k == 1) {
+    r.j = 0;
+    r.first = (int)threadIdx.x;
+    r.step = (int)blockDim.x;
+  } else {
+    r.j = (int)threadIdx.x / cells;
+    r.first = (int)threadIdx.x % cells;
+    r.step = cells;
+    if (r.j >= r.k) r.j = -1;  // lanes past the group's k * cells
+  }
+  return r;
+}
+// The group's bricks' 27 neighbor entries and their pool slots, in LDS (all lanes take
+// part); this lane's brick's home.
+__device__ __forceinline__ void loadGroupHomes(const Dir &dir, const int *active, int count, const Group &gr,
+                                               int (*recs)[54], int side) {
+  for (int e = threadIdx.x; e < gr.k * 27; e += blockDim.x) {
+    const int jj = e / 27, w = e - jj * 27;
+    const int b = gr.g * gr.k + jj;
+    if (b < count) {
+      const int r = dir.records[(size_t)active[b] * RECORD_WORDS + w];
+      recs[jj][w] = r;
+      recs[jj][27 + w] = entrySlot(r, side);
+    }
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ Home homeOf(const Dir &dir, int id, const int *rec) {
+  Home h;
+  const int *tr = dir.tileRec + (size_t)(id >> 6) * TILE_RECORD_WORDS;
+  const int l = id & 63;
+  h.bx = tr[0] * 4 + (l & 3);
+  h.by = tr[1] * 4 + ((l >> 2) & 3);
+  h.bz = tr[2] * 4 + (l >> 4);
+  h.rec = rec;
+  h.slot = rec + 27;
+  return h;
+}
+// A group of active bricks with `cells` cells each at the level swept: declares `gr`, the
+// brick index `bi` of this lane (valid when `mine`), and with HOME its MGCtx `c`.
+#define MG_BRICK_GROUP(CELLS)                                                   \
+  const int count_ = m.counts[0];                                              \
+  const int k_ = groupSize((CELLS), (int)blockDim.x);                          \
+  const int g_ = xcdItem(vb, (count_ + k_ - 1) / k_);                          \
+  if (g_ < 0) return;                                                          \
+  const Group gr = groupOf(g_, (CELLS));                                       \
+  const int bi = gr.g * gr.k + gr.j;                                           \
+  const bool mine = gr.j >= 0 && bi < count_;
+#define MG_GROUP_HOME                                                           \
+  __shared__ int recs_[MG_GROUP][54];                                          \
+  loadGroupHomes(m.dir, m.active, count_, gr, recs_, m.L.side);                \
+  if (!mine) return;                                                           \
+  const int id = m.active[bi];                                                 \
+  MGCtx c;                                                                     \
+  c.dir = m.dir;                                                               \
+  c.L = m.L;                                                                   \
+  c.side = m.L.side;                                                           \
+  c.hasBrick = true;                                                           \
+  c.hasTile = false;                                                           \
+  c.home = homeOf(m.dir, id, recs_[gr.j]);                                     \
+  const int *rec = recs_[gr.j];
+// A group of tile slots with `cells` cells each: declares `gr`, this lane's tile `t` and,
+// for a live slot, its MGCtx `c` (27 neighbor slots in LDS).
+#define MG_TILE_GROUP(CELLS)                                                    \
+  const int k_ = groupSize((CELLS), (int)blockDim.x);                          \
+  if (vb * k_ >= m.dir.capacity) return;                                       \
+  const Group gr = groupOf(vb, (CELLS));                                       \
+  const int t = gr.g * gr.k + gr.j;                                            \
+  __shared__ int tnbs_[MG_GROUP][27];                                          \
+  for (int e = threadIdx.x; e < gr.k * 27; e += blockDim.x) {                  \
+    const int jj = e / 27, w = e - jj * 27, tt = gr.g * gr.k + jj;             \
+    if (tt < m.dir.capacity) tnbs_[jj][w] = m.dir.tileRec[(size_t)tt * TILE_RECORD_WORDS + 4 + w]; \
+  }                                                                            \
+  __syncthreads();                                                             \
+  if (gr.j < 0 || t >= m.dir.capacity) return;                                 \
+  const int *trec = m.dir.tileRec + (size_t)t * TILE_RECORD_WORDS;             \
+  if (trec[3] == 0) return;                                                    \
+  MGCtx c;                                                                     \
+  c.dir = m.dir;                                                               \
+  c.L = m.L;                                                                   \
+  c.side = m.L.side;                                                           \
+  c.hasBrick = false;                                                          \
+  c.hasTile = true;                                                            \
+  c.tc[0] = trec[0];                                                           \
+  c.tc[1] = trec[1];                                                           \
+  c.tc[2] = trec[2];                                                           \
+  c.tnb = tnbs_[gr.j];
+
 // buildBrickTopology (pressure.wgsl:259-278) for every active brick (when the geometry
 // changed), or for the bricks of the zeroed list that hold a slot.
 __device__ __forceinline__ void topology_bricks_body(const MGArgs &m, const int vb, int fromZeroList) {
@@ -283,13 +387,13 @@ __device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelView<typena
 
 // initializePressure (pressure.wgsl:485-505): level 0's first iterate from zero.
 template <class St> __device__ __forceinline__ void mg_init_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
-  const int bi = xcdItem(vb, m.counts[0]);
-  if (bi < 0) return;
-  const int id = m.active[bi];
   const auto lv = levelView(m, 0, code >> 8);
+  const int sd = lv.side, n = sd * sd * sd;
+  MG_BRICK_GROUP(n)
+  if (!mine) return;
+  const int id = m.active[bi];
   const int slot = entrySlot(m.dir.entries[id], m.L.side);
-  int sd = lv.side, n = sd * sd * sd;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  for (int l = gr.first; l < n; l += gr.step) {
     size_t a = (size_t)slot * n + l;
     int e = lv.eflag[a];
     float first = 0.f;
@@ -303,14 +407,12 @@ FP_MG_PAIR(mg_init, 256)
 template <class St>
 __device__ __forceinline__ void mg_relax_bricks_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   const int level = code & 255;
-  const int bi = xcdItem(vb, m.counts[0]);
-  if (bi < 0) return;
-  const int id = m.active[bi];
-  MG_BRICK_HOME(id)
-  int slot = entrySlot(rec[13], c.side);
   const auto lv = levelView(m, level, code >> 8);
-  int sd = lv.side, n = sd * sd * sd;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  const int sd = lv.side, n = sd * sd * sd;
+  MG_BRICK_GROUP(n)
+  MG_GROUP_HOME
+  const int slot = entrySlot(rec[13], c.side);
+  for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
     relaxCell<St>(c, lv, level, p, (size_t)slot * n + l, m.weight);
@@ -320,12 +422,10 @@ FP_MG_PAIR(mg_relax_bricks, 256)
 template <class St>
 __device__ __forceinline__ void mg_relax_tiles_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   const int level = code & 255;
-  const int t = vb;
-  if (t >= m.dir.capacity) return;
-  MG_TILE_HOME(t)
   const auto lv = levelView(m, level, code >> 8);
-  int sd = lv.side, n = sd * sd * sd;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  const int sd = lv.side, n = sd * sd * sd;
+  MG_TILE_GROUP(n)
+  for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
     relaxCell<St>(c, lv, level, p, (size_t)t * n + l, m.weight);
@@ -370,26 +470,23 @@ __device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typen
 template <class St>
 __device__ __forceinline__ void mg_restrict_bricks_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   const int level = code & 255;
-  const int bi = xcdItem(vb, m.counts[0]);
-  if (bi < 0) return;
-  const int id = m.active[bi];
-  MG_BRICK_HOME(id)
-  int sd = m.lv[level].side, n = sd * sd * sd;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  const int sd = m.lv[level].side, n = sd * sd * sd;
+  MG_BRICK_GROUP(n)
+  MG_GROUP_HOME
+  (void)rec;
+  for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
     restrictCell<St>(c, m, level, code >> 8, p);
   }
 }
-FP_MG_PAIR(mg_restrict_bricks, 64)
+FP_MG_PAIR(mg_restrict_bricks, 256)
 template <class St>
 __device__ __forceinline__ void mg_restrict_tiles_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   const int level = code & 255;
-  const int t = vb;
-  if (t >= m.dir.capacity) return;
-  MG_TILE_HOME(t)
-  int sd = m.lv[level].side, n = sd * sd * sd;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  const int sd = m.lv[level].side, n = sd * sd * sd;
+  MG_TILE_GROUP(n)
+  for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
     restrictCell<St>(c, m, level, code >> 8, p);
@@ -455,13 +552,11 @@ __device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgsT<typena
 template <class St>
 __device__ __forceinline__ void mg_prolong_bricks_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   const int level = code & 255;
-  const int bi = xcdItem(vb, m.counts[0]);
-  if (bi < 0) return;
-  const int id = m.active[bi];
-  MG_BRICK_HOME(id)
-  int slot = entrySlot(rec[13], c.side);
-  int sd = m.lv[level].side, n = sd * sd * sd;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  const int sd = m.lv[level].side, n = sd * sd * sd;
+  MG_BRICK_GROUP(n)
+  MG_GROUP_HOME
+  const int slot = entrySlot(rec[13], c.side);
+  for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
     prolongCell<St>(c, m, level, code >> 8, p, (size_t)slot * n + l);
@@ -471,11 +566,9 @@ FP_MG_PAIR(mg_prolong_bricks, 256)
 template <class St>
 __device__ __forceinline__ void mg_prolong_tiles_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   const int level = code & 255;
-  const int t = vb;
-  if (t >= m.dir.capacity) return;
-  MG_TILE_HOME(t)
-  int sd = m.lv[level].side, n = sd * sd * sd;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  const int sd = m.lv[level].side, n = sd * sd * sd;
+  MG_TILE_GROUP(n)
+  for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
     prolongCell<St>(c, m, level, code >> 8, p, (size_t)t * n + l);
@@ -501,17 +594,16 @@ template <class St> __device__ __forceinline__ void mg_bottom_body(const MGArgsT
   float mine = valid ? St::ld(lv.p[lane]) : 0.f;
   if (valid && !(e & UNKNOWN)) mine = 0.f;
   value[lane] = mine;
+  // The open sides' neighbor slots, read once (-1: closed or no neighbor).
+  int nbs[6];
+  for (int sd = 0; sd < 6; sd++)
+    nbs[sd] = live && ((flags >> sd) & 1) ? trec[4 + (OFF[sd][2] + 1) * 9 + (OFF[sd][1] + 1) * 3 + (OFF[sd][0] + 1)] : -1;
   __syncthreads();
   for (int it = 0; it < BOTTOM_ITERATIONS; it++) {
     float next = 0.f;
     if (live) {
       float n[6];
-      for (int sd = 0; sd < 6; sd++) {
-        n[sd] = 0.f;
-        if (!((flags >> sd) & 1)) continue;
-        int nb = trec[4 + (OFF[sd][2] + 1) * 9 + (OFF[sd][1] + 1) * 3 + (OFF[sd][0] + 1)];
-        if (nb >= 0) n[sd] = value[nb];
-      }
+      for (int sd = 0; sd < 6; sd++) n[sd] = nbs[sd] >= 0 ? value[nbs[sd]] : 0.f;
       float w = m.weight;
       float sum = w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
       float cand = (sum - rhs) / fmaxv(diagonal(w, flags), 1e-12f);
@@ -541,15 +633,17 @@ __device__ __forceinline__ void mg_bottom64_body(const MGArgsT<typename St::T> &
   double mine[PER], rhs[PER];
   int flags[PER];
   bool live[PER];
+  // Lanes take tile slots lane, lane + 1024, ...; `per` rounds cover the capacity.
+  const int per = (cap + BOTTOM_LANES - 1) / BOTTOM_LANES;
   for (int k = 0; k < PER; k++) {
     const int t = threadIdx.x + k * BOTTOM_LANES;
-    bool valid = t < cap && m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] != 0;
+    bool valid = k < per && t < cap && m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] != 0;
     int e = valid ? lv.eflag[t] : 0;
     live[k] = valid && (e & UNKNOWN) && !(e & STORED_SOLID);
     flags[k] = e & 63;
     rhs[k] = valid ? (double)St::ld(lv.rhs[t]) : 0.0;
     mine[k] = valid && (e & UNKNOWN) ? (double)St::ld(lv.p[t]) : 0.0;
-    if (t < cap) value[t] = mine[k];
+    if (k < per && t < cap) value[t] = mine[k];
   }
   __syncthreads();
   for (int it = 0; it < BOTTOM_ITERATIONS; it++) {
@@ -576,13 +670,13 @@ __device__ __forceinline__ void mg_bottom64_body(const MGArgsT<typename St::T> &
     for (int k = 0; k < PER; k++) {
       const int t = threadIdx.x + k * BOTTOM_LANES;
       mine[k] = next[k];
-      if (t < cap) value[t] = next[k];
+      if (k < per && t < cap) value[t] = next[k];
     }
     __syncthreads();
   }
   for (int k = 0; k < PER; k++) {
     const int t = threadIdx.x + k * BOTTOM_LANES;
-    if (t < cap && m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] != 0) lv.p[t] = St::st((float)mine[k]);
+    if (k < per && t < cap && m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] != 0) lv.p[t] = St::st((float)mine[k]);
   }
 }
 FP_MG_PAIR(mg_bottom64, 1024)
@@ -619,40 +713,50 @@ __global__ void __launch_bounds__(256)
 template <class St>
 __device__ __forceinline__ void dc_residual_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   (void)code;
-  const int bi = xcdItem(vb, m.counts[0]);
-  if (bi < 0) return;
-  const int id = m.active[bi];
-  MG_BRICK_HOME(id)
-  __shared__ double red[3][256];
-  int slot = entrySlot(rec[13], c.side);
   const auto &lv = m.lv[0];
-  const float *x = m.dcX;
-  int sd = lv.side, n = sd * sd * sd;
+  const int sd = lv.side, n = sd * sd * sd;
+  MG_BRICK_GROUP(n)
+  __shared__ double red[3][256];
+  __shared__ int recs_[MG_GROUP][54];
+  loadGroupHomes(m.dir, m.active, count_, gr, recs_, m.L.side);
   double rr = 0.0, bb = 0.0;
   float big = 0.f;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
-    size_t a = (size_t)slot * n + l;
-    int e = lv.eflag[a];
-    float v = 0.f;
-    if (!(e & STORED_SOLID) && (e & UNKNOWN)) {
-      int p[3];
-      levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-      int flags = e & 63;
-      const double w = (double)m.weight;
-      double sum = 0.0;
-      for (int k = 0; k < 6; k++) {
-        if (!((flags >> k) & 1)) continue;
-        int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
-        sum += w * ((double)readAt<StoreF32>(c, x, q, 0) + (double)readAt<StoreF32>(c, m.dcXlo, q, 0));
+  if (mine) {
+    const int id = m.active[bi];
+    MGCtx c;
+    c.dir = m.dir;
+    c.L = m.L;
+    c.side = m.L.side;
+    c.hasBrick = true;
+    c.hasTile = false;
+    c.home = homeOf(m.dir, id, recs_[gr.j]);
+    const int slot = entrySlot(recs_[gr.j][13], c.side);
+    const float *x = m.dcX;
+    for (int l = gr.first; l < n; l += gr.step) {
+      size_t a = (size_t)slot * n + l;
+      int e = lv.eflag[a];
+      float v = 0.f;
+      if (!(e & STORED_SOLID) && (e & UNKNOWN)) {
+        int p[3];
+        levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
+        int flags = e & 63;
+        const double w = (double)m.weight;
+        double sum = 0.0;
+        for (int k = 0; k < 6; k++) {
+          if (!((flags >> k) & 1)) continue;
+          int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
+          sum += w * ((double)readAt<StoreF32>(c, x, q, 0) + (double)readAt<StoreF32>(c, m.dcXlo, q, 0));
+        }
+        float bv = m.dcB[a];
+        v = (float)((double)bv - (sum - (double)diagonal(m.weight, flags) * ((double)x[a] + (double)m.dcXlo[a])));
+        rr += (double)v * (double)v;
+        bb += (double)bv * (double)bv;
+        big = fmaxv(big, fabsv(v));
       }
-      float bv = m.dcB[a];
-      v = (float)((double)bv - (sum - (double)diagonal(m.weight, flags) * ((double)x[a] + (double)m.dcXlo[a])));
-      rr += (double)v * (double)v;
-      bb += (double)bv * (double)bv;
-      big = fmaxv(big, fabsv(v));
+      m.dcR[a] = v;
     }
-    m.dcR[a] = v;
   }
+  // The group's sums: one partial per group, in a fixed order.
   red[0][threadIdx.x] = rr;
   red[1][threadIdx.x] = bb;
   red[2][threadIdx.x] = (double)big;
@@ -666,9 +770,9 @@ __device__ __forceinline__ void dc_residual_body(const MGArgsT<typename St::T> &
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    m.dcPartials[3 * bi] = red[0][0];
-    m.dcPartials[3 * bi + 1] = red[1][0];
-    m.dcPartials[3 * bi + 2] = red[2][0];
+    m.dcPartials[3 * gr.g] = red[0][0];
+    m.dcPartials[3 * gr.g + 1] = red[1][0];
+    m.dcPartials[3 * gr.g + 2] = red[2][0];
   }
 }
 FP_MG_PAIR(dc_residual, 256)
@@ -680,7 +784,9 @@ template <class St> __device__ __forceinline__ void dc_norms_body(const MGArgsT<
   (void)code;
   if (vb != 0) return;
   __shared__ double red[3][256];
-  const int blocks = m.counts[0];
+  const int n0 = m.lv[0].side * m.lv[0].side * m.lv[0].side;
+  const int k0 = groupSize(n0, 256);  // the residual's groups (256 lanes)
+  const int blocks = (m.counts[0] + k0 - 1) / k0;
   double rr = 0.0, bb = 0.0, big = 0.0;
   for (int i = threadIdx.x; i < blocks; i += 256) {
     rr += m.dcPartials[3 * i];
@@ -719,14 +825,14 @@ FP_MG_PAIR(dc_norms, 256)
 // Level 0's right-hand side: r / scale, stored.
 template <class St> __device__ __forceinline__ void dc_scale_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   (void)code;
-  const int bi = xcdItem(vb, m.counts[0]);
-  if (bi < 0) return;
+  const auto &lv = m.lv[0];
+  const int n = lv.side * lv.side * lv.side;
+  MG_BRICK_GROUP(n)
+  if (!mine) return;
   const int id = m.active[bi];
   const int slot = entrySlot(m.dir.entries[id], m.L.side);
-  const auto &lv = m.lv[0];
   const float inv = 1.0f / (float)m.dcNorms[3];
-  int n = lv.side * lv.side * lv.side;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  for (int l = gr.first; l < n; l += gr.step) {
     size_t a = (size_t)slot * n + l;
     lv.rhs[a] = St::st(m.dcR[a] * inv);
   }
@@ -748,14 +854,14 @@ __device__ __forceinline__ void addToPair(float &hi, float &lo, float v) {
   hi = h;
 }
 template <class St> __device__ __forceinline__ void dc_update_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
-  const int bi = xcdItem(vb, m.counts[0]);
-  if (bi < 0) return;
+  const auto lv = levelView(m, 0, code >> 8);
+  const int n = lv.side * lv.side * lv.side;
+  MG_BRICK_GROUP(n)
+  if (!mine) return;
   const int id = m.active[bi];
   const int slot = entrySlot(m.dir.entries[id], m.L.side);
-  const auto lv = levelView(m, 0, code >> 8);
   const float scale = (float)m.dcNorms[3];
-  int n = lv.side * lv.side * lv.side;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  for (int l = gr.first; l < n; l += gr.step) {
     size_t a = (size_t)slot * n + l;
     int e = lv.eflag[a];
     float hi = 0.f, lo = 0.f;

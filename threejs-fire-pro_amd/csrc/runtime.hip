@@ -1345,7 +1345,7 @@ struct fp_sim {
       relax<St>(m, L, slotsGrid, flips);
       relax<St>(m, L, slotsGrid, flips);
     }
-    if (L + 1 < brickLevels) k_mg_restrict_bricks<St><<<slotsGrid, 64, 0, stream>>>(m, (L + 1) | flips << 8);
+    if (L + 1 < brickLevels) k_mg_restrict_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, (L + 1) | flips << 8);
     else k_mg_restrict_tiles<St><<<capacity, 64, 0, stream>>>(m, (L + 1) | flips << 8);
     cycle<St>(m, L + 1, slotsGrid, bottom64, flips);
     if (L < brickLevels) k_mg_prolong_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, L | flips << 8);
@@ -1916,7 +1916,7 @@ struct fp_batch {
         relax(L);
         relax(L);
       }
-      if (L + 1 < c.brickLevels) go(k_mg_restrict_bricks_batch<St>, 64, lay.slots, wide(), (L + 1) | f << 8);
+      if (L + 1 < c.brickLevels) go(k_mg_restrict_bricks_batch<St>, 256, lay.slots, wide(), (L + 1) | f << 8);
       else go(k_mg_restrict_tiles_batch<St>, 64, lay.tiles, wide(), (L + 1) | f << 8);
       cycle(L + 1);
       if (L < c.brickLevels) go(k_mg_prolong_bricks_batch<St>, 256, lay.slots, wide(), L | f << 8);

@@ -131,11 +131,12 @@ class Report:
 
 
 def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, dt: float = 1 / 60,
-          pressure: Optional[tuple] = None, progress=None) -> Report:
+          pressure: Optional[tuple] = None, progress=None, graphs: bool = True) -> Report:
     """Runs the request stream in real time and returns its report. `pressure`, as the
     arguments of FluidSimulation.set_pressure, is every simulation's pressure policy (the
     reference's single V-cycle when None). `progress`, if given, is called with a line of
-    text every few seconds."""
+    text every few seconds. `graphs` False launches the batched kernels one by one instead
+    of replaying captured graphs."""
     rep = Report("batched" if batched else "one simulation per launch")
     rep.pressure = pressure
     said = time.perf_counter()
@@ -143,6 +144,8 @@ def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, 
     resident: List[Request] = []
     reserved = 0
     batch = Batch() if batched else None
+    if batch is not None and not graphs:
+        batch.use_graphs(False)
     t0 = time.perf_counter()
     try:
         while pending or resident:
