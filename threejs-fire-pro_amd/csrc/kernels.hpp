@@ -188,6 +188,9 @@ FP_DECLARE_MG(dc_residual)
 FP_DECLARE_MG(dc_norms)
 FP_DECLARE_MG(dc_scale)
 FP_DECLARE_MG(dc_update)
+// The batched solve's per-level virtual-block prefixes over unfinished members.
+template <class St>
+__global__ void k_level_prefix_batch(const MGArgsT<typename St::T> *args, int n, int *out);
 constexpr int BOTTOM64_TILES = 8192;
 constexpr int BOTTOM_ITERATIONS = 19;  // PressureSolver.ts:326
 

@@ -876,6 +876,67 @@ template <class St> __device__ __forceinline__ void dc_update_body(const MGArgsT
 }
 FP_MG_PAIR(dc_update, 256)
 
+// The batched solve's virtual blocks, per level: for member i, ceil(units / k) groups
+// rounded up to a multiple of 8 (units: active bricks at a brick level, tile slots at a
+// tile level; k: groupSize at 256 lanes for brick levels, 64 for tile levels), or none
+// once its defect correction has finished; out[MAX_LEVELS] holds one block per
+// unfinished member (the per-simulation kernels). Prefix sums, `n` members, each array
+// n + 1 entries; one workgroup of 1024 lanes.
+constexpr int PREFIX_LANES = 1024;
+template <class St>
+__global__ void __launch_bounds__(PREFIX_LANES) k_level_prefix_batch(const MGArgsT<typename St::T> *args, int n, int *out) {
+  __shared__ int scan[PREFIX_LANES];
+  const int per = (n + PREFIX_LANES - 1) / PREFIX_LANES;
+  const int lo = (int)threadIdx.x * per, hi = lo + per < n ? lo + per : n;
+  const int bottom = n > 0 && args[0].counts ? args[0].bottom : 0;
+  for (int L = 0; L <= MAX_LEVELS; L++) {
+    if (L < MAX_LEVELS && L > bottom) continue;
+    int *o = out + (size_t)L * (n + 1);
+    int mine = 0;
+    for (int i = lo; i < hi; i++) {
+      const MGArgsT<typename St::T> &a = args[i];
+      if (!a.counts || dcFinished(a)) continue;
+      if (L == MAX_LEVELS) {
+        mine += 1;
+        continue;
+      }
+      const int sd = a.lv[L].side, cells = sd * sd * sd;
+      const bool tile = a.lv[L].tile != 0;
+      const int units = tile ? a.dir.capacity : a.counts[0];
+      const int k = groupSize(cells, tile ? 64 : 256);
+      mine += ((units + k - 1) / k + 7) / 8 * 8;
+    }
+    scan[threadIdx.x] = mine;
+    __syncthreads();
+    for (int d = 1; d < PREFIX_LANES; d <<= 1) {
+      int v = (int)threadIdx.x >= d ? scan[threadIdx.x - d] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    int run = scan[threadIdx.x] - mine;  // exclusive
+    for (int i = lo; i < hi; i++) {
+      o[i] = run;
+      const MGArgsT<typename St::T> &a = args[i];
+      if (!a.counts || dcFinished(a)) continue;
+      if (L == MAX_LEVELS) {
+        run += 1;
+        continue;
+      }
+      const int sd = a.lv[L].side, cells = sd * sd * sd;
+      const bool tile = a.lv[L].tile != 0;
+      const int units = tile ? a.dir.capacity : a.counts[0];
+      const int k = groupSize(cells, tile ? 64 : 256);
+      run += ((units + k - 1) / k + 7) / 8 * 8;
+    }
+    if (threadIdx.x == PREFIX_LANES - 1) o[n] = scan[PREFIX_LANES - 1];
+    __syncthreads();
+  }
+}
+template __global__ void k_level_prefix_batch<StoreF32>(const MGArgsT<float> *, int, int *);
+template __global__ void k_level_prefix_batch<StoreBF16>(const MGArgsT<uint16_t> *, int, int *);
+template __global__ void k_level_prefix_batch<StoreF16>(const MGArgsT<_Float16> *, int, int *);
+
 #define FP_MG_INSTANCE(NAME, St)                                         \
   template __global__ void k_##NAME<St>(MGArgsT<St::T>, int);           \
   template __global__ void k_##NAME##_batch<St>(const MGArgsT<St::T> *, const int *, int, int);

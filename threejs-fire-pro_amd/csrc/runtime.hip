@@ -1748,8 +1748,11 @@ struct fp_batch {
   // grids depend on the capacity alone and a captured launch sequence serves any members.
   struct Layout {
     int cap = 0;
-    size_t alloc, pre, post, rb, mg, solve, blocks, one, link, slots, slots2, zero, field, tiles, bytes;
+    size_t alloc, pre, post, rb, mg, solve, blocks, one, link, slots, slots2, zero, field, tiles, levels, bytes;
   } lay;
+  // The solve's virtual-block prefix of level L (MAX_LEVELS: one block per unfinished
+  // member), written on the device by k_level_prefix_batch.
+  size_t levelPrefix(int L) const { return lay.levels + (size_t)L * (lay.cap + 1) * sizeof(int); }
   static Layout layoutFor(int cap) {
     Layout L;
     L.cap = cap;
@@ -1768,6 +1771,7 @@ struct fp_batch {
     L.solve = take(cap * solveBytes);
     size_t *prefixes[] = {&L.blocks, &L.one, &L.link, &L.slots, &L.slots2, &L.zero, &L.field, &L.tiles};
     for (size_t *q : prefixes) *q = take((cap + 1) * sizeof(int));
+    L.levels = take((size_t)(MAX_LEVELS + 1) * (cap + 1) * sizeof(int));
     L.bytes = (o + 255) / 256 * 256;
     return L;
   }
@@ -1899,28 +1903,28 @@ struct fp_batch {
       if (!dry) launch<Args>(kernel, threads, c.args, starts, g, code);
     };
     auto relax = [&](int L) {
-      if (L < c.brickLevels) go(k_mg_relax_bricks_batch<St>, 256, lay.slots, wide(), L | f << 8);
-      else go(k_mg_relax_tiles_batch<St>, 64, lay.tiles, wide(), L | f << 8);
+      if (L < c.brickLevels) go(k_mg_relax_bricks_batch<St>, 256, levelPrefix(L), wide(), L | f << 8);
+      else go(k_mg_relax_tiles_batch<St>, 64, levelPrefix(L), wide(), L | f << 8);
       f ^= 1 << L;
     };
     std::function<void(int)> cycle = [&](int L) {
       if (L == c.bottom) {
-        if (c.bottom64) go(k_mg_bottom64_batch<St>, 1024, lay.one, perSim(), f << 8);
-        else if (c.smallBottom) go(k_mg_bottom_batch<St>, 1024, lay.one, perSim(), f << 8);
+        if (c.bottom64) go(k_mg_bottom64_batch<St>, 1024, levelPrefix(MAX_LEVELS), perSim(), f << 8);
+        else if (c.smallBottom) go(k_mg_bottom_batch<St>, 1024, levelPrefix(MAX_LEVELS), perSim(), f << 8);
         else
           for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax(L);
         return;
       }
-      if (L == 0) go(k_mg_init_batch<St>, 256, lay.slots, wide(), f << 8);
+      if (L == 0) go(k_mg_init_batch<St>, 256, levelPrefix(0), wide(), f << 8);
       else {
         relax(L);
         relax(L);
       }
-      if (L + 1 < c.brickLevels) go(k_mg_restrict_bricks_batch<St>, 256, lay.slots, wide(), (L + 1) | f << 8);
-      else go(k_mg_restrict_tiles_batch<St>, 64, lay.tiles, wide(), (L + 1) | f << 8);
+      if (L + 1 < c.brickLevels) go(k_mg_restrict_bricks_batch<St>, 256, levelPrefix(L + 1), wide(), (L + 1) | f << 8);
+      else go(k_mg_restrict_tiles_batch<St>, 64, levelPrefix(L + 1), wide(), (L + 1) | f << 8);
       cycle(L + 1);
-      if (L < c.brickLevels) go(k_mg_prolong_bricks_batch<St>, 256, lay.slots, wide(), L | f << 8);
-      else go(k_mg_prolong_tiles_batch<St>, 64, lay.tiles, wide(), L | f << 8);
+      if (L < c.brickLevels) go(k_mg_prolong_bricks_batch<St>, 256, levelPrefix(L), wide(), L | f << 8);
+      else go(k_mg_prolong_tiles_batch<St>, 64, levelPrefix(L), wide(), L | f << 8);
       f ^= 1 << L;
       relax(L);
       relax(L);
@@ -1934,16 +1938,23 @@ struct fp_batch {
   // fp_sim::correct, for up to `maxCycles` cycles; one that has finished (its own
   // tolerance met or its own cycle budget spent) is skipped by the rest, on the device,
   // so no launch waits for the host.
+  template <class St> void levelPrefixes(const SolveShape &c) {
+    k_level_prefix_batch<St><<<1, 1024, 0, stream>>>(
+        reinterpret_cast<const MGArgsT<typename St::T> *>(devArgs.p + c.args), lay.cap,
+        reinterpret_cast<int *>(devArgs.p + lay.levels));
+  }
   template <class St> void correctBatch(const SolveShape &c, int maxCycles) {
     using Args = MGArgsT<typename St::T>;
     launch<Args>(k_dc_start_batch<St>, 256, c.args, lay.slots, wide(), 0);
+    levelPrefixes<St>(c);
     for (int it = 0; it <= maxCycles; it++) {
-      launch<Args>(k_dc_residual_batch<St>, 256, c.args, lay.slots, wide(), 0);
-      launch<Args>(k_dc_norms_batch<St>, 256, c.args, lay.one, perSim(), it);
+      launch<Args>(k_dc_residual_batch<St>, 256, c.args, levelPrefix(0), wide(), 0);
+      launch<Args>(k_dc_norms_batch<St>, 256, c.args, levelPrefix(MAX_LEVELS), perSim(), it);
       if (it == maxCycles) break;
-      launch<Args>(k_dc_scale_batch<St>, 256, c.args, lay.slots, wide(), 0);
+      levelPrefixes<St>(c);  // members that finished drop out
+      launch<Args>(k_dc_scale_batch<St>, 256, c.args, levelPrefix(0), wide(), 0);
       const int flips = vcycle<St>(c, false);
-      launch<Args>(k_dc_update_batch<St>, 256, c.args, lay.slots, wide(), flips << 8);
+      launch<Args>(k_dc_update_batch<St>, 256, c.args, levelPrefix(0), wide(), flips << 8);
     }
   }
 
@@ -1974,8 +1985,10 @@ struct fp_batch {
     launch<MGArgs>(k_eflags_bricks_batch, 256, lay.mg, lay.slots, wide());
     for (int L = shape.brickLevels; L <= shape.bottom; L++)
       launch<MGArgs>(k_eflags_tiles_batch, 64, lay.mg, lay.tiles, wide(), L);
-    if (mode == 0) vcycle<StoreF32>(shape, false);
-    else if (mode == 1) correctBatch<StoreF32>(shape, maxCycles);
+    if (mode == 0) {
+      levelPrefixes<StoreF32>(shape);
+      vcycle<StoreF32>(shape, false);
+    } else if (mode == 1) correctBatch<StoreF32>(shape, maxCycles);
     else if (mode == 2) correctBatch<StoreBF16>(shape, maxCycles);
     else correctBatch<StoreF16>(shape, maxCycles);
     launch<StepArgs>(k_project_batch, 256, lay.post, lay.slots, wide());
