@@ -29,8 +29,10 @@ HBM_PEAK_GBS = 8000.0
 # compulsory traffic of the arrays the stage reads and writes, once each.
 S_F, S_V, D3 = 4, 4, 8
 STAGE_BYTES_PER_FIELD_CELL = {
-    "advect_scalars": 8 * S_F + 3 * S_V / D3,     # fields in, predictor out, velocity
-    "evolve_fields": 13 * S_F + 3 * S_V / D3,     # predictor, fields -> fields, expansion
+    # fields in; predictor and the three backtrace planes out; velocity
+    "advect_scalars": 20 * S_F + 3 * S_V / D3,
+    # predictor, fields and the backtrace planes in; fields and expansion out; velocity
+    "evolve_fields": 25 * S_F + 3 * S_V / D3,
     "advect_velocity": 6 * S_V / D3,
     "correct_velocity": 9 * S_V / D3,
 }

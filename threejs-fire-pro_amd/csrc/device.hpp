@@ -57,6 +57,10 @@ struct Params {
   float knots[3][8][2];
   int emitterCount, forceCount;
   int maccormack;
+  // Step-uniform factors, evaluated once on the host with the kernels' own fexp:
+  // e^{-burnRate dt}, e^{-dissipation dt}, e^{-cooling dt}, the same over dt / 2,
+  // e^{-damping dt}, and the flame's lifetime decrement fdt / lifespan.
+  float burnKeep, lossS, lossH, halfS, halfH, damp, lifeStep;
 };
 
 // Geometry of the grids and pools (constant per simulation).
@@ -161,8 +165,9 @@ __device__ __forceinline__ float length3(float x, float y, float z) { return sqr
 // exp in single precision: Cody & Waite reduction x = k ln2 + r with a two-part ln2,
 // the degree-7 Taylor polynomial of e^r (|r| <= ln2/2) in Horner form, and an exact
 // scaling by 2^k. Relative error below 2 ulp; evaluated with IEEE basic operations only.
-__device__ __forceinline__ float fexp(float x) {
-  if (!(x < 88.7228f)) return x != x ? x : __int_as_float(0x7f800000);
+__host__ __device__ __forceinline__ float bitsFloat(int i) { return __builtin_bit_cast(float, i); }
+__host__ __device__ __forceinline__ float fexp(float x) {
+  if (!(x < 88.7228f)) return x != x ? x : bitsFloat(0x7f800000);
   if (x < -103.972f) return 0.0f;
   float k = floorf(x * 1.44269502f + 0.5f);
   float r = x - k * 0.693145752f;
@@ -180,8 +185,8 @@ __device__ __forceinline__ float fexp(float x) {
     p = p * 2.0f;
     ki -= 1;
   }
-  if (ki > -126) return p * __int_as_float((ki + 127) << 23);
-  return (p * __int_as_float((ki + 64 + 127) << 23)) * __int_as_float((127 - 64) << 23);
+  if (ki > -126) return p * bitsFloat((ki + 127) << 23);
+  return (p * bitsFloat((ki + 64 + 127) << 23)) * bitsFloat((127 - 64) << 23);
 }
 
 // 2^x in single precision for -126 <= x <= 127: k = the nearest integer, r = (x - k) ln2

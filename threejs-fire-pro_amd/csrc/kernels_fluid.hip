@@ -81,13 +81,12 @@ __device__ __forceinline__ void boundsAt(const Ctx &c, const float *V, const int
   }
   lo = 1e5f;
   hi = -1e5f;
-  for (int z = 0; z < 2; z++)
-    for (int y = 0; y < 2; y++)
-      for (int x = 0; x < 2; x++) {
-        float v = velCell(c, V, comp, base[0] + x, base[1] + y, base[2] + z);
-        lo = fminv(lo, v);
-        hi = fmaxv(hi, v);
-      }
+  float cv[8];
+  velCorners(c, V, comp, base[0], base[1], base[2], cv);
+  for (int k = 0; k < 8; k++) {
+    lo = fminv(lo, cv[k]);
+    hi = fmaxv(hi, cv[k]);
+  }
 }
 
 // correctVelocityComponent (fluid-velocity-transport.wgsl:214-255).
@@ -150,10 +149,12 @@ __device__ __forceinline__ float4 predict(const Ctx &c, const float4 *F, const i
   dn.lmn = 1.0f;
   dn.lmx = 0.0f;
   float4 value = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 cv[8];
+  fieldCorners(c, F, base[0], base[1], base[2], cv);
   for (int z = 0; z < 2; z++)
     for (int y = 0; y < 2; y++)
       for (int x = 0; x < 2; x++) {
-        float4 smp = fieldCell(c, F, base[0] + x, base[1] + y, base[2] + z);
+        float4 smp = cv[(z * 2 + y) * 2 + x];
         dn.mn[0] = fminv(dn.mn[0], smp.x);
         dn.mx[0] = fmaxv(dn.mx[0], smp.x);
         dn.mn[1] = fminv(dn.mn[1], smp.y);
@@ -217,24 +218,21 @@ __device__ __forceinline__ void advect_scalars_body(const StepArgs &s, const int
     Donors dn;
     float4 v = predict(c, s.p.F, base, frac, dn);
     *out = make_float4(v.x, v.y, s.P.lifetimeZero ? 0.f : v.z, v.w);
+    float4 *don = s.p.Don + (size_t)slot * nF * 3 + l;
+    don[0] = make_float4(cv[0], cv[1], cv[2], dn.lmn);
+    don[nF] = make_float4(dn.mn[0], dn.mn[1], dn.mn[2], dn.lmx);
+    don[2 * nF] = make_float4(dn.mx[0], dn.mx[1], dn.mx[2], 0.f);
   }
 }
 FP_KERNEL_PAIR1(advect_scalars, 256, StepArgs, int, chunks)
 
 
-// correctedFineScalars (fluid-scalar-transport.wgsl:144-188), with the donor bounds
-// recomputed from the same backward trace instead of cached.
+// correctedFineScalars (fluid-scalar-transport.wgsl:144-188), with the donor bounds that
+// advection left in the backtrace planes (`dn`).
 __device__ __forceinline__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p, const int id[3], int slot, int l,
-                                   const float pos[3], const float cv[3]) {
+                                   const float pos[3], const float cv[3], const Donors &dn) {
   const float4 adv = p.Fp[(size_t)slot * c.L.nF + l];
   if (!P.maccormack) return adv;
-  float dep[3];
-  traceBack(c, P, p.V, pos, cv, dep);
-  int base[3];
-  float frac[3];
-  departureCell(P, id, pos, dep, base, frac);
-  Donors dn;
-  predict(c, p.F, base, frac, dn);
   bool same = dn.mn[0] == dn.mx[0] && dn.mn[1] == dn.mx[1] && dn.mn[2] == dn.mx[2] && dn.lmn == dn.lmx;
   if (same) return make_float4(dn.mn[0], dn.mn[1], clampv(dn.lmn, 0.f, 1.f), dn.mn[2]);
   float fw[3];
@@ -253,10 +251,12 @@ __device__ __forceinline__ float4 correctedScalars(const Ctx &c, const Params &P
       fr[a] = cell - fl;
     }
     float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 cvs[8];
+    fieldCorners(c, p.Fp, bs[0], bs[1], bs[2], cvs);
     for (int z = 0; z < 2; z++)
       for (int y = 0; y < 2; y++)
         for (int x = 0; x < 2; x++) {
-          float4 smp = fieldCell(c, p.Fp, bs[0] + x, bs[1] + y, bs[2] + z);
+          float4 smp = cvs[(z * 2 + y) * 2 + x];
           float wx = x ? fr[0] : 1.0f - fr[0], wy = y ? fr[1] : 1.0f - fr[1], wz = z ? fr[2] : 1.0f - fr[2];
           val.x += smp.x * wx * wy * wz;
           val.y += smp.y * wx * wy * wz;
@@ -369,7 +369,7 @@ __device__ __forceinline__ void evaluateFlame(const Params &P, const Ramps &R, f
     prod[0] = prod[1] = prod[2] = 0.f;
     return;
   }
-  after = fmaxv(0.f, before - P.fdt / P.lifespan);
+  after = fmaxv(0.f, before - P.lifeStep);
   prod[0] = P.fsmoke * P.lifespan * rampIntegral(R, 0, after, before);
   prod[1] = P.fheat * P.lifespan * rampIntegral(R, 1, after, before);
   prod[2] = P.fexpansion * (P.lifespan * rampIntegral(R, 2, after, before) / P.fdt);
@@ -418,10 +418,7 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
   __syncthreads();
   const Ramps ramps{knots, knotCounts};
   uint32_t myBusy = 0u, myExpands = 0u;
-  // Step-uniform factors, evaluated once per lane.
-  const float burnKeep = fexp(-P.burnRate * P.dt);
-  const float lossS = fexp(-P.dissipation * P.dt), lossH = fexp(-P.cooling * P.dt);
-  const float halfS = fexp(-P.dissipation * P.dt * 0.5f), halfH = fexp(-P.cooling * P.dt * 0.5f);
+  const float burnKeep = P.burnKeep, lossS = P.lossS, lossH = P.lossH, halfS = P.halfS, halfH = P.halfH;
   for (int l = chunk * 256 + threadIdx.x; l < nF; l += chunks * 256) {
     int cc[3];
     fieldCellCoords(c, l, cc);
@@ -430,9 +427,27 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     float expansion = 0.f;
     bool injected = false;
     float cv[3];
-    velocityAt(c, P, s.p.V, pos[0], pos[1], pos[2], cv);
+    Donors dn;
+    if (P.maccormack) {
+      // The cell's velocity and donor bounds from advection's backtrace.
+      const float4 *don = s.p.Don + (size_t)slot * nF * 3 + l;
+      const float4 d0 = don[0], d1 = don[nF], d2 = don[2 * nF];
+      cv[0] = d0.x;
+      cv[1] = d0.y;
+      cv[2] = d0.z;
+      dn.lmn = d0.w;
+      dn.mn[0] = d1.x;
+      dn.mn[1] = d1.y;
+      dn.mn[2] = d1.z;
+      dn.lmx = d1.w;
+      dn.mx[0] = d2.x;
+      dn.mx[1] = d2.y;
+      dn.mx[2] = d2.z;
+    } else {
+      velocityAt(c, P, s.p.V, pos[0], pos[1], pos[2], cv);
+    }
     if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
-      float4 tr = correctedScalars(c, P, s.p, cc, slot, l, pos, cv);
+      float4 tr = correctedScalars(c, P, s.p, cc, slot, l, pos, cv, dn);
       float lifetime = tr.z;
       float sp0 = 0.f, sp1 = 0.f, injectedFuel = 0.f;
       for (int i = 0; i < P.emitterCount; i++) {
@@ -708,7 +723,7 @@ __device__ __forceinline__ void forces_body(const StepArgs &s, const int vb) {
   BRICK_PREAMBLE(1)
   const Params &P = s.P;
   const int nV = s.L.nV;
-  const float damp = fexp(-P.damping * P.dt);
+  const float damp = P.damp;
   const float *vc = s.p.Vcor + (size_t)slot * nV * 3;
   const float *fh = cellF + (size_t)slot * nV * 3;
   float *o = s.p.Vf + (size_t)slot * nV * 3;

@@ -407,6 +407,7 @@ struct fp_sim {
   DBuf<uint32_t> brickState, activity, stateTmp, rankTmp, blockSums, blockPrefix, freeSlots;
   // Pools, per slot.
   DBuf<float4> F, Fp, Fn, Curl;
+  DBuf<float4> Don;  // the scalar backtraces of a step (advection to correction)
   const float4 *noise = nullptr;  // the shared curl-noise table
   DBuf<float> E, V, Vadv, Vcor, Vf, D, P0, P1;
   DBuf<uint32_t> solids;
@@ -627,6 +628,7 @@ struct fp_sim {
       HIPCHECK(hipMemsetAsync(solids.p, 0, solids.bytes(), stream));
     }
     Fp.alloc(s * nF);
+    Don.alloc(s * nF * 3);
     Fn.alloc(s * nF);
     Curl.alloc(s * nV);
     Vadv.alloc(s * nV * 3);
@@ -990,6 +992,13 @@ struct fp_sim {
     P.emitterCount = emitterCount;
     P.forceCount = forceCount;
     P.maccormack = maccormack;
+    P.burnKeep = fexp(-P.burnRate * P.dt);
+    P.lossS = fexp(-P.dissipation * P.dt);
+    P.lossH = fexp(-P.cooling * P.dt);
+    P.halfS = fexp(-P.dissipation * P.dt * 0.5f);
+    P.halfH = fexp(-P.cooling * P.dt * 0.5f);
+    P.damp = fexp(-P.damping * P.dt);
+    P.lifeStep = P.fdt / P.lifespan;
   }
 
   AllocArgs allocArgs() {
@@ -1028,6 +1037,7 @@ struct fp_sim {
     p.F = F.p;
     p.Fp = Fp.p;
     p.Fn = Fn.p;
+    p.Don = Don.p;
     p.E = E.p;
     p.V = V.p;
     p.Vadv = Vadv.p;
@@ -1642,7 +1652,7 @@ struct fp_sim {
     size_t total = tileRec.bytes() + hash.bytes() + pages.bytes() + entries.bytes() + records.bytes() +
                    active.bytes() + zeroList.bytes() + brickState.bytes() + activity.bytes() + stateTmp.bytes() +
                    rankTmp.bytes() + blockSums.bytes() + blockPrefix.bytes() + freeSlots.bytes() + F.bytes() +
-                   Fp.bytes() + Fn.bytes() + Curl.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
+                   Fp.bytes() + Fn.bytes() + Don.bytes() + Curl.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
                    Vcor.bytes() + Vf.bytes() + D.bytes() + P0.bytes() + P1.bytes() + solids.bytes() +
                    emitters.bytes() + colliders.bytes() + forces.bytes() + regions.bytes();
     for (int L = 0; L <= bottom; L++)
@@ -1656,7 +1666,7 @@ struct fp_sim {
     for (auto *x : ints) x->release();
     DBuf<uint32_t> *u32s[] = {&brickState, &activity, &stateTmp, &rankTmp, &blockSums, &blockPrefix, &freeSlots, &solids};
     for (auto *x : u32s) x->release();
-    DBuf<float4> *f4s[] = {&F, &Fp, &Fn, &Curl};
+    DBuf<float4> *f4s[] = {&F, &Fp, &Fn, &Don, &Curl};
     for (auto *x : f4s) x->release();
     DBuf<float> *fs[] = {&E, &V, &Vadv, &Vcor, &Vf, &D, &P0, &P1};
     for (auto *x : fs) x->release();

@@ -9,6 +9,10 @@ namespace fp {
 // is three planes per slot (the MAC faces +x, +y, +z of each cell).
 struct Pools {
   float4 *F, *Fp, *Fn;
+  // The scalar backtrace of each field cell, left by advection for the corrector: three
+  // planes of float4 per slot, (velocity at the cell, lifetime min), (smoke, heat, fuel
+  // min, lifetime max), (smoke, heat, fuel max, 0).
+  float4 *Don;
   float *E;
   float *V, *Vadv, *Vcor, *Vf;
   float4 *Curl;
@@ -62,6 +66,55 @@ __device__ __forceinline__ float4 curlCell(const Ctx &c, const float4 *buf, int 
   return buf[(size_t)s * c.L.nV + (((z & m) * c.L.b + (y & m)) * c.L.b + (x & m))];
 }
 
+// The 2x2x2 cells from (x, y, z) on a grid of bricks 2^sh cells a side, corner k = (z * 2 +
+// y) * 2 + x: each corner's pool slot (-1 without one) and its index inside the slot. The
+// bricks are resolved once per axis; when all of them lie in the home brick's 3^3
+// neighborhood (nearly always) the slots come from the LDS record without any hashing.
+__device__ __forceinline__ void cornerCells(const Ctx &c, int sh, int x, int y, int z, int slot[8], int local[8]) {
+  const int n = 1 << sh, m = n - 1;
+  const int p[3] = {x, y, z};
+  const int home[3] = {c.home.bx, c.home.by, c.home.bz};
+  const int mul[3] = {1, 3, 9};
+  const int lmul[3] = {1, n, n * n};
+  int bk[3][2], rel[3][2], loc[3][2];
+  bool in = true;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const int q = p[a] + e;
+      bk[a][e] = q >> sh;
+      const int d = bk[a][e] - home[a] + 1;
+      in = in && (unsigned)d <= 2u;
+      rel[a][e] = d * mul[a];
+      loc[a][e] = (q & m) * lmul[a];
+    }
+#pragma unroll
+  for (int k = 0; k < 8; k++) local[k] = loc[2][k >> 2] + loc[1][(k >> 1) & 1] + loc[0][k & 1];
+  if (in) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) slot[k] = c.home.slot[rel[2][k >> 2] + rel[1][(k >> 1) & 1] + rel[0][k & 1]];
+  } else {
+    for (int k = 0; k < 8; k++) slot[k] = slotAtBrick(c, bk[0][k & 1], bk[1][(k >> 1) & 1], bk[2][k >> 2]);
+  }
+}
+// The eight field cells from (x, y, z) (zero for cells without a slot).
+__device__ __forceinline__ void fieldCorners(const Ctx &c, const float4 *buf, int x, int y, int z, float4 v[8]) {
+  int slot[8], local[8];
+  cornerCells(c, c.L.sF, x, y, z, slot, local);
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    v[k] = slot[k] < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : buf[(size_t)slot[k] * c.L.nF + local[k]];
+}
+// The eight velocity cells of plane `comp` from (x, y, z).
+__device__ __forceinline__ void velCorners(const Ctx &c, const float *buf, int comp, int x, int y, int z, float v[8]) {
+  int slot[8], local[8];
+  cornerCells(c, c.L.sV, x, y, z, slot, local);
+  const float *plane = buf + (size_t)comp * c.L.nV;
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = slot[k] < 0 ? 0.f : plane[(size_t)slot[k] * c.L.nV * 3 + local[k]];
+}
+
 // solidCell / solidAt (fluid-common.wgsl:123-150).
 __device__ __forceinline__ bool solidCell(const Ctx &c, const Params &P, const uint32_t *solids, int x, int y,
                                           int z) {
@@ -88,6 +141,8 @@ __device__ __forceinline__ float4 sampleField(const Ctx &c, const float4 *buf, f
   int bx = (int)flx, by = (int)fly, bz = (int)flz;
   float fx = qx - flx, fy = qy - fly, fz = qz - flz;
   float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 cv[8];
+  fieldCorners(c, buf, bx, by, bz, cv);
 #pragma unroll
   for (int z = 0; z < 2; z++)
 #pragma unroll
@@ -96,7 +151,7 @@ __device__ __forceinline__ float4 sampleField(const Ctx &c, const float4 *buf, f
       for (int x = 0; x < 2; x++) {
         float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
         float w = wx * wy * wz;
-        float4 v = fieldCell(c, buf, bx + x, by + y, bz + z);
+        float4 v = cv[(z * 2 + y) * 2 + x];
         out.x += v.x * w;
         out.y += v.y * w;
         out.z += v.z * w;
@@ -111,6 +166,8 @@ __device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int c
   int bx = (int)flx, by = (int)fly, bz = (int)flz;
   float fx = qx - flx, fy = qy - fly, fz = qz - flz;
   float out = 0.f;
+  float cv[8];
+  velCorners(c, buf, comp, bx, by, bz, cv);
 #pragma unroll
   for (int z = 0; z < 2; z++)
 #pragma unroll
@@ -118,7 +175,7 @@ __device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int c
 #pragma unroll
       for (int x = 0; x < 2; x++) {
         float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
-        out += velCell(c, buf, comp, bx + x, by + y, bz + z) * (wx * wy * wz);
+        out += cv[(z * 2 + y) * 2 + x] * (wx * wy * wz);
       }
   return out;
 }
