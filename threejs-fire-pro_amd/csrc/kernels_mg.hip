@@ -341,18 +341,28 @@ __device__ __forceinline__ float openCount(int flags, int sd) {
 __device__ __forceinline__ float diagonal(float w, int flags) {
   return w * openCount(flags, 0) + w * openCount(flags, 2) + w * openCount(flags, 4);
 }
-// stencil (pressure.wgsl:376-402): the neighbor sum over open sides.
+// stencil (pressure.wgsl:376-402): the neighbor sum over open sides. `a` is the cell's own
+// index in the level array, whose slots hold cubes `sd` cells a side: a neighbor inside the
+// same cube is at a fixed offset from `a`; only neighbors across a face resolve their brick
+// or tile.
 template <class St>
 __device__ __forceinline__ float stencilSum(const MGCtx &c, const typename St::T *pv, const int p[3], int flags, int lv,
-                                            float w) {
+                                            float w, long a, int sd) {
   float n[6];
-  for (int sd = 0; sd < 6; sd++) {
-    if (!((flags >> sd) & 1)) {
-      n[sd] = 0.f;
+  const int m = sd - 1;
+  const int local[3] = {p[0] & m, p[1] & m, p[2] & m};
+  for (int k = 0; k < 6; k++) {
+    if (!((flags >> k) & 1)) {
+      n[k] = 0.f;
       continue;
     }
-    int q[3] = {p[0] + OFF[sd][0], p[1] + OFF[sd][1], p[2] + OFF[sd][2]};
-    n[sd] = readAt<St>(c, pv, q, lv);
+    const int lx = local[0] + OFF[k][0], ly = local[1] + OFF[k][1], lz = local[2] + OFF[k][2];
+    if ((unsigned)lx < (unsigned)sd && (unsigned)ly < (unsigned)sd && (unsigned)lz < (unsigned)sd) {
+      n[k] = St::ld(pv[a + OFF[k][0] + (OFF[k][1] + OFF[k][2] * sd) * sd]);
+    } else {
+      int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
+      n[k] = readAt<St>(c, pv, q, lv);
+    }
   }
   return w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
 }
@@ -367,7 +377,7 @@ __device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelView<typena
     return;
   }
   int flags = e & 63;
-  float sum = stencilSum<St>(c, lv.p, p, flags, level, w);
+  float sum = stencilSum<St>(c, lv.p, p, flags, level, w, (long)a, lv.side);
   float cand = (sum - St::ld(lv.rhs[a])) / fmaxv(diagonal(w, flags), 1e-12f);
   lv.q[a] = St::st(mixv(St::ld(lv.p[a]), cand, 0.8f));
 }
@@ -453,7 +463,7 @@ __device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typen
         if (!unknown) continue;
         if (e & STORED_SOLID) continue;
         int flags = e & 63;
-        float sum = stencilSum<St>(c, fine.p, p, flags, level - 1, m.weight);
+        float sum = stencilSum<St>(c, fine.p, p, flags, level - 1, m.weight, a, fine.side);
         float dg = diagonal(m.weight, flags);
         residual += St::ld(fine.rhs[a]) - (sum - dg * St::ld(fine.p[a]));
       }
@@ -495,25 +505,23 @@ __device__ __forceinline__ void mg_restrict_tiles_body(const MGArgsT<typename St
 FP_MG_PAIR(mg_restrict_tiles, 64)
 
 // coarseValue (pressure.wgsl:670-691): Neumann ghosts beside solids, a mirror below the floor.
-__device__ __forceinline__ bool coarseSolid(const MGCtx &c, const uint8_t *topo, const int q[3], int level) {
-  long a = addrAt(c, q, level);
-  if (a < 0) return false;
-  return (topo[a] & STORED_SOLID) != 0;
-}
+// Each coarse cell is resolved once: its topology byte and value share the address.
 template <class St>
 __device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typename St::T> &m, const LevelView<typename St::T> &cv,
                                              const int p[3], int level) {
   const int groundY = m.P.ground ? 0 : (int)0x80000000;
   int cc[3] = {p[0], p[1] > groundY ? p[1] : groundY, p[2]};
-  if (!coarseSolid(c, cv.topo, cc, level)) return readAt<St>(c, cv.p, cc, level);
+  const long a0 = addrAt(c, cc, level);
+  if (!(a0 >= 0 && (cv.topo[a0] & STORED_SOLID))) return a0 < 0 ? 0.f : St::ld(cv.p[a0]);
   float sum = 0.f, count = 0.f;
   for (int axis = 0; axis < 3; axis++)
     for (int sd = -1; sd <= 1; sd += 2) {
       int q[3] = {cc[0], cc[1], cc[2]};
       q[axis] += sd;
       q[1] = q[1] > groundY ? q[1] : groundY;
-      if (!coarseSolid(c, cv.topo, q, level)) {
-        sum += readAt<St>(c, cv.p, q, level);
+      const long a = addrAt(c, q, level);
+      if (!(a >= 0 && (cv.topo[a] & STORED_SOLID))) {
+        sum += a < 0 ? 0.f : St::ld(cv.p[a]);
         count += 1.0f;
       }
     }
@@ -742,10 +750,19 @@ __device__ __forceinline__ void dc_residual_body(const MGArgsT<typename St::T> &
         int flags = e & 63;
         const double w = (double)m.weight;
         double sum = 0.0;
+        const int msd = sd - 1;
         for (int k = 0; k < 6; k++) {
           if (!((flags >> k) & 1)) continue;
-          int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
-          sum += w * ((double)readAt<StoreF32>(c, x, q, 0) + (double)readAt<StoreF32>(c, m.dcXlo, q, 0));
+          const int lx = (p[0] & msd) + OFF[k][0], ly = (p[1] & msd) + OFF[k][1], lz = (p[2] & msd) + OFF[k][2];
+          double xv;
+          if ((unsigned)lx < (unsigned)sd && (unsigned)ly < (unsigned)sd && (unsigned)lz < (unsigned)sd) {
+            const size_t b = a + OFF[k][0] + (OFF[k][1] + OFF[k][2] * sd) * sd;
+            xv = (double)x[b] + (double)m.dcXlo[b];
+          } else {
+            int q[3] = {p[0] + OFF[k][0], p[1] + OFF[k][1], p[2] + OFF[k][2]};
+            xv = (double)readAt<StoreF32>(c, x, q, 0) + (double)readAt<StoreF32>(c, m.dcXlo, q, 0);
+          }
+          sum += w * xv;
         }
         float bv = m.dcB[a];
         v = (float)((double)bv - (sum - (double)diagonal(m.weight, flags) * ((double)x[a] + (double)m.dcXlo[a])));
