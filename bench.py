@@ -33,8 +33,10 @@ STAGE_BYTES_PER_FIELD_CELL = {
     "advect_scalars": 20 * S_F + 3 * S_V / D3,
     # predictor, fields and the backtrace planes in; fields and expansion out; velocity
     "evolve_fields": 25 * S_F + 3 * S_V / D3,
-    "advect_velocity": 6 * S_V / D3,
-    "correct_velocity": 9 * S_V / D3,
+    # velocity in; advected velocity and the four backtrace planes out
+    "advect_velocity": 22 * S_V / D3,
+    # velocity, advected velocity and the backtrace planes in; corrected velocity out
+    "correct_velocity": 25 * S_V / D3,
 }
 
 

@@ -36,19 +36,22 @@ __device__ __forceinline__ void fieldCellCoords(const Ctx &c, int l, int out[3])
 
 // ---- velocity transport -------------------------------------------------------------
 
-// advectVelocityComponent (fluid-velocity-transport.wgsl:128-141).
+// advectVelocityComponent (fluid-velocity-transport.wgsl:128-141). Also returns the face
+// velocity `fv` and the donor bounds [lo, hi] of the departure cell, which the corrector
+// would otherwise recompute from the same backtrace.
 __device__ __forceinline__ float advectComponent(const Ctx &c, const Params &P, const float *V, const int id[3],
-                                 const float pos[3], int comp, const float orig[3]) {
+                                 const float pos[3], int comp, const float orig[3], float fv[3], float &lo,
+                                 float &hi) {
   float off[3] = {0.f, 0.f, 0.f};
   off[comp] = 0.5f * P.hv;
-  float fp[3], fv[3], mid[3], v[3];
+  float fp[3], mid[3], v[3];
   for (int a = 0; a < 3; a++) fp[a] = pos[a] + off[a];
   velocityOnFace(c, V, id, comp, orig, fv);
   for (int a = 0; a < 3; a++) mid[a] = fp[a] - 0.5f * P.dt * fv[a];
   velocityAt(c, P, V, mid[0], mid[1], mid[2], v);
   float sp[3];
   for (int a = 0; a < 3; a++) sp[a] = (fp[a] - P.dt * v[a] - off[a]) * P.invHv;
-  return sampleVel(c, V, comp, sp[0], sp[1], sp[2]);
+  return sampleVelBounds(c, V, comp, sp[0], sp[1], sp[2], lo, hi);
 }
 
 __device__ __forceinline__ void advect_velocity_body(const StepArgs &s, const int vb) {
@@ -65,44 +68,28 @@ __device__ __forceinline__ void advect_velocity_body(const StepArgs &s, const in
       continue;
     }
     float orig[3] = {vs[l], vs[nV + l], vs[2 * nV + l]};
-    for (int k = 0; k < 3; k++) out[k * nV + l] = advectComponent(c, s.P, s.p.V, cc, pos, k, orig);
+    float4 *tr = s.p.Vtr + (size_t)slot * nV * 4 + l;
+    float hi[3];
+    for (int k = 0; k < 3; k++) {
+      float fv[3], lo;
+      out[k * nV + l] = advectComponent(c, s.P, s.p.V, cc, pos, k, orig, fv, lo, hi[k]);
+      tr[k * nV] = make_float4(fv[0], fv[1], fv[2], lo);
+    }
+    tr[3 * nV] = make_float4(hi[0], hi[1], hi[2], 0.f);
   }
 }
 FP_KERNEL_PAIR(advect_velocity, 256, StepArgs)
 
 
-// Donor bounds (fluid-velocity-transport.wgsl:165-212): the component's range over the
-// 2^3 cells from `base`, or [0, 0] when base's brick has no slot.
-__device__ __forceinline__ void boundsAt(const Ctx &c, const float *V, const int base[3], int comp, float &lo, float &hi) {
-  const int sV = c.L.sV;
-  if (slotAtBrick(c, base[0] >> sV, base[1] >> sV, base[2] >> sV) < 0) {
-    lo = hi = 0.f;
-    return;
-  }
-  lo = 1e5f;
-  hi = -1e5f;
-  float cv[8];
-  velCorners(c, V, comp, base[0], base[1], base[2], cv);
-  for (int k = 0; k < 8; k++) {
-    lo = fminv(lo, cv[k]);
-    hi = fmaxv(hi, cv[k]);
-  }
-}
-
-// correctVelocityComponent (fluid-velocity-transport.wgsl:214-255).
-__device__ __forceinline__ float correctComponent(const Ctx &c, const Params &P, const Pools &p, const int id[3],
-                                  const float pos[3], int comp, const float orig[3], float predicted) {
+// correctVelocityComponent (fluid-velocity-transport.wgsl:214-255), with the face velocity
+// and donor bounds that advection left in the backtrace planes.
+__device__ __forceinline__ float correctComponent(const Ctx &c, const Params &P, const Pools &p, const float pos[3],
+                                  int comp, const float orig[3], float predicted, const float fv[3], float lo,
+                                  float hi) {
   float off[3] = {0.f, 0.f, 0.f};
   off[comp] = 0.5f * P.hv;
-  float fp[3], fv[3], mid[3], v[3];
+  float fp[3];
   for (int a = 0; a < 3; a++) fp[a] = pos[a] + off[a];
-  velocityOnFace(c, p.V, id, comp, orig, fv);
-  for (int a = 0; a < 3; a++) mid[a] = fp[a] - 0.5f * P.dt * fv[a];
-  velocityAt(c, P, p.V, mid[0], mid[1], mid[2], v);
-  int base[3];
-  for (int a = 0; a < 3; a++) base[a] = (int)floorf((fp[a] - P.dt * v[a] - off[a]) * P.invHv - 0.5f);
-  float lo, hi;
-  boundsAt(c, p.V, base, comp, lo, hi);
   if (lo == hi && predicted == lo) return predicted;
   float fmid[3], fw[3], sp[3];
   for (int a = 0; a < 3; a++) fmid[a] = fp[a] + 0.5f * P.dt * fv[a];
@@ -128,7 +115,14 @@ __device__ __forceinline__ void correct_velocity_body(const StepArgs &s, const i
       continue;
     }
     float orig[3] = {vs[l], vs[nV + l], vs[2 * nV + l]};
-    for (int k = 0; k < 3; k++) out[k * nV + l] = correctComponent(c, s.P, s.p, cc, pos, k, orig, as[k * nV + l]);
+    const float4 *tr = s.p.Vtr + (size_t)slot * nV * 4 + l;
+    const float4 hi = tr[3 * nV];
+    const float his[3] = {hi.x, hi.y, hi.z};
+    for (int k = 0; k < 3; k++) {
+      const float4 t = tr[k * nV];
+      const float fv[3] = {t.x, t.y, t.z};
+      out[k * nV + l] = correctComponent(c, s.P, s.p, pos, k, orig, as[k * nV + l], fv, t.w, his[k]);
+    }
   }
 }
 FP_KERNEL_PAIR(correct_velocity, 256, StepArgs)

@@ -408,6 +408,7 @@ struct fp_sim {
   // Pools, per slot.
   DBuf<float4> F, Fp, Fn, Curl;
   DBuf<float4> Don;  // the scalar backtraces of a step (advection to correction)
+  DBuf<float4> Vtr;  // the velocity backtraces of a step
   const float4 *noise = nullptr;  // the shared curl-noise table
   DBuf<float> E, V, Vadv, Vcor, Vf, D, P0, P1;
   DBuf<uint32_t> solids;
@@ -629,6 +630,7 @@ struct fp_sim {
     }
     Fp.alloc(s * nF);
     Don.alloc(s * nF * 3);
+    Vtr.alloc(s * nV * 4);
     Fn.alloc(s * nF);
     Curl.alloc(s * nV);
     Vadv.alloc(s * nV * 3);
@@ -1038,6 +1040,7 @@ struct fp_sim {
     p.Fp = Fp.p;
     p.Fn = Fn.p;
     p.Don = Don.p;
+    p.Vtr = Vtr.p;
     p.E = E.p;
     p.V = V.p;
     p.Vadv = Vadv.p;
@@ -1652,7 +1655,7 @@ struct fp_sim {
     size_t total = tileRec.bytes() + hash.bytes() + pages.bytes() + entries.bytes() + records.bytes() +
                    active.bytes() + zeroList.bytes() + brickState.bytes() + activity.bytes() + stateTmp.bytes() +
                    rankTmp.bytes() + blockSums.bytes() + blockPrefix.bytes() + freeSlots.bytes() + F.bytes() +
-                   Fp.bytes() + Fn.bytes() + Don.bytes() + Curl.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
+                   Fp.bytes() + Fn.bytes() + Don.bytes() + Vtr.bytes() + Curl.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
                    Vcor.bytes() + Vf.bytes() + D.bytes() + P0.bytes() + P1.bytes() + solids.bytes() +
                    emitters.bytes() + colliders.bytes() + forces.bytes() + regions.bytes();
     for (int L = 0; L <= bottom; L++)
@@ -1666,7 +1669,7 @@ struct fp_sim {
     for (auto *x : ints) x->release();
     DBuf<uint32_t> *u32s[] = {&brickState, &activity, &stateTmp, &rankTmp, &blockSums, &blockPrefix, &freeSlots, &solids};
     for (auto *x : u32s) x->release();
-    DBuf<float4> *f4s[] = {&F, &Fp, &Fn, &Don, &Curl};
+    DBuf<float4> *f4s[] = {&F, &Fp, &Fn, &Don, &Vtr, &Curl};
     for (auto *x : f4s) x->release();
     DBuf<float> *fs[] = {&E, &V, &Vadv, &Vcor, &Vf, &D, &P0, &P1};
     for (auto *x : fs) x->release();

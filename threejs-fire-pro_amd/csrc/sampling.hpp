@@ -13,6 +13,10 @@ struct Pools {
   // planes of float4 per slot, (velocity at the cell, lifetime min), (smoke, heat, fuel
   // min, lifetime max), (smoke, heat, fuel max, 0).
   float4 *Don;
+  // The velocity backtrace of each velocity cell, left by advection for the corrector: four
+  // planes of float4 per slot, (face velocity, donor min) of components x, y, z, then the
+  // three donor maxima.
+  float4 *Vtr;
   float *E;
   float *V, *Vadv, *Vcor, *Vf;
   float4 *Curl;
@@ -177,6 +181,43 @@ __device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int c
         float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
         out += cv[(z * 2 + y) * 2 + x] * (wx * wy * wz);
       }
+  return out;
+}
+// sampleVel, also returning the range of the eight cells it weighs: [0, 0] when the first
+// cell's brick has no slot (the donor bounds of fluid-velocity-transport.wgsl:165-212).
+__device__ __forceinline__ float sampleVelBounds(const Ctx &c, const float *buf, int comp, float px, float py,
+                                                 float pz, float &lo, float &hi) {
+  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
+  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+  int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+  int slot[8], local[8];
+  cornerCells(c, c.L.sV, bx, by, bz, slot, local);
+  const float *plane = buf + (size_t)comp * c.L.nV;
+  float cv[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) cv[k] = slot[k] < 0 ? 0.f : plane[(size_t)slot[k] * c.L.nV * 3 + local[k]];
+  float out = 0.f;
+#pragma unroll
+  for (int z = 0; z < 2; z++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
+        out += cv[(z * 2 + y) * 2 + x] * (wx * wy * wz);
+      }
+  if (slot[0] < 0) {
+    lo = hi = 0.f;
+  } else {
+    lo = 1e5f;
+    hi = -1e5f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      lo = fminv(lo, cv[k]);
+      hi = fmaxv(hi, cv[k]);
+    }
+  }
   return out;
 }
 // velocityAt (fluid-advection.wgsl:10-21).

@@ -32,11 +32,11 @@ DIVISOR = 2
 
 def memory_estimate(target_bricks: int, brick: int = BRICK, divisor: int = DIVISOR) -> int:
     """Device bytes a simulation holds at its voxel budget: per pool slot its field,
-    backtrace, expansion, velocity, curl, divergence, pressure and multigrid arrays and
+    backtraces, expansion, velocity, curl, divergence, pressure and multigrid arrays and
     defect correction's x (two parts), r and right-hand side with 16-bit level copies, plus
     the tile tables (a tile per 16 bricks, generously)."""
     nf, nv = brick ** 3, (brick // divisor) ** 3
-    per_slot = (nf * (16 * 6 + 4) + nf // 8 + nv * (4 * 3 * 4 + 16 + 4 + 8) + int(nv * 14 * 1.15)
+    per_slot = (nf * (16 * 6 + 4) + nf // 8 + nv * (4 * 3 * 4 + 16 + 4 + 8 + 64) + int(nv * 14 * 1.15)
                 + nv * 4 * 4 + int(nv * 6 * 1.15))
     tiles = max(16, target_bricks // 16)
     per_tile = 64 * (4 * 3 + 28 * 4 + 4 * 3) + 32 * 4 + 64 * 14
