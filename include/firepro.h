@@ -233,6 +233,18 @@ void *fp_sim_stream(fp_sim *sim);
 int fp_sim_profile(fp_sim *sim, int32_t enable);
 int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *count);
 
+/* Scene lights (SceneLights.sample, src/library/SceneLights.ts:127-218, and collectLights,
+ * src/engine/shaders/scene-lights.wgsl:36-106): where the flames of the latest step light
+ * the scene from. Every active brick samples its fields on a 4^3 lattice, weighs the
+ * samples by the luminance of their emission (the render settings' flame and smoke
+ * parameters) and adds them to the anchor nearest the brick's center, height above an
+ * anchor counting a quarter (ties to the lower index). anchors: count world positions
+ * (x, y, z). sums: count x 8 floats: the luminance, the luminance-weighted offset (x, y, z)
+ * from the anchor, r, g, b, 0, summed in a fixed order. All zero before the first step or
+ * without render boxes. */
+int fp_sim_scene_lights(fp_sim *sim, const fp_render_settings *settings, const double *anchors, int32_t count,
+                        float *sums);
+
 /* Batched stepping (the serving form; no counterpart in the reference, whose simulations
  * step one at a time). fp_batch_step advances `count` simulations, all of one brick size,
  * velocity divisor and pressure policy (each with its own tolerance and cycle budget), by

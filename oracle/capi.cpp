@@ -33,6 +33,7 @@ struct Base {
   virtual int render(const fp_camera &cam, const fp_render_settings &rs, float *rgba) = 0;
   virtual int readRender(fp_render_tables *t) = 0;
   virtual int probe(const fp_camera &cam, const fp_render_settings &rs, int what, const float *in, float *out) = 0;
+  virtual int sceneLights(const fp_render_settings &rs, const double *xyz, int count, float *sums) = 0;
 };
 
 template <class R> struct Impl : Base {
@@ -235,6 +236,23 @@ template <class R> struct Impl : Base {
     }
     return -1;
   }
+  int sceneLights(const fp_render_settings &rs, const double *xyz, int count, float *sums) override {
+    checkRender(rs);
+    if constexpr (std::is_same<R, float>::value) {
+      fp_camera cam{};
+      cam.width = cam.height = 1;
+      auto p = oracle::renderParams(sim.voxel, sim.B, cam, rs, sim.time);
+      std::vector<std::array<float, 3>> anchors(count);
+      for (int i = 0; i < count; i++)
+        for (int a = 0; a < 3; a++) anchors[i][a] = (float)xyz[3 * i + a];
+      if (sim.steps == 0) {
+        std::fill(sums, sums + (size_t)8 * count, 0.0f);
+        return 0;
+      }
+      oracle::sceneLights(sim, cache.view(sim, p), anchors, sums);
+    }
+    return 0;
+  }
   // Single functions of the render restatement on the latest prepared state, for tests:
   // 0 reconstruct(cell xyz, mode) in box 0; 1 incidentLight(world xyz) in box 0;
   // 2 the jitter of pixel (x, y) at time t; 3 fexp2(x).
@@ -340,6 +358,9 @@ int orc_render(void *h, const fp_camera *cam, const fp_render_settings *rs, floa
   return guard([&] { ((Base *)h)->render(*cam, *rs, rgba); });
 }
 int orc_read_render(void *h, fp_render_tables *t) { return ((Base *)h)->readRender(t); }
+int orc_scene_lights(void *h, const fp_render_settings *rs, const double *xyz, int count, float *sums) {
+  return guard([&] { ((Base *)h)->sceneLights(*rs, xyz, count, sums); });
+}
 int orc_render_probe(void *h, const fp_camera *cam, const fp_render_settings *rs, int what, const float *in,
                      float *out) {
   int r = -1;

@@ -60,6 +60,7 @@ def load():
         "orc_brick_coords": (C.c_int, [v, P(C.c_int32), C.c_int, P(C.c_int32)]),
         "orc_rhs": (C.c_int, [v, P(C.c_double), C.c_int]),
         "orc_render": (C.c_int, [v, P(abi.Camera), P(abi.RenderSettings), P(C.c_float)]),
+        "orc_scene_lights": (C.c_int, [v, P(abi.RenderSettings), P(C.c_double), C.c_int, P(C.c_float)]),
         "orc_read_render": (C.c_int, [v, P(abi.RenderTables)]),
         "orc_render_probe": (C.c_int, [v, P(abi.Camera), P(abi.RenderSettings), C.c_int,
                                        P(C.c_float), P(C.c_float)]),
@@ -224,6 +225,15 @@ class OracleSimulation:
                                     img.ctypes.data_as(C.POINTER(C.c_float)) if image else None), self._lib)
         self._render = (camera, rs)
         return img
+
+    def scene_lights(self, anchors, settings=None) -> np.ndarray:
+        """Per anchor the 8 totals of SceneLights (luminance, weighted offset, rgb, 0)."""
+        rs = settings if settings is not None else abi.render_settings()
+        a = np.ascontiguousarray(np.asarray(anchors, np.float64).reshape(-1, 3))
+        out = np.zeros((len(a), 8), np.float32)
+        _check(self._lib.orc_scene_lights(self._h, C.byref(rs), a.ctypes.data_as(C.POINTER(C.c_double)), len(a),
+                                          out.ctypes.data_as(C.POINTER(C.c_float))), self._lib)
+        return out
 
     def read_render(self) -> dict:
         def call(t):

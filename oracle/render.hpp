@@ -797,6 +797,113 @@ struct Renderer {
   }
 };
 
+// ---- scene lights (scene-lights.wgsl:36-106; SceneLights.ts:127-218) ----------------------
+// Every active brick samples its fields on a 4^3 lattice, weighs the samples by the
+// luminance of their emission and gives the totals to the anchor nearest the brick's
+// center (height above an anchor counts a quarter; ties to the lower index). The totals
+// per anchor are summed in a fixed order: a 64-lane pairwise tree per brick, a 256-lane tree
+// per block of 256 consecutive active bricks (others' bricks adding zero), then per lane
+// the blocks j, j + 256, ... in order and a 256-lane tree. The reference adds with float
+// atomics in scheduling order; this order is the library's.
+inline void treeSum(std::vector<float> &v) {
+  for (size_t reach = v.size() / 2; reach > 0; reach >>= 1)
+    for (size_t lane = 0; lane < reach; lane++) v[lane] += v[lane + reach];
+}
+inline void sceneLights(const Sim<float> &sim, const Renderer &r, const std::vector<std::array<float, 3>> &anchors,
+                        float *sums) {
+  const int count = (int)anchors.size();
+  std::fill(sums, sums + (size_t)8 * count, 0.0f);
+  const int n = (int)sim.activeList.size();
+  if (count == 0 || n == 0 || sim.renderBoxes.empty()) return;
+  const RenderParams &P = r.P;
+  const int B = sim.B, stride = B >> 2;
+  std::vector<float> partials((size_t)n * 8, 0.0f);
+  std::vector<int> brickAnchor(n, -1);
+  for (int bi = 0; bi < n; bi++) {
+    const int id = sim.activeList[bi];
+    const int page = sim.pages[id];
+    if (page < 0) continue;
+    const auto &tc = sim.dir.snapshot[id >> 6];
+    const int l = id & 63;
+    const int brick[3] = {tc[0] * 4 + (l & 3), tc[1] * 4 + ((l >> 2) & 3), tc[2] * 4 + (l >> 4)};
+    const float center[3] = {((float)brick[0] + 0.5f) * P.brickSize, ((float)brick[1] + 0.5f) * P.brickSize,
+                             ((float)brick[2] + 0.5f) * P.brickSize};
+    float nearD[64];
+    int nearI[64];
+    float lum[64], rgbs[64][3];
+    int locals[64][3];
+    for (int lane = 0; lane < 64; lane++) {
+      const int local[3] = {(lane & 3) * stride + stride / 2, ((lane >> 2) & 3) * stride + stride / 2,
+                            (lane >> 4) * stride + stride / 2};
+      for (int a = 0; a < 3; a++) locals[lane][a] = local[a];
+      const float *fp = &sim.F[((size_t)sim.pageSlot(page) * sim.nF + ((local[2] * B + local[1]) * B + local[0])) * 4];
+      const Vec4 f{fp[0], fp[1], fp[2], fp[3]};
+      r.emission(f, rgbs[lane]);
+      lum[lane] = rgbs[lane][0] * 0.2126f + rgbs[lane][1] * 0.7152f + rgbs[lane][2] * 0.0722f;
+      float bestD = 1e38f;
+      int bestI = 0;
+      for (int i = lane; i < count; i += 64) {
+        const float dx = center[0] - anchors[i][0], dy = center[1] - anchors[i][1], dz = center[2] - anchors[i][2];
+        const float rise = dy > 0.0f ? dy * 0.25f : dy;
+        const float d = dx * dx + dz * dz + rise * rise;
+        if (d < bestD) {
+          bestD = d;
+          bestI = i;
+        }
+      }
+      nearD[lane] = bestD;
+      nearI[lane] = bestI;
+    }
+    for (int reach = 32; reach > 0; reach >>= 1)
+      for (int lane = 0; lane < reach; lane++) {
+        const float od = nearD[lane + reach];
+        const int oi = nearI[lane + reach];
+        if (od < nearD[lane] || (od == nearD[lane] && oi < nearI[lane])) {
+          nearD[lane] = od;
+          nearI[lane] = oi;
+        }
+      }
+    const int anchor = nearI[0];
+    std::vector<float> tot[7];
+    for (auto &t : tot) t.resize(64);
+    for (int lane = 0; lane < 64; lane++) {
+      float p[3];
+      for (int a = 0; a < 3; a++) p[a] = ((float)(brick[a] * B + locals[lane][a]) + 0.5f) * P.voxel;
+      tot[0][lane] = lum[lane];
+      for (int a = 0; a < 3; a++) tot[1 + a][lane] = (p[a] - anchors[anchor][a]) * lum[lane];
+      for (int a = 0; a < 3; a++) tot[4 + a][lane] = rgbs[lane][a];
+    }
+    for (int k = 0; k < 7; k++) {
+      treeSum(tot[k]);
+      partials[(size_t)bi * 8 + k] = tot[k][0];
+    }
+    brickAnchor[bi] = anchor;
+  }
+  const int BLOCK = 256;
+  const int blocks = (n + BLOCK - 1) / BLOCK;
+  std::vector<float> blockSums((size_t)blocks * count * 8, 0.0f), red(BLOCK);
+  for (int b = 0; b < blocks; b++)
+    for (int a = 0; a < count; a++)
+      for (int k = 0; k < 7; k++) {
+        for (int lane = 0; lane < BLOCK; lane++) {
+          const int bi = b * BLOCK + lane;
+          red[lane] = bi < n && brickAnchor[bi] == a ? partials[(size_t)bi * 8 + k] : 0.0f;
+        }
+        treeSum(red);
+        blockSums[((size_t)b * count + a) * 8 + k] = red[0];
+      }
+  for (int a = 0; a < count; a++)
+    for (int k = 0; k < 7; k++) {
+      for (int lane = 0; lane < BLOCK; lane++) {
+        float acc = 0.0f;
+        for (int b = lane; b < blocks; b += BLOCK) acc += blockSums[((size_t)b * count + a) * 8 + k];
+        red[lane] = acc;
+      }
+      treeSum(red);
+      sums[(size_t)a * 8 + k] = red[0];
+    }
+}
+
 // VolumeLighting.encode's schedule (VolumeLighting.ts:249-375), as state carried between
 // renders of one simulation.
 struct RenderCache {

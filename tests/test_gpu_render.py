@@ -193,3 +193,26 @@ def test_render_rejects_bad_settings():
         assert not img.any()
     finally:
         sim.dispose()
+
+
+def test_scene_lights_bit_exact():
+    """Scene-light totals (fp_sim_scene_lights) equal the restatement's, bit for bit: the
+    campfire after 40 steps with an anchor at the fire, one above it and one far away, then
+    the small scene's bursts with anchors at its sources."""
+    from firepro import scenes
+    abi = _abi()
+    rs = abi.render_settings()
+    for sc, anchors in [(scenes.campfire(steps=40), [(0.0, 0.2, 0.0), (0.0, 2.5, 0.0), (6.0, 0.0, 6.0)]),
+                        (scenes.small(steps=8, brick=8, divisor=2), [(0.0, 0.4, 0.0), (0.6, 0.4, -0.3)])]:
+        gpu, cpu = make_engines(sc)
+        try:
+            for frame, g in sc.frames():
+                apply(gpu, cpu, g)
+                gpu.step(frame)
+                cpu.step(frame)
+            print(f"case scene lights {len(anchors)} anchors", flush=True)
+            a, b = gpu.scene_lights(anchors, rs), cpu.scene_lights(anchors, rs)
+            assert np.array_equal(a, b), (a, b)
+            assert a[:, 0].max() > 0, "flames are lit"
+        finally:
+            gpu.dispose()

@@ -269,3 +269,85 @@ def test_incident_light_bounds():
     # Without a sun, no shadowing at all.
     sim.render(cam, abi.render_settings(fire_intensity=0.0, sun_intensity=0.0), image=False)
     assert np.all(sim.read_render()["light"][..., 3] == 1.0)
+
+
+def _lit_campfire(steps=24):
+    from firepro import scenes
+    from oracle import OracleSimulation
+    sc = scenes.campfire(steps=steps)
+    sim = OracleSimulation.create(sc.grid(), 32)
+    for frame, g in sc.frames():
+        sim.apply_grid(g)
+        sim.step(frame)
+    return sim
+
+
+def test_scene_lights_invariants():
+    """Scene-light totals (the restatement of collectLights): the flames' total luminance
+    and their centroid do not depend on where the anchors stand; a far anchor and the second
+    of two coincident anchors get nothing; height above an anchor counts a quarter; the
+    totals scale exactly with the flame brightness; nothing is lit before the first step."""
+    from firepro import abi, scenes
+    from oracle import OracleSimulation
+    rs = abi.render_settings()
+    fresh = OracleSimulation.create(scenes.campfire(steps=1).grid(), 32)
+    assert not fresh.scene_lights([(0, 0.2, 0)], rs).any()
+    sim = _lit_campfire()
+    one = sim.scene_lights([(0.0, 0.2, 0.0)], rs)
+    total = float(one[0, 0])
+    assert total > 0
+    moved = sim.scene_lights([(0.3, 1.0, -0.2)], rs)
+    np.testing.assert_allclose(moved[0, 0], total, rtol=1e-6)
+    np.testing.assert_allclose(moved[0, 4:7], one[0, 4:7], rtol=1e-6)
+    # The centroid anchor + offset / luminance is the same from both anchors.
+    c1 = np.array([0.0, 0.2, 0.0]) + one[0, 1:4] / one[0, 0]
+    c2 = np.array([0.3, 1.0, -0.2]) + moved[0, 1:4] / moved[0, 0]
+    np.testing.assert_allclose(c1, c2, atol=1e-5)
+    split = sim.scene_lights([(0.0, 0.2, 0.0), (40.0, 0.0, 40.0), (0.0, 0.2, 0.0)], rs)
+    assert not split[1].any() and not split[2].any(), "far and duplicate anchors get nothing"
+    np.testing.assert_array_equal(split[0], one[0])
+    # Flames rise from their anchor: height above an anchor counts a quarter. The lit bricks
+    # are the bottom layer (centers at y = 0.4 m). An anchor at y = -1 m is 1.4 m below them,
+    # 0.35 m by the rule; one at y = 1.2 m is 0.8 m above them either way. Plain distance
+    # would give them to the upper anchor; the rule gives them to the lower.
+    below_above = sim.scene_lights([(0.0, -1.0, 0.0), (0.0, 1.2, 0.0)], rs)
+    np.testing.assert_allclose(below_above[0, 0], total, rtol=1e-6)
+    assert below_above[1, 0] == 0
+    # Brightness doubles every sample's emission: the totals double exactly.
+    brighter = sim.scene_lights([(0.0, 0.2, 0.0)], abi.render_settings(fire_intensity=2 * rs.fire_intensity))
+    np.testing.assert_array_equal(brighter[0, [0, 4, 5, 6]], 2 * one[0, [0, 4, 5, 6]])
+
+
+def test_scene_light_placement():
+    """SceneLights' host part: a light at anchor + offset / luminance, coloured by the
+    normalized rgb, as bright as the luminance times the sample cube's volume (capped at
+    400), reaching to where it falls to 0.01; an explosion's light finishes once it has been
+    read ten times and is below 1 % of its peak; without clustered lighting the four lights go to the
+    anchors brightest at the viewer."""
+    from firepro.lights import LightAnchor, SceneLights
+    sl = SceneLights()
+    anchors = [LightAnchor("a", (0, 0, 0)), LightAnchor("b", (10, 0, 0), transient=True)]
+    sums = np.zeros((2, 8), np.float32)
+    sums[0, :7] = [8.0, 8.0, 16.0, 0.0, 2.0, 1.0, 0.5]
+    sums[1, :7] = [40000.0, 0, 0, 0, 1, 1, 1]
+    sl.apply(sums, anchors, 1.0, brick_cells=16, voxel_size=0.05)
+    a, b = sl.measured["a"], sl.measured["b"]
+    assert a.position == [1.0, 2.0, 0.0] and a.color == [1.0, 0.5, 0.25]
+    sample = 16 / 4 * 0.05
+    assert abs(a.intensity - 8.0 * sample ** 3 * 2) < 1e-12
+    assert b.intensity == 400.0
+    lights = sl.lights
+    assert len(lights) == 2 and abs(lights[0].distance - (lights[0].intensity / 0.01) ** 0.5) < 1e-12
+    # Viewer at the far anchor: it lights the viewer most, so it comes first.
+    sl.viewer = [10, 0, 0]
+    sl.place()
+    assert sl.lights[0].position == [10.0, 0.0, 0.0]
+    faded = sums.copy()
+    faded[1, 0] = 1.0
+    for _ in range(8):  # readings 2 to 9
+        sl.apply(faded, anchors, 1.0, 16, 0.05)
+    assert not sl.finished("b")
+    sl.apply(faded, anchors, 1.0, 16, 0.05)
+    assert sl.finished("b") and not sl.finished("a")
+    sl.apply(sums[:1], anchors[:1], 1.0, 16, 0.05)
+    assert "b" not in sl.measured

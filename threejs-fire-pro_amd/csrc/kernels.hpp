@@ -238,6 +238,14 @@ __global__ void k_smooth_smoke(RenderArgs r, int chunks, int staged);
 __global__ void k_occupancy(RenderArgs r);
 __global__ void k_light(RenderArgs r, int fresh);
 __global__ void k_march(RenderArgs r, float4 *out);
+// Scene lights: per-brick totals by nearest anchor, then per-anchor totals in a fixed order
+// (blocks of LIGHT_BLOCK bricks).
+constexpr int LIGHT_BLOCK = 256;
+__global__ void k_light_bricks(RenderArgs r, const int *tileRec, const float4 *anchors, int count, float *partials,
+                               int *brickAnchor);
+__global__ void k_light_blocks(const int *counts, const float *partials, const int *brickAnchor, int count,
+                               float *blockSums);
+__global__ void k_light_total(const int *counts, const float *blockSums, int count, float *sums);
 
 // Loads the home brick's record into LDS; every lane of the block calls it.
 __device__ __forceinline__ void loadHome(const Dir &dir, int id, int *rec, Home &h, int side) {

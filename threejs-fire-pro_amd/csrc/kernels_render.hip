@@ -644,4 +644,151 @@ __global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out) {
   out[(size_t)py * P.width + px] = result;
 }
 
+
+// ---- scene lights (scene-lights.wgsl:36-106, SceneLights.ts:127-218) ----------------------
+// Flames light the scene from anchors (emitters, burning explosions). Every active brick
+// samples its fields on a 4^3 lattice (cell centers B/8, 3B/8, ... of each axis), weighs
+// each sample by the luminance of its emission, and gives its totals to the anchor nearest
+// the brick's center, where height above an anchor counts a quarter.
+
+// anchorDistance (scene-lights.wgsl:19-23).
+__device__ __forceinline__ float anchorDistance(const float c[3], const float4 &a) {
+  const float dx = c[0] - a.x, dy = c[1] - a.y, dz = c[2] - a.z;
+  const float rise = dy > 0.0f ? dy * 0.25f : dy;
+  return dx * dx + dz * dz + rise * rise;
+}
+
+// One 64-lane workgroup per active brick: its nearest anchor (`brickAnchor`, -1 for a
+// brick without a page) and its seven totals: luminance, luminance-weighted offset from the
+// anchor, rgb, each summed over the lanes by a pairwise tree.
+__global__ void __launch_bounds__(64) k_light_bricks(RenderArgs r, const int *tileRec, const float4 *anchors, int count,
+                                                     float *partials, int *brickAnchor) {
+  const int bi = (int)blockIdx.x;
+  if (bi >= r.counts[0]) return;
+  const int lane = (int)threadIdx.x;
+  const int id = r.active[bi];
+  const int page = r.pages[id];
+  if (page < 0) {
+    if (lane == 0) brickAnchor[bi] = -1;
+    return;
+  }
+  __shared__ float nearD[64];
+  __shared__ int nearI[64];
+  __shared__ float4 pos[64];
+  __shared__ float3 col[64];
+  const RenderParams &P = r.P;
+  const int *tr = tileRec + (size_t)(id >> 6) * TILE_RECORD_WORDS;
+  const int l = id & 63;
+  const int brick[3] = {tr[0] * 4 + (l & 3), tr[1] * 4 + ((l >> 2) & 3), tr[2] * 4 + (l >> 4)};
+  const int stride = r.B >> 2;
+  const int local[3] = {(lane & 3) * stride + stride / 2, ((lane >> 2) & 3) * stride + stride / 2,
+                        (lane >> 4) * stride + stride / 2};
+  const float4 f = r.F[(size_t)pageToSlot(page, r.side) * r.nF + ((local[2] * r.B + local[1]) * r.B + local[0])];
+  float rgb[3];
+  emission(r, f, rgb);
+  const float luminance = rgb[0] * 0.2126f + rgb[1] * 0.7152f + rgb[2] * 0.0722f;
+  const float center[3] = {((float)brick[0] + 0.5f) * P.brickSize, ((float)brick[1] + 0.5f) * P.brickSize,
+                           ((float)brick[2] + 0.5f) * P.brickSize};
+  float bestD = 1e38f;
+  int bestI = 0;
+  for (int i = lane; i < count; i += 64) {
+    const float d = anchorDistance(center, anchors[i]);
+    if (d < bestD) {
+      bestD = d;
+      bestI = i;
+    }
+  }
+  nearD[lane] = bestD;
+  nearI[lane] = bestI;
+  __syncthreads();
+  for (int reach = 32; reach > 0; reach >>= 1) {
+    if (lane < reach) {
+      const float od = nearD[lane + reach];
+      const int oi = nearI[lane + reach];
+      if (od < nearD[lane] || (od == nearD[lane] && oi < nearI[lane])) {
+        nearD[lane] = od;
+        nearI[lane] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const int anchor = nearI[0];
+  const float4 a = anchors[anchor];
+  const float p[3] = {((float)(brick[0] * r.B + local[0]) + 0.5f) * P.voxel,
+                      ((float)(brick[1] * r.B + local[1]) + 0.5f) * P.voxel,
+                      ((float)(brick[2] * r.B + local[2]) + 0.5f) * P.voxel};
+  pos[lane] = make_float4(luminance, (p[0] - a.x) * luminance, (p[1] - a.y) * luminance, (p[2] - a.z) * luminance);
+  col[lane] = make_float3(rgb[0], rgb[1], rgb[2]);
+  __syncthreads();
+  for (int reach = 32; reach > 0; reach >>= 1) {
+    if (lane < reach) {
+      const float4 o = pos[lane + reach];
+      const float3 oc = col[lane + reach];
+      pos[lane] = make_float4(pos[lane].x + o.x, pos[lane].y + o.y, pos[lane].z + o.z, pos[lane].w + o.w);
+      col[lane] = make_float3(col[lane].x + oc.x, col[lane].y + oc.y, col[lane].z + oc.z);
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    float *out = partials + (size_t)bi * 8;
+    out[0] = pos[0].x;
+    out[1] = pos[0].y;
+    out[2] = pos[0].z;
+    out[3] = pos[0].w;
+    out[4] = col[0].x;
+    out[5] = col[0].y;
+    out[6] = col[0].z;
+    out[7] = 0.0f;
+    brickAnchor[bi] = anchor;
+  }
+}
+
+// Per anchor, the totals of LIGHT_BLOCK consecutive bricks by a pairwise tree (bricks of
+// other anchors add zero): blockSums[(block * count + anchor) * 8 + k]. The order is fixed,
+// so the result does not depend on scheduling (the reference adds with float atomics).
+__global__ void __launch_bounds__(LIGHT_BLOCK) k_light_blocks(const int *counts, const float *partials,
+                                                              const int *brickAnchor, int count, float *blockSums) {
+  const int n = counts[0];
+  const int block = (int)blockIdx.x;
+  if (block * LIGHT_BLOCK >= n && !(n == 0 && block == 0)) return;
+  __shared__ float red[LIGHT_BLOCK];
+  const int lane = (int)threadIdx.x, bi = block * LIGHT_BLOCK + lane;
+  const int mine = bi < n ? brickAnchor[bi] : -1;
+  for (int a = 0; a < count; a++)
+    for (int k = 0; k < 7; k++) {
+      red[lane] = mine == a ? partials[(size_t)bi * 8 + k] : 0.0f;
+      __syncthreads();
+      for (int reach = LIGHT_BLOCK / 2; reach > 0; reach >>= 1) {
+        if (lane < reach) red[lane] += red[lane + reach];
+        __syncthreads();
+      }
+      if (lane == 0) blockSums[((size_t)block * count + a) * 8 + k] = red[0];
+      __syncthreads();
+    }
+}
+
+// The anchors' totals over the blocks: lane j adds blocks j, j + 256, ... in order, then a
+// pairwise tree over the lanes. sums[anchor * 8 + k], the eighth float zero.
+__global__ void __launch_bounds__(LIGHT_BLOCK) k_light_total(const int *counts, const float *blockSums, int count,
+                                                             float *sums) {
+  __shared__ float red[LIGHT_BLOCK];
+  const int lane = (int)threadIdx.x;
+  const int blocks = counts[0] > 0 ? (counts[0] + LIGHT_BLOCK - 1) / LIGHT_BLOCK : 0;
+  for (int a = 0; a < count; a++) {
+    for (int k = 0; k < 7; k++) {
+      float acc = 0.0f;
+      for (int b = lane; b < blocks; b += LIGHT_BLOCK) acc += blockSums[((size_t)b * count + a) * 8 + k];
+      red[lane] = acc;
+      __syncthreads();
+      for (int reach = LIGHT_BLOCK / 2; reach > 0; reach >>= 1) {
+        if (lane < reach) red[lane] += red[lane + reach];
+        __syncthreads();
+      }
+      if (lane == 0) sums[(size_t)a * 8 + k] = red[0];
+      __syncthreads();
+    }
+    if (lane == 0) sums[(size_t)a * 8 + 7] = 0.0f;
+  }
+}
+
 }  // namespace fp

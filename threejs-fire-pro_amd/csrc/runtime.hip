@@ -450,6 +450,10 @@ struct fp_sim {
     DBuf<int> table;
     DBuf<float4> R, L, lut, image;
     DBuf<uint8_t> O;
+    // Scene lights: anchors, per-brick totals and anchors, per-block totals, the totals.
+    DBuf<float4> anchors;
+    DBuf<float> partials, blockSums, sums;
+    DBuf<int> brickAnchor;
   } rc;
   // Scene.
   DBuf<Emitter> emitters;
@@ -1596,6 +1600,45 @@ struct fp_sim {
     }
   }
 
+  // SceneLights.sample (SceneLights.ts:127-218) over the latest step's fields: per anchor,
+  // the luminance of the flames nearest it, their luminance-weighted offset and their rgb.
+  void sceneLights(const fp_render_settings &s, const double *xyz, int count, float *out) {
+    if (count < 0 || count > 65536) throw InvalidArgument("anchor count must be 0 to 65536.");
+    for (int i = 0; i < 3 * count; i++)
+      if (!std::isfinite(xyz[i])) throw InvalidArgument("anchor positions must be finite.");
+    fp_camera cam{};
+    cam.width = cam.height = 1;
+    RenderParams P = renderParams(cam, s);
+    std::fill(out, out + (size_t)8 * count, 0.0f);
+    const int n = (int)std::min<int64_t>(activeBricks, (int64_t)brickCount());
+    if (count == 0 || rboxes.boxes.empty() || n == 0 || !stepped) return;
+    if (!rc.lut.p) {
+      std::vector<float> t = render::blackbody();
+      rc.lut.alloc(t.size() / 4);
+      HIPCHECK(hipMemcpyAsync(rc.lut.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    }
+    std::vector<float4> a(count);
+    for (int i = 0; i < count; i++) a[i] = make_float4((float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2], 0.f);
+    const int blocks = (n + LIGHT_BLOCK - 1) / LIGHT_BLOCK;
+    if (rc.anchors.n < (size_t)count) rc.anchors.alloc(count);
+    if (rc.partials.n < (size_t)n * 8) rc.partials.alloc((size_t)n * 8);
+    if (rc.brickAnchor.n < (size_t)n) rc.brickAnchor.alloc(n);
+    if (rc.blockSums.n < (size_t)blocks * count * 8) rc.blockSums.alloc((size_t)blocks * count * 8);
+    if (rc.sums.n < (size_t)count * 8) rc.sums.alloc((size_t)count * 8);
+    HIPCHECK(hipMemcpyAsync(rc.anchors.p, a.data(), count * sizeof(float4), hipMemcpyHostToDevice, stream));
+    RenderArgs r = renderArgs(P);
+    int t = begin();
+    k_light_bricks<<<(unsigned)n, 64, 0, stream>>>(r, tileRec.p, rc.anchors.p, count, rc.partials.p, rc.brickAnchor.p);
+    k_light_blocks<<<(unsigned)blocks, LIGHT_BLOCK, 0, stream>>>(counts.p, rc.partials.p, rc.brickAnchor.p, count,
+                                                                 rc.blockSums.p);
+    k_light_total<<<1, LIGHT_BLOCK, 0, stream>>>(counts.p, rc.blockSums.p, count, rc.sums.p);
+    end(t, "scene_lights");
+    HIPCHECK(hipMemcpyAsync(out, rc.sums.p, (size_t)count * 8 * sizeof(float), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipGetLastError());
+    if (profiling) collect();
+  }
+
   void readRender(fp_render_tables *t) {
     HIPCHECK(hipStreamSynchronize(stream));
     t->light_work = rc.lastWork;
@@ -1704,6 +1747,11 @@ struct fp_sim {
     rc.lut.release();
     rc.image.release();
     rc.O.release();
+    rc.anchors.release();
+    rc.partials.release();
+    rc.blockSums.release();
+    rc.sums.release();
+    rc.brickAnchor.release();
     for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
     eventPool.clear();
     if (pinned) pinnedRelease(pinned, pinnedWords);
@@ -2384,6 +2432,15 @@ int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *co
   }
   *count = n;
   return FP_OK;
+}
+
+int fp_sim_scene_lights(fp_sim *sim, const fp_render_settings *settings, const double *anchors, int32_t count,
+                        float *sums) {
+  if (!sim || !settings || (count > 0 && (!anchors || !sums))) {
+    g_error = "fp_sim_scene_lights: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] { sim->sceneLights(*settings, anchors, count, sums); });
 }
 
 int fp_batch_create(fp_batch **out) {

@@ -188,6 +188,16 @@ class FluidSimulation:
         self._lighting_divisor = rs.lighting_divisor
         return img
 
+    def scene_lights(self, anchors, settings: Optional[abi.RenderSettings] = None) -> np.ndarray:
+        """SceneLights' totals of the latest step (fp_sim_scene_lights): per anchor (x, y, z)
+        the flames' luminance, their luminance-weighted offset from it, and their rgb."""
+        rs = settings if settings is not None else abi.render_settings()
+        a = np.ascontiguousarray(np.asarray(anchors, np.float64).reshape(-1, 3))
+        out = np.zeros((len(a), 8), np.float32)
+        abi.check(self._lib.fp_sim_scene_lights(self._h(), C.byref(rs), a.ctypes.data_as(C.POINTER(C.c_double)),
+                                                len(a), out.ctypes.data_as(C.POINTER(C.c_float))), self._lib)
+        return out
+
     def render_device(self, camera: abi.Camera, settings: abi.RenderSettings, pointer: int) -> None:
         """Render into device memory at `pointer` (width * height float4), enqueued on the
         simulation's stream."""
