@@ -59,6 +59,9 @@ def parse():
     p.add_argument("--serving-no-graphs", action="store_true",
                    help="with --serving: launch the batched kernels one by one (for kernel tracing)")
     p.add_argument("--serving-rate", type=float, default=20.0)
+    p.add_argument("--mix", type=int, default=48,
+                   help="requests of the serving workload for the batched mixed-precision comparison "
+                        "in the default line (0 skips it)")
     p.add_argument("--copy-probe", type=int, default=0,
                    help="only copy this many MiB device to device 5 times (counter calibration)")
     return p.parse_args()
@@ -150,6 +153,27 @@ def serving_line(args) -> dict:
         line["single_vcycle_all_at_once"] = {"batched": vcyc_b, "one_sim_per_launch": vcyc_s,
                                              "speedup": ratio(vcyc_b, vcyc_s)}
     return line
+
+
+def mix_comparison(count: int) -> dict:
+    """The metric's comparison, on a sample of the serving workload: `count` requests of it
+    present at once, each pressure solved to a relative residual of 1e-4 (at most 20
+    V-cycles), served batched with BF16 multigrid levels and one simulation per launch with
+    FP32 levels. `bench.py --serving` runs the whole Poisson stream."""
+    from firepro import serving
+    solve = (1e-4, 20)
+    warm = serving.requests(count, 1e9)
+    for r in warm:
+        r.steps = 1
+    serving.serve(warm, batched=True, pressure=("bf16",) + solve)
+    b = serving.serve(serving.requests(count, 1e9), batched=True, pressure=("bf16",) + solve).summary()
+    s = serving.serve(serving.requests(count, 1e9), batched=False, pressure=("fp32",) + solve).summary()
+    keep = ("voxel_steps_per_s", "wall_s", "rounds", "p50_step_ms", "p99_step_ms", "mean_cycles",
+            "max_relative_residual", "steps_within_tolerance")
+    return {"sample": f"{count} serving requests present at once (1-4 fires, voxel 0.025-0.1 m, brick 8, "
+                      "16-8192 bricks, 30-300 steps)",
+            "batched_bf16": {k: b[k] for k in keep}, "one_sim_per_launch_fp32": {k: s[k] for k in keep},
+            "speedup": b["voxel_steps_per_s"] / s["voxel_steps_per_s"] if s["voxel_steps_per_s"] else None}
 
 
 def render_phase(sim, frames, step) -> dict:
@@ -270,6 +294,8 @@ def main():
     if args.render_steps > 0:
         line["render"], line["stage_ms"] = render_phase(sim, frames[args.warmup + args.steps:], step)
     sim.dispose()
+    if args.mix > 0:
+        line["mixed_precision_batched"] = mix_comparison(args.mix)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lambda: scenes.four_fires(seed=args.seed), args.cpu_seconds)
     print(json.dumps(line))
