@@ -901,52 +901,43 @@ FP_MG_PAIR(dc_update, 256)
 // n + 1 entries; one workgroup of 1024 lanes.
 constexpr int PREFIX_LANES = 1024;
 template <class St>
+__device__ __forceinline__ int levelGroups(const MGArgsT<typename St::T> &a, int L, int cells, bool tile) {
+  if (!a.counts || dcFinished(a)) return 0;
+  if (L == MAX_LEVELS) return 1;
+  const int units = tile ? a.dir.capacity : a.counts[0];
+  const int k = groupSize(cells, tile ? 64 : 256);
+  return ((units + k - 1) / k + 7) / 8 * 8;
+}
+template <class St>
 __global__ void __launch_bounds__(PREFIX_LANES) k_level_prefix_batch(const MGArgsT<typename St::T> *args, int n, int *out) {
-  __shared__ int scan[PREFIX_LANES];
+  __shared__ int waveTotals[PREFIX_LANES / 64];
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
   const int per = (n + PREFIX_LANES - 1) / PREFIX_LANES;
   const int lo = (int)threadIdx.x * per, hi = lo + per < n ? lo + per : n;
   const int bottom = n > 0 && args[0].counts ? args[0].bottom : 0;
   for (int L = 0; L <= MAX_LEVELS; L++) {
     if (L < MAX_LEVELS && L > bottom) continue;
-    int *o = out + (size_t)L * (n + 1);
+    // Level shapes are the same for every member.
+    const int sd = L < MAX_LEVELS ? args[0].lv[L].side : 1, cells = sd * sd * sd;
+    const bool tile = L < MAX_LEVELS && args[0].lv[L].tile != 0;
     int mine = 0;
-    for (int i = lo; i < hi; i++) {
-      const MGArgsT<typename St::T> &a = args[i];
-      if (!a.counts || dcFinished(a)) continue;
-      if (L == MAX_LEVELS) {
-        mine += 1;
-        continue;
-      }
-      const int sd = a.lv[L].side, cells = sd * sd * sd;
-      const bool tile = a.lv[L].tile != 0;
-      const int units = tile ? a.dir.capacity : a.counts[0];
-      const int k = groupSize(cells, tile ? 64 : 256);
-      mine += ((units + k - 1) / k + 7) / 8 * 8;
+    for (int i = lo; i < hi; i++) mine += levelGroups<St>(args[i], L, cells, tile);
+    // Inclusive scan within the wavefront, then the wavefronts' totals.
+    int v = mine;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(v, d, 64);
+      if (lane >= d) v += t;
     }
-    scan[threadIdx.x] = mine;
+    if (lane == 63) waveTotals[wave] = v;
     __syncthreads();
-    for (int d = 1; d < PREFIX_LANES; d <<= 1) {
-      int v = (int)threadIdx.x >= d ? scan[threadIdx.x - d] : 0;
-      __syncthreads();
-      scan[threadIdx.x] += v;
-      __syncthreads();
-    }
-    int run = scan[threadIdx.x] - mine;  // exclusive
+    int run = v - mine;
+    for (int w = 0; w < wave; w++) run += waveTotals[w];
+    int *o = out + (size_t)L * (n + 1);
     for (int i = lo; i < hi; i++) {
       o[i] = run;
-      const MGArgsT<typename St::T> &a = args[i];
-      if (!a.counts || dcFinished(a)) continue;
-      if (L == MAX_LEVELS) {
-        run += 1;
-        continue;
-      }
-      const int sd = a.lv[L].side, cells = sd * sd * sd;
-      const bool tile = a.lv[L].tile != 0;
-      const int units = tile ? a.dir.capacity : a.counts[0];
-      const int k = groupSize(cells, tile ? 64 : 256);
-      run += ((units + k - 1) / k + 7) / 8 * 8;
+      run += levelGroups<St>(args[i], L, cells, tile);
     }
-    if (threadIdx.x == PREFIX_LANES - 1) o[n] = scan[PREFIX_LANES - 1];
+    if (threadIdx.x == PREFIX_LANES - 1) o[n] = run;
     __syncthreads();
   }
 }

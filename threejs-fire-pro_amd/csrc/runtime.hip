@@ -1796,6 +1796,7 @@ struct fp_batch {
   size_t pinnedWords = 0;
   DBuf<double> reports;  // per simulation: defect correction's relative residual and cycles
   double *pinnedReports = nullptr;
+  int *pinnedLive = nullptr;  // members whose defect correction is still running
   int grid = 2048;    // persistent workgroups per launch, a multiple of 8
   bool graphs = true;  // replay captured launch sequences
   double lastMs = 0;
@@ -1838,10 +1839,13 @@ struct fp_batch {
   }
 
   // The launch sequence of a step depends on these; one captured graph per distinct key.
-  using GraphKey = std::array<int, 8>;
+  using GraphKey = std::array<int, 10>;
   std::map<GraphKey, hipGraphExec_t> execs;
 
-  void create() { HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
+  void create() {
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipHostMalloc((void **)&pinnedLive, sizeof(int), hipHostMallocDefault));
+  }
   void dropGraphs() {
     for (auto &kv : execs) (void)hipGraphExecDestroy(kv.second);
     execs.clear();
@@ -1856,6 +1860,8 @@ struct fp_batch {
     if (pinnedReadback) (void)hipHostFree(pinnedReadback);
     if (pinnedReports) (void)hipHostFree(pinnedReports);
     pinnedReports = nullptr;
+    if (pinnedLive) (void)hipHostFree(pinnedLive);
+    pinnedLive = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     hostArgs = nullptr;
     pinnedReadback = nullptr;
@@ -2004,23 +2010,40 @@ struct fp_batch {
         reinterpret_cast<const MGArgsT<typename St::T> *>(devArgs.p + c.args), lay.cap,
         reinterpret_cast<int *>(devArgs.p + lay.levels));
   }
-  template <class St> void correctBatch(const SolveShape &c, int maxCycles) {
+  // Defect correction over the batch, in the order of fp_sim::correct: after `it` V-cycles
+  // the residual and its norms decide, on the device, whether a member has finished (its
+  // own tolerance met or its own cycle budget spent); finished members drop out of every
+  // later launch through the level prefixes. correctStart runs the loop's start and its
+  // first cycles; correctCycles the cycles after `first - 1` up to `last`.
+  template <class St> void correctStart(const SolveShape &c, int last) {
     using Args = MGArgsT<typename St::T>;
     launch<Args>(k_dc_start_batch<St>, 256, c.args, lay.slots, wide(), 0);
     levelPrefixes<St>(c);
-    for (int it = 0; it <= maxCycles; it++) {
-      launch<Args>(k_dc_residual_batch<St>, 256, c.args, levelPrefix(0), wide(), 0);
-      launch<Args>(k_dc_norms_batch<St>, 256, c.args, levelPrefix(MAX_LEVELS), perSim(), it);
-      if (it == maxCycles) break;
-      levelPrefixes<St>(c);  // members that finished drop out
+    launch<Args>(k_dc_residual_batch<St>, 256, c.args, levelPrefix(0), wide(), 0);
+    launch<Args>(k_dc_norms_batch<St>, 256, c.args, levelPrefix(MAX_LEVELS), perSim(), 0);
+    levelPrefixes<St>(c);
+    correctCycles<St>(c, 1, last);
+  }
+  template <class St> void correctCycles(const SolveShape &c, int first, int last) {
+    using Args = MGArgsT<typename St::T>;
+    for (int it = first; it <= last; it++) {
       launch<Args>(k_dc_scale_batch<St>, 256, c.args, levelPrefix(0), wide(), 0);
       const int flips = vcycle<St>(c, false);
       launch<Args>(k_dc_update_batch<St>, 256, c.args, levelPrefix(0), wide(), flips << 8);
+      launch<Args>(k_dc_residual_batch<St>, 256, c.args, levelPrefix(0), wide(), 0);
+      launch<Args>(k_dc_norms_batch<St>, 256, c.args, levelPrefix(MAX_LEVELS), perSim(), it);
+      levelPrefixes<St>(c);  // members that finished drop out
     }
   }
+  template <class F> void forMode(int mode, F f) {
+    if (mode == 1) f(StoreF32{});
+    else if (mode == 2) f(StoreBF16{});
+    else f(StoreF16{});
+  }
 
-  // Every kernel of a step over the uploaded arguments, in stream order.
-  void launchStep(const SolveShape &shape, int mode, int maxCycles, int zchunks, int fchunks) {
+  // A step's kernels up to the pressure solve and, with the V-cycle policy, to its end; with
+  // defect correction, the loop's start and its cycles up to `last`.
+  void launchHead(const SolveShape &shape, int mode, int last, int zchunks, int fchunks) {
     // Allocation (fluid-allocation.wgsl).
     launch<AllocArgs>(k_build_bricks_batch, 256, lay.alloc, lay.blocks, wide());
     launch<AllocArgs>(k_scan_blocks_batch, 1024, lay.alloc, lay.one, perSim());
@@ -2049,13 +2072,43 @@ struct fp_batch {
     if (mode == 0) {
       levelPrefixes<StoreF32>(shape);
       vcycle<StoreF32>(shape, false);
-    } else if (mode == 1) correctBatch<StoreF32>(shape, maxCycles);
-    else if (mode == 2) correctBatch<StoreBF16>(shape, maxCycles);
-    else correctBatch<StoreF16>(shape, maxCycles);
+      launchTail();
+    } else {
+      forMode(mode, [&](auto st) { correctStart<decltype(st)>(shape, last); });
+    }
+  }
+  // The projection and the activity readback.
+  void launchTail() {
     launch<StepArgs>(k_project_batch, 256, lay.post, lay.slots, wide());
     k_readback_batch<<<perSim(), 256, 0, stream>>>(reinterpret_cast<const ReadbackArgs *>(devArgs.p + lay.rb),
                                                    lay.cap);
   }
+
+  // Runs `body`'s launches as a graph captured once per key, or directly.
+  template <class F> void replay(const GraphKey &key, F body) {
+    if (!graphs) {
+      body();
+      return;
+    }
+    auto it = execs.find(key);
+    if (it == execs.end()) {
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      body();
+      HIPCHECK(hipStreamEndCapture(stream, &g));
+      hipGraphExec_t exec = nullptr;
+      hipError_t err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHECK(err);
+      it = execs.emplace(key, exec).first;
+    }
+    HIPCHECK(hipGraphLaunch(it->second, stream));
+  }
+
+  // Defect correction runs HEAD_CYCLES cycles with the step, then CHUNK_CYCLES at a time while
+  // any member is still running (one 4-byte readback between chunks), so a step costs the
+  // cycles its slowest member needs rather than the largest budget.
+  static constexpr int HEAD_CYCLES = 8, CHUNK_CYCLES = 2;
 
   void enqueue(fp_sim *const *sims, int n) {
     const auto e0 = std::chrono::steady_clock::now();
@@ -2148,24 +2201,26 @@ struct fp_batch {
     auto l0 = std::chrono::steady_clock::now();
     phaseMs[1] = std::chrono::duration<double, std::milli>(l0 - e0).count();
     HIPCHECK(hipMemcpyAsync(devArgs.p, hostArgs, lay.bytes, hipMemcpyHostToDevice, stream));
-    if (graphs) {
-      const GraphKey key{mode, maxCycles, shape.brickLevels, shape.bottom, smallBottom ? 1 : 0, zchunks, fchunks,
-                         lay.cap};
-      auto it = execs.find(key);
-      if (it == execs.end()) {
-        hipGraph_t g = nullptr;
-        HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        launchStep(shape, mode, maxCycles, zchunks, fchunks);
-        HIPCHECK(hipStreamEndCapture(stream, &g));
-        hipGraphExec_t exec = nullptr;
-        hipError_t err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIPCHECK(err);
-        it = execs.emplace(key, exec).first;
-      }
-      HIPCHECK(hipGraphLaunch(it->second, stream));
+    const int sb = smallBottom ? 1 : 0;
+    if (mode == 0) {
+      replay(GraphKey{0, 0, 0, shape.brickLevels, shape.bottom, sb, zchunks, fchunks, lay.cap, 0},
+             [&] { launchHead(shape, 0, 0, zchunks, fchunks); });
     } else {
-      launchStep(shape, mode, maxCycles, zchunks, fchunks);
+      int done = std::min(HEAD_CYCLES, maxCycles);
+      replay(GraphKey{0, mode, done, shape.brickLevels, shape.bottom, sb, zchunks, fchunks, lay.cap, 0},
+             [&] { launchHead(shape, mode, done, zchunks, fchunks); });
+      const int *live = reinterpret_cast<const int *>(devArgs.p + levelPrefix(MAX_LEVELS)) + lay.cap;
+      while (done < maxCycles) {
+        HIPCHECK(hipMemcpyAsync(pinnedLive, live, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        if (*pinnedLive == 0) break;
+        const int first = done + 1, last = std::min(done + CHUNK_CYCLES, maxCycles);
+        replay(GraphKey{1, mode, first, last, shape.brickLevels, shape.bottom, sb, 0, lay.cap, 0}, [&] {
+          forMode(mode, [&](auto st) { correctCycles<decltype(st)>(shape, first, last); });
+        });
+        done = last;
+      }
+      replay(GraphKey{2, 0, 0, 0, 0, 0, 0, 0, lay.cap, 0}, [&] { launchTail(); });
     }
     HIPCHECK(hipMemcpyAsync(pinnedReadback, readback.p, words * sizeof(int), hipMemcpyDeviceToHost, stream));
     if (mode != 0)
