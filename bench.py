@@ -59,6 +59,7 @@ def parse():
     p.add_argument("--serving-no-graphs", action="store_true",
                    help="with --serving: launch the batched kernels one by one (for kernel tracing)")
     p.add_argument("--serving-rate", type=float, default=20.0)
+    p.add_argument("--explosions", action="store_true", help="the brick-churn workload instead")
     p.add_argument("--mix", type=int, default=48,
                    help="requests of the serving workload for the batched mixed-precision comparison "
                         "in the default line (0 skips it)")
@@ -155,6 +156,35 @@ def serving_line(args) -> dict:
     return line
 
 
+def explosions_line(args) -> dict:
+    """The brick-churn workload: 8 radial bursts in a 20 m cube (voxel 0.05 m, brick 8) at
+    seeded steps 0-150, 400 steps, active bricks swinging with the bursts. Whole-run
+    throughput and the allocation stage's share."""
+    from firepro import abi, scenes
+    from firepro.engine import FluidSimulation
+    sc = scenes.explosions(seed=args.seed, steps=400)
+    frames = [(abi.PackedFrame(f), g) for f, g in sc.frames()]
+    sim = FluidSimulation.create(sc.grid())
+    sim.profile(True)
+    bricks, voxels = [], 0
+    t0 = time.perf_counter()
+    for frame, g in frames:
+        sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+        sim.step(frame)
+        st = sim.stats()
+        bricks.append(st.active_bricks)
+        voxels += st.active_voxels
+    wall = time.perf_counter() - t0
+    stages = sim.profile_read()
+    sim.dispose()
+    return {"metric": "active voxel-steps/s", "value": voxels / wall, "unit": "active voxel-steps/s",
+            "n_gpus": 1, "steps": len(frames), "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "brick churn: 8 radial bursts in a 20 m cube, voxel 0.05 m, brick 8, 400 steps"},
+            "ms_per_step": 1e3 * wall / len(frames), "active_bricks_min": min(bricks[150:]),
+            "active_bricks_max": max(bricks), "active_bricks_last": bricks[-1],
+            "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
+
+
 def mix_comparison(count: int) -> dict:
     """The metric's comparison, on a sample of the serving workload: `count` requests of it
     present at once, each pressure solved to a relative residual of 1e-4 (at most 20
@@ -234,6 +264,9 @@ def main():
         return
     if args.serving:
         print(json.dumps(serving_line(args)))
+        return
+    if args.explosions:
+        print(json.dumps(explosions_line(args)))
         return
     from firepro import abi, scenes
     from firepro.engine import FluidSimulation
