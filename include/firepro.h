@@ -48,7 +48,9 @@ typedef struct fp_grid {
   double voxel_size;        /* meters, 0.0001..16 (FluidSimulation.ts:362-363) */
   int32_t velocity_divisor; /* 1, 2 or 4 */
   int32_t brick_size;       /* 8, 16 or 32 */
-  int32_t smoke_divisor;    /* 1 only: coarse smoke is not built */
+  int32_t smoke_divisor;    /* 1, 2 or 4: smoke on its own grid, divisor x voxel wide (smoke.wgsl);
+                               2 and 4 are not rendered (fp_sim_render, fp_sim_scene_lights)
+                               nor batched (fp_batch_step) */
   int32_t scalar_maccormack;/* 1: bounded MacCormack scalars; 0: first order */
 } fp_grid;
 
@@ -219,9 +221,15 @@ int fp_sim_read_tables(fp_sim *sim, fp_tables *out);
 /* Cell values of the active bricks in active-list order: the field grid's four channels
  * (smoke, heat, lifetime, fuel) per cell, the velocity grid's three MAC components, the
  * pressure, and the expansion rate. NULL skips a field. Each buffer holds
- * active_count * cells-per-brick * channels floats; `ids` receives the brick ids. */
+ * active_count * cells-per-brick * channels floats; `ids` receives the brick ids. With
+ * smoke_divisor 2 or 4 the fields' smoke channel holds the step's smoke production, and
+ * fp_sim_read_smoke reads the density. */
 int fp_sim_read_state(fp_sim *sim, int32_t *ids, float *fields, float *velocity,
                       float *pressure, float *expansion, int32_t capacity_bricks);
+/* The coarse smoke grid's density (smoke_divisor 2 or 4; smoke.wgsl smokeDensity) of the
+ * active bricks in active-list order, (brick_size / smoke_divisor)^3 floats per brick;
+ * returns the brick count. FP_EINVAL for a full-resolution simulation. */
+int fp_sim_read_smoke(fp_sim *sim, int32_t *ids, float *density, int32_t capacity_bricks);
 /* Overwrite the cell values of the active bricks, same layout as fp_sim_read_state. */
 int fp_sim_write_state(fp_sim *sim, const float *fields, const float *velocity);
 /* The device stream the solver enqueues on (hipStream_t), for callers that time it. */

@@ -20,6 +20,7 @@ struct Base {
   virtual int state(int32_t *ids, float *fields, float *vel, float *pressure, float *expansion, int cap) = 0;
   virtual int state64(int32_t *ids, double *fields, double *vel, double *pressure, double *expansion, int cap) = 0;
   virtual void write(const float *fields, const float *vel) = 0;
+  virtual int smoke(int32_t *ids, float *density, int cap) = 0;
   virtual void set(double cutoff, double budget, int ground, int which) = 0;
   virtual const std::vector<float> &noise() = 0;
   virtual void contentTiles(int32_t *out, int *count) = 0;
@@ -107,6 +108,20 @@ template <class R> struct Impl : Base {
   int state64(int32_t *ids, double *f, double *v, double *p, double *e, int cap) override {
     return stateT(ids, f, v, p, e, cap);
   }
+  // The coarse smoke density of each active brick, in active-list order.
+  int smoke(int32_t *ids, float *density, int cap) override {
+    if (!sim.coarse()) throw oracle::Error("this simulation has no coarse smoke grid (smokeDivisor 1).");
+    int n = (int)sim.activeList.size();
+    if (n > cap) return -1;
+    for (int i = 0; i < n; i++) {
+      int id = sim.activeList[i];
+      if (ids) ids[i] = id;
+      int s = sim.pageSlot(sim.entries[id] & 0xffffff);
+      if (density)
+        for (int c = 0; c < sim.nS; c++) density[(size_t)i * sim.nS + c] = (float)sim.S[(size_t)s * sim.nS + c];
+    }
+    return n;
+  }
   void write(const float *fields, const float *vel) override {
     int n = (int)sim.activeList.size();
     for (int i = 0; i < n; i++) {
@@ -185,6 +200,7 @@ template <class R> struct Impl : Base {
     int ld = rs.lighting_divisor;
     if (ld != 1 && ld != 2 && ld != 4) throw oracle::Error("lightingDivisor must be 1, 2 or 4.");
     if (sim.B / ld < 1) throw oracle::Error("lightingDivisor exceeds the brick size.");
+    if (sim.coarse()) throw oracle::Error("rendering with smokeDivisor 2 or 4 is not supported.");
   }
   int render(const fp_camera &cam, const fp_render_settings &rs, float *rgba) override {
     checkRender(rs);
@@ -316,6 +332,11 @@ int orc_state(void *h, int32_t *ids, float *f, float *v, float *p, float *e, int
 }
 int orc_state64(void *h, int32_t *ids, double *f, double *v, double *p, double *e, int cap) {
   return ((Base *)h)->state64(ids, f, v, p, e, cap);
+}
+int orc_smoke(void *h, int32_t *ids, float *density, int cap) {
+  int n = -1;
+  int rc = guard([&] { n = ((Base *)h)->smoke(ids, density, cap); });
+  return rc == FP_OK ? n : rc;
 }
 int orc_write_state(void *h, const float *f, const float *v) { return guard([&] { ((Base *)h)->write(f, v); }); }
 int orc_set_cutoff(void *h, double c) { return guard([&] { ((Base *)h)->set(c, 0, 0, 0); }); }

@@ -47,6 +47,7 @@ def load():
         "orc_state64": (C.c_int, [v, P(C.c_int32), P(C.c_double), P(C.c_double),
                                   P(C.c_double), P(C.c_double), C.c_int]),
         "orc_write_state": (C.c_int, [v, P(C.c_float), P(C.c_float)]),
+        "orc_smoke": (C.c_int, [v, P(C.c_int32), P(C.c_float), C.c_int]),
         "orc_set_cutoff": (C.c_int, [v, C.c_double]),
         "orc_set_budget": (C.c_int, [v, C.c_double]),
         "orc_set_ground": (C.c_int, [v, C.c_int]),
@@ -104,6 +105,7 @@ class OracleSimulation:
         self.brick_size = brick_size
         self.brick_cells = brick_size ** 3
         self.velocity_cells = (brick_size // velocity_divisor) ** 3
+        self.smoke_cells = (brick_size // smoke_divisor) ** 3
 
     @classmethod
     def create(cls, grid: dict, bits=32):
@@ -173,6 +175,17 @@ class OracleSimulation:
             if a is not None:
                 out[k] = a[:count]
         return out
+
+    def read_smoke(self) -> dict:
+        """The coarse smoke density of each active brick (smokeDivisor 2 or 4)."""
+        n = max(1, self.stats().active_bricks)
+        ids = np.zeros(n, np.int32)
+        out = np.zeros((n, self.smoke_cells), np.float32)
+        count = self._lib.orc_smoke(self._h, ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    out.ctypes.data_as(C.POINTER(C.c_float)), n)
+        if count < 0:
+            _check(count, self._lib)
+        return {"ids": ids[:count], "smoke": out[:count]}
 
     def write_state(self, fields, velocity):
         fa = np.ascontiguousarray(fields, np.float32) if fields is not None else None

@@ -19,8 +19,8 @@
 // buffers: values are stored at the template precision, with no half-precision rounding
 // (truncateHalf, fluid-common.wgsl:107-114, and every f16 store); the activity readback is
 // applied before the next step (the earliest a mapAsync can resolve,
-// FluidSimulation.ts:1701-1746); mesh emitters/colliders/force targets, coarse smoke and
-// the ambient-free debug exports are not restated.
+// FluidSimulation.ts:1701-1746); mesh emitters/colliders/force targets and the
+// ambient-free debug exports are not restated.
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -91,6 +91,8 @@ template <class R> struct Force {
 template <class R> struct Params {
   R hf, hv, dt;
   R invHf, invHv;
+  // Coarse smoke (smokeDivisor > 1): the smoke grid's spacing and its reciprocal.
+  R hs, invHs;
   R buoyancy, cooling, smokeWeight, damping, dissipation, vorticity, cutoff;
   bool lifetimeZero;
   R time, seed;
@@ -251,13 +253,18 @@ template <class R> class Sim {
     if (d != 1 && d != 2 && d != 4) throw Error("velocityDivisor must be 1, 2 or 4.");
     B = g.brick_size;
     if (B != 8 && B != 16 && B != 32) throw Error("brickSize must be 8, 16 or 32 fine cells.");
-    if (g.smoke_divisor != 1) throw Error("smokeDivisor must be 1: coarse smoke is not built.");
+    smokeDiv = g.smoke_divisor;
+    if (smokeDiv != 1 && smokeDiv != 2 && smokeDiv != 4) throw Error("smokeDivisor must be 1, 2 or 4.");
     maccormack = g.scalar_maccormack != 0;
     b = B / d;
     sF = ilog2(B);
     sV = ilog2(b);
     nF = B * B * B;
     nV = b * b * b;
+    bs = B / smokeDiv;
+    sS = ilog2(bs);
+    nS = bs * bs * bs;
+    smokeRatio = d > smokeDiv ? d / smokeDiv : 1;
     side = poolSide(d, B);
     maxSlots = side * side * poolSlotsPerAxis(B);
     brickLevels = sV;
@@ -269,6 +276,9 @@ template <class R> class Sim {
   // ---- host state -------------------------------------------------------------------
   double voxel;
   int d, B, b, sF, sV, nF, nV, side, maxSlots, brickLevels, bottom;
+  // Coarse smoke: the divisor, the smoke brick's edge, shift and cells, and the smoke
+  // cells per velocity cell edge that buoyancy averages (FluidSimulation.ts:374-380).
+  int smokeDiv = 1, bs = 0, sS = 0, nS = 0, smokeRatio = 1;
   bool maccormack = true;
   double voxelBudget = INFINITY;
   double cutoff = 0;
@@ -314,6 +324,9 @@ template <class R> class Sim {
   // Pools, slot-major: field cells x4, expansion, velocity [slot][axis][cell], solids bits.
   std::vector<R> F, Fp, Fn, E, V, Vadv, Vcor, Vf, Curl, D, Pfine;
   std::vector<uint32_t> solids;
+  // Coarse smoke pools (smoke.wgsl): density, predictor, next, and the predictor's donor
+  // bounds per smoke cell.
+  std::vector<R> S, Sp, Sn, SLo, SHi;
   // Multigrid levels: brick levels per slot, tile levels per tile slot.
   struct Level {
     int side = 0;
@@ -395,6 +408,10 @@ template <class R> class Sim {
     rs(F, (size_t)nF * 4);
     rs(E, (size_t)nF);
     rs(V, (size_t)nV * 3);
+    if (coarse()) {
+      rs(S, (size_t)nS);
+      for (auto *v : {&Sp, &Sn, &SLo, &SHi}) v->assign((size_t)slots * nS, R(0));
+    }
     Fp.assign((size_t)slots * nF * 4, R(0));
     Fn.assign((size_t)slots * nF * 4, R(0));
     Vadv.assign((size_t)slots * nV * 3, R(0));
@@ -940,6 +957,8 @@ template <class R> class Sim {
     P.hv = (R)(float)velocitySpacing();
     P.invHf = R(1) / P.hf;
     P.invHv = R(1) / P.hv;
+    P.hs = (R)(float)(voxel * smokeDiv);
+    P.invHs = R(1) / P.hs;
     P.dt = (R)(float)dt;
     P.buoyancy = (R)(float)s.buoyancy;
     P.cooling = (R)(float)s.cooling;
@@ -997,6 +1016,11 @@ template <class R> class Sim {
     advectScalars();
     evolveFields();
     std::swap(F, Fn);
+    if (coarse()) {
+      advectSmoke();
+      evolveSmoke();
+      std::swap(S, Sn);
+    }
     if (P.vorticity != R(0)) computeCurl();
     applyForces();
     computeDivergence();
@@ -1173,6 +1197,8 @@ template <class R> class Sim {
       std::fill(F.begin() + (size_t)s * nF * 4, F.begin() + (size_t)(s + 1) * nF * 4, R(0));
       std::fill(E.begin() + (size_t)s * nF, E.begin() + (size_t)(s + 1) * nF, R(0));
       std::fill(V.begin() + (size_t)s * nV * 3, V.begin() + (size_t)(s + 1) * nV * 3, R(0));
+      // zeroSmokeBricks (smoke.wgsl:187-223)
+      if (coarse()) std::fill(S.begin() + (size_t)s * nS, S.begin() + (size_t)(s + 1) * nS, R(0));
     }
   }
 
@@ -1358,7 +1384,7 @@ template <class R> class Sim {
         R sp[3], v[4];
         for (int a = 0; a < 3; a++) sp[a] = dep[a] * P.invHf;
         sampleField(F, sp, v);
-        out[0] = v[0];
+        out[0] = coarse() ? R(0) : v[0];
         out[1] = v[1];
         out[2] = clampv(v[2], R(0), R(1));
         out[3] = v[3];
@@ -1369,7 +1395,9 @@ template <class R> class Sim {
       departureCell(c, pos, dep, base, frac);
       Donors dn;
       predict(base, frac, v, dn);
-      out[0] = v[0];
+      // Coarse smoke: the fine X channel only stages the step's smoke production
+      // (fluid-scalar-transport.wgsl:127, :137, :191-199).
+      out[0] = coarse() ? R(0) : v[0];
       out[1] = v[1];
       out[2] = P.lifetimeZero ? R(0) : v[2];
       out[3] = v[3];
@@ -1447,14 +1475,18 @@ template <class R> class Sim {
     out[3] = bounded[2];
   }
 
-  // contentSupport (fluid-content.wgsl:29-39).
+  // contentSupport (fluid-content.wgsl:29-39) on the field grid.
   uint32_t contentSupport(const int id[3], const R vel[3]) const {
-    R sidef = (R)B;
-    R ratio = P.hv / P.hf;
+    return supportOn(B, P.hv / P.hf, P.invHf, id, vel);
+  }
+  // The same on a grid of bricks `side` cells a side, velocity cells `ratio` of its cells
+  // wide and cells 1 / invH wide.
+  uint32_t supportOn(int side, R ratio, R invH, const int id[3], const R vel[3]) const {
+    R sidef = (R)side;
     bool low[3], high[3];
     for (int a = 0; a < 3; a++) {
-      R local = (R)(id[a] & (B - 1)) + R(0.5);
-      R pad = fmin(sidef, fmax(R(2), R(2) * ratio) + fabs_(vel[a]) * P.dt * P.invHf);
+      R local = (R)(id[a] & (side - 1)) + R(0.5);
+      R pad = fmin(sidef, fmax(R(2), R(2) * ratio) + fabs_(vel[a]) * P.dt * invH);
       low[a] = local < pad;
       high[a] = local + pad >= sidef;
     }
@@ -1477,6 +1509,7 @@ template <class R> class Sim {
       if (!solidAt(pos)) {
         R tr[4];
         correctedScalars(c, s, l, pos, cv, tr);
+        if (coarse()) tr[0] = 0;
         R lifetime = tr[2];
         R sp[2] = {0, 0};
         R injectedFuel = 0;
@@ -1521,7 +1554,7 @@ template <class R> class Sim {
       E[(size_t)s * nF + l] = expansion;
       if (expansion != R(0)) expands[bid] = 1;
       // holdsContent (fluid-content.wgsl:22-24)
-      if (fmax(fmax(value[0], value[2]), value[3]) > P.cutoff) busy[bid] |= contentSupport(c, cv);
+      if (fmax(fmax(coarse() ? R(0) : value[0], value[2]), value[3]) > P.cutoff) busy[bid] |= contentSupport(c, cv);
     });
     // recordContent (fluid-content.wgsl:43-64)
     for (int id : activeList) {
@@ -1535,6 +1568,163 @@ template <class R> class Sim {
       if (expands[id]) flags |= EXPANDS;
       activity[id] |= flags;
     }
+  }
+
+  // ---- coarse smoke (smoke.wgsl) --------------------------------------------------------
+  // With smokeDivisor 2 or 4 smoke density lives on its own grid, smokeDivisor fine cells
+  // wide, in pools of (B / smokeDivisor)^3 cells per slot. Fine evolution stages each
+  // step's smoke production in the fields' X channel; the coarse grid transports the
+  // density with the same velocity, adds the production's box average and decays it.
+  bool coarse() const { return smokeDiv > 1; }
+  R smokeCell(const std::vector<R> &buf, int x, int y, int z) const {
+    int s = slotAtBrick(x >> sS, y >> sS, z >> sS);
+    if (s < 0) return 0;
+    int m = bs - 1;
+    return buf[(size_t)s * nS + (((z & m) * bs + (y & m)) * bs + (x & m))];
+  }
+  // A trilinear sample of a smoke pool at `pos` in smoke cells (sampleAt,
+  // fluid-sampling.wgsl:85-119), with the range of the eight cells it weighs: the minimum
+  // from 1e30 and the maximum from 0, as smokeDonorBounds (smoke.wgsl:59-73) scans them.
+  R sampleSmoke(const std::vector<R> &buf, const R pos[3], R *lo = nullptr, R *hi = nullptr) const {
+    R q[3], f[3];
+    int base[3];
+    for (int a = 0; a < 3; a++) {
+      q[a] = pos[a] - R(0.5);
+      R fl = ffloor(q[a]);
+      base[a] = (int)fl;
+      f[a] = q[a] - fl;
+    }
+    R out = 0, mn = R(1e30), mx = 0;
+    for (int z = 0; z < 2; z++)
+      for (int y = 0; y < 2; y++)
+        for (int x = 0; x < 2; x++) {
+          R wx = x ? f[0] : R(1) - f[0], wy = y ? f[1] : R(1) - f[1], wz = z ? f[2] : R(1) - f[2];
+          R v = smokeCell(buf, base[0] + x, base[1] + y, base[2] + z);
+          out += v * (wx * wy * wz);
+          mn = fmin(mn, v);
+          mx = fmax(mx, v);
+        }
+    if (lo) *lo = mn;
+    if (hi) *hi = mx;
+    return out;
+  }
+  void centerS(const int id[3], R w[3]) const {
+    for (int a = 0; a < 3; a++) w[a] = ((R)id[a] + R(0.5)) * P.hs;
+  }
+  template <class Fn> void forActiveSmokeCells(Fn fn) {
+    for (int id : activeList) {
+      int bk[3];
+      brickOf(id, bk);
+      int s = pageSlot(entries[id] & 0xffffff);
+      for (int l = 0; l < nS; l++) {
+        int c[3] = {bk[0] * bs + (l & (bs - 1)), bk[1] * bs + ((l >> sS) & (bs - 1)), bk[2] * bs + (l >> (2 * sS))};
+        fn(id, s, l, c);
+      }
+    }
+  }
+  // advectSmoke (smoke.wgsl:104-126): the backward trace from each smoke cell center, the
+  // predictor and, for the corrector, the range of its eight donors.
+  void advectSmoke() {
+    forActiveSmokeCells([&](int, int s, int l, const int c[3]) {
+      R pos[3];
+      centerS(c, pos);
+      R density = 0, lo = 0, hi = 0;
+      if (!solidAt(pos)) {
+        R cv[3], dep[3], sp[3];
+        traceCenter(pos, cv);
+        traceBack(pos, cv, dep);
+        for (int a = 0; a < 3; a++) sp[a] = dep[a] * P.invHs;
+        density = sampleSmoke(S, sp, &lo, &hi);
+      }
+      Sp[(size_t)s * nS + l] = density;
+      SLo[(size_t)s * nS + l] = lo;
+      SHi[(size_t)s * nS + l] = hi;
+    });
+  }
+  // The box average of a field pool over the `ratio`^3 fine cells under cell `id` of a grid
+  // `ratio` fine cells wide: one trilinear sample at the shared corner of each 2^3 block
+  // (velocityCellAverage, fluid-sampling.wgsl:270-290; smokeProduction, smoke.wgsl:24-37).
+  void boxAverage4(const std::vector<R> &buf, int ratio, const int id[3], R out[4]) const {
+    if (ratio == 1) {
+      fieldCell(buf, id[0], id[1], id[2], out);
+      return;
+    }
+    int blocks = ratio / 2;
+    R r = (R)ratio;
+    R sum[4] = {0, 0, 0, 0};
+    for (int z = 0; z < blocks; z++)
+      for (int y = 0; y < blocks; y++)
+        for (int x = 0; x < blocks; x++) {
+          R corner[3] = {(R)id[0] * r + R(1) + R(2) * (R)x, (R)id[1] * r + R(1) + R(2) * (R)y,
+                         (R)id[2] * r + R(1) + R(2) * (R)z};
+          R v[4];
+          sampleField(buf, corner, v);
+          for (int k = 0; k < 4; k++) sum[k] += v[k];
+        }
+    if (blocks == 1) {
+      for (int k = 0; k < 4; k++) out[k] = sum[k];
+      return;
+    }
+    R n = (R)(blocks * blocks * blocks);
+    for (int k = 0; k < 4; k++) out[k] = sum[k] / n;
+  }
+  // evolveSmoke (smoke.wgsl:128-171): the MacCormack correction bounded by the donors, then
+  // decay and the box average of the fine production (already decayed over half a step),
+  // the dormant cleanup, and content with its support on the smoke grid.
+  void evolveSmoke() {
+    std::vector<uint32_t> busy(brickCount(), 0u);
+    const R loss = fexp(-P.dissipation * P.dt);
+    forActiveSmokeCells([&](int bid, int s, int l, const int c[3]) {
+      R pos[3];
+      centerS(c, pos);
+      R density = 0;
+      if (!solidAt(pos)) {
+        R cv[3];
+        traceCenter(pos, cv);
+        const size_t at = (size_t)s * nS + l;
+        R predictor = Sp[at];
+        R bounded = predictor;
+        if (maccormack) {
+          R lo = SLo[at], hi = SHi[at];
+          if (lo != hi || predictor != lo) {
+            R fw[3], sp[3];
+            traceForward(pos, cv, fw);
+            for (int a = 0; a < 3; a++) sp[a] = fw[a] * P.invHs;
+            R reversed = sampleSmoke(Sp, sp);
+            R corrected = predictor + R(0.5) * (S[at] - reversed);
+            bounded = (corrected < lo || corrected > hi) ? predictor : corrected;
+          }
+        }
+        R produced[4];
+        boxAverage4(F, smokeDiv, c, produced);
+        density = fmax(R(0), bounded * loss + produced[0]);
+        if (produced[0] == R(0) && produced[2] == R(0) && fmax(density, fmax(produced[1], produced[3])) < R(0.001))
+          density = 0;
+        if (density > P.cutoff) busy[bid] |= supportOn(bs, P.hv / P.hs, P.invHs, c, cv);
+      }
+      Sn[(size_t)s * nS + l] = density;
+    });
+    for (int id : activeList) {
+      if (!busy[id]) continue;
+      uint32_t local = (uint32_t)id & 63u;
+      uint32_t bits = (1u << (local & 3u)) | (16u << ((local >> 2) & 3u)) | (256u << (local >> 4));
+      activity[(size_t)brickCount() + (id >> 6)] |= bits;
+      activity[id] |= CONTENT | (busy[id] << 3);
+    }
+  }
+  // smokeVelocityAverage (smoke.wgsl:4-21): the smoke density under velocity cell `p`, for
+  // buoyancy. With velocity cells at most as wide as smoke cells, one sample at the cell's
+  // center; with velocity cells two smoke cells wide, their box average.
+  R smokeVelocityAverage(const int p[3]) const {
+    R sp[3];
+    if (smokeRatio == 1) {
+      R w[3];
+      centerV(p, w);
+      for (int a = 0; a < 3; a++) sp[a] = w[a] * P.invHs;
+    } else {
+      for (int a = 0; a < 3; a++) sp[a] = (R)p[a] * R(2) + R(1);
+    }
+    return sampleSmoke(S, sp);
   }
 
   // ---- forces and divergence (fluid-forces.wgsl) --------------------------------------
@@ -1642,30 +1832,7 @@ template <class R> class Sim {
     }
   }
   // velocityCellAverage (fluid-sampling.wgsl:420-436) of the field grid under cell `id`.
-  void velocityCellAverage4(const int id[3], R out[4]) const {
-    if (d == 1) {
-      fieldCell(F, id[0], id[1], id[2], out);
-      return;
-    }
-    int blocks = d / 2;
-    R ratio = (R)d;
-    R sum[4] = {0, 0, 0, 0};
-    for (int z = 0; z < blocks; z++)
-      for (int y = 0; y < blocks; y++)
-        for (int x = 0; x < blocks; x++) {
-          R corner[3] = {(R)id[0] * ratio + R(1) + R(2) * (R)x, (R)id[1] * ratio + R(1) + R(2) * (R)y,
-                         (R)id[2] * ratio + R(1) + R(2) * (R)z};
-          R v[4];
-          sampleField(F, corner, v);
-          for (int k = 0; k < 4; k++) sum[k] += v[k];
-        }
-    if (blocks == 1) {
-      for (int k = 0; k < 4; k++) out[k] = sum[k];
-      return;
-    }
-    R n = (R)(blocks * blocks * blocks);
-    for (int k = 0; k < 4; k++) out[k] = sum[k] / n;
-  }
+  void velocityCellAverage4(const int id[3], R out[4]) const { boxAverage4(F, d, id, out); }
   R expansionAverage(const int id[3]) const {
     if (d == 1) return fieldScalar(E, id[0], id[1], id[2]);
     int blocks = d / 2;
@@ -1720,7 +1887,8 @@ template <class R> class Sim {
     R ext[3];
     externalForces(w, ext);
     out[0] = conf[0] + ext[0] + R(0);
-    out[1] = conf[1] + ext[1] + (P.buoyancy * f[1] - P.smokeWeight * f[0]);
+    const R density = coarse() ? smokeVelocityAverage(p) : f[0];
+    out[1] = conf[1] + ext[1] + (P.buoyancy * f[1] - P.smokeWeight * density);
     out[2] = conf[2] + ext[2] + R(0);
   }
   void cellForce(const int p[3], R out[3]) const {

@@ -53,6 +53,8 @@ def run_lockstep(scene, steps=None, check_every=1):
             msgs = table_mismatches(gpu.read_tables(), cpu.read_tables())
             msgs += state_mismatches(gpu.read_state(pressure=True, expansion=True),
                                      cpu.read_state(pressure=True, expansion=True))
+            if scene.grid()["smokeDivisor"] > 1:
+                msgs += state_mismatches(gpu.read_smoke(), cpu.read_smoke(), keys=("smoke",))
             if msgs:
                 report.append((k, msgs))
     finally:

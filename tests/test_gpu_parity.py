@@ -24,6 +24,49 @@ def test_small_scene_bit_exact(brick, divisor):
     assert run_lockstep(sc) == []
 
 
+@pytest.mark.parametrize("brick,divisor,smoke", [(8, 2, 2), (8, 1, 2), (8, 2, 4), (16, 2, 2), (16, 4, 2), (16, 4, 4)])
+def test_coarse_smoke_bit_exact(brick, divisor, smoke):
+    """smokeDivisor 2 and 4 (smoke.wgsl): the coarse density, the fine fields that stage
+    its production, the velocity it drives and every table, bit-exact."""
+    sc = _scenes().small(steps=8, brick=brick, divisor=divisor, smoke=smoke)
+    print(f"case coarse smoke brick={brick} divisor={divisor} smoke={smoke}", flush=True)
+    assert run_lockstep(sc) == []
+
+
+def test_coarse_smoke_first_order_and_campfire():
+    sc = _scenes().small(steps=5, smoke=2)
+    sc.sim.options["scalarMacCormack"] = False
+    print("case coarse smoke first order", flush=True)
+    assert run_lockstep(sc) == []
+    print("case campfire smoke=2", flush=True)
+    assert run_lockstep(_scenes().campfire(steps=30, smoke=2), check_every=5) == []
+
+
+def test_coarse_smoke_refuses_render_and_batch():
+    from firepro import abi
+    from firepro.engine import Batch, FluidSimulation
+    sc = _scenes().small(steps=2, smoke=2)
+    gpu = FluidSimulation.create(sc.grid())
+    batch = Batch()
+    try:
+        frames = list(sc.frames())
+        gpu.step(frames[0][0])
+        cam = abi.camera((0.0, 1.0, 3.0), (0.0, 0.5, 0.0), 50, 16, 16)
+        with pytest.raises(ValueError, match="smokeDivisor"):
+            gpu.render(cam)
+        with pytest.raises(ValueError, match="smokeDivisor"):
+            gpu.scene_lights([(0.0, 0.3, 0.0)])
+        with pytest.raises(ValueError, match="smokeDivisor"):
+            batch.step([gpu], [frames[1][0]])
+        full = FluidSimulation.create(_scenes().small(steps=1).grid())
+        with pytest.raises(ValueError, match="coarse smoke"):
+            full.read_smoke()
+        full.dispose()
+    finally:
+        batch.dispose()
+        gpu.dispose()
+
+
 def test_first_order_scalars_bit_exact():
     sc = _scenes().small(steps=4)
     sc.sim.options["scalarMacCormack"] = False

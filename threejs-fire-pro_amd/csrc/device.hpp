@@ -61,6 +61,11 @@ struct Params {
   // e^{-burnRate dt}, e^{-dissipation dt}, e^{-cooling dt}, the same over dt / 2,
   // e^{-damping dt}, and the flame's lifetime decrement fdt / lifespan.
   float burnKeep, lossS, lossH, halfS, halfH, damp, lifeStep;
+  // Coarse smoke (smokeDivisor 2 or 4): the smoke grid's spacing and its reciprocal, and
+  // the smoke cells a velocity cell spans on each axis (1 or 2).
+  int coarse;
+  float hs, invHs;
+  int smokeRatio;
 };
 
 // Geometry of the grids and pools (constant per simulation).
@@ -68,6 +73,7 @@ struct Layout {
   int B, b, sF, sV, nF, nV;  // field and velocity brick edges, shifts, cells
   int side;                   // pool slots per row and rows per layer
   int brickLevels;
+  int sd, bs, sS, nS;         // smoke divisor, the smoke brick's edge, shift and cells
 };
 
 // The tile directory and brick entries as device arrays.

@@ -153,6 +153,8 @@ FP_DECLARE_PAIR(advect_velocity, StepArgs)
 FP_DECLARE_PAIR(correct_velocity, StepArgs)
 FP_DECLARE_PAIR1(advect_scalars, StepArgs, int)
 FP_DECLARE_PAIR1(evolve_fields, StepArgs, int)
+FP_DECLARE_PAIR(advect_smoke, StepArgs)
+FP_DECLARE_PAIR(evolve_smoke, StepArgs)
 FP_DECLARE_PAIR(curl, StepArgs)
 FP_DECLARE_PAIR(cell_forces, StepArgs)
 FP_DECLARE_PAIR(forces, StepArgs)

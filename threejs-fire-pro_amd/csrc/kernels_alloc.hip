@@ -294,6 +294,9 @@ __device__ __forceinline__ void zero_fresh_body(const AllocArgs &a, const int vb
     p.E[(size_t)slot * nF + c] = 0.f;
   }
   for (int c = chunk * 256 + threadIdx.x; c < 3 * nV; c += chunks * 256) p.V[(size_t)slot * nV * 3 + c] = 0.f;
+  // zeroSmokeBricks (smoke.wgsl:187-223)
+  if (a.L.sd > 1)
+    for (int c = chunk * 256 + threadIdx.x; c < a.L.nS; c += chunks * 256) p.S[(size_t)slot * a.L.nS + c] = 0.f;
 }
 FP_KERNEL_PAIR1(zero_fresh, 256, AllocArgs, int, chunks)
 

@@ -203,7 +203,7 @@ __device__ __forceinline__ void advect_scalars_body(const StepArgs &s, const int
     float4 *out = s.p.Fp + (size_t)slot * nF + l;
     if (!s.P.maccormack) {
       float4 v = sampleField(c, s.p.F, dep[0] * s.P.invHf, dep[1] * s.P.invHf, dep[2] * s.P.invHf);
-      *out = make_float4(v.x, v.y, clampv(v.z, 0.f, 1.f), v.w);
+      *out = make_float4(s.P.coarse ? 0.f : v.x, v.y, clampv(v.z, 0.f, 1.f), v.w);
       continue;
     }
     int base[3];
@@ -211,7 +211,9 @@ __device__ __forceinline__ void advect_scalars_body(const StepArgs &s, const int
     departureCell(s.P, cc, pos, dep, base, frac);
     Donors dn;
     float4 v = predict(c, s.p.F, base, frac, dn);
-    *out = make_float4(v.x, v.y, s.P.lifetimeZero ? 0.f : v.z, v.w);
+    // Coarse smoke: the fine X channel only stages the step's smoke production
+    // (fluid-scalar-transport.wgsl:127, :137, :191-199).
+    *out = make_float4(s.P.coarse ? 0.f : v.x, v.y, s.P.lifetimeZero ? 0.f : v.z, v.w);
     float4 *don = s.p.Don + (size_t)slot * nF * 3 + l;
     don[0] = make_float4(cv[0], cv[1], cv[2], dn.lmn);
     don[nF] = make_float4(dn.mn[0], dn.mn[1], dn.mn[2], dn.lmx);
@@ -369,20 +371,24 @@ __device__ __forceinline__ void evaluateFlame(const Params &P, const Ramps &R, f
   prod[2] = P.fexpansion * (P.lifespan * rampIntegral(R, 2, after, before) / P.fdt);
 }
 
-// contentSupport (fluid-content.wgsl:29-39).
-__device__ __forceinline__ uint32_t contentSupport(const Layout &L, const Params &P, const int id[3], const float vel[3]) {
-  float sidef = (float)L.B;
-  float ratio = P.hv / P.hf;
+// contentSupport (fluid-content.wgsl:29-39) on a grid of bricks `side` cells a side,
+// velocity cells `ratio` of its cells wide and cells 1 / invH wide.
+__device__ __forceinline__ uint32_t supportOn(int side, float ratio, float invH, float dt, const int id[3],
+                                              const float vel[3]) {
+  float sidef = (float)side;
   bool low[3], high[3];
   for (int a = 0; a < 3; a++) {
-    float local = (float)(id[a] & (L.B - 1)) + 0.5f;
-    float pad = fminv(sidef, fmaxv(2.0f, 2.0f * ratio) + fabsv(vel[a]) * P.dt * P.invHf);
+    float local = (float)(id[a] & (side - 1)) + 0.5f;
+    float pad = fminv(sidef, fmaxv(2.0f, 2.0f * ratio) + fabsv(vel[a]) * dt * invH);
     low[a] = local < pad;
     high[a] = local + pad >= sidef;
   }
   uint32_t row = 2u | (low[0] ? 1u : 0u) | (high[0] ? 4u : 0u);
   uint32_t plane = (row << 3) | (low[1] ? row : 0u) | (high[1] ? row << 6 : 0u);
   return (plane << 9) | (low[2] ? plane : 0u) | (high[2] ? plane << 18 : 0u);
+}
+__device__ __forceinline__ uint32_t contentSupport(const Layout &L, const Params &P, const int id[3], const float vel[3]) {
+  return supportOn(L.B, P.hv / P.hf, P.invHf, P.dt, id, vel);
 }
 
 // evolveFields (fluid-field-evolution.wgsl:208-283) with content recording.
@@ -442,6 +448,7 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     }
     if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
       float4 tr = correctedScalars(c, P, s.p, cc, slot, l, pos, cv, dn);
+      if (P.coarse) tr.x = 0.f;
       float lifetime = tr.z;
       float sp0 = 0.f, sp1 = 0.f, injectedFuel = 0.f;
       for (int i = 0; i < P.emitterCount; i++) {
@@ -481,7 +488,8 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     s.p.Fn[(size_t)slot * nF + l] = make_float4(value[0], value[1], value[2], value[3]);
     s.p.E[(size_t)slot * nF + l] = expansion;
     if (expansion != 0.f) myExpands = 1u;
-    if (fmaxv(fmaxv(value[0], value[2]), value[3]) > P.cutoff) myBusy |= contentSupport(s.L, P, cc, cv);
+    if (fmaxv(fmaxv(P.coarse ? 0.f : value[0], value[2]), value[3]) > P.cutoff)
+      myBusy |= contentSupport(s.L, P, cc, cv);
   }
   if (myBusy) atomicOr(&busy, myBusy);
   if (myExpands) atomicOr(&expands, 1u);
@@ -499,6 +507,7 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
   }
 }
 FP_KERNEL_PAIR1(evolve_fields, 256, StepArgs, int, chunks)
+
 
 
 // ---- forces ---------------------------------------------------------------------------
@@ -614,7 +623,9 @@ __device__ __forceinline__ void externalForces(const StepArgs &s, const float p[
   }
 }
 
-// velocityCellAverage (fluid-sampling.wgsl:420-436) of the fields under velocity cell id.
+// The box average of the field cells under cell `id` of a grid `d` field cells wide: one
+// trilinear sample at the shared corner of each 2^3 block (velocityCellAverage,
+// fluid-sampling.wgsl:270-290; smokeProduction, smoke.wgsl:24-37).
 __device__ __forceinline__ float4 velocityCellAverage(const Ctx &c, const float4 *F, int d, const int id[3]) {
   if (d == 1) return fieldCell(c, F, id[0], id[1], id[2]);
   int blocks = d / 2;
@@ -634,6 +645,21 @@ __device__ __forceinline__ float4 velocityCellAverage(const Ctx &c, const float4
   if (blocks == 1) return sum;
   float n = (float)(blocks * blocks * blocks);
   return make_float4(sum.x / n, sum.y / n, sum.z / n, sum.w / n);
+}
+
+// smokeVelocityAverage (smoke.wgsl:4-21): the coarse smoke under velocity cell `p`, for
+// buoyancy. With velocity cells at most as wide as smoke cells, one sample at the cell's
+// center; with velocity cells two smoke cells wide, their box average.
+__device__ __forceinline__ float smokeVelocityAverage(const Ctx &c, const StepArgs &s, const int p[3]) {
+  const Params &P = s.P;
+  float sp[3];
+  if (P.smokeRatio == 1) {
+    for (int a = 0; a < 3; a++) sp[a] = (((float)p[a] + 0.5f) * P.hv) * P.invHs;
+  } else {
+    for (int a = 0; a < 3; a++) sp[a] = (float)p[a] * 2.0f + 1.0f;
+  }
+  float lo, hi;
+  return sampleSmoke(c, s.p.S, sp[0], sp[1], sp[2], lo, hi);
 }
 
 // centerForce (fluid-forces.wgsl:79-104), zero where the brick does not compute.
@@ -665,7 +691,8 @@ __device__ __forceinline__ void cellForce(const Ctx &c, const StepArgs &s, const
   float ext[3];
   externalForces(s, w, ext);
   out[0] = conf[0] + ext[0] + 0.f;
-  out[1] = conf[1] + ext[1] + (P.buoyancy * f.y - P.smokeWeight * f.x);
+  const float density = P.coarse ? smokeVelocityAverage(c, s, p) : f.x;
+  out[1] = conf[1] + ext[1] + (P.buoyancy * f.y - P.smokeWeight * density);
   out[2] = conf[2] + ext[2] + 0.f;
 }
 
@@ -856,5 +883,99 @@ FP_KERNEL_PAIR(project, 256, StepArgs)
 
 
 
+
+// ---- coarse smoke (smoke.wgsl) ----------------------------------------------------------
+// With smokeDivisor 2 or 4 smoke density lives on its own grid, sd field cells wide, in
+// pools of (B / sd)^3 cells per slot. evolveFields stages each step's smoke production in
+// the fields' X channel; these kernels transport the density with the step's velocity, add
+// the production's box average and decay it. One 64-lane workgroup covers one brick.
+
+__device__ __forceinline__ void smokeCellCoords(const Ctx &c, int l, int out[3]) {
+  const int n = c.L.bs, sh = c.L.sS;
+  out[0] = c.home.bx * n + (l & (n - 1));
+  out[1] = c.home.by * n + ((l >> sh) & (n - 1));
+  out[2] = c.home.bz * n + (l >> (2 * sh));
+}
+
+// advectSmoke (smoke.wgsl:104-126): the backward trace from each smoke cell's center and
+// the predictor. For the corrector it leaves the cell's velocity and the range of the
+// predictor's eight donors in the scalar backtrace planes, which evolveFields has finished
+// with: (velocity, donor minimum) then (donor maximum, 0, 0, 0) per smoke cell.
+__device__ __forceinline__ void advect_smoke_body(const StepArgs &s, const int vb) {
+  BRICK_PREAMBLE(1)
+  const Params &P = s.P;
+  const int nS = s.L.nS;
+  float4 *don = s.p.Don + (size_t)slot * s.L.nF * 3;
+  for (int l = threadIdx.x; l < nS; l += blockDim.x) {
+    int cc[3];
+    smokeCellCoords(c, l, cc);
+    const float pos[3] = {((float)cc[0] + 0.5f) * P.hs, ((float)cc[1] + 0.5f) * P.hs, ((float)cc[2] + 0.5f) * P.hs};
+    float density = 0.f, lo = 0.f, hi = 0.f;
+    float cv[3] = {0.f, 0.f, 0.f};
+    if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+      float dep[3];
+      velocityAt(c, P, s.p.V, pos[0], pos[1], pos[2], cv);
+      traceBack(c, P, s.p.V, pos, cv, dep);
+      density = sampleSmoke(c, s.p.S, dep[0] * P.invHs, dep[1] * P.invHs, dep[2] * P.invHs, lo, hi);
+    }
+    s.p.Sp[(size_t)slot * nS + l] = density;
+    don[l] = make_float4(cv[0], cv[1], cv[2], lo);
+    don[nS + l] = make_float4(hi, 0.f, 0.f, 0.f);
+  }
+}
+FP_KERNEL_PAIR(advect_smoke, 64, StepArgs)
+
+// evolveSmoke (smoke.wgsl:128-171): the MacCormack correction bounded by the donors, decay,
+// the box average of the fine production (already decayed over half a step), the dormant
+// cleanup, and content with its support on the smoke grid.
+__device__ __forceinline__ void evolve_smoke_body(const StepArgs &s, const int vb) {
+  BRICK_PREAMBLE(1)
+  __shared__ uint32_t busy;
+  if (threadIdx.x == 0) busy = 0u;
+  __syncthreads();
+  const Params &P = s.P;
+  const int nS = s.L.nS;
+  const float4 *don = s.p.Don + (size_t)slot * s.L.nF * 3;
+  const float ratio = P.hv / P.hs;
+  uint32_t myBusy = 0u;
+  for (int l = threadIdx.x; l < nS; l += blockDim.x) {
+    int cc[3];
+    smokeCellCoords(c, l, cc);
+    const float pos[3] = {((float)cc[0] + 0.5f) * P.hs, ((float)cc[1] + 0.5f) * P.hs, ((float)cc[2] + 0.5f) * P.hs};
+    const size_t at = (size_t)slot * nS + l;
+    float density = 0.f;
+    if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+      const float4 d0 = don[l];
+      const float cv[3] = {d0.x, d0.y, d0.z};
+      const float predictor = s.p.Sp[at];
+      float bounded = predictor;
+      if (P.maccormack) {
+        const float lo = d0.w, hi = don[nS + l].x;
+        if (lo != hi || predictor != lo) {
+          float fw[3], rlo, rhi;
+          traceForward(c, P, s.p.V, pos, cv, fw);
+          const float reversed = sampleSmoke(c, s.p.Sp, fw[0] * P.invHs, fw[1] * P.invHs, fw[2] * P.invHs, rlo, rhi);
+          const float corrected = predictor + 0.5f * (s.p.S[at] - reversed);
+          bounded = (corrected < lo || corrected > hi) ? predictor : corrected;
+        }
+      }
+      const float4 produced = velocityCellAverage(c, s.p.F, s.L.sd, cc);
+      density = fmaxv(0.f, bounded * P.lossS + produced.x);
+      if (produced.x == 0.f && produced.z == 0.f && fmaxv(density, fmaxv(produced.y, produced.w)) < 0.001f)
+        density = 0.f;
+      if (density > P.cutoff) myBusy |= supportOn(s.L.bs, ratio, P.invHs, P.dt, cc, cv);
+    }
+    s.p.Sn[at] = density;
+  }
+  if (myBusy) atomicOr(&busy, myBusy);
+  __syncthreads();
+  if (threadIdx.x == 0 && busy) {
+    uint32_t local = (uint32_t)id & 63u;
+    uint32_t bits = (1u << (local & 3u)) | (16u << ((local >> 2) & 3u)) | (256u << (local >> 4));
+    atomicOr(&s.activity[s.brickCount + (id >> 6)], bits);
+    atomicOr(&s.activity[id], CONTENT | (busy << 3));
+  }
+}
+FP_KERNEL_PAIR(evolve_smoke, 64, StepArgs)
 
 }  // namespace fp

@@ -377,6 +377,9 @@ struct fp_sim {
   // Grid (construction-only) and derived layout.
   double voxel = 0;
   int d = 1, B = 16, b = 16, sF = 4, sV = 4, nF = 0, nV = 0, side = 0, maxSlots = 0, brickLevels = 0, bottom = 0;
+  // Coarse smoke: the divisor, the smoke brick's edge, shift and cells, and the smoke cells
+  // a velocity cell spans on each axis (FluidSimulation.ts:374-380).
+  int smokeDiv = 1, smokeEdge = 0, smokeShift = 0, nS = 0, smokeRatio = 1;
   bool maccormack = true;
   // Settings written before each step.
   double voxelBudget = INFINITY, cutoff = 0;
@@ -411,6 +414,7 @@ struct fp_sim {
   DBuf<float4> Vtr;  // the velocity backtraces of a step
   const float4 *noise = nullptr;  // the shared curl-noise table
   DBuf<float> E, V, Vadv, Vcor, Vf, D, P0, P1;
+  DBuf<float> S, Sp, Sn;  // coarse smoke density, predictor and next (smokeDivisor 2 or 4)
   DBuf<uint32_t> solids;
   struct LevelBufs {
     DBuf<float> p, q, rhs;
@@ -506,6 +510,10 @@ struct fp_sim {
     L.nV = nV;
     L.side = side;
     L.brickLevels = brickLevels;
+    L.sd = smokeDiv;
+    L.bs = smokeEdge;
+    L.sS = smokeShift;
+    L.nS = nS;
     return L;
   }
   Dir dirArgs() const {
@@ -567,7 +575,8 @@ struct fp_sim {
     if (d != 1 && d != 2 && d != 4) throw InvalidArgument("velocityDivisor must be 1, 2 or 4.");
     B = g.brick_size;
     if (B != 8 && B != 16 && B != 32) throw InvalidArgument("brickSize must be 8, 16 or 32 fine cells.");
-    if (g.smoke_divisor != 1) throw InvalidArgument("smokeDivisor must be 1: coarse smoke is not built.");
+    smokeDiv = g.smoke_divisor;
+    if (smokeDiv != 1 && smokeDiv != 2 && smokeDiv != 4) throw InvalidArgument("smokeDivisor must be 1, 2 or 4.");
     maccormack = g.scalar_maccormack != 0;
     b = B / d;
     sF = 0;
@@ -577,6 +586,11 @@ struct fp_sim {
     nF = B * B * B;
     nV = b * b * b;
     if (b < 2) throw InvalidArgument("velocity bricks need at least 2 cells a side.");
+    smokeEdge = B / smokeDiv;
+    smokeShift = 0;
+    while ((1 << smokeShift) < smokeEdge) smokeShift++;
+    nS = smokeEdge * smokeEdge * smokeEdge;
+    smokeRatio = d > smokeDiv ? d / smokeDiv : 1;
     side = host::poolSide(d, B);
     maxSlots = side * side * host::poolSlotsPerAxis(B);
     brickLevels = sV;
@@ -624,15 +638,21 @@ struct fp_sim {
       F.grow(s * nF, stream);
       E.grow(s * nF, stream);
       V.grow(s * nV * 3, stream);
+      if (smokeDiv > 1) S.grow(s * nS, stream);
       solids.grow(s * (nF / 32), stream);
     } else {
       F.alloc(s * nF);
       E.alloc(s * nF);
       V.alloc(s * nV * 3);
+      if (smokeDiv > 1) S.alloc(s * nS);
       solids.alloc(s * (nF / 32));
       HIPCHECK(hipMemsetAsync(solids.p, 0, solids.bytes(), stream));
     }
     Fp.alloc(s * nF);
+    if (smokeDiv > 1) {
+      Sp.alloc(s * nS);
+      Sn.alloc(s * nS);
+    }
     Don.alloc(s * nF * 3);
     Vtr.alloc(s * nV * 4);
     Fn.alloc(s * nF);
@@ -977,6 +997,10 @@ struct fp_sim {
     P.hv = (float)velocitySpacing();
     P.invHf = 1.0f / P.hf;
     P.invHv = 1.0f / P.hv;
+    P.coarse = smokeDiv > 1 ? 1 : 0;
+    P.hs = (float)(voxel * smokeDiv);
+    P.invHs = 1.0f / P.hs;
+    P.smokeRatio = smokeRatio;
     P.dt = (float)dt;
     P.buoyancy = (float)s.buoyancy;
     P.cooling = (float)s.cooling;
@@ -1045,6 +1069,9 @@ struct fp_sim {
     p.Fn = Fn.p;
     p.Don = Don.p;
     p.Vtr = Vtr.p;
+    p.S = S.p;
+    p.Sp = Sp.p;
+    p.Sn = Sn.p;
     p.E = E.p;
     p.V = V.p;
     p.Vadv = Vadv.p;
@@ -1190,6 +1217,16 @@ struct fp_sim {
     std::swap(F, Fn);
     s.p.F = F.p;
     s.p.Fn = Fn.p;
+    if (smokeDiv > 1) {
+      // Coarse smoke after the fine evolution: the backtrace planes are free again.
+      int ms = begin();
+      k_advect_smoke<<<slotsGrid, 64, 0, stream>>>(s);
+      k_evolve_smoke<<<slotsGrid, 64, 0, stream>>>(s);
+      end(ms, "smoke");
+      std::swap(S, Sn);
+      s.p.S = S.p;
+      s.p.Sn = Sn.p;
+    }
     int m5 = begin();
     if (P.vorticity != 0.f) k_curl<<<slotsGrid, 256, 0, stream>>>(s);
     // The velocity predictor is dead after correction: it holds the cell forces.
@@ -1576,6 +1613,7 @@ struct fp_sim {
   }
 
   void renderImage(const fp_camera &cam, const fp_render_settings &s, float *out, bool deviceOut) {
+    renderable();
     RenderParams P = renderParams(cam, s);
     renderPrepare(P);
     size_t pixels = (size_t)P.width * P.height;
@@ -1602,7 +1640,13 @@ struct fp_sim {
 
   // SceneLights.sample (SceneLights.ts:127-218) over the latest step's fields: per anchor,
   // the luminance of the flames nearest it, their luminance-weighted offset and their rgb.
+  // The render path reads smoke from the fields' X channel, which holds only the step's
+  // production when smoke has its own coarse grid.
+  void renderable() const {
+    if (smokeDiv > 1) throw InvalidArgument("rendering with smokeDivisor 2 or 4 is not supported.");
+  }
   void sceneLights(const fp_render_settings &s, const double *xyz, int count, float *out) {
+    renderable();
     if (count < 0 || count > 65536) throw InvalidArgument("anchor count must be 0 to 65536.");
     for (int i = 0; i < 3 * count; i++)
       if (!std::isfinite(xyz[i])) throw InvalidArgument("anchor positions must be finite.");
@@ -1640,6 +1684,7 @@ struct fp_sim {
   }
 
   void readRender(fp_render_tables *t) {
+    renderable();
     HIPCHECK(hipStreamSynchronize(stream));
     t->light_work = rc.lastWork;
     t->box_count = rc.boxCount;
@@ -1699,7 +1744,8 @@ struct fp_sim {
                    active.bytes() + zeroList.bytes() + brickState.bytes() + activity.bytes() + stateTmp.bytes() +
                    rankTmp.bytes() + blockSums.bytes() + blockPrefix.bytes() + freeSlots.bytes() + F.bytes() +
                    Fp.bytes() + Fn.bytes() + Don.bytes() + Vtr.bytes() + Curl.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
-                   Vcor.bytes() + Vf.bytes() + D.bytes() + P0.bytes() + P1.bytes() + solids.bytes() +
+                   Vcor.bytes() + Vf.bytes() + D.bytes() + P0.bytes() + P1.bytes() + solids.bytes() + S.bytes() +
+                   Sp.bytes() + Sn.bytes() +
                    emitters.bytes() + colliders.bytes() + forces.bytes() + regions.bytes();
     for (int L = 0; L <= bottom; L++)
       total += lv[L].p.bytes() + lv[L].q.bytes() + lv[L].rhs.bytes() + lv[L].topo.bytes() + lv[L].eflag.bytes();
@@ -1714,7 +1760,7 @@ struct fp_sim {
     for (auto *x : u32s) x->release();
     DBuf<float4> *f4s[] = {&F, &Fp, &Fn, &Don, &Vtr, &Curl};
     for (auto *x : f4s) x->release();
-    DBuf<float> *fs[] = {&E, &V, &Vadv, &Vcor, &Vf, &D, &P0, &P1};
+    DBuf<float> *fs[] = {&E, &V, &Vadv, &Vcor, &Vf, &D, &P0, &P1, &S, &Sp, &Sn};
     for (auto *x : fs) x->release();
     for (int L = 0; L < MAX_LEVELS; L++) {
       lv[L].p.release();
@@ -1914,6 +1960,7 @@ struct fp_batch {
       if (!sims[i]) throw InvalidArgument("batch member is null.");
       if (sims[i]->B != sims[0]->B || sims[i]->d != sims[0]->d)
         throw InvalidArgument("a batch's simulations must share brick size and velocity divisor.");
+      if (sims[i]->smokeDiv != 1) throw InvalidArgument("batched steps need full-resolution smoke (smokeDivisor 1).");
       if (sims[i]->pressureMode != sims[0]->pressureMode)
         throw InvalidArgument("a batch's simulations must share the pressure policy.");
       for (int j = 0; j < i; j++)
@@ -2458,6 +2505,32 @@ int fp_sim_write_state(fp_sim *sim, const float *fields, const float *velocity) 
                            hipMemcpyHostToDevice));
     }
   });
+}
+
+int fp_sim_read_smoke(fp_sim *sim, int32_t *ids, float *density, int32_t capacity) {
+  if (!sim) {
+    g_error = "fp_sim_read_smoke: null argument";
+    return FP_EINVAL;
+  }
+  int result = 0;
+  int rc = guard([&] {
+    if (sim->smokeDiv == 1) throw InvalidArgument("this simulation has no coarse smoke grid (smokeDivisor 1).");
+    int count = 0;
+    if (sim->stepped) HIPCHECK(hipMemcpy(&count, sim->counts.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (count > capacity) throw InvalidArgument("smoke buffers hold fewer bricks than are active.");
+    std::vector<int32_t> act(count), entries(sim->brickCount());
+    if (count) HIPCHECK(hipMemcpy(act.data(), sim->active.p, count * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(entries.data(), sim->entries.p, entries.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int nS = sim->nS;
+    for (int i = 0; i < count; i++) {
+      if (ids) ids[i] = act[i];
+      if (density)
+        HIPCHECK(hipMemcpy(density + (size_t)i * nS, sim->S.p + (size_t)sim->pageToSlotHost(entries[act[i]] & 0xffffff) * nS,
+                           nS * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    result = count;
+  });
+  return rc != FP_OK ? rc : result;
 }
 
 void *fp_sim_stream(fp_sim *sim) { return sim ? (void *)sim->stream : nullptr; }

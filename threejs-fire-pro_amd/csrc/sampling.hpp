@@ -17,6 +17,8 @@ struct Pools {
   // planes of float4 per slot, (face velocity, donor min) of components x, y, z, then the
   // three donor maxima.
   float4 *Vtr;
+  // Coarse smoke density, predictor and next (smokeDivisor 2 or 4), (B / sd)^3 per slot.
+  float *S, *Sp, *Sn;
   float *E;
   float *V, *Vadv, *Vcor, *Vf;
   float4 *Curl;
@@ -117,6 +119,41 @@ __device__ __forceinline__ void velCorners(const Ctx &c, const float *buf, int c
   const float *plane = buf + (size_t)comp * c.L.nV;
 #pragma unroll
   for (int k = 0; k < 8; k++) v[k] = slot[k] < 0 ? 0.f : plane[(size_t)slot[k] * c.L.nV * 3 + local[k]];
+}
+
+// The eight smoke cells from (x, y, z) of a coarse smoke pool.
+__device__ __forceinline__ void smokeCorners(const Ctx &c, const float *buf, int x, int y, int z, float v[8]) {
+  int slot[8], local[8];
+  cornerCells(c, c.L.sS, x, y, z, slot, local);
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = slot[k] < 0 ? 0.f : buf[(size_t)slot[k] * c.L.nS + local[k]];
+}
+// A trilinear sample of a smoke pool at (px, py, pz) in smoke cells, with the range of the
+// eight cells it weighs: the minimum from 1e30 and the maximum from 0, as smokeDonorBounds
+// (smoke.wgsl:59-73) scans them.
+__device__ __forceinline__ float sampleSmoke(const Ctx &c, const float *buf, float px, float py, float pz,
+                                             float &lo, float &hi) {
+  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
+  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+  float cv[8];
+  smokeCorners(c, buf, (int)flx, (int)fly, (int)flz, cv);
+  float out = 0.f;
+  lo = 1e30f;
+  hi = 0.f;
+#pragma unroll
+  for (int z = 0; z < 2; z++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
+        const float v = cv[(z * 2 + y) * 2 + x];
+        out += v * (wx * wy * wz);
+        lo = fminv(lo, v);
+        hi = fmaxv(hi, v);
+      }
+  return out;
 }
 
 // solidCell / solidAt (fluid-common.wgsl:123-150).

@@ -457,6 +457,7 @@ def load(path: str = LIBRARY):
         "fp_sim_read_state": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_float),
                                         P(C.c_float), P(C.c_float), C.c_int32]),
         "fp_sim_write_state": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float)]),
+        "fp_sim_read_smoke": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_float), C.c_int32]),
         "fp_sim_stream": (C.c_void_p, [C.c_void_p]),
         "fp_sim_profile": (C.c_int, [C.c_void_p, C.c_int32]),
         "fp_sim_profile_read": (C.c_int, [C.c_void_p, P(C.c_char_p), P(C.c_double),

@@ -30,6 +30,8 @@ class FluidSimulation:
         self.brick_size = brick_size
         self.brick_cells = brick_size ** 3
         self.velocity_cells = (brick_size // velocity_divisor) ** 3
+        self.smoke_divisor = smoke_divisor
+        self.smoke_cells = (brick_size // smoke_divisor) ** 3
 
     # FluidSimulation.create (FluidSimulation.ts:419-475) is the constructor above.
     @classmethod
@@ -157,6 +159,17 @@ class FluidSimulation:
             if a is not None:
                 out[k] = a[:count]
         return out
+
+    def read_smoke(self) -> dict:
+        """The coarse smoke grid's density per active brick (smokeDivisor 2 or 4)."""
+        n = max(1, self.stats().active_bricks)
+        ids = np.zeros(n, np.int32)
+        out = np.zeros((n, self.smoke_cells), np.float32)
+        count = self._lib.fp_sim_read_smoke(self._h(), ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            out.ctypes.data_as(C.POINTER(C.c_float)), n)
+        if count < 0:
+            abi.check(count, self._lib)
+        return {"ids": ids[:count], "smoke": out[:count]}
 
     def write_state(self, fields: Optional[np.ndarray], velocity: Optional[np.ndarray]) -> None:
         ptr = lambda a: (np.ascontiguousarray(a, np.float32).ctypes.data_as(C.POINTER(C.c_float))

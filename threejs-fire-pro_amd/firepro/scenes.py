@@ -64,9 +64,10 @@ class Scene:
                 "scalarMacCormack": o["scalarMacCormack"]}
 
 
-def campfire(seed: int = 1, steps: int = 60) -> Scene:
+def campfire(seed: int = 1, steps: int = 60, smoke: int = 1) -> Scene:
     """Plain method, small: one spherical fuel+heat emitter r 0.3 at (0, 0.2, 0)."""
     sim = FireSimulation({"voxelSize": 0.05, "brickSize": 16, "velocityDivisor": 2, "seed": seed,
+                          "smokeDivisor": smoke,
                           "fuel": {"enabled": True, "ignitionHeat": 0.5, "burnRate": 4}})
     sim.add_emitter({"position": (0, 0.2, 0), "shape": {"radius": 0.3},
                      "emission": {"flame": 0, "heatRate": 10, "smokeRate": 0, "fuelRate": 2}})
@@ -116,10 +117,12 @@ def explosions(seed: int = 1, steps: int = 400) -> Scene:
 
 def small(seed: int = 1, steps: int = 12, brick: int = 8, divisor: int = 2,
           voxel: float = 0.05, collider: bool = True, turbulence: bool = True,
-          burst: bool = True) -> Scene:
+          burst: bool = True, smoke: int = 1) -> Scene:
     """A seconds-sized scene touching every stage: an emitter with fuel and velocity, a
-    smoke source, a burst with variation, a turbulence force and a box collider."""
+    smoke source, a burst with variation, a turbulence force and a box collider; `smoke` is
+    the smoke grid's divisor."""
     sim = FireSimulation({"voxelSize": voxel, "brickSize": brick, "velocityDivisor": divisor,
+                          "smokeDivisor": smoke,
                           "seed": seed, "fuel": {"enabled": True, "ignitionHeat": 0.5,
                                                  "burnRate": 4}})
     sim.add_fire({"position": (0.05, 0.3, 0.02), "shape": {"radius": 0.25},
@@ -139,7 +142,8 @@ def small(seed: int = 1, steps: int = 12, brick: int = 8, divisor: int = 2,
         if burst and k == 3:
             x.trigger((-0.4, 0.5, 0.3))
 
-    return Scene(f"small-b{brick}-d{divisor}", sim, steps, hook=hook)
+    suffix = f"-s{smoke}" if smoke > 1 else ""
+    return Scene(f"small-b{brick}-d{divisor}{suffix}", sim, steps, hook=hook)
 
 
 SCENES: Dict[str, Callable[..., Scene]] = {
