@@ -60,6 +60,8 @@ def parse():
                    help="with --serving: launch the batched kernels one by one (for kernel tracing)")
     p.add_argument("--serving-rate", type=float, default=20.0)
     p.add_argument("--explosions", action="store_true", help="the brick-churn workload instead")
+    p.add_argument("--presets", type=int, default=0, metavar="STEPS",
+                   help="headless runs of the shipped presets (tests/golden/presets), STEPS each")
     p.add_argument("--mix", type=int, default=48,
                    help="requests of the serving workload for the batched mixed-precision comparison "
                         "in the default line (0 skips it)")
@@ -185,6 +187,41 @@ def explosions_line(args) -> dict:
             "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
 
 
+def presets_lines(steps: int):
+    """Each shipped preset with sphere emitters, played headless (bursts fired at the start)
+    for `steps` fixed steps: whole-run throughput and the step time."""
+    import glob
+    from firepro import abi, document
+    from firepro.engine import FluidSimulation
+    here = os.path.join(ROOT, "tests", "golden", "presets")
+    for path in sorted(glob.glob(os.path.join(here, "*.json"))):
+        with open(path) as f:
+            doc = json.load(f)
+        try:
+            sc = document.preset_scene(doc, steps)
+        except abi.FireProError as e:
+            yield {"preset": doc["id"], "skipped": str(e)}
+            continue
+        frames = [(abi.PackedFrame(fr), g) for fr, g in sc.frames()]
+        sim = FluidSimulation.create(sc.grid())
+        voxels, peak, step_ms = 0, 0, []
+        t0 = time.perf_counter()
+        for frame, g in frames:
+            s0 = time.perf_counter()
+            sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+            sim.step(frame)
+            step_ms.append(1e3 * (time.perf_counter() - s0))
+            st = sim.stats()
+            voxels += st.active_voxels
+            peak = max(peak, st.active_bricks)
+        wall = time.perf_counter() - t0
+        sim.dispose()
+        yield {"preset": doc["id"], "metric": "active voxel-steps/s", "value": voxels / wall,
+               "steps": steps, "ms_per_step": 1e3 * wall / steps,
+               "p50_step_ms": float(np.percentile(step_ms, 50)), "p99_step_ms": float(np.percentile(step_ms, 99)),
+               "peak_active_bricks": peak, "grid": sc.grid()}
+
+
 def mix_comparison(count: int) -> dict:
     """The metric's comparison, on a sample of the serving workload: `count` requests of it
     present at once, each pressure solved to a relative residual of 1e-4 (at most 20
@@ -267,6 +304,10 @@ def main():
         return
     if args.explosions:
         print(json.dumps(explosions_line(args)))
+        return
+    if args.presets:
+        for line in presets_lines(args.presets):
+            print(json.dumps(line), flush=True)
         return
     from firepro import abi, scenes
     from firepro.engine import FluidSimulation

@@ -138,6 +138,62 @@ def resolve_simulation(options: Optional[dict] = None) -> dict:
     return out
 
 
+def _vector(value, label, lo=-1e6, hi=1e6):
+    """options.ts:306-310"""
+    if not isinstance(value, (list, tuple)) or len(value) != 3:
+        raise ValueError(f"{label} must contain three numbers.")
+    return [_number(v, f"{label}[{i}]", lo, hi) for i, v in enumerate(value)]
+
+
+def _direction(value, label):
+    """options.ts:311-318: normalized, keeping vectors already of unit length."""
+    v = _vector(value, label)
+    mag = math.hypot(*v)
+    if mag == 0:
+        raise ValueError(f"{label} must be nonzero.")
+    div = 1 if abs(mag - 1) < 1e-12 else mag
+    return [0 if c == 0 else c / div for c in v]
+
+
+def _boolean(value, label):
+    if not isinstance(value, bool):
+        raise ValueError(f"{label} must be a boolean.")
+    return value
+
+
+def resolve_force(options: dict) -> dict:
+    """resolveForce (options.ts:602-662) for a new force."""
+    if not isinstance(options, dict):
+        raise ValueError("force must be an options object.")
+    kind = options.get("type")
+    allowed = {"wind": ["direction", "strength", "active"],
+               "turbulence": ["strength", "scale", "active"],
+               "vortex": ["center", "axis", "strength", "radius", "lift", "inward", "active"]}.get(
+        kind, ["center", "strength", "radius", "active"])
+    _keys(options, ["type", *allowed], "force")
+    active = _boolean(options.get("active", True), "force.active")
+    if kind == "wind":
+        return {"type": kind, "active": active,
+                "direction": _direction(options.get("direction"), "wind.direction"),
+                "strength": _number(options.get("strength"), "wind.strength", *LIMITS["wind"]["strength"])}
+    if kind == "turbulence":
+        return {"type": kind, "active": active,
+                "strength": _number(options.get("strength"), "turbulence.strength",
+                                    *LIMITS["turbulence"]["strength"]),
+                "scale": _number(options.get("scale"), "turbulence.scale", *LIMITS["turbulence"]["scale"])}
+    if kind in ("vortex", "radial"):
+        out = {"type": kind, "active": active,
+               "center": _vector(options.get("center", [0, 0, 0]), "force.center"),
+               "strength": _number(options.get("strength"), "force.strength", *LIMITS[kind]["strength"]),
+               "radius": _number(options.get("radius", 1), "force.radius", *LIMITS[kind]["radius"])}
+        if kind == "vortex":
+            out.update(axis=_direction(options.get("axis", [0, 1, 0]), "vortex.axis"),
+                       lift=_number(options.get("lift", 0), "vortex.lift", *LIMITS["vortex"]["lift"]),
+                       inward=_number(options.get("inward", 0), "vortex.inward", *LIMITS["vortex"]["inward"]))
+        return out
+    raise ValueError("force.type must be wind, turbulence, vortex or radial.")
+
+
 # emitterDefaults (options.ts:480-502)
 EMITTER_KINDS = {
     "fire": {"radius": 0.38, "flame": 1, "heatRate": 1.5, "smokeRate": 0, "speed": 1.4},
@@ -319,7 +375,7 @@ class FireSimulation:
     def add_force(self, options: dict, targets=None):
         if targets is not None and any(t not in self._emitters for t in targets):
             raise ValueError("Force target must be a live emitter owned by this simulation.")
-        f = Force(options, targets)
+        f = Force(resolve_force(options), targets)
         self._forces.append(f)
         return f
 
