@@ -29,6 +29,7 @@ extern "C" {
 
 #define FP_SHAPE_SPHERE 0
 #define FP_SHAPE_BOX 1
+#define FP_SHAPE_MESH 2 /* sources only: emission from a mesh surface (MeshSourceRasterizer.ts) */
 
 #define FP_FORCE_WIND 0
 #define FP_FORCE_TURBULENCE 1
@@ -84,6 +85,12 @@ typedef struct fp_source {
   double velocity_response, outward_speed;
   int32_t has_variation;
   double variation_period, variation_strength;
+  /* FP_SHAPE_MESH: the mesh's transformed surface samples in world space, (x, y, z, weight)
+   * each, and its surface area in m^2 (MeshSurface.ts, handles.ts:160-183). Ignored for
+   * other shapes. */
+  const double *samples;
+  int32_t sample_count;
+  double surface_area;
 } fp_source;
 
 /* SimulationForce (src/engine/types.ts:284-293). `targeted` = sources given (even none). */

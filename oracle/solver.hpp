@@ -19,13 +19,14 @@
 // buffers: values are stored at the template precision, with no half-precision rounding
 // (truncateHalf, fluid-common.wgsl:107-114, and every f16 store); the activity readback is
 // applied before the next step (the earliest a mapAsync can resolve,
-// FluidSimulation.ts:1701-1746); mesh emitters/colliders/force targets and the
+// FluidSimulation.ts:1701-1746); mesh colliders, forces targeting mesh emitters and the
 // ambient-free debug exports are not restated.
 #pragma once
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -359,6 +360,12 @@ template <class R> class Sim {
 
   Params<R> P;
   std::vector<Emitter<R>> emitters;
+  // Mesh emission: records of 8 values behind a zero record 0 (flame, heat, smoke, fuel,
+  // motion xyz, response), stored as f32 like the reference's record buffer, and the brick
+  // tables (per brick id a table or -1, then per table a record per field cell).
+  std::vector<R> meshRecords{std::vector<R>(8, R(0))};
+  std::vector<int32_t> meshTables;
+  int meshCount = 0;
   std::vector<Collider<R>> colliders;
   std::vector<Force<R>> forces;
 
@@ -709,14 +716,27 @@ template <class R> class Sim {
     std::vector<BrickBox> sourceBricks;
     for (int i = 0; i < f.source_count; i++) {
       const fp_source &s = f.sources[i];
-      double extent = std::max(std::max(std::max(s.size[0], s.size[1]), s.size[2]), 0.03) +
-                      s.outward_speed * 0.25;
       double lo[3], hi[3];
-      for (int a = 0; a < 3; a++) {
-        lo[a] = s.position[a] - extent;
-        hi[a] = s.position[a] + extent;
+      if (s.shape == FP_SHAPE_MESH && s.samples && s.sample_count > 0) {
+        // The bounds of a mesh source's samples (FluidSimulation.ts:880-894).
+        for (int a = 0; a < 3; a++) {
+          lo[a] = INFINITY;
+          hi[a] = -INFINITY;
+          for (int k = 0; k < s.sample_count; k++) {
+            lo[a] = std::min(lo[a], s.samples[4 * (size_t)k + a]);
+            hi[a] = std::max(hi[a], s.samples[4 * (size_t)k + a]);
+          }
+        }
+        needBox(lo, hi, margin + voxel);
+      } else {
+        double extent = std::max(std::max(std::max(s.size[0], s.size[1]), s.size[2]), 0.03) +
+                        s.outward_speed * 0.25;
+        for (int a = 0; a < 3; a++) {
+          lo[a] = s.position[a] - extent;
+          hi[a] = s.position[a] + extent;
+        }
+        needBox(lo, hi, margin);
       }
-      needBox(lo, hi, margin);
       // bricksOver (FluidSimulation.ts:862-869)
       BrickBox bb;
       double size = B * voxel;
@@ -871,10 +891,95 @@ template <class R> class Sim {
 
   // packScene (FluidSimulation.ts:1150-1366) for sphere/box emitters and colliders and
   // analytic forces. Returns whether the colliders' packed words changed.
+  // MeshSourceRasterizer.prepare (MeshSourceRasterizer.ts:66-166): each sample spreads
+  // weight * area over its eight trilinear field cells in units of the cells' mean face
+  // area; a covered cell's mask min(1, coverage) scales the source's rates into it. Then
+  // the brick tables (FluidSimulation.ts:1329-1351).
+  void packMesh(const fp_frame &f) {
+    using Cell = std::array<int, 3>;
+    struct Acc {
+      double flame = 0, heat = 0, smoke = 0, fuel = 0, motion[3] = {0, 0, 0}, response = 0;
+    };
+    std::map<Cell, Acc> cells;
+    const double h = voxel, meanFace = (h * h + h * h + h * h) / 3;
+    size_t visits = 0;
+    for (int i = 0; i < f.source_count; i++) {
+      const fp_source &s = f.sources[i];
+      if (s.shape != FP_SHAPE_MESH) continue;
+      if (!s.samples || s.sample_count <= 0 || !(s.surface_area > 0))
+        throw Error("Mesh sources require surface samples and a positive surface area.");
+      visits += (size_t)s.sample_count * 8;
+      if (visits > 2000000) throw Error("Mesh source sampling exceeds two million cell visits.");
+      std::map<Cell, double> coverage;
+      for (int k = 0; k < s.sample_count; k++) {
+        const double *p = s.samples + 4 * (size_t)k;
+        double g[3], fr[3];
+        int base[3];
+        for (int a = 0; a < 3; a++) {
+          g[a] = p[a] / h - 0.5;
+          base[a] = (int)std::floor(g[a]);
+          fr[a] = g[a] - base[a];
+        }
+        for (int z = 0; z < 2; z++)
+          for (int y = 0; y < 2; y++)
+            for (int x = 0; x < 2; x++) {
+              double t = (x ? fr[0] : 1 - fr[0]) * (y ? fr[1] : 1 - fr[1]) * (z ? fr[2] : 1 - fr[2]);
+              coverage[Cell{base[0] + x, base[1] + y, base[2] + z}] += p[3] * s.surface_area * t / meanFace;
+            }
+      }
+      for (const auto &cv : coverage) {
+        double mask = std::min(1.0, cv.second);
+        if (mask <= 0) continue;
+        Acc &c = cells[cv.first];
+        c.flame = std::max(c.flame, s.flame * mask);
+        c.heat += s.heat_rate * mask;
+        c.smoke += s.smoke_rate * mask;
+        c.fuel += s.fuel_rate * mask;
+        double response = s.velocity_response * mask;
+        for (int a = 0; a < 3; a++) c.motion[a] += s.velocity[a] * response;
+        c.response += response;
+      }
+    }
+    meshRecords.assign(8, R(0));
+    meshCount = (int)cells.size();
+    std::map<int, int> tables;
+    std::vector<std::pair<size_t, int>> placed;
+    int record = 1;
+    for (const auto &c : cells) {
+      const double v[8] = {c.second.flame, c.second.heat, c.second.smoke, c.second.fuel,
+                           c.second.motion[0], c.second.motion[1], c.second.motion[2], c.second.response};
+      for (double x : v) meshRecords.push_back((R)(float)x);
+      const Cell &q = c.first;
+      int id = brickIdOf(q[0] >> sF, q[1] >> sF, q[2] >> sF);
+      if (id >= 0) {
+        auto t = tables.emplace(id, (int)tables.size()).first->second;
+        int m = B - 1;
+        placed.push_back({(size_t)t * nF + (size_t)(((q[2] & m) * B + (q[1] & m)) * B + (q[0] & m)), record});
+      }
+      record++;
+    }
+    meshTables.assign((size_t)brickCount() + tables.size() * (size_t)nF, 0);
+    std::fill(meshTables.begin(), meshTables.begin() + brickCount(), -1);
+    for (const auto &t : tables) meshTables[t.first] = t.second;
+    for (const auto &p : placed) meshTables[(size_t)brickCount() + p.first] = p.second;
+  }
+  // meshSourceAt (fluid-sources.wgsl:8-22): the record of the field cell at `w`
+  // (fieldCell, solids.wgsl:9-11), 0 for none.
+  int meshRecordAt(const R w[3]) const {
+    int c[3];
+    for (int a = 0; a < 3; a++) c[a] = (int)ffloor(w[a] * P.invHf + R(0.25));
+    int id = brickIdOf(c[0] >> sF, c[1] >> sF, c[2] >> sF);
+    if (id < 0 || meshTables[id] < 0) return 0;
+    int m = B - 1;
+    return meshTables[(size_t)brickCount() + (size_t)meshTables[id] * nF + ((c[2] & m) * B + (c[1] & m)) * B + (c[0] & m)];
+  }
+
   bool packScene(const fp_frame &f) {
+    packMesh(f);
     emitters.clear();
     for (int i = 0; i < f.source_count; i++) {
       const fp_source &s = f.sources[i];
+      if (s.shape == FP_SHAPE_MESH) continue;
       Emitter<R> e;
       for (int a = 0; a < 3; a++) {
         e.pos[a] = (R)(float)s.position[a];
@@ -1073,7 +1178,7 @@ template <class R> class Sim {
               int nb = brickIdOf(bk[0] + x, bk[1] + y, bk[2] + z);
               if (nb < 0) continue;
               int toward = (-z + 1) * 9 + (-y + 1) * 3 + (-x + 1);
-              listed = ((activity[nb] >> 3) & (1u << toward)) != 0u;
+              listed = ((activity[nb] >> 3) & (1u << toward)) != 0u || (meshCount > 0 && meshTables[nb] >= 0);
             }
         if (open && !listed) listed = sourceNear(bk);
         int page = pages[index];
@@ -1523,6 +1628,13 @@ template <class R> class Sim {
           sp[1] += e.heat * var[0] * wgt * P.dt;
           injectedFuel += e.fuel * wgt * P.dt;
         }
+        if (meshCount > 0) {
+          const R *mesh = &meshRecords[8 * (size_t)meshRecordAt(pos)];
+          lifetime = fmax(lifetime, mesh[0]);
+          sp[0] += mesh[2] * P.dt;
+          sp[1] += mesh[1] * P.dt;
+          injectedFuel += mesh[3] * P.dt;
+        }
         R remaining = 0, burned = 0;
         if (P.fuelEnabled) {
           remaining = tr[3] + injectedFuel;
@@ -1911,6 +2023,10 @@ template <class R> class Sim {
         radial = dd[comp] / fmax(length3(dd[0], dd[1], dd[2]), R(1e-6)) * e.outward;
       }
       result = mixv(result, e.vel[comp] + radial, R(1) - fexp(-e.response * wgt * P.dt));
+    }
+    if (meshCount > 0) {
+      const R *motion = &meshRecords[8 * (size_t)meshRecordAt(face) + 4];
+      if (motion[3] > R(0)) result = mixv(result, motion[comp] / motion[3], R(1) - fexp(-motion[3] * P.dt));
     }
     return result;
   }

@@ -13,7 +13,7 @@ import pytest
 from firepro import abi, document
 
 PRESETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "presets")
-MESH_SHAPED = {"campfire", "flaming-letters"}
+TEXT_SHAPED = {"flaming-letters"}
 
 
 def presets():
@@ -69,11 +69,12 @@ def test_validation_errors(mutate, message):
         document.validate_document(doc)
 
 
-def test_playback_builds_sphere_presets():
-    """Sphere presets build; bursts fire in the first frame only; emitters keep their
-    document positions; mesh-shaped emitters are refused."""
+def test_playback_builds_presets():
+    """Every preset but the text one builds; bursts fire in the first frame only; emitters
+    keep their document positions; the campfire's disk is a mesh source with its surface
+    samples; letter emitters are refused."""
     for name, doc in presets().items():
-        if name in MESH_SHAPED:
+        if name in TEXT_SHAPED:
             with pytest.raises(abi.FireProError, match="mesh surface"):
                 document.Playback(doc)
             continue
@@ -88,7 +89,13 @@ def test_playback_builds_sphere_presets():
         assert len(second.sources) == len(steady)
         for e, src in zip(steady, first.sources):
             assert tuple(src.position) == tuple(e["position"])
-            assert src.size == (e["radius"],) * 3
+            if e["shape"] == "sphere":
+                assert src.size == (e["radius"],) * 3 and src.samples is None
+            else:
+                assert src.shape == "mesh" and src.size == (d["simulation"]["voxelSize"],) * 3
+                area = 0.5 * 40 * e["radius"] ** 2 * math.sin(2 * math.pi / 40)  # the 40-gon
+                assert src.surface_area == pytest.approx(area, rel=1e-6)
+                assert len(src.samples) == min(8192, max(256, math.ceil(area / 0.05 ** 2 * 8)))
         for e, src in zip(bursts, first.sources[len(steady):]):
             assert tuple(src.position) == tuple(e["position"])
             assert src.heat_rate == pytest.approx(e["burst"]["charge"]["heat"] * 60)

@@ -67,6 +67,14 @@ def test_coarse_smoke_refuses_render_and_batch():
         gpu.dispose()
 
 
+@pytest.mark.parametrize("smoke", [1, 2])
+def test_mesh_emitters_bit_exact(smoke):
+    """Mesh emitters (box, torus, disk): the rasterized records and brick tables drive
+    allocation, emission and the source velocity identically on both sides."""
+    print(f"case meshes smoke={smoke}", flush=True)
+    assert run_lockstep(_scenes().meshes(steps=10, smoke=smoke)) == []
+
+
 def test_first_order_scalars_bit_exact():
     sc = _scenes().small(steps=4)
     sc.sim.options["scalarMacCormack"] = False

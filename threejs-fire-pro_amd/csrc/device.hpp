@@ -66,6 +66,7 @@ struct Params {
   int coarse;
   float hs, invHs;
   int smokeRatio;
+  int meshRecords;  // mesh emission records this step (0: no mesh sources)
 };
 
 // Geometry of the grids and pools (constant per simulation).

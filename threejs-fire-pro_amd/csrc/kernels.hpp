@@ -23,6 +23,7 @@ struct AllocArgs {
   int colliderCount;
   int buildSolids;  // draw the colliders into solid bits this step
   int solidsEvery;  // into every active slot's, else the fresh slots'
+  const int *meshTables = nullptr;  // per brick id its mesh emission table or -1; null without mesh sources
   Pools p;
 };
 
@@ -39,6 +40,11 @@ struct StepArgs {
   const Emitter *emitters;
   const ForceRec *forces;
   const Region *regions;
+  // Mesh emission (fluid-sources.wgsl:4-22): per brick id a table index or -1, then per
+  // table a record index per field cell; records are (flame, heat, smoke, fuel) and
+  // (motion xyz, response), record 0 all zero. Null without mesh sources.
+  const int *meshTables = nullptr;
+  const float4 *meshRecords = nullptr;
   const float *pressure = nullptr;    // the solved pressure the projection reads
   const float *pressureLo = nullptr;  // its low part when it is a pair hi + lo (defect correction), else null
 };

@@ -80,7 +80,7 @@ __device__ __forceinline__ void build_bricks_body(const AllocArgs &a, const int 
             if (ntile < 0) continue;
             int nid = ntile * 64 + (((nb[2] & 3) * 4 + (nb[1] & 3)) * 4 + (nb[0] & 3));
             int toward = (-z + 1) * 9 + (-y + 1) * 3 + (-x + 1);
-            listed = ((a.activity[nid] >> 3) & (1u << toward)) != 0u;
+            listed = ((a.activity[nid] >> 3) & (1u << toward)) != 0u || (a.meshTables && a.meshTables[nid] >= 0);
           }
       if (open && !listed) listed = sourceNear(a, b);
       int page = a.pages[index];

@@ -371,6 +371,19 @@ __device__ __forceinline__ void evaluateFlame(const Params &P, const Ramps &R, f
   prod[2] = P.fexpansion * (P.lifespan * rampIntegral(R, 2, after, before) / P.fdt);
 }
 
+// meshSourceAt (fluid-sources.wgsl:8-22): the index of the mesh emission record of the field
+// cell at world position w (fieldCell, solids.wgsl:9-11), 0 (the zero record) for none.
+__device__ __forceinline__ int meshRecordAt(const StepArgs &s, const float w[3]) {
+  const int x = (int)floorf(w[0] * s.P.invHf + 0.25f), y = (int)floorf(w[1] * s.P.invHf + 0.25f),
+            z = (int)floorf(w[2] * s.P.invHf + 0.25f);
+  const int sh = s.L.sF, m = s.L.B - 1;
+  const int id = brickIdOf(s.dir, x >> sh, y >> sh, z >> sh);
+  if (id < 0) return 0;
+  const int table = s.meshTables[id];
+  if (table < 0) return 0;
+  return s.meshTables[s.brickCount + table * s.L.nF + ((z & m) * s.L.B + (y & m)) * s.L.B + (x & m)];
+}
+
 // contentSupport (fluid-content.wgsl:29-39) on a grid of bricks `side` cells a side,
 // velocity cells `ratio` of its cells wide and cells 1 / invH wide.
 __device__ __forceinline__ uint32_t supportOn(int side, float ratio, float invH, float dt, const int id[3],
@@ -461,6 +474,13 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
         sp0 += e.smoke * wgt * P.dt;
         sp1 += e.heat * var[0] * wgt * P.dt;
         injectedFuel += e.fuel * wgt * P.dt;
+      }
+      if (P.meshRecords) {
+        const float4 mesh = s.meshRecords[2 * meshRecordAt(s, pos)];
+        lifetime = fmaxv(lifetime, mesh.x);
+        sp0 += mesh.z * P.dt;
+        sp1 += mesh.y * P.dt;
+        injectedFuel += mesh.w * P.dt;
       }
       float remaining = 0.f, burned = 0.f;
       if (P.fuelEnabled) {
@@ -712,6 +732,13 @@ __device__ __forceinline__ float sourceVelocity(const StepArgs &s, float value, 
     }
     float vc = comp == 0 ? e.vel[0] : comp == 1 ? e.vel[1] : e.vel[2];
     result = mixv(result, vc + radial, 1.0f - fexp(-e.response * wgt * s.P.dt));
+  }
+  if (s.P.meshRecords) {
+    const float4 motion = s.meshRecords[2 * meshRecordAt(s, face) + 1];
+    if (motion.w > 0.f) {
+      const float mc = comp == 0 ? motion.x : comp == 1 ? motion.y : motion.z;
+      result = mixv(result, mc / motion.w, 1.0f - fexp(-motion.w * s.P.dt));
+    }
   }
   return result;
 }

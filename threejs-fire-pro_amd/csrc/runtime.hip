@@ -26,6 +26,7 @@
 #include "host_directory.hpp"
 #include "kernels.hpp"
 #include "render_host.hpp"
+#include "mesh_host.hpp"
 
 namespace fp {
 
@@ -415,6 +416,14 @@ struct fp_sim {
   const float4 *noise = nullptr;  // the shared curl-noise table
   DBuf<float> E, V, Vadv, Vcor, Vf, D, P0, P1;
   DBuf<float> S, Sp, Sn;  // coarse smoke density, predictor and next (smokeDivisor 2 or 4)
+  // Mesh emission: the records (8 floats each behind a zero record) and the brick tables,
+  // with what they were built from.
+  DBuf<float> meshRecs;
+  DBuf<int> meshTables;
+  mesh::Records meshHost;
+  std::vector<int32_t> meshTablesHost;
+  std::vector<double> meshKey;
+  int meshTablesVersion = -1, meshBricks = -1, tilesVersion = 0;
   DBuf<uint32_t> solids;
   struct LevelBufs {
     DBuf<float> p, q, rhs;
@@ -739,6 +748,7 @@ struct fp_sim {
 
   // applyTiles (FluidSimulation.ts:663-737).
   void applyTiles() {
+    tilesVersion++;
     int needed = std::max(MIN_TILES, dir.slotsUsed());
     if (needed > capacity) {
       int cap = capacity;
@@ -818,13 +828,27 @@ struct fp_sim {
     std::vector<render::BrickBox> sourceBricks;
     for (int i = 0; i < f.source_count; i++) {
       const fp_source &s = f.sources[i];
-      double extent = std::max(std::max(std::max(s.size[0], s.size[1]), s.size[2]), 0.03) + s.outward_speed * 0.25;
       double lo[3], hi[3];
-      for (int a = 0; a < 3; a++) {
-        lo[a] = s.position[a] - extent;
-        hi[a] = s.position[a] + extent;
+      if (s.shape == FP_SHAPE_MESH && s.samples && s.sample_count > 0) {
+        // A mesh source needs the bounds of its samples, a cell past the margin.
+        for (int a = 0; a < 3; a++) {
+          lo[a] = INFINITY;
+          hi[a] = -INFINITY;
+        }
+        for (int k = 0; k < s.sample_count; k++)
+          for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], s.samples[4 * (size_t)k + a]);
+            hi[a] = std::max(hi[a], s.samples[4 * (size_t)k + a]);
+          }
+        needBox(lo, hi, margin + voxel);
+      } else {
+        double extent = std::max(std::max(std::max(s.size[0], s.size[1]), s.size[2]), 0.03) + s.outward_speed * 0.25;
+        for (int a = 0; a < 3; a++) {
+          lo[a] = s.position[a] - extent;
+          hi[a] = s.position[a] + extent;
+        }
+        needBox(lo, hi, margin);
       }
-      needBox(lo, hi, margin);
       render::BrickBox bb;  // bricksOver (FluidSimulation.ts:862-869)
       for (int a = 0; a < 3; a++) {
         bb.lo[a] = (int)std::floor(lo[a] / (B * voxel));
@@ -901,10 +925,50 @@ struct fp_sim {
   std::vector<Collider> hostColliders;
   std::vector<ForceRec> hostForces;
   std::vector<Region> hostRegions;
+  // Mesh emission records and brick tables (FluidSimulation.ts:1318-1352), rebuilt when the
+  // mesh sources or the tiles change.
+  void packMesh(const fp_frame &f) {
+    std::vector<double> key;
+    for (int i = 0; i < f.source_count; i++) {
+      const fp_source &s = f.sources[i];
+      if (s.shape != FP_SHAPE_MESH) continue;
+      key.insert(key.end(), {(double)s.sample_count, s.surface_area, s.flame, s.heat_rate, s.smoke_rate, s.fuel_rate,
+                             s.velocity[0], s.velocity[1], s.velocity[2], s.velocity_response});
+      if (s.samples && s.sample_count > 0) key.insert(key.end(), s.samples, s.samples + 4 * (size_t)s.sample_count);
+    }
+    bool records = key != meshKey;
+    if (records) {
+      try {
+        meshHost = mesh::rasterize(f, voxel);
+      } catch (const std::invalid_argument &e) {
+        throw InvalidArgument(e.what());
+      }
+      meshKey = key;
+      upload(meshRecs, meshHost.data);
+    }
+    if (meshHost.count() == 0) {
+      meshTablesVersion = -1;
+      return;
+    }
+    if (records || meshTablesVersion != tilesVersion || meshBricks != brickCount()) {
+      meshTablesHost = mesh::tables(meshHost, brickCount(), sF, [&](int bx, int by, int bz) {
+        int slot = dir.slotAt(bx >> 2, by >> 2, bz >> 2);
+        return slot < 0 ? -1 : slot * 64 + ((bz & 3) * 4 + (by & 3)) * 4 + (bx & 3);
+      });
+      upload(meshTables, meshTablesHost);
+      meshTablesVersion = tilesVersion;
+      meshBricks = brickCount();
+    }
+  }
+
   bool packScene(const fp_frame &f) {
+    packMesh(f);
     hostEmitters.clear();
     for (int i = 0; i < f.source_count; i++) {
       const fp_source &s = f.sources[i];
+      if (s.shape == FP_SHAPE_MESH) continue;  // emits through the mesh tables
+      if (s.shape != FP_SHAPE_SPHERE && s.shape != FP_SHAPE_BOX)
+        throw InvalidArgument("source.shape must be sphere, box or mesh.");
       Emitter e;
       for (int a = 0; a < 3; a++) {
         e.pos[a] = (float)s.position[a];
@@ -1001,6 +1065,7 @@ struct fp_sim {
     P.hs = (float)(voxel * smokeDiv);
     P.invHs = 1.0f / P.hs;
     P.smokeRatio = smokeRatio;
+    P.meshRecords = meshHost.count();
     P.dt = (float)dt;
     P.buoyancy = (float)s.buoyancy;
     P.cooling = (float)s.cooling;
@@ -1059,6 +1124,7 @@ struct fp_sim {
     a.colliderCount = colliderCount;
     a.buildSolids = (colliderCount > 0 || solidsStale) ? 1 : 0;
     a.solidsEvery = solidsStale ? 1 : 0;
+    a.meshTables = meshHost.count() ? meshTables.p : nullptr;
     a.p = pools();
     return a;
   }
@@ -1096,6 +1162,8 @@ struct fp_sim {
     s.emitters = emitters.p;
     s.forces = forces.p;
     s.regions = regions.p;
+    s.meshTables = meshHost.count() ? meshTables.p : nullptr;
+    s.meshRecords = meshHost.count() ? reinterpret_cast<const float4 *>(meshRecs.p) : nullptr;
     return s;
   }
   MGArgs mgArgs() {

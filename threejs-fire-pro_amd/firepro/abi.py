@@ -11,11 +11,13 @@ import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(HERE, "libfirepro.so")
 
 FP_OK, FP_EINVAL, FP_EDEVICE, FP_ESTATE, FP_ENOMEM = 0, -1, -2, -3, -4
-SHAPES = {"sphere": 0, "box": 1}
+SHAPES = {"sphere": 0, "box": 1, "mesh": 2}
 FORCE_TYPES = {"wind": 0, "turbulence": 1, "vortex": 2, "radial": 3}
 MAX_FLAME_KNOTS = 8
 BRICK_RECORD_WORDS = 28
@@ -64,6 +66,8 @@ class Source(C.Structure):
         ("velocity_response", C.c_double), ("outward_speed", C.c_double),
         ("has_variation", C.c_int32),
         ("variation_period", C.c_double), ("variation_strength", C.c_double),
+        ("samples", C.POINTER(C.c_double)), ("sample_count", C.c_int32),
+        ("surface_area", C.c_double),
     ]
 
 
@@ -299,6 +303,9 @@ class SourceData:
     velocity_response: float = 0.0
     outward_speed: float = 0.0
     variation: Optional[Sequence[float]] = None  # (period, strength)
+    # Mesh sources: world-space surface samples, rows (x, y, z, weight), and the surface area.
+    samples: Optional[np.ndarray] = None
+    surface_area: float = 0.0
 
 
 @dataclass
@@ -341,7 +348,7 @@ class FrameData:
     colliders: List[ColliderData] = field(default_factory=list)
 
 
-def _source(s: SourceData) -> Source:
+def _source(s: SourceData, keep: Optional[list] = None) -> Source:
     out = Source()
     out.id = s.id
     out.shape = SHAPES[s.shape]
@@ -354,6 +361,13 @@ def _source(s: SourceData) -> Source:
     if s.variation is not None:
         out.has_variation = 1
         out.variation_period, out.variation_strength = s.variation
+    if s.samples is not None:
+        arr = np.ascontiguousarray(s.samples, np.float64).reshape(-1, 4)
+        if keep is not None:
+            keep.append(arr)
+        out.samples = arr.ctypes.data_as(C.POINTER(C.c_double))
+        out.sample_count = len(arr)
+        out.surface_area = s.surface_area
     return out
 
 
@@ -377,7 +391,8 @@ class PackedFrame:
         if f.fuel is not None:
             fr.has_fuel = 1
             fr.fuel = Fuel(int(bool(f.fuel[0])), f.fuel[1], f.fuel[2])
-        self.sources = (Source * max(1, len(f.sources)))(*[_source(s) for s in f.sources])
+        self.samples = []  # the mesh samples the sources point into
+        self.sources = (Source * max(1, len(f.sources)))(*[_source(s, self.samples) for s in f.sources])
         fr.source_count = len(f.sources)
         fr.sources = self.sources
         self.force_sources = []
@@ -390,7 +405,7 @@ class PackedFrame:
             fo.strength, fo.scale, fo.lift, fo.inward = fd.strength, fd.scale, fd.lift, fd.inward
             if fd.sources is not None:
                 fo.targeted = 1
-                arr = (Source * max(1, len(fd.sources)))(*[_source(s) for s in fd.sources])
+                arr = (Source * max(1, len(fd.sources)))(*[_source(s, self.samples) for s in fd.sources])
                 self.force_sources.append(arr)
                 fo.source_count = len(fd.sources)
                 fo.sources = arr

@@ -9,8 +9,9 @@ forces with their target emitters, and the colliders. The editor's recipes (scen
 the moving emitter's wander, the projectiles' launcher) belong to the editor, not to the
 document, and are not run.
 
-Emitter shapes other than a sphere (box, disk, torus, letters) emit from a mesh surface
-(MeshSourceRasterizer.ts); they are not built here, and `Playback` raises for them.
+Box, disk and torus emitters emit from the surface of the mesh playback builds for them
+(firepro.mesh). Letter emitters need the editor's font geometry, which is not built, and
+`Playback` raises for them.
 """
 from __future__ import annotations
 
@@ -21,7 +22,7 @@ from typing import Dict, Iterator, List, Optional, Tuple
 
 import numpy as np
 
-from . import abi
+from . import abi, mesh
 from .library import LIMITS, FireSimulation, _direction, _number, resolve_force, resolve_simulation
 
 FORMAT, VERSION = "fire-pro", 11
@@ -220,12 +221,17 @@ class Playback:
                 x = self.simulation.add_explosion(source["burst"])
                 self.explosions.append((x, list(source["position"])))
                 continue
-            if source["shape"] != "sphere":
+            if source["shape"] == "letters":
                 raise abi.FireProError(
-                    f"emitter {source['id']}: {source['shape']} emitters emit from a mesh surface, "
-                    "which is not supported (sphere emitters only).")
-            opts = dict(source["options"], shape={"radius": source["radius"]},
-                        position=list(source["position"]))
+                    f"emitter {source['id']}: letters emitters emit from a mesh surface of text "
+                    "geometry, which is not supported.")
+            if source["shape"] == "sphere":
+                shape = {"radius": source["radius"]}
+            else:
+                shape = {"type": "mesh", "geometry": mesh.document_geometry(
+                    source["shape"], source["radius"], source["size"])}
+            opts = dict(source["options"], shape=shape, position=list(source["position"]),
+                        rotation=list(source["rotation"]))
             self.emitters[source["id"]] = self.simulation.add_emitter(opts)
         for force in d["forces"]:
             targets = None if force["targets"] is None else [self.emitters[t] for t in force["targets"]]

@@ -204,16 +204,34 @@ EMITTER_KINDS = {
 
 
 class Emitter:
-    """A continuous sphere emitter (handles.ts:81-226)."""
+    """A continuous emitter (handles.ts:81-198): a sphere, or a mesh surface given as
+    `{"type": "mesh", "geometry": mesh.Geometry}` and placed by `position` and `rotation`
+    (Euler XYZ, radians), sampled once at creation (MeshSurface.ts)."""
 
     def __init__(self, sim: "FireSimulation", hid: int, kind: str, options: dict):
         d = EMITTER_KINDS[kind]
         self.id = hid
         self.position = list(options.get("position", (0.0, 0.0, 0.0)))
+        self.rotation = list(options.get("rotation", (0.0, 0.0, 0.0)))
         self.active = options.get("active", True)
         shape = options.get("shape", {})
-        self.radius = _number(shape.get("radius", d["radius"]), "shape.radius",
-                              *LIMITS["emitter"]["radius"])
+        self.surface = None
+        self._placed = None
+        if shape.get("type", "sphere") == "mesh":
+            from . import mesh
+            if "radius" in shape:
+                raise ValueError("Mesh source does not accept radius.")
+            geometry = shape.get("geometry")
+            if not isinstance(geometry, mesh.Geometry):
+                raise ValueError("shape.geometry must be a mesh.Geometry.")
+            cell = sim.options["voxelSize"]
+            self.surface = mesh.MeshSurface(geometry, cell, (int(sim.options["seed"]) + hid) & 0xFFFFFFFF)
+            self.radius = cell
+        elif shape.get("type", "sphere") != "sphere":
+            raise ValueError("shape.type must be sphere or mesh.")
+        else:
+            self.radius = _number(shape.get("radius", d["radius"]), "shape.radius",
+                                  *LIMITS["emitter"]["radius"])
         em = options.get("emission", {})
         self.emission = {
             "flame": _number(em.get("flame", d["flame"]), "emission.flame", *LIMITS["emitter"]["flame"]),
@@ -238,17 +256,29 @@ class Emitter:
         self._sim = sim
 
     def source(self, include_inactive=False) -> Optional[abi.SourceData]:
-        """Emitter.source (handles.ts:147-197)."""
+        """Emitter.source (handles.ts:147-197): the world transform turns the velocity
+        direction (normalized again) and carries the mesh samples, which are transformed
+        again only when the pose changes."""
+        from . import mesh
         if not self.active and not include_inactive:
             return None
+        matrix = mesh.compose(self.position, self.rotation)
         v = [0.0, 0.0, 0.0]
         if self.velocity:
-            v = [c * self.velocity["speed"] for c in self.velocity["direction"]]
+            v = [c * self.velocity["speed"] for c in mesh.transform_direction(self.velocity["direction"], matrix)]
+        samples, area = None, 0.0
+        if self.surface is not None:
+            pose = (tuple(self.position), tuple(self.rotation))
+            if self._placed is None or self._placed[0] != pose:
+                self._placed = (pose, mesh.transform_samples(self.surface.samples, matrix))
+            samples, area = self._placed[1], self.surface.surface_area
         return abi.SourceData(
-            id=self.id, shape="sphere", position=tuple(self.position), size=(self.radius,) * 3,
+            id=self.id, shape="mesh" if self.surface is not None else "sphere",
+            position=tuple(float(x) for x in matrix[:3, 3]), size=(self.radius,) * 3,
             flame=self.emission["flame"], heat_rate=self.emission["heatRate"],
             smoke_rate=self.emission["smokeRate"], fuel_rate=self.emission["fuelRate"],
-            velocity=tuple(v), velocity_response=60 if self.velocity else 0, outward_speed=0)
+            velocity=tuple(v), velocity_response=60 if self.velocity else 0, outward_speed=0,
+            samples=samples, surface_area=area)
 
     def remove(self):
         self.removed = True

@@ -146,6 +146,26 @@ def small(seed: int = 1, steps: int = 12, brick: int = 8, divisor: int = 2,
     return Scene(f"small-b{brick}-d{divisor}{suffix}", sim, steps, hook=hook)
 
 
+def meshes(seed: int = 1, steps: int = 12, smoke: int = 1) -> Scene:
+    """Mesh emitters of the three shapes presets use: a tilted box blowing flame and fuel
+    sideways, a torus of heat and smoke, and a disk of flame on the ground, with turbulence."""
+    from . import mesh
+    sim = FireSimulation({"voxelSize": 0.05, "brickSize": 8, "velocityDivisor": 2, "smokeDivisor": smoke,
+                          "seed": seed, "fuel": {"enabled": True, "ignitionHeat": 0.5, "burnRate": 4}})
+    sim.add_emitter({"shape": {"type": "mesh", "geometry": mesh.box(0.4, 0.1, 0.3)},
+                     "position": (0.3, 0.35, 0.0), "rotation": (0.2, 0.5, 0.0),
+                     "emission": {"flame": 1, "heatRate": 2, "fuelRate": 1},
+                     "velocity": {"direction": (1, 1, 0), "speed": 2}})
+    sim.add_emitter({"shape": {"type": "mesh", "geometry": mesh.document_geometry("torus", 0.25, (1, 1, 1))},
+                     "position": (-0.4, 0.45, 0.1), "emission": {"heatRate": 3, "smokeRate": 2}})
+    sim.add_emitter({"shape": {"type": "mesh", "geometry": mesh.document_geometry("disk", 0.3, (1, 1, 1))},
+                     "position": (0.0, 0.15, -0.4), "emission": {"flame": 1, "heatRate": 1.5}})
+    sim.add_force({"type": "turbulence", "strength": 1.5, "scale": 2})
+    suffix = f"-s{smoke}" if smoke > 1 else ""
+    return Scene(f"meshes{suffix}", sim, steps)
+
+
 SCENES: Dict[str, Callable[..., Scene]] = {
     "campfire": campfire, "four_fires": four_fires, "explosions": explosions, "small": small,
+    "meshes": meshes,
 }
