@@ -50,8 +50,7 @@ typedef struct fp_grid {
   int32_t velocity_divisor; /* 1, 2 or 4 */
   int32_t brick_size;       /* 8, 16 or 32 */
   int32_t smoke_divisor;    /* 1, 2 or 4: smoke on its own grid, divisor x voxel wide (smoke.wgsl);
-                               2 and 4 are not rendered (fp_sim_render, fp_sim_scene_lights)
-                               nor batched (fp_batch_step) */
+                               2 and 4 are not batched (fp_batch_step) */
   int32_t scalar_maccormack;/* 1: bounded MacCormack scalars; 0: first order */
 } fp_grid;
 
@@ -198,6 +197,9 @@ typedef struct fp_render_tables {
   int32_t box_capacity, page_capacity, brick_capacity;
   int32_t box_count, page_count, brick_count;
   int32_t light_work; /* 1 every listed brick lit, 2 only slots given this step, 0 none, -1 idle */
+  /* smoke_divisor 2 or 4: each listed brick's smoothed coarse smoke, ((B/smoke_divisor)+1)^3
+   * floats with apron (smoothCoarseSmoke); NULL skips it */
+  float *smoke;
 } fp_render_tables;
 
 typedef struct fp_sim fp_sim;

@@ -200,7 +200,6 @@ template <class R> struct Impl : Base {
     int ld = rs.lighting_divisor;
     if (ld != 1 && ld != 2 && ld != 4) throw oracle::Error("lightingDivisor must be 1, 2 or 4.");
     if (sim.B / ld < 1) throw oracle::Error("lightingDivisor exceeds the brick size.");
-    if (sim.coarse()) throw oracle::Error("rendering with smokeDivisor 2 or 4 is not supported.");
   }
   int render(const fp_camera &cam, const fp_render_settings &rs, float *rgba) override {
     checkRender(rs);
@@ -243,6 +242,10 @@ template <class R> struct Impl : Base {
           if (t->field) std::copy(&cache.R[s * nR], &cache.R[s * nR] + nR, t->field + n * nR);
           if (t->light) std::copy(&cache.L[s * nL], &cache.L[s * nL] + nL, t->light + n * nL);
           if (t->occupancy) std::copy(&cache.O[(size_t)s * nO], &cache.O[(size_t)s * nO] + nO, t->occupancy + (size_t)n * nO);
+          if (t->smoke && sim.coarse()) {
+            size_t nRS = (size_t)(sim.bs + 1) * (sim.bs + 1) * (sim.bs + 1);
+            std::copy(&cache.RS[s * nRS], &cache.RS[s * nRS] + nRS, t->smoke + n * nRS);
+          }
         }
         n++;
       }

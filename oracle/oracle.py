@@ -105,6 +105,7 @@ class OracleSimulation:
         self.brick_size = brick_size
         self.brick_cells = brick_size ** 3
         self.velocity_cells = (brick_size // velocity_divisor) ** 3
+        self.smoke_divisor = smoke_divisor
         self.smoke_cells = (brick_size // smoke_divisor) ** 3
 
     @classmethod
@@ -252,7 +253,8 @@ class OracleSimulation:
         def call(t):
             if self._lib.orc_read_render(self._h, C.byref(t)) != 0:
                 raise RuntimeError("render tables need a single-precision oracle")
-        return abi.read_render_tables(call, self.brick_size, self._render[1].lighting_divisor)
+        return abi.read_render_tables(call, self.brick_size, self._render[1].lighting_divisor,
+                                      self.smoke_divisor)
 
     def probe(self, what: int, values) -> np.ndarray:
         """A single function of the render restatement on the latest render's state:

@@ -16,8 +16,10 @@
 // reference's one-texel aprons, because their apron values are computed, not copied.
 // Deliberate differences: no half-precision stores (the reference keeps rgba16float
 // render fields, light and blackbody texels); pow(x, 1.5) is x * sqrt(x); exp is fexp
-// and exp2 is fexp2; full-resolution smoke only (smokeDivisor 1), no half-resolution
-// slices, no scene depth and no debug views.
+// and exp2 is fexp2; no half-resolution slices, no scene depth and no debug views. With
+// coarse smoke, the scene lights' sample at a lattice point reads the logical neighbor
+// cells, where the reference's in-slot texture fetch can reach the atlas texel before the
+// slot (brick 8 with smokeDivisor 4).
 #pragma once
 #include <algorithm>
 #include <array>
@@ -176,10 +178,22 @@ struct Renderer {
   const std::vector<float> &lut;
   std::vector<float> &R, &L;
   std::vector<uint8_t> &O;
+  // Coarse smoke (smokeDivisor 2 or 4): the smoothed smoke with one-texel aprons,
+  // (bs+1)^3 floats per slot, and the smoke lattice's brick edge, shift and cells.
+  std::vector<float> &RS;
+  bool coarse;
+  int bs, sS, nS, ES, nRS;
 
   Renderer(const Sim<float> &s, const RenderParams &p, std::vector<RBox> &bx, std::vector<int32_t> &tb,
-           const std::vector<float> &lt, std::vector<float> &r, std::vector<float> &l, std::vector<uint8_t> &o)
-      : sim(s), P(p), boxes(bx), table(tb), lut(lt), R(r), L(l), O(o) {
+           const std::vector<float> &lt, std::vector<float> &r, std::vector<float> &l, std::vector<uint8_t> &o,
+           std::vector<float> &rs)
+      : sim(s), P(p), boxes(bx), table(tb), lut(lt), R(r), L(l), O(o), RS(rs) {
+    coarse = s.smokeDiv > 1;
+    bs = s.bs;
+    sS = s.sS;
+    nS = s.nS;
+    ES = bs + 1;
+    nRS = ES * ES * ES;
     B = s.B;
     sF = s.sF;
     side = s.side;
@@ -329,15 +343,90 @@ struct Renderer {
     return v;
   }
 
-  // sampleWorld (volume-common.wgsl:44-55): zero outside the box, else the render field
-  // at p in cells.
+  // ---- coarse smoke ----
+  // The simulation's coarse density at smoke cell (x, y, z) through the page table.
+  float rawSmokeCell(const RBox &bx, int x, int y, int z) const {
+    int pg = pageAt(bx, x >> sS, y >> sS, z >> sS);
+    if (pg < 0) return 0.0f;
+    int m = bs - 1;
+    return sim.S[(size_t)slotOf(pg) * nS + (((z & m) * bs + (y & m)) * bs + (x & m))];
+  }
+  float &smokeTexel(int slot, int x, int y, int z) const {
+    return RS[(size_t)slot * nRS + ((size_t)z * ES + y) * ES + x];
+  }
+  // fireProField on the smoke lattice (volume-bricks.wgsl:56-104), position in smoke cells,
+  // over the simulation's density (`raw`) or the smoothed render smoke.
+  float smokeField(const RBox &bx, float px, float py, float pz, bool raw) const {
+    float sz = (float)bs;
+    float q[3] = {px, py, pz};
+    for (int a = 0; a < 3; a++)
+      q[a] = fmin(fmax(q[a], (float)bx.origin[a] * sz + 0.5f), (float)(bx.origin[a] + bx.size[a]) * sz - 0.5f);
+    int base[3], brick[3];
+    float f[3];
+    for (int a = 0; a < 3; a++) {
+      float g = std::floor(q[a] - 0.5f);
+      base[a] = (int)g;
+      brick[a] = base[a] >> sS;
+      f[a] = (q[a] - 0.5f) - g;
+    }
+    int pg = pageAt(bx, brick[0], brick[1], brick[2]);
+    int last = bs - 1;
+    if (pg < 0 && (base[0] & last) < last && (base[1] & last) < last && (base[2] & last) < last) return 0.0f;
+    int slot = pg >= 0 ? slotOf(pg) : -1;
+    float v = 0.0f;
+    for (int z = 0; z < 2; z++)
+      for (int y = 0; y < 2; y++)
+        for (int x = 0; x < 2; x++) {
+          float w = (x ? f[0] : 1.0f - f[0]) * (y ? f[1] : 1.0f - f[1]) * (z ? f[2] : 1.0f - f[2]);
+          int c[3] = {base[0] + x, base[1] + y, base[2] + z};
+          float t;
+          if (raw) {
+            t = rawSmokeCell(bx, c[0], c[1], c[2]);
+          } else if (slot >= 0) {
+            t = smokeTexel(slot, c[0] - brick[0] * bs, c[1] - brick[1] * bs, c[2] - brick[2] * bs);
+          } else {
+            int p2 = pageAt(bx, c[0] >> sS, c[1] >> sS, c[2] >> sS);
+            if (p2 < 0) continue;
+            t = smokeTexel(slotOf(p2), c[0] & last, c[1] & last, c[2] & last);
+          }
+          v += t * w;
+        }
+    return v;
+  }
+  // renderSmokeAt (volume-common.wgsl:74-78): the smoothed smoke at a field position.
+  float renderSmokeAt(const RBox &bx, float fx, float fy, float fz) const {
+    float ratio = (float)(B / bs);
+    return smokeField(bx, fx / ratio, fy / ratio, fz / ratio, false);
+  }
+  // cachedSmokeCell (lighting.wgsl:133-152).
+  float cachedSmokeCell(const RBox &bx, int x, int y, int z) const {
+    int m = bs - 1;
+    int b[3] = {x >> sS, y >> sS, z >> sS}, l[3] = {x & m, y & m, z & m};
+    int pg = pageAt(bx, b[0], b[1], b[2]);
+    if (pg >= 0) return smokeTexel(slotOf(pg), l[0], l[1], l[2]);
+    for (int mk = 1; mk < 8; mk++) {
+      int st[3] = {mk & 1, (mk >> 1) & 1, (mk >> 2) & 1};
+      bool skip = false;
+      for (int a = 0; a < 3; a++) skip = skip || (st[a] && l[a] != 0);
+      if (skip) continue;
+      int prev = pageAt(bx, b[0] - st[0], b[1] - st[1], b[2] - st[2]);
+      if (prev >= 0) return smokeTexel(slotOf(prev), l[0] + st[0] * bs, l[1] + st[1] * bs, l[2] + st[2] * bs);
+    }
+    return 0.0f;
+  }
+
+  // sampleWorld (volume-common.wgsl:41-51): zero outside the box, else the render field
+  // at p in cells, its smoke from the coarse grid when smoke has one.
   Vec4 sampleWorld(const RBox &bx, float px, float py, float pz) const {
     float size = P.brickSize;
     float p[3] = {px, py, pz};
     for (int a = 0; a < 3; a++)
       if (p[a] < (float)bx.origin[a] * size || p[a] > (float)(bx.origin[a] + bx.size[a]) * size)
         return {0, 0, 0, 0};
-    return renderSample(bx, px / P.voxel, py / P.voxel, pz / P.voxel);
+    float fx = px / P.voxel, fy = py / P.voxel, fz = pz / P.voxel;
+    Vec4 f = renderSample(bx, fx, fy, fz);
+    if (coarse) f.x = renderSmokeAt(bx, fx, fy, fz);
+    return f;
   }
 
   // The blackbody texel pair at kelvin, linearly filtered (fire-emission.wgsl:43-45).
@@ -462,6 +551,15 @@ struct Renderer {
         for (int x = 0; x < E; x++) {
           int cell[3] = {brick[0] * B + x, brick[1] * B + y, brick[2] * B + z};
           Vec4 original = fieldCell(bx, cell[0], cell[1], cell[2]);
+          if (coarse) {
+            // The fields as they are (VolumeLighting.ts:206-208).
+            float *o = const_cast<float *>(texel(R, slot, E, x, y, z));
+            o[0] = original.x;
+            o[1] = original.y;
+            o[2] = original.z;
+            o[3] = original.w;
+            continue;
+          }
           float density = 0.0f;
           for (int corner = 0; corner < 8; corner++)
             density += smokeSample(bx, (float)cell[0] + (float)(corner & 1), (float)cell[1] + (float)((corner >> 1) & 1),
@@ -474,13 +572,31 @@ struct Renderer {
         }
   }
 
-  // buildOccupancy (lighting.wgsl:150-238), full-resolution smoke: per 8^3 block, the
-  // 12^3 cells of the block and its two-cell halo whose bricks have slots.
+  // smoothCoarseSmoke (lighting.wgsl:105-129): every texel of the slot and its apron.
+  void smoothCoarseBrick(int page, const RBox &bx, const int brick[3]) {
+    int slot = slotOf(page);
+    for (int z = 0; z < ES; z++)
+      for (int y = 0; y < ES; y++)
+        for (int x = 0; x < ES; x++) {
+          int cell[3] = {brick[0] * bs + x, brick[1] * bs + y, brick[2] * bs + z};
+          float density = 0.0f;
+          for (int corner = 0; corner < 8; corner++)
+            density += smokeField(bx, (float)cell[0] + (float)(corner & 1), (float)cell[1] + (float)((corner >> 1) & 1),
+                                  (float)cell[2] + (float)((corner >> 2) & 1), true);
+          smokeTexel(slot, x, y, z) = fmax(0.0f, density * 0.125f);
+        }
+  }
+
+  // buildOccupancy (lighting.wgsl:150-253): per 8^3 block, the 12^3 cells of the block and
+  // its two-cell halo whose bricks have slots. With coarse smoke the fine cells give flame
+  // only, and the block is visible when its largest glow plus the largest smoothed smoke
+  // among its smoke donors (clamped to the box) passes the cutoff.
   void occupancyBrick(int page, const RBox &bx, const int brick[3]) {
     int slot = slotOf(page);
     for (int block = 0; block < nO; block++) {
       int b[3] = {block % perSide, (block / perSide) % perSide, block / (perSide * perSide)};
       uint8_t flags = 0;
+      float maxGlow = 0.0f;
       for (int unit = 0; unit < 216; unit++) {
         int cell[3] = {brick[0] * B + b[0] * 8 - 2 + (unit % 6) * 2, brick[1] * B + b[1] * 8 - 2 + ((unit / 6) % 6) * 2,
                        brick[2] * B + b[2] * 8 - 2 + (unit / 36) * 2};
@@ -491,9 +607,30 @@ struct Renderer {
           const float *f = texel(R, s2, E, (cell[0] & m) + (corner & 1), (cell[1] & m) + ((corner >> 1) & 1),
                                  (cell[2] & m) + ((corner >> 2) & 1));
           float glow = glowOf(f[2]);
+          float smoke = coarse ? 0.0f : f[0];
           if (glow > 0.002f) flags |= 1;
-          if (f[0] + glow > 0.002f) flags |= 2;
+          if (smoke + glow > 0.002f) flags |= 2;
+          maxGlow = fmax(maxGlow, glow);
         }
+      }
+      if (coarse) {
+        float ratio = (float)(B / bs), maxSmoke = 0.0f;
+        int lo[3], hi[3], boxLo[3], boxHi[3];
+        for (int a = 0; a < 3; a++) {
+          int origin = brick[a] * B + b[a] * 8 - 2;
+          lo[a] = (int)std::floor((float)origin / ratio - 0.5f);
+          hi[a] = (int)std::floor((float)(origin + 12) / ratio - 0.5f) + 1;
+          boxLo[a] = bx.origin[a] * bs;
+          boxHi[a] = (bx.origin[a] + bx.size[a]) * bs - 1;
+        }
+        for (int z = lo[2]; z <= hi[2]; z++)
+          for (int y = lo[1]; y <= hi[1]; y++)
+            for (int x = lo[0]; x <= hi[0]; x++) {
+              int c[3] = {x, y, z};
+              for (int a = 0; a < 3; a++) c[a] = std::min(std::max(c[a], boxLo[a]), boxHi[a]);
+              maxSmoke = fmax(maxSmoke, cachedSmokeCell(bx, c[0], c[1], c[2]));
+            }
+        if (maxSmoke + maxGlow > 0.002f) flags |= 2;
       }
       O[(size_t)slot * nO + block] = flags;
     }
@@ -529,6 +666,7 @@ struct Renderer {
         int page, bi, brick[3];
         if (!slotted(id, page, bi, brick)) continue;
         smoothBrick(page, boxes[bi], brick);
+        if (coarse) smoothCoarseBrick(page, boxes[bi], brick);
       }
     if (occupancy)
       for (int id : sim.activeList) {
@@ -764,6 +902,8 @@ struct Renderer {
         continue;
       }
       Vec4 f = reconstruct(bx, cell, (flags & 1) ? P.filter : 0.0f);
+      // fireProRenderField (volume-reconstruction.wgsl:100-116): smoke from its own lattice.
+      if (coarse) f.x = renderSmokeAt(bx, cell[0], cell[1], cell[2]);
       if (f.x + glowOf(f.z) > 0.002f) {
         // fireProShade (volume-march.wgsl:103-138)
         float glow[3];
@@ -837,7 +977,15 @@ inline void sceneLights(const Sim<float> &sim, const Renderer &r, const std::vec
                             (lane >> 4) * stride + stride / 2};
       for (int a = 0; a < 3; a++) locals[lane][a] = local[a];
       const float *fp = &sim.F[((size_t)sim.pageSlot(page) * sim.nF + ((local[2] * B + local[1]) * B + local[0])) * 4];
-      const Vec4 f{fp[0], fp[1], fp[2], fp[3]};
+      Vec4 f{fp[0], fp[1], fp[2], fp[3]};
+      if (sim.coarse()) {
+        // The coarse density sampled at the lattice point (scene-lights.wgsl:54-61).
+        const float ratio = (float)(B / sim.bs);
+        float q[3];
+        for (int a = 0; a < 3; a++)
+          q[a] = (float)(brick[a] * sim.bs) + ((float)local[a] + 0.5f) / ratio;
+        f.x = sim.sampleSmoke(sim.S, q);
+      }
       r.emission(f, rgbs[lane]);
       lum[lane] = rgbs[lane][0] * 0.2126f + rgbs[lane][1] * 0.7152f + rgbs[lane][2] * 0.0722f;
       float bestD = 1e38f;
@@ -915,7 +1063,7 @@ struct RenderCache {
   std::vector<RBox> boxes;
   std::vector<int32_t> table;
   std::vector<float> lut = blackbodyTable();
-  std::vector<float> R, L;
+  std::vector<float> R, L, RS;
   std::vector<uint8_t> O;
 
   // SceneVolumeRenderer.invalidate (SceneVolumeRenderer.ts:254-257) on a reset.
@@ -932,7 +1080,7 @@ struct RenderCache {
   }
 
   Renderer view(const Sim<float> &sim, const RenderParams &p) {
-    return Renderer(sim, p, boxes, table, lut, R, L, O);
+    return Renderer(sim, p, boxes, table, lut, R, L, O, RS);
   }
 
   // Brings the page tables and pools up to the simulation's latest step; returns the
@@ -952,6 +1100,7 @@ struct RenderCache {
       R.assign((size_t)poolSlots * E * E * E * 4, 0.0f);
       L.assign((size_t)poolSlots * LE * LE * LE * 4, 0.0f);
       O.assign((size_t)poolSlots * nO, 0);
+      if (sim.coarse()) RS.assign((size_t)poolSlots * (sim.bs + 1) * (sim.bs + 1) * (sim.bs + 1), 0.0f);
     }
     bool changed = stale;
     std::string k = key(p);

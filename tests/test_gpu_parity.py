@@ -42,8 +42,7 @@ def test_coarse_smoke_first_order_and_campfire():
     assert run_lockstep(_scenes().campfire(steps=30, smoke=2), check_every=5) == []
 
 
-def test_coarse_smoke_refuses_render_and_batch():
-    from firepro import abi
+def test_coarse_smoke_refuses_batch():
     from firepro.engine import Batch, FluidSimulation
     sc = _scenes().small(steps=2, smoke=2)
     gpu = FluidSimulation.create(sc.grid())
@@ -51,11 +50,6 @@ def test_coarse_smoke_refuses_render_and_batch():
     try:
         frames = list(sc.frames())
         gpu.step(frames[0][0])
-        cam = abi.camera((0.0, 1.0, 3.0), (0.0, 0.5, 0.0), 50, 16, 16)
-        with pytest.raises(ValueError, match="smokeDivisor"):
-            gpu.render(cam)
-        with pytest.raises(ValueError, match="smokeDivisor"):
-            gpu.scene_lights([(0.0, 0.3, 0.0)])
         with pytest.raises(ValueError, match="smokeDivisor"):
             batch.step([gpu], [frames[1][0]])
         full = FluidSimulation.create(_scenes().small(steps=1).grid())

@@ -26,7 +26,7 @@ def render_mismatches(gpu, cpu, cam, rs, label):
     ta, tb = gpu.read_render(), cpu.read_render()
     if ta["light_work"] != tb["light_work"]:
         out.append(f"{label}: light work {ta['light_work']} != {tb['light_work']}")
-    for k in RENDER_KEYS:
+    for k in RENDER_KEYS + (("smoke",) if "smoke" in ta else ()):
         if ta[k].shape != tb[k].shape:
             out.append(f"{label}: {k} shape {ta[k].shape} != {tb[k].shape}")
         elif not np.array_equal(ta[k], tb[k]):
@@ -193,6 +193,33 @@ def test_render_rejects_bad_settings():
         assert not img.any()
     finally:
         sim.dispose()
+
+
+@pytest.mark.parametrize("brick,smoke,steps", [(16, 2, 40), (8, 4, 12), (16, 4, 12)])
+def test_coarse_smoke_render_bit_exact(brick, smoke, steps):
+    """Rendering with smoke on its own grid: the fields as they are, the smoke smoothed on
+    its lattice, occupancy from glow and smoke donors, light and march reading the coarse
+    smoke, bit-exact; the campfire (brick 16) or the small scene."""
+    from firepro import scenes
+    abi = _abi()
+    sc = (scenes.campfire(steps=steps, smoke=smoke) if brick == 16 and smoke == 2
+          else scenes.small(steps=steps, brick=brick, divisor=2, smoke=smoke))
+    gpu, cpu = make_engines(sc)
+    try:
+        for frame, g in sc.frames():
+            apply(gpu, cpu, g)
+            gpu.step(frame)
+            cpu.step(frame)
+        cam = abi.camera((-3.0, 1.6, 4.0), (0, 0.8, 0), 44, 96, 96)
+        print(f"case coarse render brick={brick} smoke={smoke}", flush=True)
+        msgs, img = render_mismatches(gpu, cpu, cam, abi.render_settings(), "coarse")
+        assert msgs == []
+        assert img[..., 3].max() > 0.05, "the volume is in view"
+        anchors = [(0.0, 0.3, 0.0), (0.6, 0.4, -0.2)]
+        a, b = gpu.scene_lights(anchors), cpu.scene_lights(anchors)
+        assert np.array_equal(a, b), (a, b)
+    finally:
+        gpu.dispose()
 
 
 def test_scene_lights_bit_exact():

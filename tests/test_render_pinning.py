@@ -351,3 +351,50 @@ def test_scene_light_placement():
     assert sl.finished("b") and not sl.finished("a")
     sl.apply(sums[:1], anchors[:1], 1.0, 16, 0.05)
     assert "b" not in sl.measured
+
+
+def test_coarse_smoke_render_field_is_the_121_filter_of_the_smoke_grid():
+    """With smokeDivisor 2 the render smoke is the separable [1,2,1]/4 filter of the coarse
+    density (scipy) on the smoke lattice, the render field holds the fine fields unfiltered,
+    and every block whose own smoothed smoke passes the cutoff is marked visible."""
+    abi, scenes, _, _ = _mods()
+    B, sd = 8, 2
+    n = B // sd
+    sim = _run(scenes.small(steps=8, brick=B, divisor=2, smoke=sd))
+    sim.render(abi.camera((0, 1, 4), (0, 1, 0), 45, 4, 4), image=False)
+    rt = sim.read_render()
+    sm = sim.read_smoke()
+    st = sim.read_state(velocity=False)
+    dense, origin, _ = _dense(sim, sm["ids"], sm["smoke"][..., None], n)
+    dense = dense[..., 0].astype(np.float64)
+    k = np.array([1.0, 2.0, 1.0]) / 4
+    smooth = dense
+    for axis in range(3):
+        smooth = ndimage.correlate1d(smooth, k, axis=axis, mode="constant")
+    fine = {int(i): f for i, f in zip(st["ids"], st["fields"])}
+    lcoords = sim.brick_coords(rt["bricks"])
+    box = rt["boxes"][0]
+    box_lo, box_hi = box[:3] * n, (box[:3] + box[4:7]) * n - 1
+    ES, E = n + 1, B + 1
+    checked = 0
+    for i, c in enumerate(lcoords):
+        tex = rt["smoke"][i].reshape(ES, ES, ES)
+        field = rt["field"][i].reshape(E, E, E, 4)
+        np.testing.assert_array_equal(field[:B, :B, :B].reshape(-1, 4), fine[int(rt["bricks"][i])])
+        for lz in range(ES):
+            for ly in range(ES):
+                for lx in range(ES):
+                    cell = c * n + np.array([lx, ly, lz])
+                    d = cell - origin
+                    if np.any(cell - 1 < box_lo) or np.any(cell + 1 > box_hi):
+                        continue
+                    if np.any(d < 1) or np.any(d + 1 >= np.array(dense.shape[::-1])):
+                        continue
+                    want = smooth[d[2], d[1], d[0]]
+                    assert abs(tex[lz, ly, lx] - want) <= 1e-6 * max(1.0, abs(want)), (cell, want)
+                    checked += 1
+        own = tex[:n, :n, :n]
+        for b in range(len(rt["occupancy"][i])):
+            if own.max() > 0.002 and B == 8:
+                assert rt["occupancy"][i][b] & 2, (c, own.max())
+    assert checked > 200

@@ -239,10 +239,19 @@ struct RenderArgs {
   uint8_t *O;
   const float4 *lut;
   RenderParams P;
+  // Coarse smoke (smokeDivisor 2 or 4): the smoke brick's shift, edge and cells, the render
+  // smoke's slot edge (edge + 1) and texels, the simulation's density and its smoothed copy
+  // with one-texel aprons, and the directory (the scene lights read neighbors through it).
+  int coarse;
+  int sS, bs, nS, ES, nRS;
+  const float *S;
+  float *RS;
+  Dir dir;
 };
 
 __global__ void k_export_pages(RenderArgs r);
 __global__ void k_smooth_smoke(RenderArgs r, int chunks, int staged);
+__global__ void k_smooth_coarse(RenderArgs r);
 __global__ void k_occupancy(RenderArgs r);
 __global__ void k_light(RenderArgs r, int fresh);
 __global__ void k_march(RenderArgs r, float4 *out);

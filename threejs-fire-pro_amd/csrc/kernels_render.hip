@@ -104,13 +104,82 @@ __device__ __forceinline__ float4 renderSample(const RenderArgs &r, const RBox &
   return v;
 }
 
-// sampleWorld (volume-common.wgsl:44-55).
+// ---- coarse smoke (smokeDivisor 2 or 4) ----
+// The simulation's coarse density at smoke cell (x, y, z) through the box's page table
+// (fireProLoad on the smoke lattice), zero without a slot.
+__device__ __forceinline__ float rawSmokeCell(const RenderArgs &r, const RBox &bx, int x, int y, int z) {
+  int pg = pageAt(r, bx, x >> r.sS, y >> r.sS, z >> r.sS);
+  if (pg < 0) return 0.f;
+  const int m = r.bs - 1;
+  return r.S[(size_t)pageToSlot(pg, r.side) * r.nS + (((z & m) * r.bs + (y & m)) * r.bs + (x & m))];
+}
+// A texel of the smoothed render smoke, slot-major with one-texel aprons.
+__device__ __forceinline__ float &smokeTexel(const RenderArgs &r, int slot, int x, int y, int z) {
+  return r.RS[(size_t)slot * r.nRS + ((size_t)z * r.ES + y) * r.ES + x];
+}
+// fireProField on the smoke lattice (volume-bricks.wgsl:56-104), position in smoke cells:
+// over the simulation's density (`raw`, whose slots' aprons hold their neighbors' cells)
+// or over the smoothed render smoke, whose slots carry computed aprons.
+__device__ __forceinline__ float smokeField(const RenderArgs &r, const RBox &bx, float px, float py, float pz,
+                                            bool raw) {
+  const float sz = (float)r.bs;
+  float q[3] = {px, py, pz};
+  const int o[3] = {bx.ox, bx.oy, bx.oz}, s[3] = {bx.sx, bx.sy, bx.sz};
+#pragma unroll
+  for (int a = 0; a < 3; a++) q[a] = fminv(fmaxv(q[a], (float)o[a] * sz + 0.5f), (float)(o[a] + s[a]) * sz - 0.5f);
+  int base[3], brick[3];
+  float f[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    float g = floorf(q[a] - 0.5f);
+    base[a] = (int)g;
+    brick[a] = base[a] >> r.sS;
+    f[a] = (q[a] - 0.5f) - g;
+  }
+  const int pg = pageAt(r, bx, brick[0], brick[1], brick[2]);
+  const int last = r.bs - 1;
+  if (pg < 0 && (base[0] & last) < last && (base[1] & last) < last && (base[2] & last) < last) return 0.f;
+  const int slot = pg >= 0 ? pageToSlot(pg, r.side) : -1;
+  float v = 0.f;
+#pragma unroll
+  for (int z = 0; z < 2; z++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        float wx = x ? f[0] : 1.0f - f[0], wy = y ? f[1] : 1.0f - f[1], wz = z ? f[2] : 1.0f - f[2];
+        float w = wx * wy * wz;
+        const int c[3] = {base[0] + x, base[1] + y, base[2] + z};
+        float t;
+        if (raw) {
+          t = rawSmokeCell(r, bx, c[0], c[1], c[2]);
+        } else if (slot >= 0) {
+          t = smokeTexel(r, slot, c[0] - brick[0] * r.bs, c[1] - brick[1] * r.bs, c[2] - brick[2] * r.bs);
+        } else {
+          int p2 = pageAt(r, bx, c[0] >> r.sS, c[1] >> r.sS, c[2] >> r.sS);
+          if (p2 < 0) continue;
+          t = smokeTexel(r, pageToSlot(p2, r.side), c[0] & last, c[1] & last, c[2] & last);
+        }
+        v += t * w;
+      }
+  return v;
+}
+// The smoothed smoke at a field position (renderSmokeAt, volume-common.wgsl:74-78).
+__device__ __forceinline__ float renderSmokeAt(const RenderArgs &r, const RBox &bx, float fx, float fy, float fz) {
+  const float ratio = (float)(r.B / r.bs);
+  return smokeField(r, bx, fx / ratio, fy / ratio, fz / ratio, false);
+}
+
+// sampleWorld (volume-common.wgsl:41-51).
 __device__ __forceinline__ float4 sampleWorld(const RenderArgs &r, const RBox &bx, float px, float py, float pz) {
   const float size = r.P.brickSize;
   if (px < (float)bx.ox * size || py < (float)bx.oy * size || pz < (float)bx.oz * size ||
       px > (float)(bx.ox + bx.sx) * size || py > (float)(bx.oy + bx.sy) * size || pz > (float)(bx.oz + bx.sz) * size)
     return make_float4(0.f, 0.f, 0.f, 0.f);
-  return renderSample(r, bx, px / r.P.voxel, py / r.P.voxel, pz / r.P.voxel);
+  const float fx = px / r.P.voxel, fy = py / r.P.voxel, fz = pz / r.P.voxel;
+  float4 f = renderSample(r, bx, fx, fy, fz);
+  if (r.coarse) f.x = renderSmokeAt(r, bx, fx, fy, fz);
+  return f;
 }
 
 // fireProEmission (fire-emission.wgsl:26-56), smoke = (albedo, soot glow).
@@ -291,6 +360,12 @@ __global__ void __launch_bounds__(256) k_smooth_smoke(RenderArgs r, int chunks, 
     int lx = t % E, ly = (t / E) % E, lz = t / (E * E);
     int cell[3] = {first[0] + lx, first[1] + ly, first[2] + lz};
     float4 original = fieldCell(r, bx, cell[0], cell[1], cell[2]);
+    if (r.coarse) {
+      // Smoke lives on its own grid: the render field is the fields as they are
+      // (VolumeLighting.ts:206-208), with aprons.
+      r.R[(size_t)slot * r.nR + t] = original;
+      continue;
+    }
     float density = 0.0f;
     for (int corner = 0; corner < 8; corner++) {
       float q[3] = {(float)cell[0] + (float)(corner & 1), (float)cell[1] + (float)((corner >> 1) & 1),
@@ -325,9 +400,49 @@ __global__ void __launch_bounds__(256) k_smooth_smoke(RenderArgs r, int chunks, 
   }
 }
 
-// buildOccupancy (lighting.wgsl:150-238) at full-resolution smoke: per 8^3 block, the
-// 12^3 cells of the block and its two-cell halo whose bricks have slots. A workgroup per
-// listed brick; its lanes OR their flags through LDS.
+// smoothCoarseSmoke (lighting.wgsl:105-129): the [1,2,1]/4 filter per axis on the smoke
+// lattice, as eight half-cell taps of the simulation's density, at every texel of a listed
+// brick's render smoke slot, the far apron computed like the interior. A workgroup per
+// listed brick.
+__global__ void __launch_bounds__(256) k_smooth_coarse(RenderArgs r) {
+  const int i = xcdItem((int)blockIdx.x, r.counts[0]);
+  if (i < 0) return;
+  int page, brick[3];
+  RBox bx;
+  if (!slotted(r, r.active[i], page, bx, brick)) return;
+  const int slot = pageToSlot(page, r.side), ES = r.ES;
+  for (int t = threadIdx.x; t < r.nRS; t += blockDim.x) {
+    const int cell[3] = {brick[0] * r.bs + t % ES, brick[1] * r.bs + (t / ES) % ES, brick[2] * r.bs + t / (ES * ES)};
+    float density = 0.0f;
+    for (int corner = 0; corner < 8; corner++)
+      density += smokeField(r, bx, (float)cell[0] + (float)(corner & 1), (float)cell[1] + (float)((corner >> 1) & 1),
+                            (float)cell[2] + (float)((corner >> 2) & 1), true);
+    r.RS[(size_t)slot * r.nRS + t] = fmaxv(0.0f, density * 0.125f);
+  }
+}
+
+// cachedSmokeCell (lighting.wgsl:133-152): the smoothed smoke of a smoke cell; for a
+// brick without a slot, the apron texel a slot before it computed for the cell.
+__device__ __forceinline__ float cachedSmokeCell(const RenderArgs &r, const RBox &bx, int x, int y, int z) {
+  const int n = r.bs, m = n - 1, sh = r.sS;
+  const int b[3] = {x >> sh, y >> sh, z >> sh}, l[3] = {x & m, y & m, z & m};
+  const int pg = pageAt(r, bx, b[0], b[1], b[2]);
+  if (pg >= 0) return smokeTexel(r, pageToSlot(pg, r.side), l[0], l[1], l[2]);
+  for (int mk = 1; mk < 8; mk++) {
+    const int st[3] = {mk & 1, (mk >> 1) & 1, (mk >> 2) & 1};
+    if ((st[0] && l[0]) || (st[1] && l[1]) || (st[2] && l[2])) continue;
+    const int prev = pageAt(r, bx, b[0] - st[0], b[1] - st[1], b[2] - st[2]);
+    if (prev >= 0)
+      return smokeTexel(r, pageToSlot(prev, r.side), l[0] + st[0] * n, l[1] + st[1] * n, l[2] + st[2] * n);
+  }
+  return 0.0f;
+}
+
+// buildOccupancy (lighting.wgsl:150-238): per 8^3 block, the 12^3 cells of the block and
+// its two-cell halo whose bricks have slots. A workgroup per listed brick; its lanes OR
+// their flags through LDS. With coarse smoke the fine cells give flame only, and a block is
+// visible when the largest glow there plus the largest smoothed smoke among the smoke
+// donors of the block and halo passes the cutoff.
 __global__ void __launch_bounds__(256) k_occupancy(RenderArgs r) {
   const int i = xcdItem((int)blockIdx.x, r.counts[0]);
   if (i < 0) return;
@@ -335,9 +450,13 @@ __global__ void __launch_bounds__(256) k_occupancy(RenderArgs r) {
   int page, brick[3];
   RBox bx;
   if (!slotted(r, id, page, bx, brick)) return;
-  __shared__ uint32_t flagsOf[64];
+  __shared__ uint32_t flagsOf[64], glowMax[64], smokeMax[64];
   const int B = r.B, sF = r.sF, m = B - 1, ps = r.perSide;
-  for (int b = threadIdx.x; b < r.nO; b += 256) flagsOf[b] = 0u;
+  for (int b = threadIdx.x; b < r.nO; b += 256) {
+    flagsOf[b] = 0u;
+    glowMax[b] = 0u;
+    smokeMax[b] = 0u;
+  }
   __syncthreads();
   // 216 units of 2^3 cells per block.
   for (int w = threadIdx.x; w < r.nO * 216; w += 256) {
@@ -349,14 +468,46 @@ __global__ void __launch_bounds__(256) k_occupancy(RenderArgs r) {
     if (pg < 0) continue;
     int s2 = pageToSlot(pg, r.side);
     uint32_t flags = 0u;
+    float glowHigh = 0.0f;
     for (int corner = 0; corner < 8; corner++) {
       float4 f = *texel(r.R, s2, r.E, (cx & m) + (corner & 1), (cy & m) + ((corner >> 1) & 1), (cz & m) + ((corner >> 2) & 1));
       float glow = glowOf(f.z);
+      float smoke = r.coarse ? 0.0f : f.x;
       if (glow > 0.002f) flags |= 1u;
-      if (f.x + glow > 0.002f) flags |= 2u;
+      if (smoke + glow > 0.002f) flags |= 2u;
+      glowHigh = fmaxv(glowHigh, glow);
     }
     if (flags) atomicOr(&flagsOf[block], flags);
+    if (r.coarse && glowHigh > 0.0f) atomicMax(&glowMax[block], __float_as_uint(glowHigh));
   }
+  if (r.coarse) {
+    // Every trilinear smoke donor of each block and its halo, clamped to the box.
+    const float ratio = (float)(B / r.bs);
+    const int n = r.bs;
+    const int boxLo[3] = {bx.ox * n, bx.oy * n, bx.oz * n};
+    const int boxHi[3] = {(bx.ox + bx.sx) * n - 1, (bx.oy + bx.sy) * n - 1, (bx.oz + bx.sz) * n - 1};
+    for (int block = 0; block < r.nO; block++) {
+      const int b3[3] = {block % ps, (block / ps) % ps, block / (ps * ps)};
+      int lo[3], size[3];
+      for (int a = 0; a < 3; a++) {
+        const int origin = brick[a] * B + b3[a] * 8 - 2;
+        lo[a] = (int)floorf((float)origin / ratio - 0.5f);
+        const int hi = (int)floorf((float)(origin + 12) / ratio - 0.5f) + 1;
+        size[a] = hi - lo[a] + 1;
+      }
+      float laneSmoke = 0.0f;
+      for (int c = threadIdx.x; c < size[0] * size[1] * size[2]; c += 256) {
+        int q[3] = {lo[0] + c % size[0], lo[1] + (c / size[0]) % size[1], lo[2] + c / (size[0] * size[1])};
+        for (int a = 0; a < 3; a++) q[a] = q[a] < boxLo[a] ? boxLo[a] : q[a] > boxHi[a] ? boxHi[a] : q[a];
+        laneSmoke = fmaxv(laneSmoke, cachedSmokeCell(r, bx, q[0], q[1], q[2]));
+      }
+      if (laneSmoke > 0.0f) atomicMax(&smokeMax[block], __float_as_uint(laneSmoke));
+    }
+  }
+  __syncthreads();
+  if (r.coarse)
+    for (int b = threadIdx.x; b < r.nO; b += 256)
+      if (__uint_as_float(smokeMax[b]) + __uint_as_float(glowMax[b]) > 0.002f) flagsOf[b] |= 2u;
   __syncthreads();
   const int slot = pageToSlot(page, r.side);
   for (int b = threadIdx.x; b < r.nO; b += 256) r.O[(size_t)slot * r.nO + b] = (uint8_t)flagsOf[b];
@@ -617,6 +768,8 @@ __global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out) {
         continue;
       }
       float4 f = reconstruct(r, bx, cell, (flags & 1u) ? P.filter : 0.0f);
+      // Smoke was smoothed once on its own lattice (fireProRenderField, volume-reconstruction.wgsl:100-116).
+      if (r.coarse) f.x = renderSmokeAt(r, bx, cell[0], cell[1], cell[2]);
       if (f.x + glowOf(f.z) > 0.002f) {
         // fireProShade (volume-march.wgsl:103-138)
         float glow[3];
@@ -683,7 +836,32 @@ __global__ void __launch_bounds__(64) k_light_bricks(RenderArgs r, const int *ti
   const int stride = r.B >> 2;
   const int local[3] = {(lane & 3) * stride + stride / 2, ((lane >> 2) & 3) * stride + stride / 2,
                         (lane >> 4) * stride + stride / 2};
-  const float4 f = r.F[(size_t)pageToSlot(page, r.side) * r.nF + ((local[2] * r.B + local[1]) * r.B + local[0])];
+  float4 f = r.F[(size_t)pageToSlot(page, r.side) * r.nF + ((local[2] * r.B + local[1]) * r.B + local[0])];
+  if (r.coarse) {
+    // The simulation's coarse density sampled at the lattice point (scene-lights.wgsl:54-61),
+    // cells of neighbor bricks through the directory.
+    const float ratio = (float)(r.B / r.bs);
+    const int n = r.bs, mk = n - 1;
+    float q[3], fr[3];
+    int base[3];
+    for (int a = 0; a < 3; a++) {
+      q[a] = (float)(brick[a] * n) + ((float)local[a] + 0.5f) / ratio - 0.5f;
+      const float g = floorf(q[a]);
+      base[a] = (int)g;
+      fr[a] = q[a] - g;
+    }
+    float v = 0.0f;
+    for (int z = 0; z < 2; z++)
+      for (int y = 0; y < 2; y++)
+        for (int x = 0; x < 2; x++) {
+          const int c[3] = {base[0] + x, base[1] + y, base[2] + z};
+          const int e = entryLookup(r.dir, c[0] >> r.sS, c[1] >> r.sS, c[2] >> r.sS);
+          if (e < 0) continue;
+          const float w = (x ? fr[0] : 1.0f - fr[0]) * (y ? fr[1] : 1.0f - fr[1]) * (z ? fr[2] : 1.0f - fr[2]);
+          v += r.S[(size_t)entrySlot(e, r.side) * r.nS + (((c[2] & mk) * n + (c[1] & mk)) * n + (c[0] & mk))] * w;
+        }
+    f.x = v;
+  }
   float rgb[3];
   emission(r, f, rgb);
   const float luminance = rgb[0] * 0.2126f + rgb[1] * 0.7152f + rgb[2] * 0.0722f;

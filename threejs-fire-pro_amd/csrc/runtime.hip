@@ -462,6 +462,7 @@ struct fp_sim {
     DBuf<int4> boxes, tiles;
     DBuf<int> table;
     DBuf<float4> R, L, lut, image;
+    DBuf<float> RS;  // smoothed coarse smoke with aprons (smokeDivisor 2 or 4)
     DBuf<uint8_t> O;
     // Scene lights: anchors, per-brick totals and anchors, per-block totals, the totals.
     DBuf<float4> anchors;
@@ -1582,6 +1583,15 @@ struct fp_sim {
     r.O = rc.O.p;
     r.lut = rc.lut.p;
     r.P = P;
+    r.coarse = smokeDiv > 1 ? 1 : 0;
+    r.sS = smokeShift;
+    r.bs = smokeEdge;
+    r.nS = nS;
+    r.ES = smokeEdge + 1;
+    r.nRS = r.ES * r.ES * r.ES;
+    r.S = S.p;
+    r.RS = rc.RS.p;
+    r.dir = dirArgs();
     return r;
   }
 
@@ -1636,6 +1646,10 @@ struct fp_sim {
       rc.R.alloc((size_t)poolSlots * r.nR);
       rc.L.alloc((size_t)poolSlots * r.nL);
       rc.O.alloc((size_t)poolSlots * r.nO);
+      if (smokeDiv > 1) {
+        rc.RS.alloc((size_t)poolSlots * r.nRS);
+        HIPCHECK(hipMemsetAsync(rc.RS.p, 0, rc.RS.bytes(), stream));
+      }
       HIPCHECK(hipMemsetAsync(rc.R.p, 0, rc.R.bytes(), stream));
       HIPCHECK(hipMemsetAsync(rc.L.p, 0, rc.L.bytes(), stream));
       HIPCHECK(hipMemsetAsync(rc.O.p, 0, rc.O.bytes(), stream));
@@ -1660,9 +1674,10 @@ struct fp_sim {
     if (pagesDue) k_export_pages<<<(unsigned)((brickCount() + 255) / 256), 256, 0, stream>>>(r);
     if (smooth) {
       int chunks = (r.nR + 1023) / 1024;
-      bool staged = B <= 16;
+      bool staged = B <= 16 && smokeDiv == 1;
       size_t lds = staged ? (size_t)(B + 3) * (B + 3) * (B + 3) * sizeof(float) : 0;
       k_smooth_smoke<<<slotsGrid * chunks, 256, lds, stream>>>(r, chunks, staged ? 1 : 0);
+      if (smokeDiv > 1) k_smooth_coarse<<<slotsGrid, 256, 0, stream>>>(r);
     }
     if (occupancy) k_occupancy<<<slotsGrid, 256, 0, stream>>>(r);
     end(t, "render_prepare");
@@ -1681,7 +1696,6 @@ struct fp_sim {
   }
 
   void renderImage(const fp_camera &cam, const fp_render_settings &s, float *out, bool deviceOut) {
-    renderable();
     RenderParams P = renderParams(cam, s);
     renderPrepare(P);
     size_t pixels = (size_t)P.width * P.height;
@@ -1708,13 +1722,7 @@ struct fp_sim {
 
   // SceneLights.sample (SceneLights.ts:127-218) over the latest step's fields: per anchor,
   // the luminance of the flames nearest it, their luminance-weighted offset and their rgb.
-  // The render path reads smoke from the fields' X channel, which holds only the step's
-  // production when smoke has its own coarse grid.
-  void renderable() const {
-    if (smokeDiv > 1) throw InvalidArgument("rendering with smokeDivisor 2 or 4 is not supported.");
-  }
   void sceneLights(const fp_render_settings &s, const double *xyz, int count, float *out) {
-    renderable();
     if (count < 0 || count > 65536) throw InvalidArgument("anchor count must be 0 to 65536.");
     for (int i = 0; i < 3 * count; i++)
       if (!std::isfinite(xyz[i])) throw InvalidArgument("anchor positions must be finite.");
@@ -1752,7 +1760,6 @@ struct fp_sim {
   }
 
   void readRender(fp_render_tables *t) {
-    renderable();
     HIPCHECK(hipStreamSynchronize(stream));
     t->light_work = rc.lastWork;
     t->box_count = rc.boxCount;
@@ -1797,6 +1804,11 @@ struct fp_sim {
         if (t->occupancy)
           HIPCHECK(hipMemcpy(t->occupancy + (size_t)listed * nO, rc.O.p + (size_t)slot * nO, (size_t)nO,
                              hipMemcpyDeviceToHost));
+        if (t->smoke && smokeDiv > 1 && rc.RS.p) {
+          const size_t nRS = (size_t)(smokeEdge + 1) * (smokeEdge + 1) * (smokeEdge + 1);
+          HIPCHECK(hipMemcpy(t->smoke + (size_t)listed * nRS, rc.RS.p + (size_t)slot * nRS, nRS * sizeof(float),
+                             hipMemcpyDeviceToHost));
+        }
       }
       listed++;
     }
@@ -1857,6 +1869,7 @@ struct fp_sim {
     rc.tiles.release();
     rc.table.release();
     rc.R.release();
+    rc.RS.release();
     rc.L.release();
     rc.lut.release();
     rc.image.release();

@@ -141,6 +141,7 @@ class RenderTables(C.Structure):
         ("box_capacity", C.c_int32), ("page_capacity", C.c_int32), ("brick_capacity", C.c_int32),
         ("box_count", C.c_int32), ("page_count", C.c_int32), ("brick_count", C.c_int32),
         ("light_work", C.c_int32),
+        ("smoke", C.POINTER(C.c_float)),
     ]
 
 
@@ -221,7 +222,7 @@ class DeviceBuffer:
             self.ptr = C.c_void_p()
 
 
-def read_render_tables(call, brick_size: int, lighting_divisor: int) -> dict:
+def read_render_tables(call, brick_size: int, lighting_divisor: int, smoke_divisor: int = 1) -> dict:
     """Run a read-render entry point (`call(RenderTables) -> status`) twice: once for the
     counts, once into arrays of those sizes."""
     import numpy as np
@@ -237,6 +238,8 @@ def read_render_tables(call, brick_size: int, lighting_divisor: int) -> dict:
         "occupancy": np.zeros((max(1, t.brick_count), nO), np.uint8),
         "blackbody": np.zeros((512, 4), np.float32),
     }
+    if smoke_divisor > 1:
+        out["smoke"] = np.zeros((max(1, t.brick_count), (brick_size // smoke_divisor + 1) ** 3), np.float32)
     u = RenderTables()
     for k, a in out.items():
         ct = {np.int32: C.c_int32, np.float32: C.c_float, np.uint8: C.c_uint8}[a.dtype.type]
@@ -245,8 +248,9 @@ def read_render_tables(call, brick_size: int, lighting_divisor: int) -> dict:
     call(u)
     out["boxes"] = out["boxes"][:u.box_count]
     out["page_table"] = out["page_table"][:u.page_count]
-    for k in ("bricks", "field", "light", "occupancy"):
-        out[k] = out[k][:u.brick_count]
+    for k in ("bricks", "field", "light", "occupancy", "smoke"):
+        if k in out:
+            out[k] = out[k][:u.brick_count]
     out["light_work"] = u.light_work
     return out
 

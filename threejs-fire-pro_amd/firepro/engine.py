@@ -222,7 +222,7 @@ class FluidSimulation:
         ld = getattr(self, "_lighting_divisor", 4)
         return abi.read_render_tables(
             lambda t: abi.check(self._lib.fp_sim_read_render(self._h(), C.byref(t)), self._lib),
-            self.brick_size, ld)
+            self.brick_size, ld, self.smoke_divisor)
 
     def stream(self) -> int:
         return self._lib.fp_sim_stream(self._h())
