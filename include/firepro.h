@@ -177,6 +177,8 @@ typedef struct fp_render_settings {
   int32_t filter;          /* 0 trilinear, 1 quadratic, 2 cubic B-spline */
   int32_t ray_steps;       /* samples a ray takes at most across the boxes */
   int32_t lighting_divisor;/* 1, 2 or 4 field cells per lighting texel */
+  int32_t half_resolution; /* 1: march at half resolution into depth slices first and resume
+                              from them (VolumeSlices.ts; the reference's default) */
 } fp_render_settings;
 
 /* The render preparation of the latest render, for parity and inspection: the boxes

@@ -110,6 +110,7 @@ struct RenderParams {
   float filter, raySteps, time;
   float ivp[16];     // inverse view-projection, column-major
   int width, height;
+  bool halfRes;      // march at half resolution into depth slices first (VolumeSlices.ts)
 };
 
 inline RenderParams renderParams(double voxel, int B, const fp_camera &cam, const fp_render_settings &s,
@@ -146,6 +147,7 @@ inline RenderParams renderParams(double voxel, int B, const fp_camera &cam, cons
   for (int i = 0; i < 16; i++) p.ivp[i] = (float)cam.inverse_view_projection[i];
   p.width = cam.width;
   p.height = cam.height;
+  p.halfRes = s.half_resolution != 0;
   return p;
 }
 
@@ -802,7 +804,156 @@ struct Renderer {
   }
 
   // fireProSceneVolume's beauty path for pixel (px, py), row 0 at the top.
-  void pixel(int px, int py, float out[4]) const {
+  // fireProRay (volume-march.wgsl:18-24) through screen position uv.
+  void ray(const float uv[2], float o[3], float d[3]) const {
+    float clip[2] = {uv[0] * 2.0f + -1.0f, uv[1] * -2.0f + 1.0f};
+    float nearIn[4] = {clip[0], clip[1], 0.0f, 1.0f}, farIn[4] = {clip[0], clip[1], 0.99999f, 1.0f};
+    float nh[4], fh[4];
+    transform(nearIn, nh);
+    transform(farIn, fh);
+    float dd[3];
+    for (int a = 0; a < 3; a++) o[a] = nh[a] / nh[3];
+    for (int a = 0; a < 3; a++) dd[a] = fh[a] / fh[3] - o[a];
+    normalize3(dd, d);
+  }
+  // fireProSpan (volume-march.wgsl:39-48).
+  void span(const float o[3], const float d[3], float &sx, float &sy, float &sz) const {
+    sx = 1e30f;
+    sy = 0.0f;
+    sz = 0.0f;
+    for (const RBox &bx : boxes) {
+      float en, ex;
+      hit(o, d, bx, en, ex);
+      if (ex > en) {
+        sx = fmin(sx, en);
+        sy = fmax(sy, ex);
+        sz = sz + ex - en;
+      }
+    }
+  }
+  // fireProDirect (volume-march.wgsl:141-146).
+  void directLight(const float d[3], float direct[3]) const {
+    float ln[3];
+    normalize3(P.light, ln);
+    float g = P.optical[3];
+    float mu = d[0] * ln[0] + d[1] * ln[1] + d[2] * ln[2];
+    float base = fmax(0.05f, 1.0f + g * g - 2.0f * g * mu);
+    float phase = (1.0f - g * g) / (base * std::sqrt(base));
+    for (int a = 0; a < 3; a++) direct[a] = P.sceneLight[a] * P.light[3] * (0.35f + phase * 0.3f);
+  }
+
+  // The emission and extinction of a sample (fireProShade, volume-march.wgsl:103-138)
+  // integrated over ds into the ray's state.
+  void shade(const RBox &bx, const float cell[3], uint8_t flags, const float direct[3], float ds,
+             float &transmittance, float radiance[3]) const {
+    Vec4 f = reconstruct(bx, cell, (flags & 1) ? P.filter : 0.0f);
+    // fireProRenderField (volume-reconstruction.wgsl:100-116): smoke from its own lattice.
+    if (coarse) f.x = renderSmokeAt(bx, cell[0], cell[1], cell[2]);
+    if (!(f.x + glowOf(f.z) > 0.002f)) return;
+    float lightRatio = (float)P.lightDivisor;
+    float glow[3];
+    emission(f, glow);
+    float density = fmax(0.0f, f.x) * P.optical[0];
+    float scat[3] = {0, 0, 0};
+    if (density > 0.0f) {
+      Vec4 inc = lightSample(bx, cell[0] / lightRatio, cell[1] / lightRatio, cell[2] / lightRatio);
+      float irgb[3] = {inc.x, inc.y, inc.z};
+      for (int a = 0; a < 3; a++) scat[a] = P.smoke[a] * (P.ambient[a] + direct[a] * inc.w + irgb[a] * 0.65f) * density;
+    }
+    float ext = extinctionOf(f, P);
+    float tau = ext * ds;
+    float integral = ds * (1.0f - 0.5f * tau + tau * tau / 6.0f);
+    if (tau >= 0.001f) integral = (1.0f - fexp(-tau)) / ext;
+    for (int a = 0; a < 3; a++) radiance[a] += transmittance * (scat[a] + glow[a]) * integral;
+    transmittance *= 1.0f - fmin(1.0f, ext * integral);
+  }
+  // The 8-cell block flags of the sample at p (cell c) in box bx and the skip size.
+  uint8_t blockFlags(const RBox &bx, const int c[3], float &size) const {
+    int pg = pageAt(bx, c[0] >> P.shift, c[1] >> P.shift, c[2] >> P.shift);
+    size = P.brickSize;
+    if (pg < 0) return 0;
+    int m = B - 1;
+    int blk = (((c[2] & m) >> 3) * perSide + ((c[1] & m) >> 3)) * perSide + ((c[0] & m) >> 3);
+    size = P.voxel * 8.0f;
+    return O[(size_t)slotOf(pg) * nO + blk];
+  }
+
+  static constexpr int SLICES = 8;  // VolumeSlices.ts DEPTH_SLICES
+  // buildSlices (volume-slices.wgsl:27-122) for texel (px, py) of an sw x sh grid: the
+  // march's state at SLICES evenly spaced depths from the first entry to the last exit,
+  // into out[k * sw * sh].
+  void slice(int px, int py, int sw, int sh, float *out) const {
+    float uv[2] = {((float)px + 0.5f) / (float)sw, ((float)py + 0.5f) / (float)sh};
+    float o[3], d[3], sx, sy, sz;
+    ray(uv, o, d);
+    span(o, d, sx, sy, sz);
+    float spacing = fmax(sy - sx, 0.0f) / (float)SLICES;
+    float stepSize = fmax(P.voxel * 0.5f, sz / P.raySteps);
+    float jit = jitter((uint32_t)px, (uint32_t)py, P.time);
+    float direct[3];
+    directLight(d, direct);
+    float safe[3], exitSide[3];
+    for (int a = 0; a < 3; a++) {
+      safe[a] = fabs_(d[a]) > 0.000001f ? d[a] : 0.000001f;
+      exitSide[a] = d[a] > 0.0f ? 1.0f : 0.0f;
+    }
+    float t = sx, transmittance = 1.0f, radiance[3] = {0, 0, 0};
+    int recorded = 0, cur = -1;
+    float depth = sx + spacing, curExit = t;
+    auto store = [&](int k) {
+      float *v = out + (size_t)k * sw * sh * 4;
+      v[0] = radiance[0];
+      v[1] = radiance[1];
+      v[2] = radiance[2];
+      v[3] = 1.0f - transmittance;
+    };
+    while (!(recorded >= SLICES || t >= sy || transmittance < 0.008f)) {
+      if (t >= curExit) {
+        float nx = 1e30f, ny = 0.0f;
+        int ni = -1;
+        for (int i = 0; i < (int)boxes.size(); i++) {
+          float en, ex;
+          hit(o, d, boxes[i], en, ex);
+          if (ex > en && ex > t && en < nx) {
+            nx = en;
+            ny = ex;
+            ni = i;
+          }
+        }
+        if (ni < 0) break;
+        cur = ni;
+        curExit = ny;
+        t = fmax(t, nx);
+      } else {
+        const RBox &bx = boxes[cur];
+        // Steps stop at the next recorded depth and at the box's exit.
+        float ds = fmin(stepSize, fmin(depth, curExit) - t);
+        float at = t + jit * ds;
+        float p[3] = {o[0] + d[0] * at, o[1] + d[1] * at, o[2] + d[2] * at};
+        float cell[3] = {p[0] / P.voxel, p[1] / P.voxel, p[2] / P.voxel};
+        int c[3] = {(int)std::floor(cell[0]), (int)std::floor(cell[1]), (int)std::floor(cell[2])};
+        float size;
+        uint8_t flags = blockFlags(bx, c, size);
+        if (!(flags & 2)) {
+          float dist[3];
+          for (int a = 0; a < 3; a++) dist[a] = ((std::floor(p[a] / size) + exitSide[a]) * size - p[a]) / safe[a];
+          float distance = fmin(dist[0], fmin(dist[1], dist[2]));
+          t = fmin(t + fmax(1.0f, std::floor(distance / stepSize) + 1.0f) * stepSize, curExit);
+        } else {
+          shade(bx, cell, flags, direct, ds, transmittance, radiance);
+          t += ds;
+        }
+      }
+      while (recorded < SLICES && !(t < depth)) {
+        store(recorded);
+        recorded++;
+        depth = sx + spacing * (float)(recorded + 1);
+      }
+    }
+    for (int k = recorded; k < SLICES; k++) store(k);
+  }
+
+  void pixel(int px, int py, float out[4], const float *slices = nullptr, int sw = 0, int sh = 0) const {
     out[0] = out[1] = out[2] = out[3] = 0.0f;
     float uv[2] = {((float)px + 0.5f) / (float)P.width, ((float)py + 0.5f) / (float)P.height};
     float clip[2] = {uv[0] * 2.0f + -1.0f, uv[1] * -2.0f + 1.0f};
@@ -831,27 +982,46 @@ struct Renderer {
     if (end <= sx) return;
     float stepSize = fmax(P.voxel * 0.5f, sz / P.raySteps);
     float jit = jitter((uint32_t)px, (uint32_t)py, P.time);
-    // fireProDirect (volume-march.wgsl:141-146)
-    float ln[3];
-    normalize3(P.light, ln);
-    float g = P.optical[3];
-    float mu = d[0] * ln[0] + d[1] * ln[1] + d[2] * ln[2];
-    float base = fmax(0.05f, 1.0f + g * g - 2.0f * g * mu);
-    float phase = (1.0f - g * g) / (base * std::sqrt(base));
     float direct[3];
-    for (int a = 0; a < 3; a++) direct[a] = P.sceneLight[a] * P.light[3] * (0.35f + phase * 0.3f);
-    march(o, d, sx, end, stepSize, jit, direct, out);
+    directLight(d, direct);
+    float t0 = sx, trans0 = 1.0f, rad0[3] = {0, 0, 0};
+    if (slices) {
+      // scene-volume.wgsl:147-161: resume from the last stored depth in front of the scene
+      // (with no scene depth, the last one), a linear fetch at (uv, (k - 1/2) / count):
+      // one layer, bilinear in x and y, clamped to the edge.
+      float count = (float)SLICES;
+      float s = std::floor((end - sx) / (sy - sx) * count);
+      if (s >= 1.0f) {
+        float k = fmin(s, count);
+        float fx0 = uv[0] * (float)sw - 0.5f, fy0 = uv[1] * (float)sh - 0.5f;
+        float gx = std::floor(fx0), gy = std::floor(fy0);
+        float fx = fx0 - gx, fy = fy0 - gy;
+        int x0 = std::min(std::max((int)gx, 0), sw - 1), x1 = std::min(std::max((int)gx + 1, 0), sw - 1);
+        int y0 = std::min(std::max((int)gy, 0), sh - 1), y1 = std::min(std::max((int)gy + 1, 0), sh - 1);
+        const float *layer = slices + (size_t)((int)k - 1) * sw * sh * 4;
+        auto at = [&](int x, int y, int ch) { return layer[((size_t)y * sw + x) * 4 + ch]; };
+        float w00 = (1.0f - fx) * (1.0f - fy), w10 = fx * (1.0f - fy), w01 = (1.0f - fx) * fy, w11 = fx * fy;
+        float v[4];
+        for (int ch = 0; ch < 4; ch++)
+          v[ch] = ((at(x0, y0, ch) * w00 + at(x1, y0, ch) * w10) + at(x0, y1, ch) * w01) + at(x1, y1, ch) * w11;
+        t0 = sx + (sy - sx) * k / count;
+        trans0 = 1.0f - v[3];
+        for (int a = 0; a < 3; a++) rad0[a] = v[a];
+      }
+    }
+    march(o, d, t0, trans0, rad0, end, stepSize, jit, direct, out);
   }
 
-  // fireProMarch (volume-march.wgsl:183-266) from t0 with full transmittance.
-  void march(const float o[3], const float d[3], float t0, float end, float stepSize, float jit,
-             const float direct[3], float out[4]) const {
+  // fireProMarch (volume-march.wgsl:183-266) from t0 with transmittance trans0 and
+  // radiance rad0.
+  void march(const float o[3], const float d[3], float t0, float trans0, const float rad0[3], float end,
+             float stepSize, float jit, const float direct[3], float out[4]) const {
     float safe[3], exitSide[3];
     for (int a = 0; a < 3; a++) {
       safe[a] = fabs_(d[a]) > 0.000001f ? d[a] : 0.000001f;
       exitSide[a] = d[a] > 0.0f ? 1.0f : 0.0f;
     }
-    float t = t0, transmittance = 1.0f, radiance[3] = {0, 0, 0};
+    float t = t0, transmittance = trans0, radiance[3] = {rad0[0], rad0[1], rad0[2]};
     int cur = -1;
     float curExit = t;
     float lightRatio = (float)P.lightDivisor;
@@ -932,8 +1102,18 @@ struct Renderer {
   }
 
   void image(float *rgba) const {
+    std::vector<float> slices;
+    int sw = 0, sh = 0;
+    if (P.halfRes) {
+      sw = std::max(1, (P.width + 1) / 2);
+      sh = std::max(1, (P.height + 1) / 2);
+      slices.assign((size_t)sw * sh * SLICES * 4, 0.0f);
+      for (int y = 0; y < sh; y++)
+        for (int x = 0; x < sw; x++) slice(x, y, sw, sh, &slices[((size_t)y * sw + x) * 4]);
+    }
     for (int y = 0; y < P.height; y++)
-      for (int x = 0; x < P.width; x++) pixel(x, y, &rgba[((size_t)y * P.width + x) * 4]);
+      for (int x = 0; x < P.width; x++)
+        pixel(x, y, &rgba[((size_t)y * P.width + x) * 4], P.halfRes ? slices.data() : nullptr, sw, sh);
   }
 };
 

@@ -86,8 +86,9 @@ def test_render_schedule_every_step():
     assert 2 in works and 1 in works, works
 
 
-@pytest.mark.parametrize("divisor,filt,sun", [(1, "cubic", 3.0), (2, "quadratic", 0.0), (4, "trilinear", 1.0)])
-def test_render_options(divisor, filt, sun):
+@pytest.mark.parametrize("divisor,filt,sun,half", [(1, "cubic", 3.0, False), (2, "quadratic", 0.0, True),
+                                                   (4, "trilinear", 1.0, False), (4, "cubic", 3.0, True)])
+def test_render_options(divisor, filt, sun, half):
     from firepro import scenes
     abi = _abi()
     sc = scenes.small(steps=6, brick=16, divisor=2)
@@ -97,10 +98,10 @@ def test_render_options(divisor, filt, sun):
             apply(gpu, cpu, g)
             gpu.step(frame)
             cpu.step(frame)
-        cam = abi.camera((2.5, 2.0, -3.0), (0.0, 0.5, 0.0), 40, 40, 32)
+        cam = abi.camera((2.5, 2.0, -3.0), (0.0, 0.5, 0.0), 40, 41, 33)
         rs = abi.render_settings(lighting_divisor=divisor, filter=filt, sun_intensity=sun,
-                                 ambient=(0.05, 0.06, 0.08), ray_steps=64)
-        print(f"case options divisor={divisor} filter={filt} sun={sun}", flush=True)
+                                 ambient=(0.05, 0.06, 0.08), ray_steps=64, half_resolution=half)
+        print(f"case options divisor={divisor} filter={filt} sun={sun} half={half}", flush=True)
         msgs, _ = render_mismatches(gpu, cpu, cam, rs, "options")
         assert msgs == []
     finally:

@@ -254,7 +254,8 @@ __global__ void k_smooth_smoke(RenderArgs r, int chunks, int staged);
 __global__ void k_smooth_coarse(RenderArgs r);
 __global__ void k_occupancy(RenderArgs r);
 __global__ void k_light(RenderArgs r, int fresh);
-__global__ void k_march(RenderArgs r, float4 *out);
+__global__ void k_march(RenderArgs r, float4 *out, const float4 *slices, int sw, int sh);
+__global__ void k_slices(RenderArgs r, float4 *slices, int sw, int sh);
 // Scene lights: per-brick totals by nearest anchor, then per-anchor totals in a fixed order
 // (blocks of LIGHT_BLOCK bricks).
 constexpr int LIGHT_BLOCK = 256;

@@ -18,6 +18,7 @@ namespace fp {
 
 constexpr int SHADOW_STEPS = 32;  // types.ts SHADOW_STEPS
 constexpr int LUT_SIZE = 512;     // blackbody.ts BLACKBODY_TABLE_SIZE
+constexpr int SLICES = 8;         // VolumeSlices.ts DEPTH_SLICES
 
 struct RBox {
   int ox, oy, oz, first, sx, sy, sz;
@@ -667,13 +668,8 @@ __device__ __forceinline__ float jitter(uint32_t px, uint32_t py, float time) {
 
 // fireProSceneVolume's beauty path (scene-volume.wgsl:38-48, :146-167) with the march of
 // volume-march.wgsl:183-266: a lane per pixel, 16 x 16 pixel tiles per workgroup.
-__global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out) {
-  const RenderParams &P = r.P;
-  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (px >= P.width || py >= P.height) return;
-  float4 result = make_float4(0.f, 0.f, 0.f, 0.f);
-  // fireProRay (volume-march.wgsl:18-24)
-  float uvx = ((float)px + 0.5f) / (float)P.width, uvy = ((float)py + 0.5f) / (float)P.height;
+// fireProRay (volume-march.wgsl:18-24): the ray through screen position (uvx, uvy).
+__device__ __forceinline__ void rayOf(const RenderParams &P, float uvx, float uvy, float o[3], float d[3]) {
   float cx = uvx * 2.0f + -1.0f, cy = uvy * -2.0f + 1.0f;
   float nh[4], fh[4];
 #pragma unroll
@@ -681,69 +677,72 @@ __global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out) {
     nh[k] = ((P.ivp[k] * cx + P.ivp[4 + k] * cy) + P.ivp[8 + k] * 0.0f) + P.ivp[12 + k] * 1.0f;
     fh[k] = ((P.ivp[k] * cx + P.ivp[4 + k] * cy) + P.ivp[8 + k] * 0.99999f) + P.ivp[12 + k] * 1.0f;
   }
-  float o[3], d[3], dd[3];
+  float dd[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) o[a] = nh[a] / nh[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) dd[a] = fh[a] / fh[3] - o[a];
   normalize3(dd, d);
-  // fireProSpan (volume-march.wgsl:39-48)
-  float sx = 1e30f, sy = 0.0f, sz = 0.0f;
-  for (int i = 0; i < r.boxCount; i++) {
-    float en, ex;
-    hit(r, o, d, loadBox(r, i), en, ex);
-    if (ex > en) {
-      sx = fminv(sx, en);
-      sy = fmaxv(sy, ex);
-      sz = sz + ex - en;
-    }
+}
+
+// A ray's march through the boxes (fireProMarch, volume-march.wgsl:183-266), from t with
+// the state (transmittance, radiance) so far. With `count` > 0 it is the half-resolution
+// march of buildSlices (volume-slices.wgsl:27-122): steps also stop at each of `count`
+// evenly spaced depths from `sx` to `sy`, where the state is stored into `slots`.
+struct MarchState {
+  float t, transmittance, rad[3];
+};
+__device__ __forceinline__ void march(const RenderArgs &r, const float o[3], const float d[3], float stepSize, float jit,
+                                      float end, MarchState &st, int count = 0, float sx = 0.f, float sy = 0.f,
+                                      float4 *slots = nullptr, size_t stride = 0) {
+  const RenderParams &P = r.P;
+  // fireProDirect (volume-march.wgsl:141-146)
+  float ln[3];
+  normalize3(P.light, ln);
+  const float g = P.optical[3];
+  float mu = d[0] * ln[0] + d[1] * ln[1] + d[2] * ln[2];
+  float base = fmaxv(0.05f, 1.0f + g * g - 2.0f * g * mu);
+  float phase = (1.0f - g * g) / (base * sqrtf(base));
+  float direct[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) direct[a] = P.sceneLight[a] * P.light[3] * (0.35f + phase * 0.3f);
+  float safe[3], exitSide[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    safe[a] = fabsv(d[a]) > 0.000001f ? d[a] : 0.000001f;
+    exitSide[a] = d[a] > 0.0f ? 1.0f : 0.0f;
   }
-  if (sx < sy) {
-    const float end = sy;
-    const float stepSize = fmaxv(P.voxel * 0.5f, sz / P.raySteps);
-    const float jit = jitter((uint32_t)px, (uint32_t)py, P.time);
-    // fireProDirect (volume-march.wgsl:141-146)
-    float ln[3];
-    normalize3(P.light, ln);
-    const float g = P.optical[3];
-    float mu = d[0] * ln[0] + d[1] * ln[1] + d[2] * ln[2];
-    float base = fmaxv(0.05f, 1.0f + g * g - 2.0f * g * mu);
-    float phase = (1.0f - g * g) / (base * sqrtf(base));
-    float direct[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) direct[a] = P.sceneLight[a] * P.light[3] * (0.35f + phase * 0.3f);
-    float safe[3], exitSide[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      safe[a] = fabsv(d[a]) > 0.000001f ? d[a] : 0.000001f;
-      exitSide[a] = d[a] > 0.0f ? 1.0f : 0.0f;
-    }
-    float t = sx, transmittance = 1.0f, rad[3] = {0.f, 0.f, 0.f};
-    RBox bx{0, 0, 0, 0, 0, 0, 0};
-    float curExit = t;
-    const float lightRatio = (float)P.lightDivisor;
-    const int m = r.B - 1, ps = r.perSide;
-    while (true) {
-      if (t >= end || transmittance < 0.008f) break;
-      if (t >= curExit) {
-        float nx = 1e30f, ny = 0.0f;
-        int ni = -1;
-        for (int i = 0; i < r.boxCount; i++) {
-          float en, ex;
-          hit(r, o, d, loadBox(r, i), en, ex);
-          if (ex > en && ex > t && en < nx) {
-            nx = en;
-            ny = ex;
-            ni = i;
-          }
+  float t = st.t, transmittance = st.transmittance, rad[3] = {st.rad[0], st.rad[1], st.rad[2]};
+  RBox bx{0, 0, 0, 0, 0, 0, 0};
+  float curExit = t;
+  const float lightRatio = (float)P.lightDivisor;
+  const int m = r.B - 1, ps = r.perSide;
+  const bool record = count > 0;
+  const float spacing = record ? fmaxv(sy - sx, 0.0f) / (float)count : 0.0f;
+  float depth = sx + spacing;
+  int recorded = 0;
+  while (true) {
+    if (record ? (recorded >= count || t >= sy || transmittance < 0.008f) : (t >= end || transmittance < 0.008f)) break;
+    if (t >= curExit) {
+      // fireProEnter (volume-march.wgsl:160-172)
+      float nx = 1e30f, ny = 0.0f;
+      int ni = -1;
+      for (int i = 0; i < r.boxCount; i++) {
+        float en, ex;
+        hit(r, o, d, loadBox(r, i), en, ex);
+        if (ex > en && ex > t && en < nx) {
+          nx = en;
+          ny = ex;
+          ni = i;
         }
-        if (ni < 0) break;
-        bx = loadBox(r, ni);
-        curExit = ny;
-        t = fmaxv(t, nx);
-        continue;
       }
-      float ds = fminv(stepSize, fminv(end, curExit) - t);
+      if (ni < 0) break;
+      bx = loadBox(r, ni);
+      curExit = ny;
+      t = fmaxv(t, nx);
+      if (!record) continue;
+    } else {
+      float ds = fminv(stepSize, fminv(record ? depth : end, curExit) - t);
       float at = t + jit * ds;
       float p[3] = {o[0] + d[0] * at, o[1] + d[1] * at, o[2] + d[2] * at};
       float cell[3] = {p[0] / P.voxel, p[1] / P.voxel, p[2] / P.voxel};
@@ -765,34 +764,131 @@ __global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out) {
         }
         float distance = fminv(dist[0], fminv(dist[1], dist[2]));
         t = fminv(t + fmaxv(1.0f, floorf(distance / stepSize) + 1.0f) * stepSize, curExit);
-        continue;
-      }
-      float4 f = reconstruct(r, bx, cell, (flags & 1u) ? P.filter : 0.0f);
-      // Smoke was smoothed once on its own lattice (fireProRenderField, volume-reconstruction.wgsl:100-116).
-      if (r.coarse) f.x = renderSmokeAt(r, bx, cell[0], cell[1], cell[2]);
-      if (f.x + glowOf(f.z) > 0.002f) {
-        // fireProShade (volume-march.wgsl:103-138)
-        float glow[3];
-        emission(r, f, glow);
-        float density = fmaxv(0.0f, f.x) * P.optical[0];
-        float scat[3] = {0.f, 0.f, 0.f};
-        if (density > 0.0f) {
-          float4 inc = lightSample(r, bx, cell[0] / lightRatio, cell[1] / lightRatio, cell[2] / lightRatio);
-          float irgb[3] = {inc.x, inc.y, inc.z};
+      } else {
+        float4 f = reconstruct(r, bx, cell, (flags & 1u) ? P.filter : 0.0f);
+        // Smoke was smoothed once on its own lattice (fireProRenderField, volume-reconstruction.wgsl:100-116).
+        if (r.coarse) f.x = renderSmokeAt(r, bx, cell[0], cell[1], cell[2]);
+        if (f.x + glowOf(f.z) > 0.002f) {
+          // fireProShade (volume-march.wgsl:103-138)
+          float glow[3];
+          emission(r, f, glow);
+          float density = fmaxv(0.0f, f.x) * P.optical[0];
+          float scat[3] = {0.f, 0.f, 0.f};
+          if (density > 0.0f) {
+            float4 inc = lightSample(r, bx, cell[0] / lightRatio, cell[1] / lightRatio, cell[2] / lightRatio);
+            float irgb[3] = {inc.x, inc.y, inc.z};
 #pragma unroll
-          for (int a = 0; a < 3; a++) scat[a] = P.smoke[a] * (P.ambient[a] + direct[a] * inc.w + irgb[a] * 0.65f) * density;
+            for (int a = 0; a < 3; a++)
+              scat[a] = P.smoke[a] * (P.ambient[a] + direct[a] * inc.w + irgb[a] * 0.65f) * density;
+          }
+          float ext = extinctionOf(f, P);
+          float tau = ext * ds;
+          float integral = ds * (1.0f - 0.5f * tau + tau * tau / 6.0f);
+          if (tau >= 0.001f) integral = (1.0f - fexp(-tau)) / ext;
+#pragma unroll
+          for (int a = 0; a < 3; a++) rad[a] += transmittance * (scat[a] + glow[a]) * integral;
+          transmittance *= 1.0f - fminv(1.0f, ext * integral);
         }
-        float ext = extinctionOf(f, P);
-        float tau = ext * ds;
-        float integral = ds * (1.0f - 0.5f * tau + tau * tau / 6.0f);
-        if (tau >= 0.001f) integral = (1.0f - fexp(-tau)) / ext;
-#pragma unroll
-        for (int a = 0; a < 3; a++) rad[a] += transmittance * (scat[a] + glow[a]) * integral;
-        transmittance *= 1.0f - fminv(1.0f, ext * integral);
+        t += ds;
       }
-      t += ds;
     }
-    result = make_float4(rad[0], rad[1], rad[2], 1.0f - transmittance);
+    if (!record) continue;
+    // A skip or a jump between boxes can pass several depths; each keeps the current values.
+    while (recorded < count && t >= depth) {
+      slots[recorded * stride] = make_float4(rad[0], rad[1], rad[2], 1.0f - transmittance);
+      recorded++;
+      depth = sx + spacing * (float)(recorded + 1);
+    }
+  }
+  // Depths past the exit or an opaque ray hold the final values.
+  for (int k = recorded; k < count; k++) slots[k * stride] = make_float4(rad[0], rad[1], rad[2], 1.0f - transmittance);
+  st.t = t;
+  st.transmittance = transmittance;
+  st.rad[0] = rad[0];
+  st.rad[1] = rad[1];
+  st.rad[2] = rad[2];
+}
+
+// fireProSpan (volume-march.wgsl:39-48): the first entry, the last exit and the length
+// inside the boxes.
+__device__ __forceinline__ void spanOf(const RenderArgs &r, const float o[3], const float d[3], float &sx, float &sy,
+                                       float &sz) {
+  sx = 1e30f;
+  sy = 0.0f;
+  sz = 0.0f;
+  for (int i = 0; i < r.boxCount; i++) {
+    float en, ex;
+    hit(r, o, d, loadBox(r, i), en, ex);
+    if (ex > en) {
+      sx = fminv(sx, en);
+      sy = fmaxv(sy, ex);
+      sz = sz + ex - en;
+    }
+  }
+}
+
+// buildSlices (volume-slices.wgsl:27-122): the half-resolution march, a lane per texel of
+// a (ceil(w/2), ceil(h/2)) grid, storing its state at SLICES evenly spaced depths from the
+// first box entry to the last exit: slices[(k * sh + y) * sw + x].
+__global__ void __launch_bounds__(256) k_slices(RenderArgs r, float4 *slices, int sw, int sh) {
+  const RenderParams &P = r.P;
+  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= sw || py >= sh) return;
+  float o[3], d[3], sx, sy, sz;
+  rayOf(P, ((float)px + 0.5f) / (float)sw, ((float)py + 0.5f) / (float)sh, o, d);
+  spanOf(r, o, d, sx, sy, sz);
+  const float stepSize = fmaxv(P.voxel * 0.5f, sz / P.raySteps);
+  MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}};
+  march(r, o, d, stepSize, jitter((uint32_t)px, (uint32_t)py, P.time), sy, st, SLICES, sx, sy,
+        slices + ((size_t)py * sw + px), (size_t)sw * sh);
+}
+
+// fireProSceneVolume's beauty path (scene-volume.wgsl:38-48, :146-167) with the march of
+// volume-march.wgsl:183-266: a lane per pixel, 16 x 16 pixel tiles per workgroup. With
+// `slices` (half resolution), the march resumes from the last stored depth in front of the
+// scene, sampled bilinearly; with no scene depth that is the last one.
+__global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out, const float4 *slices, int sw, int sh) {
+  const RenderParams &P = r.P;
+  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= P.width || py >= P.height) return;
+  float4 result = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float uvx = ((float)px + 0.5f) / (float)P.width, uvy = ((float)py + 0.5f) / (float)P.height;
+  float o[3], d[3], sx, sy, sz;
+  rayOf(P, uvx, uvy, o, d);
+  spanOf(r, o, d, sx, sy, sz);
+  if (sx < sy) {
+    const float end = sy;
+    const float stepSize = fmaxv(P.voxel * 0.5f, sz / P.raySteps);
+    MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}};
+    if (slices) {
+      const float count = (float)SLICES;
+      const float slice = floorf((end - sx) / (sy - sx) * count);
+      if (slice >= 1.0f) {
+        const float k = fminv(slice, count);
+        // A linear fetch at (uv, (k - 1/2) / count): one depth layer, bilinear in x and y,
+        // clamped to the edge.
+        const float fx0 = uvx * (float)sw - 0.5f, fy0 = uvy * (float)sh - 0.5f;
+        const float gx = floorf(fx0), gy = floorf(fy0);
+        const float fx = fx0 - gx, fy = fy0 - gy;
+        const int x0 = min(max((int)gx, 0), sw - 1), x1 = min(max((int)gx + 1, 0), sw - 1);
+        const int y0 = min(max((int)gy, 0), sh - 1), y1 = min(max((int)gy + 1, 0), sh - 1);
+        const float4 *layer = slices + (size_t)((int)k - 1) * sw * sh;
+        const float4 v00 = layer[(size_t)y0 * sw + x0], v10 = layer[(size_t)y0 * sw + x1];
+        const float4 v01 = layer[(size_t)y1 * sw + x0], v11 = layer[(size_t)y1 * sw + x1];
+        const float w00 = (1.0f - fx) * (1.0f - fy), w10 = fx * (1.0f - fy), w01 = (1.0f - fx) * fy, w11 = fx * fy;
+        const float4 sv = make_float4(((v00.x * w00 + v10.x * w10) + v01.x * w01) + v11.x * w11,
+                                      ((v00.y * w00 + v10.y * w10) + v01.y * w01) + v11.y * w11,
+                                      ((v00.z * w00 + v10.z * w10) + v01.z * w01) + v11.z * w11,
+                                      ((v00.w * w00 + v10.w * w10) + v01.w * w01) + v11.w * w11);
+        st.t = sx + (sy - sx) * k / count;
+        st.transmittance = 1.0f - sv.w;
+        st.rad[0] = sv.x;
+        st.rad[1] = sv.y;
+        st.rad[2] = sv.z;
+      }
+    }
+    march(r, o, d, stepSize, jitter((uint32_t)px, (uint32_t)py, P.time), end, st);
+    result = make_float4(st.rad[0], st.rad[1], st.rad[2], 1.0f - st.transmittance);
   }
   out[(size_t)py * P.width + px] = result;
 }

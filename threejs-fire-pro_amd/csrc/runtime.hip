@@ -463,6 +463,7 @@ struct fp_sim {
     DBuf<int> table;
     DBuf<float4> R, L, lut, image;
     DBuf<float> RS;  // smoothed coarse smoke with aprons (smokeDivisor 2 or 4)
+    DBuf<float4> slices;  // the half-resolution march's depth slices
     DBuf<uint8_t> O;
     // Scene lights: anchors, per-brick totals and anchors, per-block totals, the totals.
     DBuf<float4> anchors;
@@ -1706,8 +1707,21 @@ struct fp_sim {
     }
     RenderArgs r = renderArgs(P);
     dim3 grid((unsigned)((P.width + 15) / 16), (unsigned)((P.height + 15) / 16));
+    const float4 *slices = nullptr;
+    int sw = 0, sh = 0;
+    if (s.half_resolution) {
+      // VolumeSlices.march (VolumeSlices.ts:83-131): half the size on each axis, 8 depths.
+      sw = std::max(1, (P.width + 1) / 2);
+      sh = std::max(1, (P.height + 1) / 2);
+      size_t n = (size_t)sw * sh * 8;
+      if (rc.slices.n < n) rc.slices.alloc(n);
+      int t0 = begin();
+      k_slices<<<dim3((unsigned)((sw + 15) / 16), (unsigned)((sh + 15) / 16)), 256, 0, stream>>>(r, rc.slices.p, sw, sh);
+      end(t0, "render_slices");
+      slices = rc.slices.p;
+    }
     int t = begin();
-    k_march<<<grid, 256, 0, stream>>>(r, target);
+    k_march<<<grid, 256, 0, stream>>>(r, target, slices, sw, sh);
     end(t, "render_march");
     HIPCHECK(hipGetLastError());
     if (!deviceOut) {
@@ -1870,6 +1884,7 @@ struct fp_sim {
     rc.table.release();
     rc.R.release();
     rc.RS.release();
+    rc.slices.release();
     rc.L.release();
     rc.lut.release();
     rc.image.release();

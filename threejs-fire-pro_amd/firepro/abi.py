@@ -129,6 +129,7 @@ class RenderSettings(C.Structure):
         ("sun_direction", Vec3), ("sun_intensity", C.c_double),
         ("sun_color", Vec3), ("ambient", Vec3),
         ("filter", C.c_int32), ("ray_steps", C.c_int32), ("lighting_divisor", C.c_int32),
+        ("half_resolution", C.c_int32),
     ]
 
 
@@ -167,6 +168,7 @@ RENDER_DEFAULTS = {
     "shadow_density": 3.5, "smoke_color": hex_color(0x25292E),
     "sun_direction": (0.5, 1.0, 0.3), "sun_intensity": 3.0, "sun_color": (1.0, 1.0, 1.0),
     "ambient": (0.0, 0.0, 0.0), "filter": "cubic", "ray_steps": 256, "lighting_divisor": 4,
+    "half_resolution": True,  # options.ts:364
 }
 
 
@@ -186,6 +188,7 @@ def render_settings(**overrides) -> RenderSettings:
     rs.filter = FILTERS.index(f) if isinstance(f, str) else int(f)
     rs.ray_steps = int(o["ray_steps"])
     rs.lighting_divisor = int(o["lighting_divisor"])
+    rs.half_resolution = int(bool(o["half_resolution"]))
     return rs
 
 
