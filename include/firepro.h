@@ -104,9 +104,14 @@ typedef struct fp_force {
 
 /* SimulationCollider (src/engine/types.ts:271-281), sphere or box. */
 typedef struct fp_collider {
-  int32_t shape;
+  int32_t shape;     /* FP_SHAPE_SPHERE, FP_SHAPE_BOX or FP_SHAPE_MESH */
   double position[3], size[3];
   double axes[3][3]; /* the box's axes in world space (rows); world axes by default */
+  /* FP_SHAPE_MESH: the mesh's occupancy of a box of field cells (MeshColliderMask,
+   * MeshCollider.ts voxelize): size[0] * size[1] * size[2] words, nonzero inside, x fastest,
+   * for the cells from lo; position, size and axes are then unused. */
+  const uint32_t *occupancy;
+  int32_t lo[3], cells[3];
 } fp_collider;
 
 /* SimulationFrame (src/engine/types.ts:295-302). */

@@ -72,8 +72,10 @@ template <class R> struct Emitter {
 };
 template <class R> struct Collider {
   R center[3], extent[3];
-  bool box;
+  int box;       // 0 sphere, 1 box, 2 mesh occupancy
   R axes[3][3];  // columns a, b, c
+  int lo[3] = {0, 0, 0}, size[3] = {0, 0, 0};
+  std::vector<uint8_t> inside;  // mesh: occupancy of its box of cells, x fastest
 };
 template <class R> struct Region {
   R center[3];
@@ -294,6 +296,7 @@ template <class R> class Sim {
   bool solidsStale = true, geometryDirty = true;
   int colliderCount = 0;
   std::vector<uint32_t> prevColliders;
+  std::vector<uint8_t> prevOccupancy;
   std::vector<uint64_t> sourceIds;
   std::vector<std::pair<std::array<int, 3>, uint32_t>> contentTiles;
   int64_t contentStep = -1;
@@ -1002,9 +1005,30 @@ template <class R> class Sim {
     // Colliders: 28 words each, compared with the last upload (BufferUploads.ts:6-16).
     int n = f.collider_count;
     std::vector<uint32_t> packed((size_t)std::max(1, n) * 28, 0u);
+    std::vector<uint8_t> occupancy;
     colliders.clear();
     for (int i = 0; i < n; i++) {
       const fp_collider &c = f.colliders[i];
+      if (c.shape == FP_SHAPE_MESH) {
+        // A mesh collider's occupancy of its box of cells (FluidSimulation.ts:1221-1260).
+        if (!c.occupancy || c.cells[0] <= 0 || c.cells[1] <= 0 || c.cells[2] <= 0)
+          throw Error("Mesh collider occupancy does not match its size.");
+        Collider<R> cr{};
+        cr.box = 2;
+        size_t cells = (size_t)c.cells[0] * c.cells[1] * c.cells[2];
+        cr.inside.resize(cells);
+        for (size_t k = 0; k < cells; k++) cr.inside[k] = c.occupancy[k] != 0;
+        for (int a = 0; a < 3; a++) {
+          cr.lo[a] = c.lo[a];
+          cr.size[a] = c.cells[a];
+          packed[(size_t)i * 28 + 20 + a] = (uint32_t)c.lo[a];
+          packed[(size_t)i * 28 + 24 + a] = (uint32_t)c.cells[a];
+        }
+        packed[(size_t)i * 28 + 3] = 2;
+        occupancy.insert(occupancy.end(), cr.inside.begin(), cr.inside.end());
+        colliders.push_back(cr);
+        continue;
+      }
       float w[20] = {0};
       for (int a = 0; a < 3; a++) {
         w[a] = (float)c.position[a];
@@ -1013,7 +1037,7 @@ template <class R> class Sim {
       }
       w[3] = c.shape == FP_SHAPE_BOX ? 1.0f : 0.0f;
       std::memcpy(&packed[(size_t)i * 28], w, sizeof(w));
-      Collider<R> cr;
+      Collider<R> cr{};
       for (int a = 0; a < 3; a++) {
         cr.center[a] = (R)w[a];
         cr.extent[a] = (R)w[4 + a];
@@ -1022,8 +1046,9 @@ template <class R> class Sim {
       cr.box = c.shape == FP_SHAPE_BOX;
       colliders.push_back(cr);
     }
-    bool changed = packed != prevColliders || n != colliderCount;
+    bool changed = packed != prevColliders || n != colliderCount || occupancy != prevOccupancy;
     prevColliders = packed;
+    prevOccupancy = occupancy;
     colliderCount = n;
     // Forces (ForceFields.ts:41-114), analytic targets only.
     forces.clear();
@@ -1251,6 +1276,12 @@ template <class R> class Sim {
   }
 
   bool insideCollider(const Collider<R> &c, int x, int y, int z) const {
+    if (c.box == 2) {
+      int q[3] = {x - c.lo[0], y - c.lo[1], z - c.lo[2]};
+      for (int a = 0; a < 3; a++)
+        if (q[a] < 0 || q[a] >= c.size[a]) return false;
+      return c.inside[((size_t)q[2] * c.size[1] + q[1]) * c.size[0] + q[0]] != 0;
+    }
     R w[3] = {((R)x + R(0.5)) * P.hf, ((R)y + R(0.5)) * P.hf, ((R)z + R(0.5)) * P.hf};
     R rel[3] = {w[0] - c.center[0], w[1] - c.center[1], w[2] - c.center[2]};
     R dd[3];
@@ -1258,7 +1289,7 @@ template <class R> class Sim {
       R proj = c.axes[k][0] * rel[0] + c.axes[k][1] * rel[1] + c.axes[k][2] * rel[2];
       dd[k] = fabs_(proj / c.extent[k]);
     }
-    if (c.box) return dd[0] < R(1) && dd[1] < R(1) && dd[2] < R(1);
+    if (c.box == 1) return dd[0] < R(1) && dd[1] < R(1) && dd[2] < R(1);
     return dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] < R(1);
   }
   void drawSolids(int id, int page) {

@@ -61,12 +61,13 @@ def test_coarse_smoke_refuses_batch():
         gpu.dispose()
 
 
-@pytest.mark.parametrize("smoke", [1, 2])
-def test_mesh_emitters_bit_exact(smoke):
+@pytest.mark.parametrize("smoke,colliders", [(1, False), (2, False), (1, True)])
+def test_mesh_emitters_bit_exact(smoke, colliders):
     """Mesh emitters (box, torus, disk): the rasterized records and brick tables drive
-    allocation, emission and the source velocity identically on both sides."""
-    print(f"case meshes smoke={smoke}", flush=True)
-    assert run_lockstep(_scenes().meshes(steps=10, smoke=smoke)) == []
+    allocation, emission and the source velocity identically on both sides; with mesh
+    colliders (a tilted box, an octahedron) their occupancy draws the same solids."""
+    print(f"case meshes smoke={smoke} colliders={colliders}", flush=True)
+    assert run_lockstep(_scenes().meshes(steps=10, smoke=smoke, colliders=colliders)) == []
 
 
 def test_first_order_scalars_bit_exact():

@@ -104,3 +104,83 @@ def test_mesh_emission_conserves_coverage():
     dt = 1 / 60
     half = math.exp(-frame.cooling * dt * 0.5)
     assert heat == pytest.approx(3.0 * dt * half * total_mask, rel=1e-5)
+
+
+def _inside_count(mask, lo, spacing, inside):
+    """The cells of the box whose centers `inside` (world coordinates) accepts, and the mask's."""
+    nz, ny, nx = mask.shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    centers = (np.stack([x, y, z], -1) + np.array(lo) + 0.5) * spacing
+    return inside(centers), mask.astype(bool)
+
+
+@pytest.mark.parametrize("rotation", [(0.0, 0.0, 0.0), (0.3, 0.2, 0.1)])
+def test_voxelized_box_is_its_cell_centers(rotation):
+    """MeshCollider.voxelize of a closed box marks exactly the cells whose centers lie inside
+    the placed box (a cell of margin around the bounds stays empty)."""
+    pos, cell = (0.1, 0.5, -0.05), 0.05
+    m = mesh.compose(pos, rotation)
+    mask, lo, size = mesh.voxelize(mesh.box(0.5, 0.3, 0.4), m, cell)
+    assert mask.shape == (size[2], size[1], size[0])
+    r = m[:3, :3]
+
+    def inside(c):
+        local = (c - np.array(pos)) @ r  # world to the box's axes
+        return (np.abs(local[..., 0]) < 0.25) & (np.abs(local[..., 1]) < 0.15) & (np.abs(local[..., 2]) < 0.2)
+
+    want, got = _inside_count(mask, lo, cell, inside)
+    # Cells whose centers sit within rounding of a face may go either way.
+    local = ((np.stack(np.meshgrid(*[np.arange(n) for n in size[::-1]], indexing="ij")[::-1], -1)
+              + np.array(lo) + 0.5) * cell - np.array(pos)) @ r
+    margin = np.min(np.abs(np.abs(local) - np.array([0.25, 0.15, 0.2])), axis=-1) < 1e-9
+    assert np.array_equal(want[~margin], got[~margin])
+    assert got.sum() > 100 and not got[0].any() and not got[-1].any()
+
+
+def test_voxelized_octahedron_and_closedness():
+    """A closed octahedron voxelizes to the cells with |x| + |y| + |z| < r about its center;
+    an open mesh and an inconsistently wound one are refused."""
+    cell, r, c0 = 0.05, 0.31, np.array([0.02, 0.4, -0.03])
+    mask, lo, _ = mesh.voxelize(mesh.octahedron(r), mesh.compose(c0, (0, 0, 0)), cell)
+    want, got = _inside_count(mask, lo, cell, lambda c: np.abs(c - c0).sum(-1) < r)
+    assert np.array_equal(want, got) and got.sum() > 100
+    box = mesh.box(0.3, 0.2, 0.2)
+    mesh.check_closed(box)
+    with pytest.raises(ValueError, match="closed"):
+        mesh.check_closed(mesh.Geometry(box.positions, box.indices[:-3]))
+    flipped = box.indices.copy()
+    flipped[:3] = flipped[:3][::-1]
+    with pytest.raises(ValueError, match="consistently wound"):
+        mesh.check_closed(mesh.Geometry(box.positions, flipped))
+
+
+def test_mesh_collider_solids_on_the_oracle():
+    """The oracle draws a mesh collider's occupancy into the solids: with a mesh box over
+    the emitter, cells inside the box get no heat, cells beside it do."""
+    import oracle
+    from firepro import scenes
+    sc = scenes.meshes(steps=3, colliders=True)
+    orc = oracle.OracleSimulation.create(sc.grid())
+    for frame, g in sc.frames():
+        orc.apply_grid(g)
+        orc.step(frame)
+    frame = sc.sim.build_frame()
+    assert [c.shape for c in frame.colliders] == ["mesh", "mesh"]
+    st = orc.read_state(velocity=False)
+    B = 8
+    coords = orc.brick_coords(st["ids"])
+    heat = {}
+    for c, f in zip(coords, st["fields"]):
+        cells = np.stack(np.meshgrid(np.arange(B), np.arange(B), np.arange(B), indexing="ij")[::-1], -1).reshape(-1, 3)
+        for q, v in zip(cells + c * B, f):
+            heat[tuple(q)] = v
+    solid = 0
+    for col in frame.colliders:
+        mask, lo, size = col.occupancy
+        for z, y, x in zip(*np.nonzero(mask)):
+            q = (x + lo[0], y + lo[1], z + lo[2])
+            if q in heat:
+                assert not heat[q].any(), q
+                solid += 1
+    hot = [q for q, v in heat.items() if v[1] > 0]
+    assert solid > 50 and len(hot) > 100

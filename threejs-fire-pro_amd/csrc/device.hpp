@@ -29,8 +29,9 @@ struct Emitter {
 };
 struct Collider {
   float center[3], extent[3];
-  int box;
+  int box;           // 0 sphere, 1 box, 2 mesh occupancy
   float axes[3][3];  // columns
+  int lo[3], size[3], maskOffset;  // mesh: its box of field cells and its first mask word
 };
 struct Region {
   float center[3], kind, size[3];

@@ -19,6 +19,7 @@ struct AllocArgs {
   int *active, *zeroList;
   const Emitter *emitters;
   const Collider *colliders;
+  const uint32_t *colliderMasks;  // mesh colliders' occupancy, one bit per cell
   uint32_t *solids;
   int colliderCount;
   int buildSolids;  // draw the colliders into solid bits this step

@@ -231,7 +231,14 @@ FP_KERNEL_PAIR(link_bricks, 64, AllocArgs)
 
 // Colliders into the solid bits of one brick (buildSolids, :485-556): a workgroup per
 // brick, a word of 32 cells per lane and pass.
-__device__ __forceinline__ bool insideCollider(const Collider &c, float hf, int x, int y, int z) {
+__device__ __forceinline__ bool insideCollider(const Collider &c, const uint32_t *masks, float hf, int x, int y, int z) {
+  if (c.box == 2) {
+    // A mesh's occupancy of its box of cells (lighting.wgsl insideCollider, fluid-allocation.wgsl:274-282).
+    const int q[3] = {x - c.lo[0], y - c.lo[1], z - c.lo[2]};
+    if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= c.size[0] || q[1] >= c.size[1] || q[2] >= c.size[2]) return false;
+    const uint32_t index = (uint32_t)((q[2] * c.size[1] + q[1]) * c.size[0] + q[0]);
+    return ((masks[c.maskOffset + (index >> 5)] >> (index & 31u)) & 1u) != 0u;
+  }
   float w[3] = {((float)x + 0.5f) * hf, ((float)y + 0.5f) * hf, ((float)z + 0.5f) * hf};
   float rel[3] = {w[0] - c.center[0], w[1] - c.center[1], w[2] - c.center[2]};
   float dd[3];
@@ -239,7 +246,7 @@ __device__ __forceinline__ bool insideCollider(const Collider &c, float hf, int 
     float proj = c.axes[k][0] * rel[0] + c.axes[k][1] * rel[1] + c.axes[k][2] * rel[2];
     dd[k] = fabsv(proj / c.extent[k]);
   }
-  if (c.box) return dd[0] < 1.0f && dd[1] < 1.0f && dd[2] < 1.0f;
+  if (c.box == 1) return dd[0] < 1.0f && dd[1] < 1.0f && dd[2] < 1.0f;
   return dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] < 1.0f;
 }
 
@@ -270,7 +277,7 @@ __device__ __forceinline__ void build_solids_body(const AllocArgs &a, const int 
       int x = b[0] * B + (int)(bit & (B - 1)), y = b[1] * B + (int)((bit >> sh) & (B - 1)),
           z = b[2] * B + (int)(bit >> (2 * sh));
       for (int ci = 0; ci < a.colliderCount; ci++)
-        if (insideCollider(a.colliders[ci], a.P.hf, x, y, z)) {
+        if (insideCollider(a.colliders[ci], a.colliderMasks, a.P.hf, x, y, z)) {
           bits |= 1u << k;
           break;
         }

@@ -394,7 +394,7 @@ struct fp_sim {
   int64_t steps = 0;
   bool mayHaveFlame = false, solidsStale = true, geometryDirty = true;
   int colliderCount = 0;
-  std::vector<uint32_t> prevColliders;
+  std::vector<uint32_t> prevColliders, prevMasks;
   std::vector<uint64_t> sourceIds;
   std::vector<std::pair<std::array<int, 3>, uint32_t>> contentTiles;
   int freeCount = 0, lastFailures = 0, lastZeroed = 0;
@@ -473,6 +473,7 @@ struct fp_sim {
   // Scene.
   DBuf<Emitter> emitters;
   DBuf<Collider> colliders;
+  DBuf<uint32_t> colliderMasks;
   DBuf<ForceRec> forces;
   DBuf<Region> regions;
   int emitterCount = 0, forceCount = 0;
@@ -992,11 +993,34 @@ struct fp_sim {
     }
     int n = f.collider_count;
     std::vector<uint32_t> packed((size_t)std::max(1, n) * 28, 0u);
+    std::vector<uint32_t> masks;
     hostColliders.clear();
     for (int i = 0; i < n; i++) {
       const fp_collider &c = f.colliders[i];
+      if (c.shape == FP_SHAPE_MESH) {
+        // A mesh's occupancy, packed one bit per cell (FluidSimulation.ts:1221-1260).
+        if (!c.occupancy || c.cells[0] <= 0 || c.cells[1] <= 0 || c.cells[2] <= 0)
+          throw InvalidArgument("Mesh collider occupancy does not match its size.");
+        Collider cr{};
+        cr.box = 2;
+        cr.maskOffset = (int)masks.size();
+        const size_t cells = (size_t)c.cells[0] * c.cells[1] * c.cells[2];
+        masks.resize(masks.size() + (cells + 31) / 32, 0u);
+        for (size_t k = 0; k < cells; k++)
+          if (c.occupancy[k]) masks[cr.maskOffset + (k >> 5)] |= 1u << (k & 31);
+        int32_t words[8] = {c.lo[0], c.lo[1], c.lo[2], cr.maskOffset, c.cells[0], c.cells[1], c.cells[2], 0};
+        std::memcpy(&packed[(size_t)i * 28 + 20], words, sizeof(words));
+        const float kind = 2.0f;
+        std::memcpy(&packed[(size_t)i * 28 + 3], &kind, 4);
+        for (int a = 0; a < 3; a++) {
+          cr.lo[a] = c.lo[a];
+          cr.size[a] = c.cells[a];
+        }
+        hostColliders.push_back(cr);
+        continue;
+      }
       if (c.shape != FP_SHAPE_SPHERE && c.shape != FP_SHAPE_BOX)
-        throw InvalidArgument("collider.shape.type must be sphere or box.");
+        throw InvalidArgument("collider.shape.type must be sphere, box or mesh.");
       float w[20] = {0};
       for (int a = 0; a < 3; a++) {
         w[a] = (float)c.position[a];
@@ -1005,7 +1029,7 @@ struct fp_sim {
       }
       w[3] = c.shape == FP_SHAPE_BOX ? 1.0f : 0.0f;
       std::memcpy(&packed[(size_t)i * 28], w, sizeof(w));
-      Collider cr;
+      Collider cr{};
       for (int a = 0; a < 3; a++) {
         cr.center[a] = w[a];
         cr.extent[a] = w[4 + a];
@@ -1014,9 +1038,11 @@ struct fp_sim {
       cr.box = c.shape == FP_SHAPE_BOX;
       hostColliders.push_back(cr);
     }
-    bool changed = packed != prevColliders || n != colliderCount;
+    bool changed = packed != prevColliders || n != colliderCount || masks != prevMasks;
     prevColliders = packed;
+    prevMasks = masks;
     colliderCount = n;
+    upload(colliderMasks, masks);
     hostForces.clear();
     hostRegions.clear();
     for (int i = 0; i < f.force_count; i++) {
@@ -1126,6 +1152,7 @@ struct fp_sim {
     a.colliderCount = colliderCount;
     a.buildSolids = (colliderCount > 0 || solidsStale) ? 1 : 0;
     a.solidsEvery = solidsStale ? 1 : 0;
+    a.colliderMasks = colliderMasks.p;
     a.meshTables = meshHost.count() ? meshTables.p : nullptr;
     a.p = pools();
     return a;
@@ -1865,6 +1892,7 @@ struct fp_sim {
     }
     emitters.release();
     colliders.release();
+    colliderMasks.release();
     forces.release();
     regions.release();
     for (int L = 0; L < MAX_LEVELS; L++) {

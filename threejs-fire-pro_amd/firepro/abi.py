@@ -83,7 +83,8 @@ class Force(C.Structure):
 
 class Collider(C.Structure):
     _fields_ = [("shape", C.c_int32), ("position", Vec3), ("size", Vec3),
-                ("axes", (C.c_double * 3) * 3)]
+                ("axes", (C.c_double * 3) * 3),
+                ("occupancy", C.POINTER(C.c_uint32)), ("lo", C.c_int32 * 3), ("cells", C.c_int32 * 3)]
 
 
 class Frame(C.Structure):
@@ -333,6 +334,8 @@ class ColliderData:
     position: Sequence[float]
     size: Sequence[float]
     axes: Sequence[Sequence[float]] = ((1, 0, 0), (0, 1, 0), (0, 0, 1))
+    # Mesh colliders: (occupancy (nz, ny, nx) uint32, lo cell, (nx, ny, nz)) of a box of field cells.
+    occupancy: Optional[tuple] = None
 
 
 @dataclass
@@ -429,6 +432,16 @@ class PackedFrame:
             for k in range(3):
                 for a in range(3):
                     co.axes[k][a] = cd.axes[k][a]
+            if cd.occupancy is not None:
+                mask, lo, cells = cd.occupancy
+                arr = np.ascontiguousarray(mask, np.uint32).reshape(-1)
+                if arr.size != cells[0] * cells[1] * cells[2]:
+                    raise ValueError("Mesh collider occupancy does not match its size.")
+                self.samples.append(arr)
+                co.occupancy = arr.ctypes.data_as(C.POINTER(C.c_uint32))
+                for a in range(3):
+                    co.lo[a] = int(lo[a])
+                    co.cells[a] = int(cells[a])
             cols.append(co)
         self.colliders = (Collider * max(1, len(cols)))(*cols)
         fr.collider_count = len(cols)

@@ -332,21 +332,41 @@ class Force:
 
 
 class Collider:
-    """Sphere or box collider (handles.ts:302-355); `axes` are the box's world axes."""
+    """Sphere, box or mesh collider (handles.ts:284-355); `axes` are the box's world axes.
+    A mesh collider is a closed mesh.Geometry placed by `position` and `rotation` (Euler XYZ),
+    voxelized into field cells again whenever its pose changes (MeshCollider.ts)."""
 
-    def __init__(self, shape: str, position, radius=None, size=None, axes=None):
+    def __init__(self, shape: str, position, radius=None, size=None, axes=None, geometry=None,
+                 rotation=(0.0, 0.0, 0.0), spacing: float = 0.0):
         self.shape = shape
         self.position = list(position)
+        self.rotation = list(rotation)
+        self.geometry = geometry
+        self.spacing = spacing
+        self._mask = None
         if shape == "sphere":
             self.size = (radius,) * 3
         elif shape == "box":
             self.size = tuple(s * 0.5 for s in size)
+        elif shape == "mesh":
+            from . import mesh
+            if not isinstance(geometry, mesh.Geometry):
+                raise ValueError("Mesh collider object must be a mesh.Geometry.")
+            mesh.check_closed(geometry)
+            self.size = (0.0, 0.0, 0.0)
         else:
-            raise ValueError("collider.shape.type must be sphere or box.")
+            raise ValueError("collider.shape.type must be sphere, box or mesh.")
         self.axes = axes or ((1, 0, 0), (0, 1, 0), (0, 0, 1))
 
     def frame(self) -> abi.ColliderData:
-        return abi.ColliderData(self.shape, tuple(self.position), self.size, self.axes)
+        if self.shape != "mesh":
+            return abi.ColliderData(self.shape, tuple(self.position), self.size, self.axes)
+        from . import mesh
+        pose = (tuple(self.position), tuple(self.rotation), self.spacing)
+        if self._mask is None or self._mask[0] != pose:
+            occupancy = mesh.voxelize(self.geometry, mesh.compose(self.position, self.rotation), self.spacing)
+            self._mask = (pose, occupancy)
+        return abi.ColliderData("mesh", tuple(self.position), (0.0, 0.0, 0.0), occupancy=self._mask[1])
 
 
 class FireSimulation:
@@ -409,8 +429,9 @@ class FireSimulation:
         self._forces.append(f)
         return f
 
-    def add_collider(self, shape: str, position, radius=None, size=None, axes=None):
-        c = Collider(shape, position, radius, size, axes)
+    def add_collider(self, shape: str, position, radius=None, size=None, axes=None, geometry=None,
+                     rotation=(0.0, 0.0, 0.0)):
+        c = Collider(shape, position, radius, size, axes, geometry, rotation, self.options["voxelSize"])
         self._colliders.append(c)
         return c
 

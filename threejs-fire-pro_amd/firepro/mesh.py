@@ -203,3 +203,107 @@ def document_geometry(shape: str, radius: float, size: Sequence[float]) -> Geome
     if shape == "torus":
         return rotate_x(torus(radius, radius * 0.4, 12, 40), -math.pi / 2)
     raise ValueError(f"no mesh for emitter shape {shape}.")
+
+
+def check_closed(g: Geometry) -> None:
+    """MeshCollider's constructor (MeshCollider.ts:15-78): every edge of a nondegenerate
+    triangle is shared by exactly two triangles that run it in opposite directions."""
+    tri = g.triangles()
+    edges = {}
+    count = 0
+    for t in tri:
+        a, b, c = t
+        n = np.cross(b - a, c - a)
+        if float(n @ n) == 0:
+            continue
+        count += 1
+        keys = [tuple(float(v) for v in p) for p in t]
+        for e in range(3):
+            u, v = keys[e], keys[(e + 1) % 3]
+            forward = u < v
+            key = (u, v) if forward else (v, u)
+            cnt, wind = edges.get(key, (0, 0))
+            edges[key] = (cnt + 1, wind + (1 if forward else -1))
+    if not count or any(c != 2 or w != 0 for c, w in edges.values()):
+        raise ValueError("Mesh colliders require closed, consistently wound manifold surfaces. "
+                         "Weld matching seam positions and close open edges.")
+
+
+def voxelize(g: Geometry, matrix: np.ndarray, spacing: float) -> Tuple[np.ndarray, Tuple[int, int, int],
+                                                                        Tuple[int, int, int]]:
+    """Collider.meshMask and MeshCollider.voxelize (handles.ts:298-325, MeshCollider.ts:80-171):
+    the cells of the world lattice (cells `spacing` wide from the origin) whose centers the
+    closed mesh placed by `matrix` contains, over the mesh's bounds with a cell of margin.
+    Each triangle is rasterized over the cell centers of the x-y plane with a top-left tie
+    rule; its depth there is a crossing of the z column, signed by its winding; a cell is
+    inside where the crossings below its center sum to nonzero. Returns the occupancy
+    (nz, ny, nx), the box's first cell and its size (nx, ny, nz)."""
+    p = g.positions.astype(np.float64)
+    # Box3.applyMatrix4 of the geometry's bounding box: the bounds of its eight corners.
+    blo, bhi = p.min(axis=0), p.max(axis=0)
+    corners = np.array([[(blo, bhi)[k >> a & 1][a] for a in range(3)] for k in range(8)])
+    world = transform_samples(np.c_[corners, np.zeros(8)], matrix)[:, :3]
+    lo = tuple(int(math.floor(world[:, a].min() / spacing)) - 1 for a in range(3))
+    size = tuple(int(math.ceil(world[:, a].max() / spacing)) + 1 - lo[a] for a in range(3))
+    nx, ny, nz = size
+    tri = g.triangles()
+    verts = transform_samples(np.c_[tri.reshape(-1, 3), np.zeros(len(tri) * 3)], matrix)[:, :3]
+    # The voxelizer's box coordinates: cells from lo (toBox of handles.ts:312-318).
+    cellv = np.empty_like(verts)
+    dom = [size[a] * spacing for a in range(3)]
+    shifted = verts - np.array([(lo[0] + size[0] / 2) * spacing, lo[1] * spacing, (lo[2] + size[2] / 2) * spacing])
+    cellv[:, 0] = (shifted[:, 0] + dom[0] / 2) / spacing
+    cellv[:, 1] = shifted[:, 1] / spacing
+    cellv[:, 2] = (shifted[:, 2] + dom[2] / 2) / spacing
+    cellv = cellv.reshape(-1, 3, 3)
+
+    def edge(a, b, x, y):
+        return (a[0] - x) * (b[1] - y) - (a[1] - y) * (b[0] - x)
+
+    def owns(a, b):
+        return b[1] > a[1] or (b[1] == a[1] and b[0] < a[0])
+
+    columns = [[] for _ in range(nx * ny)]
+    for a, b, c in cellv:
+        area = edge(a, b, c[0], c[1])
+        if area == 0:
+            continue
+        sign = 1 if area > 0 else -1
+        if area < 0:
+            b, c = c, b
+            area = -area
+        x0 = max(0, math.ceil(min(a[0], b[0], c[0]) - 0.5))
+        x1 = min(nx - 1, math.floor(max(a[0], b[0], c[0]) - 0.5))
+        y0 = max(0, math.ceil(min(a[1], b[1], c[1]) - 0.5))
+        y1 = min(ny - 1, math.floor(max(a[1], b[1], c[1]) - 0.5))
+        for y in range(y0, y1 + 1):
+            for x in range(x0, x1 + 1):
+                v = (edge(b, c, x + 0.5, y + 0.5), edge(c, a, x + 0.5, y + 0.5), edge(a, b, x + 0.5, y + 0.5))
+                pairs = ((b, c), (c, a), (a, b))
+                if any(val < 0 or (val == 0 and not owns(*pairs[i])) for i, val in enumerate(v)):
+                    continue
+                columns[y * nx + x].append(((v[0] * a[2] + v[1] * b[2] + v[2] * c[2]) / area, sign))
+    mask = np.zeros((nz, ny, nx), np.uint32)
+    for col, crossings in enumerate(columns):
+        crossings.sort(key=lambda e: e[0])
+        winding, cursor = 0, 0
+        for z in range(nz):
+            while cursor < len(crossings) and crossings[cursor][0] <= z + 0.5:
+                winding += crossings[cursor][1]
+                cursor += 1
+            if winding != 0:
+                mask[z, col // nx, col % nx] = 1
+    return mask, lo, size
+
+
+def octahedron(radius: float) -> Geometry:
+    """A closed octahedron of the given vertex radius, outward winding: the points with
+    |x| + |y| + |z| <= radius."""
+    r = radius
+    pos = [(r, 0, 0), (-r, 0, 0), (0, r, 0), (0, -r, 0), (0, 0, r), (0, 0, -r)]
+    idx = []
+    for sx, x in ((1, 0), (-1, 1)):
+        for sy, y in ((1, 2), (-1, 3)):
+            for sz, z in ((1, 4), (-1, 5)):
+                idx += [x, y, z] if sx * sy * sz > 0 else [x, z, y]
+    return Geometry(np.array(pos, np.float32), np.array(idx, np.int64))

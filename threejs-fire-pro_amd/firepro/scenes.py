@@ -146,9 +146,10 @@ def small(seed: int = 1, steps: int = 12, brick: int = 8, divisor: int = 2,
     return Scene(f"small-b{brick}-d{divisor}{suffix}", sim, steps, hook=hook)
 
 
-def meshes(seed: int = 1, steps: int = 12, smoke: int = 1) -> Scene:
+def meshes(seed: int = 1, steps: int = 12, smoke: int = 1, colliders: bool = False) -> Scene:
     """Mesh emitters of the three shapes presets use: a tilted box blowing flame and fuel
-    sideways, a torus of heat and smoke, and a disk of flame on the ground, with turbulence."""
+    sideways, a torus of heat and smoke, and a disk of flame on the ground, with turbulence;
+    with `colliders`, a tilted mesh box over the disk and a mesh octahedron in the plume."""
     from . import mesh
     sim = FireSimulation({"voxelSize": 0.05, "brickSize": 8, "velocityDivisor": 2, "smokeDivisor": smoke,
                           "seed": seed, "fuel": {"enabled": True, "ignitionHeat": 0.5, "burnRate": 4}})
@@ -161,7 +162,10 @@ def meshes(seed: int = 1, steps: int = 12, smoke: int = 1) -> Scene:
     sim.add_emitter({"shape": {"type": "mesh", "geometry": mesh.document_geometry("disk", 0.3, (1, 1, 1))},
                      "position": (0.0, 0.15, -0.4), "emission": {"flame": 1, "heatRate": 1.5}})
     sim.add_force({"type": "turbulence", "strength": 1.5, "scale": 2})
-    suffix = f"-s{smoke}" if smoke > 1 else ""
+    if colliders:
+        sim.add_collider("mesh", (0.05, 0.55, -0.35), geometry=mesh.box(0.35, 0.08, 0.3), rotation=(0.3, 0.2, 0.1))
+        sim.add_collider("mesh", (-0.35, 0.9, 0.1), geometry=mesh.octahedron(0.18), rotation=(0.0, 0.4, 0.0))
+    suffix = ("-s%d" % smoke if smoke > 1 else "") + ("-colliders" if colliders else "")
     return Scene(f"meshes{suffix}", sim, steps)
 
 
