@@ -19,8 +19,7 @@
 // buffers: values are stored at the template precision, with no half-precision rounding
 // (truncateHalf, fluid-common.wgsl:107-114, and every f16 store); the activity readback is
 // applied before the next step (the earliest a mapAsync can resolve,
-// FluidSimulation.ts:1701-1746); mesh colliders, forces targeting mesh emitters and the
-// ambient-free debug exports are not restated.
+// FluidSimulation.ts:1701-1746); the ambient-free debug exports are not restated.
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -87,6 +86,8 @@ template <class R> struct Force {
   R axis[3], strength, center[3], kind, scale, lift, inward;
   bool targeted;
   std::vector<Region<R>> regions;
+  bool meshTargets = false;
+  std::map<std::array<int, 3>, bool> covered;  // velocity cells its mesh targets cover
 };
 
 // World-to-cell conversions multiply by the reciprocal of the spacing, rounded once: WGSL
@@ -1067,6 +1068,35 @@ template <class R> class Sim {
       F_.targeted = fc.targeted != 0;
       for (int k = 0; k < fc.source_count; k++) {
         const fp_source &s = fc.sources[k];
+        if (s.shape == FP_SHAPE_MESH) {
+          // The velocity cells the target's samples cover (ForceFields.ts:114-179): coverage
+          // above zero on the velocity lattice, in bricks that have ids this step.
+          if (!s.samples || s.sample_count <= 0 || !(s.surface_area > 0))
+            throw Error("Mesh sources require surface samples and a positive surface area.");
+          F_.meshTargets = true;
+          const double h = velocitySpacing(), meanFace = (h * h + h * h + h * h) / 3;
+          std::map<std::array<int, 3>, double> cov;
+          for (int q = 0; q < s.sample_count; q++) {
+            const double *p = s.samples + 4 * (size_t)q;
+            double fr[3];
+            int base[3];
+            for (int a = 0; a < 3; a++) {
+              double g = p[a] / h - 0.5;
+              base[a] = (int)std::floor(g);
+              fr[a] = g - base[a];
+            }
+            for (int z = 0; z < 2; z++)
+              for (int y = 0; y < 2; y++)
+                for (int x = 0; x < 2; x++) {
+                  double t = (x ? fr[0] : 1 - fr[0]) * (y ? fr[1] : 1 - fr[1]) * (z ? fr[2] : 1 - fr[2]);
+                  cov[{base[0] + x, base[1] + y, base[2] + z}] += p[3] * s.surface_area * t / meanFace;
+                }
+          }
+          for (const auto &c : cov)
+            if (std::min(1.0, c.second) > 0 && brickIdOf(c.first[0] >> sV, c.first[1] >> sV, c.first[2] >> sV) >= 0)
+              F_.covered[c.first] = true;
+          continue;
+        }
         Region<R> r;
         for (int a = 0; a < 3; a++) {
           r.center[a] = (R)(float)s.position[a];
@@ -1925,11 +1955,13 @@ template <class R> class Sim {
                                          (a == 0 ? c0 : a == 1 ? c1 : c2)[2] * s[2];
   }
   // coverage + externalForces (force-fields.wgsl:312-365).
-  void externalForces(const R p[3], R out[3]) const {
+  void externalForces(const int cell[3], const R p[3], R out[3]) const {
     out[0] = out[1] = out[2] = 0;
     for (const Force<R> &f : forces) {
       R weight = 1;
-      if (f.targeted) {
+      if (f.targeted && f.meshTargets && f.covered.count({cell[0], cell[1], cell[2]})) {
+        weight = 1;  // meshCovers (force-fields.wgsl:27-58)
+      } else if (f.targeted) {
         R covered = 0;
         for (const Region<R> &r : f.regions) {
           R dd[3];
@@ -2028,7 +2060,7 @@ template <class R> class Sim {
       for (int a = 0; a < 3; a++) conf[a] = cr[a] * P.vorticity * P.hv;
     }
     R ext[3];
-    externalForces(w, ext);
+    externalForces(p, w, ext);
     out[0] = conf[0] + ext[0] + R(0);
     const R density = coarse() ? smokeVelocityAverage(p) : f[0];
     out[1] = conf[1] + ext[1] + (P.buoyancy * f[1] - P.smokeWeight * density);

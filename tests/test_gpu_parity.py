@@ -61,13 +61,15 @@ def test_coarse_smoke_refuses_batch():
         gpu.dispose()
 
 
-@pytest.mark.parametrize("smoke,colliders", [(1, False), (2, False), (1, True)])
-def test_mesh_emitters_bit_exact(smoke, colliders):
+@pytest.mark.parametrize("smoke,colliders,targeted", [(1, False, False), (2, False, False), (1, True, False),
+                                                      (1, False, True)])
+def test_mesh_emitters_bit_exact(smoke, colliders, targeted):
     """Mesh emitters (box, torus, disk): the rasterized records and brick tables drive
     allocation, emission and the source velocity identically on both sides; with mesh
-    colliders (a tilted box, an octahedron) their occupancy draws the same solids."""
-    print(f"case meshes smoke={smoke} colliders={colliders}", flush=True)
-    assert run_lockstep(_scenes().meshes(steps=10, smoke=smoke, colliders=colliders)) == []
+    colliders (a tilted box, an octahedron) their occupancy draws the same solids; with a
+    force targeting mesh emitters, their covered velocity cells feel it fully."""
+    print(f"case meshes smoke={smoke} colliders={colliders} targeted={targeted}", flush=True)
+    assert run_lockstep(_scenes().meshes(steps=10, smoke=smoke, colliders=colliders, targeted=targeted)) == []
 
 
 def test_first_order_scalars_bit_exact():

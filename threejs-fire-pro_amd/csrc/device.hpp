@@ -40,6 +40,7 @@ struct Region {
 struct ForceRec {
   float axis[3], strength, center[3], kind, scale, lift, inward;
   int targeted, firstRegion, regionCount;
+  int meshTargets;  // its targets include mesh emitters: their covered velocity cells count 1
 };
 
 // Per-step parameters (fluid-common.wgsl Params + FlameParams), by value.

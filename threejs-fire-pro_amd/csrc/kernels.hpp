@@ -46,6 +46,10 @@ struct StepArgs {
   // (motion xyz, response), record 0 all zero. Null without mesh sources.
   const int *meshTables = nullptr;
   const float4 *meshRecords = nullptr;
+  // Velocity cells that forces' mesh targets cover (force-fields.wgsl:12-47): per brick id
+  // the first word of its entries or 0xffffffff, then a count and per entry a force index
+  // and one bit per velocity cell. Null without such targets.
+  const uint32_t *forceCoverage = nullptr;
   const float *pressure = nullptr;    // the solved pressure the projection reads
   const float *pressureLo = nullptr;  // its low part when it is a pair hi + lo (defect correction), else null
 };

@@ -591,13 +591,32 @@ __device__ __forceinline__ void curlNoise(const float4 *noise, const float p[3],
 }
 
 // coverage + externalForces (force-fields.wgsl:312-365).
-__device__ __forceinline__ void externalForces(const StepArgs &s, const float p[3], float out[3]) {
+// meshCovers (force-fields.wgsl:27-47): whether force `index`'s mesh targets cover velocity
+// cell `cell`.
+__device__ __forceinline__ bool meshCovers(const StepArgs &s, int index, const int cell[3]) {
+  const int sh = s.L.sV, side = s.L.b, m = side - 1;
+  const int id = brickIdOf(s.dir, cell[0] >> sh, cell[1] >> sh, cell[2] >> sh);
+  if (id < 0) return false;
+  const uint32_t first = s.forceCoverage[id];
+  if (first == 0xffffffffu) return false;
+  const uint32_t bit = (uint32_t)(((cell[2] & m) * side + (cell[1] & m)) * side + (cell[0] & m));
+  const uint32_t words = (uint32_t)(side * side * side + 31) / 32u;
+  for (uint32_t e = 0; e < s.forceCoverage[first]; e++) {
+    const uint32_t row = first + 1u + e * (1u + words);
+    if (s.forceCoverage[row] == (uint32_t)index) return ((s.forceCoverage[row + 1u + (bit >> 5)] >> (bit & 31u)) & 1u) != 0u;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void externalForces(const StepArgs &s, const int cell[3], const float p[3], float out[3]) {
   const Params &P = s.P;
   out[0] = out[1] = out[2] = 0.f;
   for (int fi = 0; fi < P.forceCount; fi++) {
     const ForceRec &f = s.forces[fi];
     float weight = 1.f;
-    if (f.targeted) {
+    if (f.targeted && f.meshTargets && meshCovers(s, fi, cell)) {
+      weight = 1.f;
+    } else if (f.targeted) {
       float covered = 0.f;
       for (int r = 0; r < f.regionCount; r++) {
         const Region &g = s.regions[f.firstRegion + r];
@@ -709,7 +728,7 @@ __device__ __forceinline__ void cellForce(const Ctx &c, const StepArgs &s, const
     for (int a = 0; a < 3; a++) conf[a] = cr[a] * P.vorticity * P.hv;
   }
   float ext[3];
-  externalForces(s, w, ext);
+  externalForces(s, p, w, ext);
   out[0] = conf[0] + ext[0] + 0.f;
   const float density = P.coarse ? smokeVelocityAverage(c, s, p) : f.x;
   out[1] = conf[1] + ext[1] + (P.buoyancy * f.y - P.smokeWeight * density);

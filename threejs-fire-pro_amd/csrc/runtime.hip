@@ -474,6 +474,9 @@ struct fp_sim {
   DBuf<Emitter> emitters;
   DBuf<Collider> colliders;
   DBuf<uint32_t> colliderMasks;
+  DBuf<uint32_t> forceCoverage;  // velocity cells mesh targets cover, by brick (ForceFields.ts)
+  std::vector<uint32_t> coverageHost;
+  bool hasForceCoverage = false;
   DBuf<ForceRec> forces;
   DBuf<Region> regions;
   int emitterCount = 0, forceCount = 0;
@@ -1045,6 +1048,7 @@ struct fp_sim {
     upload(colliderMasks, masks);
     hostForces.clear();
     hostRegions.clear();
+    bool anyMeshTargets = false;
     for (int i = 0; i < f.force_count; i++) {
       const fp_force &fc = f.forces[i];
       if (fc.type < 0 || fc.type > 3) throw InvalidArgument("force.type must be wind, turbulence, vortex or radial.");
@@ -1060,9 +1064,16 @@ struct fp_sim {
       r.inward = (float)fc.inward;
       r.targeted = fc.targeted != 0;
       r.firstRegion = (int)hostRegions.size();
-      r.regionCount = fc.targeted ? fc.source_count : 0;
-      for (int k = 0; k < r.regionCount; k++) {
+      r.regionCount = 0;
+      r.meshTargets = 0;
+      for (int k = 0; fc.targeted && k < fc.source_count; k++) {
         const fp_source &s = fc.sources[k];
+        if (s.shape == FP_SHAPE_MESH) {
+          r.meshTargets = 1;  // its cells come from the coverage table
+          anyMeshTargets = true;
+          continue;
+        }
+        r.regionCount++;
         Region g;
         for (int a = 0; a < 3; a++) {
           g.center[a] = (float)s.position[a];
@@ -1074,6 +1085,18 @@ struct fp_sim {
       }
       hostForces.push_back(r);
     }
+    if (anyMeshTargets) {
+      try {
+        coverageHost = mesh::forceCoverage(f, velocitySpacing(), brickCount(), sV, [&](int bx, int by, int bz) {
+          int slot = dir.slotAt(bx >> 2, by >> 2, bz >> 2);
+          return slot < 0 ? -1 : slot * 64 + ((bz & 3) * 4 + (by & 3)) * 4 + (bx & 3);
+        });
+      } catch (const std::invalid_argument &e) {
+        throw InvalidArgument(e.what());
+      }
+      upload(forceCoverage, coverageHost);
+    }
+    hasForceCoverage = anyMeshTargets;
     upload(emitters, hostEmitters);
     upload(colliders, hostColliders);
     upload(forces, hostForces);
@@ -1192,6 +1215,7 @@ struct fp_sim {
     s.forces = forces.p;
     s.regions = regions.p;
     s.meshTables = meshHost.count() ? meshTables.p : nullptr;
+    s.forceCoverage = hasForceCoverage ? forceCoverage.p : nullptr;
     s.meshRecords = meshHost.count() ? reinterpret_cast<const float4 *>(meshRecs.p) : nullptr;
     return s;
   }
@@ -1893,6 +1917,7 @@ struct fp_sim {
     emitters.release();
     colliders.release();
     colliderMasks.release();
+    forceCoverage.release();
     forces.release();
     regions.release();
     for (int L = 0; L < MAX_LEVELS; L++) {

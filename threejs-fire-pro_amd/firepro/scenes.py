@@ -146,18 +146,21 @@ def small(seed: int = 1, steps: int = 12, brick: int = 8, divisor: int = 2,
     return Scene(f"small-b{brick}-d{divisor}{suffix}", sim, steps, hook=hook)
 
 
-def meshes(seed: int = 1, steps: int = 12, smoke: int = 1, colliders: bool = False) -> Scene:
+def meshes(seed: int = 1, steps: int = 12, smoke: int = 1, colliders: bool = False,
+           targeted: bool = False) -> Scene:
     """Mesh emitters of the three shapes presets use: a tilted box blowing flame and fuel
     sideways, a torus of heat and smoke, and a disk of flame on the ground, with turbulence;
-    with `colliders`, a tilted mesh box over the disk and a mesh octahedron in the plume."""
+    with `colliders`, a tilted mesh box over the disk and a mesh octahedron in the plume;
+    with `targeted`, a vortex acting only on the box's and the torus's cells and a sphere
+    smoke source's region."""
     from . import mesh
     sim = FireSimulation({"voxelSize": 0.05, "brickSize": 8, "velocityDivisor": 2, "smokeDivisor": smoke,
                           "seed": seed, "fuel": {"enabled": True, "ignitionHeat": 0.5, "burnRate": 4}})
-    sim.add_emitter({"shape": {"type": "mesh", "geometry": mesh.box(0.4, 0.1, 0.3)},
+    box = sim.add_emitter({"shape": {"type": "mesh", "geometry": mesh.box(0.4, 0.1, 0.3)},
                      "position": (0.3, 0.35, 0.0), "rotation": (0.2, 0.5, 0.0),
                      "emission": {"flame": 1, "heatRate": 2, "fuelRate": 1},
                      "velocity": {"direction": (1, 1, 0), "speed": 2}})
-    sim.add_emitter({"shape": {"type": "mesh", "geometry": mesh.document_geometry("torus", 0.25, (1, 1, 1))},
+    ring = sim.add_emitter({"shape": {"type": "mesh", "geometry": mesh.document_geometry("torus", 0.25, (1, 1, 1))},
                      "position": (-0.4, 0.45, 0.1), "emission": {"heatRate": 3, "smokeRate": 2}})
     sim.add_emitter({"shape": {"type": "mesh", "geometry": mesh.document_geometry("disk", 0.3, (1, 1, 1))},
                      "position": (0.0, 0.15, -0.4), "emission": {"flame": 1, "heatRate": 1.5}})
@@ -165,7 +168,11 @@ def meshes(seed: int = 1, steps: int = 12, smoke: int = 1, colliders: bool = Fal
     if colliders:
         sim.add_collider("mesh", (0.05, 0.55, -0.35), geometry=mesh.box(0.35, 0.08, 0.3), rotation=(0.3, 0.2, 0.1))
         sim.add_collider("mesh", (-0.35, 0.9, 0.1), geometry=mesh.octahedron(0.18), rotation=(0.0, 0.4, 0.0))
-    suffix = ("-s%d" % smoke if smoke > 1 else "") + ("-colliders" if colliders else "")
+    if targeted:
+        puff = sim.add_smoke({"position": (0.5, 0.6, 0.4), "shape": {"radius": 0.15}})
+        sim.add_force({"type": "vortex", "center": (0.0, 0.5, 0.0), "strength": 6, "radius": 0.5,
+                       "lift": 2, "inward": 1}, [box, ring, puff])
+    suffix = ("-s%d" % smoke if smoke > 1 else "") + ("-colliders" if colliders else "") + ("-targeted" if targeted else "")
     return Scene(f"meshes{suffix}", sim, steps)
 
 
