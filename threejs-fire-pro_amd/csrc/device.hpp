@@ -118,7 +118,10 @@ __device__ __forceinline__ int entryLookup(const Dir &d, int bx, int by, int bz)
 
 __device__ __forceinline__ int pageToSlot(int page, int side) {
   uint32_t p = (uint32_t)page;
-  return (int)(((p >> 16) * (uint32_t)side + ((p >> 8) & 255u)) * (uint32_t)side + (p & 255u));
+  // A page holds 8-bit coordinates, so side <= 256 and the masks keep every product in the
+  // range of the full-rate 24-bit multiplies.
+  const uint32_t r = (uint32_t)side & 511u;
+  return (int)((((p >> 16) & 255u) * r + ((p >> 8) & 255u)) * r + (p & 255u));
 }
 __device__ __forceinline__ int slotToPage(int slot, int side) {
   uint32_t s = (uint32_t)slot, r = (uint32_t)side;
@@ -129,11 +132,16 @@ __device__ __forceinline__ int entrySlot(int e, int side) {
 }
 
 // A work item's home brick: its coordinates and the 27 neighbor entries of its record,
-// staged in LDS, so lookups within one brick need no hash probe.
+// staged in LDS, so lookups within one brick need no hash probe. The entries are in (z * 3 +
+// y) * 3 + x order (neighbor offsets plus one); the same bricks' pool slots sit at (z << 4) |
+// (y << 2) | x, so that the hot slot lookups index with shifts, not quarter-rate multiplies.
+constexpr int HOME_SLOT_WORDS = 43;
+constexpr int HOME_WORDS = 27 + HOME_SLOT_WORDS;
+__device__ __forceinline__ int homeSlotWord(int w) { return (w / 9) << 4 | ((w / 3) % 3) << 2 | (w % 3); }
 struct Home {
   int bx, by, bz;
   const int *rec;   // 27 entries (LDS)
-  const int *slot;  // the same 27 bricks' pool slots, -1 without one (LDS)
+  const int *slot;  // the same 27 bricks' pool slots, -1 without one (LDS, shift layout)
 };
 
 __device__ __forceinline__ int entryAt(const Home &h, const Dir &d, int bx, int by, int bz) {

@@ -275,7 +275,7 @@ __device__ __forceinline__ void loadHome(const Dir &dir, int id, int *rec, Home 
   if (threadIdx.x < 27) {
     int e = dir.records[(size_t)id * RECORD_WORDS + threadIdx.x];
     rec[threadIdx.x] = e;
-    rec[27 + threadIdx.x] = entrySlot(e, side);
+    rec[27 + homeSlotWord(threadIdx.x)] = entrySlot(e, side);
   }
   const int *tr = dir.tileRec + (size_t)(id >> 6) * TILE_RECORD_WORDS;
   int l = id & 63;

@@ -13,7 +13,7 @@ namespace fp {
   const int chunk = item % (CHUNKS);                        \
   (void)chunk;                                              \
   const int id = s.active[bi];                              \
-  __shared__ int rec[54];                                   \
+  __shared__ int rec[HOME_WORDS];                                   \
   Ctx c;                                                    \
   c.dir = s.dir;                                            \
   c.L = s.L;                                                \
@@ -57,8 +57,8 @@ __device__ __forceinline__ float advectComponent(const Ctx &c, const Params &P, 
 __device__ __forceinline__ void advect_velocity_body(const StepArgs &s, const int vb) {
   BRICK_PREAMBLE(1)
   const int nV = s.L.nV;
-  float *out = s.p.Vadv + (size_t)slot * nV * 3;
-  const float *vs = s.p.V + (size_t)slot * nV * 3;
+  float *out = s.p.Vadv + slotBase(slot * 3, s.L.sV);
+  const float *vs = s.p.V + slotBase(slot * 3, s.L.sV);
   for (int l = threadIdx.x; l < nV; l += blockDim.x) {
     int cc[3];
     velCellCoords(c, l, cc);
@@ -68,7 +68,7 @@ __device__ __forceinline__ void advect_velocity_body(const StepArgs &s, const in
       continue;
     }
     float orig[3] = {vs[l], vs[nV + l], vs[2 * nV + l]};
-    float4 *tr = s.p.Vtr + (size_t)slot * nV * 4 + l;
+    float4 *tr = s.p.Vtr + slotBase(slot * 4, s.L.sV) + l;
     float hi[3];
     for (int k = 0; k < 3; k++) {
       float fv[3], lo;
@@ -103,9 +103,9 @@ __device__ __forceinline__ float correctComponent(const Ctx &c, const Params &P,
 __device__ __forceinline__ void correct_velocity_body(const StepArgs &s, const int vb) {
   BRICK_PREAMBLE(1)
   const int nV = s.L.nV;
-  float *out = s.p.Vcor + (size_t)slot * nV * 3;
-  const float *vs = s.p.V + (size_t)slot * nV * 3;
-  const float *as = s.p.Vadv + (size_t)slot * nV * 3;
+  float *out = s.p.Vcor + slotBase(slot * 3, s.L.sV);
+  const float *vs = s.p.V + slotBase(slot * 3, s.L.sV);
+  const float *as = s.p.Vadv + slotBase(slot * 3, s.L.sV);
   for (int l = threadIdx.x; l < nV; l += blockDim.x) {
     int cc[3];
     velCellCoords(c, l, cc);
@@ -115,7 +115,7 @@ __device__ __forceinline__ void correct_velocity_body(const StepArgs &s, const i
       continue;
     }
     float orig[3] = {vs[l], vs[nV + l], vs[2 * nV + l]};
-    const float4 *tr = s.p.Vtr + (size_t)slot * nV * 4 + l;
+    const float4 *tr = s.p.Vtr + slotBase(slot * 4, s.L.sV) + l;
     const float4 hi = tr[3 * nV];
     const float his[3] = {hi.x, hi.y, hi.z};
     for (int k = 0; k < 3; k++) {
@@ -200,7 +200,7 @@ __device__ __forceinline__ void advect_scalars_body(const StepArgs &s, const int
     float cv[3], dep[3];
     velocityAt(c, s.P, s.p.V, pos[0], pos[1], pos[2], cv);
     traceBack(c, s.P, s.p.V, pos, cv, dep);
-    float4 *out = s.p.Fp + (size_t)slot * nF + l;
+    float4 *out = s.p.Fp + slotBase(slot, s.L.sF) + l;
     if (!s.P.maccormack) {
       float4 v = sampleField(c, s.p.F, dep[0] * s.P.invHf, dep[1] * s.P.invHf, dep[2] * s.P.invHf);
       *out = make_float4(s.P.coarse ? 0.f : v.x, v.y, clampv(v.z, 0.f, 1.f), v.w);
@@ -214,7 +214,7 @@ __device__ __forceinline__ void advect_scalars_body(const StepArgs &s, const int
     // Coarse smoke: the fine X channel only stages the step's smoke production
     // (fluid-scalar-transport.wgsl:127, :137, :191-199).
     *out = make_float4(s.P.coarse ? 0.f : v.x, v.y, s.P.lifetimeZero ? 0.f : v.z, v.w);
-    float4 *don = s.p.Don + (size_t)slot * nF * 3 + l;
+    float4 *don = s.p.Don + slotBase(slot * 3, s.L.sF) + l;
     don[0] = make_float4(cv[0], cv[1], cv[2], dn.lmn);
     don[nF] = make_float4(dn.mn[0], dn.mn[1], dn.mn[2], dn.lmx);
     don[2 * nF] = make_float4(dn.mx[0], dn.mx[1], dn.mx[2], 0.f);
@@ -227,13 +227,13 @@ FP_KERNEL_PAIR1(advect_scalars, 256, StepArgs, int, chunks)
 // advection left in the backtrace planes (`dn`).
 __device__ __forceinline__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p, const int id[3], int slot, int l,
                                    const float pos[3], const float cv[3], const Donors &dn) {
-  const float4 adv = p.Fp[(size_t)slot * c.L.nF + l];
+  const float4 adv = p.Fp[slotBase(slot, c.L.sF) + l];
   if (!P.maccormack) return adv;
   bool same = dn.mn[0] == dn.mx[0] && dn.mn[1] == dn.mx[1] && dn.mn[2] == dn.mx[2] && dn.lmn == dn.lmx;
   if (same) return make_float4(dn.mn[0], dn.mn[1], clampv(dn.lmn, 0.f, 1.f), dn.mn[2]);
   float fw[3];
   traceForward(c, P, p.V, pos, cv, fw);
-  const float4 orig = p.F[(size_t)slot * c.L.nF + l];
+  const float4 orig = p.F[slotBase(slot, c.L.sF) + l];
   float lifetime = adv.z;
   float rev[3];
   if (dn.lmn != dn.lmx || adv.z != dn.lmn) {
@@ -443,7 +443,7 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     Donors dn;
     if (P.maccormack) {
       // The cell's velocity and donor bounds from advection's backtrace.
-      const float4 *don = s.p.Don + (size_t)slot * nF * 3 + l;
+      const float4 *don = s.p.Don + slotBase(slot * 3, s.L.sF) + l;
       const float4 d0 = don[0], d1 = don[nF], d2 = don[2 * nF];
       cv[0] = d0.x;
       cv[1] = d0.y;
@@ -505,8 +505,8 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     }
     if (!injected && value[2] == 0.f && value[0] < 0.001f && value[1] < 0.001f && value[3] < 0.001f)
       value[0] = value[1] = value[2] = value[3] = 0.f;
-    s.p.Fn[(size_t)slot * nF + l] = make_float4(value[0], value[1], value[2], value[3]);
-    s.p.E[(size_t)slot * nF + l] = expansion;
+    s.p.Fn[slotBase(slot, s.L.sF) + l] = make_float4(value[0], value[1], value[2], value[3]);
+    s.p.E[slotBase(slot, s.L.sF) + l] = expansion;
     if (expansion != 0.f) myExpands = 1u;
     if (fmaxv(fmaxv(P.coarse ? 0.f : value[0], value[2]), value[3]) > P.cutoff)
       myBusy |= contentSupport(s.L, P, cc, cv);
@@ -555,7 +555,7 @@ __device__ __forceinline__ void curl_body(const StepArgs &s, const int vb) {
     float dz0 = (curlCentered(c, V, x, y, z + 1, 0) - curlCentered(c, V, x, y, z - 1, 0)) * hx;
     float dz1 = (curlCentered(c, V, x, y, z + 1, 1) - curlCentered(c, V, x, y, z - 1, 1)) * hx;
     float c0 = dy1 - dz1, c1 = dz0 - dx1, c2 = dx0 - dy0;
-    s.p.Curl[(size_t)slot * nV + l] = make_float4(c0, c1, c2, length3(c0, c1, c2));
+    s.p.Curl[slotBase(slot, s.L.sV) + l] = make_float4(c0, c1, c2, length3(c0, c1, c2));
   }
 }
 FP_KERNEL_PAIR(curl, 256, StepArgs)
@@ -769,7 +769,7 @@ __device__ __forceinline__ void cell_forces_body(const StepArgs &s, const int vb
   float *out = s.p.Vadv;
   BRICK_PREAMBLE(1)
   const int nV = s.L.nV;
-  float *o = out + (size_t)slot * nV * 3;
+  float *o = out + slotBase(slot * 3, s.L.sV);
   for (int l = threadIdx.x; l < nV; l += blockDim.x) {
     int p[3];
     velCellCoords(c, l, p);
@@ -791,9 +791,9 @@ __device__ __forceinline__ void forces_body(const StepArgs &s, const int vb) {
   const Params &P = s.P;
   const int nV = s.L.nV;
   const float damp = P.damp;
-  const float *vc = s.p.Vcor + (size_t)slot * nV * 3;
-  const float *fh = cellF + (size_t)slot * nV * 3;
-  float *o = s.p.Vf + (size_t)slot * nV * 3;
+  const float *vc = s.p.Vcor + slotBase(slot * 3, s.L.sV);
+  const float *fh = cellF + slotBase(slot * 3, s.L.sV);
+  float *o = s.p.Vf + slotBase(slot * 3, s.L.sV);
   for (int l = threadIdx.x; l < nV; l += blockDim.x) {
     int p[3];
     velCellCoords(c, l, p);
@@ -829,7 +829,7 @@ __device__ __forceinline__ void divergence_body(const StepArgs &s, const int vb)
   BRICK_PREAMBLE(1)
   const Params &P = s.P;
   const int nV = s.L.nV;
-  const float *vf = s.p.Vf + (size_t)slot * nV * 3;
+  const float *vf = s.p.Vf + slotBase(slot * 3, s.L.sV);
   const int d = s.L.B / s.L.b;
   for (int l = threadIdx.x; l < nV; l += blockDim.x) {
     int p[3];
@@ -865,7 +865,7 @@ __device__ __forceinline__ void divergence_body(const StepArgs &s, const int vb)
           }
       avg = blocks == 1 ? sum : sum / (float)(blocks * blocks * blocks);
     }
-    rhs0[(size_t)slot * nV + l] = h * h * (div - avg);
+    rhs0[slotBase(slot, s.L.sV) + l] = h * h * (div - avg);
   }
 }
 FP_KERNEL_PAIR(divergence, 256, StepArgs)
@@ -892,8 +892,8 @@ __device__ __forceinline__ void project_body(const StepArgs &s, const int vb) {
   BRICK_PREAMBLE(1)
   const Params &P = s.P;
   const int nV = s.L.nV;
-  const float *vf = s.p.Vf + (size_t)slot * nV * 3;
-  float *o = s.p.V + (size_t)slot * nV * 3;
+  const float *vf = s.p.Vf + slotBase(slot * 3, s.L.sV);
+  float *o = s.p.V + slotBase(slot * 3, s.L.sV);
   for (int l = threadIdx.x; l < nV; l += blockDim.x) {
     int p[3];
     velCellCoords(c, l, p);
@@ -902,8 +902,8 @@ __device__ __forceinline__ void project_body(const StepArgs &s, const int vb) {
       o[l] = o[nV + l] = o[2 * nV + l] = 0.f;
       continue;
     }
-    float cp = pressure[(size_t)slot * nV + l];
-    const float cpLo = s.pressureLo ? s.pressureLo[(size_t)slot * nV + l] : 0.f;
+    float cp = pressure[slotBase(slot, s.L.sV) + l];
+    const float cpLo = s.pressureLo ? s.pressureLo[slotBase(slot, s.L.sV) + l] : 0.f;
     bool here = pressureUnknown(c, P, p);
     for (int a = 0; a < 3; a++) {
       int nb[3] = {p[0], p[1], p[2]};
@@ -951,7 +951,7 @@ __device__ __forceinline__ void advect_smoke_body(const StepArgs &s, const int v
   BRICK_PREAMBLE(1)
   const Params &P = s.P;
   const int nS = s.L.nS;
-  float4 *don = s.p.Don + (size_t)slot * s.L.nF * 3;
+  float4 *don = s.p.Don + slotBase(slot * 3, s.L.sF);
   for (int l = threadIdx.x; l < nS; l += blockDim.x) {
     int cc[3];
     smokeCellCoords(c, l, cc);
@@ -964,7 +964,7 @@ __device__ __forceinline__ void advect_smoke_body(const StepArgs &s, const int v
       traceBack(c, P, s.p.V, pos, cv, dep);
       density = sampleSmoke(c, s.p.S, dep[0] * P.invHs, dep[1] * P.invHs, dep[2] * P.invHs, lo, hi);
     }
-    s.p.Sp[(size_t)slot * nS + l] = density;
+    s.p.Sp[slotBase(slot, s.L.sS) + l] = density;
     don[l] = make_float4(cv[0], cv[1], cv[2], lo);
     don[nS + l] = make_float4(hi, 0.f, 0.f, 0.f);
   }
@@ -981,14 +981,14 @@ __device__ __forceinline__ void evolve_smoke_body(const StepArgs &s, const int v
   __syncthreads();
   const Params &P = s.P;
   const int nS = s.L.nS;
-  const float4 *don = s.p.Don + (size_t)slot * s.L.nF * 3;
+  const float4 *don = s.p.Don + slotBase(slot * 3, s.L.sF);
   const float ratio = P.hv / P.hs;
   uint32_t myBusy = 0u;
   for (int l = threadIdx.x; l < nS; l += blockDim.x) {
     int cc[3];
     smokeCellCoords(c, l, cc);
     const float pos[3] = {((float)cc[0] + 0.5f) * P.hs, ((float)cc[1] + 0.5f) * P.hs, ((float)cc[2] + 0.5f) * P.hs};
-    const size_t at = (size_t)slot * nS + l;
+    const size_t at = slotBase(slot, s.L.sS) + l;
     float density = 0.f;
     if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
       const float4 d0 = don[l];

@@ -114,7 +114,7 @@ __device__ __forceinline__ void levelCell(int home0, int home1, int home2, int s
 
 // Home set-up for a block over one active brick.
 #define MG_BRICK_HOME(ID)                          \
-  __shared__ int rec[54];                          \
+  __shared__ int rec[HOME_WORDS];                          \
   MGCtx c;                                         \
   c.dir = m.dir;                                   \
   c.L = m.L;                                       \
@@ -172,14 +172,14 @@ __device__ __forceinline__ Group groupOf(int g, int cells) {
 // The group's bricks' 27 neighbor entries and their pool slots, in LDS (all lanes take
 // part); this lane's brick's home.
 __device__ __forceinline__ void loadGroupHomes(const Dir &dir, const int *active, int count, const Group &gr,
-                                               int (*recs)[54], int side) {
+                                               int (*recs)[HOME_WORDS], int side) {
   for (int e = threadIdx.x; e < gr.k * 27; e += blockDim.x) {
     const int jj = e / 27, w = e - jj * 27;
     const int b = gr.g * gr.k + jj;
     if (b < count) {
       const int r = dir.records[(size_t)active[b] * RECORD_WORDS + w];
       recs[jj][w] = r;
-      recs[jj][27 + w] = entrySlot(r, side);
+      recs[jj][27 + homeSlotWord(w)] = entrySlot(r, side);
     }
   }
   __syncthreads();
@@ -206,7 +206,7 @@ __device__ __forceinline__ Home homeOf(const Dir &dir, int id, const int *rec) {
   const int bi = gr.g * gr.k + gr.j;                                           \
   const bool mine = gr.j >= 0 && bi < count_;
 #define MG_GROUP_HOME                                                           \
-  __shared__ int recs_[MG_GROUP][54];                                          \
+  __shared__ int recs_[MG_GROUP][HOME_WORDS];                                          \
   loadGroupHomes(m.dir, m.active, count_, gr, recs_, m.L.side);                \
   if (!mine) return;                                                           \
   const int id = m.active[bi];                                                 \
@@ -725,7 +725,7 @@ __device__ __forceinline__ void dc_residual_body(const MGArgsT<typename St::T> &
   const int sd = lv.side, n = sd * sd * sd;
   MG_BRICK_GROUP(n)
   __shared__ double red[3][256];
-  __shared__ int recs_[MG_GROUP][54];
+  __shared__ int recs_[MG_GROUP][HOME_WORDS];
   loadGroupHomes(m.dir, m.active, count_, gr, recs_, m.L.side);
   double rr = 0.0, bb = 0.0;
   float big = 0.f;

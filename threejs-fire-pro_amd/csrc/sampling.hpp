@@ -37,39 +37,42 @@ struct Ctx {
 __device__ __forceinline__ int slotAtBrick(const Ctx &c, int bx, int by, int bz) {
   int dx = bx - c.home.bx, dy = by - c.home.by, dz = bz - c.home.bz;
   if ((unsigned)(dx + 1) <= 2u && (unsigned)(dy + 1) <= 2u && (unsigned)(dz + 1) <= 2u)
-    return c.home.slot[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+    return c.home.slot[(dz + 1) << 4 | (dy + 1) << 2 | (dx + 1)];
   return entrySlot(entryLookup(c.dir, bx, by, bz), c.side);
 }
+
+// Brick edges are powers of two: a cell's index inside its brick and a slot's first cell
+// are shifts (sh the edge's log2), not multiplies, which are quarter rate on the VALU.
+__device__ __forceinline__ int cellIndex(int sh, int x, int y, int z) {
+  const int m = (1 << sh) - 1;
+  return (((z & m) << sh | (y & m)) << sh) | (x & m);
+}
+__device__ __forceinline__ size_t slotBase(int slot, int sh) { return (size_t)(uint32_t)slot << (3 * sh); }
 
 __device__ __forceinline__ float4 fieldCell(const Ctx &c, const float4 *buf, int x, int y, int z) {
   int s = slotAtBrick(c, x >> c.L.sF, y >> c.L.sF, z >> c.L.sF);
   if (s < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
-  int m = c.L.B - 1;
-  return buf[(size_t)s * c.L.nF + (((z & m) * c.L.B + (y & m)) * c.L.B + (x & m))];
+  return buf[slotBase(s, c.L.sF) + cellIndex(c.L.sF, x, y, z)];
 }
 __device__ __forceinline__ float fieldScalar(const Ctx &c, const float *buf, int x, int y, int z) {
   int s = slotAtBrick(c, x >> c.L.sF, y >> c.L.sF, z >> c.L.sF);
   if (s < 0) return 0.f;
-  int m = c.L.B - 1;
-  return buf[(size_t)s * c.L.nF + (((z & m) * c.L.B + (y & m)) * c.L.B + (x & m))];
+  return buf[slotBase(s, c.L.sF) + cellIndex(c.L.sF, x, y, z)];
 }
 __device__ __forceinline__ float velCell(const Ctx &c, const float *buf, int comp, int x, int y, int z) {
   int s = slotAtBrick(c, x >> c.L.sV, y >> c.L.sV, z >> c.L.sV);
   if (s < 0) return 0.f;
-  int m = c.L.b - 1;
-  return buf[(size_t)s * c.L.nV * 3 + (size_t)comp * c.L.nV + (((z & m) * c.L.b + (y & m)) * c.L.b + (x & m))];
+  return buf[slotBase(s * 3 + comp, c.L.sV) + cellIndex(c.L.sV, x, y, z)];
 }
 __device__ __forceinline__ float velScalar(const Ctx &c, const float *buf, int x, int y, int z) {
   int s = slotAtBrick(c, x >> c.L.sV, y >> c.L.sV, z >> c.L.sV);
   if (s < 0) return 0.f;
-  int m = c.L.b - 1;
-  return buf[(size_t)s * c.L.nV + (((z & m) * c.L.b + (y & m)) * c.L.b + (x & m))];
+  return buf[slotBase(s, c.L.sV) + cellIndex(c.L.sV, x, y, z)];
 }
 __device__ __forceinline__ float4 curlCell(const Ctx &c, const float4 *buf, int x, int y, int z) {
   int s = slotAtBrick(c, x >> c.L.sV, y >> c.L.sV, z >> c.L.sV);
   if (s < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
-  int m = c.L.b - 1;
-  return buf[(size_t)s * c.L.nV + (((z & m) * c.L.b + (y & m)) * c.L.b + (x & m))];
+  return buf[slotBase(s, c.L.sV) + cellIndex(c.L.sV, x, y, z)];
 }
 
 // The 2x2x2 cells from (x, y, z) on a grid of bricks 2^sh cells a side, corner k = (z * 2 +
@@ -80,8 +83,7 @@ __device__ __forceinline__ void cornerCells(const Ctx &c, int sh, int x, int y, 
   const int n = 1 << sh, m = n - 1;
   const int p[3] = {x, y, z};
   const int home[3] = {c.home.bx, c.home.by, c.home.bz};
-  const int mul[3] = {1, 3, 9};
-  const int lmul[3] = {1, n, n * n};
+  const int lsh[3] = {0, sh, 2 * sh};
   int bk[3][2], rel[3][2], loc[3][2];
   bool in = true;
 #pragma unroll
@@ -92,8 +94,8 @@ __device__ __forceinline__ void cornerCells(const Ctx &c, int sh, int x, int y, 
       bk[a][e] = q >> sh;
       const int d = bk[a][e] - home[a] + 1;
       in = in && (unsigned)d <= 2u;
-      rel[a][e] = d * mul[a];
-      loc[a][e] = (q & m) * lmul[a];
+      rel[a][e] = d << (2 * a);
+      loc[a][e] = (q & m) << lsh[a];
     }
 #pragma unroll
   for (int k = 0; k < 8; k++) local[k] = loc[2][k >> 2] + loc[1][(k >> 1) & 1] + loc[0][k & 1];
@@ -110,15 +112,14 @@ __device__ __forceinline__ void fieldCorners(const Ctx &c, const float4 *buf, in
   cornerCells(c, c.L.sF, x, y, z, slot, local);
 #pragma unroll
   for (int k = 0; k < 8; k++)
-    v[k] = slot[k] < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : buf[(size_t)slot[k] * c.L.nF + local[k]];
+    v[k] = slot[k] < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : buf[slotBase(slot[k], c.L.sF) + local[k]];
 }
 // The eight velocity cells of plane `comp` from (x, y, z).
 __device__ __forceinline__ void velCorners(const Ctx &c, const float *buf, int comp, int x, int y, int z, float v[8]) {
   int slot[8], local[8];
   cornerCells(c, c.L.sV, x, y, z, slot, local);
-  const float *plane = buf + (size_t)comp * c.L.nV;
 #pragma unroll
-  for (int k = 0; k < 8; k++) v[k] = slot[k] < 0 ? 0.f : plane[(size_t)slot[k] * c.L.nV * 3 + local[k]];
+  for (int k = 0; k < 8; k++) v[k] = slot[k] < 0 ? 0.f : buf[slotBase(slot[k] * 3 + comp, c.L.sV) + local[k]];
 }
 
 // The eight smoke cells from (x, y, z) of a coarse smoke pool.
@@ -126,7 +127,7 @@ __device__ __forceinline__ void smokeCorners(const Ctx &c, const float *buf, int
   int slot[8], local[8];
   cornerCells(c, c.L.sS, x, y, z, slot, local);
 #pragma unroll
-  for (int k = 0; k < 8; k++) v[k] = slot[k] < 0 ? 0.f : buf[(size_t)slot[k] * c.L.nS + local[k]];
+  for (int k = 0; k < 8; k++) v[k] = slot[k] < 0 ? 0.f : buf[slotBase(slot[k], c.L.sS) + local[k]];
 }
 // A trilinear sample of a smoke pool at (px, py, pz) in smoke cells, with the range of the
 // eight cells it weighs: the minimum from 1e30 and the maximum from 0, as smokeDonorBounds
@@ -230,10 +231,9 @@ __device__ __forceinline__ float sampleVelBounds(const Ctx &c, const float *buf,
   float fx = qx - flx, fy = qy - fly, fz = qz - flz;
   int slot[8], local[8];
   cornerCells(c, c.L.sV, bx, by, bz, slot, local);
-  const float *plane = buf + (size_t)comp * c.L.nV;
   float cv[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) cv[k] = slot[k] < 0 ? 0.f : plane[(size_t)slot[k] * c.L.nV * 3 + local[k]];
+  for (int k = 0; k < 8; k++) cv[k] = slot[k] < 0 ? 0.f : buf[slotBase(slot[k] * 3 + comp, c.L.sV) + local[k]];
   float out = 0.f;
 #pragma unroll
   for (int z = 0; z < 2; z++)
