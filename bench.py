@@ -245,9 +245,10 @@ def mix_comparison(count: int) -> dict:
 
 def render_phase(sim, frames, step) -> dict:
     """After the timed steps: a step, then the 1920x1080 ray-march with self-shadowing,
-    several times; the render stages' event-timed averages (preparation every step,
-    the whole light every second step, the march every frame), and the scene lights'
-    totals with an anchor at each fire."""
+    several times, at half resolution (the reference's default: the march into depth
+    slices at 960x540, then the full-resolution resume from them); the render stages'
+    event-timed averages (preparation every step, the whole light every second step, the
+    slices and march every frame), and the scene lights' totals with an anchor at each fire."""
     from firepro import abi
     w, h = 1920, 1080
     cam = abi.camera((0.0, 3.0, 9.0), (0.0, 1.5, 0.0), 50, w, h)
@@ -272,7 +273,7 @@ def render_phase(sim, frames, step) -> dict:
         buf.close()
     out = {"width": w, "height": h, "renders": len(walls),
            "wall_ms_mean": 1e3 * float(np.mean(walls)) if walls else None}
-    for k in ("render_prepare", "render_light", "render_march", "scene_lights"):
+    for k in ("render_prepare", "render_light", "render_slices", "render_march", "scene_lights"):
         if k in stages:
             out[k + "_ms"] = round(stages[k], 4)
     solver = {k: round(v, 4) for k, v in stages.items() if not k.startswith("render_") and k != "scene_lights"}
