@@ -106,10 +106,12 @@ __device__ __forceinline__ uint8_t cellTopology(const MGCtx &m, const Params &P,
   return flags;
 }
 
+// Level sides are powers of two: the cell's offsets are masks and shifts.
 __device__ __forceinline__ void levelCell(int home0, int home1, int home2, int sd, int l, int p[3]) {
-  p[0] = home0 * sd + l % sd;
-  p[1] = home1 * sd + (l / sd) % sd;
-  p[2] = home2 * sd + l / (sd * sd);
+  const int k = __builtin_ctz((unsigned)sd), msk = sd - 1;
+  p[0] = home0 * sd + (l & msk);
+  p[1] = home1 * sd + ((l >> k) & msk);
+  p[2] = home2 * sd + (l >> (2 * k));
 }
 
 // Home set-up for a block over one active brick.

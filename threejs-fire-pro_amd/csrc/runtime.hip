@@ -1322,11 +1322,13 @@ struct fp_sim {
     end(m0, "allocation");
 
     StepArgs s = stepArgs();
+    // Velocity-grid kernels loop over a brick's nV cells: lanes for every cell up to 256.
+    const unsigned vLanes = (unsigned)std::min(256, std::max(64, nV));
     int m1 = begin();
-    k_advect_velocity<<<slotsGrid, 256, 0, stream>>>(s);
+    k_advect_velocity<<<slotsGrid, vLanes, 0, stream>>>(s);
     end(m1, "advect_velocity");
     int m2 = begin();
-    k_correct_velocity<<<slotsGrid, 256, 0, stream>>>(s);
+    k_correct_velocity<<<slotsGrid, vLanes, 0, stream>>>(s);
     end(m2, "correct_velocity");
     const int fchunks = std::max(1, nF / 256);
     int m3 = begin();
@@ -1349,11 +1351,11 @@ struct fp_sim {
       s.p.Sn = Sn.p;
     }
     int m5 = begin();
-    if (P.vorticity != 0.f) k_curl<<<slotsGrid, 256, 0, stream>>>(s);
+    if (P.vorticity != 0.f) k_curl<<<slotsGrid, vLanes, 0, stream>>>(s);
     // The velocity predictor is dead after correction: it holds the cell forces.
-    k_cell_forces<<<slotsGrid, 256, 0, stream>>>(s);
-    k_forces<<<slotsGrid, 256, 0, stream>>>(s);
-    k_divergence<<<slotsGrid, 256, 0, stream>>>(s);
+    k_cell_forces<<<slotsGrid, vLanes, 0, stream>>>(s);
+    k_forces<<<slotsGrid, vLanes, 0, stream>>>(s);
+    k_divergence<<<slotsGrid, vLanes, 0, stream>>>(s);
     end(m5, "forces_divergence");
     int m6 = begin();
     solvePressure(slotsGrid);
@@ -1361,7 +1363,7 @@ struct fp_sim {
     int m7 = begin();
     s.pressure = latestPressure;
     s.pressureLo = latestPressureLo;
-    k_project<<<slotsGrid, 256, 0, stream>>>(s);
+    k_project<<<slotsGrid, vLanes, 0, stream>>>(s);
     end(m7, "project");
     k_readback<<<(unsigned)((capacity + 255) / 256 + 1), 256, 0, stream>>>(counts.p, poolState.p, activity.p + n,
                                                                            capacity, readback.p);
@@ -1380,13 +1382,15 @@ struct fp_sim {
   void solvePressure(unsigned slotsGrid) {
     MGArgs m = mgArgs();
     m.topologyAll = geometryDirty ? 1 : 0;
+    // The brick levels hold at most nV cells per brick.
+    const unsigned vLanes = (unsigned)std::min(256, std::max(64, nV));
     if (geometryDirty) {
-      k_topology_bricks<<<slotsGrid, 256, 0, stream>>>(m, 0);
+      k_topology_bricks<<<slotsGrid, vLanes, 0, stream>>>(m, 0);
       for (int L = brickLevels; L <= bottom; L++) k_topology_tiles<<<capacity, 64, 0, stream>>>(m, L);
       geometryDirty = false;
     }
-    k_topology_bricks<<<slotsGrid * 2, 256, 0, stream>>>(m, 1);
-    k_eflags_bricks<<<slotsGrid, 256, 0, stream>>>(m);
+    k_topology_bricks<<<slotsGrid * 2, vLanes, 0, stream>>>(m, 1);
+    k_eflags_bricks<<<slotsGrid, vLanes, 0, stream>>>(m);
     for (int L = brickLevels; L <= bottom; L++) k_eflags_tiles<<<capacity, 64, 0, stream>>>(m, L);
     if (pressureMode == 0) {
       int flips = 0;
