@@ -158,6 +158,7 @@ FP_DECLARE_PAIR(free_bricks, AllocArgs)
 FP_DECLARE_PAIR(allocate_bricks, AllocArgs)
 FP_DECLARE_PAIR(link_bricks, AllocArgs)
 FP_DECLARE_PAIR(build_solids, AllocArgs)
+__global__ void k_fresh_solids(AllocArgs a);
 FP_DECLARE_PAIR1(zero_fresh, AllocArgs, int)
 
 FP_DECLARE_PAIR(advect_velocity, StepArgs)
@@ -178,7 +179,8 @@ FP_DECLARE_PAIR(project, StepArgs)
 FP_DECLARE_PAIR1(topology_bricks, MGArgs, int)
 FP_DECLARE_PAIR1(topology_tiles, MGArgs, int)
 FP_DECLARE_PAIR(eflags_bricks, MGArgs)
-FP_DECLARE_PAIR1(eflags_tiles, MGArgs, int)
+__global__ void k_fresh_topology(MGArgs m);
+FP_DECLARE_PAIR(eflags_tiles, MGArgs)
 #define FP_DECLARE_MG(NAME)                                                                       \
   template <class St> __global__ void k_##NAME(MGArgsT<typename St::T> a, int code);              \
   template <class St>                                                                             \

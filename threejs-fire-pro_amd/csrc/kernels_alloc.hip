@@ -250,42 +250,42 @@ __device__ __forceinline__ bool insideCollider(const Collider &c, const uint32_t
   return dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] < 1.0f;
 }
 
+// One brick (its id and page) with a wavefront: a lane per cell and pass, the pass's 64
+// bits from a ballot (two words, cells in index order).
+__device__ __forceinline__ void solidBrick(const AllocArgs &a, int index, int page) {
+  const int *tr = a.dir.tileRec + (size_t)(index >> 6) * TILE_RECORD_WORDS;
+  const int l = index & 63;
+  const int b[3] = {tr[0] * 4 + (l & 3), tr[1] * 4 + ((l >> 2) & 3), tr[2] * 4 + (l >> 4)};
+  const int B = a.L.B, sh = a.L.sF, nF = a.L.nF;
+  const size_t base = (size_t)pageToSlot(page, a.L.side) * (size_t)(nF >> 5);
+  for (int cell0 = 0; cell0 < nF; cell0 += 64) {
+    const int cell = cell0 + (int)threadIdx.x;
+    const int x = b[0] * B + (cell & (B - 1)), y = b[1] * B + ((cell >> sh) & (B - 1)), z = b[2] * B + (cell >> (2 * sh));
+    bool inside = false;
+    for (int ci = 0; ci < a.colliderCount && !inside; ci++)
+      inside = insideCollider(a.colliders[ci], a.colliderMasks, a.P.hf, x, y, z);
+    const uint64_t bits = __ballot(inside);
+    if (threadIdx.x < 2) a.solids[base + (size_t)(cell0 >> 5) + threadIdx.x] = (uint32_t)(bits >> (32 * threadIdx.x));
+  }
+}
 __device__ __forceinline__ void build_solids_body(const AllocArgs &a, const int vb) {
   if (!a.buildSolids) return;
-  const int everySlot = a.solidsEvery;
   const int i = vb;
-  int index, page;
-  if (everySlot) {
+  if (a.solidsEvery) {
     if (i >= a.counts[0]) return;
-    index = a.active[i];
-    page = a.entries[index] & 0xffffff;
+    const int index = a.active[i];
+    solidBrick(a, index, a.entries[index] & 0xffffff);
   } else {
     if (i >= a.counts[1]) return;
-    index = a.zeroList[2 * i];
-    page = a.zeroList[2 * i + 1] & 0xffffff;
-  }
-  const int *tr = a.dir.tileRec + (size_t)(index >> 6) * TILE_RECORD_WORDS;
-  int l = index & 63;
-  int b[3] = {tr[0] * 4 + (l & 3), tr[1] * 4 + ((l >> 2) & 3), tr[2] * 4 + (l >> 4)};
-  const int B = a.L.B, sh = a.L.sF;
-  const int words = a.L.nF >> 5;
-  int slot = pageToSlot(page, a.L.side);
-  for (int word = threadIdx.x; word < words; word += 64) {
-    uint32_t bits = 0;
-    for (int k = 0; k < 32; k++) {
-      uint32_t bit = (uint32_t)word * 32u + k;
-      int x = b[0] * B + (int)(bit & (B - 1)), y = b[1] * B + (int)((bit >> sh) & (B - 1)),
-          z = b[2] * B + (int)(bit >> (2 * sh));
-      for (int ci = 0; ci < a.colliderCount; ci++)
-        if (insideCollider(a.colliders[ci], a.colliderMasks, a.P.hf, x, y, z)) {
-          bits |= 1u << k;
-          break;
-        }
-    }
-    a.solids[(size_t)slot * words + word] = bits;
+    solidBrick(a, a.zeroList[2 * i], a.zeroList[2 * i + 1] & 0xffffff);
   }
 }
 FP_KERNEL_PAIR(build_solids, 64, AllocArgs)
+// The solids of the zeroed list's bricks only, walked by a fixed grid.
+__global__ void __launch_bounds__(64) k_fresh_solids(AllocArgs a) {
+  for (int i = (int)blockIdx.x; i < a.counts[1]; i += (int)gridDim.x)
+    solidBrick(a, a.zeroList[2 * i], a.zeroList[2 * i + 1] & 0xffffff);
+}
 
 
 // A fresh slot is still, empty air (zeroVelocityBricks / zeroFieldBricks,

@@ -249,11 +249,7 @@ __device__ __forceinline__ Home homeOf(const Dir &dir, int id, const int *rec) {
 
 // buildBrickTopology (pressure.wgsl:259-278) for every active brick (when the geometry
 // changed), or for the bricks of the zeroed list that hold a slot.
-__device__ __forceinline__ void topology_bricks_body(const MGArgs &m, const int vb, int fromZeroList) {
-  if (!fromZeroList && !m.topologyAll) return;
-  const int bi = xcdItem(vb, fromZeroList ? m.counts[1] + m.counts[5] : m.counts[0]);
-  if (bi < 0) return;
-  const int id = fromZeroList ? m.zeroList[2 * bi] : m.active[bi];
+__device__ __forceinline__ void topology_brick(const MGArgs &m, const int id) {
   if (m.dir.entries[id] < 0) return;
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
@@ -266,7 +262,22 @@ __device__ __forceinline__ void topology_bricks_body(const MGArgs &m, const int 
     }
   }
 }
+__device__ __forceinline__ void topology_bricks_body(const MGArgs &m, const int vb, int fromZeroList) {
+  if (!fromZeroList && !m.topologyAll) return;
+  const int bi = xcdItem(vb, fromZeroList ? m.counts[1] + m.counts[5] : m.counts[0]);
+  if (bi < 0) return;
+  topology_brick(m, fromZeroList ? m.zeroList[2 * bi] : m.active[bi]);
+}
 FP_KERNEL_PAIR1(topology_bricks, 256, MGArgs, int, fromZeroList)
+// The same over the zeroed list only, walked by a fixed grid: a step allocates few bricks,
+// and a grid sized for the pool would be almost all empty workgroups.
+__global__ void __launch_bounds__(256) k_fresh_topology(MGArgs m) {
+  const int count = m.counts[1] + m.counts[5];
+  for (int bi = (int)blockIdx.x; bi < count; bi += (int)gridDim.x) {
+    topology_brick(m, m.zeroList[2 * bi]);
+    __syncthreads();  // the LDS record is loaded again for the next brick
+  }
+}
 
 // buildTileTopology (pressure.wgsl:281-305) of one tile level, every tile slot.
 __device__ __forceinline__ void topology_tiles_body(const MGArgs &m, const int vb, int lv) {
@@ -306,19 +317,34 @@ __device__ __forceinline__ void eflags_bricks_body(const MGArgs &m, const int vb
   }
 }
 FP_KERNEL_PAIR(eflags_bricks, 256, MGArgs)
-__device__ __forceinline__ void eflags_tiles_body(const MGArgs &m, const int vb, int lv) {
+// The step's flags of every tile level of one tile slot, from one wavefront (64 lanes, a
+// lane per brick of the tile, (z * 4 + y) * 4 + x). A cell of the first tile level is one
+// brick, unknown when the brick has a slot and takes part in the solve; a cell of a
+// coarser tile level covers its eight children's bricks, so it is unknown when all of
+// them are (the brick test of activeCell over the cell's bricks).
+__device__ __forceinline__ void eflags_tiles_body(const MGArgs &m, const int vb) {
   const int t = vb;
   if (t >= m.dir.capacity) return;
-  MG_TILE_HOME(t)
-  const int sd = m.lv[lv].side, n = sd * sd * sd;
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
-    int p[3];
-    levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    size_t a = (size_t)t * n + l;
-    m.lv[lv].eflag[a] = (uint8_t)(m.lv[lv].topo[a] | (activeCell(c, m.P, p, lv) ? UNKNOWN : 0));
+  if (m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] == 0) return;
+  const int l = (int)threadIdx.x, id = t * 64 + l;
+  const bool part = m.dir.entries[id] >= 0 && m.dir.records[(size_t)id * RECORD_WORDS + PRESSURE_FLAG] != 0;
+  const uint64_t bricks = __ballot(part);
+  for (int lv = m.L.brickLevels; lv <= m.bottom; lv++) {
+    const int k = 2 - (lv - m.L.brickLevels);  // log2 of the level's side in the tile
+    if (k < 0) break;
+    const int sd = 1 << k, n = 1 << (3 * k), span = 4 >> k;
+    if (l >= n) continue;
+    const int c[3] = {(l & (sd - 1)) * span, ((l >> k) & (sd - 1)) * span, (l >> (2 * k)) * span};
+    bool all = true;
+    for (int z = 0; z < span; z++)
+      for (int y = 0; y < span; y++)
+        for (int x = 0; x < span; x++)
+          all = all && ((bricks >> (((c[2] + z) * 4 + c[1] + y) * 4 + c[0] + x)) & 1u);
+    const size_t a = (size_t)t * n + l;
+    m.lv[lv].eflag[a] = (uint8_t)(m.lv[lv].topo[a] | (all ? UNKNOWN : 0));
   }
 }
-FP_KERNEL_PAIR1(eflags_tiles, 64, MGArgs, int, lv)
+FP_KERNEL_PAIR(eflags_tiles, 64, MGArgs)
 
 // A level's arrays as one launch sees them: its current iterate p, the other array q.
 template <class T> struct LevelView {

@@ -368,6 +368,8 @@ static const float4 *sharedNoise() {
 }
 
 constexpr int MIN_TILES = 16;
+// Workgroups that walk a step's freshly allocated bricks (a few per step, rarely thousands).
+constexpr unsigned FRESH_GRID = 2048;
 constexpr double MAX_DT = 1.0 / 30.0;
 
 }  // namespace fp
@@ -1315,7 +1317,10 @@ struct fp_sim {
     k_free_bricks<<<blocks, 256, 0, stream>>>(a);
     k_allocate_bricks<<<blocks, 256, 0, stream>>>(a);
     k_link_bricks<<<(slotsGrid + 63) / 64, 64, 0, stream>>>(a);
-    if (a.buildSolids) k_build_solids<<<slotsGrid, 64, 0, stream>>>(a);
+    if (a.buildSolids) {
+      if (a.solidsEvery) k_build_solids<<<slotsGrid, 64, 0, stream>>>(a);
+      else k_fresh_solids<<<std::min(slotsGrid, FRESH_GRID), 64, 0, stream>>>(a);
+    }
     solidsStale = false;
     const int zchunks = std::max(1, nF / 1024);
     k_zero_fresh<<<slotsGrid * zchunks, 256, 0, stream>>>(a, zchunks);
@@ -1389,9 +1394,9 @@ struct fp_sim {
       for (int L = brickLevels; L <= bottom; L++) k_topology_tiles<<<capacity, 64, 0, stream>>>(m, L);
       geometryDirty = false;
     }
-    k_topology_bricks<<<slotsGrid * 2, vLanes, 0, stream>>>(m, 1);
+    k_fresh_topology<<<std::min(slotsGrid * 2, FRESH_GRID), vLanes, 0, stream>>>(m);
     k_eflags_bricks<<<slotsGrid, vLanes, 0, stream>>>(m);
-    for (int L = brickLevels; L <= bottom; L++) k_eflags_tiles<<<capacity, 64, 0, stream>>>(m, L);
+    if (bottom >= brickLevels) k_eflags_tiles<<<capacity, 64, 0, stream>>>(m);
     if (pressureMode == 0) {
       int flips = 0;
       cycle<StoreF32>(m, 0, slotsGrid, false, flips);
@@ -2267,8 +2272,7 @@ struct fp_batch {
       launch<MGArgs>(k_topology_tiles_batch, 64, lay.mg, lay.tiles, wide(), L);
     launch<MGArgs>(k_topology_bricks_batch, 256, lay.mg, lay.slots2, wide(), 1);
     launch<MGArgs>(k_eflags_bricks_batch, 256, lay.mg, lay.slots, wide());
-    for (int L = shape.brickLevels; L <= shape.bottom; L++)
-      launch<MGArgs>(k_eflags_tiles_batch, 64, lay.mg, lay.tiles, wide(), L);
+    if (shape.bottom >= shape.brickLevels) launch<MGArgs>(k_eflags_tiles_batch, 64, lay.mg, lay.tiles, wide());
     if (mode == 0) {
       levelPrefixes<StoreF32>(shape);
       vcycle<StoreF32>(shape, false);
