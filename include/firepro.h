@@ -217,7 +217,10 @@ const char *fp_version(void);
 
 /* FluidSimulation.create (FluidSimulation.ts:359-475): pools, tiles, curl noise. */
 int fp_sim_create(const fp_grid *grid, fp_sim **out);
-/* FluidSimulation.step (FluidSimulation.ts:1367-1373): one substep of `dt` in (0, 1/30]. */
+/* FluidSimulation.step (FluidSimulation.ts:1367-1373): one substep of `dt` in (0, 1/30].
+ * Returns once the step's activity readback has landed (FluidSimulation.ts:1514-1527); the
+ * forces, pressure solve and projection may still run on the simulation's stream. Every
+ * other entry point waits for them before it reads or changes the simulation. */
 int fp_sim_step(fp_sim *sim, const fp_frame *frame, double dt);
 /* FluidSimulation.reset (FluidSimulation.ts:768-779). */
 int fp_sim_reset(fp_sim *sim);
@@ -248,7 +251,8 @@ int fp_sim_read_state(fp_sim *sim, int32_t *ids, float *fields, float *velocity,
 int fp_sim_read_smoke(fp_sim *sim, int32_t *ids, float *density, int32_t capacity_bricks);
 /* Overwrite the cell values of the active bricks, same layout as fp_sim_read_state. */
 int fp_sim_write_state(fp_sim *sim, const float *fields, const float *velocity);
-/* The device stream the solver enqueues on (hipStream_t), for callers that time it. */
+/* The device stream the solver enqueues on (hipStream_t), for callers that time it or
+ * wait on it (a step's tail is queued there when fp_sim_step returns). */
 void *fp_sim_stream(fp_sim *sim);
 /* Timing of the next steps, read back as averages in ms: enable 1 records device events
  * around every stage (and the render's) plus the host phases of a step (host_*); enable 2

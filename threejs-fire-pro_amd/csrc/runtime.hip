@@ -395,6 +395,8 @@ struct fp_sim {
   double time = 0;
   int64_t steps = 0;
   bool mayHaveFlame = false, solidsStale = true, geometryDirty = true;
+  hipEvent_t readbackEvent = nullptr;  // recorded after each step's activity readback
+  bool stepPending = false;            // the last step's tail may still run on the stream
   int colliderCount = 0;
   std::vector<uint32_t> prevColliders, prevMasks;
   std::vector<uint64_t> sourceIds;
@@ -621,6 +623,7 @@ struct fp_sim {
   }
 
   void allocate() {
+    drain();  // released blocks may be reused at once: the last step's tail must be done
     int slots = host::initialPoolSlots(side, slotLimit());
     poolSlots = 0;
     usedSlots = 0;
@@ -650,6 +653,7 @@ struct fp_sim {
   // Pools for `slots` slots; with `keep`, live state (fields, expansion, velocity, solids)
   // carries over and every slot keeps its place.
   void allocatePools(int slots, bool keep) {
+    drain();  // released blocks may be reused at once: the last step's tail must be done
     size_t s = (size_t)slots;
     if (keep) {
       F.grow(s * nF, stream);
@@ -700,6 +704,7 @@ struct fp_sim {
   // Tile resources for `cap` tile slots, keeping each brick's page, state, flags, entry and
   // record (FluidSimulation.ts:663-705).
   void growTiles(int cap) {
+    drain();  // released blocks may be reused at once: the last step's tail must be done
     size_t bricks = (size_t)cap * 64;
     size_t old = (size_t)capacity * 64;
     auto growFill = [&](DBuf<int> &buf, size_t count, int fill) {
@@ -922,7 +927,10 @@ struct fp_sim {
 
   template <class T> void upload(DBuf<T> &buf, const std::vector<T> &host) {
     size_t n = std::max<size_t>(1, host.size());
-    if (buf.n < n) buf.alloc(n + n / 2);
+    if (buf.n < n) {
+      drain();  // the old block goes back to the cache
+      buf.alloc(n + n / 2);
+    }
     if (!host.empty())
       HIPCHECK(hipMemcpyAsync(buf.p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, stream));
   }
@@ -1355,6 +1363,15 @@ struct fp_sim {
       s.p.S = S.p;
       s.p.Sn = Sn.p;
     }
+    // The activity readback (FluidSimulation.ts:1514-1523): the counts, pool state and tile
+    // content flags are final once the fields have evolved, so they are copied out here and
+    // the host's next step overlaps the forces, the pressure solve and the projection.
+    k_readback<<<(unsigned)((capacity + 255) / 256 + 1), 256, 0, stream>>>(counts.p, poolState.p, activity.p + n,
+                                                                           capacity, readback.p);
+    HIPCHECK(hipMemcpyAsync(pinned, readback.p, (4 + (size_t)capacity) * sizeof(int), hipMemcpyDeviceToHost, stream));
+    k_reset_failures<<<1, 1, 0, stream>>>(poolState.p);
+    if (!readbackEvent) HIPCHECK(hipEventCreateWithFlags(&readbackEvent, hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(readbackEvent, stream));
     int m5 = begin();
     if (P.vorticity != 0.f) k_curl<<<slotsGrid, vLanes, 0, stream>>>(s);
     // The velocity predictor is dead after correction: it holds the cell forces.
@@ -1370,13 +1387,16 @@ struct fp_sim {
     s.pressureLo = latestPressureLo;
     k_project<<<slotsGrid, vLanes, 0, stream>>>(s);
     end(m7, "project");
-    k_readback<<<(unsigned)((capacity + 255) / 256 + 1), 256, 0, stream>>>(counts.p, poolState.p, activity.p + n,
-                                                                           capacity, readback.p);
-    HIPCHECK(hipMemcpyAsync(pinned, readback.p, (4 + (size_t)capacity) * sizeof(int), hipMemcpyDeviceToHost, stream));
-    k_reset_failures<<<1, 1, 0, stream>>>(poolState.p);
     hostMark("host_enqueue", enqueued);
     Clock::time_point w0 = Clock::now();
-    HIPCHECK(hipStreamSynchronize(stream));
+    if (profiling && profileLevel != 2) {
+      HIPCHECK(hipStreamSynchronize(stream));  // every stage's interval is read below
+    } else {
+      // The step returns once its readback has landed; the rest stays queued on the stream
+      // (drain() before anything reads or changes device state).
+      HIPCHECK(hipEventSynchronize(readbackEvent));
+      stepPending = true;
+    }
     hostMark("host_wait", w0);
     HIPCHECK(hipGetLastError());
     if (profiling) collect();
@@ -1443,6 +1463,7 @@ struct fp_sim {
     if (capacity > BOTTOM64_TILES) throw InvalidArgument("defect correction supports up to 8192 tile slots.");
     size_t cells0 = (size_t)poolSlots * nV;
     if (dcX.n != cells0) {
+      drain();  // released blocks may be reused at once: the last step's tail must be done
       dcX.alloc(cells0);
       dcXlo.alloc(cells0);
       dcR.alloc(cells0);
@@ -1455,6 +1476,7 @@ struct fp_sim {
       for (int L = 0; L <= bottom; L++) {
         size_t n = L == 0 ? cells0 : lv[L].p.n;
         if (lv16[L].p.n != n) {
+          drain();
           lv16[L].p.alloc(n);
           lv16[L].q.alloc(n);
           lv16[L].rhs.alloc(n);
@@ -1906,8 +1928,18 @@ struct fp_sim {
     return total;
   }
 
+  // Finishes the queued remainder of the last step.
+  void drain() {
+    if (!stepPending) return;
+    stepPending = false;
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
   void destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
+    stepPending = false;
+    if (readbackEvent) (void)hipEventDestroy(readbackEvent);
+    readbackEvent = nullptr;
     DBuf<int> *ints[] = {&tileRec, &hash, &pages, &entries, &records, &active, &zeroList, &poolState, &counts, &createdSlots, &readback};
     for (auto *x : ints) x->release();
     DBuf<uint32_t> *u32s[] = {&brickState, &activity, &stateTmp, &rankTmp, &blockSums, &blockPrefix, &freeSlots, &solids};
@@ -2537,7 +2569,7 @@ int fp_sim_read_bricks(fp_sim *sim, double *mins, int32_t *count) {
     g_error = "fp_sim_read_bricks: null argument";
     return FP_EINVAL;
   }
-  return guard([&] {
+  return guard([&] { sim->drain();
     int n = sim->brickCount();
     std::vector<int32_t> pages(n);
     HIPCHECK(hipMemcpy(pages.data(), sim->pages.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -2561,7 +2593,7 @@ int fp_sim_read_tables(fp_sim *sim, fp_tables *t) {
     g_error = "fp_sim_read_tables: null argument";
     return FP_EINVAL;
   }
-  return guard([&] {
+  return guard([&] { sim->drain();
     int n = sim->brickCount();
     t->brick_count = n;
     t->pool_slots = sim->poolSlots;
@@ -2607,7 +2639,7 @@ int fp_sim_read_state(fp_sim *sim, int32_t *ids, float *fields, float *velocity,
     return FP_EINVAL;
   }
   int result = 0;
-  int rc = guard([&] {
+  int rc = guard([&] { sim->drain();
     int count = 0;
     if (sim->stepped) HIPCHECK(hipMemcpy(&count, sim->counts.p, sizeof(int), hipMemcpyDeviceToHost));
     if (count > capacity) throw InvalidArgument("state buffers hold fewer bricks than are active.");
@@ -2644,7 +2676,7 @@ int fp_sim_write_state(fp_sim *sim, const float *fields, const float *velocity) 
     g_error = "fp_sim_write_state: null argument";
     return FP_EINVAL;
   }
-  return guard([&] {
+  return guard([&] { sim->drain();
     int count = 0;
     if (sim->stepped) HIPCHECK(hipMemcpy(&count, sim->counts.p, sizeof(int), hipMemcpyDeviceToHost));
     std::vector<int32_t> act(count), entries(sim->brickCount());
@@ -2670,7 +2702,7 @@ int fp_sim_read_smoke(fp_sim *sim, int32_t *ids, float *density, int32_t capacit
     return FP_EINVAL;
   }
   int result = 0;
-  int rc = guard([&] {
+  int rc = guard([&] { sim->drain();
     if (sim->smokeDiv == 1) throw InvalidArgument("this simulation has no coarse smoke grid (smokeDivisor 1).");
     int count = 0;
     if (sim->stepped) HIPCHECK(hipMemcpy(&count, sim->counts.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -2725,7 +2757,7 @@ int fp_sim_scene_lights(fp_sim *sim, const fp_render_settings *settings, const d
     g_error = "fp_sim_scene_lights: null argument";
     return FP_EINVAL;
   }
-  return guard([&] { sim->sceneLights(*settings, anchors, count, sums); });
+  return guard([&] { sim->drain(); sim->sceneLights(*settings, anchors, count, sums); });
 }
 
 int fp_batch_create(fp_batch **out) {
@@ -2751,7 +2783,11 @@ int fp_batch_step(fp_batch *batch, fp_sim *const *sims, const fp_frame *frames, 
     g_error = "fp_batch_step: null argument";
     return FP_EINVAL;
   }
-  return guard([&] { batch->step(sims, frames, count, dt); });
+  return guard([&] {
+    for (int i = 0; i < count; i++)
+      if (sims[i]) sims[i]->drain();  // a single step's tail may still be queued
+    batch->step(sims, frames, count, dt);
+  });
 }
 
 int fp_batch_last_ms(const fp_batch *batch, double *ms) {
@@ -2817,7 +2853,7 @@ int fp_sim_render(fp_sim *sim, const fp_camera *camera, const fp_render_settings
     g_error = "fp_sim_render: null argument";
     return FP_EINVAL;
   }
-  return guard([&] { sim->renderImage(*camera, *settings, rgba, false); });
+  return guard([&] { sim->drain(); sim->renderImage(*camera, *settings, rgba, false); });
 }
 
 int fp_sim_render_device(fp_sim *sim, const fp_camera *camera, const fp_render_settings *settings,
@@ -2826,7 +2862,7 @@ int fp_sim_render_device(fp_sim *sim, const fp_camera *camera, const fp_render_s
     g_error = "fp_sim_render_device: null argument";
     return FP_EINVAL;
   }
-  return guard([&] { sim->renderImage(*camera, *settings, rgba_device, true); });
+  return guard([&] { sim->drain(); sim->renderImage(*camera, *settings, rgba_device, true); });
 }
 
 int fp_sim_read_render(fp_sim *sim, fp_render_tables *out) {
@@ -2834,7 +2870,7 @@ int fp_sim_read_render(fp_sim *sim, fp_render_tables *out) {
     g_error = "fp_sim_read_render: null argument";
     return FP_EINVAL;
   }
-  return guard([&] { sim->readRender(out); });
+  return guard([&] { sim->drain(); sim->readRender(out); });
 }
 
 }  // extern "C"
