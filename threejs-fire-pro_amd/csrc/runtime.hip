@@ -405,6 +405,7 @@ struct fp_sim {
   host::TileDirectory dir;
   int capacity = 0;
   std::vector<int32_t> tileRecHost, hashHost, tilesHost;
+  std::vector<int> created;  // tile slots created by the last table upload
   std::vector<host::TileBox> clusters;
   uint32_t hashMask = 0;
   hipStream_t stream = nullptr;
@@ -647,6 +648,7 @@ struct fp_sim {
     capacity = 0;
     growTiles(MIN_TILES);
     applyTiles();
+    HIPCHECK(hipStreamSynchronize(stream));  // no step's readback follows the uploads yet
     rboxes.update(contentTiles, {}, ground, dir, tilesHost, clusters);
   }
 
@@ -773,19 +775,26 @@ struct fp_sim {
     hostMark("host_tiles_tables", t0);
     HIPCHECK(hipMemcpyAsync(tileRec.p, tileRecHost.data(), tileRecHost.size() * sizeof(int32_t),
                             hipMemcpyHostToDevice, stream));
-    if (hash.n != hashHost.size()) hash.alloc(hashHost.size());
+    if (hash.n != hashHost.size()) {
+      drain();  // the old block goes back to the cache
+      hash.alloc(hashHost.size());
+    }
     HIPCHECK(hipMemcpyAsync(hash.p, hashHost.data(), hashHost.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     hashMask = (uint32_t)(hashHost.size() / 4 - 1);
-    std::vector<int> created = dir.takeCreated();
+    created = dir.takeCreated();
     if (!created.empty()) {
-      if (createdSlots.n < created.size()) createdSlots.alloc(std::max<size_t>(created.size(), 64));
+      if (createdSlots.n < created.size()) {
+        drain();
+        createdSlots.alloc(std::max<size_t>(created.size(), 64));
+      }
       HIPCHECK(hipMemcpyAsync(createdSlots.p, created.data(), created.size() * sizeof(int), hipMemcpyHostToDevice,
                               stream));
       int n = (int)created.size() * 64;
       k_clear_tiles<<<(n + 255) / 256, 256, 0, stream>>>(pages.p, entries.p, brickState.p, activity.p,
                                                           createdSlots.p, (int)created.size());
     }
-    HIPCHECK(hipStreamSynchronize(stream));  // host vectors are reused
+    // The copies read tileRecHost, hashHost and created when the stream reaches them; each
+    // step waits for its readback, queued after them, before these are rebuilt.
     geometryDirty = true;
     rboxes.stale = true;  // FluidSimulation.ts:708-709
   }
