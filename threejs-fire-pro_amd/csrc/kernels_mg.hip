@@ -372,18 +372,14 @@ __device__ __forceinline__ float diagonal(float w, int flags) {
 // stencil (pressure.wgsl:376-402): the neighbor sum over open sides. `a` is the cell's own
 // index in the level array, whose slots hold cubes `sd` cells a side: a neighbor inside the
 // same cube is at a fixed offset from `a`; only neighbors across a face resolve their brick
-// or tile.
+// or tile. The six neighbors are read whatever the flags say (their addresses do not depend
+// on them), so the reads overlap the cell's own flag read; closed sides then count zero.
 template <class St>
-__device__ __forceinline__ float stencilSum(const MGCtx &c, const typename St::T *pv, const int p[3], int flags, int lv,
-                                            float w, long a, int sd) {
-  float n[6];
+__device__ __forceinline__ void neighbors(const MGCtx &c, const typename St::T *pv, const int p[3], int lv, long a,
+                                          int sd, float n[6]) {
   const int m = sd - 1;
   const int local[3] = {p[0] & m, p[1] & m, p[2] & m};
   for (int k = 0; k < 6; k++) {
-    if (!((flags >> k) & 1)) {
-      n[k] = 0.f;
-      continue;
-    }
     const int lx = local[0] + OFF[k][0], ly = local[1] + OFF[k][1], lz = local[2] + OFF[k][2];
     if ((unsigned)lx < (unsigned)sd && (unsigned)ly < (unsigned)sd && (unsigned)lz < (unsigned)sd) {
       n[k] = St::ld(pv[a + OFF[k][0] + (OFF[k][1] + OFF[k][2] * sd) * sd]);
@@ -392,22 +388,37 @@ __device__ __forceinline__ float stencilSum(const MGCtx &c, const typename St::T
       n[k] = readAt<St>(c, pv, q, lv);
     }
   }
-  return w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
+}
+__device__ __forceinline__ float openSum(const float n[6], int flags, float w) {
+  float o[6];
+  for (int k = 0; k < 6; k++) o[k] = ((flags >> k) & 1) ? n[k] : 0.f;
+  return w * (o[0] + o[1]) + w * (o[2] + o[3]) + w * (o[4] + o[5]);
+}
+template <class St>
+__device__ __forceinline__ float stencilSum(const MGCtx &c, const typename St::T *pv, const int p[3], int flags, int lv,
+                                            float w, long a, int sd) {
+  float n[6];
+  neighbors<St>(c, pv, p, lv, a, sd, n);
+  return openSum(n, flags, w);
 }
 
 // relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8, p to q.
 template <class St>
 __device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelView<typename St::T> &lv, int level, const int p[3],
                                           size_t a, float w) {
-  int e = lv.eflag[a];
+  // Every read is issued before the flags are looked at.
+  const int e = lv.eflag[a];
+  const float own = St::ld(lv.p[a]), rhs = St::ld(lv.rhs[a]);
+  float n[6];
+  neighbors<St>(c, lv.p, p, level, (long)a, lv.side, n);
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
     lv.q[a] = St::st(0.f);
     return;
   }
-  int flags = e & 63;
-  float sum = stencilSum<St>(c, lv.p, p, flags, level, w, (long)a, lv.side);
-  float cand = (sum - St::ld(lv.rhs[a])) / fmaxv(diagonal(w, flags), 1e-12f);
-  lv.q[a] = St::st(mixv(St::ld(lv.p[a]), cand, 0.8f));
+  const int flags = e & 63;
+  const float sum = openSum(n, flags, w);
+  const float cand = (sum - rhs) / fmaxv(diagonal(w, flags), 1e-12f);
+  lv.q[a] = St::st(mixv(own, cand, 0.8f));
 }
 
 // The single and batched kernels of a solve stage; a simulation whose defect correction
