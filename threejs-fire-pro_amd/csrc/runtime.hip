@@ -2084,6 +2084,9 @@ struct fp_batch {
   }
 
   // The launch sequence of a step depends on these; one captured graph per distinct key.
+  // Lanes of the velocity-grid and brick-level launches: one per velocity cell of a brick,
+  // 64 to 256 (members share B and d). Any count is correct; graphs keep the one captured.
+  unsigned vLanes = 256;
   using GraphKey = std::array<int, 10>;
   std::map<GraphKey, hipGraphExec_t> execs;
 
@@ -2155,6 +2158,7 @@ struct fp_batch {
 
   void step(fp_sim *const *sims, const fp_frame *frames, int n, double dt) {
     if (n <= 0) return;
+    vLanes = (unsigned)std::min(256, std::max(64, sims[0] ? sims[0]->nV : 256));
     for (int i = 0; i < n; i++) {
       if (!sims[i]) throw InvalidArgument("batch member is null.");
       if (sims[i]->B != sims[0]->B || sims[i]->d != sims[0]->d)
@@ -2299,20 +2303,20 @@ struct fp_batch {
     launch<AllocArgs>(k_build_solids_batch, 64, lay.alloc, lay.slots, wide());
     launch<AllocArgs>(k_zero_fresh_batch, 256, lay.alloc, lay.zero, wide(), zchunks);
     // Transport, combustion, forces.
-    launch<StepArgs>(k_advect_velocity_batch, 256, lay.pre, lay.slots, wide());
-    launch<StepArgs>(k_correct_velocity_batch, 256, lay.pre, lay.slots, wide());
+    launch<StepArgs>(k_advect_velocity_batch, vLanes, lay.pre, lay.slots, wide());
+    launch<StepArgs>(k_correct_velocity_batch, vLanes, lay.pre, lay.slots, wide());
     launch<StepArgs>(k_advect_scalars_batch, 256, lay.pre, lay.field, wide(), fchunks);
     launch<StepArgs>(k_evolve_fields_batch, 256, lay.pre, lay.field, wide(), fchunks);
-    launch<StepArgs>(k_curl_batch, 256, lay.post, lay.slots, wide());
-    launch<StepArgs>(k_cell_forces_batch, 256, lay.post, lay.slots, wide());
-    launch<StepArgs>(k_forces_batch, 256, lay.post, lay.slots, wide());
-    launch<StepArgs>(k_divergence_batch, 256, lay.post, lay.slots, wide());
+    launch<StepArgs>(k_curl_batch, vLanes, lay.post, lay.slots, wide());
+    launch<StepArgs>(k_cell_forces_batch, vLanes, lay.post, lay.slots, wide());
+    launch<StepArgs>(k_forces_batch, vLanes, lay.post, lay.slots, wide());
+    launch<StepArgs>(k_divergence_batch, vLanes, lay.post, lay.slots, wide());
     // Pressure: topology, flags, then one V-cycle or defect correction.
-    launch<MGArgs>(k_topology_bricks_batch, 256, lay.mg, lay.slots, wide(), 0);
+    launch<MGArgs>(k_topology_bricks_batch, vLanes, lay.mg, lay.slots, wide(), 0);
     for (int L = shape.brickLevels; L <= shape.bottom; L++)
       launch<MGArgs>(k_topology_tiles_batch, 64, lay.mg, lay.tiles, wide(), L);
-    launch<MGArgs>(k_topology_bricks_batch, 256, lay.mg, lay.slots2, wide(), 1);
-    launch<MGArgs>(k_eflags_bricks_batch, 256, lay.mg, lay.slots, wide());
+    launch<MGArgs>(k_topology_bricks_batch, vLanes, lay.mg, lay.slots2, wide(), 1);
+    launch<MGArgs>(k_eflags_bricks_batch, vLanes, lay.mg, lay.slots, wide());
     if (shape.bottom >= shape.brickLevels) launch<MGArgs>(k_eflags_tiles_batch, 64, lay.mg, lay.tiles, wide());
     if (mode == 0) {
       levelPrefixes<StoreF32>(shape);
@@ -2324,7 +2328,7 @@ struct fp_batch {
   }
   // The projection and the activity readback.
   void launchTail() {
-    launch<StepArgs>(k_project_batch, 256, lay.post, lay.slots, wide());
+    launch<StepArgs>(k_project_batch, vLanes, lay.post, lay.slots, wide());
     k_readback_batch<<<perSim(), 256, 0, stream>>>(reinterpret_cast<const ReadbackArgs *>(devArgs.p + lay.rb),
                                                    lay.cap);
   }
