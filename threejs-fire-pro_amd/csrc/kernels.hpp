@@ -262,7 +262,8 @@ __global__ void k_smooth_coarse(RenderArgs r);
 __global__ void k_occupancy(RenderArgs r);
 __global__ void k_light(RenderArgs r, int fresh);
 __global__ void k_march(RenderArgs r, float4 *out, const float4 *slices, int sw, int sh);
-__global__ void k_slices(RenderArgs r, float4 *slices, int sw, int sh);
+__global__ void k_slices(RenderArgs r, float4 *slices, int sw, int sh, const int *order, uint32_t *cost);
+__global__ void k_order_tiles(const uint32_t *cost, int tiles, int *order);
 // Scene lights: per-brick totals by nearest anchor, then per-anchor totals in a fixed order
 // (blocks of LIGHT_BLOCK bricks).
 constexpr int LIGHT_BLOCK = 256;

@@ -691,6 +691,7 @@ __device__ __forceinline__ void rayOf(const RenderParams &P, float uvx, float uv
 // evenly spaced depths from `sx` to `sy`, where the state is stored into `slots`.
 struct MarchState {
   float t, transmittance, rad[3];
+  int steps;  // loop iterations taken (a cost measure for scheduling)
 };
 __device__ __forceinline__ void march(const RenderArgs &r, const float o[3], const float d[3], float stepSize, float jit,
                                       float end, MarchState &st, int count = 0, float sx = 0.f, float sy = 0.f,
@@ -721,8 +722,10 @@ __device__ __forceinline__ void march(const RenderArgs &r, const float o[3], con
   const float spacing = record ? fmaxv(sy - sx, 0.0f) / (float)count : 0.0f;
   float depth = sx + spacing;
   int recorded = 0;
+  int steps = 0;
   while (true) {
     if (record ? (recorded >= count || t >= sy || transmittance < 0.008f) : (t >= end || transmittance < 0.008f)) break;
+    steps++;
     if (t >= curExit) {
       // fireProEnter (volume-march.wgsl:160-172)
       float nx = 1e30f, ny = 0.0f;
@@ -803,6 +806,7 @@ __device__ __forceinline__ void march(const RenderArgs &r, const float o[3], con
   // Depths past the exit or an opaque ray hold the final values.
   for (int k = recorded; k < count; k++) slots[k * stride] = make_float4(rad[0], rad[1], rad[2], 1.0f - transmittance);
   st.t = t;
+  st.steps += steps;
   st.transmittance = transmittance;
   st.rad[0] = rad[0];
   st.rad[1] = rad[1];
@@ -830,17 +834,51 @@ __device__ __forceinline__ void spanOf(const RenderArgs &r, const float o[3], co
 // buildSlices (volume-slices.wgsl:27-122): the half-resolution march, a lane per texel of
 // a (ceil(w/2), ceil(h/2)) grid, storing its state at SLICES evenly spaced depths from the
 // first box entry to the last exit: slices[(k * sh + y) * sw + x].
-__global__ void __launch_bounds__(256) k_slices(RenderArgs r, float4 *slices, int sw, int sh) {
+__device__ __forceinline__ int sliceTexel(const RenderArgs &r, float4 *slices, int sw, int sh, int px, int py) {
   const RenderParams &P = r.P;
-  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (px >= sw || py >= sh) return;
   float o[3], d[3], sx, sy, sz;
   rayOf(P, ((float)px + 0.5f) / (float)sw, ((float)py + 0.5f) / (float)sh, o, d);
   spanOf(r, o, d, sx, sy, sz);
   const float stepSize = fmaxv(P.voxel * 0.5f, sz / P.raySteps);
-  MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}};
+  MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}, 0};
   march(r, o, d, stepSize, jitter((uint32_t)px, (uint32_t)py, P.time), sy, st, SLICES, sx, sy,
         slices + ((size_t)py * sw + px), (size_t)sw * sh);
+  return st.steps;
+}
+// The same over 16 x 16 texel tiles, a workgroup each, taken in the order `order` (tile ids
+// row-major, longest first from the last frame) or in id order; each tile's cost, the sum
+// over its wavefronts of their longest march, goes to `cost`.
+__global__ void __launch_bounds__(256) k_slices(RenderArgs r, float4 *slices, int sw, int sh, const int *order,
+                                                uint32_t *cost) {
+  __shared__ uint32_t waveCost[4];
+  const int tilesX = (sw + 15) >> 4;
+  const int tile = order ? order[blockIdx.x] : (int)blockIdx.x;
+  const int px = (tile % tilesX) * 16 + (threadIdx.x & 15), py = (tile / tilesX) * 16 + (threadIdx.x >> 4);
+  if (threadIdx.x < 4) waveCost[threadIdx.x] = 0u;
+  __syncthreads();
+  const int steps = (px < sw && py < sh) ? sliceTexel(r, slices, sw, sh, px, py) : 0;
+  atomicMax(&waveCost[threadIdx.x >> 6], (uint32_t)steps);
+  __syncthreads();
+  if (threadIdx.x == 0 && cost) cost[tile] = waveCost[0] + waveCost[1] + waveCost[2] + waveCost[3];
+}
+// Tile ids by decreasing cost (by its binary order of magnitude; ties in any order), one
+// workgroup: the next frame starts its longest tiles first, so the short ones fill the tail.
+__global__ void __launch_bounds__(1024) k_order_tiles(const uint32_t *cost, int tiles, int *order) {
+  __shared__ int start[33];
+  if (threadIdx.x < 33) start[threadIdx.x] = 0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < tiles; t += blockDim.x) atomicAdd(&start[__clz((int)cost[t])], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sum = 0;
+    for (int b = 0; b < 33; b++) {
+      const int c = start[b];
+      start[b] = sum;
+      sum += c;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < tiles; t += blockDim.x) order[atomicAdd(&start[__clz((int)cost[t])], 1)] = t;
 }
 
 // fireProSceneVolume's beauty path (scene-volume.wgsl:38-48, :146-167) with the march of
@@ -859,7 +897,7 @@ __global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out, const 
   if (sx < sy) {
     const float end = sy;
     const float stepSize = fmaxv(P.voxel * 0.5f, sz / P.raySteps);
-    MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}};
+    MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}, 0};
     if (slices) {
       const float count = (float)SLICES;
       const float slice = floorf((end - sx) / (sy - sx) * count);

@@ -469,6 +469,11 @@ struct fp_sim {
     DBuf<float4> R, L, lut, image;
     DBuf<float> RS;  // smoothed coarse smoke with aprons (smokeDivisor 2 or 4)
     DBuf<float4> slices;  // the half-resolution march's depth slices
+    // The slice march's 16 x 16 tiles: last frame's cost per tile and the order it implies
+    // (longest first); orderTiles is the tile count the order was made for (0: none yet).
+    DBuf<uint32_t> tileCost;
+    DBuf<int> tileOrder;
+    int orderTiles = 0;
     DBuf<uint8_t> O;
     // Scene lights: anchors, per-brick totals and anchors, per-block totals, the totals.
     DBuf<float4> anchors;
@@ -1806,9 +1811,18 @@ struct fp_sim {
       sh = std::max(1, (P.height + 1) / 2);
       size_t n = (size_t)sw * sh * 8;
       if (rc.slices.n < n) rc.slices.alloc(n);
+      const int tiles = ((sw + 15) / 16) * ((sh + 15) / 16);
+      if (rc.tileCost.n < (size_t)tiles) {
+        rc.tileCost.alloc(tiles);
+        rc.tileOrder.alloc(tiles);
+        rc.orderTiles = 0;
+      }
+      const int *order = rc.orderTiles == tiles ? rc.tileOrder.p : nullptr;
       int t0 = begin();
-      k_slices<<<dim3((unsigned)((sw + 15) / 16), (unsigned)((sh + 15) / 16)), 256, 0, stream>>>(r, rc.slices.p, sw, sh);
+      k_slices<<<(unsigned)tiles, 256, 0, stream>>>(r, rc.slices.p, sw, sh, order, rc.tileCost.p);
       end(t0, "render_slices");
+      k_order_tiles<<<1, 1024, 0, stream>>>(rc.tileCost.p, tiles, rc.tileOrder.p);
+      rc.orderTiles = tiles;
       slices = rc.slices.p;
     }
     int t = begin();
@@ -1988,6 +2002,9 @@ struct fp_sim {
     rc.R.release();
     rc.RS.release();
     rc.slices.release();
+    rc.tileCost.release();
+    rc.tileOrder.release();
+    rc.orderTiles = 0;
     rc.L.release();
     rc.lut.release();
     rc.image.release();
