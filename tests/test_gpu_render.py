@@ -58,6 +58,38 @@ def test_campfire_render_bit_exact():
         gpu.dispose()
 
 
+def test_half_resolution_tile_order():
+    """The slice march takes its tiles in the order the previous frame's costs give (the
+    first render in id order): renders of one state and of the next steps stay equal to the
+    restatement's, frame after frame."""
+    from firepro import scenes
+    abi = _abi()
+    sc = scenes.campfire(steps=40)
+    gpu, cpu = make_engines(sc)
+    try:
+        frames = list(sc.frames())
+        for frame, g in frames[:38]:
+            apply(gpu, cpu, g)
+            gpu.step(frame)
+            cpu.step(frame)
+        cam = abi.camera((-4.12, 1.96, 5.30), (0, 1.25, 0), 44, 160, 128)
+        rs = abi.render_settings(half_resolution=True)
+        for k in range(2):
+            print(f"case tile order, same state, render {k}", flush=True)
+            msgs, img = render_mismatches(gpu, cpu, cam, rs, f"render {k}")
+            assert msgs == []
+        for frame, g in frames[38:]:
+            apply(gpu, cpu, g)
+            gpu.step(frame)
+            cpu.step(frame)
+            print("case tile order, next step", flush=True)
+            msgs, img = render_mismatches(gpu, cpu, cam, rs, "next step")
+            assert msgs == []
+        assert img[..., 3].max() > 0.1, "the fire is in view"
+    finally:
+        gpu.dispose()
+
+
 def test_render_schedule_every_step():
     """A render after every step: lighting alternates between whole builds and the slots
     given out in between (lightFresh); settings changes rebuild it."""
