@@ -629,7 +629,7 @@ struct fp_sim {
   }
 
   void allocate() {
-    drain();  // released blocks may be reused at once: the last step's tail must be done
+    drain();  // the last step's queued tail finishes before its buffers are replaced
     int slots = host::initialPoolSlots(side, slotLimit());
     poolSlots = 0;
     usedSlots = 0;
@@ -660,7 +660,7 @@ struct fp_sim {
   // Pools for `slots` slots; with `keep`, live state (fields, expansion, velocity, solids)
   // carries over and every slot keeps its place.
   void allocatePools(int slots, bool keep) {
-    drain();  // released blocks may be reused at once: the last step's tail must be done
+    drain();  // the last step's queued tail finishes before its buffers are replaced
     size_t s = (size_t)slots;
     if (keep) {
       F.grow(s * nF, stream);
@@ -711,7 +711,7 @@ struct fp_sim {
   // Tile resources for `cap` tile slots, keeping each brick's page, state, flags, entry and
   // record (FluidSimulation.ts:663-705).
   void growTiles(int cap) {
-    drain();  // released blocks may be reused at once: the last step's tail must be done
+    drain();  // the last step's queued tail finishes before its buffers are replaced
     size_t bricks = (size_t)cap * 64;
     size_t old = (size_t)capacity * 64;
     auto growFill = [&](DBuf<int> &buf, size_t count, int fill) {
@@ -781,7 +781,7 @@ struct fp_sim {
     HIPCHECK(hipMemcpyAsync(tileRec.p, tileRecHost.data(), tileRecHost.size() * sizeof(int32_t),
                             hipMemcpyHostToDevice, stream));
     if (hash.n != hashHost.size()) {
-      drain();  // the old block goes back to the cache
+      drain();  // the last step's queued tail may still read the old block
       hash.alloc(hashHost.size());
     }
     HIPCHECK(hipMemcpyAsync(hash.p, hashHost.data(), hashHost.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
@@ -942,7 +942,7 @@ struct fp_sim {
   template <class T> void upload(DBuf<T> &buf, const std::vector<T> &host) {
     size_t n = std::max<size_t>(1, host.size());
     if (buf.n < n) {
-      drain();  // the old block goes back to the cache
+      drain();  // the last step's queued tail may still read the old block
       buf.alloc(n + n / 2);
     }
     if (!host.empty())
@@ -1477,7 +1477,7 @@ struct fp_sim {
     if (capacity > BOTTOM64_TILES) throw InvalidArgument("defect correction supports up to 8192 tile slots.");
     size_t cells0 = (size_t)poolSlots * nV;
     if (dcX.n != cells0) {
-      drain();  // released blocks may be reused at once: the last step's tail must be done
+      drain();  // the last step's queued tail finishes before its buffers are replaced
       dcX.alloc(cells0);
       dcXlo.alloc(cells0);
       dcR.alloc(cells0);
