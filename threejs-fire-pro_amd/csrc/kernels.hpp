@@ -4,6 +4,9 @@
 
 namespace fp {
 
+// Most bricks (or tile slots) one multigrid workgroup takes at once.
+constexpr int MG_GROUP = 32;
+
 struct AllocArgs {
   Dir dir;
   Layout L;

@@ -148,7 +148,6 @@ __device__ __forceinline__ void levelCell(int home0, int home1, int home2, int s
 // (active bricks or tile slots) with `cells` lanes each: group g covers items g*k .. g*k+k-1
 // with k = lanes / cells (at most MG_GROUP); lane t works on cell t % cells of item
 // t / cells. With cells >= lanes, k = 1 and the lanes stride over the item's cells.
-constexpr int MG_GROUP = 32;
 __device__ __forceinline__ int groupSize(int cells, int lanes) {
   return cells >= lanes ? 1 : (lanes / cells < MG_GROUP ? lanes / cells : MG_GROUP);
 }

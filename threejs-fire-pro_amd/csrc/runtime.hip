@@ -1547,8 +1547,16 @@ struct fp_sim {
 
   // One V-cycle from level L (PressureSolver.ts:303-363). `flips` tracks which array holds
   // each level's iterate: a sweep writes the other array and flips its level's bit.
+  // Workgroups of a grouped multigrid launch over brick level L: groups of groupSize(cells,
+  // 256) bricks (the kernels' grouping) for at most poolSlots bricks, in whole XCD rounds.
+  unsigned groupGrid(int L) const {
+    const int sd = b >> L, n = sd * sd * sd;
+    const int k = n >= 256 ? 1 : std::min(256 / n, MG_GROUP);
+    const int groups = (std::max(1, poolSlots) + k - 1) / k;
+    return (unsigned)((groups + 7) / 8 * 8);
+  }
   template <class St> void relax(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid, int &flips) {
-    if (L < brickLevels) k_mg_relax_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, L | flips << 8);
+    if (L < brickLevels) k_mg_relax_bricks<St><<<groupGrid(L), 256, 0, stream>>>(m, L | flips << 8);
     else k_mg_relax_tiles<St><<<capacity, 64, 0, stream>>>(m, L | flips << 8);
     flips ^= 1 << L;
   }
@@ -1560,15 +1568,15 @@ struct fp_sim {
         for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax<St>(m, L, slotsGrid, flips);
       return;
     }
-    if (L == 0) k_mg_init<St><<<slotsGrid, 256, 0, stream>>>(m, flips << 8);
+    if (L == 0) k_mg_init<St><<<groupGrid(0), 256, 0, stream>>>(m, flips << 8);
     else {
       relax<St>(m, L, slotsGrid, flips);
       relax<St>(m, L, slotsGrid, flips);
     }
-    if (L + 1 < brickLevels) k_mg_restrict_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, (L + 1) | flips << 8);
+    if (L + 1 < brickLevels) k_mg_restrict_bricks<St><<<groupGrid(L + 1), 256, 0, stream>>>(m, (L + 1) | flips << 8);
     else k_mg_restrict_tiles<St><<<capacity, 64, 0, stream>>>(m, (L + 1) | flips << 8);
     cycle<St>(m, L + 1, slotsGrid, bottom64, flips);
-    if (L < brickLevels) k_mg_prolong_bricks<St><<<slotsGrid, 256, 0, stream>>>(m, L | flips << 8);
+    if (L < brickLevels) k_mg_prolong_bricks<St><<<groupGrid(L), 256, 0, stream>>>(m, L | flips << 8);
     else k_mg_prolong_tiles<St><<<capacity, 64, 0, stream>>>(m, L | flips << 8);
     flips ^= 1 << L;
     relax<St>(m, L, slotsGrid, flips);
