@@ -306,6 +306,11 @@ __device__ __forceinline__ void zero_fresh_body(const AllocArgs &a, const int vb
     for (int c = chunk * 256 + threadIdx.x; c < a.L.nS; c += chunks * 256) p.S[(size_t)slot * a.L.nS + c] = 0.f;
 }
 FP_KERNEL_PAIR1(zero_fresh, 256, AllocArgs, int, chunks)
+// The same over the zeroed list only, walked by a fixed grid (a step zeroes few bricks).
+__global__ void __launch_bounds__(256) k_fresh_zero(AllocArgs a, int chunks) {
+  const int total = a.counts[1] * chunks;
+  for (int v = (int)blockIdx.x; v < total; v += (int)gridDim.x) zero_fresh_body(a, v, chunks);
+}
 
 
 }  // namespace fp

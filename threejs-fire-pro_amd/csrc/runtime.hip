@@ -1345,7 +1345,7 @@ struct fp_sim {
     }
     solidsStale = false;
     const int zchunks = std::max(1, nF / 1024);
-    k_zero_fresh<<<slotsGrid * zchunks, 256, 0, stream>>>(a, zchunks);
+    k_fresh_zero<<<std::min(slotsGrid * (unsigned)zchunks, FRESH_GRID), 256, 0, stream>>>(a, zchunks);
     end(m0, "allocation");
 
     StepArgs s = stepArgs();
