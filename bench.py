@@ -3,13 +3,18 @@
 Workload: four fire spheres at seeded positions in a 6 m x 4 m x 6 m region, curl-noise
 turbulence (strength 1, scale 2), one sphere collider of radius 0.5 m, 0.025 m voxels,
 8-cell bricks, plain FP32, 300 steps at 1/60 s. A step is one solver step of the whole
-simulation. Warmup steps advance the scene untimed (by default to step 240); the timed
-steps follow (steps 240-299). Inputs are synthetic and already resident on the device; the
-host only hands each step's plain-data frame over.
+simulation. The scene is first advanced untimed to step 240 (--state-step), whatever the
+warmup, so the timed steps always see the late, large state (about 15.6 K active bricks);
+then --warmup steps run untimed and --steps steps are timed (by default steps 245-299).
+Inputs are synthetic and already resident on the device; the host only hands each step's
+plain-data frame over.
 
-Prints one JSON line with the metric, p50/p99 step latency, the dominant kernel's roofline
-(algorithmic bytes over its event-timed duration) and a CPU baseline (the FP32 restatement
-on one host core, over a bounded sample of the same workload).
+Prints one JSON line with the metric, p50/p99 step latency (device time per step from HIP
+events on the simulation's stream), the dominant kernel's roofline (its algorithmic bytes
+over its event-timed duration), the metric's comparison (the serving stream batched with
+BF16 multigrid levels against one simulation per launch in FP32) and a CPU baseline (the
+FP32 restatement on all host threads and on one, over a bounded sample of the same
+workload).
 """
 from __future__ import annotations
 
@@ -25,29 +30,34 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "threejs-fire-pro_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per active field cell of each stage (plain FP32, velocityDivisor 2):
-# compulsory traffic of the arrays the stage reads and writes, once each.
+# Algorithmic bytes per active field cell of each stage kernel (plain FP32, velocityDivisor
+# 2): the compulsory traffic of the arrays the reference's stage reads and writes, once each.
 S_F, S_V, D3 = 4, 4, 8
 STAGE_BYTES_PER_FIELD_CELL = {
-    # fields in; predictor and the three backtrace planes out; velocity
-    "advect_scalars": 20 * S_F + 3 * S_V / D3,
-    # predictor, fields and the backtrace planes in; fields and expansion out; velocity
-    "evolve_fields": 25 * S_F + 3 * S_V / D3,
-    # velocity in; advected velocity and the four backtrace planes out
-    "advect_velocity": 22 * S_V / D3,
-    # velocity, advected velocity and the backtrace planes in; corrected velocity out
-    "correct_velocity": 25 * S_V / D3,
+    "advect_scalars": 8 * S_F + 3 * S_V / D3,    # fields -> predictor, velocity under the cell
+    "evolve_fields": 13 * S_F + 3 * S_V / D3,    # predictor, fields, velocity -> fields, expansion
+    "advect_velocity": 6 * S_V / D3,             # u -> advected u (per velocity cell, over d^3)
+    "correct_velocity": 9 * S_V / D3,            # u, advected u -> corrected u
+}
+# Bytes beyond those the kernels move by design: the backtrace planes (three float4 per
+# field cell, four per velocity cell) that the advection writes and the correction reads.
+STAGE_OVERHEAD_PER_FIELD_CELL = {
+    "advect_scalars": 12 * S_F, "evolve_fields": 12 * S_F,
+    "advect_velocity": 16 * S_V / D3, "correct_velocity": 16 * S_V / D3,
 }
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=60)
-    p.add_argument("--warmup", type=int, default=240)
+    p.add_argument("--steps", type=int, default=55)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--state-step", type=int, default=240,
+                   help="steps advanced untimed before the warmup: the state the timed steps see")
     p.add_argument("--seed", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--cpu-seconds", type=float, default=15.0,
+                   help="seconds of the CPU baseline on all host threads (one thread gets two thirds)")
     p.add_argument("--render-steps", type=int, default=6,
                    help="steps after the timed ones, each followed by a 1920x1080 render")
     p.add_argument("--serving", action="store_true",
@@ -62,9 +72,9 @@ def parse():
     p.add_argument("--explosions", action="store_true", help="the brick-churn workload instead")
     p.add_argument("--presets", type=int, default=0, metavar="STEPS",
                    help="headless runs of the shipped presets (tests/golden/presets), STEPS each")
-    p.add_argument("--mix", type=int, default=48,
-                   help="requests of the serving workload for the batched mixed-precision comparison "
-                        "in the default line (0 skips it)")
+    p.add_argument("--mix", type=int, default=256,
+                   help="requests of the serving stream (Poisson at --serving-rate) for the batched "
+                        "mixed-precision comparison in the default line (0 skips it)")
     p.add_argument("--copy-probe", type=int, default=0,
                    help="only copy this many MiB device to device 5 times (counter calibration)")
     return p.parse_args()
@@ -83,26 +93,52 @@ def copy_probe(mib: int) -> None:
     print(json.dumps({"copy_probe_bytes": n, "copies": 5}))
 
 
+def host_threads() -> int:
+    """Threads the host may use: the CPUs this process may run on, capped by
+    OMP_NUM_THREADS when it is set (a shared box gives each GPU its share)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
 def cpu_baseline(scene_fn, seconds: float) -> dict:
-    """The FP32 restatement on one core over the first steps of the same scene, stopping
-    after about `seconds` of work."""
+    """The FP32 restatement over the first steps of the same scene: on all host threads
+    for about `seconds` of work (the reported value), then on one thread for two thirds of
+    that. Its per-brick passes run on OpenMP threads with results independent of the count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
     from oracle import OracleSimulation
-    sc = scene_fn()
-    sim = OracleSimulation.create(sc.grid(), 32)
-    voxels, elapsed, steps = 0, 0.0, 0
-    for frame, g in sc.frames():
-        sim.apply_grid(g)
-        t0 = time.perf_counter()
-        sim.step(frame)
-        elapsed += time.perf_counter() - t0
-        voxels += sim.stats().active_voxels
-        steps += 1
-        if elapsed > seconds:
-            break
-    return {"value": voxels / elapsed, "unit": "active voxel-steps/s", "cores": 1, "kind": "port",
-            "sample": f"first {steps} steps of the same scene, FP32 CPU restatement, single thread, "
-                      f"{elapsed:.1f} s"}
+
+    def run(threads, budget):
+        oracle.set_threads(threads)
+        sc = scene_fn()
+        sim = OracleSimulation.create(sc.grid(), 32)
+        voxels, elapsed, steps, bricks = 0, 0.0, 0, 0
+        for frame, g in sc.frames():
+            sim.apply_grid(g)
+            t0 = time.perf_counter()
+            sim.step(frame)
+            elapsed += time.perf_counter() - t0
+            st = sim.stats()
+            voxels += st.active_voxels
+            bricks = st.active_bricks
+            steps += 1
+            if elapsed > budget:
+                break
+        return voxels / elapsed, steps, elapsed, bricks
+
+    threads = host_threads()
+    many, steps_m, el_m, bricks_m = run(threads, seconds)
+    one, steps_1, el_1, bricks_1 = run(1, seconds * 2 / 3)
+    oracle.set_threads(threads)
+    return {"value": many, "unit": "active voxel-steps/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "single_core_value": one,
+            "sample": f"steps 0-{steps_m - 1} of the same scene (active bricks rising to {bricks_m}), FP32 CPU "
+                      f"restatement, OpenMP over bricks on {threads} threads, {el_m:.1f} s; on one thread steps "
+                      f"0-{steps_1 - 1} (to {bricks_1} bricks), {el_1:.1f} s"}
 
 
 def serving_line(args) -> dict:
@@ -222,23 +258,30 @@ def presets_lines(steps: int):
                "peak_active_bricks": peak, "grid": sc.grid()}
 
 
-def mix_comparison(count: int) -> dict:
-    """The metric's comparison, on a sample of the serving workload: `count` requests of it
-    present at once, each pressure solved to a relative residual of 1e-4 (at most 20
-    V-cycles), served batched with BF16 multigrid levels and one simulation per launch with
-    FP32 levels. `bench.py --serving` runs the whole Poisson stream."""
+def mix_comparison(count: int, rate: float) -> dict:
+    """The metric's comparison on the serving workload: `count` requests arriving as a
+    Poisson stream at `rate` per second (1-4 fires, voxel 0.025-0.1 m, brick 8, 16-8192
+    bricks, 30-300 steps, admitted by a 200 GB HBM budget), each step's pressure solved to a
+    relative residual |b - Ax| / |b| of 1e-4 (at most 20 V-cycles): served with continuous
+    batching and BF16 multigrid levels, and one simulation per launch with FP32 levels. Step
+    latencies are to device completion on both sides."""
     from firepro import serving
     solve = (1e-4, 20)
     warm = serving.requests(count, 1e9)
     for r in warm:
         r.steps = 1
-    serving.serve(warm, batched=True, pressure=("bf16",) + solve)
-    b = serving.serve(serving.requests(count, 1e9), batched=True, pressure=("bf16",) + solve).summary()
-    s = serving.serve(serving.requests(count, 1e9), batched=False, pressure=("fp32",) + solve).summary()
-    keep = ("voxel_steps_per_s", "wall_s", "rounds", "p50_step_ms", "p99_step_ms", "mean_cycles",
-            "max_relative_residual", "steps_within_tolerance")
-    return {"sample": f"{count} serving requests present at once (1-4 fires, voxel 0.025-0.1 m, brick 8, "
-                      "16-8192 bricks, 30-300 steps)",
+    serving.serve(warm, batched=True, pressure=("bf16",) + solve)  # device blocks, streams, graphs
+    say = lambda line: print(line, file=sys.stderr, flush=True)  # noqa: E731
+    b = serving.serve(serving.requests(count, rate), batched=True, pressure=("bf16",) + solve,
+                      progress=say).summary()
+    s = serving.serve(serving.requests(count, rate), batched=False, pressure=("fp32",) + solve,
+                      progress=say).summary()
+    keep = ("voxel_steps_per_s", "wall_s", "rounds", "sim_steps", "requests", "p50_step_ms", "p99_step_ms",
+            "p50_request_s", "p99_request_s", "peak_resident", "mean_cycles", "max_relative_residual",
+            "steps_within_tolerance")
+    return {"sample": f"{count} serving requests, Poisson arrivals at {rate}/s (1-4 fires, voxel 0.025-0.1 m, "
+                      "brick 8, 16-8192 bricks, 30-300 steps, 200 GB HBM budget, pressure to relative residual "
+                      "1e-4 in at most 20 V-cycles)",
             "batched_bf16": {k: b[k] for k in keep}, "one_sim_per_launch_fp32": {k: s[k] for k in keep},
             "speedup": b["voxel_steps_per_s"] / s["voxel_steps_per_s"] if s["voxel_steps_per_s"] else None}
 
@@ -280,16 +323,17 @@ def render_phase(sim, frames, step) -> dict:
     return out, solver
 
 
-def measured_traffic(cells: float):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter
-    summary (FETCH_SIZE scaled by the copy calibration, WRITE_SIZE), per active cell,
-    times this run's cells; None without the summary."""
-    path = os.path.join(ROOT, "profiles", "evolve_fields_four_fires_b8_hbm_traffic.json")
+def measured_traffic(kernel: str, cells: float):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
+    same command (FETCH_SIZE scaled by the plain-copy calibration, plus WRITE_SIZE), per
+    active cell, times this run's cells; None without a summary for the kernel."""
+    path = os.path.join(ROOT, "profiles", "four_fires_b8_hbm_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    return t["bytes_per_cell"] * cells
+    per = t.get("kernels", {}).get(kernel)
+    return per["bytes_per_cell"] * cells if per else None
 
 
 def main():
@@ -318,7 +362,8 @@ def main():
     except Exception:
         sync = lambda: None  # noqa: E731
 
-    total = args.warmup + args.steps + args.render_steps
+    first = args.state_step + args.warmup  # the first timed step
+    total = first + args.steps + args.render_steps
     scene_fn = lambda: scenes.four_fires(seed=args.seed, steps=total)  # noqa: E731
     sc = scene_fn()
     frames = [(abi.PackedFrame(f), g) for f, g in sc.frames()]
@@ -328,27 +373,32 @@ def main():
         sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
         sim.step(frame)
 
-    for frame, g in frames[: args.warmup]:
+    for frame, g in frames[:first]:  # the untimed advance to the timed state, then the warmup
         step(frame, g)
     sim.profile(2)  # events around the dominant kernel only, inside the timed region
+    timer = abi.StreamTimer(sim.stream())
     sync()
-    latencies, voxels = [], 0
+    voxels, bricks_seen = 0, []
     t0 = time.perf_counter()
-    for frame, g in frames[args.warmup:args.warmup + args.steps]:
-        s0 = time.perf_counter()
-        step(frame, g)  # returns once the step's readback has landed
-        latencies.append(time.perf_counter() - s0)
-        voxels += sim.stats().active_voxels
+    for frame, g in frames[first:first + args.steps]:
+        timer.mark()
+        step(frame, g)  # returns once the step's readback has landed; its tail stays queued
+        timer.mark()
+        st = sim.stats()
+        voxels += st.active_voxels
+        bricks_seen.append(st.active_bricks)
     sync()
     wall = time.perf_counter() - t0
+    latencies = timer.elapsed()  # per step: device time to the completion of its last kernel
+    timer.close()
     stages = sim.profile_read()
-    st = sim.stats()
-    bricks = st.active_bricks
     value = voxels / wall
-    # The dominant kernel: the stage with the largest share of the step.
+    # The dominant kernel: the stage with the largest event-timed share of the step.
     cells = voxels / args.steps  # mean active field cells per step
     best = max((k for k in stages if k in STAGE_BYTES_PER_FIELD_CELL), key=lambda k: stages[k])
-    achieved = STAGE_BYTES_PER_FIELD_CELL[best] * cells / (stages[best] * 1e-3) / 1e9
+    algorithmic = STAGE_BYTES_PER_FIELD_CELL[best] * cells
+    achieved = algorithmic / (stages[best] * 1e-3) / 1e9
+    traffic = measured_traffic(best, cells)
     line = {
         "metric": "active voxel-steps/s", "value": value, "unit": "active voxel-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -356,21 +406,32 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "four-fires plain FP32 single simulation: 4 fire spheres in 6x4x6 m, "
                                "turbulence 1/2, sphere collider r 0.5, voxel 0.025 m, brick 8, "
-                               f"steps {args.warmup}-{args.warmup + args.steps - 1} of 300",
-                   "active_bricks": bricks, "active_voxels": bricks * 512},
-        "p50_step_ms": float(np.percentile(latencies, 50) * 1e3),
-        "p99_step_ms": float(np.percentile(latencies, 99) * 1e3),
+                               f"timed steps {first}-{first + args.steps - 1} (after an untimed advance to "
+                               f"step {args.state_step} and {args.warmup} warmup steps)",
+                   "active_bricks": bricks_seen[-1], "active_bricks_min": min(bricks_seen),
+                   "active_voxels": bricks_seen[-1] * 512},
+        "p50_step_ms": float(np.percentile(latencies, 50)),
+        "p99_step_ms": float(np.percentile(latencies, 99)),
+        "step_latency": "device time per step on the simulation's stream, HIP events before and after each "
+                        "step call (enqueue to completion of its last kernel)",
         "timed_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": {"kernel": best, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(cells),
-                     "algorithmic_bytes": STAGE_BYTES_PER_FIELD_CELL[best] * cells},
+                     "traffic": traffic, "algorithmic_bytes": algorithmic,
+                     "algorithmic_bytes_per_cell": STAGE_BYTES_PER_FIELD_CELL[best],
+                     "overhead_bytes_per_cell": STAGE_OVERHEAD_PER_FIELD_CELL[best],
+                     "cells": cells, "kernel_ms": stages[best]},
     }
     if args.render_steps > 0:
-        line["render"], line["stage_ms"] = render_phase(sim, frames[args.warmup + args.steps:], step)
+        line["render"], line["stage_ms"] = render_phase(sim, frames[first + args.steps:], step)
+        # Every stage kernel's fraction of peak over its algorithmic bytes, from the event-timed
+        # steps right after the timed ones (near the same state).
+        line["stage_roofline"] = {
+            k: round(STAGE_BYTES_PER_FIELD_CELL[k] * cells / (line["stage_ms"][k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            for k in STAGE_BYTES_PER_FIELD_CELL if line["stage_ms"].get(k)}
     sim.dispose()
     if args.mix > 0:
-        line["mixed_precision_batched"] = mix_comparison(args.mix)
+        line["mixed_precision_batched"] = mix_comparison(args.mix, args.serving_rate)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lambda: scenes.four_fires(seed=args.seed), args.cpu_seconds)
     print(json.dumps(line))

@@ -2,6 +2,9 @@
 // loaded with ctypes by the tests, smoke() and bench.py's CPU baseline. Entry points mirror
 // include/firepro.h's, prefixed orc_, with the arithmetic width chosen at creation.
 #include <chrono>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <type_traits>
 #include <string>
 
@@ -35,6 +38,8 @@ struct Base {
   virtual int readRender(fp_render_tables *t) = 0;
   virtual int probe(const fp_camera &cam, const fp_render_settings &rs, int what, const float *in, float *out) = 0;
   virtual int sceneLights(const fp_render_settings &rs, const double *xyz, int count, float *sums) = 0;
+  virtual int stage(int which, double *out, int cap) = 0;
+  virtual int forceProbe(const int32_t *cells, int n, double *out) = 0;
 };
 
 template <class R> struct Impl : Base {
@@ -174,6 +179,33 @@ template <class R> struct Impl : Base {
     for (int i = 0; i < n; i++) {
       int s = sim.pageSlot(sim.entries[sim.activeList[i]] & 0xffffff);
       for (int c = 0; c < sim.nV; c++) out[(size_t)i * sim.nV + c] = (double)sim.D[(size_t)s * sim.nV + c];
+    }
+    return n;
+  }
+  // An intermediate array of the last step, per active brick in active-list order, in
+  // FP64: 0 the advected velocity, 1 the corrected velocity, 2 the forced velocity (three
+  // planes of nV faces each), 3 the curl (four channels per velocity cell), 4 the scalar
+  // predictor (four channels per field cell). For pinning tests of the stages.
+  int stage(int which, double *out, int cap) override {
+    const std::vector<R> *buf[] = {&sim.Vadv, &sim.Vcor, &sim.Vf, &sim.Curl, &sim.Fp};
+    const size_t per[] = {(size_t)sim.nV * 3, (size_t)sim.nV * 3, (size_t)sim.nV * 3, (size_t)sim.nV * 4,
+                          (size_t)sim.nF * 4};
+    if (which < 0 || which > 4) throw oracle::Error("stage must be 0 to 4.");
+    int n = (int)sim.activeList.size();
+    if (n > cap) return -1;
+    for (int i = 0; i < n; i++) {
+      size_t s = (size_t)sim.pageSlot(sim.entries[sim.activeList[i]] & 0xffffff);
+      for (size_t c = 0; c < per[which]; c++) out[i * per[which] + c] = (double)(*buf[which])[s * per[which] + c];
+    }
+    return n;
+  }
+  // centerForce (fluid-forces.wgsl:79-104) at velocity cells (x, y, z triples) on the last
+  // step's fields and curl: confinement, external forces and buoyancy, in FP64.
+  int forceProbe(const int32_t *cells, int n, double *out) override {
+    for (int i = 0; i < n; i++) {
+      R f[3];
+      sim.centerForce(cells + 3 * i, f);
+      for (int a = 0; a < 3; a++) out[3 * i + a] = (double)f[a];
     }
     return n;
   }
@@ -319,6 +351,18 @@ template <class F> int guard(F fn) {
 extern "C" {
 const char *orc_last_error(void) { return g_error.c_str(); }
 
+// Host threads for the per-brick passes and the image rows (OpenMP); results do not depend
+// on the count. Returns the count in effect.
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 int orc_create(const fp_grid *grid, int bits, void **out) {
   return guard([&] {
     if (bits == 64) *out = new Impl<double>(*grid);
@@ -370,6 +414,12 @@ int orc_tile_tables(void *h, int32_t *records, int32_t *hash, int cap, int hashC
   return ((Base *)h)->tileTables(records, hash, cap, hashCap);
 }
 int orc_brick_coords(void *h, const int32_t *ids, int n, int32_t *out) { return ((Base *)h)->brickCoords(ids, n, out); }
+int orc_stage(void *h, int which, double *out, int cap) {
+  int r = 0;
+  int rc = guard([&] { r = ((Base *)h)->stage(which, out, cap); });
+  return rc != FP_OK ? rc : r;
+}
+int orc_force_probe(void *h, const int32_t *cells, int n, double *out) { return ((Base *)h)->forceProbe(cells, n, out); }
 int orc_rhs(void *h, double *out, int cap) { return ((Base *)h)->rhs(out, cap); }
 int orc_set_pressure(void *h, int mode, double tol, int max_cycles) {
   return guard([&] { ((Base *)h)->setPressure(mode, tol, max_cycles); });

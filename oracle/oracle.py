@@ -73,6 +73,9 @@ def load():
         "orc_emitter_weight": (C.c_double, [P(C.c_double), P(C.c_double), C.c_int, P(C.c_double)]),
         "orc_burst_perlin": (C.c_double, [C.c_uint32, C.c_double, C.c_double, C.c_double]),
         "orc_time_steps": (C.c_double, [v, P(abi.Frame), C.c_double, C.c_int]),
+        "orc_set_threads": (C.c_int, [C.c_int]),
+        "orc_stage": (C.c_int, [v, C.c_int, P(C.c_double), C.c_int]),
+        "orc_force_probe": (C.c_int, [v, P(C.c_int32), C.c_int, P(C.c_double)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -269,6 +272,31 @@ class OracleSimulation:
             raise RuntimeError(f"probe {what} failed")
         return np.array(out[:], np.float32)
 
+    STAGES = {"advected_velocity": (0, 3, "v"), "corrected_velocity": (1, 3, "v"),
+              "forced_velocity": (2, 3, "v"), "curl": (3, 4, "v"), "predictor": (4, 4, "f")}
+
+    def stage(self, name: str) -> np.ndarray:
+        """An intermediate array of the last step per active brick (FP64): velocities as
+        [brick, 3, nV], the curl as [brick, nV, 4], the scalar predictor as [brick, nF, 4]."""
+        which, comps, grid = self.STAGES[name]
+        st = self.stats()
+        n = st.active_bricks
+        cells = self.velocity_cells if grid == "v" else self.brick_cells
+        out = np.zeros(max(n, 1) * comps * cells, np.float64)
+        got = self._lib.orc_stage(self._h, which, out.ctypes.data_as(C.POINTER(C.c_double)), max(n, 1))
+        _check(0 if got >= 0 else got, self._lib)
+        out = out[: got * comps * cells]
+        return out.reshape(got, 3, cells) if comps == 3 else out.reshape(got, cells, comps)
+
+    def force_probe(self, cells) -> np.ndarray:
+        """The cell-centred force (confinement, external forces, buoyancy) at velocity cells
+        `cells` (integer x, y, z rows) on the last step's fields and curl (FP64)."""
+        c = np.ascontiguousarray(np.asarray(cells, np.int32).reshape(-1, 3))
+        out = np.zeros((len(c), 3), np.float64)
+        self._lib.orc_force_probe(self._h, c.ctypes.data_as(C.POINTER(C.c_int32)), len(c),
+                                  out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
     def brick_coords(self, ids) -> np.ndarray:
         ids = np.ascontiguousarray(ids, np.int32)
         out = np.zeros((len(ids), 3), np.int32)
@@ -323,3 +351,9 @@ def blackbody() -> np.ndarray:
 def round16(mode: int, x: float) -> float:
     """The multigrid storage rounding of the restatement: 2 bfloat16, 3 binary16."""
     return load().orc_round16(mode, float(x))
+
+
+def set_threads(n: int = 0) -> int:
+    """Host threads for the restatement's per-brick passes and image rows (0 keeps the
+    current count); returns the count in effect. Results do not depend on it."""
+    return load().orc_set_threads(int(n))

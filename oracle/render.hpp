@@ -1108,9 +1108,13 @@ struct Renderer {
       sw = std::max(1, (P.width + 1) / 2);
       sh = std::max(1, (P.height + 1) / 2);
       slices.assign((size_t)sw * sh * SLICES * 4, 0.0f);
+      // Rows are independent (each texel and pixel is computed alone): the host's threads
+      // share them (OpenMP), with the same result at any thread count.
+#pragma omp parallel for schedule(dynamic, 1)
       for (int y = 0; y < sh; y++)
         for (int x = 0; x < sw; x++) slice(x, y, sw, sh, &slices[((size_t)y * sw + x) * 4]);
     }
+#pragma omp parallel for schedule(dynamic, 1)
     for (int y = 0; y < P.height; y++)
       for (int x = 0; x < P.width; x++)
         pixel(x, y, &rgba[((size_t)y * P.width + x) * 4], P.halfRes ? slices.data() : nullptr, sw, sh);

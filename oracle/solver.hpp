@@ -1368,8 +1368,14 @@ template <class R> class Sim {
     }
   }
 
+  // The per-cell passes below run their bricks on the host's threads (OpenMP): a pass
+  // writes only its own cells and its own brick's flags, so results do not depend on the
+  // thread count.
   template <class Fn> void forActiveVelocityCells(Fn fn) {
-    for (int id : activeList) {
+    const long count = (long)activeList.size();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long i = 0; i < count; i++) {
+      const int id = activeList[(size_t)i];
       int bk[3];
       brickOf(id, bk);
       int s = pageSlot(entries[id] & 0xffffff);
@@ -1380,7 +1386,10 @@ template <class R> class Sim {
     }
   }
   template <class Fn> void forActiveFieldCells(Fn fn) {
-    for (int id : activeList) {
+    const long count = (long)activeList.size();
+#pragma omp parallel for schedule(dynamic, 2)
+    for (long i = 0; i < count; i++) {
+      const int id = activeList[(size_t)i];
       int bk[3];
       brickOf(id, bk);
       int s = pageSlot(entries[id] & 0xffffff);
@@ -1785,7 +1794,10 @@ template <class R> class Sim {
     for (int a = 0; a < 3; a++) w[a] = ((R)id[a] + R(0.5)) * P.hs;
   }
   template <class Fn> void forActiveSmokeCells(Fn fn) {
-    for (int id : activeList) {
+    const long count = (long)activeList.size();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long i = 0; i < count; i++) {
+      const int id = activeList[(size_t)i];
       int bk[3];
       brickOf(id, bk);
       int s = pageSlot(entries[id] & 0xffffff);
@@ -2231,11 +2243,15 @@ template <class R> class Sim {
   }
 
   // The cells a level's sweep visits: every cell of every slotted brick on the brick
-  // levels, of every live or closing tile on the tile levels.
-  template <class Fn> void forLevelCells(int L, Fn fn) {
+  // levels, of every live or closing tile on the tile levels. With `parallel` the bricks
+  // (tiles) run on the host's threads; a pass that sums over cells passes false.
+  template <class Fn> void forLevelCells(int L, Fn fn, bool parallel = true) {
     int sd = levels[L].side;
     if (L < brickLevels) {
-      for (int id : activeList) {
+      const long count = (long)activeList.size();
+#pragma omp parallel for schedule(dynamic, 8) if (parallel)
+      for (long i = 0; i < count; i++) {
+        const int id = activeList[(size_t)i];
         int bk[3];
         brickOf(id, bk);
         for (int l = 0; l < sd * sd * sd; l++) {
@@ -2245,6 +2261,7 @@ template <class R> class Sim {
       }
       return;
     }
+#pragma omp parallel for schedule(dynamic, 8) if (parallel)
     for (int t = 0; t < capacity; t++) {
       const int32_t *r = &tables.records[(size_t)t * TILE_RECORD_WORDS];
       if (r[3] == TILE_FREE) continue;
@@ -2546,7 +2563,7 @@ template <class R> class Sim {
           big = fmax(big, fabs_(v));
         }
         r[a] = v;
-      });
+      }, false);
       lastResidual = bb > 0 ? std::sqrt(rr / bb) : 0.0;
       if (lastResidual <= pressureTol || it == pressureMaxCycles) break;
       R scale = 1;

@@ -276,3 +276,42 @@ def test_scene_lights_bit_exact():
             assert a[:, 0].max() > 0, "flames are lit"
         finally:
             gpu.dispose()
+
+
+def test_device_render_then_write_state_waits():
+    """fp_sim_render_device returns with its kernels queued. An entry point that replaces
+    the fields they read (fp_sim_write_state) waits for them first, so the render field the
+    next render reuses (same step: no new smoothing) is the one made from the old fields:
+    the sequence device render, write, render equals host render, write, render."""
+    from firepro import scenes
+    from firepro.engine import FluidSimulation
+    abi = _abi()
+    sc = scenes.campfire(steps=30)
+    cam = abi.camera((-4.12, 1.96, 5.30), (0, 1.25, 0), 44, 192, 160)
+    rs = abi.render_settings(half_resolution=False)
+    out = []
+    for device in (True, False):
+        sim = FluidSimulation.create(sc.grid())
+        buf = abi.DeviceBuffer(cam.width * cam.height * 16)
+        try:
+            for frame, g in sc.frames():
+                sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+                sim.step(frame)
+            state = sim.read_state()
+            if device:
+                sim.render_device(cam, rs, buf.ptr.value)
+            else:
+                first = sim.render(cam, rs)
+            print(f"case device render then write_state, device={device}", flush=True)
+            sim.write_state(np.ascontiguousarray(state["fields"][::-1]), None)
+            second = sim.render(cam, rs)
+            if device:
+                first = np.frombuffer(buf.read(), np.float32).reshape(cam.height, cam.width, 4).copy()
+            out.append((first, second))
+        finally:
+            buf.close()
+            sim.dispose()
+    (a1, a2), (b1, b2) = out
+    assert np.array_equal(a1, b1)
+    assert np.array_equal(a2, b2)
+    assert a1[..., 3].max() > 0.1, "the fire is in view"

@@ -396,7 +396,9 @@ struct fp_sim {
   int64_t steps = 0;
   bool mayHaveFlame = false, solidsStale = true, geometryDirty = true;
   hipEvent_t readbackEvent = nullptr;  // recorded after each step's activity readback
-  bool stepPending = false;            // the last step's tail may still run on the stream
+  // Work an entry point left queued on the stream when it returned (a step's tail after its
+  // readback, a render into a caller's device buffer); drain() waits for it.
+  bool tailPending = false;
   int colliderCount = 0;
   std::vector<uint32_t> prevColliders, prevMasks;
   std::vector<uint64_t> sourceIds;
@@ -1409,7 +1411,7 @@ struct fp_sim {
       // The step returns once its readback has landed; the rest stays queued on the stream
       // (drain() before anything reads or changes device state).
       HIPCHECK(hipEventSynchronize(readbackEvent));
-      stepPending = true;
+      tailPending = true;
     }
     hostMark("host_wait", w0);
     HIPCHECK(hipGetLastError());
@@ -1840,6 +1842,10 @@ struct fp_sim {
     if (!deviceOut) {
       HIPCHECK(hipMemcpyAsync(out, rc.image.p, pixels * sizeof(float4), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
+    } else {
+      // The render kernels stay queued and still read the pools; the next entry point that
+      // reads or replaces device state waits for them.
+      tailPending = true;
     }
     if (profiling) {
       HIPCHECK(hipStreamSynchronize(stream));
@@ -1959,16 +1965,16 @@ struct fp_sim {
     return total;
   }
 
-  // Finishes the queued remainder of the last step.
+  // Finishes whatever the last entry point left queued (a step's tail, a device render).
   void drain() {
-    if (!stepPending) return;
-    stepPending = false;
+    if (!tailPending) return;
+    tailPending = false;
     HIPCHECK(hipStreamSynchronize(stream));
   }
 
   void destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
-    stepPending = false;
+    tailPending = false;
     if (readbackEvent) (void)hipEventDestroy(readbackEvent);
     readbackEvent = nullptr;
     DBuf<int> *ints[] = {&tileRec, &hash, &pages, &entries, &records, &active, &zeroList, &poolState, &counts, &createdSlots, &readback};

@@ -193,20 +193,78 @@ def render_settings(**overrides) -> RenderSettings:
     return rs
 
 
+_hip_runtime = None
+
+
+def hip_runtime():
+    """The HIP runtime libfirepro.so runs on, through ctypes (device buffers, events)."""
+    global _hip_runtime
+    if _hip_runtime is not None:
+        return _hip_runtime
+    load()
+    with open("/proc/self/maps") as f:
+        paths = sorted({line.split()[-1] for line in f if "libamdhip64" in line})
+    if not paths:
+        raise FireProError("libfirepro.so has no HIP runtime loaded")
+    hip = C.CDLL(paths[0])
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipEventDestroy.argtypes = [C.c_void_p]
+    hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    _hip_runtime = hip
+    return hip
+
+
+class StreamTimer:
+    """Device intervals on one stream: mark() records a HIP event there; elapsed() waits for
+    the events and returns the milliseconds between consecutive pairs (begin, end) of
+    marks, each the device time from the work before `begin` finishing to the work before
+    `end` finishing. Events are reused across rounds."""
+
+    def __init__(self, stream: int):
+        self._hip = hip_runtime()
+        self.stream = C.c_void_p(stream)
+        self._free, self._marks = [], []
+
+    def mark(self) -> None:
+        if self._free:
+            e = self._free.pop()
+        else:
+            e = C.c_void_p()
+            if self._hip.hipEventCreate(C.byref(e)) != 0:
+                raise FireProError("hipEventCreate failed")
+        if self._hip.hipEventRecord(e, self.stream) != 0:
+            raise FireProError("hipEventRecord failed")
+        self._marks.append(e)
+
+    def elapsed(self) -> list:
+        out, ms = [], C.c_float()
+        if self._marks and self._hip.hipEventSynchronize(self._marks[-1]) != 0:
+            raise FireProError("hipEventSynchronize failed")
+        for a, b in zip(self._marks[0::2], self._marks[1::2]):
+            if self._hip.hipEventElapsedTime(C.byref(ms), a, b) != 0:
+                raise FireProError("hipEventElapsedTime failed")
+            out.append(float(ms.value))
+        self._free.extend(self._marks)
+        self._marks = []
+        return out
+
+    def close(self) -> None:
+        for e in self._free + self._marks:
+            self._hip.hipEventDestroy(e)
+        self._free, self._marks = [], []
+
+
 class DeviceBuffer:
     """Device memory from the HIP runtime libfirepro.so uses (for device-output entry
     points and probes), released on close()."""
 
     def __init__(self, nbytes: int):
-        load()
-        with open("/proc/self/maps") as f:
-            paths = sorted({line.split()[-1] for line in f if "libamdhip64" in line})
-        if not paths:
-            raise FireProError("libfirepro.so has no HIP runtime loaded")
-        self._hip = hip = C.CDLL(paths[0])
-        hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        hip.hipFree.argtypes = [C.c_void_p]
-        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self._hip = hip = hip_runtime()
         self.ptr = C.c_void_p()
         self.nbytes = int(nbytes)
         if hip.hipMalloc(C.byref(self.ptr), self.nbytes) != 0:
@@ -215,6 +273,15 @@ class DeviceBuffer:
     def copy_from(self, other: "DeviceBuffer", nbytes: int) -> None:
         if self._hip.hipMemcpy(self.ptr, other.ptr, int(nbytes), 3) != 0:  # device to device
             raise FireProError("hipMemcpy failed")
+
+    def read(self, nbytes: Optional[int] = None) -> bytes:
+        """The first `nbytes` (default all) as host bytes, after the device has finished."""
+        self.synchronize()
+        n = self.nbytes if nbytes is None else int(nbytes)
+        out = C.create_string_buffer(n)
+        if self._hip.hipMemcpy(out, self.ptr, n, 2) != 0:  # device to host
+            raise FireProError("hipMemcpy failed")
+        return out.raw
 
     def synchronize(self) -> None:
         if self._hip.hipDeviceSynchronize() != 0:

@@ -57,6 +57,7 @@ class Request:
     memory: int
     # filled while serving
     sim: Optional[FluidSimulation] = None
+    timer: Optional[abi.StreamTimer] = None  # one simulation per launch: its steps' device times
     done: int = 0
     admitted: float = 0.0
     finished: float = 0.0
@@ -155,6 +156,8 @@ def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, 
             while pending and pending[0].arrival <= now and reserved + pending[0].memory <= hbm_budget:
                 r = pending.pop(0)
                 r.sim = FluidSimulation.create(r.grid)
+                if not batched:
+                    r.timer = abi.StreamTimer(r.sim.stream())
                 g = r.settings
                 r.sim.cutoff, r.sim.voxel_budget, r.sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
                 if pressure is not None:
@@ -178,10 +181,13 @@ def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, 
                 for k, v in batch.phases.items():
                     rep.phases[k] = rep.phases.get(k, 0.0) + v
             else:
+                # Each step's latency is device time on the simulation's own stream, from the
+                # call to the completion of its last kernel (the call itself returns once the
+                # step's readback lands, with the tail still queued).
                 for r in resident:
-                    a = time.perf_counter()
+                    r.timer.mark()
                     r.sim.step(r.frame, dt)
-                    rep.step_ms.append((time.perf_counter() - a) * 1e3)
+                    r.timer.mark()
             rep.busy_s += time.perf_counter() - s0
             rep.rounds += 1
             c0 = time.perf_counter()
@@ -197,6 +203,10 @@ def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, 
                 rep.sim_steps += 1
                 r.done += 1
                 if r.done >= r.steps:
+                    if r.timer is not None:
+                        rep.step_ms.extend(r.timer.elapsed())
+                        r.timer.close()
+                        r.timer = None
                     r.finished = time.perf_counter() - t0
                     rep.request_s.append(r.finished - r.arrival)
                     rep.requests += 1
@@ -214,6 +224,9 @@ def serve(reqs: List[Request], batched: bool = True, hbm_budget: float = 200e9, 
         rep.wall_s = time.perf_counter() - t0
     finally:
         for r in resident:
+            if r.timer is not None:
+                r.timer.close()
+                r.timer = None
             if r.sim is not None:
                 r.sim.dispose()
         if batch is not None:
