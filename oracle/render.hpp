@@ -440,7 +440,7 @@ struct Renderer {
     for (int c = 0; c < 4; c++) out[c] = lut[(size_t)i0 * 4 + c] * (1.0f - fr) + lut[(size_t)i1 * 4 + c] * fr;
   }
 
-  // fireProEmission (fire-emission.wgsl:26-56) with smoke = (albedo, soot glow).
+  // fireProEmission (fire-emission.wgsl:22-56) with smoke = (albedo, soot glow).
   void emission(const Vec4 &f, float out[3]) const {
     out[0] = out[1] = out[2] = 0.0f;
     if (P.optical[1] <= 0.0f) return;
@@ -464,7 +464,7 @@ struct Renderer {
     }
   }
 
-  // exitDistance (lighting.wgsl:240-249).
+  // exitDistance (lighting.wgsl:258-266).
   float exitDistance(const RBox &bx, const float p[3], const float d[3]) const {
     float best = 1e30f;
     float dist[3];
@@ -517,7 +517,7 @@ struct Renderer {
     return {glow[0], glow[1], glow[2], fexp(-depth)};
   }
 
-  // integratedTransmission (volume-common.wgsl:59-65).
+  // integratedTransmission (volume-common.wgsl:55-61).
   static float integratedTransmission(float ext, float distance, float att) {
     float depth = ext * distance;
     if (depth < 0.001f) return distance * (1.0f - 0.5f * depth + depth * depth / 6.0f);
@@ -831,7 +831,7 @@ struct Renderer {
       }
     }
   }
-  // fireProDirect (volume-march.wgsl:141-146).
+  // fireProDirect (volume-march.wgsl:138-143).
   void directLight(const float d[3], float direct[3]) const {
     float ln[3];
     normalize3(P.light, ln);

@@ -59,7 +59,7 @@ inline int ilog2(int v) {
   return s;
 }
 
-// Packed scene records (fluid-common.wgsl:34-51, force-fields.wgsl:266-272).
+// Packed scene records (fluid-common.wgsl:34-51, force-fields.wgsl:4-26).
 template <class R> struct Emitter {
   R pos[3], size[3];
   bool box;
@@ -111,7 +111,7 @@ template <class R> struct Params {
   R knots[3][8][2];
 };
 
-// Sum of a piecewise-linear ramp's area from lo to hi (flame-common.wgsl:170-184).
+// Sum of a piecewise-linear ramp's area from lo to hi (flame-common.wgsl:14-28).
 template <class R> R rampIntegral(const Params<R> &p, int ch, R lo, R hi) {
   R area = 0;
   for (int i = 1; i < p.knotCount[ch]; i++) {
@@ -127,7 +127,7 @@ template <class R> R rampIntegral(const Params<R> &p, int ch, R lo, R hi) {
   return area;
 }
 
-// evaluateFlame (flame-common.wgsl:186-198): lifetime after the step, and smoke, heat and
+// evaluateFlame (flame-common.wgsl:30-42): lifetime after the step, and smoke, heat and
 // expansion produced over it.
 template <class R> void evaluateFlame(const Params<R> &p, R before, R &after, R prod[3]) {
   if (before <= R(0)) {
@@ -141,7 +141,7 @@ template <class R> void evaluateFlame(const Params<R> &p, R before, R &after, R 
   prod[2] = p.fexpansion * (p.lifespan * rampIntegral(p, 2, after, before) / p.fdt);
 }
 
-// emitterWeight (fluid-sources.wgsl:98-111).
+// emitterWeight (fluid-sources.wgsl:24-38).
 template <class R> R emitterWeight(const Emitter<R> &e, const R w[3]) {
   // Past the padded size on any axis |d| >= 1 there, so r >= 1: no weight.
   for (int a = 0; a < 3; a++)
@@ -153,7 +153,7 @@ template <class R> R emitterWeight(const Emitter<R> &e, const R w[3]) {
   return R(1) - smoothstepv(R(0.5), R(1), r);
 }
 
-// burstGradient / burstPerlin / burstPattern / burstVariation (fluid-sources.wgsl:115-156).
+// burstGradient / burstPerlin / burstPattern / burstVariation (fluid-sources.wgsl:40-82).
 template <class R> R burstGradient(uint32_t seedBits, int cx, int cy, int cz, R ox, R oy, R oz) {
   uint32_t h = (uint32_t)cx * 374761393u + (uint32_t)cy * 668265263u + (uint32_t)cz * 2246822519u +
                seedBits * 3266489917u;
@@ -595,7 +595,7 @@ template <class R> class Sim {
   }
   bool computes(int bx, int by, int bz) const { return entryAt(bx, by, bz) >= 0; }
 
-  // Whether field cell `c` is solid (fluid-common.wgsl:129-145).
+  // Whether field cell `c` is solid (solidCell, solids.wgsl:15-32).
   bool solidCell(int x, int y, int z) const {
     if (P.ground && y < 0) return true;
     if (P.colliders == 0) return false;
@@ -606,7 +606,7 @@ template <class R> class Sim {
     size_t words = (size_t)nF / 32;
     return ((solids[(size_t)s * words + (bit >> 5)] >> (bit & 31u)) & 1u) != 0u;
   }
-  // fieldCell + solidAt (fluid-common.wgsl:123-150): the quarter-cell bias.
+  // fieldCell + solidAt (solids.wgsl:9-36): the quarter-cell bias.
   bool solidAt(const R w[3]) const {
     int c[3];
     for (int a = 0; a < 3; a++) c[a] = (int)ffloor(w[a] * P.invHf + R(0.25));
@@ -1189,7 +1189,7 @@ template <class R> class Sim {
     readActivity();
   }
 
-  // ---- allocation (fluid-allocation.wgsl:343-602) ------------------------------------
+  // ---- allocation (fluid-allocation.wgsl:129-387) ------------------------------------
   std::vector<uint32_t> st;
   bool sourceNear(const int bk[3]) const {
     R size = (R)(float)(1 << sF) * P.hf;
@@ -1218,7 +1218,7 @@ template <class R> class Sim {
     st.assign(n, 0u);
     activeList.clear();
     zeroList.clear();
-    // buildBricks (:343-390)
+    // buildBricks (:129-175)
     for (int index = 0; index < n; index++) {
       uint32_t state = 0;
       int ts = tileState(index >> 6);
@@ -1244,7 +1244,7 @@ template <class R> class Sim {
       }
       st[index] = state;
     }
-    // freeBricks (:394-410): pushed in brick order above the stack.
+    // freeBricks (:179-195): pushed in brick order above the stack.
     int freed = 0;
     for (int index = 0; index < n; index++) {
       if (!(st[index] & BRICK_FREES)) continue;
@@ -1254,7 +1254,7 @@ template <class R> class Sim {
       if (st[index] & 4u) zeroList.push_back({index, (uint32_t)page});
       pages[index] = -1;
     }
-    // allocateBricks (:415-445)
+    // allocateBricks (:200-230)
     int held = heldSlots();
     int requests = 0;
     for (int index = 0; index < n; index++) {
@@ -1277,7 +1277,7 @@ template <class R> class Sim {
       uint32_t flags = activity[index];
       activity[index] = (flags & EXPANDS) ? MAY_EXPAND : 0u;
     }
-    // linkBricks (:449-483)
+    // linkBricks (:234-268)
     for (int index = 0; index < n; index++) {
       if (entries[index] < 0) continue;
       int bk[3];
@@ -1295,12 +1295,12 @@ template <class R> class Sim {
       }
       w[PRESSURE_FLAG] = unknowns;
     }
-    // finishBricks (:567-602)
+    // finishBricks (:352-387)
     int freeAfter = freeCount + freed;
     int granted = std::min(requests, std::max(freeAfter - held, 0));
     freeCount = freeAfter - granted;
     failures += (uint32_t)(requests - granted);
-    // buildSolids (:485-556) for fresh slots, or every slot after the colliders changed.
+    // buildSolids (:292-341) for fresh slots, or every slot after the colliders changed.
     if (haveColliders || solidsStale) buildSolids();
     solidsStale = false;
   }
@@ -1354,7 +1354,7 @@ template <class R> class Sim {
       if (z.second & FRESH_SLOT) drawSolids(z.first, (int)(z.second & 0xffffffu));
   }
 
-  // zeroVelocityBricks / zeroFieldBricks (fluid-zeroing.wgsl:139-169): a fresh slot is
+  // zeroVelocityBricks / zeroFieldBricks (fluid-zeroing.wgsl:76-105): a fresh slot is
   // still, empty air. A freed slot not granted again holds no brick, so nothing reads it.
   void zeroFresh() {
     for (auto &z : zeroList) {
@@ -1404,7 +1404,7 @@ template <class R> class Sim {
   void centerV(const int id[3], R w[3]) const {
     for (int a = 0; a < 3; a++) w[a] = ((R)id[a] + R(0.5)) * P.hv;
   }
-  // advectVelocityComponent (:128-141) → value and departure base cell.
+  // advectVelocityComponent (fluid-velocity-transport.wgsl:28-41) → value and departure base cell.
   R advectComponent(const int id[3], const R pos[3], int c, const R orig[3], int donor[3]) const {
     R off[3] = {0, 0, 0};
     off[c] = R(0.5) * P.hv;
@@ -1453,7 +1453,7 @@ template <class R> class Sim {
           hi = fmax(hi, v);
         }
   }
-  // correctVelocityComponent (:214-255).
+  // correctVelocityComponent (fluid-velocity-transport.wgsl:114-155).
   R correctComponent(const int id[3], const R pos[3], int c, const R orig[3], R predicted) const {
     R off[3] = {0, 0, 0};
     off[c] = R(0.5) * P.hv;
@@ -1828,7 +1828,7 @@ template <class R> class Sim {
   }
   // The box average of a field pool over the `ratio`^3 fine cells under cell `id` of a grid
   // `ratio` fine cells wide: one trilinear sample at the shared corner of each 2^3 block
-  // (velocityCellAverage, fluid-sampling.wgsl:270-290; smokeProduction, smoke.wgsl:24-37).
+  // (velocityCellAverage, fluid-sampling.wgsl:270-286; smokeProduction, smoke.wgsl:24-37).
   void boxAverage4(const std::vector<R> &buf, int ratio, const int id[3], R out[4]) const {
     if (ratio == 1) {
       fieldCell(buf, id[0], id[1], id[2], out);
@@ -1966,7 +1966,7 @@ template <class R> class Sim {
                                          (a == 0 ? c0 : a == 1 ? c1 : c2)[1] * s[1] +
                                          (a == 0 ? c0 : a == 1 ? c1 : c2)[2] * s[2];
   }
-  // coverage + externalForces (force-fields.wgsl:312-365).
+  // coverage + externalForces (force-fields.wgsl:50-103).
   void externalForces(const int cell[3], const R p[3], R out[3]) const {
     out[0] = out[1] = out[2] = 0;
     for (const Force<R> &f : forces) {
@@ -2018,7 +2018,7 @@ template <class R> class Sim {
       for (int a = 0; a < 3; a++) out[a] += acc[a] * weight;
     }
   }
-  // velocityCellAverage (fluid-sampling.wgsl:420-436) of the field grid under cell `id`.
+  // velocityCellAverage (fluid-sampling.wgsl:270-286) of the field grid under cell `id`.
   void velocityCellAverage4(const int id[3], R out[4]) const { boxAverage4(F, d, id, out); }
   R expansionAverage(const int id[3]) const {
     if (d == 1) return fieldScalar(E, id[0], id[1], id[2]);

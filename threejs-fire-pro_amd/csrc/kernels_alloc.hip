@@ -1,5 +1,5 @@
 // Brick activation, deterministic slot allocation and stream compaction
-// (fluid-allocation.wgsl:343-602). Slots are freed and granted in brick-id order: a block
+// (fluid-allocation.wgsl:129-387). Slots are freed and granted in brick-id order: a block
 // scan ranks each brick's free, request and keep within its 256-brick block, one workgroup
 // scans the block totals, and the active list is compacted in brick-id order from those
 // ranks, so every table is reproducible run to run.
@@ -55,7 +55,7 @@ __device__ __forceinline__ void blockRanks3(const bool f[3], uint32_t rank[3], u
   }
 }
 
-// buildBricks (:343-390): each brick's state; per-block totals of frees, requests, keeps.
+// buildBricks (:129-175): each brick's state; per-block totals of frees, requests, keeps.
 __device__ __forceinline__ void build_bricks_body(const AllocArgs &a, const int vb) {
   __shared__ uint32_t lds[12];
   const int index = vb * RANK_BLOCK + threadIdx.x;
@@ -108,7 +108,7 @@ FP_KERNEL_PAIR(build_bricks, 256, AllocArgs)
 
 
 // One workgroup: exclusive prefix of the block totals, then the step's commit
-// (finishBricks, :567-602): granted = min(requests, max(free + freed - held, 0)).
+// (finishBricks, :352-387): granted = min(requests, max(free + freed - held, 0)).
 __device__ __forceinline__ void scan_blocks_body(const AllocArgs &a, const int vb) {
   __shared__ uint32_t part[3][1024];
   __shared__ uint32_t carry[3];
@@ -152,7 +152,7 @@ __device__ __forceinline__ void scan_blocks_body(const AllocArgs &a, const int v
 FP_KERNEL_PAIR(scan_blocks, 1024, AllocArgs)
 
 
-// freeBricks (:394-410): freed slots go onto the stack in brick order.
+// freeBricks (:179-195): freed slots go onto the stack in brick order.
 __device__ __forceinline__ void free_bricks_body(const AllocArgs &a, const int vb) {
   const int index = vb * RANK_BLOCK + threadIdx.x;
   if (index >= a.brickCount) return;
@@ -171,7 +171,7 @@ __device__ __forceinline__ void free_bricks_body(const AllocArgs &a, const int v
 FP_KERNEL_PAIR(free_bricks, 256, AllocArgs)
 
 
-// allocateBricks (:415-445): grants from the stack top in brick order, entries, the
+// allocateBricks (:200-230): grants from the stack top in brick order, entries, the
 // compacted active list and the activity reset.
 __device__ __forceinline__ void allocate_bricks_body(const AllocArgs &a, const int vb) {
   const int index = vb * RANK_BLOCK + threadIdx.x;
@@ -201,7 +201,7 @@ __device__ __forceinline__ void allocate_bricks_body(const AllocArgs &a, const i
 FP_KERNEL_PAIR(allocate_bricks, 256, AllocArgs)
 
 
-// linkBricks (:449-483): each active brick's 27 neighbor entries and the pressure flag.
+// linkBricks (:234-268): each active brick's 27 neighbor entries and the pressure flag.
 __device__ __forceinline__ void link_bricks_body(const AllocArgs &a, const int vb) {
   const int i = vb * 64 + threadIdx.x;
   if (i >= a.counts[0]) return;
@@ -229,7 +229,7 @@ __device__ __forceinline__ void link_bricks_body(const AllocArgs &a, const int v
 FP_KERNEL_PAIR(link_bricks, 64, AllocArgs)
 
 
-// Colliders into the solid bits of one brick (buildSolids, :485-556): a workgroup per
+// Colliders into the solid bits of one brick (buildSolids, :292-341): a workgroup per
 // brick, a word of 32 cells per lane and pass.
 __device__ __forceinline__ bool insideCollider(const Collider &c, const uint32_t *masks, float hf, int x, int y, int z) {
   if (c.box == 2) {
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(64) k_fresh_solids(AllocArgs a) {
 
 
 // A fresh slot is still, empty air (zeroVelocityBricks / zeroFieldBricks,
-// fluid-zeroing.wgsl:139-169): its fields, expansion and velocity are cleared.
+// fluid-zeroing.wgsl:76-105): its fields, expansion and velocity are cleared.
 __device__ __forceinline__ void zero_fresh_body(const AllocArgs &a, const int vb, int chunks) {
   const Pools &p = a.p;
   const int i = vb / chunks, chunk = vb % chunks;

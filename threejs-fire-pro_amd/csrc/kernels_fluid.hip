@@ -36,7 +36,7 @@ __device__ __forceinline__ void fieldCellCoords(const Ctx &c, int l, int out[3])
 
 // ---- velocity transport -------------------------------------------------------------
 
-// advectVelocityComponent (fluid-velocity-transport.wgsl:128-141). Also returns the face
+// advectVelocityComponent (fluid-velocity-transport.wgsl:28-41). Also returns the face
 // velocity `fv` and the donor bounds [lo, hi] of the departure cell, which the corrector
 // would otherwise recompute from the same backtrace.
 __device__ __forceinline__ float advectComponent(const Ctx &c, const Params &P, const float *V, const int id[3],
@@ -81,7 +81,7 @@ __device__ __forceinline__ void advect_velocity_body(const StepArgs &s, const in
 FP_KERNEL_PAIR(advect_velocity, 256, StepArgs)
 
 
-// correctVelocityComponent (fluid-velocity-transport.wgsl:214-255), with the face velocity
+// correctVelocityComponent (fluid-velocity-transport.wgsl:114-156), with the face velocity
 // and donor bounds that advection left in the backtrace planes.
 __device__ __forceinline__ float correctComponent(const Ctx &c, const Params &P, const Pools &p, const float pos[3],
                                   int comp, const float orig[3], float predicted, const float fv[3], float lo,
@@ -404,7 +404,7 @@ __device__ __forceinline__ uint32_t contentSupport(const Layout &L, const Params
   return supportOn(L.B, P.hv / P.hf, P.invHf, P.dt, id, vel);
 }
 
-// evolveFields (fluid-field-evolution.wgsl:208-283) with content recording.
+// evolveFields (fluid-field-evolution.wgsl:9-84) with content recording.
 __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int vb, int chunks) {
   BRICK_PREAMBLE(chunks)
   __shared__ uint32_t busy, expands;
@@ -590,7 +590,7 @@ __device__ __forceinline__ void curlNoise(const float4 *noise, const float p[3],
   out[2] = c2[0] * s0 + c2[1] * s1 + c2[2] * s2;
 }
 
-// coverage + externalForces (force-fields.wgsl:312-365).
+// coverage + externalForces (force-fields.wgsl:50-103).
 // meshCovers (force-fields.wgsl:27-47): whether force `index`'s mesh targets cover velocity
 // cell `cell`.
 __device__ __forceinline__ bool meshCovers(const StepArgs &s, int index, const int cell[3]) {
@@ -664,7 +664,7 @@ __device__ __forceinline__ void externalForces(const StepArgs &s, const int cell
 
 // The box average of the field cells under cell `id` of a grid `d` field cells wide: one
 // trilinear sample at the shared corner of each 2^3 block (velocityCellAverage,
-// fluid-sampling.wgsl:270-290; smokeProduction, smoke.wgsl:24-37).
+// fluid-sampling.wgsl:270-286; smokeProduction, smoke.wgsl:24-37).
 __device__ __forceinline__ float4 velocityCellAverage(const Ctx &c, const float4 *F, int d, const int id[3]) {
   if (d == 1) return fieldCell(c, F, id[0], id[1], id[2]);
   int blocks = d / 2;

@@ -95,7 +95,7 @@ __device__ __forceinline__ bool blockedAt(const MGCtx &m, const Params &P, const
   return solidAtMG(m, P, solids, ((float)p[0] + 0.5f) * P.hv * scale, ((float)p[1] + 0.5f) * P.hv * scale,
                    ((float)p[2] + 0.5f) * P.hv * scale);
 }
-// cellTopology (pressure.wgsl:225-235).
+// cellTopology (pressure.wgsl:215-226).
 __device__ __forceinline__ uint8_t cellTopology(const MGCtx &m, const Params &P, const uint32_t *solids, const int p[3], int lv) {
   if (blockedAt(m, P, solids, p, lv)) return STORED_SOLID;
   uint8_t flags = 0;

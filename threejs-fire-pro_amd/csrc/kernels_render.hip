@@ -183,7 +183,7 @@ __device__ __forceinline__ float4 sampleWorld(const RenderArgs &r, const RBox &b
   return f;
 }
 
-// fireProEmission (fire-emission.wgsl:26-56), smoke = (albedo, soot glow).
+// fireProEmission (fire-emission.wgsl:22-56), smoke = (albedo, soot glow).
 __device__ __forceinline__ void emission(const RenderArgs &r, const float4 &f, float out[3]) {
   const RenderParams &P = r.P;
   out[0] = out[1] = out[2] = 0.0f;
@@ -229,7 +229,7 @@ __device__ __forceinline__ void normalize3(const float v[3], float out[3]) {
   out[2] = v[2] / len;
 }
 
-// exitDistance (lighting.wgsl:240-249).
+// exitDistance (lighting.wgsl:258-266).
 __device__ __forceinline__ float exitDistance(const RenderArgs &r, const RBox &bx, const float p[3], const float d[3]) {
   const int o[3] = {bx.ox, bx.oy, bx.oz}, s[3] = {bx.sx, bx.sy, bx.sz};
   float dist[3];
@@ -697,7 +697,7 @@ __device__ __forceinline__ void march(const RenderArgs &r, const float o[3], con
                                       float end, MarchState &st, int count = 0, float sx = 0.f, float sy = 0.f,
                                       float4 *slots = nullptr, size_t stride = 0) {
   const RenderParams &P = r.P;
-  // fireProDirect (volume-march.wgsl:141-146)
+  // fireProDirect (volume-march.wgsl:138-143)
   float ln[3];
   normalize3(P.light, ln);
   const float g = P.optical[3];

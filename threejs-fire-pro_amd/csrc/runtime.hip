@@ -229,7 +229,7 @@ template <class T> struct DBuf {
 
 // ---- small housekeeping kernels ---------------------------------------------------------
 
-// growFreeList (fluid-pool-growth.wgsl:12-22): new slots go under the stack.
+// growFreeList (fluid-pool-growth.wgsl:12-21): new slots go under the stack.
 __global__ void k_grow_free_list(uint32_t *freeSlots, int *poolState, int first, int added) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < added) freeSlots[i] = (uint32_t)(first + added - 1 - i);

@@ -157,7 +157,7 @@ __device__ __forceinline__ float sampleSmoke(const Ctx &c, const float *buf, flo
   return out;
 }
 
-// solidCell / solidAt (fluid-common.wgsl:123-150).
+// solidCell / solidAt (solids.wgsl:15-36).
 __device__ __forceinline__ bool solidCell(const Ctx &c, const Params &P, const uint32_t *solids, int x, int y,
                                           int z) {
   if (P.ground && y < 0) return true;
@@ -222,7 +222,7 @@ __device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int c
   return out;
 }
 // sampleVel, also returning the range of the eight cells it weighs: [0, 0] when the first
-// cell's brick has no slot (the donor bounds of fluid-velocity-transport.wgsl:165-212).
+// cell's brick has no slot (the donor bounds of fluid-velocity-transport.wgsl:104-112).
 __device__ __forceinline__ float sampleVelBounds(const Ctx &c, const float *buf, int comp, float px, float py,
                                                  float pz, float &lo, float &hi) {
   float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
