@@ -108,7 +108,7 @@ typedef struct fp_collider {
   double position[3], size[3];
   double axes[3][3]; /* the box's axes in world space (rows); world axes by default */
   /* FP_SHAPE_MESH: the mesh's occupancy of a box of field cells (MeshColliderMask,
-   * MeshCollider.ts voxelize): size[0] * size[1] * size[2] words, nonzero inside, x fastest,
+   * MeshCollider.ts voxelize): cells[0] * cells[1] * cells[2] words, nonzero inside, x fastest,
    * for the cells from lo; position, size and axes are then unused. */
   const uint32_t *occupancy;
   int32_t lo[3], cells[3];
@@ -140,8 +140,36 @@ typedef struct fp_stats {
   int32_t tile_capacity;    /* tile slots the per-brick arrays hold */
   int32_t tile_count;       /* tiles in the directory */
   int32_t pool_side;        /* slots per row and rows per layer */
-  int64_t memory_bytes;     /* device bytes this simulation holds */
+  int64_t memory_bytes;     /* device bytes this simulation holds (memoryBytes) */
+  int64_t mesh_memory_bytes;  /* mesh emission records and brick tables (meshMemoryBytes,
+                                 FluidSimulation.ts:849-851) */
+  int64_t force_memory_bytes; /* force records, regions and mesh coverage (forceMemoryBytes,
+                                 FluidSimulation.ts:846-848) */
 } fp_stats;
+
+/* The solver fields a debug view shows (FireSimulation.ts:125-127). */
+#define FP_SOLVER_FIELD_NONE 0
+#define FP_SOLVER_FIELD_PRESSURE 1
+#define FP_SOLVER_FIELD_DIVERGENCE 2 /* the right-hand side h^2 (div u - expansion) solved for */
+
+/* renderBricks (FluidSimulation.ts:163-189, :781-795): what the renderer marches. boxes:
+ * 6 words each (first brick x, y, z; last brick x, y, z), one per cluster of tiles with
+ * something to draw; tiles: 4 words per tile slot (tile x, y, z, the box of its cluster or
+ * -1). The device tables stay on the simulation's device: pages (each brick id's page or
+ * -1, brick_count words), the active bricks of the latest step (active_count ids) and the
+ * bricks it zeroed (zeroed_count (id, payload) pairs). Buffers are caller-owned; NULL skips
+ * one; *_count is what exists, written up to the matching capacity. */
+typedef struct fp_render_bricks {
+  int32_t version; /* changes whenever the boxes or the tiles do */
+  int32_t *boxes;
+  int32_t *tiles;
+  int32_t box_capacity, tile_capacity;
+  int32_t box_count, tile_count;
+  const int32_t *pages_device;
+  const int32_t *active_device;
+  const int32_t *zeroed_device;
+  int32_t brick_count, active_count, zeroed_count;
+} fp_render_bricks;
 
 /* Integer tables of the latest step, the bit-exact parity surface: every brick id's
  * page, entry and record, the free stack, pool state and the sorted active and zeroed
@@ -235,6 +263,16 @@ int fp_sim_stats(const fp_sim *sim, fp_stats *out);
 /* readBricks (FluidSimulation.ts:1749-1779): min corner of each slotted brick, meters,
  * in brick id order; `*count` in: capacity of `mins` (triples), out: bricks. */
 int fp_sim_read_bricks(fp_sim *sim, double *mins, int32_t *count);
+/* showSolverField (FluidSimulation.ts:1532-1559, called at FireSimulation.ts:211, :358):
+ * with FP_SOLVER_FIELD_PRESSURE or _DIVERGENCE, copy that field of the latest step into the
+ * simulation's solver-field view now and after every step (batched steps included);
+ * FP_SOLVER_FIELD_NONE stops the copies. */
+int fp_sim_show_solver_field(fp_sim *sim, int32_t field);
+/* The solver-field view of the active bricks in active-list order, velocity-grid cells
+ * (one float each) per brick; returns the brick count. FP_ESTATE when no field is shown. */
+int fp_sim_read_solver_field(fp_sim *sim, int32_t *ids, float *values, int32_t capacity_bricks);
+/* renderBricks (see fp_render_bricks). */
+int fp_sim_render_bricks(fp_sim *sim, fp_render_bricks *out);
 /* Integer tables of the latest step (see fp_tables). */
 int fp_sim_read_tables(fp_sim *sim, fp_tables *out);
 /* Cell values of the active bricks in active-list order: the field grid's four channels

@@ -37,6 +37,7 @@ def test_library_exports_every_declared_symbol():
     ("fp_fuel", abi.Fuel), ("fp_source", abi.Source), ("fp_force", abi.Force),
     ("fp_collider", abi.Collider), ("fp_frame", abi.Frame), ("fp_stats", abi.Stats),
     ("fp_tables", abi.Tables), ("fp_camera", abi.Camera), ("fp_render_settings", abi.RenderSettings),
+    ("fp_render_bricks", abi.RenderBricks), ("fp_render_tables", abi.RenderTables),
 ])
 def test_struct_layout_matches_header(name, ctype):
     src = f'#include "{HEADER}"\n#include <stdio.h>\nint main(void){{printf("%zu", sizeof({name}));return 0;}}\n'

@@ -158,3 +158,42 @@ def test_invalid_inputs_raise():
         gpu.dispose()
     with pytest.raises(abi.FireProError):
         gpu.stats()
+
+
+def test_solver_field_view():
+    """showSolverField: the divergence view holds the step's right-hand side h^2 (div u -
+    expansion), equal to the restatement's; switching to pressure copies at once and then
+    after every step, batched steps included; with no field shown the read is refused."""
+    from firepro import abi, scenes
+    from firepro.engine import Batch
+    sc = scenes.small(steps=6)
+    frames = list(sc.frames())
+    gpu, cpu = make_engines(sc)
+    batch = Batch()
+    try:
+        gpu.show_solver_field("divergence")
+        for frame, g in frames[:5]:
+            apply(gpu, cpu, g)
+            gpu.step(frame)
+            cpu.step(frame)
+        view = gpu.read_solver_field()
+        assert np.array_equal(view["ids"], gpu.read_state()["ids"])
+        assert np.array_equal(view["values"].astype(np.float64), cpu.rhs())
+        gpu.show_solver_field("pressure")
+        assert np.array_equal(gpu.read_solver_field()["values"], gpu.read_state(pressure=True)["pressure"])
+        frame, g = frames[5]
+        apply(gpu, cpu, g)
+        batch.step([gpu], [frame])
+        assert np.array_equal(gpu.read_solver_field()["values"], gpu.read_state(pressure=True)["pressure"])
+        gpu.show_solver_field(None)
+        with pytest.raises(abi.FireProError, match="no solver field"):
+            gpu.read_solver_field()
+        with pytest.raises(ValueError):
+            gpu.show_solver_field("velocity")
+        st = gpu.stats()
+        assert st.memory_bytes > 0 and st.force_memory_bytes > 0 and st.mesh_memory_bytes >= 0
+        rb = gpu.render_bricks
+        assert rb["active_count"] == st.active_bricks and len(rb["boxes"]) >= 1
+    finally:
+        batch.dispose()
+        gpu.dispose()

@@ -440,6 +440,10 @@ struct fp_sim {
   LevelBufs lv[MAX_LEVELS];
   float *latestPressure = nullptr;
   float *latestPressureLo = nullptr;  // the low part when the pressure is a pair
+  // showSolverField: the field copied after every step (FP_SOLVER_FIELD_*) and its copy,
+  // a velocity-grid float per cell in slot order.
+  int solverField = 0;
+  DBuf<float> solverView;
   // Pressure policy: 0 the reference's one cold V-cycle; 1-3 defect correction (an FP32
   // outer loop to a relative residual) around V-cycles whose levels are stored in FP32,
   // BF16 or FP16, with the coarsest level solved in FP64.
@@ -1403,6 +1407,7 @@ struct fp_sim {
     s.pressureLo = latestPressureLo;
     k_project<<<slotsGrid, vLanes, 0, stream>>>(s);
     end(m7, "project");
+    exportSolverField();
     hostMark("host_enqueue", enqueued);
     Clock::time_point w0 = Clock::now();
     if (profiling && profileLevel != 2) {
@@ -1958,11 +1963,31 @@ struct fp_sim {
                    rankTmp.bytes() + blockSums.bytes() + blockPrefix.bytes() + freeSlots.bytes() + F.bytes() +
                    Fp.bytes() + Fn.bytes() + Don.bytes() + Vtr.bytes() + Curl.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
                    Vcor.bytes() + Vf.bytes() + D.bytes() + P0.bytes() + P1.bytes() + solids.bytes() + S.bytes() +
-                   Sp.bytes() + Sn.bytes() +
-                   emitters.bytes() + colliders.bytes() + forces.bytes() + regions.bytes();
+                   Sp.bytes() + Sn.bytes() + emitters.bytes() + colliders.bytes() + colliderMasks.bytes() +
+                   solverView.bytes() + dcRhs.bytes() + dcX.bytes() + dcXlo.bytes() + dcR.bytes() +
+                   dcPartials.bytes() + dcNorms.bytes();
     for (int L = 0; L <= bottom; L++)
-      total += lv[L].p.bytes() + lv[L].q.bytes() + lv[L].rhs.bytes() + lv[L].topo.bytes() + lv[L].eflag.bytes();
+      total += lv[L].p.bytes() + lv[L].q.bytes() + lv[L].rhs.bytes() + lv[L].topo.bytes() + lv[L].eflag.bytes() +
+               lv16[L].p.bytes() + lv16[L].q.bytes() + lv16[L].rhs.bytes();
     return total;
+  }
+  // meshMemoryBytes and forceMemoryBytes (FluidSimulation.ts:846-851): the scene's mesh
+  // emission and force data, which memoryBytes leaves out.
+  size_t meshMemoryBytes() const { return meshRecs.bytes() + meshTables.bytes(); }
+  size_t forceMemoryBytes() const { return forces.bytes() + regions.bytes() + forceCoverage.bytes(); }
+
+  // encodeSolverField (FluidSimulation.ts:1550-1559): the shown field of the latest step
+  // into the solver-field view, on the stream after the step.
+  void exportSolverField() {
+    if (!solverField || !latestPressure) return;
+    const size_t n = (size_t)poolSlots * nV;
+    if (solverView.n != n) {
+      drain();  // the previous copy may still be queued
+      solverView.alloc(n);
+    }
+    const float *src = solverField == FP_SOLVER_FIELD_PRESSURE ? latestPressure : D.p;
+    HIPCHECK(hipMemcpyAsync(solverView.p, src, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    tailPending = true;
   }
 
   // Finishes whatever the last entry point left queued (a step's tail, a device render).
@@ -2003,6 +2028,7 @@ struct fp_sim {
       lv16[L].q.release();
       lv16[L].rhs.release();
     }
+    solverView.release();
     dcRhs.release();
     dcX.release();
     dcXlo.release();
@@ -2220,7 +2246,10 @@ struct fp_batch {
       for (int i = 0; i < n; i++) sims[i]->stream = own[i];
       throw;
     }
-    for (int i = 0; i < n; i++) sims[i]->stream = own[i];
+    for (int i = 0; i < n; i++) {
+      sims[i]->stream = own[i];
+      sims[i]->exportSolverField();  // on the member's own stream; the batch's has finished
+    }
     size_t off = 0;
     auto f0 = std::chrono::steady_clock::now();
     for (int i = 0; i < n; i++) {
@@ -2605,7 +2634,87 @@ int fp_sim_stats(const fp_sim *sim, fp_stats *out) {
   out->tile_count = sim->dir.count();
   out->pool_side = sim->side;
   out->memory_bytes = (int64_t)sim->memoryBytes();
+  out->mesh_memory_bytes = (int64_t)sim->meshMemoryBytes();
+  out->force_memory_bytes = (int64_t)sim->forceMemoryBytes();
   return FP_OK;
+}
+
+int fp_sim_show_solver_field(fp_sim *sim, int32_t field) {
+  if (!sim) {
+    g_error = "fp_sim_show_solver_field: null argument";
+    return FP_EINVAL;
+  }
+  if (field != FP_SOLVER_FIELD_NONE && field != FP_SOLVER_FIELD_PRESSURE && field != FP_SOLVER_FIELD_DIVERGENCE) {
+    g_error = "solver field must be pressure, divergence or none.";
+    return FP_EINVAL;
+  }
+  return guard([&] {
+    sim->drain();
+    sim->solverField = field;
+    if (!field) {
+      sim->solverView.release();
+      return;
+    }
+    if (sim->stepped) sim->exportSolverField();  // now, as well as after every step
+  });
+}
+
+int fp_sim_read_solver_field(fp_sim *sim, int32_t *ids, float *values, int32_t capacity) {
+  if (!sim) {
+    g_error = "fp_sim_read_solver_field: null argument";
+    return FP_EINVAL;
+  }
+  int result = 0;
+  int rc = guard([&] {
+    sim->drain();
+    if (!sim->solverField) throw std::logic_error("no solver field is shown (fp_sim_show_solver_field).");
+    int count = 0;
+    if (sim->stepped && sim->solverView.p) HIPCHECK(hipMemcpy(&count, sim->counts.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (count > capacity) throw InvalidArgument("solver field buffers hold fewer bricks than are active.");
+    std::vector<int32_t> act(count), entries(sim->brickCount());
+    if (count) HIPCHECK(hipMemcpy(act.data(), sim->active.p, count * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(entries.data(), sim->entries.p, entries.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int nV = sim->nV;
+    for (int i = 0; i < count; i++) {
+      if (ids) ids[i] = act[i];
+      if (values)
+        HIPCHECK(hipMemcpy(values + (size_t)i * nV,
+                           sim->solverView.p + (size_t)sim->pageToSlotHost(entries[act[i]] & 0xffffff) * nV,
+                           nV * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    result = count;
+  });
+  return rc != FP_OK ? rc : result;
+}
+
+int fp_sim_render_bricks(fp_sim *sim, fp_render_bricks *out) {
+  if (!sim || !out) {
+    g_error = "fp_sim_render_bricks: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] {
+    sim->drain();
+    const auto &rb = sim->rboxes;
+    out->version = rb.version;
+    out->box_count = (int32_t)rb.boxes.size();
+    out->tile_count = (int32_t)(rb.tiles.size() / 4);
+    if (out->boxes)
+      for (int i = 0; i < std::min(out->box_count, out->box_capacity); i++)
+        for (int a = 0; a < 3; a++) {
+          out->boxes[6 * i + a] = rb.boxes[i].lo[a];
+          out->boxes[6 * i + 3 + a] = rb.boxes[i].hi[a];
+        }
+    if (out->tiles)
+      std::memcpy(out->tiles, rb.tiles.data(), (size_t)std::min(out->tile_count, out->tile_capacity) * 4 * sizeof(int32_t));
+    int counts[8] = {0};
+    if (sim->stepped) HIPCHECK(hipMemcpy(counts, sim->counts.p, sizeof(counts), hipMemcpyDeviceToHost));
+    out->pages_device = sim->pages.p;
+    out->active_device = sim->active.p;
+    out->zeroed_device = sim->zeroList.p;
+    out->brick_count = sim->brickCount();
+    out->active_count = counts[0];
+    out->zeroed_count = counts[1] + counts[5];
+  });
 }
 
 int fp_sim_read_bricks(fp_sim *sim, double *mins, int32_t *count) {

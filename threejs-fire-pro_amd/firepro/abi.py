@@ -102,8 +102,22 @@ class Stats(C.Structure):
         ("time", C.c_double), ("steps", C.c_int64), ("active_bricks", C.c_int64),
         ("active_voxels", C.c_int64), ("pool_slots", C.c_int64), ("used_slots", C.c_int64),
         ("pool_limited", C.c_int32), ("tile_capacity", C.c_int32), ("tile_count", C.c_int32),
-        ("pool_side", C.c_int32), ("memory_bytes", C.c_int64),
+        ("pool_side", C.c_int32), ("memory_bytes", C.c_int64), ("mesh_memory_bytes", C.c_int64),
+        ("force_memory_bytes", C.c_int64),
     ]
+
+
+class RenderBricks(C.Structure):
+    _fields_ = [
+        ("version", C.c_int32), ("boxes", C.POINTER(C.c_int32)), ("tiles", C.POINTER(C.c_int32)),
+        ("box_capacity", C.c_int32), ("tile_capacity", C.c_int32), ("box_count", C.c_int32),
+        ("tile_count", C.c_int32), ("pages_device", C.c_void_p), ("active_device", C.c_void_p),
+        ("zeroed_device", C.c_void_p), ("brick_count", C.c_int32), ("active_count", C.c_int32),
+        ("zeroed_count", C.c_int32),
+    ]
+
+
+SOLVER_FIELDS = {None: 0, "pressure": 1, "divergence": 2}
 
 
 class Tables(C.Structure):
@@ -560,6 +574,9 @@ def load(path: str = LIBRARY):
                                         P(C.c_float), P(C.c_float), C.c_int32]),
         "fp_sim_write_state": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float)]),
         "fp_sim_read_smoke": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_float), C.c_int32]),
+        "fp_sim_show_solver_field": (C.c_int, [C.c_void_p, C.c_int32]),
+        "fp_sim_read_solver_field": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_float), C.c_int32]),
+        "fp_sim_render_bricks": (C.c_int, [C.c_void_p, P(RenderBricks)]),
         "fp_sim_stream": (C.c_void_p, [C.c_void_p]),
         "fp_sim_profile": (C.c_int, [C.c_void_p, C.c_int32]),
         "fp_sim_profile_read": (C.c_int, [C.c_void_p, P(C.c_char_p), P(C.c_double),

@@ -111,6 +111,55 @@ class FluidSimulation:
     def time(self) -> float:
         return self.stats().time
 
+    @property
+    def memory_bytes(self) -> int:
+        return self.stats().memory_bytes
+
+    @property
+    def mesh_memory_bytes(self) -> int:
+        """meshMemoryBytes (FluidSimulation.ts:849-851)."""
+        return self.stats().mesh_memory_bytes
+
+    @property
+    def force_memory_bytes(self) -> int:
+        """forceMemoryBytes (FluidSimulation.ts:846-848)."""
+        return self.stats().force_memory_bytes
+
+    def show_solver_field(self, field: Optional[str]) -> None:
+        """showSolverField (FluidSimulation.ts:1542-1549): 'pressure', 'divergence' or None."""
+        if field not in abi.SOLVER_FIELDS:
+            raise ValueError("solver field must be 'pressure', 'divergence' or None.")
+        abi.check(self._lib.fp_sim_show_solver_field(self._h(), abi.SOLVER_FIELDS[field]), self._lib)
+
+    def read_solver_field(self) -> dict:
+        """The shown solver field of the latest step per active brick: {ids, values[brick, cell]}."""
+        n = max(1, self.stats().active_bricks)
+        ids = np.zeros(n, np.int32)
+        out = np.zeros((n, self.velocity_cells), np.float32)
+        count = self._lib.fp_sim_read_solver_field(self._h(), ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                   out.ctypes.data_as(C.POINTER(C.c_float)), n)
+        if count < 0:
+            abi.check(count, self._lib)
+        return {"ids": ids[:count], "values": out[:count]}
+
+    @property
+    def render_bricks(self) -> dict:
+        """renderBricks (FluidSimulation.ts:781-795): version, boxes [(lo, hi)], the tile words
+        [slot, 4], and the device addresses of the pages, active and zeroed tables."""
+        rb = abi.RenderBricks()
+        abi.check(self._lib.fp_sim_render_bricks(self._h(), C.byref(rb)), self._lib)
+        boxes = np.zeros((max(1, rb.box_count), 6), np.int32)
+        tiles = np.zeros((max(1, rb.tile_count), 4), np.int32)
+        rb.boxes = boxes.ctypes.data_as(C.POINTER(C.c_int32))
+        rb.tiles = tiles.ctypes.data_as(C.POINTER(C.c_int32))
+        rb.box_capacity, rb.tile_capacity = len(boxes), len(tiles)
+        abi.check(self._lib.fp_sim_render_bricks(self._h(), C.byref(rb)), self._lib)
+        return {"version": rb.version,
+                "boxes": [(tuple(b[:3]), tuple(b[3:])) for b in boxes[:rb.box_count]],
+                "tiles": tiles[:rb.tile_count], "pages": rb.pages_device, "active": rb.active_device,
+                "zeroed": rb.zeroed_device, "brick_count": rb.brick_count, "active_count": rb.active_count,
+                "zeroed_count": rb.zeroed_count}
+
     def read_bricks(self):
         """readBricks (FluidSimulation.ts:1749-1779): each slotted brick's (min, max) corners."""
         s = self.stats()

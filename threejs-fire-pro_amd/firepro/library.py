@@ -30,6 +30,10 @@ DEFAULTS = {
     "rendering": {"filter": "cubic", "raySteps": 256, "halfResolution": True,
                   "lightingDivisor": 4},
 }
+# options.ts:219-231
+DEBUG_FIELDS = ("beauty", "lifetime", "heat", "smoke", "velocity", "flame", "fuel", "vorticity", "expansion",
+                "pressure", "divergence")
+
 # options.ts:367-405
 LIMITS = {
     "simulation": {"voxelSize": (0.002, 4)},
@@ -486,11 +490,26 @@ class FireSimulation:
 
     @property
     def stats(self) -> dict:
+        """stats (FireSimulation.ts:295-334); the simulation's memory is memoryBytes +
+        meshMemoryBytes + forceMemoryBytes (the render surface is not built here)."""
         s = self.fluid.stats()
         return {"simulationTime": s.time, "droppedTime": self.dropped_time,
                 "gridLimited": bool(s.pool_limited),
                 "activeVoxels": s.active_bricks * self.options["brickSize"] ** 3,
-                "estimatedMemoryBytes": s.memory_bytes}
+                "estimatedMemoryBytes": s.memory_bytes + s.mesh_memory_bytes + s.force_memory_bytes}
+
+    def debug(self, options: dict):
+        """debug (FireSimulation.ts:336-362) for the solver's part: a 'pressure' or
+        'divergence' field makes the engine copy that field after every step
+        (showSolverField); other fields are views of the render surface and stop the copies."""
+        _keys(options, ["bricks", "field"], "debug")
+        field = options.get("field", getattr(self, "debug_field", "beauty"))
+        if field not in DEBUG_FIELDS:
+            raise ValueError("Unknown debug field.")
+        self.debug_field = field
+        if self.fluid is not None:
+            self.fluid.show_solver_field(field if field in ("pressure", "divergence") else None)
+        return self
 
     def reset(self):
         self.fluid.reset()
