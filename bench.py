@@ -75,9 +75,72 @@ def parse():
     p.add_argument("--mix", type=int, default=256,
                    help="requests of the serving stream (Poisson at --serving-rate) for the batched "
                         "mixed-precision comparison in the default line (0 skips it)")
+    p.add_argument("--profile-summary", metavar="DIR", default=None,
+                   help="only summarize the rocprofv3 CSV output under DIR (kernel trace and/or "
+                        "counters) over the last --tail-steps solver steps, and exit")
+    p.add_argument("--tail-steps", type=int, default=20)
     p.add_argument("--copy-probe", type=int, default=0,
                    help="only copy this many MiB device to device 5 times (counter calibration)")
     return p.parse_args()
+
+
+def profile_summary(root: str, tail: int) -> None:
+    """Per-kernel averages from rocprofv3's CSV output under `root`, over the dispatches of
+    the last `tail` solver steps (from the start of the tail-th last k_build_bricks, each
+    step's first kernel, to the end): calls, mean and total duration from the kernel trace,
+    and each counter's mean value per dispatch from a counter collection. Kernels of a run
+    without solver steps (the copy probe) are summarized whole."""
+    import csv
+    import glob
+    from collections import defaultdict
+
+    def rows(pattern):
+        out = []
+        for path in sorted(glob.glob(os.path.join(root, "**", pattern), recursive=True)):
+            with open(path, newline="") as f:
+                out.extend(csv.DictReader(f))
+        return out
+
+    def window(rs):
+        # One start per dispatch (a counter collection has a row per counter).
+        starts = sorted({(int(r["Start_Timestamp"]), r.get("Dispatch_Id", "")) for r in rs
+                         if "k_build_bricks" in r["Kernel_Name"]})
+        starts = [t for t, _ in starts]
+        return starts[-tail] if len(starts) >= tail else (starts[0] if starts else 0)
+
+    def short(name):
+        return name.split("(")[0].replace("void ", "").replace("fp::", "")
+
+    trace = rows("*kernel_trace.csv")
+    if trace:
+        t0 = window(trace)
+        agg = defaultdict(list)
+        vgpr = {}
+        for r in trace:
+            if int(r["Start_Timestamp"]) >= t0:
+                agg[short(r["Kernel_Name"])].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+                vgpr[short(r["Kernel_Name"])] = r.get("Arch_VGPR_Count", r.get("VGPR_Count", ""))
+        total = sum(sum(v) for v in agg.values())
+        print(f"kernel trace, last {tail} steps: {sum(len(v) for v in agg.values())} dispatches, "
+              f"{total / 1e6:.3f} ms of kernel time")
+        print(f"{'kernel':48s} {'calls':>6s} {'avg us':>9s} {'total us':>10s} {'share':>6s} {'vgpr':>5s}")
+        for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+            print(f"{k[:48]:48s} {len(v):6d} {sum(v) / len(v) / 1e3:9.2f} {sum(v) / 1e3:10.1f} "
+                  f"{100 * sum(v) / total:5.1f}% {vgpr[k]:>5s}")
+    counters = rows("*counter_collection.csv")
+    if counters:
+        t0 = window(counters)
+        agg = defaultdict(lambda: defaultdict(list))
+        for r in counters:
+            if int(r["Start_Timestamp"]) >= t0:
+                agg[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        print(f"counters, last {tail} steps, mean per dispatch:")
+        names = sorted({c for v in agg.values() for c in v})
+        print(f"{'kernel':48s} {'calls':>6s} " + " ".join(f"{c:>16s}" for c in names))
+        for k, v in sorted(agg.items()):
+            calls = max(len(x) for x in v.values())
+            print(f"{k[:48]:48s} {calls:6d} " + " ".join(
+                f"{(sum(v[c]) / len(v[c])) if v.get(c) else float('nan'):16.1f}" for c in names))
 
 
 def copy_probe(mib: int) -> None:
@@ -341,6 +404,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.profile_summary:
+        profile_summary(args.profile_summary, args.tail_steps)
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return

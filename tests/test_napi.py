@@ -118,6 +118,7 @@ def test_addon_steps_like_the_c_abi(name, tmp_path):
     assert r.returncode == 0, r.stderr[-2000:]
     js = json.loads(r.stdout)
     sim = FluidSimulation.create(grid)
+    sim.show_solver_field("pressure")  # as the script does: its view counts in memoryBytes
     try:
         for (frame, g), got in zip(sc.frames(), js["steps"]):
             sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
