@@ -310,6 +310,14 @@ int fp_sim_profile_read(fp_sim *sim, const char **names, double *ms, int32_t *co
  * without render boxes. */
 int fp_sim_scene_lights(fp_sim *sim, const fp_render_settings *settings, const double *anchors, int32_t count,
                         float *sums);
+/* fp_sim_scene_lights without allocating device memory: no counterpart in the reference.
+ * For callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same results, bit for bit; its buffers are a fixed
+ * 2.8 MiB device array per GPU, held until the process ends, so it takes at most 64
+ * anchors over at most 65536 active bricks (FP_EINVAL beyond). Calls on one GPU share that
+ * array: calls that overlap on one GPU race. */
+int fp_sim_scene_lights_noalloc(fp_sim *sim, const fp_render_settings *settings, const double *anchors,
+                                int32_t count, float *sums);
 
 /* Batched stepping (the serving form; no counterpart in the reference, whose simulations
  * step one at a time). fp_batch_step advances `count` simulations, all of one brick size,

@@ -367,6 +367,54 @@ static const float4 *sharedNoise() {
   return t;
 }
 
+// ---- the allocation-free scene lights (fp_sim_scene_lights_noalloc) ----
+// Their buffers live in one file-scope device array, `scratch`, at fixed offsets: the
+// anchors, the blackbody table, per-brick totals and nearest anchors, per-block totals and
+// the anchors' totals. Each GPU holds its own copy until the process ends.
+constexpr int NOALLOC_MAX_BRICKS = 65536;
+constexpr int NOALLOC_MAX_ANCHORS = 64;
+constexpr size_t NA_ANCHORS = 0;                                                  // float4 per anchor
+constexpr size_t NA_LUT = NA_ANCHORS + (size_t)NOALLOC_MAX_ANCHORS * sizeof(float4);  // 512 float4
+constexpr size_t NA_PARTIALS = NA_LUT + 512 * sizeof(float4);                     // 8 floats per brick
+constexpr size_t NA_BRICK_ANCHOR = NA_PARTIALS + (size_t)NOALLOC_MAX_BRICKS * 8 * sizeof(float);
+constexpr size_t NA_BLOCK_SUMS = NA_BRICK_ANCHOR + (size_t)NOALLOC_MAX_BRICKS * sizeof(int);
+constexpr size_t NA_SUMS =
+    NA_BLOCK_SUMS + (size_t)(NOALLOC_MAX_BRICKS / LIGHT_BLOCK) * NOALLOC_MAX_ANCHORS * 8 * sizeof(float);
+constexpr size_t NA_BYTES = NA_SUMS + (size_t)NOALLOC_MAX_ANCHORS * 8 * sizeof(float);
+static_assert(NA_LUT % 256 == 0 && NA_PARTIALS % 256 == 0 && NA_BRICK_ANCHOR % 256 == 0 &&
+                  NA_BLOCK_SUMS % 256 == 0 && NA_SUMS % 256 == 0,
+              "scratch regions start on 256-byte boundaries");
+
+}  // namespace fp
+
+// In host code this name is a stand-in, not a device address: host code uses the address
+// hipGetSymbolAddress gives for the current device (fp::noallocScratch).
+static __device__ __align__(256) unsigned char scratch[fp::NA_BYTES];
+
+namespace fp {
+
+// The device address of `scratch` on the current device, found once per device (with that
+// device current) and kept; the first call also writes the blackbody table into it.
+static unsigned char *noallocScratch() {
+  static std::mutex mu;
+  static std::map<int, unsigned char *> addresses;
+  std::lock_guard<std::mutex> lock(mu);
+  int d = 0;
+  HIPCHECK(hipGetDevice(&d));
+  unsigned char *&ptr = addresses[d];
+  if (!ptr) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw DeviceFailure(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    static const std::vector<float> lut = render::blackbody();  // host, built once per process
+    HIPCHECK(hipMemcpy(static_cast<unsigned char *>(p) + NA_LUT, lut.data(), lut.size() * sizeof(float),
+                       hipMemcpyHostToDevice));
+    ptr = static_cast<unsigned char *>(p);
+  }
+  return ptr;
+}
+
 constexpr int MIN_TILES = 16;
 // Workgroups that walk a step's freshly allocated bricks (a few per step, rarely thousands).
 constexpr unsigned FRESH_GRID = 2048;
@@ -1897,6 +1945,44 @@ struct fp_sim {
     if (profiling) collect();
   }
 
+  // sceneLights without allocating: the same kernels on buffers in `scratch` (the
+  // device address `base`), for at most NOALLOC_MAX_ANCHORS anchors over at most
+  // NOALLOC_MAX_BRICKS active bricks. Every scratch word a call reads it writes first, so
+  // nothing needs clearing between calls; calls that overlap on one GPU share the array
+  // and race.
+  void sceneLightsNoAlloc(const fp_render_settings &s, const double *xyz, int count, float *out,
+                          unsigned char *base) {
+    if (count < 0 || count > NOALLOC_MAX_ANCHORS)
+      throw InvalidArgument("the allocation-free scene lights take 0 to 64 anchors.");
+    for (int i = 0; i < 3 * count; i++)
+      if (!std::isfinite(xyz[i])) throw InvalidArgument("anchor positions must be finite.");
+    fp_camera cam{};
+    cam.width = cam.height = 1;
+    RenderParams P = renderParams(cam, s);
+    std::fill(out, out + (size_t)8 * count, 0.0f);
+    const int n = (int)std::min<int64_t>(activeBricks, (int64_t)brickCount());
+    if (n > NOALLOC_MAX_BRICKS)
+      throw InvalidArgument("the allocation-free scene lights take at most 65536 active bricks.");
+    if (count == 0 || rboxes.boxes.empty() || n == 0 || !stepped) return;
+    float4 a[NOALLOC_MAX_ANCHORS];
+    for (int i = 0; i < count; i++) a[i] = make_float4((float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2], 0.f);
+    float4 *anchors = reinterpret_cast<float4 *>(base + NA_ANCHORS);
+    float *partials = reinterpret_cast<float *>(base + NA_PARTIALS);
+    int *brickAnchor = reinterpret_cast<int *>(base + NA_BRICK_ANCHOR);
+    float *blockSums = reinterpret_cast<float *>(base + NA_BLOCK_SUMS);
+    float *sums = reinterpret_cast<float *>(base + NA_SUMS);
+    const int blocks = (n + LIGHT_BLOCK - 1) / LIGHT_BLOCK;
+    HIPCHECK(hipMemcpyAsync(anchors, a, count * sizeof(float4), hipMemcpyHostToDevice, stream));
+    RenderArgs r = renderArgs(P);
+    r.lut = reinterpret_cast<const float4 *>(base + NA_LUT);
+    k_light_bricks<<<(unsigned)n, 64, 0, stream>>>(r, tileRec.p, anchors, count, partials, brickAnchor);
+    k_light_blocks<<<(unsigned)blocks, LIGHT_BLOCK, 0, stream>>>(counts.p, partials, brickAnchor, count, blockSums);
+    k_light_total<<<1, LIGHT_BLOCK, 0, stream>>>(counts.p, blockSums, count, sums);
+    HIPCHECK(hipMemcpyAsync(out, sums, (size_t)count * 8 * sizeof(float), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipGetLastError());
+  }
+
   void readRender(fp_render_tables *t) {
     HIPCHECK(hipStreamSynchronize(stream));
     t->light_work = rc.lastWork;
@@ -2911,6 +2997,19 @@ int fp_sim_scene_lights(fp_sim *sim, const fp_render_settings *settings, const d
     return FP_EINVAL;
   }
   return guard([&] { sim->drain(); sim->sceneLights(*settings, anchors, count, sums); });
+}
+
+int fp_sim_scene_lights_noalloc(fp_sim *sim, const fp_render_settings *settings, const double *anchors,
+                                int32_t count, float *sums) {
+  if (!sim || !settings || (count > 0 && (!anchors || !sums))) {
+    g_error = "fp_sim_scene_lights_noalloc: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] {
+    sim->drain();
+    unsigned char *base = noallocScratch();
+    sim->sceneLightsNoAlloc(*settings, anchors, count, sums, base);
+  });
 }
 
 int fp_batch_create(fp_batch **out) {

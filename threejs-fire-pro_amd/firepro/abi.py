@@ -591,6 +591,8 @@ def load(path: str = LIBRARY):
         "fp_sim_pressure_report": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_double)]),
         "fp_sim_render": (C.c_int, [C.c_void_p, P(Camera), P(RenderSettings), P(C.c_float)]),
         "fp_sim_scene_lights": (C.c_int, [C.c_void_p, P(RenderSettings), P(C.c_double), C.c_int32, P(C.c_float)]),
+        "fp_sim_scene_lights_noalloc": (C.c_int, [C.c_void_p, P(RenderSettings), P(C.c_double), C.c_int32,
+                                                  P(C.c_float)]),
         "fp_sim_render_device": (C.c_int, [C.c_void_p, P(Camera), P(RenderSettings), C.c_void_p]),
         "fp_sim_read_render": (C.c_int, [C.c_void_p, P(RenderTables)]),
     }

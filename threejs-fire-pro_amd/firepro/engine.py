@@ -260,6 +260,17 @@ class FluidSimulation:
                                                 len(a), out.ctypes.data_as(C.POINTER(C.c_float))), self._lib)
         return out
 
+    def scene_lights_noalloc(self, anchors, settings: Optional[abi.RenderSettings] = None) -> np.ndarray:
+        """The same totals without allocating device memory (fp_sim_scene_lights_noalloc):
+        at most 64 anchors over at most 65536 active bricks."""
+        rs = settings if settings is not None else abi.render_settings()
+        a = np.ascontiguousarray(np.asarray(anchors, np.float64).reshape(-1, 3))
+        out = np.zeros((len(a), 8), np.float32)
+        abi.check(self._lib.fp_sim_scene_lights_noalloc(
+            self._h(), C.byref(rs), a.ctypes.data_as(C.POINTER(C.c_double)), len(a),
+            out.ctypes.data_as(C.POINTER(C.c_float))), self._lib)
+        return out
+
     def render_device(self, camera: abi.Camera, settings: abi.RenderSettings, pointer: int) -> None:
         """Render into device memory at `pointer` (width * height float4), enqueued on the
         simulation's stream."""
