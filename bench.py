@@ -52,6 +52,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=55)
     p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--brick", type=int, default=8, choices=(8, 16),
+                   help="field brick edge of the four-fires workload (16 with --max-voxels 64e6: the "
+                        "large-brick roofline variant whose working set exceeds the Infinity Cache)")
+    p.add_argument("--max-voxels", type=float, default=16e6)
     p.add_argument("--state-step", type=int, default=240,
                    help="steps advanced untimed before the warmup: the state the timed steps see")
     p.add_argument("--seed", type=int, default=1)
@@ -386,11 +390,11 @@ def render_phase(sim, frames, step) -> dict:
     return out, solver
 
 
-def measured_traffic(kernel: str, cells: float):
+def measured_traffic(kernel: str, cells: float, brick: int = 8):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
     same command (FETCH_SIZE scaled by the plain-copy calibration, plus WRITE_SIZE), per
     active cell, times this run's cells; None without a summary for the kernel."""
-    path = os.path.join(ROOT, "profiles", "four_fires_b8_hbm_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"four_fires_b{brick}_hbm_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -430,7 +434,8 @@ def main():
 
     first = args.state_step + args.warmup  # the first timed step
     total = first + args.steps + args.render_steps
-    scene_fn = lambda: scenes.four_fires(seed=args.seed, steps=total)  # noqa: E731
+    scene_fn = lambda: scenes.four_fires(seed=args.seed, steps=total, brick=args.brick,  # noqa: E731
+                                         max_voxels=int(args.max_voxels))
     sc = scene_fn()
     frames = [(abi.PackedFrame(f), g) for f, g in sc.frames()]
     sim = FluidSimulation.create(sc.grid())
@@ -464,18 +469,19 @@ def main():
     best = max((k for k in stages if k in STAGE_BYTES_PER_FIELD_CELL), key=lambda k: stages[k])
     algorithmic = STAGE_BYTES_PER_FIELD_CELL[best] * cells
     achieved = algorithmic / (stages[best] * 1e-3) / 1e9
-    traffic = measured_traffic(best, cells)
+    traffic = measured_traffic(best, cells, args.brick)
     line = {
         "metric": "active voxel-steps/s", "value": value, "unit": "active voxel-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "four-fires plain FP32 single simulation: 4 fire spheres in 6x4x6 m, "
-                               "turbulence 1/2, sphere collider r 0.5, voxel 0.025 m, brick 8, "
+                               f"turbulence 1/2, sphere collider r 0.5, voxel 0.025 m, brick {args.brick}, "
+                               f"maxVoxels {args.max_voxels:.3g}, "
                                f"timed steps {first}-{first + args.steps - 1} (after an untimed advance to "
                                f"step {args.state_step} and {args.warmup} warmup steps)",
                    "active_bricks": bricks_seen[-1], "active_bricks_min": min(bricks_seen),
-                   "active_voxels": bricks_seen[-1] * 512},
+                   "active_voxels": bricks_seen[-1] * args.brick ** 3},
         "p50_step_ms": float(np.percentile(latencies, 50)),
         "p99_step_ms": float(np.percentile(latencies, 99)),
         "step_latency": "device time per step on the simulation's stream, HIP events before and after each "
