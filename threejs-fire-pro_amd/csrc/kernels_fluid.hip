@@ -180,6 +180,21 @@ __device__ __forceinline__ void traceForward(const Ctx &c, const Params &P, cons
   velocityAt(c, P, V, m[0], m[1], m[2], v);
   for (int a = 0; a < 3; a++) out[a] = pos[a] + P.dt * v[a];
 }
+// The same traces through a workgroup's velocity window.
+__device__ __forceinline__ void traceBack(const Ctx &c, const VelWindow &w, const Params &P, const float *V,
+                                          const float pos[3], const float cv[3], float out[3]) {
+  float m[3], v[3];
+  for (int a = 0; a < 3; a++) m[a] = pos[a] - 0.5f * P.dt * cv[a];
+  velocityAt(c, w, P, V, m[0], m[1], m[2], v);
+  for (int a = 0; a < 3; a++) out[a] = pos[a] - P.dt * v[a];
+}
+__device__ __forceinline__ void traceForward(const Ctx &c, const VelWindow &w, const Params &P, const float *V,
+                                             const float pos[3], const float cv[3], float out[3]) {
+  float m[3], v[3];
+  for (int a = 0; a < 3; a++) m[a] = pos[a] + 0.5f * P.dt * cv[a];
+  velocityAt(c, w, P, V, m[0], m[1], m[2], v);
+  for (int a = 0; a < 3; a++) out[a] = pos[a] + P.dt * v[a];
+}
 __device__ __forceinline__ void departureCell(const Params &P, const int id[3], const float pos[3],
                                               const float dep[3], int base[3], float frac[3]) {
   for (int a = 0; a < 3; a++) {
@@ -190,49 +205,67 @@ __device__ __forceinline__ void departureCell(const Params &P, const int id[3], 
   }
 }
 
+// The field-cell kernels give each brick max(1, nF / 256) chunks of 256 lanes, one cell a
+// lane: chunk k holds the brick's cells [256 k, 256 k + 256), a slab of whole z layers (or
+// part of one), whose local z range this is.
+__device__ __forceinline__ void chunkSlab(const Layout &L, int chunk, int &zlo, int &zhi) {
+  const int first = chunk * 256, last = (first + 256 < L.nF ? first + 256 : L.nF) - 1;
+  zlo = first >> (2 * L.sF);
+  zhi = last >> (2 * L.sF);
+}
+
 __device__ __forceinline__ void advect_scalars_body(const StepArgs &s, const int vb, int chunks) {
   BRICK_PREAMBLE(chunks)
+  (void)chunks;
   const int nF = s.L.nF;
-  for (int l = chunk * 256 + threadIdx.x; l < nF; l += chunks * 256) {
-    int cc[3];
-    fieldCellCoords(c, l, cc);
-    float pos[3] = {((float)cc[0] + 0.5f) * s.P.hf, ((float)cc[1] + 0.5f) * s.P.hf, ((float)cc[2] + 0.5f) * s.P.hf};
-    float cv[3], dep[3];
-    velocityAt(c, s.P, s.p.V, pos[0], pos[1], pos[2], cv);
-    traceBack(c, s.P, s.p.V, pos, cv, dep);
-    float4 *out = s.p.Fp + slotBase(slot, s.L.sF) + l;
-    if (!s.P.maccormack) {
-      float4 v = sampleField(c, s.p.F, dep[0] * s.P.invHf, dep[1] * s.P.invHf, dep[2] * s.P.invHf);
-      *out = make_float4(s.P.coarse ? 0.f : v.x, v.y, clampv(v.z, 0.f, 1.f), v.w);
-      continue;
-    }
-    int base[3];
-    float frac[3];
-    departureCell(s.P, cc, pos, dep, base, frac);
-    Donors dn;
-    float4 v = predict(c, s.p.F, base, frac, dn);
-    // Coarse smoke: the fine X channel only stages the step's smoke production
-    // (fluid-scalar-transport.wgsl:127, :137, :191-199).
-    *out = make_float4(s.P.coarse ? 0.f : v.x, v.y, s.P.lifetimeZero ? 0.f : v.z, v.w);
-    float4 *don = s.p.Don + slotBase(slot * 3, s.L.sF) + l;
-    don[0] = make_float4(cv[0], cv[1], cv[2], dn.lmn);
-    don[nF] = make_float4(dn.mn[0], dn.mn[1], dn.mn[2], dn.lmx);
-    don[2 * nF] = make_float4(dn.mx[0], dn.mx[1], dn.mx[2], 0.f);
+  __shared__ float vwin[3 * VW_CELLS];
+  VelWindow w;
+  int zlo, zhi;
+  chunkSlab(s.L, chunk, zlo, zhi);
+  stageVelWindow(c, s.p.V, zlo, zhi, vwin, w);
+  const int l = chunk * 256 + threadIdx.x;
+  if (l >= nF) return;
+  int cc[3];
+  fieldCellCoords(c, l, cc);
+  float pos[3] = {((float)cc[0] + 0.5f) * s.P.hf, ((float)cc[1] + 0.5f) * s.P.hf, ((float)cc[2] + 0.5f) * s.P.hf};
+  float cv[3], dep[3];
+  velocityAt(c, w, s.P, s.p.V, pos[0], pos[1], pos[2], cv);
+  traceBack(c, w, s.P, s.p.V, pos, cv, dep);
+  float4 *out = s.p.Fp + slotBase(slot, s.L.sF) + l;
+  if (!s.P.maccormack) {
+    float4 v = sampleField(c, s.p.F, dep[0] * s.P.invHf, dep[1] * s.P.invHf, dep[2] * s.P.invHf);
+    *out = make_float4(s.P.coarse ? 0.f : v.x, v.y, clampv(v.z, 0.f, 1.f), v.w);
+    return;
   }
+  int base[3];
+  float frac[3];
+  departureCell(s.P, cc, pos, dep, base, frac);
+  Donors dn;
+  float4 v = predict(c, s.p.F, base, frac, dn);
+  // Coarse smoke: the fine X channel only stages the step's smoke production
+  // (fluid-scalar-transport.wgsl:127, :137, :191-199).
+  *out = make_float4(s.P.coarse ? 0.f : v.x, v.y, s.P.lifetimeZero ? 0.f : v.z, v.w);
+  float4 *don = s.p.Don + slotBase(slot * 3, s.L.sF) + l;
+  don[0] = make_float4(cv[0], cv[1], cv[2], dn.lmn);
+  don[nF] = make_float4(dn.mn[0], dn.mn[1], dn.mn[2], dn.lmx);
+  don[2 * nF] = make_float4(dn.mx[0], dn.mx[1], dn.mx[2], 0.f);
 }
 FP_KERNEL_PAIR1(advect_scalars, 256, StepArgs, int, chunks)
 
 
 // correctedFineScalars (fluid-scalar-transport.wgsl:144-188), with the donor bounds that
 // advection left in the backtrace planes (`dn`).
-__device__ __forceinline__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p, const int id[3], int slot, int l,
-                                   const float pos[3], const float cv[3], const Donors &dn) {
+__device__ __forceinline__ bool uniformDonors(const Donors &dn) {
+  return dn.mn[0] == dn.mx[0] && dn.mn[1] == dn.mx[1] && dn.mn[2] == dn.mx[2] && dn.lmn == dn.lmx;
+}
+__device__ __forceinline__ float4 correctedScalars(const Ctx &c, const VelWindow &w, const Params &P, const Pools &p,
+                                   const int id[3], int slot, int l, const float pos[3], const float cv[3],
+                                   const Donors &dn) {
   const float4 adv = p.Fp[slotBase(slot, c.L.sF) + l];
   if (!P.maccormack) return adv;
-  bool same = dn.mn[0] == dn.mx[0] && dn.mn[1] == dn.mx[1] && dn.mn[2] == dn.mx[2] && dn.lmn == dn.lmx;
-  if (same) return make_float4(dn.mn[0], dn.mn[1], clampv(dn.lmn, 0.f, 1.f), dn.mn[2]);
+  if (uniformDonors(dn)) return make_float4(dn.mn[0], dn.mn[1], clampv(dn.lmn, 0.f, 1.f), dn.mn[2]);
   float fw[3];
-  traceForward(c, P, p.V, pos, cv, fw);
+  traceForward(c, w, P, p.V, pos, cv, fw);
   const float4 orig = p.F[slotBase(slot, c.L.sF) + l];
   float lifetime = adv.z;
   float rev[3];
@@ -432,15 +465,15 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
   const Ramps ramps{knots, knotCounts};
   uint32_t myBusy = 0u, myExpands = 0u;
   const float burnKeep = P.burnKeep, lossS = P.lossS, lossH = P.lossH, halfS = P.halfS, halfH = P.halfH;
-  for (int l = chunk * 256 + threadIdx.x; l < nF; l += chunks * 256) {
-    int cc[3];
-    fieldCellCoords(c, l, cc);
-    float pos[3] = {((float)cc[0] + 0.5f) * P.hf, ((float)cc[1] + 0.5f) * P.hf, ((float)cc[2] + 0.5f) * P.hf};
-    float value[4] = {0.f, 0.f, 0.f, 0.f};
-    float expansion = 0.f;
-    bool injected = false;
-    float cv[3];
-    Donors dn;
+  // One cell a lane (chunkSlab). The velocity window is staged only when some lane samples
+  // the velocity: every lane without MacCormack, else the lanes whose donors differ.
+  (void)chunks;
+  const int l = chunk * 256 + threadIdx.x;
+  const bool live = l < nF;
+  float cv[3];
+  Donors dn;
+  int sampling = 0;
+  if (live) {
     if (P.maccormack) {
       // The cell's velocity and donor bounds from advection's backtrace.
       const float4 *don = s.p.Don + slotBase(slot * 3, s.L.sF) + l;
@@ -456,11 +489,30 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
       dn.mx[0] = d2.x;
       dn.mx[1] = d2.y;
       dn.mx[2] = d2.z;
+      sampling = !uniformDonors(dn);
     } else {
-      velocityAt(c, P, s.p.V, pos[0], pos[1], pos[2], cv);
+      sampling = 1;
     }
+  }
+  __shared__ float vwin[3 * VW_CELLS];
+  VelWindow w;
+  if (__syncthreads_or(sampling)) {
+    int zlo, zhi;
+    chunkSlab(s.L, chunk, zlo, zhi);
+    stageVelWindow(c, s.p.V, zlo, zhi, vwin, w);
+  } else {
+    velWindowOff(vwin, w);
+  }
+  if (live) {
+    int cc[3];
+    fieldCellCoords(c, l, cc);
+    float pos[3] = {((float)cc[0] + 0.5f) * P.hf, ((float)cc[1] + 0.5f) * P.hf, ((float)cc[2] + 0.5f) * P.hf};
+    float value[4] = {0.f, 0.f, 0.f, 0.f};
+    float expansion = 0.f;
+    bool injected = false;
+    if (!P.maccormack) velocityAt(c, w, P, s.p.V, pos[0], pos[1], pos[2], cv);
     if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
-      float4 tr = correctedScalars(c, P, s.p, cc, slot, l, pos, cv, dn);
+      float4 tr = correctedScalars(c, w, P, s.p, cc, slot, l, pos, cv, dn);
       if (P.coarse) tr.x = 0.f;
       float lifetime = tr.z;
       float sp0 = 0.f, sp1 = 0.f, injectedFuel = 0.f;

@@ -201,15 +201,10 @@ __device__ __forceinline__ float4 sampleField(const Ctx &c, const float4 *buf, f
       }
   return out;
 }
-__device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int comp, float px, float py,
-                                           float pz) {
-  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
-  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
-  int bx = (int)flx, by = (int)fly, bz = (int)flz;
-  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+// The trilinear sum of eight velocity cells, corner k = (z * 2 + y) * 2 + x weighted by
+// (wx * wy) * wz, accumulated in corner order.
+__device__ __forceinline__ float trilinear(const float cv[8], float fx, float fy, float fz) {
   float out = 0.f;
-  float cv[8];
-  velCorners(c, buf, comp, bx, by, bz, cv);
 #pragma unroll
   for (int z = 0; z < 2; z++)
 #pragma unroll
@@ -220,6 +215,16 @@ __device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int c
         out += cv[(z * 2 + y) * 2 + x] * (wx * wy * wz);
       }
   return out;
+}
+__device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int comp, float px, float py,
+                                           float pz) {
+  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
+  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+  int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+  float cv[8];
+  velCorners(c, buf, comp, bx, by, bz, cv);
+  return trilinear(cv, fx, fy, fz);
 }
 // sampleVel, also returning the range of the eight cells it weighs: [0, 0] when the first
 // cell's brick has no slot (the donor bounds of fluid-velocity-transport.wgsl:104-112).
@@ -265,6 +270,97 @@ __device__ __forceinline__ void velocityAt(const Ctx &c, const Params &P, const 
   out[1] = sampleVel(c, V, 1, cx, (wy - 0.5f * P.hv) * P.invHv, cz);
   out[2] = sampleVel(c, V, 2, cx, cy, (wz - 0.5f * P.hv) * P.invHv);
 }
+// ---- the velocity window ------------------------------------------------------------------
+// The field cells of one workgroup, a z-slab of its brick, sample the velocity planes close
+// to that slab: at rest a cell's trilinear footprint reaches one velocity cell past the
+// slab's own, and a midpoint trace moves it by 0.5 dt |v| / hv cells. stageVelWindow copies
+// the slab's velocity cells and VW_HALO more on every side, all three planes, into LDS once
+// per workgroup (zero where a brick has no slot, as velCorners reads such cells); a sample
+// whose eight cells lie inside reads them there, any other one reads the pools. The values,
+// and so every result, are the same either way: the window saves each corner's slot
+// resolution, its 64-bit address and its global load.
+constexpr int VW_HALO = 2;
+constexpr int VW_CELLS = 768;  // 9 KB of LDS: bricks of 8 and 16 cells at velocityDivisor 2 fit
+struct VelWindow {
+  const float *v;  // [3][n], x fastest
+  int o[3];        // the velocity cell at window cell (0, 0, 0)
+  int e[3];        // extents; {1, 1, 1} when the window is off, so that no sample lies inside
+  int sy, sz, n;   // row and layer strides, cells per plane
+};
+__device__ __forceinline__ void velWindowOff(float *lds, VelWindow &w) {
+  w.v = lds;
+  w.o[0] = w.o[1] = w.o[2] = 0;
+  w.e[0] = w.e[1] = w.e[2] = 1;
+  w.sy = w.sz = 1;
+  w.n = 0;
+}
+// The window of the home brick's field cells with local z in [zlo, zhi]. Every thread of the
+// workgroup calls it with the same arguments (it may end in a barrier).
+__device__ __forceinline__ void stageVelWindow(const Ctx &c, const float *V, int zlo, int zhi, float *lds,
+                                               VelWindow &w) {
+  const int b = c.L.b, sV = c.L.sV, dsh = c.L.sF - c.L.sV;
+  const int ex = b + 2 * VW_HALO, ez = (zhi >> dsh) - (zlo >> dsh) + 1 + 2 * VW_HALO;
+  const int n = ex * ex * ez;
+  // The halo must stay inside the home brick's 3^3 neighborhood, whose slots are staged.
+  if (b < VW_HALO || n > VW_CELLS) {
+    velWindowOff(lds, w);
+    return;
+  }
+  w.v = lds;
+  w.o[0] = c.home.bx * b - VW_HALO;
+  w.o[1] = c.home.by * b - VW_HALO;
+  w.o[2] = c.home.bz * b + (zlo >> dsh) - VW_HALO;
+  w.e[0] = w.e[1] = ex;
+  w.e[2] = ez;
+  w.sy = ex;
+  w.sz = ex * ex;
+  w.n = n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int t = i / ex;
+    const int x = w.o[0] + (i - t * ex), y = w.o[1] + (t % ex), z = w.o[2] + t / ex;
+    const int s = c.home.slot[((z >> sV) - c.home.bz + 1) << 4 | ((y >> sV) - c.home.by + 1) << 2 |
+                              ((x >> sV) - c.home.bx + 1)];
+    const int local = cellIndex(sV, x, y, z);
+#pragma unroll
+    for (int k = 0; k < 3; k++) lds[k * n + i] = s < 0 ? 0.f : V[slotBase(s * 3 + k, sV) + local];
+  }
+  __syncthreads();
+}
+// sampleVel of the pool the window was staged from.
+__device__ __forceinline__ float windowVel(const Ctx &c, const VelWindow &w, const float *V, int comp, float px,
+                                           float py, float pz) {
+  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
+  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+  int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+  float cv[8];
+  const int rx = bx - w.o[0], ry = by - w.o[1], rz = bz - w.o[2];
+  if ((unsigned)rx < (unsigned)(w.e[0] - 1) && (unsigned)ry < (unsigned)(w.e[1] - 1) &&
+      (unsigned)rz < (unsigned)(w.e[2] - 1)) {
+    const float *q = w.v + __mul24(comp, w.n) + __mul24(rz, w.sz) + __mul24(ry, w.sy) + rx;
+    const int sy = w.sy, sz = w.sz;
+    cv[0] = q[0];
+    cv[1] = q[1];
+    cv[2] = q[sy];
+    cv[3] = q[sy + 1];
+    cv[4] = q[sz];
+    cv[5] = q[sz + 1];
+    cv[6] = q[sz + sy];
+    cv[7] = q[sz + sy + 1];
+  } else {
+    velCorners(c, V, comp, bx, by, bz, cv);
+  }
+  return trilinear(cv, fx, fy, fz);
+}
+// velocityAt through the window.
+__device__ __forceinline__ void velocityAt(const Ctx &c, const VelWindow &w, const Params &P, const float *V,
+                                           float wx, float wy, float wz, float out[3]) {
+  const float cx = wx * P.invHv, cy = wy * P.invHv, cz = wz * P.invHv;
+  out[0] = windowVel(c, w, V, 0, (wx - 0.5f * P.hv) * P.invHv, cy, cz);
+  out[1] = windowVel(c, w, V, 1, cx, (wy - 0.5f * P.hv) * P.invHv, cz);
+  out[2] = windowVel(c, w, V, 2, cx, cy, (wz - 0.5f * P.hv) * P.invHv);
+}
+
 // velocityOnFace (fluid-advection.wgsl:26-44).
 __device__ __forceinline__ void velocityOnFace(const Ctx &c, const float *V, const int id[3], int comp,
                                                const float orig[3], float out[3]) {
