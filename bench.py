@@ -83,6 +83,13 @@ def parse():
                    help="only summarize the rocprofv3 CSV output under DIR (kernel trace and/or "
                         "counters) over the last --tail-steps solver steps, and exit")
     p.add_argument("--tail-steps", type=int, default=20)
+    p.add_argument("--traffic-summary", metavar="PREFIX", default=None,
+                   help="only turn the FETCH_SIZE and WRITE_SIZE counter runs PREFIX_FETCH_SIZE and "
+                        "PREFIX_WRITE_SIZE into HBM bytes per launch (--copy-dirs, --cells, --out), and exit")
+    p.add_argument("--copy-dirs", default="gpurun_out/copy", metavar="PREFIX",
+                   help="the copy probe's counter runs PREFIX_FETCH_SIZE and PREFIX_WRITE_SIZE")
+    p.add_argument("--cells", type=float, default=0.0, help="active field cells per launch")
+    p.add_argument("--out", default=None, help="JSON output of --traffic-summary")
     p.add_argument("--copy-probe", type=int, default=0,
                    help="only copy this many MiB device to device 5 times (counter calibration)")
     return p.parse_args()
@@ -94,16 +101,10 @@ def profile_summary(root: str, tail: int) -> None:
     step's first kernel, to the end): calls, mean and total duration from the kernel trace,
     and each counter's mean value per dispatch from a counter collection. Kernels of a run
     without solver steps (the copy probe) are summarized whole."""
-    import csv
-    import glob
     from collections import defaultdict
 
     def rows(pattern):
-        out = []
-        for path in sorted(glob.glob(os.path.join(root, "**", pattern), recursive=True)):
-            with open(path, newline="") as f:
-                out.extend(csv.DictReader(f))
-        return out
+        return _profile_rows(root, pattern)
 
     def window(rs):
         # One start per dispatch (a counter collection has a row per counter).
@@ -131,20 +132,70 @@ def profile_summary(root: str, tail: int) -> None:
         for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
             print(f"{k[:48]:48s} {len(v):6d} {sum(v) / len(v) / 1e3:9.2f} {sum(v) / 1e3:10.1f} "
                   f"{100 * sum(v) / total:5.1f}% {vgpr[k]:>5s}")
-    counters = rows("*counter_collection.csv")
-    if counters:
-        t0 = window(counters)
-        agg = defaultdict(lambda: defaultdict(list))
-        for r in counters:
-            if int(r["Start_Timestamp"]) >= t0:
-                agg[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    agg = counter_means(root, tail)
+    if agg:
         print(f"counters, last {tail} steps, mean per dispatch:")
-        names = sorted({c for v in agg.values() for c in v})
+        names = sorted({c for v in agg.values() for c in v if c != "calls"})
         print(f"{'kernel':48s} {'calls':>6s} " + " ".join(f"{c:>16s}" for c in names))
         for k, v in sorted(agg.items()):
-            calls = max(len(x) for x in v.values())
-            print(f"{k[:48]:48s} {calls:6d} " + " ".join(
-                f"{(sum(v[c]) / len(v[c])) if v.get(c) else float('nan'):16.1f}" for c in names))
+            print(f"{k[:48]:48s} {v['calls']:6d} " + " ".join(f"{v.get(c, float('nan')):16.1f}" for c in names))
+
+
+def _profile_rows(root: str, pattern: str):
+    import csv
+    import glob
+    out = []
+    for path in sorted(glob.glob(os.path.join(root, "**", pattern), recursive=True)):
+        with open(path, newline="") as f:
+            out.extend(csv.DictReader(f))
+    return out
+
+
+def counter_means(root: str, tail: int) -> dict:
+    """{kernel: {counter: mean per dispatch, "calls": dispatches}} of a rocprofv3 counter
+    collection under `root`, over the last `tail` solver steps (the whole run without any)."""
+    from collections import defaultdict
+    rs = _profile_rows(root, "*counter_collection.csv")
+    starts = sorted({(int(r["Start_Timestamp"]), r.get("Dispatch_Id", "")) for r in rs
+                     if "k_build_bricks" in r["Kernel_Name"]})
+    t0 = starts[-tail][0] if len(starts) >= tail else (starts[0][0] if starts else 0)
+    agg = defaultdict(lambda: defaultdict(list))
+    for r in rs:
+        if int(r["Start_Timestamp"]) >= t0:
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("fp::", "")
+            agg[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {**{c: sum(x) / len(x) for c, x in v.items()}, "calls": max(len(x) for x in v.values())}
+            for k, v in agg.items()}
+
+
+def traffic_summary(prefix: str, copy: str, cells: float, tail: int, out: str) -> None:
+    """HBM bytes per launch of each profiled kernel from two counter runs (prefix_FETCH_SIZE,
+    prefix_WRITE_SIZE), each scaled by what the same counter reads for the plain copy probe
+    (copy_FETCH_SIZE, copy_WRITE_SIZE: 256 MiB read and written per dispatch); per active
+    field cell with `cells` cells a launch. Writes the table to stdout and JSON to `out`."""
+    fetch = counter_means(prefix + "_FETCH_SIZE", tail)
+    write = counter_means(prefix + "_WRITE_SIZE", tail)
+    kib = 256 * 1024
+    cf = counter_means(copy + "_FETCH_SIZE", tail)
+    cw = counter_means(copy + "_WRITE_SIZE", tail)
+    fscale = kib / next(v["FETCH_SIZE"] for k, v in cf.items() if "copyBuffer" in k)
+    wscale = kib / next(v["WRITE_SIZE"] for k, v in cw.items() if "copyBuffer" in k)
+    kernels = {}
+    for k in sorted(set(fetch) & set(write)):
+        if not k.startswith("k_"):
+            continue
+        f, w = fetch[k]["FETCH_SIZE"], write[k]["WRITE_SIZE"]
+        b = (f * fscale + w * wscale) * 1024
+        kernels[k[2:].split("<")[0]] = {
+            "fetch_kib": round(f, 1), "write_kib": round(w, 1), "bytes_per_launch": b, "bytes_per_cell": b / cells}
+    doc = {"cells": cells, "fetch_scale": fscale, "write_scale": wscale, "kernels": kernels}
+    with open(out, "w") as fo:
+        json.dump(doc, fo, indent=1)
+    print(f"calibration: FETCH_SIZE x {fscale:.5f}, WRITE_SIZE x {wscale:.5f} (256 MiB copy probe)")
+    print(f"{'kernel':34s} {'FETCH KiB':>11s} {'WRITE KiB':>11s} {'MB/launch':>10s} {'B/cell':>8s}")
+    for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["bytes_per_launch"]):
+        print(f"{k[:34]:34s} {v['fetch_kib']:11.1f} {v['write_kib']:11.1f} "
+              f"{v['bytes_per_launch'] / 1e6:10.1f} {v['bytes_per_cell']:8.2f}")
 
 
 def copy_probe(mib: int) -> None:
@@ -408,6 +459,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.traffic_summary:
+        traffic_summary(args.traffic_summary, args.copy_dirs, args.cells, args.tail_steps, args.out)
+        return
     if args.profile_summary:
         profile_summary(args.profile_summary, args.tail_steps)
         return

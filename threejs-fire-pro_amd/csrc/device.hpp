@@ -165,6 +165,13 @@ __device__ __forceinline__ int xcdItem(int w, int count) {
   return i < count ? i : -1;
 }
 
+// The OR of v over the wavefront's lanes, in every lane (all lanes active).
+__device__ __forceinline__ uint32_t waveOr(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off);
+  return v;
+}
+
 // ---- scalar math, identical trees on host and device -----------------------------------
 __device__ __forceinline__ float fminv(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float fmaxv(float a, float b) { return a < b ? b : a; }

@@ -39,23 +39,27 @@ __device__ __forceinline__ void fieldCellCoords(const Ctx &c, int l, int out[3])
 // advectVelocityComponent (fluid-velocity-transport.wgsl:28-41). Also returns the face
 // velocity `fv` and the donor bounds [lo, hi] of the departure cell, which the corrector
 // would otherwise recompute from the same backtrace.
-__device__ __forceinline__ float advectComponent(const Ctx &c, const Params &P, const float *V, const int id[3],
-                                 const float pos[3], int comp, const float orig[3], float fv[3], float &lo,
-                                 float &hi) {
+__device__ __forceinline__ float advectComponent(const Ctx &c, const VelWindow &w, const Params &P, const float *V,
+                                 const int id[3], const float pos[3], int comp, const float orig[3], float fv[3],
+                                 float &lo, float &hi) {
   float off[3] = {0.f, 0.f, 0.f};
   off[comp] = 0.5f * P.hv;
   float fp[3], mid[3], v[3];
   for (int a = 0; a < 3; a++) fp[a] = pos[a] + off[a];
-  velocityOnFace(c, V, id, comp, orig, fv);
+  velocityOnFace(c, w, V, id, comp, orig, fv);
   for (int a = 0; a < 3; a++) mid[a] = fp[a] - 0.5f * P.dt * fv[a];
-  velocityAt(c, P, V, mid[0], mid[1], mid[2], v);
+  velocityAt(c, w, P, V, mid[0], mid[1], mid[2], v);
   float sp[3];
   for (int a = 0; a < 3; a++) sp[a] = (fp[a] - P.dt * v[a] - off[a]) * P.invHv;
-  return sampleVelBounds(c, V, comp, sp[0], sp[1], sp[2], lo, hi);
+  return windowVelBounds(c, w, V, comp, sp[0], sp[1], sp[2], lo, hi);
 }
 
 __device__ __forceinline__ void advect_velocity_body(const StepArgs &s, const int vb) {
   BRICK_PREAMBLE(1)
+  // Every sample reads V: the whole velocity brick's window.
+  __shared__ float vwin[3 * VW_CELLS];
+  VelWindow w;
+  stageVelWindow(c, s.p.V, 0, s.L.B - 1, vwin, w);
   const int nV = s.L.nV;
   float *out = s.p.Vadv + slotBase(slot * 3, s.L.sV);
   const float *vs = s.p.V + slotBase(slot * 3, s.L.sV);
@@ -72,7 +76,7 @@ __device__ __forceinline__ void advect_velocity_body(const StepArgs &s, const in
     float hi[3];
     for (int k = 0; k < 3; k++) {
       float fv[3], lo;
-      out[k * nV + l] = advectComponent(c, s.P, s.p.V, cc, pos, k, orig, fv, lo, hi[k]);
+      out[k * nV + l] = advectComponent(c, w, s.P, s.p.V, cc, pos, k, orig, fv, lo, hi[k]);
       tr[k * nV] = make_float4(fv[0], fv[1], fv[2], lo);
     }
     tr[3 * nV] = make_float4(hi[0], hi[1], hi[2], 0.f);
@@ -180,20 +184,13 @@ __device__ __forceinline__ void traceForward(const Ctx &c, const Params &P, cons
   velocityAt(c, P, V, m[0], m[1], m[2], v);
   for (int a = 0; a < 3; a++) out[a] = pos[a] + P.dt * v[a];
 }
-// The same traces through a workgroup's velocity window.
+// The backward trace through a workgroup's velocity window.
 __device__ __forceinline__ void traceBack(const Ctx &c, const VelWindow &w, const Params &P, const float *V,
                                           const float pos[3], const float cv[3], float out[3]) {
   float m[3], v[3];
   for (int a = 0; a < 3; a++) m[a] = pos[a] - 0.5f * P.dt * cv[a];
   velocityAt(c, w, P, V, m[0], m[1], m[2], v);
   for (int a = 0; a < 3; a++) out[a] = pos[a] - P.dt * v[a];
-}
-__device__ __forceinline__ void traceForward(const Ctx &c, const VelWindow &w, const Params &P, const float *V,
-                                             const float pos[3], const float cv[3], float out[3]) {
-  float m[3], v[3];
-  for (int a = 0; a < 3; a++) m[a] = pos[a] + 0.5f * P.dt * cv[a];
-  velocityAt(c, w, P, V, m[0], m[1], m[2], v);
-  for (int a = 0; a < 3; a++) out[a] = pos[a] + P.dt * v[a];
 }
 __device__ __forceinline__ void departureCell(const Params &P, const int id[3], const float pos[3],
                                               const float dep[3], int base[3], float frac[3]) {
@@ -258,14 +255,13 @@ FP_KERNEL_PAIR1(advect_scalars, 256, StepArgs, int, chunks)
 __device__ __forceinline__ bool uniformDonors(const Donors &dn) {
   return dn.mn[0] == dn.mx[0] && dn.mn[1] == dn.mx[1] && dn.mn[2] == dn.mx[2] && dn.lmn == dn.lmx;
 }
-__device__ __forceinline__ float4 correctedScalars(const Ctx &c, const VelWindow &w, const Params &P, const Pools &p,
-                                   const int id[3], int slot, int l, const float pos[3], const float cv[3],
-                                   const Donors &dn) {
+__device__ __forceinline__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p, const int id[3],
+                                   int slot, int l, const float pos[3], const float cv[3], const Donors &dn) {
   const float4 adv = p.Fp[slotBase(slot, c.L.sF) + l];
   if (!P.maccormack) return adv;
   if (uniformDonors(dn)) return make_float4(dn.mn[0], dn.mn[1], clampv(dn.lmn, 0.f, 1.f), dn.mn[2]);
   float fw[3];
-  traceForward(c, w, P, p.V, pos, cv, fw);
+  traceForward(c, P, p.V, pos, cv, fw);
   const float4 orig = p.F[slotBase(slot, c.L.sF) + l];
   float lifetime = adv.z;
   float rev[3];
@@ -465,14 +461,36 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
   const Ramps ramps{knots, knotCounts};
   uint32_t myBusy = 0u, myExpands = 0u;
   const float burnKeep = P.burnKeep, lossS = P.lossS, lossH = P.lossH, halfS = P.halfS, halfH = P.halfH;
-  // One cell a lane (chunkSlab). The velocity window is staged only when some lane samples
-  // the velocity: every lane without MacCormack, else the lanes whose donors differ.
+  // One cell a lane (chunkSlab). Only the lanes whose donors differ trace forward, through
+  // the pools: a velocity window staged for the whole workgroup costs more than it saves.
   (void)chunks;
   const int l = chunk * 256 + threadIdx.x;
   const bool live = l < nF;
+  // Emitters whose padded box misses every cell center of this slab weigh nothing in it
+  // (emitterWeight's first test, decided here for all the slab's cells at once: w - pos is
+  // monotone in w): a mask of the others among the first 32; later ones are always tested.
+  uint32_t nearby = 0xffffffffu;
+  {
+    int zlo, zhi;
+    chunkSlab(s.L, chunk, zlo, zhi);
+    const int B = s.L.B;
+    const int lo[3] = {c.home.bx * B, c.home.by * B, c.home.bz * B + zlo};
+    const int hi[3] = {lo[0] + B - 1, lo[1] + B - 1, c.home.bz * B + zhi};
+    const int count = P.emitterCount < 32 ? P.emitterCount : 32;
+    for (int i = 0; i < count; i++) {
+      const Emitter &e = s.emitters[i];
+      for (int a = 0; a < 3; a++) {
+        const float pad = fmaxv(e.size[a], 0.03f);
+        const float wlo = ((float)lo[a] + 0.5f) * P.hf, whi = ((float)hi[a] + 0.5f) * P.hf;
+        if (wlo - e.pos[a] >= pad || whi - e.pos[a] <= -pad) {
+          nearby &= ~(1u << i);
+          break;
+        }
+      }
+    }
+  }
   float cv[3];
   Donors dn;
-  int sampling = 0;
   if (live) {
     if (P.maccormack) {
       // The cell's velocity and donor bounds from advection's backtrace.
@@ -489,19 +507,7 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
       dn.mx[0] = d2.x;
       dn.mx[1] = d2.y;
       dn.mx[2] = d2.z;
-      sampling = !uniformDonors(dn);
-    } else {
-      sampling = 1;
     }
-  }
-  __shared__ float vwin[3 * VW_CELLS];
-  VelWindow w;
-  if (__syncthreads_or(sampling)) {
-    int zlo, zhi;
-    chunkSlab(s.L, chunk, zlo, zhi);
-    stageVelWindow(c, s.p.V, zlo, zhi, vwin, w);
-  } else {
-    velWindowOff(vwin, w);
   }
   if (live) {
     int cc[3];
@@ -510,13 +516,14 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     float value[4] = {0.f, 0.f, 0.f, 0.f};
     float expansion = 0.f;
     bool injected = false;
-    if (!P.maccormack) velocityAt(c, w, P, s.p.V, pos[0], pos[1], pos[2], cv);
+    if (!P.maccormack) velocityAt(c, P, s.p.V, pos[0], pos[1], pos[2], cv);
     if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
-      float4 tr = correctedScalars(c, w, P, s.p, cc, slot, l, pos, cv, dn);
+      float4 tr = correctedScalars(c, P, s.p, cc, slot, l, pos, cv, dn);
       if (P.coarse) tr.x = 0.f;
       float lifetime = tr.z;
       float sp0 = 0.f, sp1 = 0.f, injectedFuel = 0.f;
       for (int i = 0; i < P.emitterCount; i++) {
+        if (i < 32 && !((nearby >> i) & 1u)) continue;
         const Emitter &e = s.emitters[i];
         float wgt = emitterWeight(e, pos);
         if (wgt == 0.f) continue;
@@ -563,8 +570,13 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     if (fmaxv(fmaxv(P.coarse ? 0.f : value[0], value[2]), value[3]) > P.cutoff)
       myBusy |= contentSupport(s.L, P, cc, cv);
   }
-  if (myBusy) atomicOr(&busy, myBusy);
-  if (myExpands) atomicOr(&expands, 1u);
+  // One LDS atomic per wavefront: the lanes' masks are OR-reduced across it first.
+  myBusy = waveOr(myBusy);
+  myExpands = waveOr(myExpands);
+  if ((threadIdx.x & 63) == 0) {
+    if (myBusy) atomicOr(&busy, myBusy);
+    if (myExpands) atomicOr(&expands, 1u);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t flags = 0u;

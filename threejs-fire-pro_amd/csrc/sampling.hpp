@@ -226,42 +226,6 @@ __device__ __forceinline__ float sampleVel(const Ctx &c, const float *buf, int c
   velCorners(c, buf, comp, bx, by, bz, cv);
   return trilinear(cv, fx, fy, fz);
 }
-// sampleVel, also returning the range of the eight cells it weighs: [0, 0] when the first
-// cell's brick has no slot (the donor bounds of fluid-velocity-transport.wgsl:104-112).
-__device__ __forceinline__ float sampleVelBounds(const Ctx &c, const float *buf, int comp, float px, float py,
-                                                 float pz, float &lo, float &hi) {
-  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
-  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
-  int bx = (int)flx, by = (int)fly, bz = (int)flz;
-  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
-  int slot[8], local[8];
-  cornerCells(c, c.L.sV, bx, by, bz, slot, local);
-  float cv[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) cv[k] = slot[k] < 0 ? 0.f : buf[slotBase(slot[k] * 3 + comp, c.L.sV) + local[k]];
-  float out = 0.f;
-#pragma unroll
-  for (int z = 0; z < 2; z++)
-#pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int x = 0; x < 2; x++) {
-        float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
-        out += cv[(z * 2 + y) * 2 + x] * (wx * wy * wz);
-      }
-  if (slot[0] < 0) {
-    lo = hi = 0.f;
-  } else {
-    lo = 1e5f;
-    hi = -1e5f;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      lo = fminv(lo, cv[k]);
-      hi = fmaxv(hi, cv[k]);
-    }
-  }
-  return out;
-}
 // velocityAt (fluid-advection.wgsl:10-21).
 __device__ __forceinline__ void velocityAt(const Ctx &c, const Params &P, const float *V, float wx, float wy,
                                            float wz, float out[3]) {
@@ -352,18 +316,56 @@ __device__ __forceinline__ float windowVel(const Ctx &c, const VelWindow &w, con
   }
   return trilinear(cv, fx, fy, fz);
 }
-// velocityAt through the window.
-__device__ __forceinline__ void velocityAt(const Ctx &c, const VelWindow &w, const Params &P, const float *V,
-                                           float wx, float wy, float wz, float out[3]) {
-  const float cx = wx * P.invHv, cy = wy * P.invHv, cz = wz * P.invHv;
-  out[0] = windowVel(c, w, V, 0, (wx - 0.5f * P.hv) * P.invHv, cy, cz);
-  out[1] = windowVel(c, w, V, 1, cx, (wy - 0.5f * P.hv) * P.invHv, cz);
-  out[2] = windowVel(c, w, V, 2, cx, cy, (wz - 0.5f * P.hv) * P.invHv);
+// windowVel, also returning the range of the eight cells it weighs: [0, 0] when the first
+// cell's brick has no slot (the donor bounds of fluid-velocity-transport.wgsl:104-112); for
+// cells inside the window that brick's slot is read from the home record.
+__device__ __forceinline__ float windowVelBounds(const Ctx &c, const VelWindow &w, const float *V, int comp,
+                                                 float px, float py, float pz, float &lo, float &hi) {
+  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
+  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+  int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+  float cv[8];
+  bool none;
+  const int rx = bx - w.o[0], ry = by - w.o[1], rz = bz - w.o[2];
+  if ((unsigned)rx < (unsigned)(w.e[0] - 1) && (unsigned)ry < (unsigned)(w.e[1] - 1) &&
+      (unsigned)rz < (unsigned)(w.e[2] - 1)) {
+    const float *q = w.v + __mul24(comp, w.n) + __mul24(rz, w.sz) + __mul24(ry, w.sy) + rx;
+    const int sy = w.sy, sz = w.sz, sV = c.L.sV;
+    cv[0] = q[0];
+    cv[1] = q[1];
+    cv[2] = q[sy];
+    cv[3] = q[sy + 1];
+    cv[4] = q[sz];
+    cv[5] = q[sz + 1];
+    cv[6] = q[sz + sy];
+    cv[7] = q[sz + sy + 1];
+    none = c.home.slot[((bz >> sV) - c.home.bz + 1) << 4 | ((by >> sV) - c.home.by + 1) << 2 |
+                       ((bx >> sV) - c.home.bx + 1)] < 0;
+  } else {
+    int slot[8], local[8];
+    cornerCells(c, c.L.sV, bx, by, bz, slot, local);
+#pragma unroll
+    for (int k = 0; k < 8; k++) cv[k] = slot[k] < 0 ? 0.f : V[slotBase(slot[k] * 3 + comp, c.L.sV) + local[k]];
+    none = slot[0] < 0;
+  }
+  const float out = trilinear(cv, fx, fy, fz);
+  if (none) {
+    lo = hi = 0.f;
+  } else {
+    lo = 1e5f;
+    hi = -1e5f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      lo = fminv(lo, cv[k]);
+      hi = fmaxv(hi, cv[k]);
+    }
+  }
+  return out;
 }
-
-// velocityOnFace (fluid-advection.wgsl:26-44).
-__device__ __forceinline__ void velocityOnFace(const Ctx &c, const float *V, const int id[3], int comp,
-                                               const float orig[3], float out[3]) {
+// velocityOnFace (fluid-advection.wgsl:26-44) through the window.
+__device__ __forceinline__ void velocityOnFace(const Ctx &c, const VelWindow &w, const float *V, const int id[3],
+                                               int comp, const float orig[3], float out[3]) {
   float fp[3];
   for (int a = 0; a < 3; a++) fp[a] = (float)id[a] + 0.5f;
   fp[comp] += 0.5f;
@@ -374,8 +376,16 @@ __device__ __forceinline__ void velocityOnFace(const Ctx &c, const float *V, con
     }
     float pos[3] = {fp[0], fp[1], fp[2]};
     pos[k] = fp[k] - 0.5f;
-    out[k] = sampleVel(c, V, k, pos[0], pos[1], pos[2]);
+    out[k] = windowVel(c, w, V, k, pos[0], pos[1], pos[2]);
   }
+}
+// velocityAt through the window.
+__device__ __forceinline__ void velocityAt(const Ctx &c, const VelWindow &w, const Params &P, const float *V,
+                                           float wx, float wy, float wz, float out[3]) {
+  const float cx = wx * P.invHv, cy = wy * P.invHv, cz = wz * P.invHv;
+  out[0] = windowVel(c, w, V, 0, (wx - 0.5f * P.hv) * P.invHv, cy, cz);
+  out[1] = windowVel(c, w, V, 1, cx, (wy - 0.5f * P.hv) * P.invHv, cz);
+  out[2] = windowVel(c, w, V, 2, cx, cy, (wz - 0.5f * P.hv) * P.invHv);
 }
 
 }  // namespace fp
