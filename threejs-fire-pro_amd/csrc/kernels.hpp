@@ -196,7 +196,7 @@ FP_DECLARE_MG(mg_restrict_bricks)
 FP_DECLARE_MG(mg_restrict_tiles)
 FP_DECLARE_MG(mg_prolong_bricks)
 FP_DECLARE_MG(mg_prolong_tiles)
-FP_DECLARE_MG(mg_bottom)    // relaxCoarse: 19 sweeps in one workgroup, up to 1024 tile slots
+FP_DECLARE_MG(mg_bottom)    // relaxCoarse: 19 sweeps in one workgroup, up to BOTTOM64_TILES tile slots
 FP_DECLARE_MG(mg_bottom64)  // the same in FP64, up to BOTTOM64_TILES tile slots
 // Defect correction: r = b - A x (FP32 with FP64 row dot products) and per-brick FP64 sums;
 // their totals, the power-of-two scale and the stopping test of cycle `code` (one

@@ -17,7 +17,7 @@ struct MGCtx {
   Home home;
   bool hasTile;
   int tc[3];
-  const int *tnb;  // the home tile's 27 neighbor slots (LDS)
+  const int *tnb;  // the home tile's 27 neighbor slots (LDS, or the tile record itself)
 };
 
 __device__ __forceinline__ int levelShift(const Layout &L, int lv) {
@@ -122,7 +122,7 @@ __device__ __forceinline__ void levelCell(int home0, int home1, int home2, int s
   c.L = m.L;                                       \
   c.side = m.L.side;                               \
   c.hasBrick = true;                               \
-  c.hasTile = false;                               \
+  brickTile(c, ID);                                \
   loadHome(m.dir, ID, rec, c.home, m.L.side);
 
 // Home set-up for a block over one tile slot; returns if the slot is free.
@@ -196,6 +196,17 @@ __device__ __forceinline__ Home homeOf(const Dir &dir, int id, const int *rec) {
   h.slot = rec + 27;
   return h;
 }
+// A brick context's tile neighborhood: the tile holding brick `id` and its 26 neighbor
+// slots, read in place from the tile record, so that cells of the tile levels near the brick
+// resolve their tile slot without a hash probe.
+__device__ __forceinline__ void brickTile(MGCtx &c, int id) {
+  const int *tr = c.dir.tileRec + (size_t)(id >> 6) * TILE_RECORD_WORDS;
+  c.hasTile = true;
+  c.tc[0] = tr[0];
+  c.tc[1] = tr[1];
+  c.tc[2] = tr[2];
+  c.tnb = tr + 4;
+}
 // A group of active bricks with `cells` cells each at the level swept: declares `gr`, the
 // brick index `bi` of this lane (valid when `mine`), and with HOME its MGCtx `c`.
 #define MG_BRICK_GROUP(CELLS)                                                   \
@@ -216,7 +227,7 @@ __device__ __forceinline__ Home homeOf(const Dir &dir, int id, const int *rec) {
   c.L = m.L;                                                                   \
   c.side = m.L.side;                                                           \
   c.hasBrick = true;                                                           \
-  c.hasTile = false;                                                           \
+  brickTile(c, id);                                                            \
   c.home = homeOf(m.dir, id, recs_[gr.j]);                                     \
   const int *rec = recs_[gr.j];
 // A group of tile slots with `cells` cells each: declares `gr`, this lane's tile `t` and,
@@ -622,45 +633,66 @@ __device__ __forceinline__ void mg_prolong_tiles_body(const MGArgsT<typename St:
 }
 FP_MG_PAIR(mg_prolong_tiles, 64)
 
-// relaxCoarse (pressure.wgsl:549-599): every bottom iteration in one workgroup, a tile slot
-// per lane, values in LDS. The result lands in the bottom level's current iterate.
+// relaxCoarse (pressure.wgsl:549-599): every bottom iteration in one workgroup, up to
+// BOTTOM64_TILES / 1024 tile slots per lane (lane, lane + 1024, ...), values in LDS. The
+// result lands in the bottom level's current iterate.
 constexpr int BOTTOM_LANES = 1024;
 template <class St> __device__ __forceinline__ void mg_bottom_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   if (vb != 0) return;
-  __shared__ float value[BOTTOM_LANES];
-  const int lane = threadIdx.x;
+  constexpr int PER = BOTTOM64_TILES / BOTTOM_LANES;
+  __shared__ float value[BOTTOM64_TILES];
   const auto lv = levelView(m, m.bottom, code >> 8);
   const int cap = m.dir.capacity;
-  const int *trec = m.dir.tileRec + (size_t)(lane < cap ? lane : 0) * TILE_RECORD_WORDS;
-  bool valid = lane < cap && trec[3] != 0;
-  int e = valid ? lv.eflag[lane] : 0;
-  bool live = valid && (e & UNKNOWN) && !(e & STORED_SOLID);
-  int flags = e & 63;
-  float rhs = valid ? St::ld(lv.rhs[lane]) : 0.f;
-  float mine = valid ? St::ld(lv.p[lane]) : 0.f;
-  if (valid && !(e & UNKNOWN)) mine = 0.f;
-  value[lane] = mine;
-  // The open sides' neighbor slots, read once (-1: closed or no neighbor).
-  int nbs[6];
-  for (int sd = 0; sd < 6; sd++)
-    nbs[sd] = live && ((flags >> sd) & 1) ? trec[4 + (OFF[sd][2] + 1) * 9 + (OFF[sd][1] + 1) * 3 + (OFF[sd][0] + 1)] : -1;
+  const int per = (cap + BOTTOM_LANES - 1) / BOTTOM_LANES;
+  const float w = m.weight;
+  float mine[PER], rhs[PER];
+  int flags[PER], nbs[PER][6];
+  bool valid[PER], live[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int t = threadIdx.x + k * BOTTOM_LANES;
+    const bool in = k < per && t < cap;
+    const int *trec = m.dir.tileRec + (size_t)(in ? t : 0) * TILE_RECORD_WORDS;
+    valid[k] = in && trec[3] != 0;
+    const int e = valid[k] ? lv.eflag[t] : 0;
+    live[k] = valid[k] && (e & UNKNOWN) && !(e & STORED_SOLID);
+    flags[k] = e & 63;
+    rhs[k] = valid[k] ? St::ld(lv.rhs[t]) : 0.f;
+    mine[k] = valid[k] && (e & UNKNOWN) ? St::ld(lv.p[t]) : 0.f;
+    if (in) value[t] = mine[k];
+    // The open sides' neighbor slots, read once (-1: closed or no neighbor).
+#pragma unroll
+    for (int sd = 0; sd < 6; sd++)
+      nbs[k][sd] = live[k] && ((flags[k] >> sd) & 1)
+                       ? trec[4 + (OFF[sd][2] + 1) * 9 + (OFF[sd][1] + 1) * 3 + (OFF[sd][0] + 1)]
+                       : -1;
+  }
   __syncthreads();
   for (int it = 0; it < BOTTOM_ITERATIONS; it++) {
-    float next = 0.f;
-    if (live) {
-      float n[6];
-      for (int sd = 0; sd < 6; sd++) n[sd] = nbs[sd] >= 0 ? value[nbs[sd]] : 0.f;
-      float w = m.weight;
-      float sum = w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
-      float cand = (sum - rhs) / fmaxv(diagonal(w, flags), 1e-12f);
-      next = mixv(mine, cand, 0.8f);
+    float next[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      next[k] = 0.f;
+      if (live[k]) {
+        float n[6];
+        for (int sd = 0; sd < 6; sd++) n[sd] = nbs[k][sd] >= 0 ? value[nbs[k][sd]] : 0.f;
+        float sum = w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
+        float cand = (sum - rhs[k]) / fmaxv(diagonal(w, flags[k]), 1e-12f);
+        next[k] = mixv(mine[k], cand, 0.8f);
+      }
     }
     __syncthreads();
-    value[lane] = next;
-    mine = next;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int t = threadIdx.x + k * BOTTOM_LANES;
+      mine[k] = next[k];
+      if (k < per && t < cap) value[t] = next[k];
+    }
     __syncthreads();
   }
-  if (valid) lv.p[lane] = St::st(mine);
+#pragma unroll
+  for (int k = 0; k < PER; k++)
+    if (valid[k]) lv.p[threadIdx.x + k * BOTTOM_LANES] = St::st(mine[k]);
 }
 FP_MG_PAIR(mg_bottom, 1024)
 
@@ -774,7 +806,7 @@ __device__ __forceinline__ void dc_residual_body(const MGArgsT<typename St::T> &
     c.L = m.L;
     c.side = m.L.side;
     c.hasBrick = true;
-    c.hasTile = false;
+    brickTile(c, id);
     c.home = homeOf(m.dir, id, recs_[gr.j]);
     const int slot = entrySlot(recs_[gr.j][13], c.side);
     const float *x = m.dcX;

@@ -1618,7 +1618,7 @@ struct fp_sim {
   template <class St> void cycle(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid, bool bottom64, int &flips) {
     if (L == bottom) {
       if (bottom64) k_mg_bottom64<St><<<1, 1024, 0, stream>>>(m, flips << 8);
-      else if (capacity <= 1024) k_mg_bottom<St><<<1, 1024, 0, stream>>>(m, flips << 8);
+      else if (capacity <= BOTTOM64_TILES) k_mg_bottom<St><<<1, 1024, 0, stream>>>(m, flips << 8);
       else
         for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax<St>(m, L, slotsGrid, flips);
       return;
@@ -2530,7 +2530,7 @@ struct fp_batch {
       cZero[i] = slotsGrid * zchunks;
       cField[i] = slotsGrid * fchunks;
       cTiles[i] = s.capacity;
-      smallBottom = smallBottom && s.capacity <= 1024;
+      smallBottom = smallBottom && s.capacity <= BOTTOM64_TILES;
       alloc[i] = s.allocArgs();
       s.solidsStale = false;
       pre[i] = s.stepArgs();
