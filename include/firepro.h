@@ -337,7 +337,9 @@ int fp_batch_last_ms(const fp_batch *batch, double *ms);
 /* Its phases in ms: ms[0] host halves, [1] argument packing, [2] launches, [3] the wait for
  * the device, [4] the activity readbacks. */
 int fp_batch_phases(const fp_batch *batch, double *ms);
-/* 1 (the default) replays captured HIP graphs; 0 launches every kernel on each step. */
+/* 1 (the default) replays captured HIP graphs; 0 launches every kernel on each step. A batch
+ * created under the rocprofv3 tool (ROCP_TOOL_LIBRARIES or a rocprofiler LD_PRELOAD set)
+ * starts with 0. */
 int fp_batch_use_graphs(fp_batch *batch, int32_t enable);
 
 /* The pressure policy of the next steps. Defect correction keeps the solution x in FP32,

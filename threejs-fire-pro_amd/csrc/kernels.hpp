@@ -185,6 +185,7 @@ FP_DECLARE_PAIR1(topology_tiles, MGArgs, int)
 FP_DECLARE_PAIR(eflags_bricks, MGArgs)
 __global__ void k_fresh_topology(MGArgs m);
 FP_DECLARE_PAIR(eflags_tiles, MGArgs)
+__global__ void k_mg_flags_init(MGArgs m, int brickGroups);  // eflags of every level + mg_init, one launch
 #define FP_DECLARE_MG(NAME)                                                                       \
   template <class St> __global__ void k_##NAME(MGArgsT<typename St::T> a, int code);              \
   template <class St>                                                                             \

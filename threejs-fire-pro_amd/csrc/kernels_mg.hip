@@ -273,7 +273,9 @@ __device__ __forceinline__ void topology_brick(const MGArgs &m, const int id) {
   }
 }
 __device__ __forceinline__ void topology_bricks_body(const MGArgs &m, const int vb, int fromZeroList) {
-  if (!fromZeroList && !m.topologyAll) return;
+  // With the geometry changed, the pass over every active brick (launched first) has
+  // covered the zeroed list's bricks that hold a slot.
+  if (fromZeroList ? m.topologyAll : !m.topologyAll) return;
   const int bi = xcdItem(vb, fromZeroList ? m.counts[1] + m.counts[5] : m.counts[0]);
   if (bi < 0) return;
   topology_brick(m, fromZeroList ? m.zeroList[2 * bi] : m.active[bi]);
@@ -289,22 +291,29 @@ __global__ void __launch_bounds__(256) k_fresh_topology(MGArgs m) {
   }
 }
 
-// buildTileTopology (pressure.wgsl:281-305) of one tile level, every tile slot.
-__device__ __forceinline__ void topology_tiles_body(const MGArgs &m, const int vb, int lv) {
+// buildTileTopology (pressure.wgsl:281-305) of one tile level (`only`), or of every tile
+// level (`only` < 0), for every tile slot.
+__device__ __forceinline__ void topology_tiles_body(const MGArgs &m, const int vb, int only) {
   if (!m.topologyAll) return;
   const int t = vb;
   if (t >= m.dir.capacity) return;
-  const int sd = m.lv[lv].side, n = sd * sd * sd;
+  const int first = only < 0 ? m.L.brickLevels : only, last = only < 0 ? m.bottom : only;
   const int *trec0 = m.dir.tileRec + (size_t)t * TILE_RECORD_WORDS;
   if (trec0[3] == 0) {
-    for (int l = threadIdx.x; l < n; l += blockDim.x) m.lv[lv].topo[(size_t)t * n + l] = 0;
+    for (int lv = first; lv <= last; lv++) {
+      const int sd = m.lv[lv].side, n = sd * sd * sd;
+      for (int l = threadIdx.x; l < n; l += blockDim.x) m.lv[lv].topo[(size_t)t * n + l] = 0;
+    }
     return;
   }
   MG_TILE_HOME(t)
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
-    int p[3];
-    levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    m.lv[lv].topo[(size_t)t * n + l] = cellTopology(c, m.P, m.solids, p, lv);
+  for (int lv = first; lv <= last; lv++) {
+    const int sd = m.lv[lv].side, n = sd * sd * sd;
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+      int p[3];
+      levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
+      m.lv[lv].topo[(size_t)t * n + l] = cellTopology(c, m.P, m.solids, p, lv);
+    }
   }
 }
 FP_KERNEL_PAIR1(topology_tiles, 64, MGArgs, int, lv)
@@ -412,6 +421,60 @@ __device__ __forceinline__ float stencilSum(const MGCtx &c, const typename St::T
   return openSum(n, flags, w);
 }
 
+// The step's flags of every level and level 0's first iterate (initializePressure,
+// pressure.wgsl:485-505) in one launch, for the single V-cycle. Workgroups below
+// `brickGroups` take groups of active bricks with one lane per level-0 cell (the grouping of
+// the level-0 sweeps): each lane sets the flags of its cells of every brick level, as
+// eflags_bricks does, and the first iterate of its level-0 cells from the flags it has just
+// set. The workgroups after them take four tile slots each, a wavefront per slot, as
+// eflags_tiles does.
+__global__ void __launch_bounds__(256) k_mg_flags_init(MGArgs m, int brickGroups) {
+  const int vb = (int)blockIdx.x;
+  if (vb >= brickGroups) {
+    const int t = (vb - brickGroups) * 4 + (int)(threadIdx.x >> 6);
+    if (t >= m.dir.capacity) return;
+    if (m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] == 0) return;
+    const int l = (int)(threadIdx.x & 63), id = t * 64 + l;
+    const bool part = m.dir.entries[id] >= 0 && m.dir.records[(size_t)id * RECORD_WORDS + PRESSURE_FLAG] != 0;
+    const uint64_t bricks = __ballot(part);
+    for (int lv = m.L.brickLevels; lv <= m.bottom; lv++) {
+      const int k = 2 - (lv - m.L.brickLevels);
+      if (k < 0) break;
+      const int sd = 1 << k, n = 1 << (3 * k), span = 4 >> k;
+      if (l >= n) continue;
+      const int cc[3] = {(l & (sd - 1)) * span, ((l >> k) & (sd - 1)) * span, (l >> (2 * k)) * span};
+      bool all = true;
+      for (int z = 0; z < span; z++)
+        for (int y = 0; y < span; y++)
+          for (int x = 0; x < span; x++)
+            all = all && ((bricks >> (((cc[2] + z) * 4 + cc[1] + y) * 4 + cc[0] + x)) & 1u);
+      const size_t a = (size_t)t * n + l;
+      m.lv[lv].eflag[a] = (uint8_t)(m.lv[lv].topo[a] | (all ? UNKNOWN : 0));
+    }
+    return;
+  }
+  const int n0 = m.lv[0].side * m.lv[0].side * m.lv[0].side;
+  MG_BRICK_GROUP(n0)
+  MG_GROUP_HOME
+  const int slot = entrySlot(rec[13], c.side);
+  for (int lv = 0; lv < m.L.brickLevels; lv++) {
+    const int sd = m.lv[lv].side, n = sd * sd * sd;
+    for (int l = gr.first; l < n; l += gr.step) {
+      int p[3];
+      levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
+      const size_t a = (size_t)slot * n + l;
+      const int e = m.lv[lv].topo[a] | (activeCell(c, m.P, p, lv) ? UNKNOWN : 0);
+      m.lv[lv].eflag[a] = (uint8_t)e;
+      if (lv == 0) {
+        float first = 0.f;
+        if (!(e & STORED_SOLID) && (e & UNKNOWN))
+          first = mixv(0.f, (0.f - m.lv[0].rhs[a]) / fmaxv(diagonal(m.weight, e & 63), 1e-12f), 0.8f);
+        m.lv[0].p[a] = first;
+      }
+    }
+  }
+}
+
 // relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8, p to q.
 template <class St>
 __device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelView<typename St::T> &lv, int level, const int p[3],
@@ -494,28 +557,63 @@ FP_MG_PAIR(mg_relax_tiles, 64)
 
 // restrictCell (pressure.wgsl:601-647): the coarse right-hand side is half the sum of the
 // eight children's residuals; the coarse first iterate is one Jacobi step from zero.
+// Every fine level's cube (a brick's or a tile's cells of that level) has an even side, so
+// the eight children share one cube: one owner lookup covers them, and the 24 cells beside
+// their 2^3 block resolve their cube once per face. All reads are issued before any flag is
+// tested; the residuals are then added in child order, as one child at a time would.
 template <class St>
 __device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, int flips,
                                              const int id[3]) {
   const auto fine = levelView(m, level - 1, flips);
   const auto coarse = levelView(m, level, flips);
+  const int sd = fine.side, msk = sd - 1;
+  const int p0[3] = {id[0] * 2, id[1] * 2, id[2] * 2};
+  const long a0 = addrAt(c, p0, level - 1);
+  const int stride[3] = {1, sd, sd * sd};
+  // Face f (+x, -x, +y, -y, +z, -z) holds four cells, j = u + 2 v over its two other axes
+  // in axis order.
+  float side[6][4];
+#pragma unroll
+  for (int f = 0; f < 6; f++) {
+    const int ax = f >> 1, step = (f & 1) ? -1 : 2;
+    const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
+    long qa;
+    if ((unsigned)((p0[ax] & msk) + step) < (unsigned)sd) {
+      qa = a0 < 0 ? -1 : a0 + step * stride[ax];
+    } else {
+      int q[3] = {p0[0], p0[1], p0[2]};
+      q[ax] += step;
+      qa = addrAt(c, q, level - 1);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      side[f][j] = qa < 0 ? 0.f : St::ld(fine.p[qa + (j & 1) * stride[u] + (j >> 1) * stride[v]]);
+  }
+  float own[8], rhs8[8];
+  int e[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const long a = a0 + (k & 1) + ((k >> 1) & 1) * stride[1] + (k >> 2) * stride[2];
+    e[k] = a0 < 0 ? 0 : fine.eflag[a];
+    own[k] = a0 < 0 ? 0.f : St::ld(fine.p[a]);
+    rhs8[k] = a0 < 0 ? 0.f : St::ld(fine.rhs[a]);
+  }
   float residual = 0.f;
   bool every = true;
-  for (int z = 0; z < 2; z++)
-    for (int y = 0; y < 2; y++)
-      for (int x = 0; x < 2; x++) {
-        int p[3] = {id[0] * 2 + x, id[1] * 2 + y, id[2] * 2 + z};
-        long a = addrAt(c, p, level - 1);
-        int e = a < 0 ? 0 : fine.eflag[a];
-        bool unknown = (e & UNKNOWN) != 0;
-        every = every && unknown;
-        if (!unknown) continue;
-        if (e & STORED_SOLID) continue;
-        int flags = e & 63;
-        float sum = stencilSum<St>(c, fine.p, p, flags, level - 1, m.weight, a, fine.side);
-        float dg = diagonal(m.weight, flags);
-        residual += St::ld(fine.rhs[a]) - (sum - dg * St::ld(fine.p[a]));
-      }
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int x = k & 1, y = (k >> 1) & 1, z = k >> 2;
+    const bool unknown = (e[k] & UNKNOWN) != 0;
+    every = every && unknown;
+    if (!unknown || (e[k] & STORED_SOLID)) continue;
+    const int flags = e[k] & 63;
+    const float n[6] = {x ? side[0][y + 2 * z] : own[k + 1], x ? own[k - 1] : side[1][y + 2 * z],
+                        y ? side[2][x + 2 * z] : own[k + 2], y ? own[k - 2] : side[3][x + 2 * z],
+                        z ? side[4][x + 2 * y] : own[k + 4], z ? own[k - 4] : side[5][x + 2 * y]};
+    const float sum = openSum(n, flags, m.weight);
+    const float dg = diagonal(m.weight, flags);
+    residual += rhs8[k] - (sum - dg * own[k]);
+  }
   float rhs = residual * 0.5f;
   long out = addrAt(c, id, level);
   coarse.rhs[out] = St::st(rhs);
@@ -577,70 +675,147 @@ __device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typen
   return sum / fmaxv(count, 1.0f);
 }
 
-// prolongateCell (pressure.wgsl:726-755): trilinear correction from the coarse level, p to q.
+// The coarse value a prolongation weighs at coarse cell cc: coarseValue, which without
+// colliders is the value at cc with its y raised to the floor (no cell at or above the floor
+// is solid then), so that read alone.
 template <class St>
-__device__ __forceinline__ void prolongCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, int flips,
-                                            const int p[3], size_t a) {
-  const auto lv = levelView(m, level, flips);
+__device__ __forceinline__ float prolongValue(const MGCtx &c, const MGArgsT<typename St::T> &m,
+                                              const LevelView<typename St::T> &cv, const int cc[3], int level) {
+  if (m.P.colliders != 0) return coarseValue<St>(c, m, cv, cc, level);
+  const int q[3] = {cc[0], m.P.ground && cc[1] < 0 ? 0 : cc[1], cc[2]};
+  return readAt<St>(c, cv.p, q, level);
+}
+
+// prolongateCell (pressure.wgsl:726-755): trilinear correction from the coarse level, p to
+// q. The fine cells of an item (a brick's or a tile's cube of `sd` cells a side) weigh the
+// coarse cells from o - 1 to o + sd / 2 on each axis, o = its first fine cell / 2: those
+// (sd / 2 + 2)^3 values are resolved once per item into LDS (`vals`, x fastest), each by
+// one lane, before any fine cell interpolates them.
+constexpr int PROLONG_VALUES = MG_GROUP * 27;  // the largest group: 32 items of 2^3 cells
+template <class St>
+__device__ __forceinline__ void stageProlong(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, int flips,
+                                             const int org[3], int sd, int first, int step, float *vals) {
   const auto cv = levelView(m, level + 1, flips);
+  const int rs = sd / 2 + 2, rn = rs * rs * rs;
+  for (int r = first; r < rn; r += step) {
+    const int t = r / rs;
+    const int cc[3] = {org[0] * (sd / 2) - 1 + (r - t * rs), org[1] * (sd / 2) - 1 + t % rs,
+                       org[2] * (sd / 2) - 1 + t / rs};
+    vals[r] = prolongValue<St>(c, m, cv, cc, level + 1);
+  }
+}
+template <class St>
+__device__ __forceinline__ void prolongCell(const MGArgsT<typename St::T> &m, int level, int flips, const int org[3],
+                                            int sd, const float *vals, const int p[3], size_t a) {
+  const auto lv = levelView(m, level, flips);
   int e = lv.eflag[a];
+  const float own = St::ld(lv.p[a]);
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
     lv.q[a] = St::st(0.f);
     return;
   }
+  const int rs = sd / 2 + 2;
   float f[3];
   int i[3];
   for (int k = 0; k < 3; k++) {
     float q = ((float)p[k] + 0.5f) * 0.5f - 0.5f;
     float fl = floorf(q);
-    i[k] = (int)fl;
+    i[k] = (int)fl - (org[k] * (sd / 2) - 1);
     f[k] = q - fl;
   }
-  const bool plain = m.P.colliders == 0 && (!m.P.ground || i[1] >= 0);
+  const float *b = vals + (i[2] * rs + i[1]) * rs + i[0];
   float v[8];
-  for (int k = 0; k < 8; k++) {
-    int cc[3] = {i[0] + (k & 1), i[1] + ((k >> 1) & 1), i[2] + (k >> 2)};
-    v[k] = plain ? readAt<St>(c, cv.p, cc, level + 1) : coarseValue<St>(c, m, cv, cc, level + 1);
-  }
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = b[((k >> 2) * rs + ((k >> 1) & 1)) * rs + (k & 1)];
   float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
                     mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
-  lv.q[a] = St::st(St::ld(lv.p[a]) + corr);
+  lv.q[a] = St::st(own + corr);
 }
 template <class St>
 __device__ __forceinline__ void mg_prolong_bricks_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
-  const int level = code & 255;
+  const int level = code & 255, flips = code >> 8;
   const int sd = m.lv[level].side, n = sd * sd * sd;
   MG_BRICK_GROUP(n)
-  MG_GROUP_HOME
-  const int slot = entrySlot(rec[13], c.side);
+  __shared__ int recs_[MG_GROUP][HOME_WORDS];
+  __shared__ float vals[PROLONG_VALUES];
+  loadGroupHomes(m.dir, m.active, count_, gr, recs_, m.L.side);
+  const int rn = (sd / 2 + 2) * (sd / 2 + 2) * (sd / 2 + 2);
+  float *mine_vals = vals + (gr.j < 0 ? 0 : gr.j) * rn;
+  MGCtx c;
+  c.dir = m.dir;
+  c.L = m.L;
+  c.side = m.L.side;
+  c.hasBrick = true;
+  int org[3] = {0, 0, 0}, slot = -1;
+  if (mine) {
+    const int id = m.active[bi];
+    brickTile(c, id);
+    c.home = homeOf(m.dir, id, recs_[gr.j]);
+    org[0] = c.home.bx;
+    org[1] = c.home.by;
+    org[2] = c.home.bz;
+    slot = entrySlot(recs_[gr.j][13], c.side);
+    stageProlong<St>(c, m, level, flips, org, sd, gr.first, gr.step, mine_vals);
+  }
+  __syncthreads();
+  if (!mine) return;
   for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
-    levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-    prolongCell<St>(c, m, level, code >> 8, p, (size_t)slot * n + l);
+    levelCell(org[0], org[1], org[2], sd, l, p);
+    prolongCell<St>(m, level, flips, org, sd, mine_vals, p, (size_t)slot * n + l);
   }
 }
 FP_MG_PAIR(mg_prolong_bricks, 256)
 template <class St>
 __device__ __forceinline__ void mg_prolong_tiles_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
-  const int level = code & 255;
+  const int level = code & 255, flips = code >> 8;
   const int sd = m.lv[level].side, n = sd * sd * sd;
-  MG_TILE_GROUP(n)
+  const int k_ = groupSize(n, (int)blockDim.x);
+  if (vb * k_ >= m.dir.capacity) return;
+  const Group gr = groupOf(vb, n);
+  const int t = gr.g * gr.k + gr.j;
+  __shared__ int tnbs_[MG_GROUP][27];
+  __shared__ float vals[PROLONG_VALUES];
+  for (int e = threadIdx.x; e < gr.k * 27; e += blockDim.x) {
+    const int jj = e / 27, w = e - jj * 27, tt = gr.g * gr.k + jj;
+    if (tt < m.dir.capacity) tnbs_[jj][w] = m.dir.tileRec[(size_t)tt * TILE_RECORD_WORDS + 4 + w];
+  }
+  __syncthreads();
+  const int rn = (sd / 2 + 2) * (sd / 2 + 2) * (sd / 2 + 2);
+  float *mine_vals = vals + (gr.j < 0 ? 0 : gr.j) * rn;
+  const int *trec = m.dir.tileRec + (size_t)(t < m.dir.capacity ? t : 0) * TILE_RECORD_WORDS;
+  const bool live = gr.j >= 0 && t < m.dir.capacity && trec[3] != 0;
+  MGCtx c;
+  c.dir = m.dir;
+  c.L = m.L;
+  c.side = m.L.side;
+  c.hasBrick = false;
+  c.hasTile = true;
+  c.tc[0] = trec[0];
+  c.tc[1] = trec[1];
+  c.tc[2] = trec[2];
+  c.tnb = tnbs_[gr.j < 0 ? 0 : gr.j];
+  if (live) stageProlong<St>(c, m, level, flips, c.tc, sd, gr.first, gr.step, mine_vals);
+  __syncthreads();
+  if (!live) return;
   for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    prolongCell<St>(c, m, level, code >> 8, p, (size_t)t * n + l);
+    prolongCell<St>(m, level, flips, c.tc, sd, mine_vals, p, (size_t)t * n + l);
   }
 }
 FP_MG_PAIR(mg_prolong_tiles, 64)
 
 // relaxCoarse (pressure.wgsl:549-599): every bottom iteration in one workgroup, up to
 // BOTTOM64_TILES / 1024 tile slots per lane (lane, lane + 1024, ...), values in LDS. The
-// result lands in the bottom level's current iterate.
+// result lands in the bottom level's current iterate. Sweeps alternate between two LDS
+// copies (read one, write the other), so one barrier separates them.
 constexpr int BOTTOM_LANES = 1024;
 template <class St> __device__ __forceinline__ void mg_bottom_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   if (vb != 0) return;
   constexpr int PER = BOTTOM64_TILES / BOTTOM_LANES;
-  __shared__ float value[BOTTOM64_TILES];
+  __shared__ float values[2][BOTTOM64_TILES];
+  float *value = values[0];
   const auto lv = levelView(m, m.bottom, code >> 8);
   const int cap = m.dir.capacity;
   const int per = (cap + BOTTOM_LANES - 1) / BOTTOM_LANES;
@@ -669,24 +844,21 @@ template <class St> __device__ __forceinline__ void mg_bottom_body(const MGArgsT
   }
   __syncthreads();
   for (int it = 0; it < BOTTOM_ITERATIONS; it++) {
-    float next[PER];
+    const float *cur = values[it & 1];
+    float *next = values[(it & 1) ^ 1];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-      next[k] = 0.f;
+      float out = 0.f;
       if (live[k]) {
         float n[6];
-        for (int sd = 0; sd < 6; sd++) n[sd] = nbs[k][sd] >= 0 ? value[nbs[k][sd]] : 0.f;
+        for (int sd = 0; sd < 6; sd++) n[sd] = nbs[k][sd] >= 0 ? cur[nbs[k][sd]] : 0.f;
         float sum = w * (n[0] + n[1]) + w * (n[2] + n[3]) + w * (n[4] + n[5]);
         float cand = (sum - rhs[k]) / fmaxv(diagonal(w, flags[k]), 1e-12f);
-        next[k] = mixv(mine[k], cand, 0.8f);
+        out = mixv(mine[k], cand, 0.8f);
       }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
       const int t = threadIdx.x + k * BOTTOM_LANES;
-      mine[k] = next[k];
-      if (k < per && t < cap) value[t] = next[k];
+      mine[k] = out;
+      if (k < per && t < cap) next[t] = out;
     }
     __syncthreads();
   }

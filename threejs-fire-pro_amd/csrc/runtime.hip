@@ -13,6 +13,7 @@
 #include <chrono>
 #include <functional>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1479,22 +1480,28 @@ struct fp_sim {
     // The brick levels hold at most nV cells per brick.
     const unsigned vLanes = (unsigned)std::min(256, std::max(64, nV));
     if (geometryDirty) {
+      // Every active brick, the step's fresh ones included, and every tile level at once.
       k_topology_bricks<<<slotsGrid, vLanes, 0, stream>>>(m, 0);
-      for (int L = brickLevels; L <= bottom; L++) k_topology_tiles<<<capacity, 64, 0, stream>>>(m, L);
+      if (bottom >= brickLevels) k_topology_tiles<<<capacity, 64, 0, stream>>>(m, -1);
       geometryDirty = false;
+    } else {
+      k_fresh_topology<<<std::min(slotsGrid * 2, FRESH_GRID), vLanes, 0, stream>>>(m);
     }
-    k_fresh_topology<<<std::min(slotsGrid * 2, FRESH_GRID), vLanes, 0, stream>>>(m);
-    k_eflags_bricks<<<slotsGrid, vLanes, 0, stream>>>(m);
-    if (bottom >= brickLevels) k_eflags_tiles<<<capacity, 64, 0, stream>>>(m);
     if (pressureMode == 0) {
+      // The flags of every level and level 0's first iterate in one launch.
+      const unsigned groups = groupGrid(0);
+      const unsigned tileBlocks = bottom >= brickLevels ? (unsigned)(capacity + 3) / 4 : 0u;
+      k_mg_flags_init<<<groups + tileBlocks, 256, 0, stream>>>(m, (int)groups);
       int flips = 0;
-      cycle<StoreF32>(m, 0, slotsGrid, false, flips);
+      cycle<StoreF32>(m, 0, slotsGrid, false, flips, true);
       latestPressure = (flips & 1) ? m.lv[0].q : m.lv[0].p;  // P0 or P1
       latestPressureLo = nullptr;
       lastCycles = 1;
       lastResidual = -1.0;
       return;
     }
+    k_eflags_bricks<<<slotsGrid, vLanes, 0, stream>>>(m);
+    if (bottom >= brickLevels) k_eflags_tiles<<<capacity, 64, 0, stream>>>(m);
     ensureCorrectionBuffers();
     if (pressureMode == 1) correct<StoreF32>(correctionArgs32(m), slotsGrid);
     else if (pressureMode == 2) correct<StoreBF16>(correctionArgs16<uint16_t>(m), slotsGrid);
@@ -1615,7 +1622,9 @@ struct fp_sim {
     else k_mg_relax_tiles<St><<<capacity, 64, 0, stream>>>(m, L | flips << 8);
     flips ^= 1 << L;
   }
-  template <class St> void cycle(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid, bool bottom64, int &flips) {
+  // With `initDone`, level 0's first iterate is already in place (k_mg_flags_init).
+  template <class St>
+  void cycle(MGArgsT<typename St::T> &m, int L, unsigned slotsGrid, bool bottom64, int &flips, bool initDone = false) {
     if (L == bottom) {
       if (bottom64) k_mg_bottom64<St><<<1, 1024, 0, stream>>>(m, flips << 8);
       else if (capacity <= BOTTOM64_TILES) k_mg_bottom<St><<<1, 1024, 0, stream>>>(m, flips << 8);
@@ -1623,8 +1632,9 @@ struct fp_sim {
         for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax<St>(m, L, slotsGrid, flips);
       return;
     }
-    if (L == 0) k_mg_init<St><<<groupGrid(0), 256, 0, stream>>>(m, flips << 8);
-    else {
+    if (L == 0) {
+      if (!initDone) k_mg_init<St><<<groupGrid(0), 256, 0, stream>>>(m, flips << 8);
+    } else {
       relax<St>(m, L, slotsGrid, flips);
       relax<St>(m, L, slotsGrid, flips);
     }
@@ -2236,6 +2246,16 @@ struct fp_batch {
   void create() {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipHostMalloc((void **)&pinnedLive, sizeof(int), hipHostMallocDefault));
+    graphs = !underProfiler();
+  }
+  // rocprofv3 loads its tool library into the process through ROCP_TOOL_LIBRARIES (and
+  // LD_PRELOAD). Its kernel tracing of replayed graphs faulted on the host side once, inside
+  // the profiler's own runtime, so a batch starts with graphs off when the tool is present:
+  // the kernels then run one launch at a time, in the same order with the same arguments.
+  static bool underProfiler() {
+    const char *tools = std::getenv("ROCP_TOOL_LIBRARIES");
+    const char *preload = std::getenv("LD_PRELOAD");
+    return (tools && *tools) || (preload && std::strstr(preload, "rocprofiler"));
   }
   void dropGraphs() {
     for (auto &kv : execs) (void)hipGraphExecDestroy(kv.second);
@@ -2459,8 +2479,7 @@ struct fp_batch {
     launch<StepArgs>(k_divergence_batch, vLanes, lay.post, lay.slots, wide());
     // Pressure: topology, flags, then one V-cycle or defect correction.
     launch<MGArgs>(k_topology_bricks_batch, vLanes, lay.mg, lay.slots, wide(), 0);
-    for (int L = shape.brickLevels; L <= shape.bottom; L++)
-      launch<MGArgs>(k_topology_tiles_batch, 64, lay.mg, lay.tiles, wide(), L);
+    if (shape.bottom >= shape.brickLevels) launch<MGArgs>(k_topology_tiles_batch, 64, lay.mg, lay.tiles, wide(), -1);
     launch<MGArgs>(k_topology_bricks_batch, vLanes, lay.mg, lay.slots2, wide(), 1);
     launch<MGArgs>(k_eflags_bricks_batch, vLanes, lay.mg, lay.slots, wide());
     if (shape.bottom >= shape.brickLevels) launch<MGArgs>(k_eflags_tiles_batch, 64, lay.mg, lay.tiles, wide());
