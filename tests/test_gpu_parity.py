@@ -24,6 +24,13 @@ def test_small_scene_bit_exact(brick, divisor):
     assert run_lockstep(sc) == []
 
 
+def test_tilted_colliders_bit_exact():
+    """A box turned about all three axes and a sphere in the plume: their solids, and the
+    cells the solver treats as solid through them, bit-exact."""
+    print("case small tilted colliders", flush=True)
+    assert run_lockstep(_scenes().small(steps=8, tilted=True)) == []
+
+
 @pytest.mark.parametrize("brick,divisor,smoke", [(8, 2, 2), (8, 1, 2), (8, 2, 4), (16, 2, 2), (16, 4, 2), (16, 4, 4)])
 def test_coarse_smoke_bit_exact(brick, divisor, smoke):
     """smokeDivisor 2 and 4 (smoke.wgsl): the coarse density, the fine fields that stage

@@ -69,7 +69,15 @@ struct Params {
   float hs, invHs;
   int smokeRatio;
   int meshRecords;  // mesh emission records this step (0: no mesh sources)
+  // A box of field cells (inclusive) holding every cell any collider makes solid: a solid
+  // bit outside it is never set, so lookups outside it are skipped.
+  int solidLo[3], solidHi[3];
 };
+// Whether field cell (x, y, z) lies in the colliders' box (Params::solidLo/solidHi).
+__device__ __forceinline__ bool inSolidBox(const Params &P, int x, int y, int z) {
+  return x >= P.solidLo[0] && x <= P.solidHi[0] && y >= P.solidLo[1] && y <= P.solidHi[1] && z >= P.solidLo[2] &&
+         z <= P.solidHi[2];
+}
 
 // Geometry of the grids and pools (constant per simulation).
 struct Layout {

@@ -97,6 +97,9 @@ template <class T> struct MGArgsT {
   int bottom;
   float weight;     // axisWeights, equal on every axis for cubic cells
   int topologyAll;  // the geometry changed: rebuild every cell's topology
+  // The levels below the first tile level run in k_mg_coarse_cycle (at most TC_MAX tile
+  // slots): their per-level kernels and the bottom kernels skip this simulation.
+  int tilesFused = 0;
   // Defect correction: right-hand side b, the solution x as an unevaluated sum of two FP32
   // values dcX + dcXlo, residual r, per-brick partial sums and the norms (sum r^2, sum b^2,
   // max |r|, scale, relative residual, finished, cycles). Once `finished` is set every
@@ -199,6 +202,16 @@ FP_DECLARE_MG(mg_prolong_bricks)
 FP_DECLARE_MG(mg_prolong_tiles)
 FP_DECLARE_MG(mg_bottom)    // relaxCoarse: 19 sweeps in one workgroup, up to BOTTOM64_TILES tile slots
 FP_DECLARE_MG(mg_bottom64)  // the same in FP64, up to BOTTOM64_TILES tile slots
+// The first tile level's two sweeps and the restriction below it, and the prolongation into
+// it and its three sweeps, one workgroup per tile slot over the tile and a three-cell halo.
+FP_DECLARE_MG(mg_tiles_down)
+FP_DECLARE_MG(mg_tiles_up)
+// The V-cycle below the first tile level in one workgroup, for simulations of at most TC_MAX
+// tile slots (MGArgsT::tilesFused).
+FP_DECLARE_MG(mg_coarse_cycle)
+constexpr int TC_MAX = 512;
+constexpr int TC_CELLS = TC_MAX * 9;  // 2^3 + 1 cells per tile slot
+constexpr int TC_LANES = 1024;
 // Defect correction: r = b - A x (FP32 with FP64 row dot products) and per-brick FP64 sums;
 // their totals, the power-of-two scale and the stopping test of cycle `code` (one
 // workgroup per simulation); level 0's right-hand side r / scale; x += scale * (level 0's

@@ -82,7 +82,7 @@ __device__ __forceinline__ bool activeCell(const MGCtx &m, const Params &P, cons
 __device__ __forceinline__ bool solidAtMG(const MGCtx &m, const Params &P, const uint32_t *solids, float wx, float wy, float wz) {
   int x = (int)floorf(wx * P.invHf + 0.25f), y = (int)floorf(wy * P.invHf + 0.25f), z = (int)floorf(wz * P.invHf + 0.25f);
   if (P.ground && y < 0) return true;
-  if (P.colliders == 0) return false;
+  if (P.colliders == 0 || !inSolidBox(P, x, y, z)) return false;
   const int sF = m.L.sF;
   int s = entrySlot(brickEntry(m, x >> sF, y >> sF, z >> sF), m.side);
   if (s < 0) return false;
@@ -561,11 +561,12 @@ FP_MG_PAIR(mg_relax_tiles, 64)
 // the eight children share one cube: one owner lookup covers them, and the 24 cells beside
 // their 2^3 block resolve their cube once per face. All reads are issued before any flag is
 // tested; the residuals are then added in child order, as one child at a time would.
+// restrictResidual returns the coarse right-hand side (before storage rounding) and whether
+// every child is an unknown; restrictCell stores it and the first iterate.
 template <class St>
-__device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, int flips,
-                                             const int id[3]) {
+__device__ __forceinline__ float restrictResidual(const MGCtx &c, const MGArgsT<typename St::T> &m, int level,
+                                                  int flips, const int id[3], bool &every) {
   const auto fine = levelView(m, level - 1, flips);
-  const auto coarse = levelView(m, level, flips);
   const int sd = fine.side, msk = sd - 1;
   const int p0[3] = {id[0] * 2, id[1] * 2, id[2] * 2};
   const long a0 = addrAt(c, p0, level - 1);
@@ -599,7 +600,7 @@ __device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typen
     rhs8[k] = a0 < 0 ? 0.f : St::ld(fine.rhs[a]);
   }
   float residual = 0.f;
-  bool every = true;
+  every = true;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const int x = k & 1, y = (k >> 1) & 1, z = k >> 2;
@@ -614,7 +615,14 @@ __device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typen
     const float dg = diagonal(m.weight, flags);
     residual += rhs8[k] - (sum - dg * own[k]);
   }
-  float rhs = residual * 0.5f;
+  return residual * 0.5f;
+}
+template <class St>
+__device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, int flips,
+                                             const int id[3]) {
+  const auto coarse = levelView(m, level, flips);
+  bool every;
+  const float rhs = restrictResidual<St>(c, m, level, flips, id, every);
   long out = addrAt(c, id, level);
   coarse.rhs[out] = St::st(rhs);
   float first = 0.f;
@@ -812,7 +820,7 @@ FP_MG_PAIR(mg_prolong_tiles, 64)
 // copies (read one, write the other), so one barrier separates them.
 constexpr int BOTTOM_LANES = 1024;
 template <class St> __device__ __forceinline__ void mg_bottom_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
-  if (vb != 0) return;
+  if (vb != 0 || m.tilesFused) return;
   constexpr int PER = BOTTOM64_TILES / BOTTOM_LANES;
   __shared__ float values[2][BOTTOM64_TILES];
   float *value = values[0];
@@ -874,7 +882,7 @@ FP_MG_PAIR(mg_bottom, 1024)
 __device__ __forceinline__ double mix64(double a, double b, double t) { return a * (1.0 - t) + b * t; }
 template <class St>
 __device__ __forceinline__ void mg_bottom64_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
-  if (vb != 0) return;
+  if (vb != 0 || m.tilesFused) return;
   constexpr int PER = BOTTOM64_TILES / BOTTOM_LANES;
   __shared__ double value[BOTTOM64_TILES];
   const auto lv = levelView(m, m.bottom, code >> 8);
@@ -930,6 +938,492 @@ __device__ __forceinline__ void mg_bottom64_body(const MGArgsT<typename St::T> &
   }
 }
 FP_MG_PAIR(mg_bottom64, 1024)
+
+// ---- the first tile level, a tile and its halo per workgroup -----------------------------
+// The first tile level (one cell per brick, 4^3 per tile slot) runs its two sweeps and the
+// restriction below it in one launch (k_mg_tiles_down), and the prolongation into it and its
+// three sweeps in another (k_mg_tiles_up), instead of one launch per stage. A workgroup takes
+// one live tile slot and the level's cells within three of it (10^3 cells from the tile's
+// 27-neighborhood) into LDS; every sweep updates one cell less on each side, the way
+// successive sweeps of the per-level kernels would, so the tile's own cells end exactly as
+// they would there (the halo cells are computed again by the neighboring tiles' workgroups).
+// Cells of tiles that are not live keep their values, as the per-level kernels never write
+// them; cells of no tile read as zero. Each kernel stores the tile's cells into the level's
+// other array (k_mg_tiles_down after the two sweeps, k_mg_tiles_up at the end), since the
+// workgroups of neighboring tiles still read the current one: the caller flips the level.
+constexpr int RG = 10, RG3 = RG * RG * RG;
+// A value as its level's storage reads it back.
+template <class St> __device__ __forceinline__ float stored(float x) { return St::ld(St::st(x)); }
+struct TileRegion {
+  float v[2][RG3];
+  float rhs[RG3];
+  uint8_t flag[RG3];
+  int8_t state[RG3];     // 1: a cell of a live tile, 0: of a tile that is not live, -1: of no tile
+  int nb[27], live[27];  // the tile's 27 neighbor slots and whether each is live
+  float coarse[216];     // k_mg_tiles_up: the 6^3 coarse values its prolongation weighs
+};
+// Loads the region of tile slot t at level L (current iterate `cur`) into LDS; returns false
+// (for the whole workgroup) when t is not a live tile slot.
+template <class St>
+__device__ __forceinline__ bool loadRegion(const MGArgsT<typename St::T> &m, const LevelView<typename St::T> &lv,
+                                           int t, TileRegion &R, int tc[3]) {
+  const int *trec = m.dir.tileRec + (size_t)t * TILE_RECORD_WORDS;
+  if (trec[3] == 0) return false;
+  tc[0] = trec[0];
+  tc[1] = trec[1];
+  tc[2] = trec[2];
+  if (threadIdx.x < 27) {
+    const int nt = trec[4 + threadIdx.x];
+    R.nb[threadIdx.x] = nt;
+    R.live[threadIdx.x] = nt >= 0 && m.dir.tileRec[(size_t)nt * TILE_RECORD_WORDS + 3] != 0;
+  }
+  __syncthreads();
+  for (int i = (int)threadIdx.x; i < RG3; i += (int)blockDim.x) {
+    const int r[3] = {i % RG, (i / RG) % RG, i / (RG * RG)};
+    int q = 0, local = 0;
+    for (int a = 2; a >= 0; a--) {
+      const int g = r[a] + 1;  // cell tc * 4 - 3 + r: tile offset (g >> 2) - 1, local g & 3
+      q = q * 3 + (g >> 2);
+      local = local * 4 + (g & 3);
+    }
+    const int nt = R.nb[q];
+    if (nt < 0) {
+      R.v[0][i] = 0.f;
+      R.rhs[i] = 0.f;
+      R.flag[i] = 0;
+      R.state[i] = -1;
+    } else {
+      const size_t a = (size_t)nt * 64 + local;
+      R.v[0][i] = St::ld(lv.p[a]);
+      R.rhs[i] = St::ld(lv.rhs[a]);
+      R.flag[i] = lv.eflag[a];
+      R.state[i] = R.live[q] ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  return true;
+}
+// One weighted Jacobi sweep (relaxCell) over region cells [lo, RG - lo)^3, src to dst.
+template <class St>
+__device__ __forceinline__ void regionSweep(TileRegion &R, int src, int lo, float w) {
+  const int n = RG - 2 * lo, n3 = n * n * n;
+  const float *a = R.v[src];
+  float *d = R.v[src ^ 1];
+  for (int j = (int)threadIdx.x; j < n3; j += (int)blockDim.x) {
+    const int i = (lo + j % n) + RG * (lo + (j / n) % n) + RG * RG * (lo + j / (n * n));
+    if (R.state[i] != 1) {
+      d[i] = a[i];
+      continue;
+    }
+    const int e = R.flag[i];
+    if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
+      d[i] = 0.f;
+      continue;
+    }
+    const float nb[6] = {a[i + 1], a[i - 1], a[i + RG], a[i - RG], a[i + RG * RG], a[i - RG * RG]};
+    const float cand = (openSum(nb, e & 63, w) - R.rhs[i]) / fmaxv(diagonal(w, e & 63), 1e-12f);
+    d[i] = stored<St>(mixv(a[i], cand, 0.8f));
+  }
+  __syncthreads();
+}
+// The tile's 64 cells (region [3, 7)^3) from R.v[src] into lv.q.
+template <class St>
+__device__ __forceinline__ void storeTile(const TileRegion &R, int src, const LevelView<typename St::T> &lv, int t) {
+  if (threadIdx.x < 64) {
+    const int l = (int)threadIdx.x;
+    const int i = (3 + (l & 3)) + RG * (3 + ((l >> 2) & 3)) + RG * RG * (3 + (l >> 4));
+    lv.q[(size_t)t * 64 + l] = St::st(R.v[src][i]);
+  }
+}
+// code: the flips (bits 0-15).
+template <class St>
+__device__ __forceinline__ void mg_tiles_down_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int Lt = m.L.brickLevels, t = vb;
+  if (t >= m.dir.capacity) return;
+  __shared__ TileRegion R;
+  const auto lv = levelView(m, Lt, code);
+  int tc[3];
+  if (!loadRegion<St>(m, lv, t, R, tc)) return;
+  const float w = m.weight;
+  regionSweep<St>(R, 0, 1, w);
+  regionSweep<St>(R, 1, 2, w);
+  storeTile<St>(R, 0, lv, t);
+  // restrictCell into the tile's 2^3 cells of the next level, from the swept values.
+  if (threadIdx.x < 8) {
+    const auto coarse = levelView(m, Lt + 1, code);
+    const int l = (int)threadIdx.x, cx = l & 1, cy = (l >> 1) & 1, cz = l >> 2;
+    const float *a = R.v[0];
+    const int i0 = (3 + 2 * cx) + RG * (3 + 2 * cy) + RG * RG * (3 + 2 * cz);
+    float residual = 0.f;
+    bool every = true;
+    for (int k = 0; k < 8; k++) {
+      const int i = i0 + (k & 1) + RG * ((k >> 1) & 1) + RG * RG * (k >> 2);
+      const int e = R.flag[i];
+      const bool unknown = (e & UNKNOWN) != 0;
+      every = every && unknown;
+      if (!unknown || (e & STORED_SOLID)) continue;
+      const int flags = e & 63;
+      const float nb[6] = {a[i + 1], a[i - 1], a[i + RG], a[i - RG], a[i + RG * RG], a[i - RG * RG]};
+      residual += R.rhs[i] - (openSum(nb, flags, w) - diagonal(w, flags) * a[i]);
+    }
+    const float rhs = residual * 0.5f;
+    const size_t out = (size_t)t * 8 + l;
+    coarse.rhs[out] = St::st(rhs);
+    float first = 0.f;
+    if (every) {
+      const int e = coarse.eflag[out];
+      if (!(e & STORED_SOLID)) first = mixv(0.f, (0.f - rhs) / fmaxv(diagonal(w, e & 63), 1e-12f), 0.8f);
+    }
+    coarse.p[out] = St::st(first);
+  }
+}
+FP_MG_PAIR(mg_tiles_down, 256)
+template <class St>
+__device__ __forceinline__ void mg_tiles_up_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int Lt = m.L.brickLevels, t = vb;
+  if (t >= m.dir.capacity) return;
+  __shared__ TileRegion R;
+  const auto lv = levelView(m, Lt, code);
+  int tc[3];
+  if (!loadRegion<St>(m, lv, t, R, tc)) return;
+  // The coarse cells [tc * 2 - 2, tc * 2 + 4) on each axis, resolved once (prolongValue).
+  {
+    MGCtx c;
+    c.dir = m.dir;
+    c.L = m.L;
+    c.side = m.L.side;
+    c.hasBrick = false;
+    c.hasTile = true;
+    c.tc[0] = tc[0];
+    c.tc[1] = tc[1];
+    c.tc[2] = tc[2];
+    c.tnb = R.nb;
+    const auto cv = levelView(m, Lt + 1, code);
+    for (int j = (int)threadIdx.x; j < 216; j += (int)blockDim.x) {
+      const int cc[3] = {tc[0] * 2 - 2 + j % 6, tc[1] * 2 - 2 + (j / 6) % 6, tc[2] * 2 - 2 + j / 36};
+      R.coarse[j] = prolongValue<St>(c, m, cv, cc, Lt + 1);
+    }
+  }
+  __syncthreads();
+  // prolongateCell over the whole region into R.v[1].
+  for (int i = (int)threadIdx.x; i < RG3; i += (int)blockDim.x) {
+    if (R.state[i] != 1) {
+      R.v[1][i] = R.v[0][i];
+      continue;
+    }
+    const int e = R.flag[i];
+    if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
+      R.v[1][i] = 0.f;
+      continue;
+    }
+    const int p[3] = {tc[0] * 4 - 3 + i % RG, tc[1] * 4 - 3 + (i / RG) % RG, tc[2] * 4 - 3 + i / (RG * RG)};
+    float f[3];
+    int b[3];
+    for (int k = 0; k < 3; k++) {
+      const float q = ((float)p[k] + 0.5f) * 0.5f - 0.5f;
+      const float fl = floorf(q);
+      b[k] = (int)fl - (tc[k] * 2 - 2);
+      f[k] = q - fl;
+    }
+    const float *c0 = R.coarse + (b[2] * 6 + b[1]) * 6 + b[0];
+    const float v[8] = {c0[0], c0[1], c0[6], c0[7], c0[36], c0[37], c0[42], c0[43]};
+    const float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
+                            mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
+    R.v[1][i] = stored<St>(R.v[0][i] + corr);
+  }
+  __syncthreads();
+  const float w = m.weight;
+  regionSweep<St>(R, 1, 1, w);
+  regionSweep<St>(R, 0, 2, w);
+  regionSweep<St>(R, 1, 3, w);
+  storeTile<St>(R, 0, lv, t);
+}
+FP_MG_PAIR(mg_tiles_up, 256)
+
+// ---- the coarser tile levels of a V-cycle in one workgroup -------------------------------
+// For a simulation of at most TC_MAX tile slots, the V-cycle below the first tile level
+// (PressureSolver.ts:303-363 from the second tile level down: two sweeps, the restriction,
+// the bottom iterations, the prolongation, three sweeps) runs in one 1024-lane workgroup,
+// both levels held in LDS. Each stage is the arithmetic of the per-level kernels, cell by
+// cell, with their storage rounding: LDS keeps each value as its level's storage reads it
+// back. Barriers separate the stages. The second tile level's final iterate is stored back
+// to the array it was read from (the per-level kernels flip it six times), where
+// k_mg_tiles_up reads it.
+struct TileCycleLds {
+  float v[2][TC_CELLS];  // iterates; level k's current one is v[cur[k]]
+  float rhs[TC_CELLS];
+  uint8_t flag[TC_CELLS];
+  uint8_t topo[TC_CELLS];
+  int tnb[TC_MAX * 27];  // each tile slot's 27 neighbor slots
+  int tc[TC_MAX * 4];    // tile coordinates and the live word
+  double b64[2][TC_MAX];
+};
+// Level k (1: the second tile level, 2 cells a side per tile slot; 2: the bottom, one).
+__device__ __forceinline__ int tcOffset(int C, int k) { return k == 1 ? 0 : C * 8; }
+// The LDS index of the level-k cell at level coordinates g (the home tile t's neighborhood,
+// else the directory's hash), -1 for a cell of no tile (addrAt of the tile levels).
+__device__ __forceinline__ int tcIndex(const TileCycleLds &S, const Dir &dir, int C, int k, int t, const int g[3]) {
+  const int sh = 2 - k, s = 1 << sh;
+  const int d[3] = {(g[0] >> sh) - S.tc[t * 4], (g[1] >> sh) - S.tc[t * 4 + 1], (g[2] >> sh) - S.tc[t * 4 + 2]};
+  int nt;
+  if ((unsigned)(d[0] + 1) <= 2u && (unsigned)(d[1] + 1) <= 2u && (unsigned)(d[2] + 1) <= 2u)
+    nt = S.tnb[t * 27 + (d[2] + 1) * 9 + (d[1] + 1) * 3 + (d[0] + 1)];
+  else
+    nt = findTile(dir, g[0] >> sh, g[1] >> sh, g[2] >> sh);
+  if (nt < 0) return -1;
+  return tcOffset(C, k) + (nt << (3 * sh)) + (((g[2] & (s - 1)) << sh | (g[1] & (s - 1))) << sh) + (g[0] & (s - 1));
+}
+// The level coordinates of cell i of level k (tile slot t = i / cells).
+__device__ __forceinline__ void tcCell(const TileCycleLds &S, int k, int i, int &t, int g[3]) {
+  const int sh = 2 - k, l = i & ((1 << (3 * sh)) - 1), s = 1 << sh;
+  t = i >> (3 * sh);
+  g[0] = (S.tc[t * 4] << sh) + (l & (s - 1));
+  g[1] = (S.tc[t * 4 + 1] << sh) + ((l >> sh) & (s - 1));
+  g[2] = (S.tc[t * 4 + 2] << sh) + (l >> (2 * sh));
+}
+// One weighted Jacobi sweep of level k (relaxCell).
+template <class St>
+__device__ __forceinline__ void tcRelax(TileCycleLds &S, const Dir &dir, int C, int k, int cur, float w) {
+  const int n = C << (3 * (2 - k)), off = tcOffset(C, k);
+  const float *src = S.v[cur];
+  float *dst = S.v[cur ^ 1];
+  for (int i = (int)threadIdx.x; i < n; i += TC_LANES) {
+    int t, g[3];
+    tcCell(S, k, i, t, g);
+    if (S.tc[t * 4 + 3] == 0) continue;
+    const int a = off + i, e = S.flag[a];
+    float nb[6];
+    for (int d = 0; d < 6; d++) {
+      const int q[3] = {g[0] + OFF[d][0], g[1] + OFF[d][1], g[2] + OFF[d][2]};
+      const int j = tcIndex(S, dir, C, k, t, q);
+      nb[d] = j < 0 ? 0.f : src[j];
+    }
+    if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
+      dst[a] = 0.f;
+      continue;
+    }
+    const float cand = (openSum(nb, e & 63, w) - S.rhs[a]) / fmaxv(diagonal(w, e & 63), 1e-12f);
+    dst[a] = stored<St>(mixv(src[a], cand, 0.8f));
+  }
+}
+// The restriction from level 1 (current iterate `cur`) into the bottom, whose first iterate
+// lands in v[0] (restrictCell: the eight children are the tile slot's 2^3 cells).
+template <class St>
+__device__ __forceinline__ void tcRestrict(TileCycleLds &S, const Dir &dir, int C, int cur, float w) {
+  const float *fv = S.v[cur];
+  const int stride[3] = {1, 2, 4};
+  for (int t = (int)threadIdx.x; t < C; t += TC_LANES) {
+    if (S.tc[t * 4 + 3] == 0) continue;
+    const int id[3] = {S.tc[t * 4], S.tc[t * 4 + 1], S.tc[t * 4 + 2]};
+    const int p0[3] = {id[0] * 2, id[1] * 2, id[2] * 2};
+    const int a0 = t * 8;
+    float side[6][4];
+    for (int f = 0; f < 6; f++) {
+      const int ax = f >> 1, step = (f & 1) ? -1 : 2;
+      const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
+      int q[3] = {p0[0], p0[1], p0[2]};
+      q[ax] += step;
+      const int qa = tcIndex(S, dir, C, 1, t, q);  // a side of 2: always across the slot
+      for (int j = 0; j < 4; j++) side[f][j] = qa < 0 ? 0.f : fv[qa + (j & 1) * stride[u] + (j >> 1) * stride[v]];
+    }
+    float residual = 0.f;
+    bool every = true;
+    for (int c = 0; c < 8; c++) {
+      const int x = c & 1, y = (c >> 1) & 1, z = c >> 2;
+      const int a = a0 + c;
+      const int e = S.flag[a];
+      const bool unknown = (e & UNKNOWN) != 0;
+      every = every && unknown;
+      if (!unknown || (e & STORED_SOLID)) continue;
+      const float nb[6] = {x ? side[0][y + 2 * z] : fv[a + 1], x ? fv[a - 1] : side[1][y + 2 * z],
+                           y ? side[2][x + 2 * z] : fv[a + 2], y ? fv[a - 2] : side[3][x + 2 * z],
+                           z ? side[4][x + 2 * y] : fv[a + 4], z ? fv[a - 4] : side[5][x + 2 * y]};
+      const int flags = e & 63;
+      residual += S.rhs[a] - (openSum(nb, flags, w) - diagonal(w, flags) * fv[a]);
+    }
+    const float rhs = residual * 0.5f;
+    const int out = tcOffset(C, 2) + t;
+    S.rhs[out] = stored<St>(rhs);
+    float first = 0.f;
+    if (every) {
+      const int e = S.flag[out];
+      if (!(e & STORED_SOLID)) first = mixv(0.f, (0.f - rhs) / fmaxv(diagonal(w, e & 63), 1e-12f), 0.8f);
+    }
+    S.v[0][out] = stored<St>(first);
+  }
+}
+// coarseValue at the bottom's cell cc (current iterate `cur`), from the home tile t.
+__device__ __forceinline__ float tcCoarseValue(const TileCycleLds &S, const Dir &dir, const Params &P, int C, int cur,
+                                               int t, const int cc0[3]) {
+  const int groundY = P.ground ? 0 : (int)0x80000000;
+  const int cc[3] = {cc0[0], cc0[1] > groundY ? cc0[1] : groundY, cc0[2]};
+  const int a0 = tcIndex(S, dir, C, 2, t, cc);
+  if (!(a0 >= 0 && (S.topo[a0] & STORED_SOLID))) return a0 < 0 ? 0.f : S.v[cur][a0];
+  float sum = 0.f, count = 0.f;
+  for (int axis = 0; axis < 3; axis++)
+    for (int sd = -1; sd <= 1; sd += 2) {
+      int q[3] = {cc[0], cc[1], cc[2]};
+      q[axis] += sd;
+      q[1] = q[1] > groundY ? q[1] : groundY;
+      const int a = tcIndex(S, dir, C, 2, t, q);
+      if (!(a >= 0 && (S.topo[a] & STORED_SOLID))) {
+        sum += a < 0 ? 0.f : S.v[cur][a];
+        count += 1.0f;
+      }
+    }
+  return sum / fmaxv(count, 1.0f);
+}
+// The prolongation from the bottom (current iterate `ccur`) into level 1 (prolongCell).
+template <class St>
+__device__ __forceinline__ void tcProlong(TileCycleLds &S, const Dir &dir, const Params &P, int C, int cur, int ccur) {
+  const int n = C * 8;
+  const float *src = S.v[cur];
+  float *dst = S.v[cur ^ 1];
+  for (int i = (int)threadIdx.x; i < n; i += TC_LANES) {
+    int t, p[3];
+    tcCell(S, 1, i, t, p);
+    if (S.tc[t * 4 + 3] == 0) continue;
+    const int e = S.flag[i];
+    if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
+      dst[i] = 0.f;
+      continue;
+    }
+    float f[3];
+    int b[3];
+    for (int x = 0; x < 3; x++) {
+      const float q = ((float)p[x] + 0.5f) * 0.5f - 0.5f;
+      const float fl = floorf(q);
+      b[x] = (int)fl;
+      f[x] = q - fl;
+    }
+    const bool plain = P.colliders == 0 && (!P.ground || b[1] >= 0);
+    float v[8];
+    for (int c = 0; c < 8; c++) {
+      const int cc[3] = {b[0] + (c & 1), b[1] + ((c >> 1) & 1), b[2] + (c >> 2)};
+      if (plain) {
+        const int j = tcIndex(S, dir, C, 2, t, cc);
+        v[c] = j < 0 ? 0.f : S.v[ccur][j];
+      } else {
+        v[c] = tcCoarseValue(S, dir, P, C, ccur, t, cc);
+      }
+    }
+    const float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
+                            mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
+    dst[i] = stored<St>(src[i] + corr);
+  }
+}
+// code: the flips (bits 0-15) and, in bit 16, the bottom iterations in FP64 (mg_bottom64)
+// rather than FP32 (mg_bottom).
+template <class St>
+__device__ __forceinline__ void mg_coarse_cycle_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int C = m.dir.capacity;
+  if (vb != 0 || !m.tilesFused || C > TC_MAX) return;
+  __shared__ TileCycleLds S;
+  const bool bottom64 = ((code >> 16) & 1) != 0;
+  const int L1 = m.L.brickLevels + 1;
+  const auto lv1 = levelView(m, L1, code & 0xffff);
+  const auto lvb = levelView(m, L1 + 1, code & 0xffff);
+  const float w = m.weight;
+  const int tid = (int)threadIdx.x;
+  for (int i = tid; i < C * 27; i += TC_LANES) {
+    const int t = i / 27;
+    S.tnb[i] = m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 4 + (i - t * 27)];
+  }
+  for (int i = tid; i < C * 4; i += TC_LANES) S.tc[i] = m.dir.tileRec[(size_t)(i >> 2) * TILE_RECORD_WORDS + (i & 3)];
+  for (int i = tid; i < C * 8; i += TC_LANES) {
+    S.v[0][i] = St::ld(lv1.p[i]);
+    S.rhs[i] = St::ld(lv1.rhs[i]);
+    S.flag[i] = lv1.eflag[i];
+    S.topo[i] = lv1.topo[i];
+  }
+  for (int i = tid; i < C; i += TC_LANES) {
+    S.flag[C * 8 + i] = lvb.eflag[i];
+    S.topo[C * 8 + i] = lvb.topo[i];
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int r = 0; r < 2; r++) {
+    tcRelax<St>(S, m.dir, C, 1, cur, w);
+    cur ^= 1;
+    __syncthreads();
+  }
+  tcRestrict<St>(S, m.dir, C, cur, w);
+  __syncthreads();
+  // The bottom iterations (relaxCoarse), one tile slot a lane, as mg_bottom or mg_bottom64.
+  {
+    const int off = tcOffset(C, 2), t = tid;
+    const bool in = t < C;
+    const bool valid = in && S.tc[t * 4 + 3] != 0;
+    const int e = valid ? S.flag[off + t] : 0;
+    const bool live = valid && (e & UNKNOWN) && !(e & STORED_SOLID);
+    const int flags = e & 63;
+    int nbs[6];
+    for (int sd = 0; sd < 6; sd++)
+      nbs[sd] = live && ((flags >> sd) & 1) ? S.tnb[t * 27 + (OFF[sd][2] + 1) * 9 + (OFF[sd][1] + 1) * 3 + (OFF[sd][0] + 1)]
+                                            : -1;
+    const float first = valid && (e & UNKNOWN) ? S.v[0][off + t] : 0.f;
+    const float rhs = valid ? S.rhs[off + t] : 0.f;
+    float result;
+    if (!bottom64) {
+      float *bv[2] = {S.v[0] + off, S.v[1] + off};
+      __syncthreads();  // every lane has read the restricted iterate
+      if (in) bv[0][t] = first;
+      __syncthreads();
+      float mine = first;
+      for (int it = 0; it < BOTTOM_ITERATIONS; it++) {
+        const float *cv = bv[it & 1];
+        float out = 0.f;
+        if (live) {
+          float nb[6];
+          for (int sd = 0; sd < 6; sd++) nb[sd] = nbs[sd] >= 0 ? cv[nbs[sd]] : 0.f;
+          const float sum = w * (nb[0] + nb[1]) + w * (nb[2] + nb[3]) + w * (nb[4] + nb[5]);
+          const float cand = (sum - rhs) / fmaxv(diagonal(w, flags), 1e-12f);
+          out = mixv(mine, cand, 0.8f);
+        }
+        mine = out;
+        if (in) bv[(it & 1) ^ 1][t] = out;
+        __syncthreads();
+      }
+      result = mine;
+    } else {
+      const double wd = (double)w;
+      double mine = (double)first;
+      if (in) S.b64[0][t] = mine;
+      __syncthreads();
+      for (int it = 0; it < BOTTOM_ITERATIONS; it++) {
+        const double *cv = S.b64[it & 1];
+        double out = 0.0;
+        if (live) {
+          double nb[6];
+          for (int sd = 0; sd < 6; sd++) nb[sd] = nbs[sd] >= 0 ? cv[nbs[sd]] : 0.0;
+          const double sum = wd * (nb[0] + nb[1]) + wd * (nb[2] + nb[3]) + wd * (nb[4] + nb[5]);
+          const double diag = wd * (double)openCount(flags, 0) + wd * (double)openCount(flags, 2) +
+                              wd * (double)openCount(flags, 4);
+          const double cand = (sum - (double)rhs) / (diag < 1e-12 ? 1e-12 : diag);
+          out = mix64(mine, cand, 0.8);
+        }
+        mine = out;
+        if (in) S.b64[(it & 1) ^ 1][t] = out;
+        __syncthreads();
+      }
+      result = (float)mine;
+    }
+    __syncthreads();
+    if (valid) S.v[0][off + t] = stored<St>(result);
+    __syncthreads();
+  }
+  tcProlong<St>(S, m.dir, m.P, C, cur, 0);
+  cur ^= 1;
+  __syncthreads();
+  for (int r = 0; r < 3; r++) {
+    tcRelax<St>(S, m.dir, C, 1, cur, w);
+    cur ^= 1;
+    __syncthreads();
+  }
+  for (int i = tid; i < C * 8; i += TC_LANES)
+    if (S.tc[(i >> 3) * 4 + 3] != 0) lv1.p[i] = St::st(S.v[cur][i]);
+}
+FP_MG_PAIR(mg_coarse_cycle, 1024)
 
 // ---- defect correction -----------------------------------------------------------------
 // The loop's start, one workgroup per pool slot: x = 0 on the slot's level-0 cells; the
@@ -1146,6 +1640,8 @@ template <class St>
 __device__ __forceinline__ int levelGroups(const MGArgsT<typename St::T> &a, int L, int cells, bool tile) {
   if (!a.counts || dcFinished(a)) return 0;
   if (L == MAX_LEVELS) return 1;
+  // The levels below the first tile level of such a member run in k_mg_coarse_cycle_batch.
+  if (tile && a.tilesFused && L > a.L.brickLevels) return 0;
   const int units = tile ? a.dir.capacity : a.counts[0];
   const int k = groupSize(cells, tile ? 64 : 256);
   return ((units + k - 1) / k + 7) / 8 * 8;
@@ -1200,6 +1696,9 @@ template __global__ void k_level_prefix_batch<StoreF16>(const MGArgsT<_Float16> 
   FP_MG_INSTANCE(mg_prolong_tiles, St)     \
   FP_MG_INSTANCE(mg_bottom, St)            \
   FP_MG_INSTANCE(mg_bottom64, St)          \
+  FP_MG_INSTANCE(mg_tiles_down, St)        \
+  FP_MG_INSTANCE(mg_tiles_up, St)          \
+  FP_MG_INSTANCE(mg_coarse_cycle, St)      \
   FP_MG_INSTANCE(dc_start, St)             \
   FP_MG_INSTANCE(dc_residual, St)          \
   FP_MG_INSTANCE(dc_norms, St)             \

@@ -1004,6 +1004,55 @@ struct fp_sim {
       HIPCHECK(hipMemcpyAsync(buf.p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, stream));
   }
 
+  // The box of field cells holding every cell the colliders make solid (Params::solidLo and
+  // solidHi), with two cells of margin. A sphere or box collider's cells satisfy
+  // |axes_k . (w - center)| < extent_k on every axis k (insideCollider), so w - center =
+  // A^-1 proj lies within sum_k |A^-1_jk| extent_k on world axis j; a mesh's cells are its
+  // box. Solid bits are always built from the current colliders (a change rebuilds them
+  // all), so none is set outside. Singular axes give the unbounded box.
+  int solidLo[3] = {0, 0, 0}, solidHi[3] = {-1, -1, -1};
+  void solidBox(const std::vector<Collider> &cs) {
+    const double big = 1e9;
+    double lo[3] = {big, big, big}, hi[3] = {-big, -big, -big};
+    bool unbounded = false;
+    for (const Collider &c : cs) {
+      if (c.box == 2) {
+        for (int a = 0; a < 3; a++) {
+          lo[a] = std::min(lo[a], (double)c.lo[a]);
+          hi[a] = std::max(hi[a], (double)c.lo[a] + c.size[a] - 1);
+        }
+        continue;
+      }
+      double m[3][3];  // rows: the axes
+      for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) m[k][a] = c.axes[k][a];
+      const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                         m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                         m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+      if (!(std::fabs(det) > 1e-6)) {
+        unbounded = true;
+        continue;
+      }
+      double inv[3][3];  // inv[j][k] = (A^-1)_jk
+      for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) {
+          const int r0 = (k + 1) % 3, r1 = (k + 2) % 3, c0 = (j + 1) % 3, c1 = (j + 2) % 3;
+          inv[j][k] = (m[r0][c0] * m[r1][c1] - m[r0][c1] * m[r1][c0]) / det;
+        }
+      for (int j = 0; j < 3; j++) {
+        double h = 0;
+        for (int k = 0; k < 3; k++) h += std::fabs(inv[j][k]) * std::fabs((double)c.extent[k]);
+        lo[j] = std::min(lo[j], std::floor(((double)c.center[j] - h) / voxel - 0.5));
+        hi[j] = std::max(hi[j], std::ceil(((double)c.center[j] + h) / voxel - 0.5));
+      }
+    }
+    for (int a = 0; a < 3; a++) {
+      const bool none = lo[a] > hi[a];
+      solidLo[a] = unbounded ? INT32_MIN : none ? 0 : (int)std::max(lo[a] - 2, -2e9);
+      solidHi[a] = unbounded ? INT32_MAX : none ? -1 : (int)std::min(hi[a] + 2, 2e9);
+    }
+  }
+
   // packScene (FluidSimulation.ts:1150-1366): sphere/box emitters and colliders, analytic
   // forces. Returns whether the colliders' packed words changed.
   std::vector<Emitter> hostEmitters;
@@ -1120,6 +1169,7 @@ struct fp_sim {
       cr.box = c.shape == FP_SHAPE_BOX;
       hostColliders.push_back(cr);
     }
+    solidBox(hostColliders);
     bool changed = packed != prevColliders || n != colliderCount || masks != prevMasks;
     prevColliders = packed;
     prevMasks = masks;
@@ -1209,6 +1259,10 @@ struct fp_sim {
     P.seed = (float)s.seed;
     std::memcpy(&P.seedBits, &P.seed, 4);
     P.colliders = colliderCount;
+    for (int a = 0; a < 3; a++) {
+      P.solidLo[a] = solidLo[a];
+      P.solidHi[a] = solidHi[a];
+    }
     P.ground = ground;
     fp_fuel fuel = f.has_fuel ? f.fuel : fp_fuel{0, 0.5, 4.0};
     P.ignition = (float)fuel.ignition_heat;
@@ -1298,6 +1352,8 @@ struct fp_sim {
     s.meshRecords = meshHost.count() ? reinterpret_cast<const float4 *>(meshRecs.p) : nullptr;
     return s;
   }
+  // Whether each V-cycle runs its tile levels in one workgroup (k_mg_tiles_cycle).
+  bool tilesFused() const { return capacity <= TC_MAX && bottom == brickLevels + 2; }
   MGArgs mgArgs() {
     MGArgs m;
     m.dir = dirArgs();
@@ -1320,6 +1376,7 @@ struct fp_sim {
     m.lv[0].q = P1.p;
     m.lv[0].rhs = D.p;
     m.bottom = bottom;
+    m.tilesFused = tilesFused() ? 1 : 0;
     float inv = 1.0f / P.hv;
     float ref = 1.0f / inv;
     float nrm = ref * inv;
@@ -1523,6 +1580,7 @@ struct fp_sim {
     h.bottom = m.bottom;
     h.weight = m.weight;
     h.topologyAll = m.topologyAll;
+    h.tilesFused = m.tilesFused;
     for (int L = 0; L <= bottom; L++) {
       h.lv[L].p = reinterpret_cast<T *>(lv16[L].p.p);
       h.lv[L].q = reinterpret_cast<T *>(lv16[L].q.p);
@@ -1630,6 +1688,18 @@ struct fp_sim {
       else if (capacity <= BOTTOM64_TILES) k_mg_bottom<St><<<1, 1024, 0, stream>>>(m, flips << 8);
       else
         for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax<St>(m, L, slotsGrid, flips);
+      return;
+    }
+    if (L == brickLevels) {
+      // The first tile level: its sweeps and the restriction below it, the levels below (in
+      // one workgroup when the tile slots fit), then the prolongation into it and its sweeps.
+      // Each halo kernel leaves the level's iterate in its other array.
+      k_mg_tiles_down<St><<<capacity, 256, 0, stream>>>(m, flips);
+      flips ^= 1 << L;
+      if (m.tilesFused) k_mg_coarse_cycle<St><<<1, TC_LANES, 0, stream>>>(m, flips | (bottom64 ? 1 << 16 : 0));
+      else cycle<St>(m, L + 1, slotsGrid, bottom64, flips);
+      k_mg_tiles_up<St><<<capacity, 256, 0, stream>>>(m, flips);
+      flips ^= 1 << L;
       return;
     }
     if (L == 0) {
@@ -2375,6 +2445,7 @@ struct fp_batch {
     size_t args;
     int brickLevels, bottom;
     bool smallBottom, bottom64;
+    bool anyFused, anyLarge;  // members whose tile levels run fused / level by level
   };
 
   // One V-cycle over the batch (PressureSolver.ts:303-363) in storage St; returns the level
@@ -2396,6 +2467,19 @@ struct fp_batch {
         else if (c.smallBottom) go(k_mg_bottom_batch<St>, 1024, levelPrefix(MAX_LEVELS), perSim(), f << 8);
         else
           for (int i = 0; i < BOTTOM_ITERATIONS; i++) relax(L);
+        return;
+      }
+      if (L == c.brickLevels) {
+        // The first tile level (fp_sim::cycle): members of at most TC_MAX tile slots run the
+        // levels below it in one workgroup each, the others level by level (the per-level
+        // kernels have no blocks for the former).
+        go(k_mg_tiles_down_batch<St>, 256, levelPrefix(L), wide(), f);
+        f ^= 1 << L;
+        if (c.anyFused)
+          go(k_mg_coarse_cycle_batch<St>, TC_LANES, levelPrefix(MAX_LEVELS), perSim(), f | (c.bottom64 ? 1 << 16 : 0));
+        if (c.anyLarge) cycle(L + 1);
+        go(k_mg_tiles_up_batch<St>, 256, levelPrefix(L), wide(), f);
+        f ^= 1 << L;
         return;
       }
       if (L == 0) go(k_mg_init_batch<St>, 256, levelPrefix(0), wide(), f << 8);
@@ -2537,7 +2621,7 @@ struct fp_batch {
     std::vector<ReadbackArgs> rb(n);
     std::vector<int> cBlocks(n), cOne(n, 1), cLink(n), cSlots(n), cSlots2(n), cZero(n), cField(n), cTiles(n);
     size_t words = 0;
-    bool smallBottom = true;
+    bool smallBottom = true, anyFused = false, anyLarge = false;
     int maxCycles = 0;
     for (int i = 0; i < n; i++) {
       fp_sim &s = *sims[i];
@@ -2550,6 +2634,7 @@ struct fp_batch {
       cField[i] = slotsGrid * fchunks;
       cTiles[i] = s.capacity;
       smallBottom = smallBottom && s.capacity <= BOTTOM64_TILES;
+      (s.tilesFused() ? anyFused : anyLarge) = true;
       alloc[i] = s.allocArgs();
       s.solidsStale = false;
       pre[i] = s.stepArgs();
@@ -2574,6 +2659,9 @@ struct fp_batch {
     shape.bottom = s0.bottom;
     shape.smallBottom = smallBottom;
     shape.bottom64 = mode != 0;
+    shape.anyFused = anyFused;
+    shape.anyLarge = anyLarge;
+    const int tm = (anyFused ? 1 : 0) | (anyLarge ? 2 : 0);
     // Where level 0's final iterate lands (P0 or P1, the same for every simulation), from a
     // dry run of the V-cycle; defect correction leaves its solution in the pair x.
     const int flips = mode == 0 ? vcycle<StoreF32>(shape, true) : 0;
@@ -2617,11 +2705,11 @@ struct fp_batch {
     HIPCHECK(hipMemcpyAsync(devArgs.p, hostArgs, lay.bytes, hipMemcpyHostToDevice, stream));
     const int sb = smallBottom ? 1 : 0;
     if (mode == 0) {
-      replay(GraphKey{0, 0, 0, shape.brickLevels, shape.bottom, sb, zchunks, fchunks, lay.cap, 0},
+      replay(GraphKey{0, 0, 0, shape.brickLevels, shape.bottom, sb, zchunks, fchunks, lay.cap, tm},
              [&] { launchHead(shape, 0, 0, zchunks, fchunks); });
     } else {
       int done = std::min(HEAD_CYCLES, maxCycles);
-      replay(GraphKey{0, mode, done, shape.brickLevels, shape.bottom, sb, zchunks, fchunks, lay.cap, 0},
+      replay(GraphKey{0, mode, done, shape.brickLevels, shape.bottom, sb, zchunks, fchunks, lay.cap, tm},
              [&] { launchHead(shape, mode, done, zchunks, fchunks); });
       const int *live = reinterpret_cast<const int *>(devArgs.p + levelPrefix(MAX_LEVELS)) + lay.cap;
       while (done < maxCycles) {
@@ -2629,7 +2717,7 @@ struct fp_batch {
         HIPCHECK(hipStreamSynchronize(stream));
         if (*pinnedLive == 0) break;
         const int first = done + 1, last = std::min(done + CHUNK_CYCLES, maxCycles);
-        replay(GraphKey{1, mode, first, last, shape.brickLevels, shape.bottom, sb, 0, lay.cap, 0}, [&] {
+        replay(GraphKey{1, mode, first, last, shape.brickLevels, shape.bottom, sb, 0, lay.cap, tm}, [&] {
           forMode(mode, [&](auto st) { correctCycles<decltype(st)>(shape, first, last); });
         });
         done = last;

@@ -161,7 +161,7 @@ __device__ __forceinline__ float sampleSmoke(const Ctx &c, const float *buf, flo
 __device__ __forceinline__ bool solidCell(const Ctx &c, const Params &P, const uint32_t *solids, int x, int y,
                                           int z) {
   if (P.ground && y < 0) return true;
-  if (P.colliders == 0) return false;
+  if (P.colliders == 0 || !inSolidBox(P, x, y, z)) return false;
   int s = slotAtBrick(c, x >> c.L.sF, y >> c.L.sF, z >> c.L.sF);
   if (s < 0) return false;
   int m = c.L.B - 1, sh = c.L.sF;

@@ -117,10 +117,11 @@ def explosions(seed: int = 1, steps: int = 400) -> Scene:
 
 def small(seed: int = 1, steps: int = 12, brick: int = 8, divisor: int = 2,
           voxel: float = 0.05, collider: bool = True, turbulence: bool = True,
-          burst: bool = True, smoke: int = 1) -> Scene:
+          burst: bool = True, smoke: int = 1, tilted: bool = False) -> Scene:
     """A seconds-sized scene touching every stage: an emitter with fuel and velocity, a
     smoke source, a burst with variation, a turbulence force and a box collider; `smoke` is
-    the smoke grid's divisor."""
+    the smoke grid's divisor. With `tilted` the box is turned about all three axes and a
+    second, spherical collider sits in the plume."""
     sim = FireSimulation({"voxelSize": voxel, "brickSize": brick, "velocityDivisor": divisor,
                           "smokeDivisor": smoke,
                           "seed": seed, "fuel": {"enabled": True, "ignitionHeat": 0.5,
@@ -132,7 +133,18 @@ def small(seed: int = 1, steps: int = 12, brick: int = 8, divisor: int = 2,
         sim.add_force({"type": "turbulence", "strength": 1.5, "scale": 2})
     sim.add_force({"type": "vortex", "center": (0.2, 0.5, 0), "strength": 2, "radius": 0.6,
                    "lift": 0.5, "inward": 0.3})
-    if collider:
+    if collider and tilted:
+        import math
+        a, b, g = 0.5, 0.3, 0.8
+        ca, sa, cb, sb, cg, sg = math.cos(a), math.sin(a), math.cos(b), math.sin(b), math.cos(g), math.sin(g)
+        # The columns of Rz(g) Ry(b) Rx(a): the box's world axes.
+        rot = [[cb * cg, sa * sb * cg - ca * sg, ca * sb * cg + sa * sg],
+               [cb * sg, sa * sb * sg + ca * cg, ca * sb * sg - sa * cg],
+               [-sb, sa * cb, ca * cb]]
+        axes = tuple(tuple(rot[r][k] for r in range(3)) for k in range(3))
+        sim.add_collider("box", (0.1, 0.95, 0.0), size=(0.5, 0.12, 0.4), axes=axes)
+        sim.add_collider("sphere", (0.0, 0.75, 0.05), radius=0.11)
+    elif collider:
         sim.add_collider("box", (0.1, 0.95, 0.0), size=(0.5, 0.12, 0.4))
     x = sim.add_explosion({"radius": 0.3, "outwardSpeed": 4,
                            "charge": {"flame": 1, "heat": 2, "fuel": 1},
@@ -142,7 +154,7 @@ def small(seed: int = 1, steps: int = 12, brick: int = 8, divisor: int = 2,
         if burst and k == 3:
             x.trigger((-0.4, 0.5, 0.3))
 
-    suffix = f"-s{smoke}" if smoke > 1 else ""
+    suffix = (f"-s{smoke}" if smoke > 1 else "") + ("-tilted" if tilted else "")
     return Scene(f"small-b{brick}-d{divisor}{suffix}", sim, steps, hook=hook)
 
 
