@@ -435,12 +435,15 @@ def render_phase(sim, frames, step) -> dict:
         buf.synchronize()
         sim.profile(True)
         walls = []
+        counts = {}
         for frame, g in frames[1:]:
             step(frame, g)
             t0 = time.perf_counter()
             sim.render_device(cam, rs, buf.ptr.value)
             buf.synchronize()
             walls.append(time.perf_counter() - t0)
+            for k, v in sim.render_stats().items():
+                counts[k] = counts.get(k, 0) + v
             anchors = [tuple(frame.sources[i].position) for i in range(frame.frame.source_count)]
             sim.scene_lights(anchors, rs)
         stages = sim.profile_read()
@@ -451,6 +454,16 @@ def render_phase(sim, frames, step) -> dict:
     for k in ("render_prepare", "render_light", "render_slices", "render_march", "scene_lights"):
         if k in stages:
             out[k + "_ms"] = round(stages[k], 4)
+    # Samples per second of each march (the ray-march is bound by its gathers, not by HBM):
+    # positions looked up in the occupancy and samples reconstructed and shaded, per render,
+    # over the pass's event-timed mean.
+    n = max(1, len(walls))
+    for key, stage in (("slice", "render_slices"), ("march", "render_march")):
+        visits, shades = counts.get(key + "_visits", 0) / n, counts.get(key + "_shades", 0) / n
+        out[key + "_samples"] = {"visited": int(visits), "shaded": int(shades)}
+        if stages.get(stage):
+            out[key + "_samples"]["visited_per_s"] = visits / (stages[stage] * 1e-3)
+            out[key + "_samples"]["shaded_per_s"] = shades / (stages[stage] * 1e-3)
     solver = {k: round(v, 4) for k, v in stages.items() if not k.startswith("render_") and k != "scene_lights"}
     return out, solver
 

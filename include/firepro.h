@@ -364,6 +364,11 @@ int fp_sim_render(fp_sim *sim, const fp_camera *camera, const fp_render_settings
 int fp_sim_render_device(fp_sim *sim, const fp_camera *camera, const fp_render_settings *settings,
                          float *rgba_device);
 int fp_sim_read_render(fp_sim *sim, fp_render_tables *out);
+/* The latest render's sample counts, for measurement (no counterpart in the reference):
+ * counts[0] the half-resolution slices' sample positions looked up in the occupancy and
+ * counts[1] those reconstructed and shaded, counts[2] and counts[3] the same of the
+ * full-resolution march. */
+int fp_sim_render_stats(fp_sim *sim, uint64_t *counts);
 
 #ifdef __cplusplus
 }

@@ -36,6 +36,7 @@ struct Base {
   virtual void pressureReport(int *cycles, double *residual) = 0;
   virtual int render(const fp_camera &cam, const fp_render_settings &rs, float *rgba) = 0;
   virtual int readRender(fp_render_tables *t) = 0;
+  uint64_t renderCounts[4] = {0, 0, 0, 0};  // the latest image's sample counts
   virtual int probe(const fp_camera &cam, const fp_render_settings &rs, int what, const float *in, float *out) = 0;
   virtual int sceneLights(const fp_render_settings &rs, const double *xyz, int count, float *sums) = 0;
   virtual int stage(int which, double *out, int cap) = 0;
@@ -238,7 +239,7 @@ template <class R> struct Impl : Base {
     if constexpr (std::is_same<R, float>::value) {
       auto p = oracle::renderParams(sim.voxel, sim.B, cam, rs, sim.time);
       lastWork = cache.prepare(sim, p);
-      if (rgba) cache.view(sim, p).image(rgba);
+      if (rgba) cache.view(sim, p).image(rgba, this->renderCounts);
     }
     return 0;
   }
@@ -432,6 +433,12 @@ int orc_render(void *h, const fp_camera *cam, const fp_render_settings *rs, floa
   return guard([&] { ((Base *)h)->render(*cam, *rs, rgba); });
 }
 int orc_read_render(void *h, fp_render_tables *t) { return ((Base *)h)->readRender(t); }
+// The latest image's sample counts: the slices' positions visited and samples shaded, then
+// the march's.
+int orc_render_counts(void *h, uint64_t *out) {
+  for (int k = 0; k < 4; k++) out[k] = ((Base *)h)->renderCounts[k];
+  return FP_OK;
+}
 int orc_scene_lights(void *h, const fp_render_settings *rs, const double *xyz, int count, float *sums) {
   return guard([&] { ((Base *)h)->sceneLights(*rs, xyz, count, sums); });
 }

@@ -63,6 +63,7 @@ def load():
         "orc_render": (C.c_int, [v, P(abi.Camera), P(abi.RenderSettings), P(C.c_float)]),
         "orc_scene_lights": (C.c_int, [v, P(abi.RenderSettings), P(C.c_double), C.c_int, P(C.c_float)]),
         "orc_read_render": (C.c_int, [v, P(abi.RenderTables)]),
+        "orc_render_counts": (C.c_int, [v, P(C.c_uint64)]),
         "orc_render_probe": (C.c_int, [v, P(abi.Camera), P(abi.RenderSettings), C.c_int,
                                        P(C.c_float), P(C.c_float)]),
         "orc_blackbody": (None, [P(C.c_float)]),
@@ -242,6 +243,13 @@ class OracleSimulation:
                                     img.ctypes.data_as(C.POINTER(C.c_float)) if image else None), self._lib)
         self._render = (camera, rs)
         return img
+
+    def render_counts(self) -> np.ndarray:
+        """The latest image's sample counts: the half-resolution slices' positions visited and
+        samples shaded, then the march's."""
+        out = (C.c_uint64 * 4)()
+        self._lib.orc_render_counts(self._h, out)
+        return np.array(out[:], np.uint64)
 
     def scene_lights(self, anchors, settings=None) -> np.ndarray:
         """Per anchor the 8 totals of SceneLights (luminance, weighted offset, rgb, 0)."""

@@ -882,7 +882,9 @@ struct Renderer {
   // buildSlices (volume-slices.wgsl:27-122) for texel (px, py) of an sw x sh grid: the
   // march's state at SLICES evenly spaced depths from the first entry to the last exit,
   // into out[k * sw * sh].
-  void slice(int px, int py, int sw, int sh, float *out) const {
+  // `visits` counts the sample positions looked up in the occupancy, `shades` those
+  // reconstructed and shaded (the render's sample counts).
+  void slice(int px, int py, int sw, int sh, float *out, uint64_t &visits, uint64_t &shades) const {
     float uv[2] = {((float)px + 0.5f) / (float)sw, ((float)py + 0.5f) / (float)sh};
     float o[3], d[3], sx, sy, sz;
     ray(uv, o, d);
@@ -926,6 +928,7 @@ struct Renderer {
         t = fmax(t, nx);
       } else {
         const RBox &bx = boxes[cur];
+        visits++;
         // Steps stop at the next recorded depth and at the box's exit.
         float ds = fmin(stepSize, fmin(depth, curExit) - t);
         float at = t + jit * ds;
@@ -940,6 +943,7 @@ struct Renderer {
           float distance = fmin(dist[0], fmin(dist[1], dist[2]));
           t = fmin(t + fmax(1.0f, std::floor(distance / stepSize) + 1.0f) * stepSize, curExit);
         } else {
+          shades++;
           shade(bx, cell, flags, direct, ds, transmittance, radiance);
           t += ds;
         }
@@ -954,6 +958,11 @@ struct Renderer {
   }
 
   void pixel(int px, int py, float out[4], const float *slices = nullptr, int sw = 0, int sh = 0) const {
+    uint64_t visits = 0, shades = 0;
+    pixel(px, py, out, slices, sw, sh, visits, shades);
+  }
+  void pixel(int px, int py, float out[4], const float *slices, int sw, int sh, uint64_t &visits,
+             uint64_t &shades) const {
     out[0] = out[1] = out[2] = out[3] = 0.0f;
     float uv[2] = {((float)px + 0.5f) / (float)P.width, ((float)py + 0.5f) / (float)P.height};
     float clip[2] = {uv[0] * 2.0f + -1.0f, uv[1] * -2.0f + 1.0f};
@@ -1009,13 +1018,14 @@ struct Renderer {
         for (int a = 0; a < 3; a++) rad0[a] = v[a];
       }
     }
-    march(o, d, t0, trans0, rad0, end, stepSize, jit, direct, out);
+    march(o, d, t0, trans0, rad0, end, stepSize, jit, direct, out, visits, shades);
   }
 
   // fireProMarch (volume-march.wgsl:183-266) from t0 with transmittance trans0 and
   // radiance rad0.
   void march(const float o[3], const float d[3], float t0, float trans0, const float rad0[3], float end,
-             float stepSize, float jit, const float direct[3], float out[4]) const {
+             float stepSize, float jit, const float direct[3], float out[4], uint64_t &visits,
+             uint64_t &shades) const {
     float safe[3], exitSide[3];
     for (int a = 0; a < 3; a++) {
       safe[a] = fabs_(d[a]) > 0.000001f ? d[a] : 0.000001f;
@@ -1047,6 +1057,7 @@ struct Renderer {
         continue;
       }
       const RBox &bx = boxes[cur];
+      visits++;
       float ds = fmin(stepSize, fmin(end, curExit) - t);
       float at = t + jit * ds;
       float p[3] = {o[0] + d[0] * at, o[1] + d[1] * at, o[2] + d[2] * at};
@@ -1071,6 +1082,7 @@ struct Renderer {
         t = fmin(t + fmax(1.0f, std::floor(distance / stepSize) + 1.0f) * stepSize, curExit);
         continue;
       }
+      shades++;
       Vec4 f = reconstruct(bx, cell, (flags & 1) ? P.filter : 0.0f);
       // fireProRenderField (volume-reconstruction.wgsl:100-116): smoke from its own lattice.
       if (coarse) f.x = renderSmokeAt(bx, cell[0], cell[1], cell[2]);
@@ -1101,23 +1113,32 @@ struct Renderer {
     out[3] = 1.0f - transmittance;
   }
 
-  void image(float *rgba) const {
+  // The image; counts[0, 1] the slices' sample positions visited and samples shaded,
+  // [2, 3] the march's (integer sums: the same at any thread count).
+  void image(float *rgba, uint64_t counts[4] = nullptr) const {
     std::vector<float> slices;
     int sw = 0, sh = 0;
+    uint64_t v0 = 0, s0 = 0, v1 = 0, s1 = 0;
     if (P.halfRes) {
       sw = std::max(1, (P.width + 1) / 2);
       sh = std::max(1, (P.height + 1) / 2);
       slices.assign((size_t)sw * sh * SLICES * 4, 0.0f);
       // Rows are independent (each texel and pixel is computed alone): the host's threads
       // share them (OpenMP), with the same result at any thread count.
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : v0, s0)
       for (int y = 0; y < sh; y++)
-        for (int x = 0; x < sw; x++) slice(x, y, sw, sh, &slices[((size_t)y * sw + x) * 4]);
+        for (int x = 0; x < sw; x++) slice(x, y, sw, sh, &slices[((size_t)y * sw + x) * 4], v0, s0);
     }
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : v1, s1)
     for (int y = 0; y < P.height; y++)
       for (int x = 0; x < P.width; x++)
-        pixel(x, y, &rgba[((size_t)y * P.width + x) * 4], P.halfRes ? slices.data() : nullptr, sw, sh);
+        pixel(x, y, &rgba[((size_t)y * P.width + x) * 4], P.halfRes ? slices.data() : nullptr, sw, sh, v1, s1);
+    if (counts) {
+      counts[0] = v0;
+      counts[1] = s0;
+      counts[2] = v1;
+      counts[3] = s1;
+    }
   }
 };
 

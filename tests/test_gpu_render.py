@@ -35,6 +35,17 @@ def render_mismatches(gpu, cpu, cam, rs, label):
     if not np.array_equal(img_g, img_c):
         d = np.abs(img_g.astype(np.float64) - img_c.astype(np.float64))
         out.append(f"{label}: image max |diff| {d.max():.3e} in {int((d > 0).sum())} of {d.size}")
+    # The marches take the same samples: positions looked up and samples shaded, by the
+    # half-resolution slices and by the full-resolution march.
+    sg = gpu.render_stats()
+    got = [sg["slice_visits"], sg["slice_shades"], sg["march_visits"], sg["march_shades"]]
+    want = [int(x) for x in cpu.render_counts()]
+    if got != want:
+        out.append(f"{label}: sample counts {got} != {want}")
+    elif img_c[..., 3].max() > 0:
+        # With half resolution and no scene depth the full-resolution pass resumes from the
+        # last slice, past which nothing is left to sample.
+        assert want[1] + want[3] > 0 and want[0] >= want[1] and want[2] >= want[3]
     return out, img_g
 
 

@@ -279,8 +279,11 @@ __global__ void k_smooth_smoke(RenderArgs r, int chunks, int staged);
 __global__ void k_smooth_coarse(RenderArgs r);
 __global__ void k_occupancy(RenderArgs r);
 __global__ void k_light(RenderArgs r, int fresh);
-__global__ void k_march(RenderArgs r, float4 *out, const float4 *slices, int sw, int sh);
-__global__ void k_slices(RenderArgs r, float4 *slices, int sw, int sh, const int *order, uint32_t *cost);
+// The march and the half-resolution slices add their samples to stats (may be null): [0, 1]
+// the slices' positions visited and samples shaded, [2, 3] the march's.
+__global__ void k_march(RenderArgs r, float4 *out, const float4 *slices, int sw, int sh, unsigned long long *stats);
+__global__ void k_slices(RenderArgs r, float4 *slices, int sw, int sh, const int *order, uint32_t *cost,
+                         unsigned long long *stats);
 __global__ void k_order_tiles(const uint32_t *cost, int tiles, int *order);
 // Scene lights: per-brick totals by nearest anchor, then per-anchor totals in a fixed order
 // (blocks of LIGHT_BLOCK bricks).

@@ -691,7 +691,9 @@ __device__ __forceinline__ void rayOf(const RenderParams &P, float uvx, float uv
 // evenly spaced depths from `sx` to `sy`, where the state is stored into `slots`.
 struct MarchState {
   float t, transmittance, rad[3];
-  int steps;  // loop iterations taken (a cost measure for scheduling)
+  int steps;   // loop iterations taken (a cost measure for scheduling)
+  int visits;  // sample positions looked up in the occupancy
+  int shades;  // of those, samples reconstructed and shaded
 };
 __device__ __forceinline__ void march(const RenderArgs &r, const float o[3], const float d[3], float stepSize, float jit,
                                       float end, MarchState &st, int count = 0, float sx = 0.f, float sy = 0.f,
@@ -722,7 +724,7 @@ __device__ __forceinline__ void march(const RenderArgs &r, const float o[3], con
   const float spacing = record ? fmaxv(sy - sx, 0.0f) / (float)count : 0.0f;
   float depth = sx + spacing;
   int recorded = 0;
-  int steps = 0;
+  int steps = 0, visits = 0, shades = 0;
   while (true) {
     if (record ? (recorded >= count || t >= sy || transmittance < 0.008f) : (t >= end || transmittance < 0.008f)) break;
     steps++;
@@ -745,6 +747,7 @@ __device__ __forceinline__ void march(const RenderArgs &r, const float o[3], con
       t = fmaxv(t, nx);
       if (!record) continue;
     } else {
+      visits++;
       float ds = fminv(stepSize, fminv(record ? depth : end, curExit) - t);
       float at = t + jit * ds;
       float p[3] = {o[0] + d[0] * at, o[1] + d[1] * at, o[2] + d[2] * at};
@@ -768,6 +771,7 @@ __device__ __forceinline__ void march(const RenderArgs &r, const float o[3], con
         float distance = fminv(dist[0], fminv(dist[1], dist[2]));
         t = fminv(t + fmaxv(1.0f, floorf(distance / stepSize) + 1.0f) * stepSize, curExit);
       } else {
+        shades++;
         float4 f = reconstruct(r, bx, cell, (flags & 1u) ? P.filter : 0.0f);
         // Smoke was smoothed once on its own lattice (fireProRenderField, volume-reconstruction.wgsl:100-116).
         if (r.coarse) f.x = renderSmokeAt(r, bx, cell[0], cell[1], cell[2]);
@@ -807,6 +811,8 @@ __device__ __forceinline__ void march(const RenderArgs &r, const float o[3], con
   for (int k = recorded; k < count; k++) slots[k * stride] = make_float4(rad[0], rad[1], rad[2], 1.0f - transmittance);
   st.t = t;
   st.steps += steps;
+  st.visits += visits;
+  st.shades += shades;
   st.transmittance = transmittance;
   st.rad[0] = rad[0];
   st.rad[1] = rad[1];
@@ -834,32 +840,45 @@ __device__ __forceinline__ void spanOf(const RenderArgs &r, const float o[3], co
 // buildSlices (volume-slices.wgsl:27-122): the half-resolution march, a lane per texel of
 // a (ceil(w/2), ceil(h/2)) grid, storing its state at SLICES evenly spaced depths from the
 // first box entry to the last exit: slices[(k * sh + y) * sw + x].
-__device__ __forceinline__ int sliceTexel(const RenderArgs &r, float4 *slices, int sw, int sh, int px, int py) {
+__device__ __forceinline__ MarchState sliceTexel(const RenderArgs &r, float4 *slices, int sw, int sh, int px, int py) {
   const RenderParams &P = r.P;
   float o[3], d[3], sx, sy, sz;
   rayOf(P, ((float)px + 0.5f) / (float)sw, ((float)py + 0.5f) / (float)sh, o, d);
   spanOf(r, o, d, sx, sy, sz);
   const float stepSize = fmaxv(P.voxel * 0.5f, sz / P.raySteps);
-  MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}, 0};
+  MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}, 0, 0, 0};
   march(r, o, d, stepSize, jitter((uint32_t)px, (uint32_t)py, P.time), sy, st, SLICES, sx, sy,
         slices + ((size_t)py * sw + px), (size_t)sw * sh);
-  return st.steps;
+  return st;
+}
+// Adds a workgroup's sample counts to stats[0] (positions visited) and stats[1] (samples
+// shaded): LDS sums, then one atomic each.
+__device__ __forceinline__ void countSamples(unsigned long long *stats, int visits, int shades) {
+  __shared__ unsigned int sums[2];
+  if (threadIdx.x < 2) sums[threadIdx.x] = 0u;
+  __syncthreads();
+  if (visits) atomicAdd(&sums[0], (unsigned int)visits);
+  if (shades) atomicAdd(&sums[1], (unsigned int)shades);
+  __syncthreads();
+  if (threadIdx.x < 2 && sums[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)sums[threadIdx.x]);
 }
 // The same over 16 x 16 texel tiles, a workgroup each, taken in the order `order` (tile ids
 // row-major, longest first from the last frame) or in id order; each tile's cost, the sum
 // over its wavefronts of their longest march, goes to `cost`.
 __global__ void __launch_bounds__(256) k_slices(RenderArgs r, float4 *slices, int sw, int sh, const int *order,
-                                                uint32_t *cost) {
+                                                uint32_t *cost, unsigned long long *stats) {
   __shared__ uint32_t waveCost[4];
   const int tilesX = (sw + 15) >> 4;
   const int tile = order ? order[blockIdx.x] : (int)blockIdx.x;
   const int px = (tile % tilesX) * 16 + (threadIdx.x & 15), py = (tile / tilesX) * 16 + (threadIdx.x >> 4);
   if (threadIdx.x < 4) waveCost[threadIdx.x] = 0u;
   __syncthreads();
-  const int steps = (px < sw && py < sh) ? sliceTexel(r, slices, sw, sh, px, py) : 0;
-  atomicMax(&waveCost[threadIdx.x >> 6], (uint32_t)steps);
+  MarchState st{0.f, 1.0f, {0.f, 0.f, 0.f}, 0, 0, 0};
+  if (px < sw && py < sh) st = sliceTexel(r, slices, sw, sh, px, py);
+  atomicMax(&waveCost[threadIdx.x >> 6], (uint32_t)st.steps);
   __syncthreads();
   if (threadIdx.x == 0 && cost) cost[tile] = waveCost[0] + waveCost[1] + waveCost[2] + waveCost[3];
+  if (stats) countSamples(stats, st.visits, st.shades);
 }
 // Tile ids by decreasing cost (by its binary order of magnitude; ties in any order), one
 // workgroup: the next frame starts its longest tiles first, so the short ones fill the tail.
@@ -885,10 +904,9 @@ __global__ void __launch_bounds__(1024) k_order_tiles(const uint32_t *cost, int 
 // volume-march.wgsl:183-266: a lane per pixel, 16 x 16 pixel tiles per workgroup. With
 // `slices` (half resolution), the march resumes from the last stored depth in front of the
 // scene, sampled bilinearly; with no scene depth that is the last one.
-__global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out, const float4 *slices, int sw, int sh) {
+__device__ __forceinline__ void marchPixel(const RenderArgs &r, float4 *out, const float4 *slices, int sw, int sh,
+                                           int px, int py, int &visits, int &shades) {
   const RenderParams &P = r.P;
-  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (px >= P.width || py >= P.height) return;
   float4 result = make_float4(0.f, 0.f, 0.f, 0.f);
   const float uvx = ((float)px + 0.5f) / (float)P.width, uvy = ((float)py + 0.5f) / (float)P.height;
   float o[3], d[3], sx, sy, sz;
@@ -897,7 +915,7 @@ __global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out, const 
   if (sx < sy) {
     const float end = sy;
     const float stepSize = fmaxv(P.voxel * 0.5f, sz / P.raySteps);
-    MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}, 0};
+    MarchState st{sx, 1.0f, {0.f, 0.f, 0.f}, 0, 0, 0};
     if (slices) {
       const float count = (float)SLICES;
       const float slice = floorf((end - sx) / (sy - sx) * count);
@@ -927,8 +945,18 @@ __global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out, const 
     }
     march(r, o, d, stepSize, jitter((uint32_t)px, (uint32_t)py, P.time), end, st);
     result = make_float4(st.rad[0], st.rad[1], st.rad[2], 1.0f - st.transmittance);
+    visits = st.visits;
+    shades = st.shades;
   }
   out[(size_t)py * P.width + px] = result;
+}
+__global__ void __launch_bounds__(256) k_march(RenderArgs r, float4 *out, const float4 *slices, int sw, int sh,
+                                               unsigned long long *stats) {
+  const RenderParams &P = r.P;
+  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  int visits = 0, shades = 0;
+  if (px < P.width && py < P.height) marchPixel(r, out, slices, sw, sh, px, py, visits, shades);
+  if (stats) countSamples(stats + 2, visits, shades);
 }
 
 

@@ -527,6 +527,9 @@ struct fp_sim {
     // The slice march's 16 x 16 tiles: last frame's cost per tile and the order it implies
     // (longest first); orderTiles is the tile count the order was made for (0: none yet).
     DBuf<uint32_t> tileCost;
+    // The latest render's sample counts (fp_sim_render_stats): the slices' positions visited
+    // and samples shaded, then the march's.
+    DBuf<unsigned long long> sampleCounts;
     DBuf<int> tileOrder;
     int orderTiles = 0;
     DBuf<uint8_t> O;
@@ -1946,6 +1949,8 @@ struct fp_sim {
     }
     RenderArgs r = renderArgs(P);
     dim3 grid((unsigned)((P.width + 15) / 16), (unsigned)((P.height + 15) / 16));
+    if (!rc.sampleCounts.p) rc.sampleCounts.alloc(4);
+    k_fill_u32<<<1, 256, 0, stream>>>(reinterpret_cast<uint32_t *>(rc.sampleCounts.p), 0u, 8);
     const float4 *slices = nullptr;
     int sw = 0, sh = 0;
     if (s.half_resolution) {
@@ -1962,14 +1967,14 @@ struct fp_sim {
       }
       const int *order = rc.orderTiles == tiles ? rc.tileOrder.p : nullptr;
       int t0 = begin();
-      k_slices<<<(unsigned)tiles, 256, 0, stream>>>(r, rc.slices.p, sw, sh, order, rc.tileCost.p);
+      k_slices<<<(unsigned)tiles, 256, 0, stream>>>(r, rc.slices.p, sw, sh, order, rc.tileCost.p, rc.sampleCounts.p);
       end(t0, "render_slices");
       k_order_tiles<<<1, 1024, 0, stream>>>(rc.tileCost.p, tiles, rc.tileOrder.p);
       rc.orderTiles = tiles;
       slices = rc.slices.p;
     }
     int t = begin();
-    k_march<<<grid, 256, 0, stream>>>(r, target, slices, sw, sh);
+    k_march<<<grid, 256, 0, stream>>>(r, target, slices, sw, sh, rc.sampleCounts.p);
     end(t, "render_march");
     HIPCHECK(hipGetLastError());
     if (!deviceOut) {
@@ -2209,6 +2214,7 @@ struct fp_sim {
     rc.RS.release();
     rc.slices.release();
     rc.tileCost.release();
+    rc.sampleCounts.release();
     rc.tileOrder.release();
     rc.orderTiles = 0;
     rc.L.release();
@@ -3222,6 +3228,19 @@ int fp_sim_render_device(fp_sim *sim, const fp_camera *camera, const fp_render_s
     return FP_EINVAL;
   }
   return guard([&] { sim->drain(); sim->renderImage(*camera, *settings, rgba_device, true); });
+}
+
+int fp_sim_render_stats(fp_sim *sim, uint64_t *counts) {
+  if (!sim || !counts) {
+    g_error = "fp_sim_render_stats: null argument";
+    return FP_EINVAL;
+  }
+  return guard([&] {
+    sim->drain();
+    for (int k = 0; k < 4; k++) counts[k] = 0;
+    if (sim->rc.sampleCounts.p)
+      HIPCHECK(hipMemcpy(counts, sim->rc.sampleCounts.p, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  });
 }
 
 int fp_sim_read_render(fp_sim *sim, fp_render_tables *out) {

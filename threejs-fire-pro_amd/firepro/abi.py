@@ -595,6 +595,7 @@ def load(path: str = LIBRARY):
                                                   P(C.c_float)]),
         "fp_sim_render_device": (C.c_int, [C.c_void_p, P(Camera), P(RenderSettings), C.c_void_p]),
         "fp_sim_read_render": (C.c_int, [C.c_void_p, P(RenderTables)]),
+        "fp_sim_render_stats": (C.c_int, [C.c_void_p, P(C.c_uint64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

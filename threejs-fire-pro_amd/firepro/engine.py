@@ -278,6 +278,13 @@ class FluidSimulation:
                                                  C.c_void_p(pointer)), self._lib)
         self._lighting_divisor = settings.lighting_divisor
 
+    def render_stats(self) -> dict:
+        """The latest render's sample counts: positions looked up in the occupancy and
+        samples shaded, by the half-resolution slices and by the full-resolution march."""
+        out = (C.c_uint64 * 4)()
+        abi.check(self._lib.fp_sim_render_stats(self._h(), out), self._lib)
+        return {"slice_visits": out[0], "slice_shades": out[1], "march_visits": out[2], "march_shades": out[3]}
+
     def read_render(self) -> dict:
         ld = getattr(self, "_lighting_divisor", 4)
         return abi.read_render_tables(
