@@ -87,6 +87,45 @@ def test_batch_equals_alone_with_joins_and_leaves(mode, tolerances):
             s.dispose()
 
 
+@pytest.mark.parametrize("mode", ["vcycle", "bf16"])
+def test_batch_mixes_tile_paths(mode):
+    """A member with over a thousand tile slots (its levels below the first tile level run
+    level by level) batched with small ones (theirs run in one workgroup each): every
+    member ends as it does alone."""
+    from firepro import scenes
+    from firepro.engine import Batch, FluidSimulation
+    scs = [scenes.scattered(steps=4), scenes.small(seed=1, steps=4), scenes.small(seed=2, steps=4, collider=False)]
+    frames = [list(sc.frames()) for sc in scs]
+    alone = [FluidSimulation.create(sc.grid()) for sc in scs]
+    mixed = [FluidSimulation.create(sc.grid()) for sc in scs]
+    if mode != "vcycle":
+        for sim in alone + mixed:
+            sim.set_pressure(mode, 0.0, 3)
+    batch = Batch()
+    try:
+        for i, sim in enumerate(alone):
+            for frame, g in frames[i]:
+                sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+                sim.step(frame)
+        for t in range(4):
+            for i, sim in enumerate(mixed):
+                g = frames[i][t][1]
+                sim.cutoff, sim.voxel_budget, sim.ground = g["cutoff"], g["maxVoxels"], g["ground"]
+            print(f"case mixed tile paths {mode} step {t}", flush=True)
+            batch.step(mixed, [frames[i][t][0] for i in range(3)])
+        assert mixed[0].stats().tile_capacity > 512, "the large member exceeds one workgroup's tile slots"
+        assert all(m.stats().tile_capacity <= 512 for m in mixed[1:])
+        for i in range(3):
+            msgs = table_mismatches(mixed[i].read_tables(), alone[i].read_tables())
+            msgs += state_mismatches(mixed[i].read_state(pressure=True, expansion=True),
+                                     alone[i].read_state(pressure=True, expansion=True))
+            assert msgs == [], (i, msgs)
+    finally:
+        batch.dispose()
+        for s in alone + mixed:
+            s.dispose()
+
+
 def test_batch_pool_growth_and_budget():
     """A member that outgrows its pool and hits its voxel budget inside a batch."""
     from firepro import scenes

@@ -92,6 +92,22 @@ def four_fires(seed: int = 1, steps: int = 300, brick: int = 8, max_voxels: int 
     return Scene(f"four-fires-b{brick}", sim, steps, camera=cam)
 
 
+def scattered(count: int = 5, spacing: float = 1.6, steps: int = 4) -> Scene:
+    """A count^3 lattice of small fires `spacing` apart (voxel 0.025 m, brick 8): each keeps
+    its own tile and the margin around it, so the simulation holds over a thousand tile
+    slots while its bricks stay few."""
+    sim = FireSimulation({"voxelSize": 0.025, "brickSize": 8, "velocityDivisor": 2, "seed": 1,
+                          "grid": {"maxVoxels": 16_000_000}})
+    h = (count - 1) * spacing / 2
+    for i in range(count):
+        for j in range(count):
+            for k in range(count):
+                sim.add_fire({"position": (i * spacing - h, 0.5 + j * spacing, k * spacing - h),
+                              "shape": {"radius": 0.08}})
+    sim.add_collider("sphere", (0.2, 0.5 + h, 0.1), radius=0.3)
+    return Scene(f"scattered-{count}", sim, steps)
+
+
 def explosions(seed: int = 1, steps: int = 400) -> Scene:
     """Brick churn: 8 radial bursts in a 20 m cube at uniform steps 0-150."""
     rng = PCG32(seed)
