@@ -186,17 +186,20 @@ template <class R> struct Impl : Base {
   // An intermediate array of the last step, per active brick in active-list order, in
   // FP64: 0 the advected velocity, 1 the corrected velocity, 2 the forced velocity (three
   // planes of nV faces each), 3 the curl (four channels per velocity cell), 4 the scalar
-  // predictor (four channels per field cell). For pinning tests of the stages.
+  // predictor (four channels per field cell), 5 the MacCormack limiter decisions (one mask
+  // per field cell). For pinning tests of the stages and the precision report.
   int stage(int which, double *out, int cap) override {
     const std::vector<R> *buf[] = {&sim.Vadv, &sim.Vcor, &sim.Vf, &sim.Curl, &sim.Fp};
     const size_t per[] = {(size_t)sim.nV * 3, (size_t)sim.nV * 3, (size_t)sim.nV * 3, (size_t)sim.nV * 4,
-                          (size_t)sim.nF * 4};
-    if (which < 0 || which > 4) throw oracle::Error("stage must be 0 to 4.");
+                          (size_t)sim.nF * 4, (size_t)sim.nF};
+    if (which < 0 || which > 5) throw oracle::Error("stage must be 0 to 5.");
     int n = (int)sim.activeList.size();
     if (n > cap) return -1;
     for (int i = 0; i < n; i++) {
       size_t s = (size_t)sim.pageSlot(sim.entries[sim.activeList[i]] & 0xffffff);
-      for (size_t c = 0; c < per[which]; c++) out[i * per[which] + c] = (double)(*buf[which])[s * per[which] + c];
+      for (size_t c = 0; c < per[which]; c++)
+        out[i * per[which] + c] =
+            which == 5 ? (double)sim.limiter[s * per[which] + c] : (double)(*buf[which])[s * per[which] + c];
     }
     return n;
   }

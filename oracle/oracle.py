@@ -281,11 +281,13 @@ class OracleSimulation:
         return np.array(out[:], np.float32)
 
     STAGES = {"advected_velocity": (0, 3, "v"), "corrected_velocity": (1, 3, "v"),
-              "forced_velocity": (2, 3, "v"), "curl": (3, 4, "v"), "predictor": (4, 4, "f")}
+              "forced_velocity": (2, 3, "v"), "curl": (3, 4, "v"), "predictor": (4, 4, "f"),
+              "limiter": (5, 1, "f")}
 
     def stage(self, name: str) -> np.ndarray:
         """An intermediate array of the last step per active brick (FP64): velocities as
-        [brick, 3, nV], the curl as [brick, nV, 4], the scalar predictor as [brick, nF, 4]."""
+        [brick, 3, nV], the curl as [brick, nV, 4], the scalar predictor as [brick, nF, 4];
+        the limiter decisions as [brick, nF] masks (uint8)."""
         which, comps, grid = self.STAGES[name]
         st = self.stats()
         n = st.active_bricks
@@ -294,6 +296,8 @@ class OracleSimulation:
         got = self._lib.orc_stage(self._h, which, out.ctypes.data_as(C.POINTER(C.c_double)), max(n, 1))
         _check(0 if got >= 0 else got, self._lib)
         out = out[: got * comps * cells]
+        if comps == 1:
+            return out.reshape(got, cells).astype(np.uint8)
         return out.reshape(got, 3, cells) if comps == 3 else out.reshape(got, cells, comps)
 
     def force_probe(self, cells) -> np.ndarray:

@@ -328,6 +328,11 @@ template <class R> class Sim {
 
   // Pools, slot-major: field cells x4, expansion, velocity [slot][axis][cell], solids bits.
   std::vector<R> F, Fp, Fn, E, V, Vadv, Vcor, Vf, Curl, D, Pfine;
+  // The last step's MacCormack limiter decisions per field cell, for the precision report:
+  // bits 0-3 the corrected smoke, heat, lifetime and fuel kept (inside the donors' range),
+  // bit 4 the smoke/heat/fuel limiter evaluated, bit 5 the lifetime one (0: first order, a
+  // solid cell or donors all equal). Each cell writes its own byte.
+  mutable std::vector<uint8_t> limiter;
   std::vector<uint32_t> solids;
   // Coarse smoke pools (smoke.wgsl): density, predictor, next, and the predictor's donor
   // bounds per smoke cell.
@@ -425,6 +430,7 @@ template <class R> class Sim {
     }
     Fp.assign((size_t)slots * nF * 4, R(0));
     Fn.assign((size_t)slots * nF * 4, R(0));
+    limiter.assign((size_t)slots * nF, 0);
     Vadv.assign((size_t)slots * nV * 3, R(0));
     Vcor.assign((size_t)slots * nV * 3, R(0));
     Vf.assign((size_t)slots * nV * 3, R(0));
@@ -1606,6 +1612,7 @@ template <class R> class Sim {
     const R *orig = &F[((size_t)s * nF + l) * 4];
     R lifetime = adv[2];
     R rev[3];
+    uint8_t decided = 16;
     if (dn.lmn != dn.lmx || adv[2] != dn.lmn) {
       // sampleScalarsExplicit (:6-21) at the forward cell.
       R cell[3], fr[3];
@@ -1630,6 +1637,7 @@ template <class R> class Sim {
       rev[2] = val[3];
       R cl = adv[2] + R(0.5) * (orig[2] - val[2]);
       lifetime = (cl < dn.lmn || cl > dn.lmx) ? adv[2] : cl;
+      decided |= (cl < dn.lmn || cl > dn.lmx) ? 32 : 32 | 4;
     } else {
       R sp[3], val[4];
       for (int a = 0; a < 3; a++) sp[a] = fw[a] * P.invHf;
@@ -1643,7 +1651,9 @@ template <class R> class Sim {
     for (int k = 0; k < 3; k++) {
       R corr = adv[ch[k]] + R(0.5) * (orig[ch[k]] - rev[k]);
       bounded[k] = (corr < dn.mn[k] || corr > dn.mx[k]) ? adv[ch[k]] : corr;
+      if (!(corr < dn.mn[k] || corr > dn.mx[k])) decided |= (uint8_t)(1 << ch[k]);
     }
+    limiter[(size_t)s * nF + l] = decided;
     out[0] = bounded[0];
     out[1] = bounded[1];
     out[2] = clampv(lifetime, R(0), R(1));
@@ -1681,6 +1691,7 @@ template <class R> class Sim {
       bool injected = false;
       R cv[3];
       traceCenter(pos, cv);
+      limiter[(size_t)s * nF + l] = 0;
       if (!solidAt(pos)) {
         R tr[4];
         correctedScalars(c, s, l, pos, cv, tr);

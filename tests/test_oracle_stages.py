@@ -207,6 +207,13 @@ def test_random_scalar_field_matches_textbook_step(maccormack):
         share = outside[c[..., 2], c[..., 1], c[..., 0]][ok].mean()
         assert 0.005 < share < 0.95, share
         assert np.abs(got - hat[c[..., 2], c[..., 1], c[..., 0]])[ok].max() > 1e-3
+        # The recorded limiter decisions (the precision report's flip counts read them): the
+        # limiter ran, and kept the corrected smoke exactly where it stays inside its donors.
+        lim = h.sim.stage("limiter")
+        evaluated = (lim & 16) != 0
+        kept = (lim & 1) != 0
+        assert evaluated[ok].all()
+        assert np.array_equal(kept[ok], ~outside[c[..., 2], c[..., 1], c[..., 0]][ok])
 
 
 def test_maccormack_makes_no_new_extrema():
