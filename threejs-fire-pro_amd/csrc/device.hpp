@@ -184,6 +184,27 @@ __device__ __forceinline__ uint32_t waveOr(uint32_t v) {
 __device__ __forceinline__ float fminv(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float fmaxv(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float fabsv(float a) { return a < 0.0f ? -a : a; }
+// The least and greatest of three, one VOP3 instruction for two comparisons where the
+// select form above takes a compare and a select per value (and the library fminf a
+// canonicalizing move per loaded operand). For operands that are not NaN the result is the
+// value a chain of fminv/fmaxv gives, except that a zero extremum may carry the other sign
+// when the operands hold both +0 and -0: used for donor ranges only, which enter comparisons
+// (sign-blind) and, for a range of one value, the scalar fields, which hold no -0.
+__device__ __forceinline__ float min3v(float a, float b, float c) {
+  float r;
+  asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+__device__ __forceinline__ float max3v(float a, float b, float c) {
+  float r;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+// The range of eight values from the starting bounds lo and hi.
+__device__ __forceinline__ void range8(const float v[8], float &lo, float &hi) {
+  lo = min3v(min3v(min3v(min3v(lo, v[0], v[1]), v[2], v[3]), v[4], v[5]), v[6], v[7]);
+  hi = max3v(max3v(max3v(max3v(hi, v[0], v[1]), v[2], v[3]), v[4], v[5]), v[6], v[7]);
+}
 __device__ __forceinline__ float clampv(float x, float lo, float hi) { return fminv(fmaxv(x, lo), hi); }
 __device__ __forceinline__ float mixv(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 __device__ __forceinline__ float smoothstepv(float e0, float e1, float x) {

@@ -149,18 +149,24 @@ __device__ __forceinline__ float4 predict(const Ctx &c, const float4 *F, const i
   float4 value = make_float4(0.f, 0.f, 0.f, 0.f);
   float4 cv[8];
   fieldCorners(c, F, base[0], base[1], base[2], cv);
+  {
+    float ch[4][8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      ch[0][k] = cv[k].x;
+      ch[1][k] = cv[k].y;
+      ch[2][k] = cv[k].w;
+      ch[3][k] = cv[k].z;
+    }
+    range8(ch[0], dn.mn[0], dn.mx[0]);
+    range8(ch[1], dn.mn[1], dn.mx[1]);
+    range8(ch[2], dn.mn[2], dn.mx[2]);
+    range8(ch[3], dn.lmn, dn.lmx);
+  }
   for (int z = 0; z < 2; z++)
     for (int y = 0; y < 2; y++)
       for (int x = 0; x < 2; x++) {
         float4 smp = cv[(z * 2 + y) * 2 + x];
-        dn.mn[0] = fminv(dn.mn[0], smp.x);
-        dn.mx[0] = fmaxv(dn.mx[0], smp.x);
-        dn.mn[1] = fminv(dn.mn[1], smp.y);
-        dn.mx[1] = fmaxv(dn.mx[1], smp.y);
-        dn.mn[2] = fminv(dn.mn[2], smp.w);
-        dn.mx[2] = fmaxv(dn.mx[2], smp.w);
-        dn.lmn = fminv(dn.lmn, smp.z);
-        dn.lmx = fmaxv(dn.lmx, smp.z);
         float wx = x ? frac[0] : 1.0f - frac[0], wy = y ? frac[1] : 1.0f - frac[1], wz = z ? frac[2] : 1.0f - frac[2];
         value.x += smp.x * wx * wy * wz;
         value.y += smp.y * wx * wy * wz;
@@ -202,11 +208,12 @@ __device__ __forceinline__ void departureCell(const Params &P, const int id[3], 
   }
 }
 
-// The field-cell kernels give each brick max(1, nF / 256) chunks of 256 lanes, one cell a
-// lane: chunk k holds the brick's cells [256 k, 256 k + 256), a slab of whole z layers (or
-// part of one), whose local z range this is.
+// The field-cell kernels give each brick max(1, nF / n) chunks of n lanes (the workgroup's
+// size), one cell a lane: chunk k holds the brick's cells [n k, n k + n), a slab of whole z
+// layers (or part of one), whose local z range this is.
 __device__ __forceinline__ void chunkSlab(const Layout &L, int chunk, int &zlo, int &zhi) {
-  const int first = chunk * 256, last = (first + 256 < L.nF ? first + 256 : L.nF) - 1;
+  const int n = (int)blockDim.x;
+  const int first = chunk * n, last = (first + n < L.nF ? first + n : L.nF) - 1;
   zlo = first >> (2 * L.sF);
   zhi = last >> (2 * L.sF);
 }
@@ -220,7 +227,7 @@ __device__ __forceinline__ void advect_scalars_body(const StepArgs &s, const int
   int zlo, zhi;
   chunkSlab(s.L, chunk, zlo, zhi);
   stageVelWindow(c, s.p.V, zlo, zhi, vwin, w);
-  const int l = chunk * 256 + threadIdx.x;
+  const int l = chunk * (int)blockDim.x + threadIdx.x;
   if (l >= nF) return;
   int cc[3];
   fieldCellCoords(c, l, cc);
@@ -462,9 +469,10 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
   uint32_t myBusy = 0u, myExpands = 0u;
   const float burnKeep = P.burnKeep, lossS = P.lossS, lossH = P.lossH, halfS = P.halfS, halfH = P.halfH;
   // One cell a lane (chunkSlab). Only the lanes whose donors differ trace forward, through
-  // the pools: a velocity window staged for the whole workgroup costs more than it saves.
+  // the pools: a velocity window staged for the whole workgroup costs more than it saves
+  // (measured). The single-simulation step launches 64 lanes a workgroup, a batch 256.
   (void)chunks;
-  const int l = chunk * 256 + threadIdx.x;
+  const int l = chunk * (int)blockDim.x + threadIdx.x;
   const bool live = l < nF;
   // Emitters whose padded box misses every cell center of this slab weigh nothing in it
   // (emitterWeight's first test, decided here for all the slab's cells at once: w - pos is

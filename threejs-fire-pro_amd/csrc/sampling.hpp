@@ -122,6 +122,21 @@ __device__ __forceinline__ void velCorners(const Ctx &c, const float *buf, int c
   for (int k = 0; k < 8; k++) v[k] = slot[k] < 0 ? 0.f : buf[slotBase(slot[k] * 3 + comp, c.L.sV) + local[k]];
 }
 
+// The trilinear sum of eight cells, corner k = (z * 2 + y) * 2 + x weighted by
+// (wx * wy) * wz, accumulated in corner order.
+__device__ __forceinline__ float trilinear(const float cv[8], float fx, float fy, float fz) {
+  float out = 0.f;
+#pragma unroll
+  for (int z = 0; z < 2; z++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
+        out += cv[(z * 2 + y) * 2 + x] * (wx * wy * wz);
+      }
+  return out;
+}
 // The eight smoke cells from (x, y, z) of a coarse smoke pool.
 __device__ __forceinline__ void smokeCorners(const Ctx &c, const float *buf, int x, int y, int z, float v[8]) {
   int slot[8], local[8];
@@ -139,22 +154,10 @@ __device__ __forceinline__ float sampleSmoke(const Ctx &c, const float *buf, flo
   float fx = qx - flx, fy = qy - fly, fz = qz - flz;
   float cv[8];
   smokeCorners(c, buf, (int)flx, (int)fly, (int)flz, cv);
-  float out = 0.f;
   lo = 1e30f;
   hi = 0.f;
-#pragma unroll
-  for (int z = 0; z < 2; z++)
-#pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int x = 0; x < 2; x++) {
-        float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
-        const float v = cv[(z * 2 + y) * 2 + x];
-        out += v * (wx * wy * wz);
-        lo = fminv(lo, v);
-        hi = fmaxv(hi, v);
-      }
-  return out;
+  range8(cv, lo, hi);
+  return trilinear(cv, fx, fy, fz);
 }
 
 // solidCell / solidAt (solids.wgsl:15-36).
@@ -198,21 +201,6 @@ __device__ __forceinline__ float4 sampleField(const Ctx &c, const float4 *buf, f
         out.y += v.y * w;
         out.z += v.z * w;
         out.w += v.w * w;
-      }
-  return out;
-}
-// The trilinear sum of eight velocity cells, corner k = (z * 2 + y) * 2 + x weighted by
-// (wx * wy) * wz, accumulated in corner order.
-__device__ __forceinline__ float trilinear(const float cv[8], float fx, float fy, float fz) {
-  float out = 0.f;
-#pragma unroll
-  for (int z = 0; z < 2; z++)
-#pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int x = 0; x < 2; x++) {
-        float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
-        out += cv[(z * 2 + y) * 2 + x] * (wx * wy * wz);
       }
   return out;
 }
@@ -279,9 +267,14 @@ __device__ __forceinline__ void stageVelWindow(const Ctx &c, const float *V, int
   w.sy = ex;
   w.sz = ex * ex;
   w.n = n;
+  // Quotients by ex as a multiply and a shift: with inv = ceil(2^16 / ex), i * inv / 2^16
+  // exceeds i / ex by less than i / 2^16, which stays below 1 / ex while i * ex < 2^16
+  // (here i < VW_CELLS and ex * ex <= VW_CELLS, so i * ex < 768 * 28).
+  const uint32_t inv = (65536u + (uint32_t)ex - 1u) / (uint32_t)ex;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int t = i / ex;
-    const int x = w.o[0] + (i - t * ex), y = w.o[1] + (t % ex), z = w.o[2] + t / ex;
+    const int t = (int)(__umul24((uint32_t)i, inv) >> 16);
+    const int u = (int)(__umul24((uint32_t)t, inv) >> 16);
+    const int x = w.o[0] + (i - t * ex), y = w.o[1] + (t - u * ex), z = w.o[2] + u;
     const int s = c.home.slot[((z >> sV) - c.home.bz + 1) << 4 | ((y >> sV) - c.home.by + 1) << 2 |
                               ((x >> sV) - c.home.bx + 1)];
     const int local = cellIndex(sV, x, y, z);
@@ -355,11 +348,7 @@ __device__ __forceinline__ float windowVelBounds(const Ctx &c, const VelWindow &
   } else {
     lo = 1e5f;
     hi = -1e5f;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      lo = fminv(lo, cv[k]);
-      hi = fmaxv(hi, cv[k]);
-    }
+    range8(cv, lo, hi);
   }
   return out;
 }
