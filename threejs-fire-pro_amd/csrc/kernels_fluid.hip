@@ -134,10 +134,35 @@ FP_KERNEL_PAIR(correct_velocity, 256, StepArgs)
 
 // ---- scalar transport -----------------------------------------------------------------
 
+// contentSupport (fluid-content.wgsl:29-39) on a grid of bricks `side` cells a side,
+// velocity cells `ratio` of its cells wide and cells 1 / invH wide.
+__device__ __forceinline__ uint32_t supportOn(int side, float ratio, float invH, float dt, const int id[3],
+                                              const float vel[3]) {
+  float sidef = (float)side;
+  bool low[3], high[3];
+  for (int a = 0; a < 3; a++) {
+    float local = (float)(id[a] & (side - 1)) + 0.5f;
+    float pad = fminv(sidef, fmaxv(2.0f, 2.0f * ratio) + fabsv(vel[a]) * dt * invH);
+    low[a] = local < pad;
+    high[a] = local + pad >= sidef;
+  }
+  uint32_t row = 2u | (low[0] ? 1u : 0u) | (high[0] ? 4u : 0u);
+  uint32_t plane = (row << 3) | (low[1] ? row : 0u) | (high[1] ? row << 6 : 0u);
+  return (plane << 9) | (low[2] ? plane : 0u) | (high[2] ? plane << 18 : 0u);
+}
+__device__ __forceinline__ uint32_t contentSupport(const Layout &L, const Params &P, const int id[3], const float vel[3]) {
+  return supportOn(L.B, P.hv / P.hf, P.invHf, P.dt, id, vel);
+}
+
+
 struct Donors {
   float mn[3], mx[3], lmn, lmx;  // smoke, heat, fuel; lifetime
 };
 
+// sameDonors (fluid-scalar-transport.wgsl:64-67).
+__device__ __forceinline__ bool uniformDonors(const Donors &dn) {
+  return dn.mn[0] == dn.mx[0] && dn.mn[1] == dn.mx[1] && dn.mn[2] == dn.mx[2] && dn.lmn == dn.lmx;
+}
 // predictScalarsWithBounds (fluid-scalar-transport.wgsl:38-62).
 __device__ __forceinline__ float4 predict(const Ctx &c, const float4 *F, const int base[3], const float frac[3], Donors &dn) {
   for (int k = 0; k < 3; k++) {
@@ -188,6 +213,14 @@ __device__ __forceinline__ void traceForward(const Ctx &c, const Params &P, cons
   float m[3], v[3];
   for (int a = 0; a < 3; a++) m[a] = pos[a] + 0.5f * P.dt * cv[a];
   velocityAt(c, P, V, m[0], m[1], m[2], v);
+  for (int a = 0; a < 3; a++) out[a] = pos[a] + P.dt * v[a];
+}
+// The forward trace through a workgroup's velocity window.
+__device__ __forceinline__ void traceForward(const Ctx &c, const VelWindow &w, const Params &P, const float *V,
+                                             const float pos[3], const float cv[3], float out[3]) {
+  float m[3], v[3];
+  for (int a = 0; a < 3; a++) m[a] = pos[a] + 0.5f * P.dt * cv[a];
+  velocityAt(c, w, P, V, m[0], m[1], m[2], v);
   for (int a = 0; a < 3; a++) out[a] = pos[a] + P.dt * v[a];
 }
 // The backward trace through a workgroup's velocity window.
@@ -249,26 +282,27 @@ __device__ __forceinline__ void advect_scalars_body(const StepArgs &s, const int
   // Coarse smoke: the fine X channel only stages the step's smoke production
   // (fluid-scalar-transport.wgsl:127, :137, :191-199).
   *out = make_float4(s.P.coarse ? 0.f : v.x, v.y, s.P.lifetimeZero ? 0.f : v.z, v.w);
+  // The corrector's forward trace (correctedFineScalars, fluid-scalar-transport.wgsl:150-152)
+  // depends only on the velocity, so it runs here through the window; the corrector needs
+  // it only where the donors differ.
+  float fw[3] = {0.f, 0.f, 0.f};
+  if (!uniformDonors(dn)) traceForward(c, w, s.P, s.p.V, pos, cv, fw);
+  const uint32_t support = contentSupport(s.L, s.P, cc, cv);
   float4 *don = s.p.Don + slotBase(slot * 3, s.L.sF) + l;
-  don[0] = make_float4(cv[0], cv[1], cv[2], dn.lmn);
+  don[0] = make_float4(fw[0], fw[1], fw[2], dn.lmn);
   don[nF] = make_float4(dn.mn[0], dn.mn[1], dn.mn[2], dn.lmx);
-  don[2 * nF] = make_float4(dn.mx[0], dn.mx[1], dn.mx[2], 0.f);
+  don[2 * nF] = make_float4(dn.mx[0], dn.mx[1], dn.mx[2], __uint_as_float(support));
 }
 FP_KERNEL_PAIR1(advect_scalars, 256, StepArgs, int, chunks)
 
 
-// correctedFineScalars (fluid-scalar-transport.wgsl:144-188), with the donor bounds that
-// advection left in the backtrace planes (`dn`).
-__device__ __forceinline__ bool uniformDonors(const Donors &dn) {
-  return dn.mn[0] == dn.mx[0] && dn.mn[1] == dn.mx[1] && dn.mn[2] == dn.mx[2] && dn.lmn == dn.lmx;
-}
+// correctedFineScalars (fluid-scalar-transport.wgsl:144-188), with the donor bounds and the
+// forward departure point `fw` that advection left in the backtrace planes.
 __device__ __forceinline__ float4 correctedScalars(const Ctx &c, const Params &P, const Pools &p, const int id[3],
-                                   int slot, int l, const float pos[3], const float cv[3], const Donors &dn) {
+                                   int slot, int l, const float pos[3], const float fw[3], const Donors &dn) {
   const float4 adv = p.Fp[slotBase(slot, c.L.sF) + l];
   if (!P.maccormack) return adv;
   if (uniformDonors(dn)) return make_float4(dn.mn[0], dn.mn[1], clampv(dn.lmn, 0.f, 1.f), dn.mn[2]);
-  float fw[3];
-  traceForward(c, P, p.V, pos, cv, fw);
   const float4 orig = p.F[slotBase(slot, c.L.sF) + l];
   float lifetime = adv.z;
   float rev[3];
@@ -420,26 +454,6 @@ __device__ __forceinline__ int meshRecordAt(const StepArgs &s, const float w[3])
   return s.meshTables[s.brickCount + table * s.L.nF + ((z & m) * s.L.B + (y & m)) * s.L.B + (x & m)];
 }
 
-// contentSupport (fluid-content.wgsl:29-39) on a grid of bricks `side` cells a side,
-// velocity cells `ratio` of its cells wide and cells 1 / invH wide.
-__device__ __forceinline__ uint32_t supportOn(int side, float ratio, float invH, float dt, const int id[3],
-                                              const float vel[3]) {
-  float sidef = (float)side;
-  bool low[3], high[3];
-  for (int a = 0; a < 3; a++) {
-    float local = (float)(id[a] & (side - 1)) + 0.5f;
-    float pad = fminv(sidef, fmaxv(2.0f, 2.0f * ratio) + fabsv(vel[a]) * dt * invH);
-    low[a] = local < pad;
-    high[a] = local + pad >= sidef;
-  }
-  uint32_t row = 2u | (low[0] ? 1u : 0u) | (high[0] ? 4u : 0u);
-  uint32_t plane = (row << 3) | (low[1] ? row : 0u) | (high[1] ? row << 6 : 0u);
-  return (plane << 9) | (low[2] ? plane : 0u) | (high[2] ? plane << 18 : 0u);
-}
-__device__ __forceinline__ uint32_t contentSupport(const Layout &L, const Params &P, const int id[3], const float vel[3]) {
-  return supportOn(L.B, P.hv / P.hf, P.invHf, P.dt, id, vel);
-}
-
 // evolveFields (fluid-field-evolution.wgsl:9-84) with content recording.
 __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int vb, int chunks) {
   BRICK_PREAMBLE(chunks)
@@ -497,16 +511,18 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
       }
     }
   }
-  float cv[3];
+  float cv[3] = {0.f, 0.f, 0.f}, fw[3] = {0.f, 0.f, 0.f};
+  uint32_t support = 0u;
   Donors dn;
   if (live) {
     if (P.maccormack) {
-      // The cell's velocity and donor bounds from advection's backtrace.
+      // The forward departure point, donor bounds and content support from advection.
       const float4 *don = s.p.Don + slotBase(slot * 3, s.L.sF) + l;
       const float4 d0 = don[0], d1 = don[nF], d2 = don[2 * nF];
-      cv[0] = d0.x;
-      cv[1] = d0.y;
-      cv[2] = d0.z;
+      fw[0] = d0.x;
+      fw[1] = d0.y;
+      fw[2] = d0.z;
+      support = __float_as_uint(d2.w);
       dn.lmn = d0.w;
       dn.mn[0] = d1.x;
       dn.mn[1] = d1.y;
@@ -526,7 +542,7 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     bool injected = false;
     if (!P.maccormack) velocityAt(c, P, s.p.V, pos[0], pos[1], pos[2], cv);
     if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
-      float4 tr = correctedScalars(c, P, s.p, cc, slot, l, pos, cv, dn);
+      float4 tr = correctedScalars(c, P, s.p, cc, slot, l, pos, fw, dn);
       if (P.coarse) tr.x = 0.f;
       float lifetime = tr.z;
       float sp0 = 0.f, sp1 = 0.f, injectedFuel = 0.f;
@@ -576,7 +592,7 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     s.p.E[slotBase(slot, s.L.sF) + l] = expansion;
     if (expansion != 0.f) myExpands = 1u;
     if (fmaxv(fmaxv(P.coarse ? 0.f : value[0], value[2]), value[3]) > P.cutoff)
-      myBusy |= contentSupport(s.L, P, cc, cv);
+      myBusy |= P.maccormack ? support : contentSupport(s.L, P, cc, cv);
   }
   // One LDS atomic per wavefront: the lanes' masks are OR-reduced across it first.
   myBusy = waveOr(myBusy);

@@ -10,8 +10,8 @@ namespace fp {
 struct Pools {
   float4 *F, *Fp, *Fn;
   // The scalar backtrace of each field cell, left by advection for the corrector: three
-  // planes of float4 per slot, (velocity at the cell, lifetime min), (smoke, heat, fuel
-  // min, lifetime max), (smoke, heat, fuel max, 0).
+  // planes of float4 per slot, (forward departure point, lifetime min), (smoke, heat, fuel
+  // min, lifetime max), (smoke, heat, fuel max, content support bits).
   float4 *Don;
   // The velocity backtrace of each velocity cell, left by advection for the corrector: four
   // planes of float4 per slot, (face velocity, donor min) of components x, y, z, then the
