@@ -151,6 +151,15 @@ __device__ __forceinline__ int batchOwner(const int *starts, int n, int v) {
   __global__ void __launch_bounds__(BOUNDS) k_##NAME##_batch(const ARGS *args, const int *starts, int n, PTYPE PNAME) { \
     FP_BATCH_LOOP(NAME, NAME##_body(args[sim], v - starts[sim], PNAME))                                      \
   }
+// As FP_KERNEL_PAIR1, compiled for WAVES resident wavefronts a SIMD (at most 512 / WAVES
+// VGPRs a lane).
+#define FP_KERNEL_PAIR1_OCC(NAME, BOUNDS, WAVES, ARGS, PTYPE, PNAME)                                          \
+  __global__ void __launch_bounds__(BOUNDS) __attribute__((amdgpu_waves_per_eu(WAVES)))                     \
+      k_##NAME(ARGS a, PTYPE PNAME) { NAME##_body(a, (int)blockIdx.x, PNAME); }                               \
+  __global__ void __launch_bounds__(BOUNDS) __attribute__((amdgpu_waves_per_eu(WAVES)))                     \
+      k_##NAME##_batch(const ARGS *args, const int *starts, int n, PTYPE PNAME) {                              \
+    FP_BATCH_LOOP(NAME, NAME##_body(args[sim], v - starts[sim], PNAME))                                      \
+  }
 #define FP_DECLARE_PAIR(NAME, ARGS) \
   __global__ void k_##NAME(ARGS a);   \
   __global__ void k_##NAME##_batch(const ARGS *args, const int *starts, int n);
