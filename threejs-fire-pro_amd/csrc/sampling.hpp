@@ -271,15 +271,37 @@ __device__ __forceinline__ void stageVelWindow(const Ctx &c, const float *V, int
   // exceeds i / ex by less than i / 2^16, which stays below 1 / ex while i * ex < 2^16
   // (here i < VW_CELLS and ex * ex <= VW_CELLS, so i * ex < 768 * 28).
   const uint32_t inv = (65536u + (uint32_t)ex - 1u) / (uint32_t)ex;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int t = (int)(__umul24((uint32_t)i, inv) >> 16);
-    const int u = (int)(__umul24((uint32_t)t, inv) >> 16);
-    const int x = w.o[0] + (i - t * ex), y = w.o[1] + (t - u * ex), z = w.o[2] + u;
-    const int s = c.home.slot[((z >> sV) - c.home.bz + 1) << 4 | ((y >> sV) - c.home.by + 1) << 2 |
-                              ((x >> sV) - c.home.bx + 1)];
-    const int local = cellIndex(sV, x, y, z);
+  // Four cells a lane per pass, all their loads issued before the first LDS store, so a
+  // lane waits for memory once per pass rather than once per cell.
+  constexpr int U = 4;
+  const int lanes = (int)blockDim.x;
+  for (int first = threadIdx.x; first < n; first += U * lanes) {
+    float v[U][3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) lds[k * n + i] = s < 0 ? 0.f : V[slotBase(s * 3 + k, sV) + local];
+    for (int j = 0; j < U; j++) {
+      const int i = first + j * lanes;
+      v[j][0] = v[j][1] = v[j][2] = 0.f;
+      if (i < n) {
+        const int t = (int)(__umul24((uint32_t)i, inv) >> 16);
+        const int u = (int)(__umul24((uint32_t)t, inv) >> 16);
+        const int x = w.o[0] + (i - t * ex), y = w.o[1] + (t - u * ex), z = w.o[2] + u;
+        const int s = c.home.slot[((z >> sV) - c.home.bz + 1) << 4 | ((y >> sV) - c.home.by + 1) << 2 |
+                                  ((x >> sV) - c.home.bx + 1)];
+        const int local = cellIndex(sV, x, y, z);
+        if (s >= 0) {
+#pragma unroll
+          for (int k = 0; k < 3; k++) v[j][k] = V[slotBase(s * 3 + k, sV) + local];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int i = first + j * lanes;
+      if (i < n) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) lds[k * n + i] = v[j][k];
+      }
+    }
   }
   __syncthreads();
 }
