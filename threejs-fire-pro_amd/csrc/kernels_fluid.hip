@@ -86,8 +86,10 @@ FP_KERNEL_PAIR(advect_velocity, 256, StepArgs)
 
 
 // correctVelocityComponent (fluid-velocity-transport.wgsl:114-156), with the face velocity
-// and donor bounds that advection left in the backtrace planes.
-__device__ __forceinline__ float correctComponent(const Ctx &c, const Params &P, const Pools &p, const float pos[3],
+// and donor bounds that advection left in the backtrace planes; velocity samples through
+// the window `wv`, the advected velocity through `wa`.
+__device__ __forceinline__ float correctComponent(const Ctx &c, const VelWindow &wv, const VelWindow &wa,
+                                  const Params &P, const Pools &p, const float pos[3],
                                   int comp, const float orig[3], float predicted, const float fv[3], float lo,
                                   float hi) {
   float off[3] = {0.f, 0.f, 0.f};
@@ -97,9 +99,9 @@ __device__ __forceinline__ float correctComponent(const Ctx &c, const Params &P,
   if (lo == hi && predicted == lo) return predicted;
   float fmid[3], fw[3], sp[3];
   for (int a = 0; a < 3; a++) fmid[a] = fp[a] + 0.5f * P.dt * fv[a];
-  velocityAt(c, P, p.V, fmid[0], fmid[1], fmid[2], fw);
+  velocityAt(c, wv, P, p.V, fmid[0], fmid[1], fmid[2], fw);
   for (int a = 0; a < 3; a++) sp[a] = (fp[a] + P.dt * fw[a] - off[a]) * P.invHv;
-  float reversed = sampleVel(c, p.Vadv, comp, sp[0], sp[1], sp[2]);
+  float reversed = windowVel(c, wa, p.Vadv, comp, sp[0], sp[1], sp[2]);
   float corrected = predicted + 0.5f * (orig[comp] - reversed);
   return (corrected < lo || corrected > hi) ? predicted : corrected;
 }
@@ -110,6 +112,13 @@ __device__ __forceinline__ void correct_velocity_body(const StepArgs &s, const i
   float *out = s.p.Vcor + slotBase(slot * 3, s.L.sV);
   const float *vs = s.p.V + slotBase(slot * 3, s.L.sV);
   const float *as = s.p.Vadv + slotBase(slot * 3, s.L.sV);
+  // The midpoint samples read the velocity through the brick's window; the reversed
+  // sample's window of the advected planes is off: a second 9 KB of LDS a workgroup costs
+  // more resident wavefronts than it saves (measured).
+  __shared__ float vwin[3 * VW_CELLS];
+  VelWindow wv, wa;
+  stageVelWindow(c, s.p.V, 0, s.L.B - 1, vwin, wv);
+  velWindowOff(vwin, wa);
   for (int l = threadIdx.x; l < nV; l += blockDim.x) {
     int cc[3];
     velCellCoords(c, l, cc);
@@ -125,7 +134,7 @@ __device__ __forceinline__ void correct_velocity_body(const StepArgs &s, const i
     for (int k = 0; k < 3; k++) {
       const float4 t = tr[k * nV];
       const float fv[3] = {t.x, t.y, t.z};
-      out[k * nV + l] = correctComponent(c, s.P, s.p, pos, k, orig, as[k * nV + l], fv, t.w, his[k]);
+      out[k * nV + l] = correctComponent(c, wv, wa, s.P, s.p, pos, k, orig, as[k * nV + l], fv, t.w, his[k]);
     }
   }
 }
