@@ -962,6 +962,53 @@ struct TileRegion {
   int nb[27], live[27];  // the tile's 27 neighbor slots and whether each is live
   float coarse[216];     // k_mg_tiles_up: the 6^3 coarse values its prolongation weighs
 };
+// The region's cells from the 27 owners in R.nb (slots of a level whose cubes are 4^3, -1
+// for none) and R.live, after a barrier that published them. Four cells a lane per pass,
+// their loads issued before the first LDS store.
+template <class St>
+__device__ __forceinline__ void fillRegion(const LevelView<typename St::T> &lv, TileRegion &R) {
+  constexpr int U = 4;
+  const int lanes = (int)blockDim.x;
+  for (int first = (int)threadIdx.x; first < RG3; first += U * lanes) {
+    float pv[U], rv[U];
+    int fv[U], sv[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int i = first + j * lanes;
+      pv[j] = rv[j] = 0.f;
+      fv[j] = 0;
+      sv[j] = -1;
+      if (i < RG3) {
+        const int r[3] = {i % RG, (i / RG) % RG, i / (RG * RG)};
+        int q = 0, local = 0;
+        for (int a = 2; a >= 0; a--) {
+          const int g = r[a] + 1;  // cell origin * 4 - 3 + r: owner offset (g >> 2) - 1, local g & 3
+          q = q * 3 + (g >> 2);
+          local = local * 4 + (g & 3);
+        }
+        const int nt = R.nb[q];
+        if (nt >= 0) {
+          const size_t a = (size_t)nt * 64 + local;
+          pv[j] = St::ld(lv.p[a]);
+          rv[j] = St::ld(lv.rhs[a]);
+          fv[j] = lv.eflag[a];
+          sv[j] = R.live[q] ? 1 : 0;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int i = first + j * lanes;
+      if (i < RG3) {
+        R.v[0][i] = pv[j];
+        R.rhs[i] = rv[j];
+        R.flag[i] = (uint8_t)fv[j];
+        R.state[i] = (int8_t)sv[j];
+      }
+    }
+  }
+  __syncthreads();
+}
 // Loads the region of tile slot t at level L (current iterate `cur`) into LDS; returns false
 // (for the whole workgroup) when t is not a live tile slot.
 template <class St>
@@ -978,29 +1025,7 @@ __device__ __forceinline__ bool loadRegion(const MGArgsT<typename St::T> &m, con
     R.live[threadIdx.x] = nt >= 0 && m.dir.tileRec[(size_t)nt * TILE_RECORD_WORDS + 3] != 0;
   }
   __syncthreads();
-  for (int i = (int)threadIdx.x; i < RG3; i += (int)blockDim.x) {
-    const int r[3] = {i % RG, (i / RG) % RG, i / (RG * RG)};
-    int q = 0, local = 0;
-    for (int a = 2; a >= 0; a--) {
-      const int g = r[a] + 1;  // cell tc * 4 - 3 + r: tile offset (g >> 2) - 1, local g & 3
-      q = q * 3 + (g >> 2);
-      local = local * 4 + (g & 3);
-    }
-    const int nt = R.nb[q];
-    if (nt < 0) {
-      R.v[0][i] = 0.f;
-      R.rhs[i] = 0.f;
-      R.flag[i] = 0;
-      R.state[i] = -1;
-    } else {
-      const size_t a = (size_t)nt * 64 + local;
-      R.v[0][i] = St::ld(lv.p[a]);
-      R.rhs[i] = St::ld(lv.rhs[a]);
-      R.flag[i] = lv.eflag[a];
-      R.state[i] = R.live[q] ? 1 : 0;
-    }
-  }
-  __syncthreads();
+  fillRegion<St>(lv, R);
   return true;
 }
 // One weighted Jacobi sweep (relaxCell) over region cells [lo, RG - lo)^3, src to dst.
@@ -1034,6 +1059,37 @@ __device__ __forceinline__ void storeTile(const TileRegion &R, int src, const Le
     const int i = (3 + (l & 3)) + RG * (3 + ((l >> 2) & 3)) + RG * RG * (3 + (l >> 4));
     lv.q[(size_t)t * 64 + l] = St::st(R.v[src][i]);
   }
+}
+// prolongateCell (pressure.wgsl:726-755) over the whole region of the cube at `org` (in
+// cubes), R.v[0] to R.v[1], from the 6^3 coarse values in R.coarse.
+template <class St>
+__device__ __forceinline__ void regionProlong(TileRegion &R, const int org[3]) {
+  for (int i = (int)threadIdx.x; i < RG3; i += (int)blockDim.x) {
+    if (R.state[i] != 1) {
+      R.v[1][i] = R.v[0][i];
+      continue;
+    }
+    const int e = R.flag[i];
+    if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
+      R.v[1][i] = 0.f;
+      continue;
+    }
+    const int p[3] = {org[0] * 4 - 3 + i % RG, org[1] * 4 - 3 + (i / RG) % RG, org[2] * 4 - 3 + i / (RG * RG)};
+    float f[3];
+    int b[3];
+    for (int k = 0; k < 3; k++) {
+      const float q = ((float)p[k] + 0.5f) * 0.5f - 0.5f;
+      const float fl = floorf(q);
+      b[k] = (int)fl - (org[k] * 2 - 2);
+      f[k] = q - fl;
+    }
+    const float *c0 = R.coarse + (b[2] * 6 + b[1]) * 6 + b[0];
+    const float v[8] = {c0[0], c0[1], c0[6], c0[7], c0[36], c0[37], c0[42], c0[43]};
+    const float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
+                            mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
+    R.v[1][i] = stored<St>(R.v[0][i] + corr);
+  }
+  __syncthreads();
 }
 // code: the flips (bits 0-15).
 template <class St>
@@ -1105,33 +1161,7 @@ __device__ __forceinline__ void mg_tiles_up_body(const MGArgsT<typename St::T> &
     }
   }
   __syncthreads();
-  // prolongateCell over the whole region into R.v[1].
-  for (int i = (int)threadIdx.x; i < RG3; i += (int)blockDim.x) {
-    if (R.state[i] != 1) {
-      R.v[1][i] = R.v[0][i];
-      continue;
-    }
-    const int e = R.flag[i];
-    if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
-      R.v[1][i] = 0.f;
-      continue;
-    }
-    const int p[3] = {tc[0] * 4 - 3 + i % RG, tc[1] * 4 - 3 + (i / RG) % RG, tc[2] * 4 - 3 + i / (RG * RG)};
-    float f[3];
-    int b[3];
-    for (int k = 0; k < 3; k++) {
-      const float q = ((float)p[k] + 0.5f) * 0.5f - 0.5f;
-      const float fl = floorf(q);
-      b[k] = (int)fl - (tc[k] * 2 - 2);
-      f[k] = q - fl;
-    }
-    const float *c0 = R.coarse + (b[2] * 6 + b[1]) * 6 + b[0];
-    const float v[8] = {c0[0], c0[1], c0[6], c0[7], c0[36], c0[37], c0[42], c0[43]};
-    const float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
-                            mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
-    R.v[1][i] = stored<St>(R.v[0][i] + corr);
-  }
-  __syncthreads();
+  regionProlong<St>(R, tc);
   const float w = m.weight;
   regionSweep<St>(R, 1, 1, w);
   regionSweep<St>(R, 0, 2, w);
@@ -1139,6 +1169,7 @@ __device__ __forceinline__ void mg_tiles_up_body(const MGArgsT<typename St::T> &
   storeTile<St>(R, 0, lv, t);
 }
 FP_MG_PAIR(mg_tiles_up, 256)
+
 
 // ---- the coarser tile levels of a V-cycle in one workgroup -------------------------------
 // For a simulation of at most TC_MAX tile slots, the V-cycle below the first tile level
