@@ -97,6 +97,7 @@ template <class T> struct MGArgsT {
   int bottom;
   float weight;     // axisWeights, equal on every axis for cubic cells
   int topologyAll;  // the geometry changed: rebuild every cell's topology
+  int topologyTiles = 0;  // the tile set changed: rebuild the tile levels' topology
   // The levels below the first tile level run in k_mg_coarse_cycle (at most TC_MAX tile
   // slots): their per-level kernels and the bottom kernels skip this simulation.
   int tilesFused = 0;
@@ -196,6 +197,7 @@ FP_DECLARE_PAIR1(topology_bricks, MGArgs, int)
 FP_DECLARE_PAIR1(topology_tiles, MGArgs, int)
 FP_DECLARE_PAIR(eflags_bricks, MGArgs)
 __global__ void k_fresh_topology(MGArgs m);
+__global__ void k_near_topology(MGArgs m, const int *tiles, int count);
 FP_DECLARE_PAIR(eflags_tiles, MGArgs)
 __global__ void k_mg_flags_init(MGArgs m, int brickGroups);  // eflags of every level + mg_init, one launch
 #define FP_DECLARE_MG(NAME)                                                                       \

@@ -272,6 +272,18 @@ __device__ __forceinline__ void topology_brick(const MGArgs &m, const int id) {
     }
   }
 }
+// Whether a collider can make any cell that a topology of brick `id` samples solid: the
+// brick's field cells and one brick around them against the colliders' box of field cells.
+// Elsewhere a brick's topology depends on the floor alone.
+__device__ __forceinline__ bool nearSolids(const MGArgs &m, int id) {
+  if (m.P.colliders == 0) return false;
+  const int *tr = m.dir.tileRec + (size_t)(id >> 6) * TILE_RECORD_WORDS;
+  const int l = id & 63, B = m.L.B;
+  const int b[3] = {tr[0] * 4 + (l & 3), tr[1] * 4 + ((l >> 2) & 3), tr[2] * 4 + (l >> 4)};
+  for (int a = 0; a < 3; a++)
+    if (b[a] * B + 2 * B - 1 < m.P.solidLo[a] || b[a] * B - B > m.P.solidHi[a]) return false;
+  return true;
+}
 __device__ __forceinline__ void topology_bricks_body(const MGArgs &m, const int vb, int fromZeroList) {
   // With the geometry changed, the pass over every active brick (launched first) has
   // covered the zeroed list's bricks that hold a slot.
@@ -291,10 +303,20 @@ __global__ void __launch_bounds__(256) k_fresh_topology(MGArgs m) {
   }
 }
 
+// When only the tile set changed: the bricks of the listed tile slots (64 workgroups a tile)
+// that hold a slot and lie near the colliders (nearSolids).
+__global__ void __launch_bounds__(256) k_near_topology(MGArgs m, const int *tiles, int count) {
+  const int vb = (int)blockIdx.x;
+  if (vb >= count * 64) return;
+  const int id = tiles[vb >> 6] * 64 + (vb & 63);
+  if (m.dir.entries[id] < 0 || !nearSolids(m, id)) return;
+  topology_brick(m, id);
+}
+
 // buildTileTopology (pressure.wgsl:281-305) of one tile level (`only`), or of every tile
 // level (`only` < 0), for every tile slot.
 __device__ __forceinline__ void topology_tiles_body(const MGArgs &m, const int vb, int only) {
-  if (!m.topologyAll) return;
+  if (!m.topologyAll && !m.topologyTiles) return;
   const int t = vb;
   if (t >= m.dir.capacity) return;
   const int first = only < 0 ? m.L.brickLevels : only, last = only < 0 ? m.bottom : only;

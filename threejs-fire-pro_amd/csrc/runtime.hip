@@ -444,6 +444,9 @@ struct fp_sim {
   double time = 0;
   int64_t steps = 0;
   bool mayHaveFlame = false, solidsStale = true, geometryDirty = true;
+  // The tile set changed since the topology was last built (and nothing else that
+  // geometryDirty stands for): tile slots moved, bricks came and went.
+  bool tilesMoved = false;
   hipEvent_t readbackEvent = nullptr;  // recorded after each step's activity readback
   // Work an entry point left queued on the stream when it returned (a step's tail after its
   // readback, a render into a caller's device buffer); drain() waits for it.
@@ -464,6 +467,9 @@ struct fp_sim {
 
   // Per brick id.
   DBuf<int> tileRec, hash, pages, entries, records, active, zeroList, poolState, counts, createdSlots, readback;
+  // Tile slots near the colliders (nearTopology).
+  DBuf<int> nearTiles;
+  std::vector<int> nearTilesHost;
   DBuf<uint32_t> brickState, activity, stateTmp, rankTmp, blockSums, blockPrefix, freeSlots;
   // Pools, per slot.
   DBuf<float4> F, Fp, Fn, Curl;
@@ -858,7 +864,7 @@ struct fp_sim {
     }
     // The copies read tileRecHost, hashHost and created when the stream reaches them; each
     // step waits for its readback, queued after them, before these are rebuilt.
-    geometryDirty = true;
+    tilesMoved = true;
     rboxes.stale = true;  // FluidSimulation.ts:708-709
   }
 
@@ -1535,11 +1541,41 @@ struct fp_sim {
     if (profiling) collect();
   }
 
+  // The topology of the bricks near the colliders: the live tile slots whose bricks lie
+  // within a brick of the colliders' box of field cells, listed on the host, then every
+  // brick of them that holds a slot tested on the device (nearSolids).
+  void nearTopology(const MGArgs &m, unsigned lanes) {
+    nearTilesHost.clear();
+    const long long side = 4LL * B;  // field cells a tile side
+    for (int t = 0; t < capacity && t < (int)dir.coords.size(); t++) {
+      const std::array<int, 4> &c = dir.coords[t];
+      if (!c[3]) continue;
+      bool near = true;
+      for (int a = 0; a < 3; a++)
+        if ((long long)c[a] * side + side + B - 1 < solidLo[a] || (long long)c[a] * side - B > solidHi[a]) near = false;
+      if (near) nearTilesHost.push_back(t);
+    }
+    if (nearTilesHost.empty()) return;
+    if (nearTiles.n < nearTilesHost.size()) {
+      drain();
+      nearTiles.alloc(std::max<size_t>(nearTilesHost.size(), 64));
+    }
+    // Read when the stream reaches the copy; each step waits for its readback, queued after
+    // it, before the list is rebuilt.
+    HIPCHECK(hipMemcpyAsync(nearTiles.p, nearTilesHost.data(), nearTilesHost.size() * sizeof(int),
+                            hipMemcpyHostToDevice, stream));
+    k_near_topology<<<(unsigned)nearTilesHost.size() * 64, lanes, 0, stream>>>(m, nearTiles.p,
+                                                                               (int)nearTilesHost.size());
+  }
+
   // PressureSolver.solve (PressureSolver.ts:371-402): one cold-start V-cycle, or the
   // defect-correction loop around V-cycles.
   void solvePressure(unsigned slotsGrid) {
     MGArgs m = mgArgs();
+    const bool tiles = tilesMoved;
+    tilesMoved = false;
     m.topologyAll = geometryDirty ? 1 : 0;
+    m.topologyTiles = geometryDirty || tiles ? 1 : 0;
     // The brick levels hold at most nV cells per brick.
     const unsigned vLanes = (unsigned)std::min(256, std::max(64, nV));
     if (geometryDirty) {
@@ -1549,6 +1585,14 @@ struct fp_sim {
       geometryDirty = false;
     } else {
       k_fresh_topology<<<std::min(slotsGrid * 2, FRESH_GRID), vLanes, 0, stream>>>(m);
+      if (tiles) {
+        // Only the tile set changed: the tile levels' topology in full (their slots moved), and
+        // of the bricks only those whose cells a collider may reach; elsewhere a brick's
+        // topology depends on the floor alone and is unchanged. A new brick beside a collider
+        // can close its neighbors' sides, so those are built again.
+        if (colliderCount > 0) nearTopology(m, vLanes);
+        if (bottom >= brickLevels) k_topology_tiles<<<capacity, 64, 0, stream>>>(m, -1);
+      }
     }
     if (pressureMode == 0) {
       // The flags of every level and level 0's first iterate in one launch.
@@ -1586,6 +1630,7 @@ struct fp_sim {
     h.bottom = m.bottom;
     h.weight = m.weight;
     h.topologyAll = m.topologyAll;
+    h.topologyTiles = m.topologyTiles;
     h.tilesFused = m.tilesFused;
     for (int L = 0; L <= bottom; L++) {
       h.lv[L].p = reinterpret_cast<T *>(lv16[L].p.p);
@@ -2650,8 +2695,10 @@ struct fp_batch {
       std::swap(s.F, s.Fn);
       post[i] = s.stepArgs();
       mg[i] = s.mgArgs();
-      mg[i].topologyAll = s.geometryDirty ? 1 : 0;
+      mg[i].topologyAll = s.geometryDirty || s.tilesMoved ? 1 : 0;
+      mg[i].topologyTiles = mg[i].topologyAll;
       s.geometryDirty = false;
+      s.tilesMoved = false;
       if (mode != 0) {
         s.ensureCorrectionBuffers();
         if (mode == 1) solve32.push_back(s.correctionArgs32(mg[i]));
