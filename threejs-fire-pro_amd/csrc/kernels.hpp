@@ -217,6 +217,8 @@ FP_DECLARE_MG(mg_bottom64)  // the same in FP64, up to BOTTOM64_TILES tile slots
 // it and its three sweeps, one workgroup per tile slot over the tile and a three-cell halo.
 FP_DECLARE_MG(mg_tiles_down)
 FP_DECLARE_MG(mg_tiles_up)
+FP_DECLARE_MG(mg_bricks_down)
+FP_DECLARE_MG(mg_bricks_up)
 // The V-cycle below the first tile level in one workgroup, for simulations of at most TC_MAX
 // tile slots (MGArgsT::tilesFused).
 FP_DECLARE_MG(mg_coarse_cycle)

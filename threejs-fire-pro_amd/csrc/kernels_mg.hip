@@ -977,6 +977,7 @@ constexpr int RG = 10, RG3 = RG * RG * RG;
 // A value as its level's storage reads it back.
 template <class St> __device__ __forceinline__ float stored(float x) { return St::ld(St::st(x)); }
 struct TileRegion {
+  static constexpr int N = RG;
   float v[2][RG3];
   float rhs[RG3];
   uint8_t flag[RG3];
@@ -1050,15 +1051,22 @@ __device__ __forceinline__ bool loadRegion(const MGArgsT<typename St::T> &m, con
   fillRegion<St>(lv, R);
   return true;
 }
-// One weighted Jacobi sweep (relaxCell) over region cells [lo, RG - lo)^3, src to dst.
-template <class St>
-__device__ __forceinline__ void regionSweep(TileRegion &R, int src, int lo, float w) {
+// Whether region cell i keeps its value through a sweep (a cell of a tile that is not live, or
+// of no tile).
+__device__ __forceinline__ bool held(const TileRegion &R, int i) { return R.state[i] != 1; }
+struct BrickRegion;
+__device__ __forceinline__ bool held(const BrickRegion &R, int i);
+// One weighted Jacobi sweep (relaxCell) over region cells [lo, N - lo)^3 of a region N
+// cells a side, src to dst.
+template <class St, class Rg>
+__device__ __forceinline__ void regionSweep(Rg &R, int src, int lo, float w) {
+  constexpr int RG = Rg::N;
   const int n = RG - 2 * lo, n3 = n * n * n;
   const float *a = R.v[src];
   float *d = R.v[src ^ 1];
   for (int j = (int)threadIdx.x; j < n3; j += (int)blockDim.x) {
     const int i = (lo + j % n) + RG * (lo + (j / n) % n) + RG * RG * (lo + j / (n * n));
-    if (R.state[i] != 1) {
+    if (held(R, i)) {
       d[i] = a[i];
       continue;
     }
@@ -1191,6 +1199,200 @@ __device__ __forceinline__ void mg_tiles_up_body(const MGArgsT<typename St::T> &
   storeTile<St>(R, 0, lv, t);
 }
 FP_MG_PAIR(mg_tiles_up, 256)
+
+// ---- the last brick level, a tile of bricks and its halo per workgroup -------------------
+// When the last brick level holds 2^3 cells a brick (and the first tile level one), its two
+// sweeps and the restriction below it run in one launch (k_mg_bricks_down), and the
+// prolongation into it and its three sweeps in another (k_mg_bricks_up), as the first tile
+// level's halo kernels do. A workgroup takes one tile slot in use: the level's 8^3 cells of
+// its 4^3 bricks and the cells within three of them, 14^3 cells of the 8^3 bricks around,
+// whose pool slots come from the 27 neighbor tiles' brick entries. Every brick with a pool
+// slot is active and swept by the per-level kernels, so cells of a brick with a slot are
+// updated, and cells of a brick without one read as zero: their flag byte is zero (not an
+// unknown), so a sweep or the prolongation writes them zero again, and no per-cell state is
+// kept (39.9 KB of LDS, four workgroups a CU). Each kernel stores the tile's bricks' cells
+// into the level's other array; the caller flips the level.
+constexpr int BR = 14, BR3 = BR * BR * BR;
+struct BrickRegion {
+  static constexpr int N = BR;
+  float v[2][BR3];
+  float rhs[BR3];
+  uint8_t flag[BR3];
+  int slot[512];      // the region's 8^3 bricks (tile-relative -2 .. 5 on each axis)
+  int tnb[27];
+  float coarse[512];  // k_mg_bricks_up: the 8^3 first-tile-level values its prolongation weighs
+};
+__device__ __forceinline__ bool held(const BrickRegion &, int) { return false; }
+// Loads level L's region around tile slot t; false (for the whole workgroup) when t is not
+// in use. tc: the tile's coordinates.
+template <class St>
+__device__ __forceinline__ bool loadBrickRegion(const MGArgsT<typename St::T> &m, const LevelView<typename St::T> &lv,
+                                                int t, BrickRegion &R, int tc[3]) {
+  const int *trec = m.dir.tileRec + (size_t)t * TILE_RECORD_WORDS;
+  if (trec[3] == 0) return false;
+  tc[0] = trec[0];
+  tc[1] = trec[1];
+  tc[2] = trec[2];
+  if (threadIdx.x < 27) R.tnb[threadIdx.x] = trec[4 + threadIdx.x];
+  __syncthreads();
+  for (int j = (int)threadIdx.x; j < 512; j += (int)blockDim.x) {
+    // Brick b = j's coordinates - 2 (tile-relative): tile offset b >> 2, brick b & 3.
+    const int b[3] = {(j & 7) - 2, ((j >> 3) & 7) - 2, (j >> 6) - 2};
+    const int nt = R.tnb[((b[2] >> 2) + 1) * 9 + ((b[1] >> 2) + 1) * 3 + ((b[0] >> 2) + 1)];
+    R.slot[j] = nt < 0 ? -1 : entrySlot(m.dir.entries[(size_t)nt * 64 + (((b[2] & 3) * 4 + (b[1] & 3)) * 4 + (b[0] & 3))],
+                                        m.L.side);
+  }
+  __syncthreads();
+  // Every cell's loads in one pass (11 a lane at 256 lanes): the region's LDS, not registers,
+  // bounds the resident workgroups.
+  constexpr int U = (BR3 + 255) / 256;
+  const int lanes = (int)blockDim.x;
+  for (int first = (int)threadIdx.x; first < BR3; first += U * lanes) {
+    float pv[U], rv[U];
+    int fv[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int i = first + j * lanes;
+      pv[j] = rv[j] = 0.f;
+      fv[j] = 0;
+      if (i < BR3) {
+        // Cell g = r - 3 (tile-relative): brick (g >> 1) + 2 of the region, local g & 1.
+        const int g[3] = {i % BR - 3, (i / BR) % BR - 3, i / (BR * BR) - 3};
+        const int sl = R.slot[(((g[2] >> 1) + 2) * 8 + ((g[1] >> 1) + 2)) * 8 + ((g[0] >> 1) + 2)];
+        if (sl >= 0) {
+          const size_t a = (size_t)sl * 8 + (((g[2] & 1) * 2 + (g[1] & 1)) * 2 + (g[0] & 1));
+          pv[j] = St::ld(lv.p[a]);
+          rv[j] = St::ld(lv.rhs[a]);
+          fv[j] = lv.eflag[a];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int i = first + j * lanes;
+      if (i < BR3) {
+        R.v[0][i] = pv[j];
+        R.rhs[i] = rv[j];
+        R.flag[i] = (uint8_t)fv[j];
+      }
+    }
+  }
+  __syncthreads();
+  return true;
+}
+// The tile's bricks' cells (region [3, 11)^3) from R.v[src] into lv.q.
+template <class St>
+__device__ __forceinline__ void storeBricks(const BrickRegion &R, int src, const LevelView<typename St::T> &lv) {
+  for (int l = (int)threadIdx.x; l < 512; l += (int)blockDim.x) {
+    const int g[3] = {l & 7, (l >> 3) & 7, l >> 6};
+    const int sl = R.slot[(((g[2] >> 1) + 2) * 8 + ((g[1] >> 1) + 2)) * 8 + ((g[0] >> 1) + 2)];
+    if (sl < 0) continue;
+    const int i = (3 + g[0]) + BR * (3 + g[1]) + BR * BR * (3 + g[2]);
+    lv.q[(size_t)sl * 8 + (((g[2] & 1) * 2 + (g[1] & 1)) * 2 + (g[0] & 1))] = St::st(R.v[src][i]);
+  }
+}
+// code: the flips (bits 0-15). Level L = brickLevels - 1.
+template <class St>
+__device__ __forceinline__ void mg_bricks_down_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int L = m.L.brickLevels - 1, t = vb;
+  if (t >= m.dir.capacity) return;
+  __shared__ BrickRegion R;
+  const auto lv = levelView(m, L, code);
+  int tc[3];
+  if (!loadBrickRegion<St>(m, lv, t, R, tc)) return;
+  const float w = m.weight;
+  regionSweep<St>(R, 0, 1, w);
+  regionSweep<St>(R, 1, 2, w);
+  storeBricks<St>(R, 0, lv);
+  // restrictCell into the first tile level's cell of each of the tile's 64 bricks, from the
+  // swept values (its eight children are the brick's 2^3 cells).
+  if (threadIdx.x < 64) {
+    const auto coarse = levelView(m, L + 1, code);
+    const int l = (int)threadIdx.x, bx = l & 3, by = (l >> 2) & 3, bz = l >> 4;
+    const float *a = R.v[0];
+    const int i0 = (3 + 2 * bx) + BR * (3 + 2 * by) + BR * BR * (3 + 2 * bz);
+    float residual = 0.f;
+    bool every = true;
+    for (int k = 0; k < 8; k++) {
+      const int i = i0 + (k & 1) + BR * ((k >> 1) & 1) + BR * BR * (k >> 2);
+      const int e = R.flag[i];
+      const bool unknown = (e & UNKNOWN) != 0;
+      every = every && unknown;
+      if (!unknown || (e & STORED_SOLID)) continue;
+      const int flags = e & 63;
+      const float nb[6] = {a[i + 1], a[i - 1], a[i + BR], a[i - BR], a[i + BR * BR], a[i - BR * BR]};
+      residual += R.rhs[i] - (openSum(nb, flags, w) - diagonal(w, flags) * a[i]);
+    }
+    const float rhs = residual * 0.5f;
+    const size_t out = (size_t)t * 64 + l;
+    coarse.rhs[out] = St::st(rhs);
+    float first = 0.f;
+    if (every) {
+      const int e = coarse.eflag[out];
+      if (!(e & STORED_SOLID)) first = mixv(0.f, (0.f - rhs) / fmaxv(diagonal(w, e & 63), 1e-12f), 0.8f);
+    }
+    coarse.p[out] = St::st(first);
+  }
+}
+FP_MG_PAIR(mg_bricks_down, 256)
+template <class St>
+__device__ __forceinline__ void mg_bricks_up_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  const int L = m.L.brickLevels - 1, t = vb;
+  if (t >= m.dir.capacity) return;
+  __shared__ BrickRegion R;
+  const auto lv = levelView(m, L, code);
+  int tc[3];
+  if (!loadBrickRegion<St>(m, lv, t, R, tc)) return;
+  // The first tile level's cells [tc * 4 - 2, tc * 4 + 6) on each axis, resolved once
+  // (prolongValue).
+  {
+    MGCtx c;
+    c.dir = m.dir;
+    c.L = m.L;
+    c.side = m.L.side;
+    c.hasBrick = false;
+    c.hasTile = true;
+    c.tc[0] = tc[0];
+    c.tc[1] = tc[1];
+    c.tc[2] = tc[2];
+    c.tnb = R.tnb;
+    const auto cv = levelView(m, L + 1, code);
+    for (int j = (int)threadIdx.x; j < 512; j += (int)blockDim.x) {
+      const int cc[3] = {tc[0] * 4 - 2 + (j & 7), tc[1] * 4 - 2 + ((j >> 3) & 7), tc[2] * 4 - 2 + (j >> 6)};
+      R.coarse[j] = prolongValue<St>(c, m, cv, cc, L + 1);
+    }
+  }
+  __syncthreads();
+  // prolongateCell over the whole region into R.v[1].
+  for (int i = (int)threadIdx.x; i < BR3; i += (int)blockDim.x) {
+    const int e = R.flag[i];
+    if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
+      R.v[1][i] = 0.f;
+      continue;
+    }
+    const int p[3] = {tc[0] * 8 - 3 + i % BR, tc[1] * 8 - 3 + (i / BR) % BR, tc[2] * 8 - 3 + i / (BR * BR)};
+    float f[3];
+    int b[3];
+    for (int k = 0; k < 3; k++) {
+      const float q = ((float)p[k] + 0.5f) * 0.5f - 0.5f;
+      const float fl = floorf(q);
+      b[k] = (int)fl - (tc[k] * 4 - 2);
+      f[k] = q - fl;
+    }
+    const float *c0 = R.coarse + (b[2] * 8 + b[1]) * 8 + b[0];
+    const float v[8] = {c0[0], c0[1], c0[8], c0[9], c0[64], c0[65], c0[72], c0[73]};
+    const float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
+                            mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
+    R.v[1][i] = stored<St>(R.v[0][i] + corr);
+  }
+  __syncthreads();
+  const float w = m.weight;
+  regionSweep<St>(R, 1, 1, w);
+  regionSweep<St>(R, 0, 2, w);
+  regionSweep<St>(R, 1, 3, w);
+  storeBricks<St>(R, 0, lv);
+}
+FP_MG_PAIR(mg_bricks_up, 256)
 
 
 // ---- the coarser tile levels of a V-cycle in one workgroup -------------------------------
@@ -1751,6 +1953,8 @@ template __global__ void k_level_prefix_batch<StoreF16>(const MGArgsT<_Float16> 
   FP_MG_INSTANCE(mg_bottom64, St)          \
   FP_MG_INSTANCE(mg_tiles_down, St)        \
   FP_MG_INSTANCE(mg_tiles_up, St)          \
+  FP_MG_INSTANCE(mg_bricks_down, St)       \
+  FP_MG_INSTANCE(mg_bricks_up, St)         \
   FP_MG_INSTANCE(mg_coarse_cycle, St)      \
   FP_MG_INSTANCE(dc_start, St)             \
   FP_MG_INSTANCE(dc_residual, St)          \

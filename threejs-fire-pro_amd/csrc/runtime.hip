@@ -1753,6 +1753,18 @@ struct fp_sim {
       flips ^= 1 << L;
       return;
     }
+    if (L >= 1 && L == brickLevels - 1 && m.lv[L].side == 2) {
+      // The last brick level at 2^3 cells a brick: its two sweeps and the restriction below
+      // it, then after the tile levels the prolongation into it and its three sweeps, each
+      // one launch of a tile of bricks and its halo per workgroup (the result in the other
+      // array).
+      k_mg_bricks_down<St><<<capacity, 256, 0, stream>>>(m, flips);
+      flips ^= 1 << L;
+      cycle<St>(m, L + 1, slotsGrid, bottom64, flips);
+      k_mg_bricks_up<St><<<capacity, 256, 0, stream>>>(m, flips);
+      flips ^= 1 << L;
+      return;
+    }
     if (L == 0) {
       if (!initDone) k_mg_init<St><<<groupGrid(0), 256, 0, stream>>>(m, flips << 8);
     } else {
