@@ -1054,8 +1054,8 @@ __device__ __forceinline__ bool loadRegion(const MGArgsT<typename St::T> &m, con
 // Whether region cell i keeps its value through a sweep (a cell of a tile that is not live, or
 // of no tile).
 __device__ __forceinline__ bool held(const TileRegion &R, int i) { return R.state[i] != 1; }
-struct BrickRegion;
-__device__ __forceinline__ bool held(const BrickRegion &R, int i);
+template <int SD> struct BrickRegion;
+template <int SD> __device__ __forceinline__ bool held(const BrickRegion<SD> &R, int i);
 // One weighted Jacobi sweep (relaxCell) over region cells [lo, N - lo)^3 of a region N
 // cells a side, src to dst.
 template <class St, class Rg>
@@ -1200,34 +1200,46 @@ __device__ __forceinline__ void mg_tiles_up_body(const MGArgsT<typename St::T> &
 }
 FP_MG_PAIR(mg_tiles_up, 256)
 
-// ---- the last brick level, a tile of bricks and its halo per workgroup -------------------
-// When the last brick level holds 2^3 cells a brick (and the first tile level one), its two
-// sweeps and the restriction below it run in one launch (k_mg_bricks_down), and the
-// prolongation into it and its three sweeps in another (k_mg_bricks_up), as the first tile
-// level's halo kernels do. A workgroup takes one tile slot in use: the level's 8^3 cells of
-// its 4^3 bricks and the cells within three of them, 14^3 cells of the 8^3 bricks around,
-// whose pool slots come from the 27 neighbor tiles' brick entries. Every brick with a pool
-// slot is active and swept by the per-level kernels, so cells of a brick with a slot are
-// updated, and cells of a brick without one read as zero: their flag byte is zero (not an
-// unknown), so a sweep or the prolongation writes them zero again, and no per-cell state is
-// kept (39.9 KB of LDS, four workgroups a CU). Each kernel stores the tile's bricks' cells
-// into the level's other array; the caller flips the level.
-constexpr int BR = 14, BR3 = BR * BR * BR;
-struct BrickRegion {
-  static constexpr int N = BR;
-  float v[2][BR3];
-  float rhs[BR3];
-  uint8_t flag[BR3];
-  int slot[512];      // the region's 8^3 bricks (tile-relative -2 .. 5 on each axis)
+// ---- brick levels by tile of bricks and halo ---------------------------------------------
+// A brick level whose cubes hold SD^3 cells runs its sweeps with the restriction below them
+// (k_mg_bricks_down) and the prolongation into it with its three sweeps (k_mg_bricks_up) in
+// one launch each, as the first tile level's halo kernels do. They run at the last brick
+// level when it holds 2^3 cells a brick (above a first tile level of one cell a brick). At
+// level 0 of 4^3 cells (SD = 4: 22^3 cells and 146 KB of LDS a tile, one workgroup a CU) the
+// same scheme took 73 us against 65 us for the launches it would replace: the regions of
+// sparse tiles are mostly empty cells, which every sweep still visits. A workgroup takes one tile slot in use: the level's
+// (4 SD)^3 cells of its 4^3 bricks and the cells within three of them, whose pool slots
+// come from the 27 neighbor tiles' brick entries. Every brick with a pool slot is active and
+// swept by the per-level kernels, so cells of a brick with a slot are updated, and cells of
+// a brick without one read as zero: their flag byte is zero (not an unknown), so a sweep or
+// the prolongation writes them zero again, and no per-cell state is kept. Each kernel stores
+// the tile's bricks' cells into the level's other array; the caller flips the level.
+template <int SD> struct BrickRegion {
+  static constexpr int N = 4 * SD + 6, N3 = N * N * N;
+  static constexpr int B0 = SD == 2 ? -2 : -1, NB = SD == 2 ? 8 : 6;  // region bricks B0 .. B0 + NB - 1
+  static constexpr int NC = SD == 2 ? 8 : 12;  // coarse cells a side the prolongation weighs
+  float v[2][N3];
+  float rhs[N3];
+  uint8_t flag[N3];
+  int slot[NB * NB * NB];  // the region's bricks (tile-relative B0 .. B0 + NB - 1 on each axis)
   int tnb[27];
-  float coarse[512];  // k_mg_bricks_up: the 8^3 first-tile-level values its prolongation weighs
+  float coarse[NC * NC * NC];  // the coarse values the prolongation weighs
 };
-__device__ __forceinline__ bool held(const BrickRegion &, int) { return false; }
+template <int SD> __device__ __forceinline__ bool held(const BrickRegion<SD> &, int) { return false; }
+// The region's pool slot of the brick holding tile-relative cell g (tile-relative bricks
+// B0 ..), -1 without one, and the cell's index inside its brick.
+template <int SD>
+__device__ __forceinline__ int regionSlot(const BrickRegion<SD> &R, const int g[3], int &local) {
+  constexpr int sh = SD == 2 ? 1 : 2, NB = BrickRegion<SD>::NB, B0 = BrickRegion<SD>::B0;
+  local = (((g[2] & (SD - 1)) * SD + (g[1] & (SD - 1))) * SD) + (g[0] & (SD - 1));
+  return R.slot[(((g[2] >> sh) - B0) * NB + ((g[1] >> sh) - B0)) * NB + ((g[0] >> sh) - B0)];
+}
 // Loads level L's region around tile slot t; false (for the whole workgroup) when t is not
 // in use. tc: the tile's coordinates.
-template <class St>
+template <class St, int SD>
 __device__ __forceinline__ bool loadBrickRegion(const MGArgsT<typename St::T> &m, const LevelView<typename St::T> &lv,
-                                                int t, BrickRegion &R, int tc[3]) {
+                                                int t, BrickRegion<SD> &R, int tc[3]) {
+  using Rg = BrickRegion<SD>;
   const int *trec = m.dir.tileRec + (size_t)t * TILE_RECORD_WORDS;
   if (trec[3] == 0) return false;
   tc[0] = trec[0];
@@ -1235,19 +1247,20 @@ __device__ __forceinline__ bool loadBrickRegion(const MGArgsT<typename St::T> &m
   tc[2] = trec[2];
   if (threadIdx.x < 27) R.tnb[threadIdx.x] = trec[4 + threadIdx.x];
   __syncthreads();
-  for (int j = (int)threadIdx.x; j < 512; j += (int)blockDim.x) {
-    // Brick b = j's coordinates - 2 (tile-relative): tile offset b >> 2, brick b & 3.
-    const int b[3] = {(j & 7) - 2, ((j >> 3) & 7) - 2, (j >> 6) - 2};
+  for (int j = (int)threadIdx.x; j < Rg::NB * Rg::NB * Rg::NB; j += (int)blockDim.x) {
+    // Tile-relative brick b: tile offset b >> 2, brick b & 3.
+    const int b[3] = {j % Rg::NB + Rg::B0, (j / Rg::NB) % Rg::NB + Rg::B0, j / (Rg::NB * Rg::NB) + Rg::B0};
     const int nt = R.tnb[((b[2] >> 2) + 1) * 9 + ((b[1] >> 2) + 1) * 3 + ((b[0] >> 2) + 1)];
     R.slot[j] = nt < 0 ? -1 : entrySlot(m.dir.entries[(size_t)nt * 64 + (((b[2] & 3) * 4 + (b[1] & 3)) * 4 + (b[0] & 3))],
                                         m.L.side);
   }
   __syncthreads();
-  // Every cell's loads in one pass (11 a lane at 256 lanes): the region's LDS, not registers,
-  // bounds the resident workgroups.
-  constexpr int U = (BR3 + 255) / 256;
+  // Every cell's loads in one pass: the region's LDS, not registers, bounds the resident
+  // workgroups.
+  constexpr int N = Rg::N, N3 = Rg::N3, n = SD * SD * SD;
   const int lanes = (int)blockDim.x;
-  for (int first = (int)threadIdx.x; first < BR3; first += U * lanes) {
+  constexpr int U = SD == 2 ? (N3 + 255) / 256 : (N3 + 1023) / 1024;
+  for (int first = (int)threadIdx.x; first < N3; first += U * lanes) {
     float pv[U], rv[U];
     int fv[U];
 #pragma unroll
@@ -1255,12 +1268,12 @@ __device__ __forceinline__ bool loadBrickRegion(const MGArgsT<typename St::T> &m
       const int i = first + j * lanes;
       pv[j] = rv[j] = 0.f;
       fv[j] = 0;
-      if (i < BR3) {
-        // Cell g = r - 3 (tile-relative): brick (g >> 1) + 2 of the region, local g & 1.
-        const int g[3] = {i % BR - 3, (i / BR) % BR - 3, i / (BR * BR) - 3};
-        const int sl = R.slot[(((g[2] >> 1) + 2) * 8 + ((g[1] >> 1) + 2)) * 8 + ((g[0] >> 1) + 2)];
+      if (i < N3) {
+        const int g[3] = {i % N - 3, (i / N) % N - 3, i / (N * N) - 3};  // tile-relative cell
+        int local;
+        const int sl = regionSlot(R, g, local);
         if (sl >= 0) {
-          const size_t a = (size_t)sl * 8 + (((g[2] & 1) * 2 + (g[1] & 1)) * 2 + (g[0] & 1));
+          const size_t a = (size_t)sl * n + local;
           pv[j] = St::ld(lv.p[a]);
           rv[j] = St::ld(lv.rhs[a]);
           fv[j] = lv.eflag[a];
@@ -1270,7 +1283,7 @@ __device__ __forceinline__ bool loadBrickRegion(const MGArgsT<typename St::T> &m
 #pragma unroll
     for (int j = 0; j < U; j++) {
       const int i = first + j * lanes;
-      if (i < BR3) {
+      if (i < N3) {
         R.v[0][i] = pv[j];
         R.rhs[i] = rv[j];
         R.flag[i] = (uint8_t)fv[j];
@@ -1280,23 +1293,78 @@ __device__ __forceinline__ bool loadBrickRegion(const MGArgsT<typename St::T> &m
   __syncthreads();
   return true;
 }
-// The tile's bricks' cells (region [3, 11)^3) from R.v[src] into lv.q.
-template <class St>
-__device__ __forceinline__ void storeBricks(const BrickRegion &R, int src, const LevelView<typename St::T> &lv) {
-  for (int l = (int)threadIdx.x; l < 512; l += (int)blockDim.x) {
-    const int g[3] = {l & 7, (l >> 3) & 7, l >> 6};
-    const int sl = R.slot[(((g[2] >> 1) + 2) * 8 + ((g[1] >> 1) + 2)) * 8 + ((g[0] >> 1) + 2)];
+// The tile's bricks' cells (region [3, 3 + 4 SD)^3) from R.v[src] into lv.q.
+template <class St, int SD>
+__device__ __forceinline__ void storeBricks(const BrickRegion<SD> &R, int src, const LevelView<typename St::T> &lv) {
+  constexpr int N = BrickRegion<SD>::N, T = 4 * SD, n = SD * SD * SD;
+  for (int l = (int)threadIdx.x; l < T * T * T; l += (int)blockDim.x) {
+    const int g[3] = {l % T, (l / T) % T, l / (T * T)};
+    int local;
+    const int sl = regionSlot(R, g, local);
     if (sl < 0) continue;
-    const int i = (3 + g[0]) + BR * (3 + g[1]) + BR * BR * (3 + g[2]);
-    lv.q[(size_t)sl * 8 + (((g[2] & 1) * 2 + (g[1] & 1)) * 2 + (g[0] & 1))] = St::st(R.v[src][i]);
+    const int i = (3 + g[0]) + N * (3 + g[1]) + N * N * (3 + g[2]);
+    lv.q[(size_t)sl * n + local] = St::st(R.v[src][i]);
   }
 }
-// code: the flips (bits 0-15). Level L = brickLevels - 1.
+// prolongateCell over the whole region into R.v[1], from the NC^3 coarse values in R.coarse
+// (coarse cells from the tile's coarse origin - 2; the coarse level holds 2 SD cells a tile
+// side).
+template <class St, int SD>
+__device__ __forceinline__ void brickRegionProlong(BrickRegion<SD> &R, const int tc[3]) {
+  constexpr int N = BrickRegion<SD>::N, N3 = BrickRegion<SD>::N3, NC = BrickRegion<SD>::NC;
+  for (int i = (int)threadIdx.x; i < N3; i += (int)blockDim.x) {
+    const int e = R.flag[i];
+    if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
+      R.v[1][i] = 0.f;
+      continue;
+    }
+    const int p[3] = {tc[0] * 4 * SD - 3 + i % N, tc[1] * 4 * SD - 3 + (i / N) % N, tc[2] * 4 * SD - 3 + i / (N * N)};
+    float f[3];
+    int b[3];
+    for (int k = 0; k < 3; k++) {
+      const float q = ((float)p[k] + 0.5f) * 0.5f - 0.5f;
+      const float fl = floorf(q);
+      b[k] = (int)fl - (tc[k] * 2 * SD - 2);
+      f[k] = q - fl;
+    }
+    const float *c0 = R.coarse + (b[2] * NC + b[1]) * NC + b[0];
+    const float v[8] = {c0[0], c0[1], c0[NC], c0[NC + 1], c0[NC * NC], c0[NC * NC + 1], c0[NC * NC + NC],
+                        c0[NC * NC + NC + 1]};
+    const float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
+                            mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
+    R.v[1][i] = stored<St>(R.v[0][i] + corr);
+  }
+  __syncthreads();
+}
+// The coarse values the prolongation into level L weighs (prolongValue at level L + 1).
+template <class St, int SD>
+__device__ __forceinline__ void stageBrickCoarse(const MGArgsT<typename St::T> &m, int L, int code, BrickRegion<SD> &R,
+                                                 const int tc[3]) {
+  constexpr int NC = BrickRegion<SD>::NC;
+  MGCtx c;
+  c.dir = m.dir;
+  c.L = m.L;
+  c.side = m.L.side;
+  c.hasBrick = false;
+  c.hasTile = true;
+  c.tc[0] = tc[0];
+  c.tc[1] = tc[1];
+  c.tc[2] = tc[2];
+  c.tnb = R.tnb;
+  const auto cv = levelView(m, L + 1, code);
+  for (int j = (int)threadIdx.x; j < NC * NC * NC; j += (int)blockDim.x) {
+    const int cc[3] = {tc[0] * 2 * SD - 2 + j % NC, tc[1] * 2 * SD - 2 + (j / NC) % NC, tc[2] * 2 * SD - 2 + j / (NC * NC)};
+    R.coarse[j] = prolongValue<St>(c, m, cv, cc, L + 1);
+  }
+  __syncthreads();
+}
+// code: the flips (bits 0-15). Level L = brickLevels - 1, 2^3 cells a brick.
 template <class St>
 __device__ __forceinline__ void mg_bricks_down_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
   const int L = m.L.brickLevels - 1, t = vb;
   if (t >= m.dir.capacity) return;
-  __shared__ BrickRegion R;
+  __shared__ BrickRegion<2> R;
+  constexpr int BR = BrickRegion<2>::N;
   const auto lv = levelView(m, L, code);
   int tc[3];
   if (!loadBrickRegion<St>(m, lv, t, R, tc)) return;
@@ -1335,62 +1403,25 @@ __device__ __forceinline__ void mg_bricks_down_body(const MGArgsT<typename St::T
   }
 }
 FP_MG_PAIR(mg_bricks_down, 256)
-template <class St>
-__device__ __forceinline__ void mg_bricks_up_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
-  const int L = m.L.brickLevels - 1, t = vb;
-  if (t >= m.dir.capacity) return;
-  __shared__ BrickRegion R;
+// The prolongation into brick level L (SD^3 cells a brick) and its three sweeps.
+template <class St, int SD>
+__device__ __forceinline__ void bricksUp(const MGArgsT<typename St::T> &m, int L, int t, int code, BrickRegion<SD> &R) {
   const auto lv = levelView(m, L, code);
   int tc[3];
   if (!loadBrickRegion<St>(m, lv, t, R, tc)) return;
-  // The first tile level's cells [tc * 4 - 2, tc * 4 + 6) on each axis, resolved once
-  // (prolongValue).
-  {
-    MGCtx c;
-    c.dir = m.dir;
-    c.L = m.L;
-    c.side = m.L.side;
-    c.hasBrick = false;
-    c.hasTile = true;
-    c.tc[0] = tc[0];
-    c.tc[1] = tc[1];
-    c.tc[2] = tc[2];
-    c.tnb = R.tnb;
-    const auto cv = levelView(m, L + 1, code);
-    for (int j = (int)threadIdx.x; j < 512; j += (int)blockDim.x) {
-      const int cc[3] = {tc[0] * 4 - 2 + (j & 7), tc[1] * 4 - 2 + ((j >> 3) & 7), tc[2] * 4 - 2 + (j >> 6)};
-      R.coarse[j] = prolongValue<St>(c, m, cv, cc, L + 1);
-    }
-  }
-  __syncthreads();
-  // prolongateCell over the whole region into R.v[1].
-  for (int i = (int)threadIdx.x; i < BR3; i += (int)blockDim.x) {
-    const int e = R.flag[i];
-    if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
-      R.v[1][i] = 0.f;
-      continue;
-    }
-    const int p[3] = {tc[0] * 8 - 3 + i % BR, tc[1] * 8 - 3 + (i / BR) % BR, tc[2] * 8 - 3 + i / (BR * BR)};
-    float f[3];
-    int b[3];
-    for (int k = 0; k < 3; k++) {
-      const float q = ((float)p[k] + 0.5f) * 0.5f - 0.5f;
-      const float fl = floorf(q);
-      b[k] = (int)fl - (tc[k] * 4 - 2);
-      f[k] = q - fl;
-    }
-    const float *c0 = R.coarse + (b[2] * 8 + b[1]) * 8 + b[0];
-    const float v[8] = {c0[0], c0[1], c0[8], c0[9], c0[64], c0[65], c0[72], c0[73]};
-    const float corr = mixv(mixv(mixv(v[0], v[1], f[0]), mixv(v[2], v[3], f[0]), f[1]),
-                            mixv(mixv(v[4], v[5], f[0]), mixv(v[6], v[7], f[0]), f[1]), f[2]);
-    R.v[1][i] = stored<St>(R.v[0][i] + corr);
-  }
-  __syncthreads();
+  stageBrickCoarse<St>(m, L, code, R, tc);
+  brickRegionProlong<St>(R, tc);
   const float w = m.weight;
   regionSweep<St>(R, 1, 1, w);
   regionSweep<St>(R, 0, 2, w);
   regionSweep<St>(R, 1, 3, w);
   storeBricks<St>(R, 0, lv);
+}
+template <class St>
+__device__ __forceinline__ void mg_bricks_up_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  if (vb >= m.dir.capacity) return;
+  __shared__ BrickRegion<2> R;
+  bricksUp<St>(m, m.L.brickLevels - 1, vb, code, R);
 }
 FP_MG_PAIR(mg_bricks_up, 256)
 
