@@ -159,6 +159,19 @@ __device__ __forceinline__ int entryAt(const Home &h, const Dir &d, int bx, int 
   return entryLookup(d, bx, by, bz);
 }
 
+// Whether every brick of the box [lo, hi] (at most two bricks a side) has an entry, as loops
+// over the box would find: the box's eight corners tested without loops or early exits
+// (duplicates where lo == hi), so lanes do not diverge on trip counts. A box empty on an axis
+// holds no brick: true.
+__device__ __forceinline__ bool bricksPresent(const Home &h, const Dir &d, const int lo[3], const int hi[3]) {
+  if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return true;
+  bool all = true;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    all &= entryAt(h, d, (k & 1) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1], (k & 4) ? hi[2] : lo[2]) >= 0;
+  return all;
+}
+
 // The work item of (virtual) workgroup `w` among `count`, such that each XCD works through one
 // contiguous run of items. MI355X hands workgroups to its 8 XCDs round-robin by id, and
 // each XCD has its own L2; items in brick-id order are spatial neighbors, so a contiguous

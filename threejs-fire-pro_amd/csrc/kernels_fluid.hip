@@ -983,6 +983,7 @@ __device__ __forceinline__ bool pressureUnknown(const Ctx &c, const Params &P, c
     hi[a] = p[a] >> sV;
   }
   if (P.ground) lo[1] = lo[1] > 0 ? lo[1] : 0;
+  // The loops' early exits pay here (bricksPresent made k_project 27 -> 39 us).
   for (int z = lo[2]; z <= hi[2]; z++)
     for (int y = lo[1]; y <= hi[1]; y++)
       for (int x = lo[0]; x <= hi[0]; x++)
