@@ -48,6 +48,23 @@ __device__ __forceinline__ long addrAt(const MGCtx &m, const int p[3], int lv) {
   return (long)owner * sd * sd * sd + (lz * sd + ly) * sd + lx;
 }
 
+// addrAt of a brick-level cell within the home brick's 3^3 neighborhood, through the home
+// slot table (the caller guarantees the neighborhood): no lookup and no branch on where the
+// cell lies.
+__device__ __forceinline__ long addrHome(const MGCtx &m, const int p[3], int lv) {
+  const int sh = levelShift(m.L, lv), sd = 1 << sh, msk = sd - 1;
+  const int rel = (((p[2] >> sh) - m.home.bz + 1) << 4) | (((p[1] >> sh) - m.home.by + 1) << 2) |
+                  ((p[0] >> sh) - m.home.bx + 1);
+  const int owner = m.home.slot[rel];
+  const int local = (((p[2] & msk) << sh | (p[1] & msk)) << sh) | (p[0] & msk);
+  return owner < 0 ? -1 : (long)owner * sd * sd * sd + local;
+}
+// addrAt, or addrHome for HOME (a brick level in the home brick's neighborhood).
+template <bool HOME>
+__device__ __forceinline__ long addrOf(const MGCtx &m, const int p[3], int lv) {
+  return HOME ? addrHome(m, p, lv) : addrAt(m, p, lv);
+}
+
 // activeCell (pressure.wgsl:101-132).
 __device__ __forceinline__ bool activeCell(const MGCtx &m, const Params &P, const int p[3], int lv) {
   const Layout &L = m.L;
@@ -401,9 +418,9 @@ template <class T> __device__ __forceinline__ LevelView<T> levelView(const MGArg
   return LevelView<T>{f ? l.q : l.p, f ? l.p : l.q, l.rhs, l.topo, l.eflag, l.side};
 }
 
-template <class St>
+template <class St, bool HOME = false>
 __device__ __forceinline__ float readAt(const MGCtx &c, const typename St::T *v, const int p[3], int lv) {
-  long a = addrAt(c, p, lv);
+  long a = addrOf<HOME>(c, p, lv);
   return a < 0 ? 0.f : St::ld(v[a]);
 }
 __device__ __forceinline__ float openCount(int flags, int sd) {
@@ -499,15 +516,41 @@ __global__ void __launch_bounds__(256) k_mg_flags_init(MGArgs m, int brickGroups
   }
 }
 
-// relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8, p to q.
+// The six neighbors of brick-level cell p (level cubes sd = 2^sh cells a side) through the
+// home brick's slot table: a neighbor lies in the home brick or one beside it, so every lane
+// reads a slot from LDS and loads (from slot 0 where there is none, then zero), with no
+// branch on where the neighbor lies.
 template <class St>
+__device__ __forceinline__ void brickNeighbors(const MGCtx &c, const typename St::T *pv, const int p[3], int sh,
+                                               float n[6]) {
+  constexpr int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};  // OFF
+  const int sd = 1 << sh, m = sd - 1, cells = sd * sd * sd;
+  const int home[3] = {c.home.bx, c.home.by, c.home.bz};
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    int rel = 0, local = 0;
+#pragma unroll
+    for (int ax = 2; ax >= 0; ax--) {
+      const int q = p[ax] + off[k][ax];
+      rel |= ((q >> sh) - home[ax] + 1) << (2 * ax);
+      local = local * sd + (q & m);
+    }
+    const int slot = c.home.slot[rel];
+    const float v = St::ld(pv[(size_t)(slot < 0 ? 0 : slot) * cells + local]);
+    n[k] = slot < 0 ? 0.f : v;
+  }
+}
+// relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8, p to q. BRICK: a brick
+// level (neighbors through brickNeighbors).
+template <class St, bool BRICK = false>
 __device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelView<typename St::T> &lv, int level, const int p[3],
                                           size_t a, float w) {
   // Every read is issued before the flags are looked at.
   const int e = lv.eflag[a];
   const float own = St::ld(lv.p[a]), rhs = St::ld(lv.rhs[a]);
   float n[6];
-  neighbors<St>(c, lv.p, p, level, (long)a, lv.side, n);
+  if (BRICK) brickNeighbors<St>(c, lv.p, p, levelShift(c.L, level), n);
+  else neighbors<St>(c, lv.p, p, level, (long)a, lv.side, n);
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
     lv.q[a] = St::st(0.f);
     return;
@@ -561,7 +604,7 @@ __device__ __forceinline__ void mg_relax_bricks_body(const MGArgsT<typename St::
   for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-    relaxCell<St>(c, lv, level, p, (size_t)slot * n + l, m.weight);
+    relaxCell<St, true>(c, lv, level, p, (size_t)slot * n + l, m.weight);
   }
 }
 FP_MG_PAIR(mg_relax_bricks, 256)
@@ -587,13 +630,13 @@ FP_MG_PAIR(mg_relax_tiles, 64)
 // tested; the residuals are then added in child order, as one child at a time would.
 // restrictResidual returns the coarse right-hand side (before storage rounding) and whether
 // every child is an unknown; restrictCell stores it and the first iterate.
-template <class St>
+template <class St, bool HOME = false>
 __device__ __forceinline__ float restrictResidual(const MGCtx &c, const MGArgsT<typename St::T> &m, int level,
                                                   int flips, const int id[3], bool &every) {
   const auto fine = levelView(m, level - 1, flips);
   const int sd = fine.side, msk = sd - 1;
   const int p0[3] = {id[0] * 2, id[1] * 2, id[2] * 2};
-  const long a0 = addrAt(c, p0, level - 1);
+  const long a0 = addrOf<HOME>(c, p0, level - 1);
   const int stride[3] = {1, sd, sd * sd};
   // Face f (+x, -x, +y, -y, +z, -z) holds four cells, j = u + 2 v over its two other axes
   // in axis order.
@@ -603,7 +646,11 @@ __device__ __forceinline__ float restrictResidual(const MGCtx &c, const MGArgsT<
     const int ax = f >> 1, step = (f & 1) ? -1 : 2;
     const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
     long qa;
-    if ((unsigned)((p0[ax] & msk) + step) < (unsigned)sd) {
+    if (HOME) {
+      int q[3] = {p0[0], p0[1], p0[2]};
+      q[ax] += step;
+      qa = addrHome(c, q, level - 1);
+    } else if ((unsigned)((p0[ax] & msk) + step) < (unsigned)sd) {
       qa = a0 < 0 ? -1 : a0 + step * stride[ax];
     } else {
       int q[3] = {p0[0], p0[1], p0[2]};
@@ -641,13 +688,13 @@ __device__ __forceinline__ float restrictResidual(const MGCtx &c, const MGArgsT<
   }
   return residual * 0.5f;
 }
-template <class St>
+template <class St, bool HOME = false>
 __device__ __forceinline__ void restrictCell(const MGCtx &c, const MGArgsT<typename St::T> &m, int level, int flips,
                                              const int id[3]) {
   const auto coarse = levelView(m, level, flips);
   bool every;
-  const float rhs = restrictResidual<St>(c, m, level, flips, id, every);
-  long out = addrAt(c, id, level);
+  const float rhs = restrictResidual<St, HOME>(c, m, level, flips, id, every);
+  long out = addrOf<HOME>(c, id, level);
   coarse.rhs[out] = St::st(rhs);
   float first = 0.f;
   if (every) {
@@ -666,7 +713,7 @@ __device__ __forceinline__ void mg_restrict_bricks_body(const MGArgsT<typename S
   for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-    restrictCell<St>(c, m, level, code >> 8, p);
+    restrictCell<St, true>(c, m, level, code >> 8, p);
   }
 }
 FP_MG_PAIR(mg_restrict_bricks, 256)
@@ -685,12 +732,12 @@ FP_MG_PAIR(mg_restrict_tiles, 64)
 
 // coarseValue (pressure.wgsl:670-691): Neumann ghosts beside solids, a mirror below the floor.
 // Each coarse cell is resolved once: its topology byte and value share the address.
-template <class St>
+template <class St, bool HOME = false>
 __device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typename St::T> &m, const LevelView<typename St::T> &cv,
                                              const int p[3], int level) {
   const int groundY = m.P.ground ? 0 : (int)0x80000000;
   int cc[3] = {p[0], p[1] > groundY ? p[1] : groundY, p[2]};
-  const long a0 = addrAt(c, cc, level);
+  const long a0 = addrOf<HOME>(c, cc, level);
   if (!(a0 >= 0 && (cv.topo[a0] & STORED_SOLID))) return a0 < 0 ? 0.f : St::ld(cv.p[a0]);
   float sum = 0.f, count = 0.f;
   for (int axis = 0; axis < 3; axis++)
@@ -698,7 +745,7 @@ __device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typen
       int q[3] = {cc[0], cc[1], cc[2]};
       q[axis] += sd;
       q[1] = q[1] > groundY ? q[1] : groundY;
-      const long a = addrAt(c, q, level);
+      const long a = addrOf<HOME>(c, q, level);
       if (!(a >= 0 && (cv.topo[a] & STORED_SOLID))) {
         sum += a < 0 ? 0.f : St::ld(cv.p[a]);
         count += 1.0f;
@@ -710,12 +757,12 @@ __device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typen
 // The coarse value a prolongation weighs at coarse cell cc: coarseValue, which without
 // colliders is the value at cc with its y raised to the floor (no cell at or above the floor
 // is solid then), so that read alone.
-template <class St>
+template <class St, bool HOME = false>
 __device__ __forceinline__ float prolongValue(const MGCtx &c, const MGArgsT<typename St::T> &m,
                                               const LevelView<typename St::T> &cv, const int cc[3], int level) {
-  if (m.P.colliders != 0) return coarseValue<St>(c, m, cv, cc, level);
+  if (m.P.colliders != 0) return coarseValue<St, HOME>(c, m, cv, cc, level);
   const int q[3] = {cc[0], m.P.ground && cc[1] < 0 ? 0 : cc[1], cc[2]};
-  return readAt<St>(c, cv.p, q, level);
+  return readAt<St, HOME>(c, cv.p, q, level);
 }
 
 // prolongateCell (pressure.wgsl:726-755): trilinear correction from the coarse level, p to
@@ -733,7 +780,9 @@ __device__ __forceinline__ void stageProlong(const MGCtx &c, const MGArgsT<typen
     const int t = r / rs;
     const int cc[3] = {org[0] * (sd / 2) - 1 + (r - t * rs), org[1] * (sd / 2) - 1 + t % rs,
                        org[2] * (sd / 2) - 1 + t / rs};
-    vals[r] = prolongValue<St>(c, m, cv, cc, level + 1);
+    // A brick context's coarse cells of a brick level lie in its home neighborhood.
+    vals[r] = c.hasBrick && level + 1 < m.L.brickLevels ? prolongValue<St, true>(c, m, cv, cc, level + 1)
+                                                         : prolongValue<St>(c, m, cv, cc, level + 1);
   }
 }
 template <class St>
