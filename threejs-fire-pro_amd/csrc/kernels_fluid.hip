@@ -637,7 +637,7 @@ FP_KERNEL_PAIR1_OCC(evolve_fields, 256, 6, StepArgs, int, chunks)
 
 __device__ __forceinline__ float curlCentered(const Ctx &c, const float *V, int x, int y, int z, int comp) {
   int bx = x - (comp == 0), by = y - (comp == 1), bz = z - (comp == 2);
-  float a = velCell(c, V, comp, x, y, z), b = velCell(c, V, comp, bx, by, bz);
+  float a = velCellNear(c, V, comp, x, y, z), b = velCellNear(c, V, comp, bx, by, bz);
   return 0.5f * (a + b);
 }
 
@@ -769,14 +769,14 @@ __device__ __forceinline__ void externalForces(const StepArgs &s, const int cell
 // trilinear sample at the shared corner of each 2^3 block (velocityCellAverage,
 // fluid-sampling.wgsl:270-286; smokeProduction, smoke.wgsl:24-37).
 __device__ __forceinline__ float4 velocityCellAverage(const Ctx &c, const float4 *F, int d, const int id[3]) {
-  if (d == 1) return fieldCell(c, F, id[0], id[1], id[2]);
+  if (d == 1) return fieldCellNear(c, F, id[0], id[1], id[2]);
   int blocks = d / 2;
   float ratio = (float)d;
   float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int z = 0; z < blocks; z++)
     for (int y = 0; y < blocks; y++)
       for (int x = 0; x < blocks; x++) {
-        float4 v = sampleField(c, F, (float)id[0] * ratio + 1.0f + 2.0f * (float)x,
+        float4 v = sampleFieldNear(c, F, (float)id[0] * ratio + 1.0f + 2.0f * (float)x,
                                (float)id[1] * ratio + 1.0f + 2.0f * (float)y,
                                (float)id[2] * ratio + 1.0f + 2.0f * (float)z);
         sum.x += v.x;
@@ -813,7 +813,7 @@ __device__ __forceinline__ void cellForce(const Ctx &c, const StepArgs &s, const
     return;
   }
   float w[3] = {((float)p[0] + 0.5f) * P.hv, ((float)p[1] + 0.5f) * P.hv, ((float)p[2] + 0.5f) * P.hv};
-  if (solidAt(c, P, s.p.solids, w[0], w[1], w[2])) {
+  if (solidAtNear(c, P, s.p.solids, w[0], w[1], w[2])) {
     out[0] = out[1] = out[2] = 0.f;
     return;
   }
@@ -821,10 +821,10 @@ __device__ __forceinline__ void cellForce(const Ctx &c, const StepArgs &s, const
   float4 f = velocityCellAverage(c, s.p.F, d, p);
   float conf[3] = {0.f, 0.f, 0.f};
   if (P.vorticity != 0.f) {
-    float4 cc = curlCell(c, s.p.Curl, p[0], p[1], p[2]);
-    float eta[3] = {curlCell(c, s.p.Curl, p[0] + 1, p[1], p[2]).w - curlCell(c, s.p.Curl, p[0] - 1, p[1], p[2]).w,
-                    curlCell(c, s.p.Curl, p[0], p[1] + 1, p[2]).w - curlCell(c, s.p.Curl, p[0], p[1] - 1, p[2]).w,
-                    curlCell(c, s.p.Curl, p[0], p[1], p[2] + 1).w - curlCell(c, s.p.Curl, p[0], p[1], p[2] - 1).w};
+    float4 cc = curlCellNear(c, s.p.Curl, p[0], p[1], p[2]);
+    float eta[3] = {curlCellNear(c, s.p.Curl, p[0] + 1, p[1], p[2]).w - curlCellNear(c, s.p.Curl, p[0] - 1, p[1], p[2]).w,
+                    curlCellNear(c, s.p.Curl, p[0], p[1] + 1, p[2]).w - curlCellNear(c, s.p.Curl, p[0], p[1] - 1, p[2]).w,
+                    curlCellNear(c, s.p.Curl, p[0], p[1], p[2] + 1).w - curlCellNear(c, s.p.Curl, p[0], p[1], p[2] - 1).w};
     float len = length3(eta[0], eta[1], eta[2]) + 0.0001f;
     float n[3] = {eta[0] / len, eta[1] / len, eta[2] / len};
     float cr[3] = {n[1] * cc.z - n[2] * cc.y, n[2] * cc.x - n[0] * cc.z, n[0] * cc.y - n[1] * cc.x};
@@ -900,9 +900,9 @@ __device__ __forceinline__ void forces_body(const StepArgs &s, const int vb) {
   for (int l = threadIdx.x; l < nV; l += blockDim.x) {
     int p[3];
     velCellCoords(c, l, p);
-    float force[3] = {0.5f * (fh[l] + velCell(c, cellF, 0, p[0] + 1, p[1], p[2])),
-                      0.5f * (fh[nV + l] + velCell(c, cellF, 1, p[0], p[1] + 1, p[2])),
-                      0.5f * (fh[2 * nV + l] + velCell(c, cellF, 2, p[0], p[1], p[2] + 1))};
+    float force[3] = {0.5f * (fh[l] + velCellNear(c, cellF, 0, p[0] + 1, p[1], p[2])),
+                      0.5f * (fh[nV + l] + velCellNear(c, cellF, 1, p[0], p[1] + 1, p[2])),
+                      0.5f * (fh[2 * nV + l] + velCellNear(c, cellF, 2, p[0], p[1], p[2] + 1))};
     float pos[3] = {((float)p[0] + 0.5f) * P.hv, ((float)p[1] + 0.5f) * P.hv, ((float)p[2] + 0.5f) * P.hv};
     float v[3];
     for (int a = 0; a < 3; a++) v[a] = vc[a * nV + l] * damp + P.dt * force[a];
@@ -911,13 +911,13 @@ __device__ __forceinline__ void forces_body(const StepArgs &s, const int vb) {
           fz[3] = {pos[0], pos[1], pos[2] + half};
     float out0 = sourceVelocity(s, v[0], fx, 0), out1 = sourceVelocity(s, v[1], fy, 1),
           out2 = sourceVelocity(s, v[2], fz, 2);
-    if (solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+    if (solidAtNear(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
       o[l] = o[nV + l] = o[2 * nV + l] = 0.f;
       continue;
     }
-    if (solidAt(c, P, s.p.solids, pos[0] + P.hv, pos[1], pos[2])) out0 = 0.f;
-    if (solidAt(c, P, s.p.solids, pos[0], pos[1] + P.hv, pos[2])) out1 = 0.f;
-    if (solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2] + P.hv)) out2 = 0.f;
+    if (solidAtNear(c, P, s.p.solids, pos[0] + P.hv, pos[1], pos[2])) out0 = 0.f;
+    if (solidAtNear(c, P, s.p.solids, pos[0], pos[1] + P.hv, pos[2])) out1 = 0.f;
+    if (solidAtNear(c, P, s.p.solids, pos[0], pos[1], pos[2] + P.hv)) out2 = 0.f;
     o[l] = out0;
     o[nV + l] = out1;
     o[2 * nV + l] = out2;
@@ -938,12 +938,12 @@ __device__ __forceinline__ void divergence_body(const StepArgs &s, const int vb)
     int p[3];
     velCellCoords(c, l, p);
     float h = P.hv;
-    float div = (vf[l] - velCell(c, s.p.Vf, 0, p[0] - 1, p[1], p[2])) * P.invHv +
-                (vf[nV + l] - velCell(c, s.p.Vf, 1, p[0], p[1] - 1, p[2])) * P.invHv +
-                (vf[2 * nV + l] - velCell(c, s.p.Vf, 2, p[0], p[1], p[2] - 1)) * P.invHv;
+    float div = (vf[l] - velCellNear(c, s.p.Vf, 0, p[0] - 1, p[1], p[2])) * P.invHv +
+                (vf[nV + l] - velCellNear(c, s.p.Vf, 1, p[0], p[1] - 1, p[2])) * P.invHv +
+                (vf[2 * nV + l] - velCellNear(c, s.p.Vf, 2, p[0], p[1], p[2] - 1)) * P.invHv;
     float avg;
     if (d == 1) {
-      avg = fieldScalar(c, s.p.E, p[0], p[1], p[2]);
+      avg = fieldScalarNear(c, s.p.E, p[0], p[1], p[2]);
     } else {
       int blocks = d / 2;
       float ratio = (float)d;
@@ -962,7 +962,7 @@ __device__ __forceinline__ void divergence_body(const StepArgs &s, const int vb)
               for (int yy = 0; yy < 2; yy++)
                 for (int xx = 0; xx < 2; xx++) {
                   float w = (xx ? fx : 1.0f - fx) * (yy ? fy : 1.0f - fy) * (zz ? fz : 1.0f - fz);
-                  v += fieldScalar(c, s.p.E, bx + xx, by + yy, bz + zz) * w;
+                  v += fieldScalarNear(c, s.p.E, bx + xx, by + yy, bz + zz) * w;
                 }
             sum += v;
           }
@@ -1002,7 +1002,7 @@ __device__ __forceinline__ void project_body(const StepArgs &s, const int vb) {
     int p[3];
     velCellCoords(c, l, p);
     float pos[3] = {((float)p[0] + 0.5f) * P.hv, ((float)p[1] + 0.5f) * P.hv, ((float)p[2] + 0.5f) * P.hv};
-    if (solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+    if (solidAtNear(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
       o[l] = o[nV + l] = o[2 * nV + l] = 0.f;
       continue;
     }
@@ -1015,14 +1015,14 @@ __device__ __forceinline__ void project_body(const StepArgs &s, const int vb) {
       float npos[3] = {((float)nb[0] + 0.5f) * P.hv, ((float)nb[1] + 0.5f) * P.hv, ((float)nb[2] + 0.5f) * P.hv};
       float value = vf[a * nV + l];
       float res;
-      if (solidAt(c, P, s.p.solids, npos[0], npos[1], npos[2])) res = 0.f;
+      if (solidAtNear(c, P, s.p.solids, npos[0], npos[1], npos[2])) res = 0.f;
       else if (!here && !pressureUnknown(c, P, nb)) res = 0.f;
-      else if (!s.pressureLo) res = value - (velScalar(c, pressure, nb[0], nb[1], nb[2]) - cp) * P.invHv;
+      else if (!s.pressureLo) res = value - (velScalarNear(c, pressure, nb[0], nb[1], nb[2]) - cp) * P.invHv;
       else {
         // A pair hi + lo: the high parts' difference (exact for neighbors within a factor
         // of two) plus the low parts'.
-        const float g = (velScalar(c, pressure, nb[0], nb[1], nb[2]) - cp) +
-                        (velScalar(c, s.pressureLo, nb[0], nb[1], nb[2]) - cpLo);
+        const float g = (velScalarNear(c, pressure, nb[0], nb[1], nb[2]) - cp) +
+                        (velScalarNear(c, s.pressureLo, nb[0], nb[1], nb[2]) - cpLo);
         res = value - g * P.invHv;
       }
       o[a * nV + l] = res;

@@ -75,6 +75,40 @@ __device__ __forceinline__ float4 curlCell(const Ctx &c, const float4 *buf, int 
   return buf[slotBase(s, c.L.sV) + cellIndex(c.L.sV, x, y, z)];
 }
 
+// ---- accesses within the home brick's neighborhood ---------------------------------------
+// The force, divergence and projection kernels read cells at most two cells from their
+// brick, always inside its 3^3 neighborhood: their slots come from the LDS table alone, and
+// every lane loads (from slot 0 where the brick has none, then zero), so lanes take the same
+// instructions whatever brick a cell lies in.
+__device__ __forceinline__ int nearSlot(const Ctx &c, int bx, int by, int bz) {
+  return c.home.slot[((bz - c.home.bz + 1) << 4) | ((by - c.home.by + 1) << 2) | (bx - c.home.bx + 1)];
+}
+__device__ __forceinline__ float velCellNear(const Ctx &c, const float *buf, int comp, int x, int y, int z) {
+  const int s = nearSlot(c, x >> c.L.sV, y >> c.L.sV, z >> c.L.sV);
+  const float v = buf[slotBase((s < 0 ? 0 : s) * 3 + comp, c.L.sV) + cellIndex(c.L.sV, x, y, z)];
+  return s < 0 ? 0.f : v;
+}
+__device__ __forceinline__ float velScalarNear(const Ctx &c, const float *buf, int x, int y, int z) {
+  const int s = nearSlot(c, x >> c.L.sV, y >> c.L.sV, z >> c.L.sV);
+  const float v = buf[slotBase(s < 0 ? 0 : s, c.L.sV) + cellIndex(c.L.sV, x, y, z)];
+  return s < 0 ? 0.f : v;
+}
+__device__ __forceinline__ float4 curlCellNear(const Ctx &c, const float4 *buf, int x, int y, int z) {
+  const int s = nearSlot(c, x >> c.L.sV, y >> c.L.sV, z >> c.L.sV);
+  const float4 v = buf[slotBase(s < 0 ? 0 : s, c.L.sV) + cellIndex(c.L.sV, x, y, z)];
+  return s < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : v;
+}
+__device__ __forceinline__ float fieldScalarNear(const Ctx &c, const float *buf, int x, int y, int z) {
+  const int s = nearSlot(c, x >> c.L.sF, y >> c.L.sF, z >> c.L.sF);
+  const float v = buf[slotBase(s < 0 ? 0 : s, c.L.sF) + cellIndex(c.L.sF, x, y, z)];
+  return s < 0 ? 0.f : v;
+}
+__device__ __forceinline__ float4 fieldCellNear(const Ctx &c, const float4 *buf, int x, int y, int z) {
+  const int s = nearSlot(c, x >> c.L.sF, y >> c.L.sF, z >> c.L.sF);
+  const float4 v = buf[slotBase(s < 0 ? 0 : s, c.L.sF) + cellIndex(c.L.sF, x, y, z)];
+  return s < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : v;
+}
+
 // The 2x2x2 cells from (x, y, z) on a grid of bricks 2^sh cells a side, corner k = (z * 2 +
 // y) * 2 + x: each corner's pool slot (-1 without one) and its index inside the slot. The
 // bricks are resolved once per axis; when all of them lie in the home brick's 3^3
@@ -176,6 +210,46 @@ __device__ __forceinline__ bool solidAt(const Ctx &c, const Params &P, const uin
   int x = (int)floorf(wx * P.invHf + 0.25f), y = (int)floorf(wy * P.invHf + 0.25f),
       z = (int)floorf(wz * P.invHf + 0.25f);
   return solidCell(c, P, solids, x, y, z);
+}
+
+// solidAt for a cell within the home brick's neighborhood (nearSlot).
+__device__ __forceinline__ bool solidAtNear(const Ctx &c, const Params &P, const uint32_t *solids, float wx, float wy,
+                                            float wz) {
+  const int x = (int)floorf(wx * P.invHf + 0.25f), y = (int)floorf(wy * P.invHf + 0.25f),
+            z = (int)floorf(wz * P.invHf + 0.25f);
+  if (P.ground && y < 0) return true;
+  if (P.colliders == 0 || !inSolidBox(P, x, y, z)) return false;
+  const int s = nearSlot(c, x >> c.L.sF, y >> c.L.sF, z >> c.L.sF);
+  if (s < 0) return false;
+  const int m = c.L.B - 1, sh = c.L.sF;
+  const uint32_t bit = (uint32_t)(((((z & m) << sh) | (y & m)) << sh) | (x & m));
+  return ((solids[(size_t)s * (c.L.nF >> 5) + (bit >> 5)] >> (bit & 31u)) & 1u) != 0u;
+}
+// sampleField over cells within the home brick's neighborhood (nearSlot).
+__device__ __forceinline__ float4 sampleFieldNear(const Ctx &c, const float4 *buf, float px, float py, float pz) {
+  float qx = px - 0.5f, qy = py - 0.5f, qz = pz - 0.5f;
+  float flx = floorf(qx), fly = floorf(qy), flz = floorf(qz);
+  int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  float fx = qx - flx, fy = qy - fly, fz = qz - flz;
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 cv[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) cv[k] = fieldCellNear(c, buf, bx + (k & 1), by + ((k >> 1) & 1), bz + (k >> 2));
+#pragma unroll
+  for (int z = 0; z < 2; z++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        float wx = x ? fx : 1.0f - fx, wy = y ? fy : 1.0f - fy, wz = z ? fz : 1.0f - fz;
+        float w = wx * wy * wz;
+        float4 v = cv[(z * 2 + y) * 2 + x];
+        out.x += v.x * w;
+        out.y += v.y * w;
+        out.z += v.z * w;
+        out.w += v.w * w;
+      }
+  return out;
 }
 
 // Trilinear sample of the field grid, position in cell units with centers at .5
