@@ -67,7 +67,7 @@ __device__ __forceinline__ void advect_velocity_body(const StepArgs &s, const in
     int cc[3];
     velCellCoords(c, l, cc);
     float pos[3] = {((float)cc[0] + 0.5f) * s.P.hv, ((float)cc[1] + 0.5f) * s.P.hv, ((float)cc[2] + 0.5f) * s.P.hv};
-    if (solidAt(c, s.P, s.p.solids, pos[0], pos[1], pos[2])) {
+    if (solidAtNear(c, s.P, s.p.solids, pos[0], pos[1], pos[2])) {
       out[l] = out[nV + l] = out[2 * nV + l] = 0.f;
       continue;
     }
@@ -123,7 +123,7 @@ __device__ __forceinline__ void correct_velocity_body(const StepArgs &s, const i
     int cc[3];
     velCellCoords(c, l, cc);
     float pos[3] = {((float)cc[0] + 0.5f) * s.P.hv, ((float)cc[1] + 0.5f) * s.P.hv, ((float)cc[2] + 0.5f) * s.P.hv};
-    if (solidAt(c, s.P, s.p.solids, pos[0], pos[1], pos[2])) {
+    if (solidAtNear(c, s.P, s.p.solids, pos[0], pos[1], pos[2])) {
       out[l] = out[nV + l] = out[2 * nV + l] = 0.f;
       continue;
     }
@@ -554,7 +554,7 @@ __device__ __forceinline__ void evolve_fields_body(const StepArgs &s, const int 
     float expansion = 0.f;
     bool injected = false;
     if (!P.maccormack) velocityAt(c, P, s.p.V, pos[0], pos[1], pos[2], cv);
-    if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+    if (!solidAtNear(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
       float4 tr = correctedScalars(c, P, s.p, cc, slot, l, pos, fw, dn);
       if (P.coarse) tr.x = 0.f;
       float lifetime = tr.z;
@@ -1062,7 +1062,7 @@ __device__ __forceinline__ void advect_smoke_body(const StepArgs &s, const int v
     const float pos[3] = {((float)cc[0] + 0.5f) * P.hs, ((float)cc[1] + 0.5f) * P.hs, ((float)cc[2] + 0.5f) * P.hs};
     float density = 0.f, lo = 0.f, hi = 0.f;
     float cv[3] = {0.f, 0.f, 0.f};
-    if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+    if (!solidAtNear(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
       float dep[3];
       velocityAt(c, P, s.p.V, pos[0], pos[1], pos[2], cv);
       traceBack(c, P, s.p.V, pos, cv, dep);
@@ -1094,7 +1094,7 @@ __device__ __forceinline__ void evolve_smoke_body(const StepArgs &s, const int v
     const float pos[3] = {((float)cc[0] + 0.5f) * P.hs, ((float)cc[1] + 0.5f) * P.hs, ((float)cc[2] + 0.5f) * P.hs};
     const size_t at = slotBase(slot, s.L.sS) + l;
     float density = 0.f;
-    if (!solidAt(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
+    if (!solidAtNear(c, P, s.p.solids, pos[0], pos[1], pos[2])) {
       const float4 d0 = don[l];
       const float cv[3] = {d0.x, d0.y, d0.z};
       const float predictor = s.p.Sp[at];

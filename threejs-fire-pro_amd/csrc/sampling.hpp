@@ -76,10 +76,11 @@ __device__ __forceinline__ float4 curlCell(const Ctx &c, const float4 *buf, int 
 }
 
 // ---- accesses within the home brick's neighborhood ---------------------------------------
-// The force, divergence and projection kernels read cells at most two cells from their
-// brick, always inside its 3^3 neighborhood: their slots come from the LDS table alone, and
-// every lane loads (from slot 0 where the brick has none, then zero), so lanes take the same
-// instructions whatever brick a cell lies in.
+// Cells at most two cells from the work item's brick (the force, divergence and projection
+// kernels' reads, every kernel's solid test of its own cell) lie inside the brick's 3^3
+// neighborhood: their slots come from the LDS table alone, and every lane loads (from slot 0
+// where the brick has none, then zero), so lanes take the same instructions whatever brick
+// a cell lies in.
 __device__ __forceinline__ int nearSlot(const Ctx &c, int bx, int by, int bz) {
   return c.home.slot[((bz - c.home.bz + 1) << 4) | ((by - c.home.by + 1) << 2) | (bx - c.home.bx + 1)];
 }
