@@ -892,10 +892,10 @@ FP_MG_PAIR(mg_prolong_tiles, 64)
 // result lands in the bottom level's current iterate. Sweeps alternate between two LDS
 // copies (read one, write the other), so one barrier separates them.
 constexpr int BOTTOM_LANES = 1024;
-template <class St> __device__ __forceinline__ void mg_bottom_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
-  if (vb != 0 || m.tilesFused) return;
-  constexpr int PER = BOTTOM64_TILES / BOTTOM_LANES;
-  __shared__ float values[2][BOTTOM64_TILES];
+// PER: the tile slots a lane holds in registers (the capacity over 1024, rounded up to a
+// power of two by the caller), so that a lane runs only the rounds the capacity needs.
+template <class St, int PER>
+__device__ __forceinline__ void bottomSweeps(const MGArgsT<typename St::T> &m, int code, float (*values)[BOTTOM64_TILES]) {
   float *value = values[0];
   const auto lv = levelView(m, m.bottom, code >> 8);
   const int cap = m.dir.capacity;
@@ -947,17 +947,23 @@ template <class St> __device__ __forceinline__ void mg_bottom_body(const MGArgsT
   for (int k = 0; k < PER; k++)
     if (valid[k]) lv.p[threadIdx.x + k * BOTTOM_LANES] = St::st(mine[k]);
 }
+template <class St> __device__ __forceinline__ void mg_bottom_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  if (vb != 0 || m.tilesFused) return;
+  __shared__ float values[2][BOTTOM64_TILES];
+  const int per = (m.dir.capacity + BOTTOM_LANES - 1) / BOTTOM_LANES;
+  if (per <= 1) bottomSweeps<St, 1>(m, code, values);
+  else if (per <= 2) bottomSweeps<St, 2>(m, code, values);
+  else if (per <= 4) bottomSweeps<St, 4>(m, code, values);
+  else bottomSweeps<St, BOTTOM64_TILES / BOTTOM_LANES>(m, code, values);
+}
 FP_MG_PAIR(mg_bottom, 1024)
 
 // The coarsest level's sweeps in FP64: the same weighted Jacobi as relaxCoarse, up to
 // BOTTOM64_TILES / 1024 tile slots per lane, values in LDS. The result is rounded to FP32,
 // then stored.
 __device__ __forceinline__ double mix64(double a, double b, double t) { return a * (1.0 - t) + b * t; }
-template <class St>
-__device__ __forceinline__ void mg_bottom64_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
-  if (vb != 0 || m.tilesFused) return;
-  constexpr int PER = BOTTOM64_TILES / BOTTOM_LANES;
-  __shared__ double value[BOTTOM64_TILES];
+template <class St, int PER>
+__device__ __forceinline__ void bottom64Sweeps(const MGArgsT<typename St::T> &m, int code, double *value) {
   const auto lv = levelView(m, m.bottom, code >> 8);
   const int cap = m.dir.capacity;
   const double w = (double)m.weight;
@@ -1009,6 +1015,16 @@ __device__ __forceinline__ void mg_bottom64_body(const MGArgsT<typename St::T> &
     const int t = threadIdx.x + k * BOTTOM_LANES;
     if (k < per && t < cap && m.dir.tileRec[(size_t)t * TILE_RECORD_WORDS + 3] != 0) lv.p[t] = St::st((float)mine[k]);
   }
+}
+template <class St>
+__device__ __forceinline__ void mg_bottom64_body(const MGArgsT<typename St::T> &m, const int vb, int code) {
+  if (vb != 0 || m.tilesFused) return;
+  __shared__ double value[BOTTOM64_TILES];
+  const int per = (m.dir.capacity + BOTTOM_LANES - 1) / BOTTOM_LANES;
+  if (per <= 1) bottom64Sweeps<St, 1>(m, code, value);
+  else if (per <= 2) bottom64Sweeps<St, 2>(m, code, value);
+  else if (per <= 4) bottom64Sweeps<St, 4>(m, code, value);
+  else bottom64Sweeps<St, BOTTOM64_TILES / BOTTOM_LANES>(m, code, value);
 }
 FP_MG_PAIR(mg_bottom64, 1024)
 
