@@ -59,10 +59,20 @@ __device__ __forceinline__ long addrHome(const MGCtx &m, const int p[3], int lv)
   const int local = (((p[2] & msk) << sh | (p[1] & msk)) << sh) | (p[0] & msk);
   return owner < 0 ? -1 : (long)owner * sd * sd * sd + local;
 }
-// addrAt, or addrHome for HOME (a brick level in the home brick's neighborhood).
-template <bool HOME>
+// addrAt of a tile-level cell within the context tile's 3^3 neighborhood, through its 27
+// neighbor slots (the caller guarantees the neighborhood).
+__device__ __forceinline__ long addrTileNear(const MGCtx &m, const int p[3], int lv) {
+  const int sh = levelShift(m.L, lv), sd = 1 << sh, msk = sd - 1;
+  const int owner = m.tnb[((p[2] >> sh) - m.tc[2] + 1) * 9 + ((p[1] >> sh) - m.tc[1] + 1) * 3 + ((p[0] >> sh) - m.tc[0] + 1)];
+  const int local = (((p[2] & msk) << sh | (p[1] & msk)) << sh) | (p[0] & msk);
+  return owner < 0 ? -1 : (long)owner * sd * sd * sd + local;
+}
+// addrAt, or for NEAR a cell within one cell of the context's cube: through the home brick's
+// slots at a brick level, the context tile's neighbors at a tile level.
+template <bool NEAR>
 __device__ __forceinline__ long addrOf(const MGCtx &m, const int p[3], int lv) {
-  return HOME ? addrHome(m, p, lv) : addrAt(m, p, lv);
+  if (!NEAR) return addrAt(m, p, lv);
+  return lv < m.L.brickLevels ? addrHome(m, p, lv) : addrTileNear(m, p, lv);
 }
 
 // activeCell (pressure.wgsl:101-132).
@@ -542,15 +552,28 @@ __device__ __forceinline__ void brickNeighbors(const MGCtx &c, const typename St
 }
 // relaxCell (pressure.wgsl:468-479): weighted Jacobi, omega = 0.8, p to q. BRICK: a brick
 // level (neighbors through brickNeighbors).
-template <class St, bool BRICK = false>
+template <class St, bool BRICK = false, bool TILE = false>
 __device__ __forceinline__ void relaxCell(const MGCtx &c, const LevelView<typename St::T> &lv, int level, const int p[3],
                                           size_t a, float w) {
   // Every read is issued before the flags are looked at.
   const int e = lv.eflag[a];
   const float own = St::ld(lv.p[a]), rhs = St::ld(lv.rhs[a]);
   float n[6];
-  if (BRICK) brickNeighbors<St>(c, lv.p, p, levelShift(c.L, level), n);
-  else neighbors<St>(c, lv.p, p, level, (long)a, lv.side, n);
+  if (BRICK) {
+    brickNeighbors<St>(c, lv.p, p, levelShift(c.L, level), n);
+  } else if (TILE) {
+    // A tile level's neighbors lie in the context tile or one beside it.
+    constexpr int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};  // OFF
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const int q[3] = {p[0] + off[k][0], p[1] + off[k][1], p[2] + off[k][2]};
+      const long qa = addrTileNear(c, q, level);
+      const float v = St::ld(lv.p[qa < 0 ? 0 : qa]);
+      n[k] = qa < 0 ? 0.f : v;
+    }
+  } else {
+    neighbors<St>(c, lv.p, p, level, (long)a, lv.side, n);
+  }
   if ((e & STORED_SOLID) || !(e & UNKNOWN)) {
     lv.q[a] = St::st(0.f);
     return;
@@ -617,7 +640,7 @@ __device__ __forceinline__ void mg_relax_tiles_body(const MGArgsT<typename St::T
   for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    relaxCell<St>(c, lv, level, p, (size_t)t * n + l, m.weight);
+    relaxCell<St, false, true>(c, lv, level, p, (size_t)t * n + l, m.weight);
   }
 }
 FP_MG_PAIR(mg_relax_tiles, 64)
@@ -649,7 +672,7 @@ __device__ __forceinline__ float restrictResidual(const MGCtx &c, const MGArgsT<
     if (HOME) {
       int q[3] = {p0[0], p0[1], p0[2]};
       q[ax] += step;
-      qa = addrHome(c, q, level - 1);
+      qa = addrOf<true>(c, q, level - 1);
     } else if ((unsigned)((p0[ax] & msk) + step) < (unsigned)sd) {
       qa = a0 < 0 ? -1 : a0 + step * stride[ax];
     } else {
@@ -725,27 +748,31 @@ __device__ __forceinline__ void mg_restrict_tiles_body(const MGArgsT<typename St
   for (int l = gr.first; l < n; l += gr.step) {
     int p[3];
     levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-    restrictCell<St>(c, m, level, code >> 8, p);
+    if (level - 1 >= m.L.brickLevels) restrictCell<St, true>(c, m, level, code >> 8, p);
+    else restrictCell<St>(c, m, level, code >> 8, p);
   }
 }
 FP_MG_PAIR(mg_restrict_tiles, 64)
 
 // coarseValue (pressure.wgsl:670-691): Neumann ghosts beside solids, a mirror below the floor.
 // Each coarse cell is resolved once: its topology byte and value share the address.
-template <class St, bool HOME = false>
+// NEAR: p and its six neighbors lie in the context's neighborhood (a brick context at a
+// brick level); reads the floor moved use the general lookup.
+template <class St, bool NEAR = false>
 __device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typename St::T> &m, const LevelView<typename St::T> &cv,
                                              const int p[3], int level) {
   const int groundY = m.P.ground ? 0 : (int)0x80000000;
   int cc[3] = {p[0], p[1] > groundY ? p[1] : groundY, p[2]};
-  const long a0 = addrOf<HOME>(c, cc, level);
+  const long a0 = NEAR && cc[1] == p[1] ? addrOf<true>(c, cc, level) : addrAt(c, cc, level);
   if (!(a0 >= 0 && (cv.topo[a0] & STORED_SOLID))) return a0 < 0 ? 0.f : St::ld(cv.p[a0]);
   float sum = 0.f, count = 0.f;
   for (int axis = 0; axis < 3; axis++)
     for (int sd = -1; sd <= 1; sd += 2) {
       int q[3] = {cc[0], cc[1], cc[2]};
       q[axis] += sd;
+      const int y = q[1];
       q[1] = q[1] > groundY ? q[1] : groundY;
-      const long a = addrOf<HOME>(c, q, level);
+      const long a = NEAR && cc[1] == p[1] && q[1] == y ? addrOf<true>(c, q, level) : addrAt(c, q, level);
       if (!(a >= 0 && (cv.topo[a] & STORED_SOLID))) {
         sum += a < 0 ? 0.f : St::ld(cv.p[a]);
         count += 1.0f;
@@ -757,12 +784,17 @@ __device__ __forceinline__ float coarseValue(const MGCtx &c, const MGArgsT<typen
 // The coarse value a prolongation weighs at coarse cell cc: coarseValue, which without
 // colliders is the value at cc with its y raised to the floor (no cell at or above the floor
 // is solid then), so that read alone.
-template <class St, bool HOME = false>
+// NEAR: cc lies within one cell of the context's cube, read through its neighborhood (not
+// for a cell the floor moves, nor for the solid rule's neighbors, which may lie farther).
+// NEAR_SOLID: the solid rule's neighbors lie in the neighborhood too (coarseValue NEAR).
+template <class St, bool NEAR = false, bool NEAR_SOLID = false>
 __device__ __forceinline__ float prolongValue(const MGCtx &c, const MGArgsT<typename St::T> &m,
                                               const LevelView<typename St::T> &cv, const int cc[3], int level) {
-  if (m.P.colliders != 0) return coarseValue<St, HOME>(c, m, cv, cc, level);
-  const int q[3] = {cc[0], m.P.ground && cc[1] < 0 ? 0 : cc[1], cc[2]};
-  return readAt<St, HOME>(c, cv.p, q, level);
+  if (m.P.colliders != 0) return coarseValue<St, NEAR_SOLID>(c, m, cv, cc, level);
+  const bool floor = m.P.ground && cc[1] < 0;
+  if (NEAR && !floor) return readAt<St, true>(c, cv.p, cc, level);
+  const int q[3] = {cc[0], floor ? 0 : cc[1], cc[2]};
+  return readAt<St>(c, cv.p, q, level);
 }
 
 // prolongateCell (pressure.wgsl:726-755): trilinear correction from the coarse level, p to
@@ -780,9 +812,16 @@ __device__ __forceinline__ void stageProlong(const MGCtx &c, const MGArgsT<typen
     const int t = r / rs;
     const int cc[3] = {org[0] * (sd / 2) - 1 + (r - t * rs), org[1] * (sd / 2) - 1 + t % rs,
                        org[2] * (sd / 2) - 1 + t / rs};
-    // A brick context's coarse cells of a brick level lie in its home neighborhood.
-    vals[r] = c.hasBrick && level + 1 < m.L.brickLevels ? prolongValue<St, true>(c, m, cv, cc, level + 1)
-                                                         : prolongValue<St>(c, m, cv, cc, level + 1);
+    // The coarse cells lie within one cell of the item's coarse cube: in the home brick's
+    // neighborhood at a brick level, the context tile's at a tile level.
+    // At a brick level with at least two cells a brick, the solid rule's neighbors (one
+    // cell farther) stay in the home neighborhood as well.
+    if (level + 1 < m.L.brickLevels && c.hasBrick)
+      vals[r] = prolongValue<St, true, true>(c, m, cv, cc, level + 1);
+    else if (level + 1 >= m.L.brickLevels && c.hasTile)
+      vals[r] = prolongValue<St, true>(c, m, cv, cc, level + 1);
+    else
+      vals[r] = prolongValue<St>(c, m, cv, cc, level + 1);
   }
 }
 template <class St>
@@ -1254,7 +1293,7 @@ __device__ __forceinline__ void mg_tiles_up_body(const MGArgsT<typename St::T> &
     const auto cv = levelView(m, Lt + 1, code);
     for (int j = (int)threadIdx.x; j < 216; j += (int)blockDim.x) {
       const int cc[3] = {tc[0] * 2 - 2 + j % 6, tc[1] * 2 - 2 + (j / 6) % 6, tc[2] * 2 - 2 + j / 36};
-      R.coarse[j] = prolongValue<St>(c, m, cv, cc, Lt + 1);
+      R.coarse[j] = prolongValue<St, true>(c, m, cv, cc, Lt + 1);
     }
   }
   __syncthreads();
@@ -1421,7 +1460,7 @@ __device__ __forceinline__ void stageBrickCoarse(const MGArgsT<typename St::T> &
   const auto cv = levelView(m, L + 1, code);
   for (int j = (int)threadIdx.x; j < NC * NC * NC; j += (int)blockDim.x) {
     const int cc[3] = {tc[0] * 2 * SD - 2 + j % NC, tc[1] * 2 * SD - 2 + (j / NC) % NC, tc[2] * 2 * SD - 2 + j / (NC * NC)};
-    R.coarse[j] = prolongValue<St>(c, m, cv, cc, L + 1);
+    R.coarse[j] = prolongValue<St, true>(c, m, cv, cc, L + 1);
   }
   __syncthreads();
 }
