@@ -163,12 +163,18 @@ __device__ __forceinline__ int entryAt(const Home &h, const Dir &d, int bx, int 
 // over the box would find: the box's eight corners tested without loops or early exits
 // (duplicates where lo == hi), so lanes do not diverge on trip counts. A box empty on an axis
 // holds no brick: true.
+// NEAR: the box lies in the home brick's 3^3 neighborhood, read from the LDS record alone.
+template <bool NEAR = false>
 __device__ __forceinline__ bool bricksPresent(const Home &h, const Dir &d, const int lo[3], const int hi[3]) {
   if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return true;
   bool all = true;
 #pragma unroll
-  for (int k = 0; k < 8; k++)
-    all &= entryAt(h, d, (k & 1) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1], (k & 4) ? hi[2] : lo[2]) >= 0;
+  for (int k = 0; k < 8; k++) {
+    const int b[3] = {(k & 1) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1], (k & 4) ? hi[2] : lo[2]};
+    const int e = NEAR ? h.rec[(b[2] - h.bz + 1) * 9 + (b[1] - h.by + 1) * 3 + (b[0] - h.bx + 1)]
+                       : entryAt(h, d, b[0], b[1], b[2]);
+    all &= e >= 0;
+  }
   return all;
 }
 

@@ -98,8 +98,10 @@ __device__ __forceinline__ bool activeCell(const MGCtx &m, const Params &P, cons
     hi[a] = (p[a] * span + span - 1) >> L.sV;
   }
   if (P.ground) lo[1] = lo[1] > 0 ? lo[1] : 0;
-  // At most two bricks a side (a brick-level cell spans at most half a brick).
-  if (m.hasBrick) return bricksPresent(m.home, m.dir, lo, hi);
+  // At most two bricks a side (a brick-level cell spans at most half a brick), beside the
+  // home brick of a brick context's own cells; the floor only raises lo, and a box it leaves
+  // non-empty still starts at most one brick below home.
+  if (m.hasBrick) return bricksPresent<true>(m.home, m.dir, lo, hi);
   for (int z = lo[2]; z <= hi[2]; z++)
     for (int y = lo[1]; y <= hi[1]; y++)
       for (int x = lo[0]; x <= hi[0]; x++)
