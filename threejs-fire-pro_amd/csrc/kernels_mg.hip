@@ -294,13 +294,20 @@ __device__ __forceinline__ void topology_brick(const MGArgs &m, const int id) {
   if (m.dir.entries[id] < 0) return;
   MG_BRICK_HOME(id)
   int slot = entrySlot(rec[13], c.side);
-  for (int lv = 0; lv < m.L.brickLevels; lv++) {
-    int sd = m.lv[lv].side, n = sd * sd * sd;
-    for (int l = threadIdx.x; l < n; l += blockDim.x) {
-      int p[3];
-      levelCell(c.home.bx, c.home.by, c.home.bz, sd, l, p);
-      m.lv[lv].topo[(size_t)slot * n + l] = cellTopology(c, m.P, m.solids, p, lv);
+  // Every brick level's cells as one index space: a lane's cells of different levels are
+  // independent, so one pass of the workgroup covers them all.
+  int total = 0;
+  for (int lv = 0; lv < m.L.brickLevels; lv++) total += m.lv[lv].side * m.lv[lv].side * m.lv[lv].side;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int lv = 0, l = i, n = m.lv[0].side * m.lv[0].side * m.lv[0].side;
+    while (l >= n) {
+      l -= n;
+      lv++;
+      n = m.lv[lv].side * m.lv[lv].side * m.lv[lv].side;
     }
+    int p[3];
+    levelCell(c.home.bx, c.home.by, c.home.bz, m.lv[lv].side, l, p);
+    m.lv[lv].topo[(size_t)slot * n + l] = cellTopology(c, m.P, m.solids, p, lv);
   }
 }
 // Whether a collider can make any cell that a topology of brick `id` samples solid: the
@@ -360,16 +367,29 @@ __device__ __forceinline__ void topology_tiles_body(const MGArgs &m, const int v
     return;
   }
   MG_TILE_HOME(t)
-  for (int lv = first; lv <= last; lv++) {
-    const int sd = m.lv[lv].side, n = sd * sd * sd;
-    for (int l = threadIdx.x; l < n; l += blockDim.x) {
-      int p[3];
-      levelCell(c.tc[0], c.tc[1], c.tc[2], sd, l, p);
-      m.lv[lv].topo[(size_t)t * n + l] = cellTopology(c, m.P, m.solids, p, lv);
+  // The levels' cells as one index space, one pass of the workgroup (as topology_brick).
+  int total = 0;
+  for (int lv = first; lv <= last; lv++) total += m.lv[lv].side * m.lv[lv].side * m.lv[lv].side;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int lv = first, l = i, n = m.lv[first].side * m.lv[first].side * m.lv[first].side;
+    while (l >= n) {
+      l -= n;
+      lv++;
+      n = m.lv[lv].side * m.lv[lv].side * m.lv[lv].side;
     }
+    int p[3];
+    levelCell(c.tc[0], c.tc[1], c.tc[2], m.lv[lv].side, l, p);
+    m.lv[lv].topo[(size_t)t * n + l] = cellTopology(c, m.P, m.solids, p, lv);
   }
 }
-FP_KERNEL_PAIR1(topology_tiles, 64, MGArgs, int, lv)
+FP_KERNEL_PAIR1(topology_tiles, 128, MGArgs, int, lv)
+
+// buildTileTopology for the listed tile slots only (a workgroup each), when only the tile set
+// changed: the created tiles and the tiles near the colliders.
+__global__ void __launch_bounds__(128) k_topology_tiles_list(MGArgs m, const int *tiles, int count) {
+  if ((int)blockIdx.x >= count) return;
+  topology_tiles_body(m, tiles[blockIdx.x], -1);
+}
 
 // The step's flags of the brick levels: topology plus the unknown bit.
 __device__ __forceinline__ void eflags_bricks_body(const MGArgs &m, const int vb) {

@@ -467,9 +467,10 @@ struct fp_sim {
 
   // Per brick id.
   DBuf<int> tileRec, hash, pages, entries, records, active, zeroList, poolState, counts, createdSlots, readback;
-  // Tile slots near the colliders (nearTopology).
-  DBuf<int> nearTiles;
-  std::vector<int> nearTilesHost;
+  // Tile slots near the colliders (nearTopology) and those whose tile-level topology a
+  // tile-set change rebuilds (tileTopology).
+  DBuf<int> nearTiles, topoTiles;
+  std::vector<int> nearTilesHost, topoTilesHost;
   DBuf<uint32_t> brickState, activity, stateTmp, rankTmp, blockSums, blockPrefix, freeSlots;
   // Pools, per slot.
   DBuf<float4> F, Fp, Fn, Curl;
@@ -1568,6 +1569,34 @@ struct fp_sim {
                                                                                (int)nearTilesHost.size());
   }
 
+  // The tile levels' topology of the tiles created by the last table upload and, with
+  // colliders, of the live tiles within a tile of the colliders' box (a tile-level cell's
+  // neighbor samples lie up to a tile away).
+  void tileTopology(const MGArgs &m) {
+    topoTilesHost.assign(created.begin(), created.end());
+    if (colliderCount > 0) {
+      const long long side = 4LL * B;
+      for (int t = 0; t < capacity && t < (int)dir.coords.size(); t++) {
+        const std::array<int, 4> &c = dir.coords[t];
+        if (!c[3] || std::find(created.begin(), created.end(), t) != created.end()) continue;
+        bool near = true;
+        for (int a = 0; a < 3; a++)
+          if ((long long)c[a] * side + 2 * side - 1 < solidLo[a] || (long long)c[a] * side - side > solidHi[a]) near = false;
+        if (near) topoTilesHost.push_back(t);
+      }
+    }
+    if (topoTilesHost.empty()) return;
+    if (topoTiles.n < topoTilesHost.size()) {
+      drain();
+      topoTiles.alloc(std::max<size_t>(topoTilesHost.size(), 64));
+    }
+    // Read when the stream reaches the copy; each step waits for its readback, queued after
+    // it, before the list is rebuilt.
+    HIPCHECK(hipMemcpyAsync(topoTiles.p, topoTilesHost.data(), topoTilesHost.size() * sizeof(int),
+                            hipMemcpyHostToDevice, stream));
+    k_topology_tiles_list<<<(unsigned)topoTilesHost.size(), 128, 0, stream>>>(m, topoTiles.p, (int)topoTilesHost.size());
+  }
+
   // PressureSolver.solve (PressureSolver.ts:371-402): one cold-start V-cycle, or the
   // defect-correction loop around V-cycles.
   void solvePressure(unsigned slotsGrid) {
@@ -1576,22 +1605,24 @@ struct fp_sim {
     tilesMoved = false;
     m.topologyAll = geometryDirty ? 1 : 0;
     m.topologyTiles = geometryDirty || tiles ? 1 : 0;
-    // The brick levels hold at most nV cells per brick.
+    // The brick levels hold at most nV cells per brick; their topology, every level's cells
+    // at once, takes up to 1.15 nV lanes.
     const unsigned vLanes = (unsigned)std::min(256, std::max(64, nV));
+    const unsigned tLanes = nV >= 256 ? 256u : 128u;
     if (geometryDirty) {
       // Every active brick, the step's fresh ones included, and every tile level at once.
-      k_topology_bricks<<<slotsGrid, vLanes, 0, stream>>>(m, 0);
-      if (bottom >= brickLevels) k_topology_tiles<<<capacity, 64, 0, stream>>>(m, -1);
+      k_topology_bricks<<<slotsGrid, tLanes, 0, stream>>>(m, 0);
+      if (bottom >= brickLevels) k_topology_tiles<<<capacity, 128, 0, stream>>>(m, -1);
       geometryDirty = false;
     } else {
-      k_fresh_topology<<<std::min(slotsGrid * 2, FRESH_GRID), vLanes, 0, stream>>>(m);
+      k_fresh_topology<<<std::min(slotsGrid * 2, FRESH_GRID), tLanes, 0, stream>>>(m);
       if (tiles) {
-        // Only the tile set changed: the tile levels' topology in full (their slots moved), and
-        // of the bricks only those whose cells a collider may reach; elsewhere a brick's
-        // topology depends on the floor alone and is unchanged. A new brick beside a collider
-        // can close its neighbors' sides, so those are built again.
-        if (colliderCount > 0) nearTopology(m, vLanes);
-        if (bottom >= brickLevels) k_topology_tiles<<<capacity, 64, 0, stream>>>(m, -1);
+        // Only the tile set changed. Tiles keep their slots, and away from the colliders a
+        // cell's topology depends on the floor alone: the bricks whose cells a collider may
+        // reach are built again (a new brick beside a collider can close its neighbors'
+        // sides), and of the tile levels the created tiles and those near the colliders.
+        if (colliderCount > 0) nearTopology(m, tLanes);
+        if (bottom >= brickLevels) tileTopology(m);
       }
     }
     if (pressureMode == 0) {
@@ -2629,7 +2660,7 @@ struct fp_batch {
     launch<StepArgs>(k_divergence_batch, vLanes, lay.post, lay.slots, wide());
     // Pressure: topology, flags, then one V-cycle or defect correction.
     launch<MGArgs>(k_topology_bricks_batch, vLanes, lay.mg, lay.slots, wide(), 0);
-    if (shape.bottom >= shape.brickLevels) launch<MGArgs>(k_topology_tiles_batch, 64, lay.mg, lay.tiles, wide(), -1);
+    if (shape.bottom >= shape.brickLevels) launch<MGArgs>(k_topology_tiles_batch, 128, lay.mg, lay.tiles, wide(), -1);
     launch<MGArgs>(k_topology_bricks_batch, vLanes, lay.mg, lay.slots2, wide(), 1);
     launch<MGArgs>(k_eflags_bricks_batch, vLanes, lay.mg, lay.slots, wide());
     if (shape.bottom >= shape.brickLevels) launch<MGArgs>(k_eflags_tiles_batch, 64, lay.mg, lay.tiles, wide());
