@@ -223,6 +223,8 @@ FP_DECLARE_MG(mg_bricks_up)
 // The V-cycle below the first tile level in one workgroup, for simulations of at most TC_MAX
 // tile slots (MGArgsT::tilesFused).
 FP_DECLARE_MG(mg_coarse_cycle)
+// Active bricks a k_link_bricks workgroup of 256 lanes links (32 lanes each).
+constexpr int LINK_BRICKS = 8;
 constexpr int TC_MAX = 512;
 constexpr int TC_CELLS = TC_MAX * 9;  // 2^3 + 1 cells per tile slot
 constexpr int TC_LANES = 1024;

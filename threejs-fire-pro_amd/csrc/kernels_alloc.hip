@@ -71,17 +71,25 @@ __device__ __forceinline__ void build_bricks_body(const AllocArgs &a, const int 
       int b[3] = {tr[0] * 4 + (l & 3), tr[1] * 4 + ((l >> 2) & 3), tr[2] * 4 + (l >> 4)};
       bool open = tr[3] != 2;
       bool listed = false;
-      for (int z = -1; z <= 1 && open && !listed; z++)
-        for (int y = -1; y <= 1 && !listed; y++)
-          for (int x = -1; x <= 1 && !listed; x++) {
-            int nb[3] = {b[0] + x, b[1] + y, b[2] + z};
-            int dt[3] = {(nb[0] >> 2) - tr[0], (nb[1] >> 2) - tr[1], (nb[2] >> 2) - tr[2]};
-            int ntile = tr[4 + (dt[2] + 1) * 9 + (dt[1] + 1) * 3 + (dt[0] + 1)];
-            if (ntile < 0) continue;
-            int nid = ntile * 64 + (((nb[2] & 3) * 4 + (nb[1] & 3)) * 4 + (nb[0] & 3));
-            int toward = (-z + 1) * 9 + (-y + 1) * 3 + (-x + 1);
-            listed = ((a.activity[nid] >> 3) & (1u << toward)) != 0u || (a.meshTables && a.meshTables[nid] >= 0);
-          }
+      if (open) {
+        // Whether any neighbor's content support reaches this brick (or a mesh source covers
+        // one): the 27 reads issued together and OR-ed, the same answer as a scan that stops
+        // at the first.
+        uint32_t any = 0u;
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+          const int x = k % 3 - 1, y = (k / 3) % 3 - 1, z = k / 9 - 1;
+          const int nb[3] = {b[0] + x, b[1] + y, b[2] + z};
+          const int dt[3] = {(nb[0] >> 2) - tr[0], (nb[1] >> 2) - tr[1], (nb[2] >> 2) - tr[2]};
+          const int ntile = tr[4 + (dt[2] + 1) * 9 + (dt[1] + 1) * 3 + (dt[0] + 1)];
+          if (ntile < 0) continue;
+          const int nid = ntile * 64 + (((nb[2] & 3) * 4 + (nb[1] & 3)) * 4 + (nb[0] & 3));
+          const int toward = (-z + 1) * 9 + (-y + 1) * 3 + (-x + 1);
+          any |= (a.activity[nid] >> 3) & (1u << toward);
+          if (a.meshTables && a.meshTables[nid] >= 0) any |= 1u;
+        }
+        listed = any != 0u;
+      }
       if (open && !listed) listed = sourceNear(a, b);
       int page = a.pages[index];
       bool was = (a.brickState[index] & 1u) != 0u;
@@ -202,31 +210,43 @@ FP_KERNEL_PAIR(allocate_bricks, 256, AllocArgs)
 
 
 // linkBricks (:234-268): each active brick's 27 neighbor entries and the pressure flag.
+// 32 lanes a brick, LINK_BRICKS bricks a 256-lane workgroup: lane k < 27 resolves neighbor
+// k's entry, lane 27 the pressure flag from the eight below once they are in LDS; each
+// brick's record is written as one run of 28 words.
 __device__ __forceinline__ void link_bricks_body(const AllocArgs &a, const int vb) {
-  const int i = vb * 64 + threadIdx.x;
-  if (i >= a.counts[0]) return;
-  const int index = a.active[i];
-  const int *tr = a.dir.tileRec + (size_t)(index >> 6) * TILE_RECORD_WORDS;
-  int l = index & 63;
-  int b[3] = {tr[0] * 4 + (l & 3), tr[1] * 4 + ((l >> 2) & 3), tr[2] * 4 + (l >> 4)};
-  int w[28];
-  for (int k = 0; k < 27; k++) {
-    int nb[3] = {b[0] + k % 3 - 1, b[1] + (k / 3) % 3 - 1, b[2] + k / 9 - 1};
-    int dt[3] = {(nb[0] >> 2) - tr[0], (nb[1] >> 2) - tr[1], (nb[2] >> 2) - tr[2]};
-    int ntile = tr[4 + (dt[2] + 1) * 9 + (dt[1] + 1) * 3 + (dt[0] + 1)];
-    w[k] = ntile < 0 ? -1 : a.entries[ntile * 64 + (((nb[2] & 3) * 4 + (nb[1] & 3)) * 4 + (nb[0] & 3))];
+  __shared__ int ws[LINK_BRICKS][28];
+  const int j = (int)threadIdx.x >> 5, k = (int)threadIdx.x & 31;
+  const int i = vb * LINK_BRICKS + j;
+  const bool live = i < a.counts[0];
+  int index = 0, b[3] = {0, 0, 0};
+  if (live && k < 28) {
+    index = a.active[i];
+    const int *tr = a.dir.tileRec + (size_t)(index >> 6) * TILE_RECORD_WORDS;
+    const int l = index & 63;
+    b[0] = tr[0] * 4 + (l & 3);
+    b[1] = tr[1] * 4 + ((l >> 2) & 3);
+    b[2] = tr[2] * 4 + (l >> 4);
+    if (k < 27) {
+      const int nb[3] = {b[0] + k % 3 - 1, b[1] + (k / 3) % 3 - 1, b[2] + k / 9 - 1};
+      const int dt[3] = {(nb[0] >> 2) - tr[0], (nb[1] >> 2) - tr[1], (nb[2] >> 2) - tr[2]};
+      const int ntile = tr[4 + (dt[2] + 1) * 9 + (dt[1] + 1) * 3 + (dt[0] + 1)];
+      const int w = ntile < 0 ? -1 : a.entries[ntile * 64 + (((nb[2] & 3) * 4 + (nb[1] & 3)) * 4 + (nb[0] & 3))];
+      ws[j][k] = w;
+      a.records[(size_t)index * RECORD_WORDS + k] = w;
+    }
   }
-  int unknowns = 1;
-  for (int k = 0; k < 8; k++) {
-    int dx = -(k & 1), dy = -((k >> 1) & 1), dz = -((k >> 2) & 1);
-    if (a.P.ground && b[1] + dy < 0) continue;
-    if (w[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] < 0) unknowns = 0;
+  __syncthreads();
+  if (live && k == PRESSURE_FLAG) {
+    int unknowns = 1;
+    for (int c = 0; c < 8; c++) {
+      const int dx = -(c & 1), dy = -((c >> 1) & 1), dz = -((c >> 2) & 1);
+      if (a.P.ground && b[1] + dy < 0) continue;
+      if (ws[j][(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] < 0) unknowns = 0;
+    }
+    a.records[(size_t)index * RECORD_WORDS + PRESSURE_FLAG] = unknowns;
   }
-  w[PRESSURE_FLAG] = unknowns;
-  int *out = a.records + (size_t)index * RECORD_WORDS;
-  for (int k = 0; k < 28; k++) out[k] = w[k];
 }
-FP_KERNEL_PAIR(link_bricks, 64, AllocArgs)
+FP_KERNEL_PAIR(link_bricks, 256, AllocArgs)
 
 
 // Colliders into the solid bits of one brick (buildSolids, :292-341): a workgroup per

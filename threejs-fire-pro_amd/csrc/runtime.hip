@@ -1460,7 +1460,7 @@ struct fp_sim {
     k_scan_blocks<<<1, 1024, 0, stream>>>(a);
     k_free_bricks<<<blocks, 256, 0, stream>>>(a);
     k_allocate_bricks<<<blocks, 256, 0, stream>>>(a);
-    k_link_bricks<<<(slotsGrid + 63) / 64, 64, 0, stream>>>(a);
+    k_link_bricks<<<(slotsGrid + LINK_BRICKS - 1) / LINK_BRICKS, 256, 0, stream>>>(a);
     if (a.buildSolids) {
       if (a.solidsEvery) k_build_solids<<<slotsGrid, 64, 0, stream>>>(a);
       else k_fresh_solids<<<std::min(slotsGrid, FRESH_GRID), 64, 0, stream>>>(a);
@@ -2646,7 +2646,7 @@ struct fp_batch {
     launch<AllocArgs>(k_scan_blocks_batch, 1024, lay.alloc, lay.one, perSim());
     launch<AllocArgs>(k_free_bricks_batch, 256, lay.alloc, lay.blocks, wide());
     launch<AllocArgs>(k_allocate_bricks_batch, 256, lay.alloc, lay.blocks, wide());
-    launch<AllocArgs>(k_link_bricks_batch, 64, lay.alloc, lay.link, wide());
+    launch<AllocArgs>(k_link_bricks_batch, 256, lay.alloc, lay.link, wide());
     launch<AllocArgs>(k_build_solids_batch, 64, lay.alloc, lay.slots, wide());
     launch<AllocArgs>(k_zero_fresh_batch, 256, lay.alloc, lay.zero, wide(), zchunks);
     // Transport, combustion, forces.
@@ -2724,7 +2724,7 @@ struct fp_batch {
       fp_sim &s = *sims[i];
       const int slotsGrid = (std::max(1, s.poolSlots) + 7) / 8 * 8;
       cBlocks[i] = (s.brickCount() + 255) / 256;
-      cLink[i] = (slotsGrid + 63) / 64;
+      cLink[i] = (slotsGrid + LINK_BRICKS - 1) / LINK_BRICKS;
       cSlots[i] = slotsGrid;
       cSlots2[i] = slotsGrid * 2;
       cZero[i] = slotsGrid * zchunks;
