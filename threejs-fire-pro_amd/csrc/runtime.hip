@@ -2579,6 +2579,16 @@ struct fp_batch {
         f ^= 1 << L;
         return;
       }
+      if (L >= 1 && L == c.brickLevels - 1) {
+        // The last brick level (2^3 cells a brick) by tile of bricks and halo (fp_sim::cycle),
+        // one workgroup per tile slot of each member.
+        go(k_mg_bricks_down_batch<St>, 256, levelPrefix(c.brickLevels), wide(), f);
+        f ^= 1 << L;
+        cycle(L + 1);
+        go(k_mg_bricks_up_batch<St>, 256, levelPrefix(c.brickLevels), wide(), f);
+        f ^= 1 << L;
+        return;
+      }
       if (L == 0) go(k_mg_init_batch<St>, 256, levelPrefix(0), wide(), f << 8);
       else {
         relax(L);
