@@ -102,7 +102,6 @@ class TileDirectory {
     int slot;
     int64_t needed;
     bool closing;
-    int group;  // scratch of the cluster pass
   };
   static constexpr int LINGER_STEPS = 60, RECORD_WORDS = 32, HASH_ROW = 256;
 
@@ -142,7 +141,7 @@ class TileDirectory {
           // The lowest free slot (TileDirectory.ts:100); none below firstFree_.
           int slot = firstFree_;
           while (slot < (int)slots_.size() && slots_[slot]) slot++;
-          Tile *t = new Tile{{x, y, z}, slot, step, false, -1};
+          Tile *t = new Tile{{x, y, z}, slot, step, false};
           if (slot == (int)slots_.size()) slots_.push_back(t);
           else slots_[slot] = t;
           firstFree_ = slot + 1;
@@ -208,15 +207,16 @@ class TileDirectory {
               std::vector<int32_t> &tiles) {
     if (cap < (int)slots_.size()) throw std::runtime_error("Tile capacity is below the tiles in use.");
     capacity = cap;
-    records.assign((size_t)cap * RECORD_WORDS, -1);
+    // Free slots: coordinates and the live word zero, no neighbors.
+    records.resize((size_t)cap * RECORD_WORDS);
+    for (int s = 0; s < cap; s++) {
+      int32_t *r = &records[(size_t)s * RECORD_WORDS];
+      r[0] = r[1] = r[2] = r[3] = 0;
+      std::fill(r + 4, r + RECORD_WORDS, -1);
+    }
     int size = HASH_ROW;
     while (size < (int)index_.size() * 2) size *= 2;
     hash.assign((size_t)size * 4, -1);
-    for (int s = 0; s < cap; s++) {
-      int32_t *r = &records[(size_t)s * RECORD_WORDS];
-      r[0] = r[1] = r[2] = 0;
-      r[3] = 0;
-    }
     const uint32_t mask = (uint32_t)size - 1;
     buildDense();
     for (const Tile *t : order_) {
@@ -230,14 +230,9 @@ class TileDirectory {
       while (hash[(size_t)h * 4 + 3] >= 0) h = (h + 1) & mask;
       for (int k = 0; k < 3; k++) hash[(size_t)h * 4 + k] = t->c[k];
       hash[(size_t)h * 4 + 3] = t->slot;
-      for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-          for (int dx = -1; dx <= 1; dx++) {
-            const Tile *n = near(t->c[0] + dx, t->c[1] + dy, t->c[2] + dz);
-            if (n) r[4 + (dz + 1) * 9 + (dy + 1) * 3 + dx + 1] = n->slot;
-          }
+      neighborSlots(t, r + 4);
     }
-    std::vector<int> clusterOf;  // per tile of order_
+    std::vector<int> clusterOf;  // per slot
     buildClusters(clusters, clusterOf);
     tiles.assign((size_t)cap * 4, 0);
     coords.assign(slots_.size(), {0, 0, 0, 0});
@@ -248,7 +243,7 @@ class TileDirectory {
         w[0] = t->c[0];
         w[1] = t->c[1];
         w[2] = t->c[2];
-        w[3] = clusterOf[t->group];
+        w[3] = clusterOf[s];
         coords[s] = {t->c[0], t->c[1], t->c[2], 1};
       } else {
         w[3] = -1;
@@ -263,33 +258,34 @@ class TileDirectory {
 
  private:
   // clusters (TileDirectory.ts:189-245): 26-connected groups, boxed, merged until disjoint.
-  // Each tile's `group` is its connected group; clusterOf maps groups to clusters.
+  // clusterOf maps each slot to its cluster (-1 for a free slot).
   void buildClusters(std::vector<TileBox> &out, std::vector<int> &clusterOf) {
-    for (Tile *t : order_) t->group = -1;
+    std::vector<int> &group = group_;  // per slot: the connected group
+    group.assign(slots_.size(), -1);
     std::vector<TileBox> boxes;
-    std::vector<Tile *> stack;
-    for (Tile *first : order_) {
-      if (first->group >= 0) continue;
+    std::vector<int> &stack = stack_;
+    for (const Tile *first : order_) {
+      if (group[first->slot] >= 0) continue;
       int idx = (int)boxes.size();
       TileBox box{{first->c[0], first->c[1], first->c[2]}, {first->c[0], first->c[1], first->c[2]}};
-      first->group = idx;
-      stack.assign(1, first);
+      group[first->slot] = idx;
+      stack.assign(1, first->slot);
       while (!stack.empty()) {
-        Tile *t = stack.back();
+        const Tile *t = slots_[stack.back()];
         stack.pop_back();
         for (int a = 0; a < 3; a++) {
           box.lo[a] = std::min(box.lo[a], t->c[a]);
           box.hi[a] = std::max(box.hi[a], t->c[a]);
         }
-        for (int dz = -1; dz <= 1; dz++)
-          for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-              Tile *n = near(t->c[0] + dx, t->c[1] + dy, t->c[2] + dz);
-              if (n && n->group < 0) {
-                n->group = idx;
-                stack.push_back(n);
-              }
-            }
+        int32_t nb[27];
+        neighborSlots(t, nb);
+        for (int k = 0; k < 27; k++) {
+          const int n = nb[k];
+          if (n >= 0 && group[n] < 0) {
+            group[n] = idx;
+            stack.push_back(n);
+          }
+        }
       }
       boxes.push_back(box);
     }
@@ -331,12 +327,14 @@ class TileDirectory {
         renumbered[i] = (int)out.size();
         out.push_back(boxes[i]);
       }
-    clusterOf.assign(boxes.size(), -1);
-    for (size_t i = 0; i < boxes.size(); i++) clusterOf[i] = renumbered[root((int)i)];
+    clusterOf.assign(slots_.size(), -1);
+    for (size_t s = 0; s < slots_.size(); s++)
+      if (group[s] >= 0) clusterOf[s] = renumbered[root(group[s])];
   }
 
-  // A dense grid of the tiles over their bounding box (one cell of margin), for the
-  // neighbor lookups of tables(); the flat index serves when the box is too large.
+  // A dense grid of the tiles' slots (-1 where none) over their bounding box with one free
+  // cell on every side, for the neighbor lookups of tables(); the flat index serves when the
+  // box is too large.
   void buildDense() {
     dense_.clear();
     if (order_.empty()) return;
@@ -354,27 +352,37 @@ class TileDirectory {
       cells *= (size_t)denseDim_[a];
     }
     if (cells > ((size_t)1 << 24)) return;
-    dense_.assign(cells, nullptr);
-    for (Tile *t : order_)
-      dense_[((size_t)(t->c[2] - denseLo_[2]) * denseDim_[1] + (t->c[1] - denseLo_[1])) * denseDim_[0] +
-             (t->c[0] - denseLo_[0])] = t;
+    dense_.assign(cells, -1);
+    for (const Tile *t : order_) dense_[denseCell(t)] = t->slot;
+    const ptrdiff_t sy = denseDim_[0], sz = (ptrdiff_t)denseDim_[0] * denseDim_[1];
+    for (int k = 0; k < 27; k++) denseOff_[k] = (k / 9 - 1) * sz + ((k / 3) % 3 - 1) * sy + (k % 3 - 1);
   }
-  Tile *near(int x, int y, int z) const {
-    if (dense_.empty()) return index_.find(key(x, y, z));
-    x -= denseLo_[0];
-    y -= denseLo_[1];
-    z -= denseLo_[2];
-    if ((unsigned)x >= (unsigned)denseDim_[0] || (unsigned)y >= (unsigned)denseDim_[1] ||
-        (unsigned)z >= (unsigned)denseDim_[2])
-      return nullptr;
-    return dense_[((size_t)z * denseDim_[1] + y) * denseDim_[0] + x];
+  size_t denseCell(const Tile *t) const {
+    return ((size_t)(t->c[2] - denseLo_[2]) * denseDim_[1] + (t->c[1] - denseLo_[1])) * denseDim_[0] +
+           (t->c[0] - denseLo_[0]);
+  }
+  // The slots of the 27 tiles around tile t (itself included), (dz * 3 + dy) * 3 + dx order,
+  // -1 where none. Inside the dense grid's free border a tile's neighbors are fixed offsets
+  // from its cell, with no bounds tests.
+  void neighborSlots(const Tile *t, int32_t out[27]) const {
+    if (dense_.empty()) {
+      for (int k = 0; k < 27; k++) {
+        const Tile *n = index_.find(key(t->c[0] + k % 3 - 1, t->c[1] + (k / 3) % 3 - 1, t->c[2] + k / 9 - 1));
+        out[k] = n ? n->slot : -1;
+      }
+      return;
+    }
+    const int32_t *base = &dense_[denseCell(t)];
+    for (int k = 0; k < 27; k++) out[k] = base[denseOff_[k]];
   }
 
   std::vector<Tile *> order_;
   FlatIndex<Tile *> index_;
   int firstFree_ = 0;
-  std::vector<Tile *> dense_;
+  std::vector<int32_t> dense_;
   int denseLo_[3] = {0, 0, 0}, denseDim_[3] = {0, 0, 0};
+  ptrdiff_t denseOff_[27] = {};
+  std::vector<int> group_, stack_;
   std::vector<Tile *> slots_;
   std::vector<int> created_;
 };
