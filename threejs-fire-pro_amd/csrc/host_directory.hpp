@@ -201,26 +201,31 @@ class TileDirectory {
     return out;
   }
 
-  // tables (TileDirectory.ts:151-186): tile records with 27 neighbor slots, the hash and
-  // the per-slot (coordinates, cluster) words; `coords` snapshots the slots' tiles.
-  void tables(int cap, std::vector<int32_t> &records, std::vector<int32_t> &hash, std::vector<TileBox> &clusters,
+  // tables (TileDirectory.ts:151-186): `words` gets the cap tile records with 27 neighbor
+  // slots, RECORD_WORDS each, followed by the hash (hashWords words, 4 a row); `tiles` the
+  // per-slot (coordinates, cluster) words; `coords` snapshots the slots' tiles.
+  void tables(int cap, std::vector<int32_t> &words, int &hashWords, std::vector<TileBox> &clusters,
               std::vector<int32_t> &tiles) {
     if (cap < (int)slots_.size()) throw std::runtime_error("Tile capacity is below the tiles in use.");
     capacity = cap;
+    int size = HASH_ROW;
+    while (size < (int)index_.size() * 2) size *= 2;
+    hashWords = size * 4;
+    const size_t recordWords = (size_t)cap * RECORD_WORDS;
+    words.resize(recordWords + (size_t)hashWords);
     // Free slots: coordinates and the live word zero, no neighbors.
-    records.resize((size_t)cap * RECORD_WORDS);
     for (int s = 0; s < cap; s++) {
-      int32_t *r = &records[(size_t)s * RECORD_WORDS];
+      int32_t *r = &words[(size_t)s * RECORD_WORDS];
       r[0] = r[1] = r[2] = r[3] = 0;
       std::fill(r + 4, r + RECORD_WORDS, -1);
     }
-    int size = HASH_ROW;
-    while (size < (int)index_.size() * 2) size *= 2;
-    hash.assign((size_t)size * 4, -1);
+    int32_t *const hash = words.data() + recordWords;
+    std::fill(hash, hash + hashWords, -1);
+    int32_t *const records = words.data();
     const uint32_t mask = (uint32_t)size - 1;
     buildDense();
     for (const Tile *t : order_) {
-      int32_t *r = &records[(size_t)t->slot * RECORD_WORDS];
+      int32_t *r = records + (size_t)t->slot * RECORD_WORDS;
       r[0] = t->c[0];
       r[1] = t->c[1];
       r[2] = t->c[2];

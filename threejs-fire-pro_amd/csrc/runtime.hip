@@ -458,7 +458,7 @@ struct fp_sim {
   int freeCount = 0, lastFailures = 0, lastZeroed = 0;
   host::TileDirectory dir;
   int capacity = 0;
-  std::vector<int32_t> tileRecHost, hashHost, tilesHost;
+  std::vector<int32_t> tableWords, tilesHost;
   std::vector<int> created;  // tile slots created by the last table upload
   std::vector<host::TileBox> clusters;
   uint32_t hashMask = 0;
@@ -466,11 +466,12 @@ struct fp_sim {
   bool stepped = false;
 
   // Per brick id.
-  DBuf<int> tileRec, hash, pages, entries, records, active, zeroList, poolState, counts, createdSlots, readback;
-  // Tile slots near the colliders (nearTopology) and those whose tile-level topology a
-  // tile-set change rebuilds (tileTopology).
-  DBuf<int> nearTiles, topoTiles;
-  std::vector<int> nearTilesHost, topoTilesHost;
+  DBuf<int> pages, entries, records, active, zeroList, poolState, counts, createdSlots, readback;
+  // The tile tables: capacity records of 32 words, then the tile hash from word hashOffset.
+  DBuf<int> tileRec;
+  size_t hashOffset = 0;
+  DBuf<int> tileLists;  // the tile lists of movedTileTopology
+  std::vector<int> tileListsHost;
   DBuf<uint32_t> brickState, activity, stateTmp, rankTmp, blockSums, blockPrefix, freeSlots;
   // Pools, per slot.
   DBuf<float4> F, Fp, Fn, Curl;
@@ -609,7 +610,7 @@ struct fp_sim {
   Dir dirArgs() const {
     Dir r;
     r.tileRec = tileRec.p;
-    r.hash = hash.p;
+    r.hash = tileRec.p ? tileRec.p + hashOffset : nullptr;
     r.hashMask = hashMask;
     r.entries = entries.p;
     r.records = records.p;
@@ -808,7 +809,8 @@ struct fp_sim {
     blockPrefix.alloc(blocks * 3);
     active.alloc(bricks);
     zeroList.alloc(bricks * 4);
-    tileRec.alloc((size_t)cap * 32);
+    tileRec.alloc((size_t)cap * 32 + 4 * host::TileDirectory::HASH_ROW);
+    hashOffset = (size_t)cap * 32;
     capacity = cap;
     for (int L = brickLevels; L <= bottom; L++) {
       int sd = 4 >> (L - brickLevels);
@@ -841,16 +843,18 @@ struct fp_sim {
       growTiles(cap);
     }
     Clock::time_point t0 = Clock::now();
-    dir.tables(capacity, tileRecHost, hashHost, clusters, tilesHost);
+    int hashWords = 0;
+    dir.tables(capacity, tableWords, hashWords, clusters, tilesHost);
     hostMark("host_tiles_tables", t0);
-    HIPCHECK(hipMemcpyAsync(tileRec.p, tileRecHost.data(), tileRecHost.size() * sizeof(int32_t),
-                            hipMemcpyHostToDevice, stream));
-    if (hash.n != hashHost.size()) {
+    if (tileRec.n < tableWords.size()) {
       drain();  // the last step's queued tail may still read the old block
-      hash.alloc(hashHost.size());
+      tileRec.alloc(tableWords.size() + tableWords.size() / 4);
     }
-    HIPCHECK(hipMemcpyAsync(hash.p, hashHost.data(), hashHost.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-    hashMask = (uint32_t)(hashHost.size() / 4 - 1);
+    // Records and hash in one copy.
+    HIPCHECK(hipMemcpyAsync(tileRec.p, tableWords.data(), tableWords.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                            stream));
+    hashOffset = tableWords.size() - (size_t)hashWords;
+    hashMask = (uint32_t)(hashWords / 4 - 1);
     created = dir.takeCreated();
     if (!created.empty()) {
       if (createdSlots.n < created.size()) {
@@ -863,7 +867,7 @@ struct fp_sim {
       k_clear_tiles<<<(n + 255) / 256, 256, 0, stream>>>(pages.p, entries.p, brickState.p, activity.p,
                                                           createdSlots.p, (int)created.size());
     }
-    // The copies read tileRecHost, hashHost and created when the stream reaches them; each
+    // The copies read tableWords and created when the stream reaches them; each
     // step waits for its readback, queued after them, before these are rebuilt.
     tilesMoved = true;
     rboxes.stale = true;  // FluidSimulation.ts:708-709
@@ -1004,14 +1008,28 @@ struct fp_sim {
     P.fexpansion = (float)expansion;
   }
 
+  // A scene array's upload. Kernels only read these arrays, so a block that already holds
+  // the same bytes (same block, same contents as its last upload) is left as it is: a static
+  // scene costs no copy per step.
+  struct Uploaded {
+    const void *block = nullptr;
+    std::vector<unsigned char> bytes;
+  };
+  std::map<const void *, Uploaded> uploaded;
   template <class T> void upload(DBuf<T> &buf, const std::vector<T> &host) {
     size_t n = std::max<size_t>(1, host.size());
     if (buf.n < n) {
       drain();  // the last step's queued tail may still read the old block
       buf.alloc(n + n / 2);
     }
-    if (!host.empty())
-      HIPCHECK(hipMemcpyAsync(buf.p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    if (host.empty()) return;
+    const size_t bytes = host.size() * sizeof(T);
+    Uploaded &last = uploaded[&buf];
+    const unsigned char *src = reinterpret_cast<const unsigned char *>(host.data());
+    if (last.block == buf.p && last.bytes.size() == bytes && std::memcmp(last.bytes.data(), src, bytes) == 0) return;
+    HIPCHECK(hipMemcpyAsync(buf.p, host.data(), bytes, hipMemcpyHostToDevice, stream));
+    last.block = buf.p;
+    last.bytes.assign(src, src + bytes);
   }
 
   // The box of field cells holding every cell the colliders make solid (Params::solidLo and
@@ -1542,59 +1560,50 @@ struct fp_sim {
     if (profiling) collect();
   }
 
-  // The topology of the bricks near the colliders: the live tile slots whose bricks lie
-  // within a brick of the colliders' box of field cells, listed on the host, then every
-  // brick of them that holds a slot tested on the device (nearSolids).
-  void nearTopology(const MGArgs &m, unsigned lanes) {
-    nearTilesHost.clear();
+  // After a tile-set change, the topology the change can reach. With colliders: the bricks
+  // of the live tiles within a brick of the colliders' box of field cells, listed on the
+  // host, then every brick of them that holds a slot tested on the device (nearSolids). With
+  // tile levels: their topology at the tiles created by the last table upload and, with
+  // colliders, at the live tiles within a tile of the colliders' box (a tile-level cell's
+  // neighbor samples lie up to a tile away). Both lists go up in one copy.
+  void movedTileTopology(const MGArgs &m, unsigned lanes) {
+    tileListsHost.clear();
     const long long side = 4LL * B;  // field cells a tile side
-    for (int t = 0; t < capacity && t < (int)dir.coords.size(); t++) {
-      const std::array<int, 4> &c = dir.coords[t];
-      if (!c[3]) continue;
-      bool near = true;
-      for (int a = 0; a < 3; a++)
-        if ((long long)c[a] * side + side + B - 1 < solidLo[a] || (long long)c[a] * side - B > solidHi[a]) near = false;
-      if (near) nearTilesHost.push_back(t);
-    }
-    if (nearTilesHost.empty()) return;
-    if (nearTiles.n < nearTilesHost.size()) {
-      drain();
-      nearTiles.alloc(std::max<size_t>(nearTilesHost.size(), 64));
-    }
-    // Read when the stream reaches the copy; each step waits for its readback, queued after
-    // it, before the list is rebuilt.
-    HIPCHECK(hipMemcpyAsync(nearTiles.p, nearTilesHost.data(), nearTilesHost.size() * sizeof(int),
-                            hipMemcpyHostToDevice, stream));
-    k_near_topology<<<(unsigned)nearTilesHost.size() * 64, lanes, 0, stream>>>(m, nearTiles.p,
-                                                                               (int)nearTilesHost.size());
-  }
-
-  // The tile levels' topology of the tiles created by the last table upload and, with
-  // colliders, of the live tiles within a tile of the colliders' box (a tile-level cell's
-  // neighbor samples lie up to a tile away).
-  void tileTopology(const MGArgs &m) {
-    topoTilesHost.assign(created.begin(), created.end());
-    if (colliderCount > 0) {
-      const long long side = 4LL * B;
+    if (colliderCount > 0)
       for (int t = 0; t < capacity && t < (int)dir.coords.size(); t++) {
         const std::array<int, 4> &c = dir.coords[t];
-        if (!c[3] || std::find(created.begin(), created.end(), t) != created.end()) continue;
+        if (!c[3]) continue;
         bool near = true;
         for (int a = 0; a < 3; a++)
-          if ((long long)c[a] * side + 2 * side - 1 < solidLo[a] || (long long)c[a] * side - side > solidHi[a]) near = false;
-        if (near) topoTilesHost.push_back(t);
+          if ((long long)c[a] * side + side + B - 1 < solidLo[a] || (long long)c[a] * side - B > solidHi[a]) near = false;
+        if (near) tileListsHost.push_back(t);
       }
+    const int nearCount = (int)tileListsHost.size();
+    if (bottom >= brickLevels) {
+      tileListsHost.insert(tileListsHost.end(), created.begin(), created.end());
+      if (colliderCount > 0)
+        for (int t = 0; t < capacity && t < (int)dir.coords.size(); t++) {
+          const std::array<int, 4> &c = dir.coords[t];
+          if (!c[3] || std::find(created.begin(), created.end(), t) != created.end()) continue;
+          bool near = true;
+          for (int a = 0; a < 3; a++)
+            if ((long long)c[a] * side + 2 * side - 1 < solidLo[a] || (long long)c[a] * side - side > solidHi[a])
+              near = false;
+          if (near) tileListsHost.push_back(t);
+        }
     }
-    if (topoTilesHost.empty()) return;
-    if (topoTiles.n < topoTilesHost.size()) {
+    const int topoCount = (int)tileListsHost.size() - nearCount;
+    if (tileListsHost.empty()) return;
+    if (tileLists.n < tileListsHost.size()) {
       drain();
-      topoTiles.alloc(std::max<size_t>(topoTilesHost.size(), 64));
+      tileLists.alloc(std::max<size_t>(tileListsHost.size(), 128));
     }
     // Read when the stream reaches the copy; each step waits for its readback, queued after
-    // it, before the list is rebuilt.
-    HIPCHECK(hipMemcpyAsync(topoTiles.p, topoTilesHost.data(), topoTilesHost.size() * sizeof(int),
+    // it, before the lists are rebuilt.
+    HIPCHECK(hipMemcpyAsync(tileLists.p, tileListsHost.data(), tileListsHost.size() * sizeof(int),
                             hipMemcpyHostToDevice, stream));
-    k_topology_tiles_list<<<(unsigned)topoTilesHost.size(), 128, 0, stream>>>(m, topoTiles.p, (int)topoTilesHost.size());
+    if (nearCount) k_near_topology<<<(unsigned)nearCount * 64, lanes, 0, stream>>>(m, tileLists.p, nearCount);
+    if (topoCount) k_topology_tiles_list<<<(unsigned)topoCount, 128, 0, stream>>>(m, tileLists.p + nearCount, topoCount);
   }
 
   // PressureSolver.solve (PressureSolver.ts:371-402): one cold-start V-cycle, or the
@@ -1621,8 +1630,7 @@ struct fp_sim {
         // cell's topology depends on the floor alone: the bricks whose cells a collider may
         // reach are built again (a new brick beside a collider can close its neighbors'
         // sides), and of the tile levels the created tiles and those near the colliders.
-        if (colliderCount > 0) nearTopology(m, tLanes);
-        if (bottom >= brickLevels) tileTopology(m);
+        movedTileTopology(m, tLanes);
       }
     }
     if (pressureMode == 0) {
@@ -2220,7 +2228,7 @@ struct fp_sim {
   }
 
   size_t memoryBytes() const {
-    size_t total = tileRec.bytes() + hash.bytes() + pages.bytes() + entries.bytes() + records.bytes() +
+    size_t total = tileRec.bytes() + tileLists.bytes() + pages.bytes() + entries.bytes() + records.bytes() +
                    active.bytes() + zeroList.bytes() + brickState.bytes() + activity.bytes() + stateTmp.bytes() +
                    rankTmp.bytes() + blockSums.bytes() + blockPrefix.bytes() + freeSlots.bytes() + F.bytes() +
                    Fp.bytes() + Fn.bytes() + Don.bytes() + Vtr.bytes() + Curl.bytes() + E.bytes() + V.bytes() + Vadv.bytes() +
@@ -2264,7 +2272,7 @@ struct fp_sim {
     tailPending = false;
     if (readbackEvent) (void)hipEventDestroy(readbackEvent);
     readbackEvent = nullptr;
-    DBuf<int> *ints[] = {&tileRec, &hash, &pages, &entries, &records, &active, &zeroList, &poolState, &counts, &createdSlots, &readback};
+    DBuf<int> *ints[] = {&tileRec, &tileLists, &pages, &entries, &records, &active, &zeroList, &poolState, &counts, &createdSlots, &readback};
     for (auto *x : ints) x->release();
     DBuf<uint32_t> *u32s[] = {&brickState, &activity, &stateTmp, &rankTmp, &blockSums, &blockPrefix, &freeSlots, &solids};
     for (auto *x : u32s) x->release();
