@@ -270,22 +270,22 @@ __device__ __forceinline__ bool insideCollider(const Collider &c, const uint32_t
   return dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] < 1.0f;
 }
 
-// One brick (its id and page) with a wavefront: a lane per cell and pass, the pass's 64
-// bits from a ballot (two words, cells in index order).
-__device__ __forceinline__ void solidBrick(const AllocArgs &a, int index, int page) {
+// One brick (its id and page) with a wavefront (`lane` its lane): a lane per cell and pass,
+// the pass's 64 bits from a ballot (two words, cells in index order).
+__device__ __forceinline__ void solidBrick(const AllocArgs &a, int index, int page, int lane) {
   const int *tr = a.dir.tileRec + (size_t)(index >> 6) * TILE_RECORD_WORDS;
   const int l = index & 63;
   const int b[3] = {tr[0] * 4 + (l & 3), tr[1] * 4 + ((l >> 2) & 3), tr[2] * 4 + (l >> 4)};
   const int B = a.L.B, sh = a.L.sF, nF = a.L.nF;
   const size_t base = (size_t)pageToSlot(page, a.L.side) * (size_t)(nF >> 5);
   for (int cell0 = 0; cell0 < nF; cell0 += 64) {
-    const int cell = cell0 + (int)threadIdx.x;
+    const int cell = cell0 + lane;
     const int x = b[0] * B + (cell & (B - 1)), y = b[1] * B + ((cell >> sh) & (B - 1)), z = b[2] * B + (cell >> (2 * sh));
     bool inside = false;
     for (int ci = 0; ci < a.colliderCount && !inside; ci++)
       inside = insideCollider(a.colliders[ci], a.colliderMasks, a.P.hf, x, y, z);
     const uint64_t bits = __ballot(inside);
-    if (threadIdx.x < 2) a.solids[base + (size_t)(cell0 >> 5) + threadIdx.x] = (uint32_t)(bits >> (32 * threadIdx.x));
+    if (lane < 2) a.solids[base + (size_t)(cell0 >> 5) + lane] = (uint32_t)(bits >> (32 * lane));
   }
 }
 __device__ __forceinline__ void build_solids_body(const AllocArgs &a, const int vb) {
@@ -294,18 +294,13 @@ __device__ __forceinline__ void build_solids_body(const AllocArgs &a, const int 
   if (a.solidsEvery) {
     if (i >= a.counts[0]) return;
     const int index = a.active[i];
-    solidBrick(a, index, a.entries[index] & 0xffffff);
+    solidBrick(a, index, a.entries[index] & 0xffffff, (int)threadIdx.x);
   } else {
     if (i >= a.counts[1]) return;
-    solidBrick(a, a.zeroList[2 * i], a.zeroList[2 * i + 1] & 0xffffff);
+    solidBrick(a, a.zeroList[2 * i], a.zeroList[2 * i + 1] & 0xffffff, (int)threadIdx.x);
   }
 }
 FP_KERNEL_PAIR(build_solids, 64, AllocArgs)
-// The solids of the zeroed list's bricks only, walked by a fixed grid.
-__global__ void __launch_bounds__(64) k_fresh_solids(AllocArgs a) {
-  for (int i = (int)blockIdx.x; i < a.counts[1]; i += (int)gridDim.x)
-    solidBrick(a, a.zeroList[2 * i], a.zeroList[2 * i + 1] & 0xffffff);
-}
 
 
 // A fresh slot is still, empty air (zeroVelocityBricks / zeroFieldBricks,
@@ -326,10 +321,15 @@ __device__ __forceinline__ void zero_fresh_body(const AllocArgs &a, const int vb
     for (int c = chunk * 256 + threadIdx.x; c < a.L.nS; c += chunks * 256) p.S[(size_t)slot * a.L.nS + c] = 0.f;
 }
 FP_KERNEL_PAIR1(zero_fresh, 256, AllocArgs, int, chunks)
-// The same over the zeroed list only, walked by a fixed grid (a step zeroes few bricks).
-__global__ void __launch_bounds__(256) k_fresh_zero(AllocArgs a, int chunks) {
+// The same over the zeroed list only, walked by a fixed grid (a step zeroes few bricks),
+// and with `solids` the list's solids too, a wavefront a brick (the two are independent).
+__global__ void __launch_bounds__(256) k_fresh_zero(AllocArgs a, int chunks, int solids) {
   const int total = a.counts[1] * chunks;
   for (int v = (int)blockIdx.x; v < total; v += (int)gridDim.x) zero_fresh_body(a, v, chunks);
+  if (!solids) return;
+  const int waves = (int)blockDim.x >> 6, lane = (int)threadIdx.x & 63;
+  for (int i = (int)blockIdx.x * waves + ((int)threadIdx.x >> 6); i < a.counts[1]; i += (int)gridDim.x * waves)
+    solidBrick(a, a.zeroList[2 * i], a.zeroList[2 * i + 1] & 0xffffff, lane);
 }
 
 

@@ -255,7 +255,8 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = value;
 }
-// Failed allocations count from one measurement to the next (FluidSimulation.ts:1525).
+// Failed allocations count from one measurement to the next (FluidSimulation.ts:1525): the
+// lane that reads the counter restarts it.
 __global__ void k_readback(const int *counts, int *poolState, const uint32_t *tileFlags, int cap, int *out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
@@ -263,10 +264,10 @@ __global__ void k_readback(const int *counts, int *poolState, const uint32_t *ti
     out[1] = poolState[0];
     out[2] = poolState[1];
     out[3] = counts[1] + counts[5];
+    poolState[1] = 0;
   }
   if (i < cap) out[4 + i] = (int)tileFlags[i];
 }
-__global__ void k_reset_failures(int *poolState) { poolState[1] = 0; }
 
 // The readbacks of a batch, a workgroup per simulation, each into its own words of one
 // buffer; the failure counters restart for the next step.
@@ -466,8 +467,9 @@ struct fp_sim {
   bool stepped = false;
 
   // Per brick id.
-  DBuf<int> pages, entries, records, active, zeroList, poolState, counts, createdSlots, readback;
-  // The tile tables: capacity records of 32 words, then the tile hash from word hashOffset.
+  DBuf<int> pages, entries, records, active, zeroList, poolState, counts, readback;
+  // The tile tables: capacity records of 32 words, then the tile hash from word hashOffset,
+  // then the slots the last upload created.
   DBuf<int> tileRec;
   size_t hashOffset = 0;
   DBuf<int> tileLists;  // the tile lists of movedTileTopology
@@ -846,29 +848,25 @@ struct fp_sim {
     int hashWords = 0;
     dir.tables(capacity, tableWords, hashWords, clusters, tilesHost);
     hostMark("host_tiles_tables", t0);
+    // Records, hash and the created slots in one copy.
+    const size_t tableSize = tableWords.size();
+    created = dir.takeCreated();
+    tableWords.insert(tableWords.end(), created.begin(), created.end());
     if (tileRec.n < tableWords.size()) {
       drain();  // the last step's queued tail may still read the old block
       tileRec.alloc(tableWords.size() + tableWords.size() / 4);
     }
-    // Records and hash in one copy.
     HIPCHECK(hipMemcpyAsync(tileRec.p, tableWords.data(), tableWords.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                             stream));
-    hashOffset = tableWords.size() - (size_t)hashWords;
+    hashOffset = tableSize - (size_t)hashWords;
     hashMask = (uint32_t)(hashWords / 4 - 1);
-    created = dir.takeCreated();
     if (!created.empty()) {
-      if (createdSlots.n < created.size()) {
-        drain();
-        createdSlots.alloc(std::max<size_t>(created.size(), 64));
-      }
-      HIPCHECK(hipMemcpyAsync(createdSlots.p, created.data(), created.size() * sizeof(int), hipMemcpyHostToDevice,
-                              stream));
       int n = (int)created.size() * 64;
       k_clear_tiles<<<(n + 255) / 256, 256, 0, stream>>>(pages.p, entries.p, brickState.p, activity.p,
-                                                          createdSlots.p, (int)created.size());
+                                                          tileRec.p + tableSize, (int)created.size());
     }
-    // The copies read tableWords and created when the stream reaches them; each
-    // step waits for its readback, queued after them, before these are rebuilt.
+    // The copy reads tableWords when the stream reaches it; each step waits for its
+    // readback, queued after it, before the words are rebuilt.
     tilesMoved = true;
     rboxes.stale = true;  // FluidSimulation.ts:708-709
   }
@@ -1479,13 +1477,11 @@ struct fp_sim {
     k_free_bricks<<<blocks, 256, 0, stream>>>(a);
     k_allocate_bricks<<<blocks, 256, 0, stream>>>(a);
     k_link_bricks<<<(slotsGrid + LINK_BRICKS - 1) / LINK_BRICKS, 256, 0, stream>>>(a);
-    if (a.buildSolids) {
-      if (a.solidsEvery) k_build_solids<<<slotsGrid, 64, 0, stream>>>(a);
-      else k_fresh_solids<<<std::min(slotsGrid, FRESH_GRID), 64, 0, stream>>>(a);
-    }
+    if (a.buildSolids && a.solidsEvery) k_build_solids<<<slotsGrid, 64, 0, stream>>>(a);
+    const int freshSolids = a.buildSolids && !a.solidsEvery ? 1 : 0;  // built with the zeroing
     solidsStale = false;
     const int zchunks = std::max(1, nF / 1024);
-    k_fresh_zero<<<std::min(slotsGrid * (unsigned)zchunks, FRESH_GRID), 256, 0, stream>>>(a, zchunks);
+    k_fresh_zero<<<std::min(slotsGrid * (unsigned)zchunks, FRESH_GRID), 256, 0, stream>>>(a, zchunks, freshSolids);
     end(m0, "allocation");
 
     StepArgs s = stepArgs();
@@ -1526,7 +1522,6 @@ struct fp_sim {
     k_readback<<<(unsigned)((capacity + 255) / 256 + 1), 256, 0, stream>>>(counts.p, poolState.p, activity.p + n,
                                                                            capacity, readback.p);
     HIPCHECK(hipMemcpyAsync(pinned, readback.p, (4 + (size_t)capacity) * sizeof(int), hipMemcpyDeviceToHost, stream));
-    k_reset_failures<<<1, 1, 0, stream>>>(poolState.p);
     if (!readbackEvent) HIPCHECK(hipEventCreateWithFlags(&readbackEvent, hipEventDisableTiming));
     HIPCHECK(hipEventRecord(readbackEvent, stream));
     int m5 = begin();
@@ -2272,7 +2267,7 @@ struct fp_sim {
     tailPending = false;
     if (readbackEvent) (void)hipEventDestroy(readbackEvent);
     readbackEvent = nullptr;
-    DBuf<int> *ints[] = {&tileRec, &tileLists, &pages, &entries, &records, &active, &zeroList, &poolState, &counts, &createdSlots, &readback};
+    DBuf<int> *ints[] = {&tileRec, &tileLists, &pages, &entries, &records, &active, &zeroList, &poolState, &counts, &readback};
     for (auto *x : ints) x->release();
     DBuf<uint32_t> *u32s[] = {&brickState, &activity, &stateTmp, &rankTmp, &blockSums, &blockPrefix, &freeSlots, &solids};
     for (auto *x : u32s) x->release();
