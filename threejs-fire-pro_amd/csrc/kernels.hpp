@@ -198,6 +198,7 @@ FP_DECLARE_PAIR(eflags_bricks, MGArgs)
 __global__ void k_fresh_topology(MGArgs m);
 __global__ void k_near_topology(MGArgs m, const int *tiles, int count);
 __global__ void k_topology_tiles_list(MGArgs m, const int *tiles, int count);
+__global__ void k_moved_topology(MGArgs m, int fresh, const int *tiles, int nearCount, int tileCount);
 FP_DECLARE_PAIR(eflags_tiles, MGArgs)
 __global__ void k_mg_flags_init(MGArgs m, int brickGroups);  // eflags of every level + mg_init, one launch
 #define FP_DECLARE_MG(NAME)                                                                       \

@@ -391,6 +391,34 @@ __global__ void __launch_bounds__(128) k_topology_tiles_list(MGArgs m, const int
   topology_tiles_body(m, tiles[blockIdx.x], -1);
 }
 
+// The three passes of a step whose tile set changed, in one launch of 128-lane workgroups:
+// the zeroed list's bricks (the first `fresh` workgroups, walking it as k_fresh_topology),
+// the bricks of the first `nearCount` listed tiles (64 workgroups a tile, as
+// k_near_topology) and the tile levels of the next `tileCount` (as k_topology_tiles_list).
+// Every part computes topology from the solids and the tables alone, so they are
+// independent; a brick both fresh and near gets the same bytes from both.
+__global__ void __launch_bounds__(128) k_moved_topology(MGArgs m, int fresh, const int *tiles, int nearCount,
+                                                        int tileCount) {
+  int vb = (int)blockIdx.x;
+  if (vb < fresh) {
+    const int count = m.counts[1] + m.counts[5];
+    for (int bi = vb; bi < count; bi += fresh) {
+      topology_brick(m, m.zeroList[2 * bi]);
+      __syncthreads();  // the LDS record is loaded again for the next brick
+    }
+    return;
+  }
+  vb -= fresh;
+  if (vb < nearCount * 64) {
+    const int id = tiles[vb >> 6] * 64 + (vb & 63);
+    if (m.dir.entries[id] < 0 || !nearSolids(m, id)) return;
+    topology_brick(m, id);
+    return;
+  }
+  vb -= nearCount * 64;
+  if (vb < tileCount) topology_tiles_body(m, tiles[nearCount + vb], -1);
+}
+
 // The step's flags of the brick levels: topology plus the unknown bit.
 __device__ __forceinline__ void eflags_bricks_body(const MGArgs &m, const int vb) {
   const int bi = xcdItem(vb, m.counts[0]);

@@ -1560,8 +1560,9 @@ struct fp_sim {
   // host, then every brick of them that holds a slot tested on the device (nearSolids). With
   // tile levels: their topology at the tiles created by the last table upload and, with
   // colliders, at the live tiles within a tile of the colliders' box (a tile-level cell's
-  // neighbor samples lie up to a tile away). Both lists go up in one copy.
-  void movedTileTopology(const MGArgs &m, unsigned lanes) {
+  // neighbor samples lie up to a tile away). Both lists go up in one copy. With `fresh`
+  // workgroups for the zeroed list's bricks, that pass joins the same launch (128 lanes).
+  void movedTileTopology(const MGArgs &m, unsigned lanes, unsigned fresh) {
     tileListsHost.clear();
     const long long side = 4LL * B;  // field cells a tile side
     if (colliderCount > 0)
@@ -1588,15 +1589,21 @@ struct fp_sim {
         }
     }
     const int topoCount = (int)tileListsHost.size() - nearCount;
-    if (tileListsHost.empty()) return;
-    if (tileLists.n < tileListsHost.size()) {
-      drain();
-      tileLists.alloc(std::max<size_t>(tileListsHost.size(), 128));
+    if (!tileListsHost.empty()) {
+      if (tileLists.n < tileListsHost.size()) {
+        drain();
+        tileLists.alloc(std::max<size_t>(tileListsHost.size(), 128));
+      }
+      // Read when the stream reaches the copy; each step waits for its readback, queued
+      // after it, before the lists are rebuilt.
+      HIPCHECK(hipMemcpyAsync(tileLists.p, tileListsHost.data(), tileListsHost.size() * sizeof(int),
+                              hipMemcpyHostToDevice, stream));
     }
-    // Read when the stream reaches the copy; each step waits for its readback, queued after
-    // it, before the lists are rebuilt.
-    HIPCHECK(hipMemcpyAsync(tileLists.p, tileListsHost.data(), tileListsHost.size() * sizeof(int),
-                            hipMemcpyHostToDevice, stream));
+    if (fresh) {
+      const unsigned grid = fresh + (unsigned)nearCount * 64 + (unsigned)topoCount;
+      k_moved_topology<<<grid, 128, 0, stream>>>(m, (int)fresh, tileLists.p, nearCount, topoCount);
+      return;
+    }
     if (nearCount) k_near_topology<<<(unsigned)nearCount * 64, lanes, 0, stream>>>(m, tileLists.p, nearCount);
     if (topoCount) k_topology_tiles_list<<<(unsigned)topoCount, 128, 0, stream>>>(m, tileLists.p + nearCount, topoCount);
   }
@@ -1619,13 +1626,17 @@ struct fp_sim {
       if (bottom >= brickLevels) k_topology_tiles<<<capacity, 128, 0, stream>>>(m, -1);
       geometryDirty = false;
     } else {
-      k_fresh_topology<<<std::min(slotsGrid * 2, FRESH_GRID), tLanes, 0, stream>>>(m);
+      const unsigned freshGrid = std::min(slotsGrid * 2, FRESH_GRID);
       if (tiles) {
         // Only the tile set changed. Tiles keep their slots, and away from the colliders a
         // cell's topology depends on the floor alone: the bricks whose cells a collider may
         // reach are built again (a new brick beside a collider can close its neighbors'
         // sides), and of the tile levels the created tiles and those near the colliders.
-        movedTileTopology(m, tLanes);
+        // At 128 lanes the fresh bricks' pass shares the launch.
+        if (tLanes != 128) k_fresh_topology<<<freshGrid, tLanes, 0, stream>>>(m);
+        movedTileTopology(m, tLanes, tLanes == 128 ? freshGrid : 0u);
+      } else {
+        k_fresh_topology<<<freshGrid, tLanes, 0, stream>>>(m);
       }
     }
     if (pressureMode == 0) {
