@@ -174,7 +174,7 @@ FP_DECLARE_PAIR(free_bricks, AllocArgs)
 FP_DECLARE_PAIR(allocate_bricks, AllocArgs)
 FP_DECLARE_PAIR(link_bricks, AllocArgs)
 FP_DECLARE_PAIR(build_solids, AllocArgs)
-__global__ void k_fresh_zero(AllocArgs a, int chunks, int solids);
+__global__ void k_link_fresh(AllocArgs a, int linkBlocks, int chunks, int solids);
 FP_DECLARE_PAIR1(zero_fresh, AllocArgs, int)
 
 FP_DECLARE_PAIR(advect_velocity, StepArgs)

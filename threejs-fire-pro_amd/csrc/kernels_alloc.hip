@@ -321,15 +321,26 @@ __device__ __forceinline__ void zero_fresh_body(const AllocArgs &a, const int vb
     for (int c = chunk * 256 + threadIdx.x; c < a.L.nS; c += chunks * 256) p.S[(size_t)slot * a.L.nS + c] = 0.f;
 }
 FP_KERNEL_PAIR1(zero_fresh, 256, AllocArgs, int, chunks)
-// The same over the zeroed list only, walked by a fixed grid (a step zeroes few bricks),
-// and with `solids` the list's solids too, a wavefront a brick (the two are independent).
-__global__ void __launch_bounds__(256) k_fresh_zero(AllocArgs a, int chunks, int solids) {
+// The same over the zeroed list only, walked by `grid` workgroups (a step zeroes few
+// bricks), this one `block`; with `solids` the list's solids too, a wavefront a brick (the
+// two are independent).
+__device__ __forceinline__ void freshZero(const AllocArgs &a, int block, int grid, int chunks, int solids) {
   const int total = a.counts[1] * chunks;
-  for (int v = (int)blockIdx.x; v < total; v += (int)gridDim.x) zero_fresh_body(a, v, chunks);
+  for (int v = block; v < total; v += grid) zero_fresh_body(a, v, chunks);
   if (!solids) return;
   const int waves = (int)blockDim.x >> 6, lane = (int)threadIdx.x & 63;
-  for (int i = (int)blockIdx.x * waves + ((int)threadIdx.x >> 6); i < a.counts[1]; i += (int)gridDim.x * waves)
+  for (int i = block * waves + ((int)threadIdx.x >> 6); i < a.counts[1]; i += grid * waves)
     solidBrick(a, a.zeroList[2 * i], a.zeroList[2 * i + 1] & 0xffffff, lane);
+}
+// Linking and the fresh bricks' zeroing in one launch: the first `linkBlocks` workgroups
+// link (they write the brick records), the rest walk the zeroed list (they write the pools
+// and solid bits), so neither reads what the other writes.
+__global__ void __launch_bounds__(256) k_link_fresh(AllocArgs a, int linkBlocks, int chunks, int solids) {
+  if ((int)blockIdx.x < linkBlocks) {
+    link_bricks_body(a, (int)blockIdx.x);
+    return;
+  }
+  freshZero(a, (int)blockIdx.x - linkBlocks, (int)gridDim.x - linkBlocks, chunks, solids);
 }
 
 

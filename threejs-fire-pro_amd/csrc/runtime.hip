@@ -1476,12 +1476,13 @@ struct fp_sim {
     k_scan_blocks<<<1, 1024, 0, stream>>>(a);
     k_free_bricks<<<blocks, 256, 0, stream>>>(a);
     k_allocate_bricks<<<blocks, 256, 0, stream>>>(a);
-    k_link_bricks<<<(slotsGrid + LINK_BRICKS - 1) / LINK_BRICKS, 256, 0, stream>>>(a);
     if (a.buildSolids && a.solidsEvery) k_build_solids<<<slotsGrid, 64, 0, stream>>>(a);
     const int freshSolids = a.buildSolids && !a.solidsEvery ? 1 : 0;  // built with the zeroing
     solidsStale = false;
     const int zchunks = std::max(1, nF / 1024);
-    k_fresh_zero<<<std::min(slotsGrid * (unsigned)zchunks, FRESH_GRID), 256, 0, stream>>>(a, zchunks, freshSolids);
+    const unsigned linkBlocks = (slotsGrid + LINK_BRICKS - 1) / LINK_BRICKS;
+    const unsigned zeroBlocks = std::min(slotsGrid * (unsigned)zchunks, FRESH_GRID);
+    k_link_fresh<<<linkBlocks + zeroBlocks, 256, 0, stream>>>(a, (int)linkBlocks, zchunks, freshSolids);
     end(m0, "allocation");
 
     StepArgs s = stepArgs();
