@@ -310,6 +310,23 @@ __global__ void k_light_blocks(const int *counts, const float *partials, const i
                                float *blockSums);
 __global__ void k_light_total(const int *counts, const float *blockSums, int count, float *sums);
 
+// Lane i of the activity readback (FluidSimulation.ts:1514-1525): the counts, the pool state
+// and the tiles' content flags; the lane that reads the failure counter restarts it.
+__device__ __forceinline__ void readbackLane(const int *counts, int *poolState, const uint32_t *tileFlags, int cap,
+                                             int *out, int i) {
+  if (i == 0) {
+    out[0] = counts[0];
+    out[1] = poolState[0];
+    out[2] = poolState[1];
+    out[3] = counts[1] + counts[5];
+    poolState[1] = 0;
+  }
+  if (i < cap) out[4 + i] = (int)tileFlags[i];
+}
+// The curl (workgroups below curlBlocks) and the readback (the rest) in one launch.
+__global__ void k_curl_readback(StepArgs s, int curlBlocks, const int *counts, int *poolState,
+                                const uint32_t *tileFlags, int cap, int *out);
+
 // Loads the home brick's record into LDS; every lane of the block calls it.
 __device__ __forceinline__ void loadHome(const Dir &dir, int id, int *rec, Home &h, int side) {
   if (threadIdx.x < 27) {

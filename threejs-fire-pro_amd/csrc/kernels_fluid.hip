@@ -662,6 +662,14 @@ __device__ __forceinline__ void curl_body(const StepArgs &s, const int vb) {
   }
 }
 FP_KERNEL_PAIR(curl, 256, StepArgs)
+__global__ void __launch_bounds__(256) k_curl_readback(StepArgs s, int curlBlocks, const int *counts, int *poolState,
+                                                       const uint32_t *tileFlags, int cap, int *out) {
+  if ((int)blockIdx.x < curlBlocks) {
+    curl_body(s, (int)blockIdx.x);
+    return;
+  }
+  readbackLane(counts, poolState, tileFlags, cap, out, ((int)blockIdx.x - curlBlocks) * (int)blockDim.x + (int)threadIdx.x);
+}
 
 
 // curlNoise (fluid-forces.wgsl:3-9): the periodic table, trilinear with wrapping.

@@ -258,15 +258,7 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n) {
 // Failed allocations count from one measurement to the next (FluidSimulation.ts:1525): the
 // lane that reads the counter restarts it.
 __global__ void k_readback(const int *counts, int *poolState, const uint32_t *tileFlags, int cap, int *out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    out[0] = counts[0];
-    out[1] = poolState[0];
-    out[2] = poolState[1];
-    out[3] = counts[1] + counts[5];
-    poolState[1] = 0;
-  }
-  if (i < cap) out[4 + i] = (int)tileFlags[i];
+  readbackLane(counts, poolState, tileFlags, cap, out, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // The readbacks of a batch, a workgroup per simulation, each into its own words of one
@@ -1520,13 +1512,19 @@ struct fp_sim {
     // The activity readback (FluidSimulation.ts:1514-1523): the counts, pool state and tile
     // content flags are final once the fields have evolved, so they are copied out here and
     // the host's next step overlaps the forces, the pressure solve and the projection.
-    k_readback<<<(unsigned)((capacity + 255) / 256 + 1), 256, 0, stream>>>(counts.p, poolState.p, activity.p + n,
-                                                                           capacity, readback.p);
+    // With vorticity the curl shares the readback's launch (its workgroups first).
+    int m5 = begin();
+    if (P.vorticity != 0.f) {
+      const unsigned rbBlocks = (unsigned)((capacity + (int)vLanes - 1) / (int)vLanes);
+      k_curl_readback<<<slotsGrid + rbBlocks, vLanes, 0, stream>>>(s, (int)slotsGrid, counts.p, poolState.p,
+                                                                  activity.p + n, capacity, readback.p);
+    } else {
+      k_readback<<<(unsigned)((capacity + 255) / 256 + 1), 256, 0, stream>>>(counts.p, poolState.p, activity.p + n,
+                                                                             capacity, readback.p);
+    }
     HIPCHECK(hipMemcpyAsync(pinned, readback.p, (4 + (size_t)capacity) * sizeof(int), hipMemcpyDeviceToHost, stream));
     if (!readbackEvent) HIPCHECK(hipEventCreateWithFlags(&readbackEvent, hipEventDisableTiming));
     HIPCHECK(hipEventRecord(readbackEvent, stream));
-    int m5 = begin();
-    if (P.vorticity != 0.f) k_curl<<<slotsGrid, vLanes, 0, stream>>>(s);
     // The velocity predictor is dead after correction: it holds the cell forces.
     k_cell_forces<<<slotsGrid, vLanes, 0, stream>>>(s);
     k_forces<<<slotsGrid, vLanes, 0, stream>>>(s);
